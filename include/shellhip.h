@@ -1,0 +1,158 @@
+/*
+ * shellhip.h -- C-ABI of libshellhip, the MI355X (gfx950) implementation of
+ * the explicit force-and-step path of the Loop-subdivision Koiter shell
+ * solver gmunglani/Shell.
+ *
+ * The reference has no FFI: the path sits behind C++ member functions of
+ * `class ShellSystem : public libMesh::System` (ShellSystem.h:28).  Each entry
+ * point below names the reference interface it replaces.  Plain pointers and
+ * sizes only; node vectors are FP64 node-interleaved xyz (index 3*n+c) in
+ * host memory unless an entry point says otherwise; element ids are triangle
+ * indices of the input mesh.
+ *
+ * Conventions: every call returns 0 on success or a negative SHX_E_* code;
+ * shx_last_error() gives the message.  The context owns all device memory.
+ * Calls are synchronous.  One context per host thread.  Run-time problems of
+ * the simulation are not errors: they are reported through `problem` fields
+ * with the severities of the reference's enum Problem (ShellSystem.h:48-49).
+ */
+#ifndef SHELLHIP_H
+#define SHELLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shx_ctx shx_ctx;
+
+enum {
+  SHX_OK = 0,
+  SHX_E_ARG = -1,       /* invalid argument */
+  SHX_E_MESH = -2,      /* mesh is not a closed manifold with at most one irregular vertex per triangle */
+  SHX_E_HIP = -3,       /* HIP runtime error */
+  SHX_E_CAPACITY = -4,  /* a device list overflowed its capacity */
+  SHX_E_STATE = -5      /* call out of order */
+};
+
+/* Problem severities, ShellSystem.h:48-49 */
+enum {
+  SHX_NO_PROBLEM = 0, SHX_NO_DISP_CHANGE, SHX_LARGE_DISP, SHX_STOP_SIM, SHX_TOO_MANY_CELLS,
+  SHX_SMALL_TIMESTEP, SHX_CONTACT_PROBLEM, SHX_NOT_FINITE
+};
+
+enum { SHX_FP64 = 0, SHX_FP32_SPLIT = 1 };
+
+/* Parameters: the config.cfg keys of ShellSystem::set_parameters
+ * (ShellSystem.C:170-418) that the path reads, plus build extensions marked
+ * "ext".  Time functions of the reference (muparser, Function.h) are
+ * replaced by the linear ramps below. */
+typedef struct {
+  double E1, E2, nu, G12;          /* shell_young_modulus, shell_young_modulus2, shell_poisson_ratio, shell_shear_modulus */
+  double thickness, rho;           /* thickness, shell_mass_density */
+  double memb_stiff_mult, bend_stiff_mult;
+  double damping_alpha, damping_beta, damping_gamma;
+  double pressure_final;           /* ext: p(t) = pressure_final*min(t/pressure_ramp_time,1); LOCAL base */
+  double pressure_ramp_time;       /* ext: <= 0: constant */
+  double bulk_modulus;             /* bulk_modulus */
+  double volume_final_ratio;       /* ext: target r(t) = 1-(1-ratio)*min(t/volume_ramp_time,1) */
+  double volume_ramp_time;
+  double contact_stiffness;        /* ext: > 0 replaces stiff_density*area_min (ShellSystem.C:1820) */
+  double mu_s, mu_d;
+  double self_damping, self_friction_damping, fric_stiff_mult;
+  double friction_stop_velocity, static_friction_tol;
+  double cell_size_mult, cell_margin, neighbor_margin;
+  double newmark_beta, newmark_gamma, dt;
+  double dt_min, dt_max;           /* 0: machine epsilon / unbounded */
+  double rel_loc_err_min, rel_loc_err_max;
+  int32_t adapt_dt_every;          /* 0: constant dt; else the reference's error control every step */
+  int32_t contact;                 /* shell_shell_contact: 0 none, 1 linear */
+  int32_t quirk_fulltime;          /* 1: reproduce `pressure *= fulltime` (ShellSystem.C:1480) */
+  int32_t quirk_volume_schedule;   /* 1: reproduce r(t) = max(1-floor(5t)*0.01, 0.5) (ShellSystem.C:1474) */
+  int32_t reserved;
+} shx_params;
+
+typedef struct {
+  double t, dt;                    /* time and next dt after the last accepted step */
+  double residual;                 /* ||f|| / ||f_ext|| at the last accepted force evaluation (undamped f) */
+  double fnorm, fext_norm;
+  double e_tens, e_bend, e_self, e_fric, e_kin;
+  double volume, area;
+  double du_max;
+  int32_t n_contacts, n_pairs, n_rebuilds, problem;
+  int32_t n_steps, n_rejected;     /* accepted steps and rejected attempts since shx_init_dynamics */
+  int32_t converged_step;          /* first accepted step with t >= load end and residual <= tol, or -1 */
+  int32_t reserved;
+} shx_report;
+
+/* Replaces: ShellSystem constructor + set_parameters + build_cache +
+ * compute_mass_damping (ShellSystem.C:23-142, 170-418, 490-741, 762-852).
+ * X: nv*3 mesh (control point) positions; tri: ne*3 vertex ids, outward
+ * counter-clockwise.  Computes patches, basis tables and masses on the GPU. */
+int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tri,
+               const shx_params* prm, int device);
+int shx_destroy(shx_ctx* ctx);
+const char* shx_last_error(const shx_ctx* ctx);
+
+/* Host<->device state, FP64 node-interleaved; replaces direct access to the
+ * ScalarVectors u, v, a (ShellSystem.h:116) and `time`. */
+int shx_set_state(shx_ctx* ctx, const double* u, const double* v, const double* a, double t);
+int shx_get_state(shx_ctx* ctx, double* u, double* v, double* a, double* t);
+
+/* One force evaluation at displacement u_pred with velocities v (contact
+ * uses v) and time t: compute_deformed_metrics + compute_external_force +
+ * add_shell_shell_contact_force + subtract_internal_force
+ * (ShellSystem.C:1382-1464, 1467-1585, 1792-1907, 1125-1314).
+ * precision: SHX_FP64 or SHX_FP32_SPLIT.  f: undamped total force; f_ext:
+ * pressure, volume and contact part; energies[4] = E_tens, E_bend, E_self,
+ * E_fric; elem_area[ne] deformed element areas.  Any output may be NULL. */
+int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* v, double t,
+               double* f, double* f_ext, double* energies, double* elem_area);
+
+/* apply_initial_conditions (ShellSystem.C:855-905) from the state set by
+ * shx_set_state: volume0, forced contact rebuild, a = M^-1 f. */
+int shx_init_dynamics(shx_ctx* ctx, int precision);
+
+/* n_steps accepted steps of ShellSystem::advance (ShellSystem.C:952-1122),
+ * including step rejection when adapt_dt_every > 0.  Stops early when the
+ * residual criterion is met if stop_on_converged != 0 (tol: relative
+ * residual, checked once t >= max(pressure_ramp_time, volume_ramp_time)). */
+int shx_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged,
+             shx_report* rep);
+
+/* compute_shell_shell_contact_candidates (ShellSystem.C:1910-2135) at the
+ * current positions; force != 0 rebuilds regardless of du_sum. */
+int shx_contact_rebuild(shx_ctx* ctx, int force);
+/* Sorted (eid1 < eid2) candidate list; returns the count (may exceed cap). */
+int shx_pairs(shx_ctx* ctx, uint32_t* e1, uint32_t* e2, int cap);
+
+/* Per element, the oracle layout: phi, dphi1, dphi2, d2phi11, d2phi22,
+ * d2phi12 (6x12, zero padded), A1, A2, A11, A22, A12, ff1_ref, ff2_ref, JxW,
+ * R00, R01, R10, R11, xyz -> 101 doubles (build_cache/reinit_mod output). */
+int shx_elem_tables(shx_ctx* ctx, double* out);
+int shx_mass(shx_ctx* ctx, double* Minv, double* C);
+int shx_patches(shx_ctx* ctx, int32_t* idx /* ne*12, -1 padded */, int32_t* tri /* ne*3 */);
+
+/* Largest angular frequency of M^-1 K at u = 0 (power iteration on the GPU
+ * force kernel); the stable central-difference step is 2/omega. */
+int shx_cfl_omega_max(shx_ctx* ctx, int iters, double* omega);
+
+/* Device-resident timing hooks for benchmarks: the HIP stream the context
+ * launches on (hipStream_t) and the average duration in ms of each kernel
+ * class measured with HIP events over the last shx_step call when
+ * profiling is enabled (shx_set_profiling). Classes: 0 element force,
+ * 1 node gather + Newmark, 2 contact narrow phase, 3 finalize, 4 broad phase. */
+int shx_set_profiling(shx_ctx* ctx, int enable);
+int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 5 */, int64_t* counts /* 5 */);
+
+/* Primitives exposed for parity tests (host-compiled copies of the device
+ * routines; no GPU needed). */
+void shx_tri_distance(const double* p /* 18 */, double* out /* d2, dc[3], v[2], w[2] */);
+int  shx_raw_elem_basis(int valence, double* out /* 6*(valence+6) */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,293 @@
+"""CPU FP64 oracle of the explicit force-and-step path (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package, and only as the checker or as the timed CPU baseline.
+The product library never loads it.
+
+Parity status of each oracle function (what pins it) is listed in
+oracle/PINNING.md.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "E1", "E2", "nu", "G12", "thickness", "rho", "memb_stiff_mult", "bend_stiff_mult",
+        "damping_alpha", "damping_beta", "damping_gamma", "pressure_final", "pressure_ramp_time",
+        "bulk_modulus", "volume_final_ratio", "volume_ramp_time", "contact_stiffness", "mu_s", "mu_d",
+        "self_damping", "self_friction_damping", "fric_stiff_mult", "friction_stop_velocity",
+        "static_friction_tol", "cell_size_mult", "cell_margin", "neighbor_margin", "newmark_beta",
+        "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
+        (n, ctypes.c_int32) for n in ("adapt_dt_every",
+        "contact", "quirk_fulltime", "quirk_volume_schedule", "omp_faithful")]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "t", "residual", "fnorm", "fext_norm", "e_tens", "e_bend", "e_self", "e_fric", "e_kin",
+        "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
+        "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_rejected")] + [("dt", ctypes.c_double)]
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+_U = ctypes.POINTER(ctypes.c_uint32)
+_V = ctypes.c_void_p
+
+
+def _declare(L):
+    L.orc_create.restype = _V
+    L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _D, _I, ctypes.POINTER(Params)]
+    L.orc_destroy.argtypes = [_V]
+    L.orc_last_error.restype = ctypes.c_char_p
+    for name in ("orc_patch_size",):
+        getattr(L, name).argtypes = [_V, _I]
+    L.orc_patches.argtypes = [_V, _I, _I]
+    L.orc_elem_tables.argtypes = [_V, _D]
+    L.orc_mass.argtypes = [_V, _D, _D]
+    L.orc_set_state.argtypes = [_V, _D, _D, _D, ctypes.c_double]
+    L.orc_get_state.argtypes = [_V, _D, _D, _D, _D]
+    L.orc_forces.argtypes = [_V, _D, _D, ctypes.c_double, _D, _D, _D, _D]
+    L.orc_internal_force.argtypes = [_V, _D, _D]
+    L.orc_init_dynamics.argtypes = [_V]
+    L.orc_step.argtypes = [_V, ctypes.c_int, ctypes.POINTER(Report)]
+    L.orc_contact_rebuild.argtypes = [_V, ctypes.c_int]
+    L.orc_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
+    L.orc_pair_state.argtypes = [_V, _D, _I, _D, ctypes.c_int]
+    L.orc_close_counts.argtypes = [_V, _I]
+    L.orc_cfl_omega_max.restype = ctypes.c_double
+    L.orc_cfl_omega_max.argtypes = [_V, ctypes.c_int, ctypes.c_uint64]
+    L.orc_icosphere_counts.argtypes = [ctypes.c_int, _I, _I]
+    L.orc_icosphere.argtypes = [ctypes.c_int, ctypes.c_int, _D, _I]
+    L.orc_stam_basis.argtypes = [ctypes.c_double, ctypes.c_double, _D]
+    L.orc_raw_elem_basis.argtypes = [ctypes.c_int, _D]
+    L.orc_loop_subdiv_rows.argtypes = [ctypes.c_int, _D]
+    L.orc_tri_distance.argtypes = [_D, _D]
+    L.orc_segment_distance.argtypes = [_D, _D]
+    L.orc_point_triangle.argtypes = [_D, _D]
+    L.orc_friction_apply.argtypes = [_D]
+    L.orc_contact_force.restype = ctypes.c_double
+    L.orc_contact_force.argtypes = [ctypes.c_double] * 3
+    L.orc_contact_energy.restype = ctypes.c_double
+    L.orc_contact_energy.argtypes = [ctypes.c_double] * 3
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def up_(a):
+    return a.ctypes.data_as(_U)
+
+
+def icosphere(level, round_g=True):
+    nv = ctypes.c_int32()
+    ne = ctypes.c_int32()
+    lib().orc_icosphere_counts(level, ctypes.byref(nv), ctypes.byref(ne))
+    X = np.zeros((nv.value, 3))
+    T = np.zeros((ne.value, 3), dtype=np.int32)
+    lib().orc_icosphere(level, 1 if round_g else 0, dp(X), ip(T))
+    return X, T
+
+
+def make_params(**kw):
+    p = Params()
+    defaults = default_params()
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_params():
+    return dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_stiff_mult=1.0,
+                bend_stiff_mult=1.0, damping_alpha=0.05, damping_beta=0.0, damping_gamma=0.0,
+                pressure_final=0.0, pressure_ramp_time=0.0, bulk_modulus=0.0, volume_final_ratio=1.0,
+                volume_ramp_time=0.0, contact_stiffness=0.0, mu_s=0.0, mu_d=0.0, self_damping=0.0,
+                self_friction_damping=0.0, fric_stiff_mult=1.0, friction_stop_velocity=1e-3,
+                static_friction_tol=0.1, cell_size_mult=4.0, cell_margin=0.02, neighbor_margin=0.4,
+                newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0, rel_loc_err_min=5e-5,
+                rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0, quirk_fulltime=0,
+                quirk_volume_schedule=0, omp_faithful=0)
+
+
+class Oracle:
+    """Handle on one oracle shell system (FP64, CPU)."""
+
+    def __init__(self, X, T, **params):
+        self.X = np.ascontiguousarray(X, dtype=np.float64)
+        self.T = np.ascontiguousarray(T, dtype=np.int32)
+        self.nv, self.ne = len(self.X), len(self.T)
+        self.params = make_params(**params)
+        self.h = lib().orc_create(self.nv, self.ne, dp(self.X), ip(self.T), ctypes.byref(self.params))
+        if not self.h:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def patches(self):
+        idx = np.zeros((self.ne, 12), dtype=np.int32)
+        tri = np.zeros((self.ne, 3), dtype=np.int32)
+        lib().orc_patches(self.h, ip(idx), ip(tri))
+        N = np.zeros(self.ne, dtype=np.int32)
+        lib().orc_patch_size(self.h, ip(N))
+        return idx, tri, N
+
+    def elem_tables(self):
+        out = np.zeros((self.ne, 101))
+        lib().orc_elem_tables(self.h, dp(out))
+        return out
+
+    def mass(self):
+        Minv = np.zeros(3 * self.nv)
+        C = np.zeros(3 * self.nv)
+        lib().orc_mass(self.h, dp(Minv), dp(C))
+        return Minv, C
+
+    def set_state(self, u, v=None, a=None, t=0.0):
+        z = np.zeros(3 * self.nv)
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        v = z if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        a = z if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        lib().orc_set_state(self.h, dp(u), dp(v), dp(a), t)
+
+    def get_state(self):
+        u, v, a = (np.zeros(3 * self.nv) for _ in range(3))
+        t = ctypes.c_double()
+        lib().orc_get_state(self.h, dp(u), dp(v), dp(a), ctypes.byref(t))
+        return u, v, a, t.value
+
+    def forces(self, u_pred, v=None, t=0.0):
+        u_pred = np.ascontiguousarray(u_pred, dtype=np.float64).reshape(-1)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        f = np.zeros(3 * self.nv)
+        fext = np.zeros(3 * self.nv)
+        en = np.zeros(4)
+        area = np.zeros(self.ne)
+        lib().orc_forces(self.h, dp(u_pred), dp(vv) if vv is not None else None, t, dp(f), dp(fext),
+                         dp(en), dp(area))
+        return f, fext, en, area
+
+    def internal_force(self, u_pred):
+        u_pred = np.ascontiguousarray(u_pred, dtype=np.float64).reshape(-1)
+        f = np.zeros(3 * self.nv)
+        lib().orc_internal_force(self.h, dp(u_pred), dp(f))
+        return f
+
+    def init_dynamics(self):
+        lib().orc_init_dynamics(self.h)
+
+    def step(self, n=1):
+        rep = Report()
+        lib().orc_step(self.h, n, ctypes.byref(rep))
+        return {k: getattr(rep, k) for k, _ in Report._fields_}
+
+    def rebuild(self, force=True):
+        return lib().orc_contact_rebuild(self.h, 1 if force else 0)
+
+    def pairs(self):
+        n = lib().orc_pairs(self.h, None, None, 0)
+        e1 = np.zeros(max(n, 1), dtype=np.uint32)
+        e2 = np.zeros(max(n, 1), dtype=np.uint32)
+        lib().orc_pairs(self.h, up_(e1), up_(e2), n)
+        return e1[:n], e2[:n]
+
+    def pair_state(self):
+        n = lib().orc_pairs(self.h, None, None, 0)
+        d2 = np.zeros(max(n, 1))
+        st = np.zeros(max(n, 1), dtype=np.int32)
+        so = np.zeros((max(n, 1), 3))
+        lib().orc_pair_state(self.h, dp(d2), ip(st), dp(so), n)
+        return d2[:n], st[:n], so[:n]
+
+    def cfl_omega_max(self, iters=60, seed=1):
+        return lib().orc_cfl_omega_max(self.h, iters, seed)
+
+
+def stam_basis(v, w):
+    out = np.zeros(72)
+    lib().orc_stam_basis(v, w, dp(out))
+    return out.reshape(6, 12)
+
+
+def raw_elem_basis(valence):
+    out = np.zeros(6 * (valence + 6))
+    if lib().orc_raw_elem_basis(valence, dp(out)) != 0:
+        raise ValueError(valence)
+    return out.reshape(6, valence + 6)
+
+
+def subdiv_rows(valence):
+    S = np.zeros(12 * (valence + 6))
+    lib().orc_loop_subdiv_rows(valence, dp(S))
+    return S.reshape(12, valence + 6)
+
+
+def tri_distance(P):
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(18)
+    out = np.zeros(8)
+    lib().orc_tri_distance(dp(P), dp(out))
+    return out
+
+
+def segment_distance(P):
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(12)
+    out = np.zeros(5)
+    lib().orc_segment_distance(dp(P), dp(out))
+    return out
+
+
+def point_triangle(P):
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(12)
+    out = np.zeros(5)
+    lib().orc_point_triangle(dp(P), dp(out))
+    return out
+
+
+def friction_apply(io):
+    io = np.ascontiguousarray(io, dtype=np.float64).copy()
+    lib().orc_friction_apply(dp(io))
+    return io
+
+
+def contact_force(k, h, d):
+    return lib().orc_contact_force(k, h, d)
+
+
+def contact_energy(k, h, d):
+    return lib().orc_contact_energy(k, h, d)
