@@ -77,6 +77,7 @@ def _declare(L):
     L.orc_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
     L.orc_pair_state.argtypes = [_V, _D, _I, _D, ctypes.c_int]
     L.orc_close_counts.argtypes = [_V, _I]
+    L.orc_close_sets.argtypes = [_V, _I, _I, ctypes.c_int]
     L.orc_cfl_omega_max.restype = ctypes.c_double
     L.orc_cfl_omega_max.argtypes = [_V, ctypes.c_int, ctypes.c_uint64]
     L.orc_icosphere_counts.argtypes = [ctypes.c_int, _I, _I]
@@ -234,6 +235,13 @@ class Oracle:
         so = np.zeros((max(n, 1), 3))
         lib().orc_pair_state(self.h, dp(d2), ip(st), dp(so), n)
         return d2[:n], st[:n], so[:n]
+
+    def close_sets(self):
+        start = np.zeros(self.ne + 1, dtype=np.int32)
+        n = lib().orc_close_sets(self.h, ip(start), None, 0)
+        ids = np.zeros(max(n, 1), dtype=np.int32)
+        lib().orc_close_sets(self.h, ip(start), ip(ids), n)
+        return start, ids[:n]
 
     def cfl_omega_max(self, iters=60, seed=1):
         return lib().orc_cfl_omega_max(self.h, iters, seed)

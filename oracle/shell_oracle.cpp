@@ -793,7 +793,10 @@ void compute_bounding_box(System& s)
     const double d1 = c.bmax[1] - c.bmin[1], d2 = c.bmax[2] - c.bmin[2];
     if (d1 > dmax) dmax = d1;
     if (d2 > dmax) dmax = d2;
-    part += dmax;
+    bool finite = true;
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < 3; ++q) finite = finite && std::isfinite(s.p[s.tri[3 * e + k]](q));
+    part += finite ? dmax : std::numeric_limits<double>::quiet_NaN();
     if ((e & 255) == 255 || e == s.ne - 1) { total += part; part = 0; }
   }
   s.avg_elem_size = total / s.ne;
@@ -856,6 +859,7 @@ void contact_candidates(System& s, bool force)
   compute_bounding_box(s);
   for (int q = 0; q < 3; ++q)
     if (!std::isfinite(s.bmin[q]) || !std::isfinite(s.bmax[q])) { s.reg(NOT_FINITE); return; }
+  if (!std::isfinite(s.avg_elem_size)) { s.reg(NOT_FINITE); return; }
   cell_size = P.cell_size_mult * s.avg_elem_size;
   tolerance = P.cell_margin * cell_size;
   cell_size += tolerance;
@@ -1284,6 +1288,18 @@ int orc_pair_state(void* h, double* d2, int32_t* stick, double* so, int cap)
     }
     ++k;
   }
+  return k;
+}
+
+int orc_close_sets(void* h, int32_t* start, int32_t* ids, int cap)
+{
+  System& s = *static_cast<System*>(h);
+  int k = 0;
+  for (int e = 0; e < s.ne; ++e) {
+    if (start) start[e] = k;
+    for (int c : s.ec[e].close_elems) { if (ids && k < cap) ids[k] = c; ++k; }
+  }
+  if (start) start[s.ne] = k;
   return k;
 }
 

@@ -99,6 +99,7 @@ int  orc_contact_rebuild(void* h, int force);
 int  orc_pairs(void* h, uint32_t* e1, uint32_t* e2, int cap);   /* returns count, sorted */
 int  orc_pair_state(void* h, double* d2, int32_t* stick, double* spring_origin, int cap);
 int  orc_close_counts(void* h, int32_t* counts);
+int  orc_close_sets(void* h, int32_t* start /* ne+1 */, int32_t* ids, int cap);
 double orc_cfl_omega_max(void* h, int iters, uint64_t seed);
 
 /* primitives (for tests) */

@@ -1,0 +1,254 @@
+"""shell_amd -- MI355X (gfx950) explicit force-and-step path of the
+Loop-subdivision Koiter shell solver, behind the C-ABI of include/shellhip.h.
+
+The Python layer is a thin ctypes binding used by tests and the benchmark;
+all computation runs in libshellhip.so (HIP kernels).  `ShellSystem` mirrors
+the reference's public/protected interface of class ShellSystem
+(ShellSystem.h:264-306): advance(), compute_shell_shell_contact_candidates(),
+and a force evaluation equal to compute_deformed_metrics +
+compute_external_force + subtract_internal_force.  The library must be built
+(python shell_amd/build.py); there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libshellhip.so")
+
+FP64 = 0
+FP32_SPLIT = 1
+
+PROBLEMS = ["NO_PROBLEM", "NO_DISP_CHANGE", "LARGE_DISP", "STOP_SIM", "TOO_MANY_CELLS",
+            "SMALL_TIMESTEP", "CONTACT_PROBLEM", "NOT_FINITE"]
+
+EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_get_state",
+           "shx_forces", "shx_init_dynamics", "shx_step", "shx_contact_rebuild", "shx_pairs",
+           "shx_elem_tables", "shx_mass", "shx_patches", "shx_cfl_omega_max", "shx_set_profiling",
+           "shx_kernel_times", "shx_tri_distance", "shx_raw_elem_basis"]
+
+_lib = None
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "E1", "E2", "nu", "G12", "thickness", "rho", "memb_stiff_mult", "bend_stiff_mult",
+        "damping_alpha", "damping_beta", "damping_gamma", "pressure_final", "pressure_ramp_time",
+        "bulk_modulus", "volume_final_ratio", "volume_ramp_time", "contact_stiffness", "mu_s", "mu_d",
+        "self_damping", "self_friction_damping", "fric_stiff_mult", "friction_stop_velocity",
+        "static_friction_tol", "cell_size_mult", "cell_margin", "neighbor_margin", "newmark_beta",
+        "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
+        (n, ctypes.c_int32) for n in ("adapt_dt_every", "contact", "quirk_fulltime",
+                                      "quirk_volume_schedule", "reserved")]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "t", "dt", "residual", "fnorm", "fext_norm", "e_tens", "e_bend", "e_self", "e_fric", "e_kin",
+        "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
+        "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_steps", "n_rejected", "converged_step",
+        "reserved")]
+
+
+DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_stiff_mult=1.0,
+                bend_stiff_mult=1.0, damping_alpha=0.05, damping_beta=0.0, damping_gamma=0.0,
+                pressure_final=0.0, pressure_ramp_time=0.0, bulk_modulus=0.0, volume_final_ratio=1.0,
+                volume_ramp_time=0.0, contact_stiffness=0.0, mu_s=0.0, mu_d=0.0, self_damping=0.0,
+                self_friction_damping=0.0, fric_stiff_mult=1.0, friction_stop_velocity=1e-3,
+                static_friction_tol=0.1, cell_size_mult=4.0, cell_margin=0.02, neighbor_margin=0.4,
+                newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0,
+                rel_loc_err_min=5e-5, rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0,
+                quirk_fulltime=0, quirk_volume_schedule=0, reserved=0)
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+_U = ctypes.POINTER(ctypes.c_uint32)
+_V = ctypes.c_void_p
+
+
+def lib():
+    """Load libshellhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with `python shell_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.shx_create.argtypes = [ctypes.POINTER(_V), ctypes.c_int, ctypes.c_int, _D, _I,
+                                 ctypes.POINTER(Params), ctypes.c_int]
+        L.shx_destroy.argtypes = [_V]
+        L.shx_last_error.restype = ctypes.c_char_p
+        L.shx_last_error.argtypes = [_V]
+        L.shx_set_state.argtypes = [_V, _D, _D, _D, ctypes.c_double]
+        L.shx_get_state.argtypes = [_V, _D, _D, _D, _D]
+        L.shx_forces.argtypes = [_V, ctypes.c_int, _D, _D, ctypes.c_double, _D, _D, _D, _D]
+        L.shx_init_dynamics.argtypes = [_V, ctypes.c_int]
+        L.shx_step.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                               ctypes.POINTER(Report)]
+        L.shx_contact_rebuild.argtypes = [_V, ctypes.c_int]
+        L.shx_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
+        L.shx_elem_tables.argtypes = [_V, _D]
+        L.shx_mass.argtypes = [_V, _D, _D]
+        L.shx_patches.argtypes = [_V, _I, _I]
+        L.shx_cfl_omega_max.argtypes = [_V, ctypes.c_int, _D]
+        L.shx_set_profiling.argtypes = [_V, ctypes.c_int]
+        L.shx_kernel_times.argtypes = [_V, _D, ctypes.POINTER(ctypes.c_int64)]
+        L.shx_tri_distance.argtypes = [_D, _D]
+        L.shx_raw_elem_basis.argtypes = [ctypes.c_int, _D]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _vec(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def make_params(**kw):
+    unknown = set(kw) - set(DEFAULTS)
+    if unknown:
+        raise KeyError(f"unknown parameters: {sorted(unknown)}")
+    p = Params()
+    d = dict(DEFAULTS)
+    d.update(kw)
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+class ShellError(RuntimeError):
+    pass
+
+
+class ShellSystem:
+    """One shell on one GPU (mirrors class ShellSystem, ShellSystem.h:28)."""
+
+    def __init__(self, X, T, device=0, **params):
+        self.X = np.ascontiguousarray(X, dtype=np.float64)
+        self.T = np.ascontiguousarray(T, dtype=np.int32)
+        self.nv, self.ne = len(self.X), len(self.T)
+        self.params = make_params(**params)
+        h = _V()
+        r = lib().shx_create(ctypes.byref(h), self.nv, self.ne, _dp(self.X), self.T.ctypes.data_as(_I),
+                             ctypes.byref(self.params), device)
+        if r != 0:
+            raise ShellError(f"shx_create: {r}: {lib().shx_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, r, what):
+        if r < 0:
+            raise ShellError(f"{what}: {r}: {lib().shx_last_error(self.h).decode()}")
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state
+    def set_state(self, u, v=None, a=None, t=0.0):
+        n = 3 * self.nv
+        u = _vec(u, n)
+        v = None if v is None else _vec(v, n)
+        a = None if a is None else _vec(a, n)
+        self._check(lib().shx_set_state(self.h, _dp(u), _dp(v), _dp(a), t), "shx_set_state")
+
+    def get_state(self):
+        u, v, a = (np.zeros(3 * self.nv) for _ in range(3))
+        t = ctypes.c_double()
+        self._check(lib().shx_get_state(self.h, _dp(u), _dp(v), _dp(a), ctypes.byref(t)), "shx_get_state")
+        return u, v, a, t.value
+
+    # path
+    def forces(self, u_pred, v=None, t=0.0, precision=FP64):
+        n = 3 * self.nv
+        u_pred = _vec(u_pred, n)
+        v = None if v is None else _vec(v, n)
+        f, fe = np.zeros(n), np.zeros(n)
+        en, area = np.zeros(4), np.zeros(self.ne)
+        self._check(lib().shx_forces(self.h, precision, _dp(u_pred), _dp(v), t, _dp(f), _dp(fe), _dp(en),
+                                     _dp(area)), "shx_forces")
+        return f, fe, en, area
+
+    def apply_initial_conditions(self, precision=FP64):
+        self._check(lib().shx_init_dynamics(self.h, precision), "shx_init_dynamics")
+
+    def advance(self, n_steps=1, tol=0.0, stop_on_converged=False, precision=FP64):
+        rep = Report()
+        self._check(lib().shx_step(self.h, precision, n_steps, tol, 1 if stop_on_converged else 0,
+                                   ctypes.byref(rep)), "shx_step")
+        return {k: getattr(rep, k) for k, _ in Report._fields_}
+
+    def compute_shell_shell_contact_candidates(self, force=False):
+        return self._check(lib().shx_contact_rebuild(self.h, 1 if force else 0), "shx_contact_rebuild")
+
+    def pairs(self):
+        n = self._check(lib().shx_pairs(self.h, None, None, 0), "shx_pairs")
+        e1 = np.zeros(max(n, 1), dtype=np.uint32)
+        e2 = np.zeros(max(n, 1), dtype=np.uint32)
+        self._check(lib().shx_pairs(self.h, e1.ctypes.data_as(_U), e2.ctypes.data_as(_U), n), "shx_pairs")
+        return e1[:n], e2[:n]
+
+    def elem_tables(self):
+        out = np.zeros((self.ne, 101))
+        self._check(lib().shx_elem_tables(self.h, _dp(out)), "shx_elem_tables")
+        return out
+
+    def mass(self):
+        m, c = np.zeros(3 * self.nv), np.zeros(3 * self.nv)
+        self._check(lib().shx_mass(self.h, _dp(m), _dp(c)), "shx_mass")
+        return m, c
+
+    def patches(self):
+        idx = np.zeros((self.ne, 12), dtype=np.int32)
+        tri = np.zeros((self.ne, 3), dtype=np.int32)
+        self._check(lib().shx_patches(self.h, idx.ctypes.data_as(_I), tri.ctypes.data_as(_I)), "shx_patches")
+        return idx, tri
+
+    def cfl_omega_max(self, iters=200):
+        w = ctypes.c_double()
+        self._check(lib().shx_cfl_omega_max(self.h, iters, ctypes.byref(w)), "shx_cfl_omega_max")
+        return w.value
+
+    def set_profiling(self, enable=True):
+        self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
+
+    def kernel_times(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, dtype=np.int64)
+        self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+                    "shx_kernel_times")
+        names = ["elem_force", "node_newmark", "contact_narrow", "finalize", "broad_phase"]
+        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(names)}
+
+
+def tri_distance(P):
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(18)
+    out = np.zeros(8)
+    lib().shx_tri_distance(_dp(P), _dp(out))
+    return out
+
+
+def raw_elem_basis(valence):
+    out = np.zeros(6 * (valence + 6))
+    if lib().shx_raw_elem_basis(valence, _dp(out)) != 0:
+        raise ValueError(valence)
+    return out.reshape(6, valence + 6)

@@ -1,0 +1,29 @@
+/*
+ * shx_kernels.h -- kernel entry points of libshellhip (defined in
+ * shx_kernels.hip, launched from shx_api.hip).
+ */
+#ifndef SHX_KERNELS_H
+#define SHX_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include "shx_device.h"
+
+namespace shx {
+
+/* kernel modes: a full step attempt (predictor from the current state), a
+ * plain force evaluation at the current u (shx_forces), or the initial
+ * acceleration a = M^-1 f (apply_initial_conditions) */
+enum { M_STEP = 0, M_FORCES = 1, M_INIT = 2 };
+
+__global__ void k_precompute(DPtrs P, SConst C);
+__global__ void k_mass(DPtrs P, SConst C, double damping_alpha, double damping_beta, double damping_gamma);
+__global__ void k_elem(DPtrs P, SConst C, int mode);
+__global__ void k_contact(DPtrs P, SConst C, int mode);
+__global__ void k_node(DPtrs P, SConst C, int mode, double t_force, int nblkE);
+__global__ void k_gate(DPtrs P, SConst C, int nblkE);
+__global__ void k_final(DPtrs P, SConst C, int mode, int nblkE, int nblkN, int nblkC);
+__global__ void k_broad(DPtrs P, SConst C, int posmode);
+
+} // namespace shx
+#endif

@@ -1,0 +1,989 @@
+/*
+ * shx_kernels.hip -- gfx950 kernels of libshellhip: reference geometry
+ * precompute, lumped mass, element force (metrics, strains, stresses),
+ * node gather fused with the Newmark corrector/predictor, contact narrow
+ * phase with friction, per-step finalize (reductions, adaptive dt, rebuild
+ * triggers) and the linked-cell broad phase (one cooperative kernel).
+ *
+ * All arithmetic is FP64 and compiled with -ffp-contract=off so that it
+ * rounds like the CPU oracle it is checked against.
+ */
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+
+#include "shx_device.h"
+#include "shx_kernels.h"
+#include "shx_math.h"
+
+namespace cg = cooperative_groups;
+
+namespace shx {
+
+__device__ __forceinline__ v3 ld3(const double* p, int n) { return mk(p[3 * n], p[3 * n + 1], p[3 * n + 2]); }
+__device__ __forceinline__ void st3(double* p, int n, v3 a) { p[3 * n] = a.x; p[3 * n + 1] = a.y; p[3 * n + 2] = a.z; }
+__device__ __forceinline__ v3 ldrow3(const double* t, int row, int ne, int e)
+{
+  return mk(t[(size_t)row * ne + e], t[(size_t)(row + 1) * ne + e], t[(size_t)(row + 2) * ne + e]);
+}
+
+/* u_pred = u; u_pred.add(dt, v); u_pred.add(a8, a)  (ShellSystem.C:973-975) */
+__device__ __forceinline__ double pred1(double u, double v, double a, double dt, double a8)
+{
+  double x = u;
+  x = dt * v + x;
+  x = a8 * a + x;
+  return x;
+}
+__device__ __forceinline__ v3 pred3(v3 u, v3 v, v3 a, double dt, double a8)
+{
+  return mk(pred1(u.x, v.x, a.x, dt, a8), pred1(u.y, v.y, a.y, dt, a8), pred1(u.z, v.z, a.z, dt, a8));
+}
+
+__device__ double pressure_at(const SConst& C, double t)
+{
+  double p = C.pressure_final;
+  if (C.pressure_ramp_time > 0) p *= fmin(t / C.pressure_ramp_time, 1.0);
+  if (C.quirk_fulltime) p *= 0.0;
+  return p;
+}
+
+__device__ double volume_ratio_at(const SConst& C, double t)
+{
+  if (C.quirk_volume_schedule) return fmax((1.0 - (double)((int)(t * 5.0)) * 0.01), 0.5);
+  if (C.volume_ramp_time <= 0) return C.volume_final_ratio > 0 ? C.volume_final_ratio : 1.0;
+  return 1.0 - (1.0 - C.volume_final_ratio) * fmin(t / C.volume_ramp_time, 1.0);
+}
+
+/* reparametrised basis of slot i of an element (reinit_mod,
+ * ShellSystem.C:3466-3478) from the raw basis rb (6 x 12) and R */
+struct Basis6 { double phi, d1, d2, d11, d22, d12; };
+__device__ __forceinline__ Basis6 basis_at(const double* rb, int i, double R00, double R01, double R10,
+                                           double R11)
+{
+  Basis6 b;
+  const double dx = rb[12 + i], de = rb[24 + i];
+  const double dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+  b.phi = rb[i];
+  b.d1 = dx * R00 + de * R10;
+  b.d2 = dx * R01 + de * R11;
+  b.d11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
+  b.d22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
+  b.d12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
+  return b;
+}
+
+__device__ __forceinline__ void load_rawb(const double* g, double* s)
+{
+  for (int q = threadIdx.x; q < 4 * 72; q += blockDim.x) s[q] = g[q];
+}
+
+/* ------------------------------------------------------------ precompute */
+
+/* build_cache + reinit_mod + compute_reference_metrics for one element
+ * (ShellSystem.C:490-571, 3321-3499, 1317-1379).  The Rodrigues rotation
+ * applied to both A and B in the reference (ShellSystem.C:3375-3421) cancels
+ * in R = (Rn A)^-1 (Rn B) = A^-1 B and is not formed. */
+__global__ void __launch_bounds__(256) k_precompute(DPtrs P, SConst C)
+{
+  __shared__ double s_rb[4 * 72];
+  load_rawb(P.rawb, s_rb);
+  __syncthreads();
+  const int ne = C.ne;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const double* rb = s_rb + 72 * (P.vcls[e] - 3);
+  v3 xyz = mk(0, 0, 0), b1 = xyz, b2 = xyz;
+  for (int i = 0; i < 12; ++i) {
+    const v3 x = ld3(P.X, P.pidx[(size_t)i * ne + e]);
+    axpy(xyz, x, rb[i]);
+    axpy(b1, x, rb[12 + i]);
+    axpy(b2, x, rb[24 + i]);
+  }
+  const v3 nu = crs(b1, b2);
+  const double nn = nrm(nu);
+  const double JxW = 0.5 * nn;
+  const v3 n = scl(nu, 1.0 / nn);
+  const v3 dir = mk(0, 0, 1);
+  v3 j = sub(dir, scl(n, dot(dir, n)));
+  v3 k = neg(crs(n, j));
+  const double ang = atan2(nrm(crs(j, b1)), dot(j, b1));
+  const double jfac = nn / (nrm(b1) * sin(ang));
+  const double kfac = nn / jfac;
+  j = scl(j, jfac / nrm(j));
+  k = scl(k, kfac / nrm(k));
+  /* A = [b1 b2 nu], B = [k j nu] (columns); R = A^-1 B, 2x2 block */
+  const double a00 = b1.x, a01 = b2.x, a02 = nu.x;
+  const double a10 = b1.y, a11 = b2.y, a12 = nu.y;
+  const double a20 = b1.z, a21 = b2.z, a22 = nu.z;
+  const double c00 = a11 * a22 - a12 * a21, c01 = a12 * a20 - a10 * a22, c02 = a10 * a21 - a11 * a20;
+  const double c10 = a02 * a21 - a01 * a22, c11 = a00 * a22 - a02 * a20, c12 = a01 * a20 - a00 * a21;
+  const double idet = 1 / (a00 * c00 + a01 * c01 + a02 * c02);
+  /* rows 0 and 1 of A^-1: (c00 c10 c20), (c01 c11 c21) times idet */
+  const double c20 = a01 * a12 - a02 * a11, c21 = a02 * a10 - a00 * a12;
+  const double i00 = idet * c00, i01 = idet * c10, i02 = idet * c20;
+  const double i10 = idet * c01, i11 = idet * c11, i12 = idet * c21;
+  const double R00 = i00 * k.x + i01 * k.y + i02 * k.z;
+  const double R01 = i00 * j.x + i01 * j.y + i02 * j.z;
+  const double R10 = i10 * k.x + i11 * k.y + i12 * k.z;
+  const double R11 = i10 * j.x + i11 * j.y + i12 * j.z;
+  v3 A1 = mk(0, 0, 0), A2 = A1, A11 = A1, A22 = A1, A12 = A1;
+  for (int i = 0; i < 12; ++i) {
+    const v3 x = ld3(P.X, P.pidx[(size_t)i * ne + e]);
+    const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
+    axpy(A1, x, b.d1);
+    axpy(A2, x, b.d2);
+    axpy(A11, x, b.d11);
+    axpy(A22, x, b.d22);
+    axpy(A12, x, b.d12);
+  }
+  v3 A3 = crs(A1, A2);
+  A3 = scl(A3, 1 / nrm(A3));
+  double* T = P.tab;
+  auto put = [&](int row, double v) { T[(size_t)row * ne + e] = v; };
+  auto put3 = [&](int row, v3 v) { put(row, v.x); put(row + 1, v.y); put(row + 2, v.z); };
+  put(T_R00, R00); put(T_R01, R01); put(T_R10, R10); put(T_R11, R11);
+  put(T_JXW, JxW);
+  put3(T_A1, A1); put3(T_A2, A2); put3(T_A11, A11); put3(T_A22, A22); put3(T_A12, A12);
+  put3(T_FF1, mk(dot(A1, A1), dot(A2, A2), dot(A1, A2)));
+  put3(T_FF2, mk(dot(A11, A3), dot(A22, A3), dot(A12, A3)));
+  put3(T_XYZ, xyz);
+  P.area[e] = 0.5 * nrm(crs(A1, A2));
+}
+
+/* compute_mass_damping (ShellSystem.C:762-852) as a node gather: each node
+ * sums its element contributions in element order, then C += gamma and
+ * M is inverted. */
+__global__ void __launch_bounds__(256) k_mass(DPtrs P, SConst C, double damping_alpha, double damping_beta,
+                                              double damping_gamma)
+{
+  __shared__ double s_rb[4 * 72];
+  load_rawb(P.rawb, s_rb);
+  __syncthreads();
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= C.nv) return;
+  const int ne = C.ne;
+  double m = 0, c = 0;
+  const double rh = C.rho * C.h;
+  for (int q = P.inc_start[n]; q < P.inc_start[n + 1]; ++q) {
+    const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
+    const int N = P.vcls[e] + 6;
+    const double* rb = s_rb + 72 * (P.vcls[e] - 3);
+    double lumped = 0;
+    for (int jj = 0; jj < N; ++jj) lumped += rb[jj];
+    lumped *= P.tab[(size_t)T_JXW * ne + e] * rb[i];
+    const double Me = lumped * rh;
+    double hmin = 1e300, hmax = 0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b) {
+        const double d = nrm(sub(ld3(P.X, P.tri[(size_t)a * ne + e]), ld3(P.X, P.tri[(size_t)b * ne + e])));
+        hmin = fmin(hmin, d); hmax = fmax(hmax, d);
+      }
+    const double avg_stiff = 2 * (C.K + C.D / (hmin * hmax)) / 3;
+    m += Me;
+    c += damping_alpha * Me + damping_beta * avg_stiff;
+  }
+  P.cdamp[n] = c + damping_gamma;
+  P.minv[n] = 1 / m;
+}
+
+/* --------------------------------------------------------- element force */
+
+/* One thread per element: gather u_pred over the patch, form the deformed
+ * metric (compute_deformed_metrics, ShellSystem.C:1407-1460), membrane and
+ * bending strains, orthotropic H, stress resultants and the p-vectors
+ * (subtract_internal_force, ShellSystem.C:1164-1233), and write the element's
+ * generalised stresses G.  The nodal force of patch slot i is then
+ *   f_i = -(phi1 G1 + phi2 G2 - phi11 G3 - phi22 G4 - phi12 G5) + p phi G0,
+ * which is the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis
+ * derivative (ShellSystem.C:1236-1282) plus the LOCAL pressure load
+ * (ShellSystem.C:1570-1576); the node kernel forms it as a gather. */
+__global__ void __launch_bounds__(256) k_elem(DPtrs P, SConst C, int mode)
+{
+  __shared__ double s_rb[4 * 72];
+  __shared__ double s_red[5][256];
+  const DShell& sh = *P.sh;
+  if (sh.done) return;
+  load_rawb(P.rawb, s_rb);
+  __syncthreads();
+  const int ne = C.ne;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = e < ne;
+  const bool predict = (mode == M_STEP);
+  const double dt = sh.dt, a8 = sh.nm[8];
+  const int par = sh.par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* Aa = P.A[par];
+  double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  if (valid) {
+    const double* T = P.tab;
+    const double R00 = T[(size_t)T_R00 * ne + e], R01 = T[(size_t)T_R01 * ne + e];
+    const double R10 = T[(size_t)T_R10 * ne + e], R11 = T[(size_t)T_R11 * ne + e];
+    const int N = P.vcls[e];
+    const double* rb = s_rb + 72 * (N - 3);
+    v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
+    v3 xv[3];
+    for (int i = 0; i < 12; ++i) {
+      const int n = P.pidx[(size_t)i * ne + e];
+      const v3 u = ld3(U, n);
+      v3 up = u;
+      if (predict) up = pred3(u, ld3(V, n), ld3(Aa, n), dt, a8);
+      const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
+      axpy(du1, up, b.d1);
+      axpy(du2, up, b.d2);
+      axpy(du11, up, b.d11);
+      axpy(du22, up, b.d22);
+      axpy(du12, up, b.d12);
+      const int vslot = (i == 0) ? 0 : (i == 1) ? 1 : (i == N) ? 2 : -1;
+      if (vslot >= 0) {
+        if (predict) {
+          const v3 d = sub(up, u);
+          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+        }
+        xv[vslot] = add(ld3(P.X, n), u);
+      }
+    }
+    const v3 a1 = add(ldrow3(T, T_A1, ne, e), du1);
+    const v3 a2 = add(ldrow3(T, T_A2, ne, e), du2);
+    v3 a3 = crs(a1, a2);
+    const double jac = nrm(a3);
+    a3 = scl(a3, 1 / jac);
+    const v3 a11 = add(ldrow3(T, T_A11, ne, e), du11);
+    const v3 a22 = add(ldrow3(T, T_A22, ne, e), du22);
+    const v3 a12 = add(ldrow3(T, T_A12, ne, e), du12);
+    const double ff1[3] = {dot(a1, a1), dot(a2, a2), dot(a1, a2)};
+    const double ff2[3] = {dot(a11, a3), dot(a22, a3), dot(a12, a3)};
+    const v3 g = ldrow3(T, T_FF1, ne, e);
+    const v3 fr2 = ldrow3(T, T_FF2, ne, e);
+    double alpha[3] = {0.5 * (ff1[0] - g.x), 0.5 * (ff1[1] - g.y), 0.5 * (ff1[2] - g.z)};
+    double beta[3] = {fr2.x - ff2[0], fr2.y - ff2[1], fr2.z - ff2[2]};
+    alpha[2] *= 2;
+    beta[2] *= 2;
+    /* orthotropic H, ShellSystem.C:1199-1213 */
+    double H[9];
+    H[0] = C.E1 * g.y * g.y;
+    H[1] = H[3] = C.E1 * (C.nu * g.y * g.x + (1 - C.nu) * g.z * g.z);
+    H[2] = H[6] = C.E1 * -g.y * g.z;
+    H[4] = C.E2 * g.x * g.x;
+    H[5] = H[7] = C.E2 * -g.x * g.z;
+    H[8] = C.G12 * ((1 - C.nu * C.nu2) * g.y * g.x + (1 - C.nu * C.nu2) * ((1 + C.nu) / (1 - C.nu)) * g.z * g.z);
+    const double invdet = 1 / (g.x * g.y - g.z * g.z);
+    for (int q = 0; q < 9; ++q) H[q] *= invdet * invdet;
+    double nn[3], mm[3];
+    for (int r = 0; r < 3; ++r) {
+      nn[r] = C.K * (H[r * 3] * alpha[0] + H[r * 3 + 1] * alpha[1] + H[r * 3 + 2] * alpha[2]);
+      mm[r] = C.D * (H[r * 3] * beta[0] + H[r * 3 + 1] * beta[1] + H[r * 3 + 2] * beta[2]);
+    }
+    const v3 a2xa3 = crs(a2, a3), a3xa1 = crs(a3, a1);
+    const double invjac = 1 / jac;
+    const v3 p11 = scl(add(crs(a11, a2), scl(a2xa3, ff2[0])), invjac);
+    const v3 p12 = scl(add(crs(a22, a2), scl(a2xa3, ff2[1])), invjac);
+    const v3 p13 = scl(add(crs(a12, a2), scl(a2xa3, ff2[2])), invjac);
+    const v3 p21 = scl(add(crs(a1, a11), scl(a3xa1, ff2[0])), invjac);
+    const v3 p22 = scl(add(crs(a1, a22), scl(a3xa1, ff2[1])), invjac);
+    const v3 p23 = scl(add(crs(a1, a12), scl(a3xa1, ff2[2])), invjac);
+    const double JxW = T[(size_t)T_JXW * ne + e];
+    const double m2x2 = 2 * mm[2];
+    v3 g1 = add(add(add(add(scl(a1, nn[0]), scl(a2, nn[2])), scl(p11, mm[0])), scl(p12, mm[1])), scl(p13, m2x2));
+    v3 g2 = add(add(add(add(scl(a2, nn[1]), scl(a1, nn[2])), scl(p21, mm[0])), scl(p22, mm[1])), scl(p23, m2x2));
+    const v3 G[6] = {scl(a3, JxW), scl(g1, JxW), scl(g2, JxW), scl(a3, JxW * mm[0]), scl(a3, JxW * mm[1]),
+                     scl(a3, JxW * m2x2)};
+    for (int r = 0; r < 6; ++r) {
+      P.G[(size_t)(3 * r) * ne + e] = G[r].x;
+      P.G[(size_t)(3 * r + 1) * ne + e] = G[r].y;
+      P.G[(size_t)(3 * r + 2) * ne + e] = G[r].z;
+    }
+    Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
+    Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
+    const double new_area = 0.5 * jac;
+    oarea = (mode == M_INIT) ? new_area : P.area[e];
+    P.area[e] = new_area;
+    /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
+    const v3 a = xv[0], b = xv[1], c = xv[2];
+    det = a.x * (b.y * c.z - c.y * b.z) - b.x * (a.y * c.z - c.y * a.z) + c.x * (a.y * b.z - b.y * a.z);
+  }
+  const int t = threadIdx.x;
+  s_red[0][t] = Et; s_red[1][t] = Eb; s_red[2][t] = dumax; s_red[3][t] = det; s_red[4][t] = oarea;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      s_red[0][t] += s_red[0][t + s];
+      s_red[1][t] += s_red[1][t + s];
+      s_red[2][t] = fmax(s_red[2][t], s_red[2][t + s]);
+      s_red[3][t] += s_red[3][t + s];
+      s_red[4][t] += s_red[4][t + s];
+    }
+    __syncthreads();
+  }
+  if (t < 5) P.epart[(size_t)blockIdx.x * 5 + t] = s_red[t][0];
+}
+
+/* --------------------------------------------------------------- contact */
+
+__device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, const double* V, const double* A,
+                                       bool predict, double dt, double a8)
+{
+  const v3 x = ld3(P.X, n);
+  v3 u = ld3(U, n);
+  if (predict) u = pred3(u, ld3(V, n), ld3(A, n), dt, a8);
+  return add(x, u);
+}
+
+/* add_shell_shell_contact_force, ShellSystem.C:1792-1907: per candidate
+ * pair, distance, log-barrier normal force, normal damping and Margolis
+ * friction; the force is split -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w)
+ * onto element 2 and accumulated with FP64 atomics into fc. */
+__global__ void __launch_bounds__(256) k_contact(DPtrs P, SConst C, int mode)
+{
+  __shared__ double s_red[3][256];
+  const DShell& sh = *P.sh;
+  if (sh.done) return;
+  const bool predict = (mode == M_STEP);
+  const double dt = sh.dt, a8 = sh.nm[8];
+  const int par = sh.par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* A = P.A[par];
+  const int np = sh.n_pairs, buf = sh.pair_buf;
+  const int ne = C.ne;
+  const double h = C.h;
+  double nc = 0, es = 0, ef = 0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    const int e[2] = {P.pe1[buf][p], P.pe2[buf][p]};
+    v3 X0[3], X1[3];
+    int nid[2][3];
+    for (int k = 0; k < 3; ++k) {
+      nid[0][k] = P.tri[(size_t)k * ne + e[0]];
+      nid[1][k] = P.tri[(size_t)k * ne + e[1]];
+      X0[k] = node_pos(P, nid[0][k], U, V, A, predict, dt, a8);
+      X1[k] = node_pos(P, nid[1][k], U, V, A, predict, dt, a8);
+    }
+    tri_hit hit;
+    tri_tri(X0, X1, hit);
+    P.pd2[buf][p] = hit.d2;
+    if (hit.d2 <= 0) atomicMax(&P.sh->problem, 6); /* CONTACT_PROBLEM */
+    int stick = P.pstick[buf][p];
+    v3 so = ld3(P.pso[buf], p);
+    if (hit.d2 < C.d2_friction) {
+      const double amin = fmin(P.area[e[0]], P.area[e[1]]);
+      const double adc = sqrt(hit.d2);
+      const double depth = h - adc;
+      const v3 n = mk(hit.dc.x / adc, hit.dc.y / adc, hit.dc.z / adc);
+      const double k = C.contact_stiffness > 0 ? C.contact_stiffness : C.stiff_density * amin;
+      double fn = 0, Es = 0, Ef = 0;
+      if (depth > 0) {
+        nc += 1;
+        fn = contact_force(k, h, adc);
+        Es = contact_energy(k, h, adc);
+      }
+      es += Es;
+      v3 vv[2][3];
+      for (int q = 0; q < 3; ++q) { vv[0][q] = ld3(V, nid[0][q]); vv[1][q] = ld3(V, nid[1][q]); }
+      const double u0 = 1 - hit.v0 - hit.w0, u1 = 1 - hit.v1 - hit.w1;
+      const v3 vr = sub(add(add(scl(vv[1][0], u1), scl(vv[1][1], hit.v1)), scl(vv[1][2], hit.w1)),
+                        add(add(scl(vv[0][0], u0), scl(vv[0][1], hit.v0)), scl(vv[0][2], hit.w0)));
+      const double vn = dot(vr, n);
+      v3 fcon = mk(0, 0, 0);
+      if (C.self_friction) {
+        const v3 cp = add(add(add(scl(X0[0], u0), scl(X0[1], hit.v0)), scl(X0[2], hit.w0)), scl(hit.dc, 0.5));
+        friction(Ef, fcon, fn, n, vr, vn, C.friction_stop_velocity, C.stiff_static_friction, C.mu_s, C.mu_d,
+                 C.self_friction_damping, stick, so, cp);
+        ef += Ef;
+      }
+      fn -= C.self_damping * k * vn;
+      fcon = add(fcon, scl(n, fn));
+      const double wv[2] = {hit.v0, hit.v1}, ww[2] = {hit.w0, hit.w1};
+      for (int s = 0; s < 2; ++s) {
+        const double vb = wv[s], wb = ww[s], ub = 1 - vb - wb;
+        const double pre = s == 0 ? -0.5 : 0.5;
+        const double fq[3] = {pre * fcon.x, pre * fcon.y, pre * fcon.z};
+        for (int q = 0; q < 3; ++q) {
+          atomicAdd(&P.fc[3 * nid[s][0] + q], ub * fq[q]);
+          atomicAdd(&P.fc[3 * nid[s][1] + q], vb * fq[q]);
+          atomicAdd(&P.fc[3 * nid[s][2] + q], wb * fq[q]);
+        }
+      }
+    } else {
+      so = mk(0, 0, 0);
+      stick = 0;
+    }
+    P.pstick[buf][p] = stick;
+    st3(P.pso[buf], p, so);
+  }
+  const int t = threadIdx.x;
+  s_red[0][t] = nc; s_red[1][t] = es; s_red[2][t] = ef;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) { s_red[0][t] += s_red[0][t + s]; s_red[1][t] += s_red[1][t + s]; s_red[2][t] += s_red[2][t + s]; }
+    __syncthreads();
+  }
+  if (t < 3) P.cpart[(size_t)blockIdx.x * 3 + t] = s_red[t][0];
+}
+
+/* ------------------------------------------------------------ node force */
+
+/* One thread per node: gather f from the generalised stresses of every
+ * (element, slot) whose patch holds the node, add the contact force, then
+ * the damped central-difference / Newmark update (ShellSystem.C:1006-1008,
+ * 1068-1090): a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a,
+ * v = v_pred + gamma dt a, written to the other state buffer.  Block
+ * partials: |f|^2, |f_ext|^2, max|a-a_old|, max|v_pred|, max|du|^2, E_kin. */
+__global__ void __launch_bounds__(256) k_node(DPtrs P, SConst C, int mode, double t_force, int nblkE)
+{
+  __shared__ double s_rb[4 * 72];
+  __shared__ double s_red[6][256];
+  __shared__ double s_vol;
+  const DShell& sh = *P.sh;
+  if (sh.done) return;
+  load_rawb(P.rawb, s_rb);
+  const int t = threadIdx.x;
+  if (t == 0) {
+    double V = sh.volume;
+    if (C.bulk_modulus != 0 && ((mode == M_STEP && sh.fresh) || mode == M_INIT)) {
+      double d = 0, ar = 0;
+      for (int b = 0; b < nblkE; ++b) { d += P.epart[5 * b + 3]; ar += P.epart[5 * b + 4]; }
+      V = fabs(d / 6) - ar * 0.5 * C.h;
+    }
+    s_vol = V;
+  }
+  __syncthreads();
+  const int ne = C.ne;
+  const int n = blockIdx.x * blockDim.x + t;
+  const int par = sh.par;
+  const double dt = sh.dt;
+  double r_f2 = 0, r_fe2 = 0, r_err = 0, r_vmax = 0, r_du = 0, r_ek = 0;
+  if (n < C.nv) {
+    const double t_eval = (mode == M_STEP) ? sh.t + dt : t_force;
+    double ptot = pressure_at(C, t_eval);
+    if (C.bulk_modulus != 0) {
+      const double V = s_vol;
+      const double V0 = (mode == M_INIT) ? V : sh.volume0;
+      ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / V - 1);
+    }
+    v3 fint = mk(0, 0, 0), fp = fint;
+    for (int q = P.inc_start[n]; q < P.inc_start[n + 1]; ++q) {
+      const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
+      const double* rb = s_rb + 72 * (P.vcls[e] - 3);
+      const double* T = P.tab;
+      const Basis6 b = basis_at(rb, i, T[(size_t)T_R00 * ne + e], T[(size_t)T_R01 * ne + e],
+                                T[(size_t)T_R10 * ne + e], T[(size_t)T_R11 * ne + e]);
+      const double* G = P.G;
+      v3 g[6];
+      for (int r = 0; r < 6; ++r)
+        g[r] = mk(G[(size_t)(3 * r) * ne + e], G[(size_t)(3 * r + 1) * ne + e], G[(size_t)(3 * r + 2) * ne + e]);
+      const v3 qe = sub(sub(sub(add(scl(g[1], b.d1), scl(g[2], b.d2)), scl(g[3], b.d11)), scl(g[4], b.d22)),
+                        scl(g[5], b.d12));
+      fint = sub(fint, qe);
+      axpy(fp, g[0], b.phi);
+    }
+    const v3 fcn = ld3(P.fc, n);
+    st3(P.fc, n, mk(0, 0, 0));
+    const v3 fext = add(scl(fp, ptot), fcn);
+    const v3 f = add(fint, fext);
+    r_f2 = nrm2(f);
+    r_fe2 = nrm2(fext);
+    const double mi = P.minv[n];
+    if (mode == M_FORCES) {
+      st3(P.fout, n, f);
+      st3(P.fext_out, n, fext);
+    } else if (mode == M_INIT) {
+      st3(P.A[par], n, scl(f, mi));
+    } else {
+      const double cd = P.cdamp[n];
+      const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
+      const double a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];
+      const v3 up = pred3(u, v, a, dt, a8);
+      const v3 vp = mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
+      const v3 an = mk(mi * (f.x - cd * vp.x), mi * (f.y - cd * vp.y), mi * (f.z - cd * vp.z));
+      r_err = fmax(fabs(an.x - a.x), fmax(fabs(an.y - a.y), fabs(an.z - a.z)));
+      r_vmax = fmax(fabs(vp.x), fmax(fabs(vp.y), fabs(vp.z)));
+      const v3 un = mk(a0 * an.x + up.x, a0 * an.y + up.y, a0 * an.z + up.z);
+      const v3 vn = mk(a7 * an.x + vp.x, a7 * an.y + vp.y, a7 * an.z + vp.z);
+      const v3 d = sub(un, u);
+      r_du = d.x * d.x + d.y * d.y + d.z * d.z;
+      r_ek = 0.5 * (vn.x * vn.x / mi + vn.y * vn.y / mi + vn.z * vn.z / mi);
+      st3(P.U[1 - par], n, un);
+      st3(P.V[1 - par], n, vn);
+      st3(P.A[1 - par], n, an);
+    }
+  }
+  s_red[0][t] = r_f2; s_red[1][t] = r_fe2; s_red[2][t] = r_err; s_red[3][t] = r_vmax; s_red[4][t] = r_du;
+  s_red[5][t] = r_ek;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      s_red[0][t] += s_red[0][t + s];
+      s_red[1][t] += s_red[1][t + s];
+      s_red[2][t] = fmax(s_red[2][t], s_red[2][t + s]);
+      s_red[3][t] = fmax(s_red[3][t], s_red[3][t + s]);
+      s_red[4][t] = fmax(s_red[4][t], s_red[4][t + s]);
+      s_red[5][t] += s_red[5][t + s];
+    }
+    __syncthreads();
+  }
+  if (t < 6) P.npart[(size_t)blockIdx.x * 6 + t] = s_red[t][0];
+}
+
+/* --------------------------------------------------------------- finalize */
+
+__device__ void set_newmark_dev(DShell& s, const SConst& C, double dt)
+{
+  dt = fmax(dt, C.dt_min);
+  dt = fmin(dt, C.dt_max);
+  s.dt = dt;
+  const double b = C.beta, g = C.gamma;
+  s.nm[0] = b * dt * dt;
+  s.nm[1] = b != 0 ? g / (b * dt) : 0;
+  s.nm[2] = b != 0 ? 1 / (b * dt) : 0;
+  s.nm[3] = b != 0 ? 1 / (2 * b) - 1 : 0;
+  s.nm[4] = b != 0 ? g / b - 1 : 0;
+  s.nm[5] = b != 0 ? dt * (g / (2 * b) - 1) : 0;
+  s.nm[6] = (1 - g) * dt;
+  s.nm[7] = g * dt;
+  s.nm[8] = dt * dt * (0.5 - b);
+  s.nm[9] = dt * dt * fabs(b - 1. / 6);
+}
+
+/* check_increment (ShellSystem.C:2591-2652) on a max |du|^2 */
+__device__ void check_increment_dev(DShell& s, const SConst& C, double du2max, bool require_change)
+{
+  const double d = sqrt(du2max);
+  s.du_max = d;
+  if (isfinite(d)) {
+    s.du_sum += 2 * d;
+    if (require_change && d == 0) s.problem = max(s.problem, 1);
+    else if (C.contact && d > 0.5 * C.h) s.problem = max(s.problem, 2);
+  } else s.problem = max(s.problem, 7);
+}
+
+/* broad-phase trigger at the predictor (ShellSystem.C:978, 993, 1917-1919) */
+__global__ void k_gate(DPtrs P, SConst C, int nblkE)
+{
+  if (threadIdx.x != 0) return;
+  DShell& s = *P.sh;
+  if (s.done) { s.rebuild = 0; return; }
+  double du = 0;
+  for (int b = 0; b < nblkE; ++b) du = fmax(du, P.epart[5 * b + 2]);
+  check_increment_dev(s, C, du, false);
+  s.rebuild = C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
+}
+
+/* reductions, local-error estimate and step acceptance
+ * (ShellSystem.C:1010-1117) */
+__global__ void k_final(DPtrs P, SConst C, int mode, int nblkE, int nblkN, int nblkC)
+{
+  if (threadIdx.x != 0) return;
+  DShell& s = *P.sh;
+  if (s.done) return;
+  double et = 0, eb = 0, det = 0, ar = 0;
+  for (int b = 0; b < nblkE; ++b) {
+    et += P.epart[5 * b]; eb += P.epart[5 * b + 1]; det += P.epart[5 * b + 3]; ar += P.epart[5 * b + 4];
+  }
+  double f2 = 0, fe2 = 0, err = 0, vmax = 0, du = 0, ek = 0;
+  for (int b = 0; b < nblkN; ++b) {
+    const double* q = P.npart + 6 * b;
+    f2 += q[0]; fe2 += q[1]; err = fmax(err, q[2]); vmax = fmax(vmax, q[3]); du = fmax(du, q[4]); ek += q[5];
+  }
+  double nc = 0, es = 0, ef = 0;
+  for (int b = 0; b < nblkC; ++b) { nc += P.cpart[3 * b]; es += P.cpart[3 * b + 1]; ef += P.cpart[3 * b + 2]; }
+  const double V = fabs(det / 6) - ar * 0.5 * C.h;
+  s.e_tens = et; s.e_bend = eb; s.e_self = es; s.e_fric = ef;
+  s.n_contacts = (int)nc;
+  s.fnorm = sqrt(f2);
+  s.fext_norm = sqrt(fe2);
+  if (mode == M_INIT) {
+    s.volume = s.volume0 = V;
+    s.area = ar;
+    return;
+  }
+  if (mode == M_FORCES) return;
+  s.attempts++;
+  if (s.fresh) { s.volume = V; s.area = ar; }
+  const double dt = s.dt;
+  const double target = sqrt(C.rel_loc_err_min * C.rel_loc_err_max);
+  double rel = target, new_dt = dt;
+  const bool adapt_now = C.adapt_dt_every > 0 && ((s.n_steps + 1) % C.adapt_dt_every) == 0;
+  if (adapt_now) {
+    if (isfinite(err)) rel = s.nm[9] * err / C.h;
+    else s.problem = max(s.problem, 7);
+    if (rel == 0) new_dt = 2 * dt;
+    else if (rel > C.rel_loc_err_max || rel < C.rel_loc_err_min) new_dt = dt * pow(target / rel, 1. / 3);
+    if (C.contact) {
+      if (new_dt * vmax * 10 * 2 > C.h) new_dt = C.h / (2 * 10 * vmax);
+    }
+  }
+  if (new_dt >= dt) {
+    check_increment_dev(s, C, du, true);
+    s.par = 1 - s.par;
+    s.t += dt;
+    s.n_steps++;
+    s.fresh = 1;
+    s.residual = s.fext_norm > 0 ? s.fnorm / s.fext_norm : s.fnorm;
+    s.e_kin = ek;
+    s.rebuild = C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
+    set_newmark_dev(s, C, new_dt);
+    if (s.converged_step < 0 && s.t >= C.load_end_time && s.residual <= C.conv_tol) {
+      s.converged_step = s.n_steps;
+      if (C.stop_on_converged) s.done = 1;
+    }
+    if (s.n_steps >= s.steps_target) s.done = 1;
+  } else {
+    s.n_rejected++;
+    s.fresh = 0;
+    s.rebuild = 0;
+    set_newmark_dev(s, C, new_dt);
+    if (s.dt <= 2.220446049250313e-16) { s.problem = max(s.problem, 5); s.done = 1; }
+  }
+  if (s.problem >= 7 || s.err) s.done = 1;
+}
+
+/* ------------------------------------------------------------ broad phase */
+
+__device__ __forceinline__ void vload(volatile int* p, int& out) { out = *p; }
+
+/* grid barrier with release/acquire fences around it */
+__device__ __forceinline__ void gsync(cg::grid_group& grid)
+{
+  __threadfence();
+  grid.sync();
+  __threadfence();
+}
+
+/* exclusive scan of 256 values in a block; returns the prefix, total in *tot */
+__device__ uint32_t block_scan_u32(uint32_t v, uint32_t* s, uint32_t* tot)
+{
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t add_ = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += add_;
+    __syncthreads();
+  }
+  const uint32_t incl = s[t];
+  *tot = s[255];
+  __syncthreads();
+  return incl - v;
+}
+
+/* exclusive scan of cnt[0..n) into out[0..n], out[n] = total, over the whole
+ * cooperative grid: contiguous chunk per block, block totals scanned by
+ * block 0. */
+__device__ void grid_scan(cg::grid_group& grid, const uint32_t* cnt, uint32_t* out, int n, uint32_t* tmp,
+                          uint32_t* s_scan)
+{
+  const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+  const int chunk = (n + G - 1) / G;
+  const int lo = min(n, b * chunk), hi = min(n, lo + chunk);
+  uint32_t run = 0;
+  for (int base = lo; base < hi; base += 256) {
+    const int i = base + t;
+    const uint32_t v = i < hi ? cnt[i] : 0;
+    uint32_t tot;
+    const uint32_t pre = block_scan_u32(v, s_scan, &tot);
+    if (i < hi) out[i] = run + pre;
+    run += tot;
+  }
+  if (t == 0) tmp[b] = run;
+  gsync(grid);
+  if (b == 0 && t == 0) {
+    uint32_t acc = 0;
+    for (int q = 0; q < G; ++q) { const uint32_t x = tmp[q]; tmp[q] = acc; acc += x; }
+    out[n] = acc;
+  }
+  gsync(grid);
+  const uint32_t off = tmp[b];
+  for (int i = lo + t; i < hi; i += 256) out[i] += off;
+  gsync(grid);
+}
+
+__device__ bool close_pair(const DPtrs& P, int e1, int e2)
+{
+  int lo = P.close_start[e1], hi = P.close_start[e1 + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int x = P.close_ids[mid];
+    if (x == e2) return true;
+    if (x < e2) lo = mid + 1; else hi = mid;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double ebox(const DPtrs& P, int ne, int r, int e) { return P.ebox[(size_t)r * ne + e]; }
+
+/* BoundingBox::apart, BoundingBox.h:64-69 */
+__device__ bool boxes_apart(const DPtrs& P, int ne, int a, int b, double d)
+{
+  for (int q = 0; q < 3; ++q)
+    if (ebox(P, ne, q, a) > ebox(P, ne, 3 + q, b) + d || ebox(P, ne, q, b) > ebox(P, ne, 3 + q, a) + d) return true;
+  return false;
+}
+
+/* compute_shell_shell_contact_candidates after the trigger
+ * (ShellSystem.C:1922-2113), one cooperative launch:
+ *  1 element AABBs and tile partials (compute_bounding_box);
+ *  2 global box, averaged size, cell size, d_max, grid (block 0);
+ *  3 cell index ranges, cell counters cleared;
+ *  4 counting: one atomic increment per (element, cell);
+ *  5 prefix scan of the cell counts;
+ *  6 scatter of element ids into the cells;
+ *  7 per element: surviving old pairs (stick or d2 <= d2_max) and new
+ *    candidates of its cells (first common cell only, not close, boxes within
+ *    d_max, exact d2 <= d2_max), counted;
+ *  8 prefix scan of the counts;
+ *  9 sorted merge into the other pair buffer.
+ * posmode: M_STEP positions X + u_pred, otherwise X + u. */
+__global__ void __launch_bounds__(256) k_broad(DPtrs P, SConst C, int posmode)
+{
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double s_box[6][256];
+  __shared__ double s_size[256];
+  __shared__ uint32_t s_scan[256];
+  DShell* sh = P.sh;
+  {
+    int rb;
+    vload(&sh->rebuild, rb);
+    if (!rb) return;
+  }
+  const int ne = C.ne, t = threadIdx.x, G = gridDim.x;
+  const bool predict = (posmode == M_STEP);
+  const double dt = sh->dt, a8 = sh->nm[8];
+  const int par = sh->par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* A = P.A[par];
+  /* 1 */
+  const int ntiles = (ne + 255) / 256;
+  for (int tile = blockIdx.x; tile < ntiles; tile += G) {
+    const int e = tile * 256 + t;
+    double bmn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double bmx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    double size = 0;
+    if (e < ne) {
+      bool finite = true;
+      for (int k = 0; k < 3; ++k) {
+        const v3 x = node_pos(P, P.tri[(size_t)k * ne + e], U, V, A, predict, dt, a8);
+        const double xx[3] = {x.x, x.y, x.z};
+        finite = finite && isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
+        for (int q = 0; q < 3; ++q) {
+          if (xx[q] < bmn[q]) bmn[q] = xx[q];
+          if (xx[q] > bmx[q]) bmx[q] = xx[q];
+        }
+      }
+      for (int q = 0; q < 3; ++q) { P.ebox[(size_t)q * ne + e] = bmn[q]; P.ebox[(size_t)(3 + q) * ne + e] = bmx[q]; }
+      double dmax = bmx[0] - bmn[0];
+      const double d1 = bmx[1] - bmn[1], d2 = bmx[2] - bmn[2];
+      if (d1 > dmax) dmax = d1;
+      if (d2 > dmax) dmax = d2;
+      size = finite ? dmax : __builtin_nan(""); /* poisons the tile sum: the rebuild aborts */
+    }
+    for (int q = 0; q < 3; ++q) { s_box[q][t] = bmn[q]; s_box[3 + q][t] = bmx[q]; }
+    s_size[t] = size;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s)
+        for (int q = 0; q < 3; ++q) {
+          s_box[q][t] = fmin(s_box[q][t], s_box[q][t + s]);
+          s_box[3 + q][t] = fmax(s_box[3 + q][t], s_box[3 + q][t + s]);
+        }
+      __syncthreads();
+    }
+    if (t == 0) {
+      double sum = 0;
+      const int m = min(256, ne - tile * 256);
+      for (int q = 0; q < m; ++q) sum += s_size[q];
+      for (int q = 0; q < 6; ++q) P.bpart[(size_t)tile * 7 + q] = s_box[q][0];
+      P.bpart[(size_t)tile * 7 + 6] = sum;
+    }
+    __syncthreads();
+  }
+  gsync(grid);
+  /* 2 */
+  if (blockIdx.x == 0 && t == 0) {
+    double bmn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double bmx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    double total = 0;
+    for (int tile = 0; tile < ntiles; ++tile) {
+      const double* q = P.bpart + (size_t)tile * 7;
+      for (int c = 0; c < 3; ++c) { bmn[c] = fmin(bmn[c], q[c]); bmx[c] = fmax(bmx[c], q[3 + c]); }
+      total += q[6];
+    }
+    sh->du_sum = 0;
+    sh->n_rebuilds++;
+    sh->avg_elem_size = total / ne;
+    bool ok = true;
+    for (int c = 0; c < 3; ++c) {
+      sh->bmin[c] = bmn[c]; sh->bmax[c] = bmx[c];
+      if (!isfinite(bmn[c]) || !isfinite(bmx[c])) ok = false;
+    }
+    if (!isfinite(total)) ok = false;
+    if (!ok) { sh->problem = max(sh->problem, 7); sh->rebuild = 0; }
+    else {
+      double cell = C.cell_size_mult * sh->avg_elem_size;
+      const double tol = C.cell_margin * cell;
+      cell += tol;
+      const double dmx = C.h + tol;
+      sh->cell_size = cell;
+      sh->d_max = dmx;
+      sh->d2_max = dmx * dmx;
+      unsigned nd[3];
+      for (int c = 0; c < 3; ++c) nd[c] = (unsigned)(int)(((bmx[c] - bmn[c]) + dmx) / cell + 1);
+      const unsigned ncell = nd[0] * nd[1] * nd[2];
+      for (int c = 0; c < 3; ++c) sh->nd[c] = (int)nd[c];
+      sh->n_cells = (int)ncell;
+      if (ncell > C.n_cells_max) { sh->problem = max(sh->problem, 4); sh->rebuild = 0; }
+      else if (ncell + 1 > P.cell_cap) { sh->err |= ERR_CELLS; sh->rebuild = 0; }
+    }
+  }
+  gsync(grid);
+  {
+    int rb;
+    vload(&sh->rebuild, rb);
+    if (!rb) return;
+  }
+  const double cell = sh->cell_size, dmx = sh->d_max, d2mx = sh->d2_max;
+  const double gmin[3] = {sh->bmin[0], sh->bmin[1], sh->bmin[2]};
+  const int nd0 = sh->nd[0], nd1 = sh->nd[1], nd2 = sh->nd[2];
+  const int ncell = sh->n_cells;
+  const int gid = blockIdx.x * 256 + t, gstride = G * 256;
+  /* 3 */
+  for (int c = gid; c <= ncell; c += gstride) P.cell_cnt[c] = 0;
+  for (int e = gid; e < ne; e += gstride)
+    for (int q = 0; q < 3; ++q) {
+      P.ecell[(size_t)q * ne + e] = (int)((ebox(P, ne, q, e) - gmin[q]) / cell);
+      P.ecell[(size_t)(3 + q) * ne + e] = (int)((ebox(P, ne, 3 + q, e) - gmin[q] + dmx) / cell);
+    }
+  gsync(grid);
+  /* 4 */
+  for (int e = gid; e < ne; e += gstride) {
+    const int* L = P.ecell;
+    for (int cx = L[e]; cx <= L[(size_t)3 * ne + e]; ++cx)
+      for (int cy = L[(size_t)ne + e]; cy <= L[(size_t)4 * ne + e]; ++cy)
+        for (int cz = L[(size_t)2 * ne + e]; cz <= L[(size_t)5 * ne + e]; ++cz)
+          atomicAdd(&P.cell_cnt[(size_t)cx * nd1 * nd2 + (size_t)cy * nd2 + cz], 1u);
+  }
+  gsync(grid);
+  /* 5 */
+  grid_scan(grid, P.cell_cnt, P.cell_start, ncell, P.scan_tmp, s_scan);
+  if (P.cell_start[ncell] > (uint32_t)P.ent_cap) {
+    if (blockIdx.x == 0 && t == 0) { sh->err |= ERR_ENTRIES; sh->rebuild = 0; }
+    return; /* uniform: every block reads the same total */
+  }
+  for (int c = gid; c < ncell; c += gstride) P.cell_cnt[c] = 0;
+  gsync(grid);
+  /* 6 */
+  for (int e = gid; e < ne; e += gstride) {
+    const int* L = P.ecell;
+    for (int cx = L[e]; cx <= L[(size_t)3 * ne + e]; ++cx)
+      for (int cy = L[(size_t)ne + e]; cy <= L[(size_t)4 * ne + e]; ++cy)
+        for (int cz = L[(size_t)2 * ne + e]; cz <= L[(size_t)5 * ne + e]; ++cz) {
+          const size_t c = (size_t)cx * nd1 * nd2 + (size_t)cy * nd2 + cz;
+          const uint32_t pos = P.cell_start[c] + atomicAdd(&P.cell_cnt[c], 1u);
+          P.cell_elems[pos] = e;
+        }
+  }
+  gsync(grid);
+  /* 7 */
+  const int ob = sh->pair_buf, nb = 1 - ob;
+  for (int e1 = gid; e1 < ne; e1 += gstride) {
+    const int* L = P.ecell;
+    const int lo1[3] = {L[e1], L[(size_t)ne + e1], L[(size_t)2 * ne + e1]};
+    const int hi1[3] = {L[(size_t)3 * ne + e1], L[(size_t)4 * ne + e1], L[(size_t)5 * ne + e1]};
+    const int r0 = P.row_start[ob][e1], r1 = P.row_start[ob][e1 + 1];
+    int nsurv = 0;
+    for (int r = r0; r < r1; ++r)
+      if (P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx)) ++nsurv;
+    v3 X0[3];
+    for (int k = 0; k < 3; ++k) X0[k] = node_pos(P, P.tri[(size_t)k * ne + e1], U, V, A, predict, dt, a8);
+    int nfound = 0;
+    for (int cx = lo1[0]; cx <= hi1[0]; ++cx)
+      for (int cy = lo1[1]; cy <= hi1[1]; ++cy)
+        for (int cz = lo1[2]; cz <= hi1[2]; ++cz) {
+          const size_t c = (size_t)cx * nd1 * nd2 + (size_t)cy * nd2 + cz;
+          for (uint32_t q = P.cell_start[c]; q < P.cell_start[c + 1]; ++q) {
+            const int e2 = P.cell_elems[q];
+            if (e2 <= e1) continue;
+            /* count the pair once: in the first cell the two ranges share */
+            if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)ne + e2]) ||
+                cz != max(lo1[2], L[(size_t)2 * ne + e2]))
+              continue;
+            if (close_pair(P, e1, e2)) continue;
+            bool listed = false;
+            {
+              int lo = r0, hi = r1;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const int x = P.pe2[ob][mid];
+                if (x == e2) { listed = P.pstick[ob][mid] || !(P.pd2[ob][mid] > d2mx); break; }
+                if (x < e2) lo = mid + 1; else hi = mid;
+              }
+            }
+            if (listed) continue;
+            if (boxes_apart(P, ne, e1, e2, dmx)) continue;
+            v3 X1[3];
+            for (int k = 0; k < 3; ++k) X1[k] = node_pos(P, P.tri[(size_t)k * ne + e2], U, V, A, predict, dt, a8);
+            tri_hit hit;
+            tri_tri(X0, X1, hit);
+            if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
+            if (hit.d2 > d2mx) continue;
+            if (nfound < kRowCap) {
+              P.found_e2[(size_t)e1 * kRowCap + nfound] = e2;
+              P.found_d2[(size_t)e1 * kRowCap + nfound] = hit.d2;
+            } else atomicOr(&sh->err, ERR_ROW);
+            ++nfound;
+          }
+        }
+    P.found_cnt[e1] = nsurv + min(nfound, kRowCap);
+    P.found_cnt[(size_t)ne + 1 + e1] = min(nfound, kRowCap); /* second half: number of new pairs */
+  }
+  gsync(grid);
+  /* 8 */
+  grid_scan(grid, (const uint32_t*)P.found_cnt, (uint32_t*)P.row_start[nb], ne, P.scan_tmp, s_scan);
+  const int total = P.row_start[nb][ne];
+  if (total > P.pair_cap) {
+    if (blockIdx.x == 0 && t == 0) { sh->err |= ERR_PAIRS; sh->rebuild = 0; }
+    return;
+  }
+  /* 9 */
+  for (int e1 = gid; e1 < ne; e1 += gstride) {
+    const int nf = P.found_cnt[(size_t)ne + 1 + e1];
+    int32_t* fe = P.found_e2 + (size_t)e1 * kRowCap;
+    double* fd = P.found_d2 + (size_t)e1 * kRowCap;
+    for (int a = 1; a < nf; ++a) { /* insertion sort by e2 */
+      const int ke = fe[a];
+      const double kd = fd[a];
+      int b = a - 1;
+      while (b >= 0 && fe[b] > ke) { fe[b + 1] = fe[b]; fd[b + 1] = fd[b]; --b; }
+      fe[b + 1] = ke; fd[b + 1] = kd;
+    }
+    const int r0 = P.row_start[ob][e1], r1 = P.row_start[ob][e1 + 1];
+    int w = P.row_start[nb][e1];
+    int r = r0, f = 0;
+    while (true) {
+      while (r < r1 && !(P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx))) ++r;
+      const bool hr = r < r1, hf = f < nf;
+      if (!hr && !hf) break;
+      if (hr && (!hf || P.pe2[ob][r] < fe[f])) {
+        P.pe1[nb][w] = e1; P.pe2[nb][w] = P.pe2[ob][r]; P.pd2[nb][w] = P.pd2[ob][r];
+        P.pstick[nb][w] = P.pstick[ob][r];
+        st3(P.pso[nb], w, ld3(P.pso[ob], r));
+        ++r;
+      } else {
+        P.pe1[nb][w] = e1; P.pe2[nb][w] = fe[f]; P.pd2[nb][w] = fd[f];
+        P.pstick[nb][w] = 0;
+        st3(P.pso[nb], w, mk(0, 0, 0));
+        ++f;
+      }
+      ++w;
+    }
+  }
+  gsync(grid);
+  if (blockIdx.x == 0 && t == 0) {
+    sh->pair_buf = nb;
+    sh->n_pairs = total;
+    sh->rebuild = 0;
+  }
+}
+
+} // namespace shx

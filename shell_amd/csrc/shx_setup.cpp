@@ -1,0 +1,276 @@
+/*
+ * shx_setup.cpp -- host-side mesh preparation of libshellhip (see
+ * shx_setup.h).
+ */
+#include "shx_setup.h"
+#include "shx_math.h"
+
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+
+namespace shx {
+
+namespace {
+
+inline uint64_t dkey(int a, int b) { return (uint64_t(uint32_t(a)) << 32) | uint32_t(b); }
+
+/* Twelve quartic box splines of the regular Loop patch (J. Stam, SIGGRAPH
+ * 1998 course notes; libMesh FESubdiv::regular_shape) as 12*B(u,v,w) with
+ * u = 1-v-w: entries {coefficient, power of u, power of v, power of w}.
+ * Box-spline order: lattice offsets (0,-1), (-1,0), (1,-1), (0,0), (-1,1),
+ * (2,-1), (1,0), (0,1), (-1,2), (2,0), (1,1), (0,2) for a triangle with
+ * corners (0,0), (1,0), (0,1). */
+struct BT { int8_t c, pu, pv, pw; };
+/* the twelve term lists, each closed by a zero entry */
+const BT kBoxTerms[] = {
+  {1, 4, 0, 0}, {2, 3, 1, 0}, {0, 0, 0, 0},
+  {1, 4, 0, 0}, {2, 3, 0, 1}, {0, 0, 0, 0},
+  {1, 4, 0, 0}, {2, 3, 0, 1}, {6, 3, 1, 0}, {6, 2, 1, 1}, {12, 2, 2, 0}, {6, 1, 2, 1}, {6, 1, 3, 0}, {2, 0, 3, 1}, {1, 0, 4, 0}, {0, 0, 0, 0},
+  {6, 4, 0, 0}, {24, 3, 0, 1}, {24, 2, 0, 2}, {8, 1, 0, 3}, {1, 0, 0, 4}, {24, 3, 1, 0}, {60, 2, 1, 1}, {36, 1, 1, 2}, {6, 0, 1, 3}, {24, 2, 2, 0}, {36, 1, 2, 1}, {12, 0, 2, 2}, {8, 1, 3, 0}, {6, 0, 3, 1}, {1, 0, 4, 0}, {0, 0, 0, 0},
+  {1, 4, 0, 0}, {6, 3, 0, 1}, {12, 2, 0, 2}, {6, 1, 0, 3}, {1, 0, 0, 4}, {2, 3, 1, 0}, {6, 2, 1, 1}, {6, 1, 1, 2}, {2, 0, 1, 3}, {0, 0, 0, 0},
+  {2, 1, 3, 0}, {1, 0, 4, 0}, {0, 0, 0, 0},
+  {1, 4, 0, 0}, {6, 3, 0, 1}, {12, 2, 0, 2}, {6, 1, 0, 3}, {1, 0, 0, 4}, {8, 3, 1, 0}, {36, 2, 1, 1}, {36, 1, 1, 2}, {8, 0, 1, 3}, {24, 2, 2, 0}, {60, 1, 2, 1}, {24, 0, 2, 2}, {24, 1, 3, 0}, {24, 0, 3, 1}, {6, 0, 4, 0}, {0, 0, 0, 0},
+  {1, 4, 0, 0}, {8, 3, 0, 1}, {24, 2, 0, 2}, {24, 1, 0, 3}, {6, 0, 0, 4}, {6, 3, 1, 0}, {36, 2, 1, 1}, {60, 1, 1, 2}, {24, 0, 1, 3}, {12, 2, 2, 0}, {36, 1, 2, 1}, {24, 0, 2, 2}, {6, 1, 3, 0}, {8, 0, 3, 1}, {1, 0, 4, 0}, {0, 0, 0, 0},
+  {2, 1, 0, 3}, {1, 0, 0, 4}, {0, 0, 0, 0},
+  {2, 0, 3, 1}, {1, 0, 4, 0}, {0, 0, 0, 0},
+  {2, 1, 0, 3}, {1, 0, 0, 4}, {6, 1, 1, 2}, {6, 0, 1, 3}, {6, 1, 2, 1}, {12, 0, 2, 2}, {2, 1, 3, 0}, {6, 0, 3, 1}, {1, 0, 4, 0}, {0, 0, 0, 0},
+  {1, 0, 0, 4}, {2, 0, 1, 3}, {0, 0, 0, 0},
+};
+
+const BT* box_terms(int k)
+{
+  const BT* t = kBoxTerms;
+  for (int q = 0; q < k; ++q) { while (t->c != 0) ++t; ++t; }
+  return t;
+}
+
+/* expand 12*B into monomials v^a w^b (a+b <= 4) with exact integer
+ * arithmetic: u^k = sum binom(k;i,j) (-v)^i (-w)^j */
+void box_monomials(int k, long long M[5][5])
+{
+  for (int a = 0; a < 5; ++a) for (int b = 0; b < 5; ++b) M[a][b] = 0;
+  static const long long binom[5][5] = {{1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}};
+  for (const BT* t = box_terms(k); t->c != 0; ++t)
+    for (int i = 0; i <= t->pu; ++i)
+      for (int j = 0; i + j <= t->pu; ++j) {
+        /* multinomial pu!/(i! j! (pu-i-j)!) = binom(pu,i)*binom(pu-i,j) */
+        long long m = binom[t->pu][i] * binom[t->pu - i][j];
+        if ((i + j) & 1) m = -m;
+        M[t->pv + i][t->pw + j] += t->c * m;
+      }
+}
+
+/* value and the five derivatives of the 12 box splines at (v, w) */
+void box_eval(double v, double w, double out[6][12])
+{
+  double pv[5], pw[5];
+  pv[0] = pw[0] = 1;
+  for (int q = 1; q < 5; ++q) { pv[q] = pv[q - 1] * v; pw[q] = pw[q - 1] * w; }
+  auto P = [](const double* p, int e) { return e < 0 ? 0.0 : p[e]; };
+  for (int k = 0; k < 12; ++k) {
+    long long M[5][5];
+    box_monomials(k, M);
+    double r[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 5; ++a)
+      for (int b = 0; a + b < 5; ++b) {
+        if (!M[a][b]) continue;
+        const double c = (double)M[a][b];
+        r[0] += c * P(pv, a) * P(pw, b);
+        r[1] += c * a * P(pv, a - 1) * P(pw, b);
+        r[2] += c * b * P(pv, a) * P(pw, b - 1);
+        r[3] += c * a * (a - 1) * P(pv, a - 2) * P(pw, b);
+        r[4] += c * b * (b - 1) * P(pv, a) * P(pw, b - 2);
+        r[5] += c * a * b * P(pv, a - 1) * P(pw, b - 1);
+      }
+    for (int q = 0; q < 6; ++q) out[q][k] = r[q] / 12.0;
+  }
+}
+
+} // namespace
+
+bool loop_basis_at_centroid(int valence, double out[6 * kMaxPatch])
+{
+  if (valence < 3 || valence > 6) return false;
+  const int N = valence, P = N + 6;
+  std::fill(out, out + 6 * kMaxPatch, 0.0);
+  double b[6][12];
+  const double c = 1.0 / 3.0;
+  if (N == 6) {
+    /* patch slot -> box spline: v0 (0,0), v1 (1,0), ring (1,-1), (0,-1),
+     * (-1,0), (-1,1), v2 (0,1), then (1,1), (2,0), (2,-1), (0,2), (-1,2) */
+    static const int slot_box[12] = {3, 6, 2, 0, 1, 4, 7, 10, 9, 5, 11, 8};
+    box_eval(c, c, b);
+    for (int q = 0; q < 6; ++q)
+      for (int j = 0; j < 12; ++j) out[q * kMaxPatch + j] = b[q][slot_box[j]];
+    return true;
+  }
+  /* Irregular v0: one Loop refinement; the centroid is the centroid of the
+   * central sub-triangle, whose 12-point patch is regular.  Sub-triangle
+   * parameters (1-2xi, 1-2eta); one factor -2 per derivative.
+   * Fine points as combinations of patch slots. */
+  const int V0 = 0, V1 = 1, V2 = N, Q11 = N + 1, Q20 = N + 2, Q2m = N + 3, Q02 = N + 4, Qm2 = N + 5;
+  const int Ra = 2, Rb = std::min(3, N), Rz = N - 1, Ry = std::max(N - 2, 1);
+  double S[12][kMaxPatch] = {};
+  auto E = [&](int row, int a, int bb, int cc, int d) { /* edge point */
+    S[row][a] += 0.375; S[row][bb] += 0.375; S[row][cc] += 0.125; S[row][d] += 0.125;
+  };
+  auto R6 = [&](int row, int ctr, std::initializer_list<int> nb) { /* regular vertex point */
+    S[row][ctr] += 0.625;
+    for (int q : nb) S[row][q] += 0.0625;
+  };
+  E(0, V2, Q11, V1, Q02);
+  E(1, V1, Q11, V2, Q20);
+  R6(2, V2, {Q11, Q02, Qm2, Rz, V0, V1});
+  E(3, V1, V2, V0, Q11);
+  R6(4, V1, {Q20, Q11, V2, V0, Ra, Q2m});
+  E(5, V2, Rz, V0, Qm2);
+  E(6, V0, V2, V1, Rz);
+  E(7, V0, V1, V2, Ra);
+  E(8, V1, Ra, V0, Q2m);
+  E(9, V0, Rz, V2, Ry);
+  {
+    const double cc = 0.375 + 0.25 * std::cos(2.0 * M_PI / N);
+    const double beta = (0.625 - cc * cc) / N; /* Loop (1987) */
+    S[10][V0] += 1.0 - N * beta;
+    for (int k = 1; k <= N; ++k) S[10][k] += beta;
+  }
+  E(11, V0, Ra, V1, Rb);
+  box_eval(1 - 2 * c, 1 - 2 * c, b);
+  const double f[6] = {1, -2, -2, 4, 4, 4};
+  for (int q = 0; q < 6; ++q)
+    for (int j = 0; j < P; ++j) {
+      double s = 0;
+      for (int i = 0; i < 12; ++i) s += S[i][j] * b[q][i];
+      out[q * kMaxPatch + j] = f[q] * s;
+    }
+  return true;
+}
+
+bool build_topology(int nv, int ne, const int32_t* tri_in, Topology& T, std::string& err)
+{
+  T.nv = nv; T.ne = ne;
+  std::vector<int> val(nv, 0);
+  std::unordered_map<uint64_t, int> opp; /* directed edge a->b -> third vertex */
+  opp.reserve(size_t(ne) * 3 * 2);
+  for (int e = 0; e < ne; ++e) {
+    const int* t = tri_in + 3 * e;
+    for (int k = 0; k < 3; ++k) {
+      if (t[k] < 0 || t[k] >= nv) { err = "vertex index out of range"; return false; }
+      val[t[k]]++;
+      const int a = t[k], b = t[(k + 1) % 3], c = t[(k + 2) % 3];
+      if (!opp.emplace(dkey(a, b), c).second) { err = "non-manifold or inconsistently oriented edge"; return false; }
+    }
+  }
+  for (auto& kv : opp)
+    if (!opp.count(dkey(int(kv.first & 0xffffffffu), int(kv.first >> 32)))) { err = "mesh is not closed"; return false; }
+  auto O = [&](int a, int b) { return opp.at(dkey(a, b)); };
+  T.tri.assign(size_t(ne) * 3, 0);
+  T.patch.assign(size_t(ne) * kMaxPatch, -1);
+  T.valence.assign(ne, 6);
+  for (int e = 0; e < ne; ++e) {
+    int t[3] = {tri_in[3 * e], tri_in[3 * e + 1], tri_in[3 * e + 2]};
+    int nirr = 0, at = 0;
+    for (int k = 0; k < 3; ++k) if (val[t[k]] != 6) { ++nirr; at = k; }
+    if (nirr > 1) { err = "triangle with more than one irregular vertex"; return false; }
+    const int v0 = t[at], v1 = t[(at + 1) % 3], v2 = t[(at + 2) % 3];
+    T.tri[3 * e] = v0; T.tri[3 * e + 1] = v1; T.tri[3 * e + 2] = v2;
+    const int N = val[v0];
+    if (N < 3 || N > 6) { err = "unsupported valence"; return false; }
+    T.valence[e] = uint8_t(N);
+    int32_t* p = &T.patch[size_t(e) * kMaxPatch];
+    p[0] = v0;
+    p[1] = v1;
+    for (int k = 2; k <= N; ++k) p[k] = O(p[k - 1], v0);
+    if (p[N] != v2) { err = "one-ring walk did not close"; return false; }
+    const int q11 = O(v2, v1);
+    const int q20 = O(q11, v1);
+    p[N + 1] = q11;
+    p[N + 2] = q20;
+    p[N + 3] = O(q20, v1);
+    p[N + 4] = O(v2, q11);
+    p[N + 5] = O(v2, p[N + 4]);
+  }
+  /* node -> (element, slot) incidence, element order */
+  T.inc_start.assign(nv + 1, 0);
+  for (int e = 0; e < ne; ++e)
+    for (int i = 0; i < kMaxPatch; ++i) {
+      const int n = T.patch[size_t(e) * kMaxPatch + i];
+      if (n >= 0) T.inc_start[n + 1]++;
+    }
+  for (int n = 0; n < nv; ++n) T.inc_start[n + 1] += T.inc_start[n];
+  T.inc.assign(T.inc_start[nv], 0);
+  {
+    std::vector<int32_t> fill(T.inc_start.begin(), T.inc_start.end() - 1);
+    for (int e = 0; e < ne; ++e)
+      for (int i = 0; i < kMaxPatch; ++i) {
+        const int n = T.patch[size_t(e) * kMaxPatch + i];
+        if (n >= 0) T.inc[fill[n]++] = e * 16 + i;
+      }
+  }
+  T.vtx_start.assign(nv + 1, 0);
+  for (int e = 0; e < ne; ++e) for (int k = 0; k < 3; ++k) T.vtx_start[T.tri[3 * e + k] + 1]++;
+  for (int n = 0; n < nv; ++n) T.vtx_start[n + 1] += T.vtx_start[n];
+  T.vtx.assign(T.vtx_start[nv], 0);
+  {
+    std::vector<int32_t> fill(T.vtx_start.begin(), T.vtx_start.end() - 1);
+    for (int e = 0; e < ne; ++e) for (int k = 0; k < 3; ++k) T.vtx[fill[T.tri[3 * e + k]]++] = e;
+  }
+  return true;
+}
+
+void build_close_sets(const Topology& T, const double* X, double cutoff, std::vector<int32_t>& start,
+                      std::vector<int32_t>& ids)
+{
+  const int ne = T.ne;
+  const double cutoff2 = cutoff * cutoff;
+  /* direct neighbours: triangles sharing a vertex */
+  std::vector<std::vector<int>> nb(ne);
+  for (int e = 0; e < ne; ++e) {
+    std::vector<int>& L = nb[e];
+    for (int k = 0; k < 3; ++k) {
+      const int n = T.tri[3 * e + k];
+      for (int q = T.vtx_start[n]; q < T.vtx_start[n + 1]; ++q) if (T.vtx[q] != e) L.push_back(T.vtx[q]);
+    }
+    std::sort(L.begin(), L.end());
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+  }
+  auto corners = [&](int e, v3 P[3]) {
+    for (int k = 0; k < 3; ++k) {
+      const int n = T.tri[3 * e + k];
+      P[k] = mk(X[3 * n], X[3 * n + 1], X[3 * n + 2]);
+    }
+  };
+  start.assign(ne + 1, 0);
+  ids.clear();
+  std::vector<int> close, frontier, next;
+  std::vector<char> mark(ne, 0);
+  for (int e = 0; e < ne; ++e) {
+    close = nb[e];
+    for (int c : close) mark[c] = 1;
+    frontier = close;
+    v3 Pe[3];
+    corners(e, Pe);
+    while (!frontier.empty()) {
+      next.clear();
+      for (int a : frontier)
+        for (int c : nb[a]) if (c != e && !mark[c]) next.push_back(c);
+      std::sort(next.begin(), next.end());
+      next.erase(std::unique(next.begin(), next.end()), next.end());
+      frontier.clear();
+      for (int c : next) {
+        v3 Pc[3];
+        corners(c, Pc);
+        tri_hit hit;
+        tri_tri(Pe, Pc, hit);
+        if (hit.d2 < cutoff2) { mark[c] = 1; close.push_back(c); frontier.push_back(c); }
+      }
+    }
+    std::sort(close.begin(), close.end());
+    for (int c : close) mark[c] = 0;
+    ids.insert(ids.end(), close.begin(), close.end());
+    start[e + 1] = (int32_t)ids.size();
+  }
+}
+
+} // namespace shx

@@ -1,0 +1,422 @@
+"""Pins of the CPU FP64 oracle against what the mathematics and the
+reference's own data fix (the reference ships no tests or golden vectors;
+its only data are icosasphere3.off and icosasphere4.off).  Each test is
+chosen so that a plausible slip (a wrong sign, a swapped index, a dropped
+term) fails it."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import imperfect
+
+H = 0.01
+PCR = 2 * H * H / np.sqrt(3 * (1 - 0.3 ** 2))
+
+# SHA-256 of the reference's mesh files icosasphere3.off / icosasphere4.off
+OFF_SHA = {3: "461df12da867577434b5f4db6d261d4ac66749fa1e3666df9a07814fd7a45033",
+           4: "7a0dbb5a8c29168e4c0feac280debc9304cf20065e52390aeb5129e304025622"}
+
+
+def off_text(level):
+    X, T = O.icosphere(level, round_g=False)
+    lines = ["OFF", f"{len(X)} {len(T)} 0"] + ["%g %g %g" % tuple(x) for x in X] + \
+            ["3 %d %d %d" % tuple(t) for t in T]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("level", [3, 4])
+def test_icosphere_reproduces_reference_off(level):
+    """the generator writes the reference's mesh files byte for byte"""
+    assert hashlib.sha256(off_text(level).encode()).hexdigest() == OFF_SHA[level]
+    X, T = O.icosphere(level)
+    assert len(X) == 10 * 4 ** level + 2 and len(T) == 20 * 4 ** level
+
+
+def test_box_splines_partition_and_vertex_masks():
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        v, w = rng.random(2)
+        if v + w > 1:
+            v, w = 1 - v, 1 - w
+        b = O.stam_basis(v, w)
+        assert abs(b[0].sum() - 1) < 1e-14
+        assert np.all(np.abs(b[1:].sum(axis=1)) < 1e-13)
+        assert np.all(b[0] >= -1e-16)
+    # corners of the triangle: Loop's regular limit mask 1/2 centre, 1/12 ring
+    for (v, w), centre in (((0, 0), 3), ((1, 0), 6), ((0, 1), 7)):
+        b = O.stam_basis(v, w)[0]
+        assert abs(b[centre] - 0.5) < 1e-15
+        assert np.sum(np.abs(b - 1 / 12) < 1e-15) == 6
+
+
+def test_box_splines_refine_with_loop_masks():
+    """B(x) = S^T B'(fine x): the box splines are refinable with Loop's edge
+    (3/8, 3/8, 1/8, 1/8) and regular vertex (5/8, 1/16) masks"""
+    S = O.subdiv_rows(6)
+    cvi = [3, 6, 2, 0, 1, 4, 7, 10, 9, 5, 11, 8]
+    for xi, eta in ((1 / 3, 1 / 3), (0.4, 0.3), (0.45, 0.2), (0.26, 0.3), (0.5, 0.25)):
+        coarse = O.stam_basis(xi, eta)[:, cvi]
+        fine = O.stam_basis(1 - 2 * xi, 1 - 2 * eta)
+        fac = np.array([1, -2, -2, 4, 4, 4])[:, None]
+        np.testing.assert_allclose(fac * (fine @ S), coarse, atol=2e-15)
+
+
+@pytest.mark.parametrize("valence", [3, 4, 5, 6])
+def test_patch_basis_partition_of_unity(valence):
+    b = O.raw_elem_basis(valence)
+    assert abs(b[0].sum() - 1) < 1e-14
+    assert np.all(np.abs(b[1:].sum(axis=1)) < 1e-13)
+
+
+# ---------------------------------------------------------------- limit surface by brute force
+
+def _loop_subdivide(X, T):
+    """one step of Loop subdivision on a closed mesh (generic: any valence)"""
+    nv = len(X)
+    edges = {}
+    opp = {}
+    for t in T:
+        for k in range(3):
+            a, b, c = t[k], t[(k + 1) % 3], t[(k + 2) % 3]
+            opp[(a, b)] = c
+    nb = [set() for _ in range(nv)]
+    for (a, b) in opp:
+        nb[a].add(b)
+    newX = []
+    for v in range(nv):
+        N = len(nb[v])
+        beta = (5 / 8 - (3 / 8 + np.cos(2 * np.pi / N) / 4) ** 2) / N
+        newX.append((1 - N * beta) * X[v] + beta * sum(X[q] for q in nb[v]))
+    idx = nv
+    for (a, b), c in opp.items():
+        key = (min(a, b), max(a, b))
+        if key in edges:
+            continue
+        d = opp[(b, a)]
+        newX.append(3 / 8 * (X[a] + X[b]) + 1 / 8 * (X[c] + X[d]))
+        edges[key] = idx
+        idx += 1
+    mid = lambda a, b: edges[(min(a, b), max(a, b))]
+    T2 = []
+    for a, b, c in T:
+        ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+        T2 += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+    return np.array(newX), np.array(T2), nb
+
+
+def _limit_point(X, nb, v):
+    N = len(nb[v])
+    beta = (5 / 8 - (3 / 8 + np.cos(2 * np.pi / N) / 4) ** 2) / N
+    chi = 1 / (3 / (8 * beta) + N)
+    return (1 - N * chi) * X[v] + chi * sum(X[q] for q in nb[v])
+
+
+def test_patch_basis_matches_repeated_subdivision(icosphere):
+    """value and tangents at the element centroid from the oracle's patch
+    basis versus k levels of generic Loop subdivision (the centroid stays the
+    centroid of the central sub-triangle): errors shrink by about 4 per
+    level, for an element at a valence-5 vertex and a regular one"""
+    X0, T0 = icosphere(1)
+    X0 = imperfect(X0, 0.03, seed=1)
+    o = O.Oracle(X0, T0)
+    idx, tri, N = o.patches()
+    tab = o.elem_tables()
+    for e in (int(np.argmax(N == 11)), int(np.argmax(N == 12))):
+        n = N[e]
+        rb = O.raw_elem_basis(n - 6)
+        P = X0[idx[e, :n]]
+        val, b1, b2 = rb[0] @ P, rb[1] @ P, rb[2] @ P
+        assert np.allclose(val, tab[e, 98:101], atol=1e-14)
+        X, T = X0.copy(), np.array(tri)
+        face = e
+        params = np.array([[0, 0], [1, 0], [0, 1]], dtype=float)
+        errs = []
+        for level in range(1, 6):
+            X, T, nb = _loop_subdivide(X, T)
+            face = 4 * face + 3
+            params = np.array([(params[0] + params[1]) / 2, (params[1] + params[2]) / 2,
+                               (params[2] + params[0]) / 2])
+            if level < 3:
+                continue
+            # limit points of the three corners of the nested central triangle
+            nbs = [set() for _ in range(len(X))]
+            for t in T:
+                for k in range(3):
+                    nbs[t[k]].add(t[(k + 1) % 3])
+            L = np.array([_limit_point(X, nbs, v) for v in T[face]])
+            M = np.column_stack([params[1] - params[0], params[2] - params[0]])
+            D = np.column_stack([L[1] - L[0], L[2] - L[0]])
+            grad = D @ np.linalg.inv(M)
+            errs.append((L.mean(axis=0), grad[:, 0], grad[:, 1]))
+        (c3, g13, g23), (c4, g14, g24), (c5, g15, g25) = errs
+        # corner average: O(4^-k); 3-point tangent fit: O(2^-k)
+        e_val = [np.linalg.norm(c - val) for c in (c3, c4, c5)]
+        e_tan = [np.linalg.norm(g - b1) for g in (g13, g14, g15)]
+        assert 3.0 < e_val[0] / e_val[1] < 5.0 and 3.0 < e_val[1] / e_val[2] < 5.0, e_val
+        assert e_tan[2] < e_tan[1] < e_tan[0], e_tan
+        # Richardson extrapolation from levels 4 and 5; the central triangle
+        # turns by 180 degrees each level, so the first-order tangent error
+        # alternates in sign
+        assert np.linalg.norm((4 * c5 - c4) / 3 - val) < 2e-6
+        assert np.linalg.norm((2 * g15 + g14) / 3 - b1) < min(1e-4, e_tan[0] / 5)
+        assert np.linalg.norm((2 * g25 + g24) / 3 - b2) < 1e-4
+
+
+# ------------------------------------------------------------------------- forces
+
+@pytest.fixture(scope="module")
+def l3(icosphere):
+    X, T = icosphere(3)
+    X = imperfect(X, 0.01 * H, seed=3)
+    return X, T, O.Oracle(X, T, pressure_final=-2 * PCR)
+
+
+def test_reference_frame_and_mass(l3):
+    X, T, o = l3
+    tab = o.elem_tables()
+    A1, A2 = tab[:, 72:75], tab[:, 75:78]
+    # reinit_mod: A1 is horizontal (east), A2 in the meridian plane, A1 . A2 = 0
+    assert np.abs(A1[:, 2]).max() < 1e-12
+    assert np.abs((A1 * A2).sum(1)).max() < 1e-12
+    # JxW = |A1 x A2| / 2 before and after the reparametrisation
+    np.testing.assert_allclose(0.5 * np.linalg.norm(np.cross(A1, A2), axis=1), tab[:, 93], rtol=1e-12)
+    # outward normal
+    xyz = tab[:, 98:101]
+    assert np.all((np.cross(A1, A2) * xyz).sum(1) > 0)
+    Minv, C = o.mass()
+    assert abs((1 / Minv[0::3]).sum() - H * tab[:, 93].sum()) < 1e-14
+    np.testing.assert_allclose(C, 0.05 / Minv, rtol=1e-14)
+
+
+def test_internal_force_invariants(l3):
+    X, T, o = l3
+    n = 3 * len(X)
+    assert np.abs(o.internal_force(np.zeros(n))).max() == 0
+    assert np.abs(o.internal_force(np.tile([0.01, -0.02, 0.03], len(X)))).max() < 1e-17
+    th = 0.05
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    assert np.abs(o.internal_force((X @ R.T - X).reshape(-1))).max() < 1e-16
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=n) * 1e-3
+    f = o.internal_force(u)
+    assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-18 * 0 + 1e-17 * np.abs(f).max() * len(X)
+    # f = -dE/du (central difference of E_tens + E_bend)
+    d = rng.normal(size=n) * 1e-3
+    eps = 1e-4
+
+    def energy(x):
+        _, _, en, _ = o.forces(x, None, 0.0)
+        return en[0] + en[1]
+    dE = (energy(u + eps * d) - energy(u - eps * d)) / (2 * eps)
+    assert abs(dE + (f * d).sum()) < 1e-7 * abs(dE)
+
+
+def test_membrane_and_bending_split(icosphere):
+    """bending energy scales with h^3 and membrane with h under the same
+    displacement (K ~ h, D ~ h^3)"""
+    X, T = icosphere(3)
+    rng = np.random.default_rng(2)
+    u = rng.normal(size=3 * len(X)) * 1e-4
+    e = []
+    for h in (0.01, 0.02):
+        o = O.Oracle(X, T, thickness=h)
+        e.append(o.forces(u, None, 0.0)[2])
+    assert abs(e[1][0] / e[0][0] - 2) < 1e-12
+    assert abs(e[1][1] / e[0][1] - 8) < 1e-12
+
+
+def test_orthotropic_reduces_to_isotropic_pressurised_sphere(icosphere):
+    """E1 = E2, G12 = E/(2(1+nu)) and an inflating pressure: the uniform radial
+    displacement that balances the load agrees with the membrane closed form
+    w = p R^2 (1-nu) / (2 E h) (R = radius of the limit surface)"""
+    X, T = icosphere(4)
+    E, nu, p = 1.0, 0.3, 1e-5
+    o = O.Oracle(X, T, E1=E, E2=E, G12=E / (2 * (1 + nu)), nu=nu, pressure_final=p)
+    tab = o.elem_tables()
+    R = np.linalg.norm(tab[:, 98:101], axis=1).mean()
+    nhat = (X / np.linalg.norm(X, axis=1)[:, None]).reshape(-1)
+    f0 = o.forces(np.zeros(3 * len(X)), None, 0.0)[0]
+    w1 = 1e-4
+    f1 = o.forces(w1 * nhat, None, 0.0)[0]
+    k = (f1 - f0) / w1
+    w = -np.dot(f0, k) / np.dot(k, k)
+    w_closed = p * R ** 2 * (1 - nu) / (2 * E * H)
+    assert abs(w - w_closed) < 0.02 * w_closed
+
+
+# ---------------------------------------------------------------- narrow phase / contact
+
+def _dense_tri_distance(P0, P1, n=40):
+    g = np.array([(i / n, j / n) for i in range(n + 1) for j in range(n + 1 - i)])
+    A = P0[0] + g[:, :1] * (P0[1] - P0[0]) + g[:, 1:] * (P0[2] - P0[0])
+    B = P1[0] + g[:, :1] * (P1[1] - P1[0]) + g[:, 1:] * (P1[2] - P1[0])
+    d = ((A[:, None, :] - B[None, :, :]) ** 2).sum(-1)
+    return d.min()
+
+
+def _seg_hits_tri(p, q, T):
+    """Moller-Trumbore: does segment pq cross triangle T"""
+    e1, e2, d = T[1] - T[0], T[2] - T[0], q - p
+    h = np.cross(d, e2)
+    a = e1 @ h
+    if abs(a) < 1e-14:
+        return False
+    s = p - T[0]
+    u = (s @ h) / a
+    qq = np.cross(s, e1)
+    v = (d @ qq) / a
+    t = (e2 @ qq) / a
+    return 0 <= u <= 1 and 0 <= v and u + v <= 1 and 0 <= t <= 1
+
+
+def _intersect(A, B):
+    return any(_seg_hits_tri(X[i], X[(i + 1) % 3], Y) for X, Y in ((A, B), (B, A)) for i in range(3))
+
+
+def test_triangle_distance_against_dense_sampling():
+    """for disjoint triangles the minimum over 9 edge-edge and 6 vertex-face
+    distances is the triangle distance (interpenetrating pairs are the
+    reference's CONTACT_PROBLEM case and are excluded)"""
+    rng = np.random.default_rng(4)
+    tested = 0
+    for k in range(120):
+        P = rng.normal(size=(2, 3, 3))
+        if k % 3 == 1:
+            P[1] = P[0][[1, 2, 0]] + rng.normal(size=(3, 3)) * 0.3
+        if _intersect(P[0], P[1]):
+            continue
+        tested += 1
+        out = O.tri_distance(P.reshape(-1))
+        d2, dc, v, w = out[0], out[1:4], out[4:6], out[6:8]
+        # the reported closest points are on the triangles and dc joins them
+        q0 = P[0][0] + v[0] * (P[0][1] - P[0][0]) + w[0] * (P[0][2] - P[0][0])
+        q1 = P[1][0] + v[1] * (P[1][1] - P[1][0]) + w[1] * (P[1][2] - P[1][0])
+        assert min(v.min(), w.min(), (1 - v - w).min()) > -1e-12
+        np.testing.assert_allclose(q1 - q0, dc, atol=1e-12)
+        assert abs(d2 - dc @ dc) < 1e-12
+        dense = _dense_tri_distance(P[0], P[1])
+        assert d2 <= dense + 1e-12          # never above a sampled distance
+        assert dense - d2 < 0.05 * (np.abs(P).max() ** 2)  # and close to it
+    assert tested >= 60
+
+
+def test_segment_and_point_triangle_closed_forms():
+    # parallel segments: distance of the lines
+    out = O.segment_distance([0, 0, 0, 1, 0, 0, 0.5, 1, 0, 1.5, 1, 0])
+    assert abs(np.linalg.norm(out[:3]) - 1) < 1e-15
+    # crossing skew segments
+    out = O.segment_distance([-1, 0, 0, 1, 0, 0, 0, -1, 2, 0, 1, 2])
+    np.testing.assert_allclose(out[:3], [0, 0, 2])
+    np.testing.assert_allclose(out[3:], [0.5, 0.5])
+    # point above the interior of a triangle
+    out = O.point_triangle([0.2, 0.3, 1.5, 0, 0, 0, 1, 0, 0, 0, 1, 0])
+    np.testing.assert_allclose(out, [0, 0, -1.5, 0.2, 0.3], atol=1e-15)
+    # point beyond vertex b
+    out = O.point_triangle([2, -1, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0])
+    np.testing.assert_allclose(out, [-1, 1, 0, 1, 0], atol=1e-15)
+
+
+def test_contact_force_is_energy_derivative():
+    k, h = 0.7, 0.01
+    for d in (0.002, 0.005, 0.0099):
+        eps = 1e-9
+        dE = (O.contact_energy(k, h, d + eps) - O.contact_energy(k, h, d - eps)) / (2 * eps)
+        assert abs(-dE - O.contact_force(k, h, d)) < 1e-6 * abs(O.contact_force(k, h, d))
+    assert O.contact_force(k, h, h) == 0 and O.contact_energy(k, h, h) == 0
+
+
+def _fric(f_normal, n, v, stick=0, origin=(0, 0, 0), cp=(0, 0, 0), mu_s=0.3, mu_d=0.2, ks=10.0,
+          damping=0.0, v_stop=1e-3):
+    n = np.asarray(n, float)
+    v = np.asarray(v, float)
+    io = np.zeros(24)
+    io[4] = f_normal
+    io[5:8] = n
+    io[8:11] = v
+    io[11] = v @ n
+    io[12], io[13], io[14], io[15], io[16] = v_stop, ks, mu_s, mu_d, damping
+    io[17] = stick
+    io[18:21] = origin
+    io[21:24] = cp
+    out = O.friction_apply(io)
+    return out[0], out[1:4], out[17], out[18:21]
+
+
+def test_friction_branches():
+    # slip: tangential speed above v_stop -> Coulomb, opposite to v_t, |f| = mu_d f_n
+    E, f, stick, org = _fric(2.0, [0, 0, 1], [0.1, 0, 0.5])
+    assert stick == 0 and E == 0 and np.allclose(org, 0)
+    np.testing.assert_allclose(f, [-0.2 * 2.0 * 0.1 / (0.1 + 1e-11), 0, 0])
+    # stick creation: spring origin so that the force is continuous with slip
+    vt = np.array([5e-4, 0, 0])
+    E, f, stick, org = _fric(2.0, [0, 0, 1], vt, cp=[1, 2, 3])
+    assert stick == 1
+    np.testing.assert_allclose(f, -0.2 * 2.0 * vt / (5e-4 + 1e-11), rtol=1e-12)
+    np.testing.assert_allclose(E, 0.5 * 10 * ((0.2 * 2.0 / (10 * (5e-4 + 1e-11))) * 5e-4) ** 2, rtol=1e-12)
+    # sticking spring, normal component of the spring removed
+    E, f, stick, org = _fric(2.0, [0, 0, 1], [0, 0, 0], stick=1, origin=[0.01, 0, 0.5], cp=[0.02, 0, 0])
+    assert stick == 1
+    np.testing.assert_allclose(f, [-10 * 0.01, 0, 0], atol=1e-15)
+    # break-away: spring force beyond mu_s f_n -> slip with the dynamic force
+    E, f, stick, org = _fric(0.1, [0, 0, 1], [1e-4, 0, 0], stick=1, origin=[0, 0, 0], cp=[1.0, 0, 0])
+    assert stick == 0 and E == 0
+    np.testing.assert_allclose(f, [-0.2 * 0.1 * 1e-4 / (1e-4 + 1e-11), 0, 0])
+
+
+def test_broad_phase_equals_brute_force(icosphere):
+    """linked cells at a fresh rebuild give exactly the O(ne^2) set
+    {e1 < e2, not topologically close, boxes within d_max, d2 <= d_max^2}"""
+    X, T = icosphere(3)
+    o = O.Oracle(X, T, contact=1, cell_size_mult=1.0)
+    u = X.copy()
+    u[:, 2] *= 0.006
+    u = (u - X).reshape(-1)
+    o.set_state(u)
+    n = o.rebuild(force=True)
+    e1, e2 = o.pairs()
+    _, tri, _ = o.patches()
+    P = X + u.reshape(-1, 3)
+    bmin = P[tri].min(1)
+    bmax = P[tri].max(1)
+    avg = (bmax - bmin).max(1)
+    avg = sum(avg[i:i + 256].sum() for i in range(0, len(avg), 256)) / len(avg)
+    cell = 1.0 * avg
+    tol = 0.02 * cell
+    dmax = H + tol
+    start, ids = o.close_sets()
+    close = set()
+    for e in range(len(T)):
+        for c in ids[start[e]:start[e + 1]]:
+            close.add((e, int(c)))
+    cand = []
+    ne = len(T)
+    for a in range(ne):
+        b = np.arange(a + 1, ne)
+        apart = np.any((bmin[a] > bmax[b] + dmax) | (bmin[b] > bmax[a] + dmax), axis=1)
+        for bb in b[~apart]:
+            if (a, int(bb)) in close:
+                continue
+            d2 = O.tri_distance(np.concatenate([P[tri[a]].reshape(-1), P[tri[bb]].reshape(-1)]))[0]
+            if d2 <= dmax * dmax:
+                cand.append((a, int(bb)))
+    assert n == len(cand) > 0
+    assert list(zip(e1.tolist(), e2.tolist())) == cand
+
+
+def test_energy_conservation_undamped(icosphere):
+    """central difference (beta = 0), no damping, no load: the total energy of
+    a small free vibration stays constant to O(dt^2)"""
+    X, T = icosphere(3)
+    o = O.Oracle(X, T, damping_alpha=0.0, dt=0.05)
+    rng = np.random.default_rng(1)
+    u0 = rng.normal(size=3 * len(X)) * 1e-5
+    o.set_state(u0)
+    o.init_dynamics()
+    E = []
+    for _ in range(40):
+        r = o.step(25)
+        E.append(r["e_kin"] + r["e_tens"] + r["e_bend"])
+    E = np.array(E)
+    assert (E.max() - E.min()) / E.mean() < 2e-2
