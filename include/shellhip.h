@@ -71,7 +71,9 @@ typedef struct {
   int32_t contact;                 /* shell_shell_contact: 0 none, 1 linear */
   int32_t quirk_fulltime;          /* 1: reproduce `pressure *= fulltime` (ShellSystem.C:1480) */
   int32_t quirk_volume_schedule;   /* 1: reproduce r(t) = max(1-floor(5t)*0.01, 0.5) (ShellSystem.C:1474) */
-  int32_t reserved;
+  int32_t volume_at_prediction;    /* ext: 1: volume pressure from the enclosed volume of the configuration
+                                      the forces are evaluated at; 0: the previous step's volume, as the
+                                      reference (ShellSystem.C:1097, 1476) */
 } shx_params;
 
 typedef struct {
@@ -83,8 +85,10 @@ typedef struct {
   double du_max;
   int32_t n_contacts, n_pairs, n_rebuilds, problem;
   int32_t n_steps, n_rejected;     /* accepted steps and rejected attempts since shx_init_dynamics */
-  int32_t converged_step;          /* first accepted step with t >= load end and residual <= tol, or -1 */
+  int32_t converged_step;          /* first accepted step with t >= load end and residual_deform <= tol, or -1 */
   int32_t reserved;
+  double residual_deform;          /* ||f - M a_rigid|| / ||f_ext||: residual with the least-squares rigid-body
+                                      inertial force removed (a free-floating shell drifts under the follower load) */
 } shx_report;
 
 /* Replaces: ShellSystem constructor + set_parameters + build_cache +

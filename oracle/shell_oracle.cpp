@@ -959,6 +959,7 @@ void evaluate_forces(System& s, double t)
   const bool omp = s.prm.omp_faithful != 0;
 #pragma omp parallel for if (omp)
   for (int e = 0; e < s.ne; ++e) deformed_metrics_elem(s, e, s.u_pred.data());
+  if (s.prm.volume_at_prediction) compute_enclosed_volume(s); /* positions s.p are those of u_pred */
   pressure_force(s, t);
   if (s.prm.contact) contact_force(s);
   double Et_sum = 0, Eb_sum = 0;
@@ -997,6 +998,54 @@ double l2(const std::vector<double>& x)
   return std::sqrt(acc);
 }
 
+/* Residual of the deformation: ||f - B q|| with B the rigid-body inertial
+ * forces (node i: m_i [I | -[x_i]x], translation and rotation about the
+ * origin at the force-evaluation positions x) and q their least-squares
+ * fit, from the single-pass sums B^T f and B^T B:
+ * ||f - Bq||^2 = ||f||^2 - b^T G^-1 b.  The reference has no residual; a
+ * free-floating shell under the follower load of ShellSystem.C:1570-1576
+ * drifts and spins slowly, which this definition does not count. */
+double rigid_free_norm(const System& s, const std::vector<double>& f)
+{
+  double G[6][6] = {}, b[6] = {}, f2 = 0;
+  for (int n = 0; n < s.nv; ++n) {
+    const double m = 1 / s.Minv[3 * n];
+    const Pt& x = s.p[n];
+    const Pt fn(f[3 * n], f[3 * n + 1], f[3 * n + 2]);
+    f2 += fn * fn;
+    const Pt xf = x.cross(fn);
+    for (int q = 0; q < 3; ++q) { b[q] += m * fn(q); b[3 + q] += m * xf(q); }
+    const double m2 = m * m, xx = x * x;
+    for (int q = 0; q < 3; ++q) G[q][q] += m2;
+    /* [[I, -[x]x], [[x]x, |x|^2 I - x x^T]] * m^2 */
+    const double X[3][3] = {{0, -x(2), x(1)}, {x(2), 0, -x(0)}, {-x(1), x(0), 0}};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        G[r][3 + c] -= m2 * X[r][c];
+        G[3 + r][c] += m2 * X[r][c];
+        G[3 + r][3 + c] += m2 * ((r == c ? xx : 0) - x(r) * x(c));
+      }
+  }
+  /* Cholesky solve of G y = b */
+  double L[6][6] = {};
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double sum = G[i][j];
+      for (int k = 0; k < j; ++k) sum -= L[i][k] * L[j][k];
+      if (i == j) L[i][i] = std::sqrt(std::max(sum, 1e-300));
+      else L[i][j] = sum / L[j][j];
+    }
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double sum = b[i];
+    for (int k = 0; k < i; ++k) sum -= L[i][k] * y[k];
+    y[i] = sum / L[i][i];
+  }
+  double proj = 0;
+  for (int i = 0; i < 6; ++i) proj += y[i] * y[i];
+  return std::sqrt(std::max(f2 - proj, 0.0));
+}
+
 /* ShellSystem::advance, ShellSystem.C:952-1122: step attempts repeat until
  * the local-error estimate accepts dt (adapt_dt_every > 0), as in the
  * reference; with adapt_dt_every = 0 every attempt is accepted. */
@@ -1007,7 +1056,7 @@ void advance(System& s, orc_report* rep)
   const bool adapt_now = (P.adapt_dt_every > 0 && s.time_step % P.adapt_dt_every == 0);
   const int n3 = 3 * s.nv;
   const double h = P.thickness;
-  double fnorm = 0, fext_norm = 0;
+  double fnorm = 0, fext_norm = 0, frigid = 0;
   while (true) {
     s.t += s.dt;
     const double* A = s.newmark_a;
@@ -1024,6 +1073,7 @@ void advance(System& s, orc_report* rep)
     evaluate_forces(s, s.t);
     fnorm = l2(s.f);
     fext_norm = l2(s.fext);
+    frigid = rigid_free_norm(s, s.f);
     for (int i = 0; i < n3; ++i) {
       s.a_pred[i] = s.C[i] * s.v_pred[i];
       s.f[i] -= s.a_pred[i];
@@ -1075,6 +1125,7 @@ void advance(System& s, orc_report* rep)
     rep->fnorm = fnorm;
     rep->fext_norm = fext_norm;
     rep->residual = fext_norm > 0 ? fnorm / fext_norm : fnorm;
+    rep->residual_deform = fext_norm > 0 ? frigid / fext_norm : frigid;
     rep->e_tens = s.Etens; rep->e_bend = s.Ebend; rep->e_self = s.Eself; rep->e_fric = s.Efric;
     rep->e_kin = ekin;
     rep->volume = s.volume; rep->area = s.area;

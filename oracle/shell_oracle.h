@@ -46,6 +46,8 @@ typedef struct {
   int32_t quirk_fulltime;          /* 1: pressure *= fulltime (== 0), ShellSystem.C:1480 */
   int32_t quirk_volume_schedule;   /* 1: r(t) = max(1-floor(5t)*0.01, 0.5), ShellSystem.C:1474 */
   int32_t omp_faithful;            /* 1: OpenMP loops with critical assembly (CPU baseline timing) */
+  int32_t volume_at_prediction;    /* ext: 1: enclosed volume of the force-evaluation configuration
+                                      instead of the previous step's (ShellSystem.C:1097, 2505) */
 } orc_params;
 
 typedef struct {
@@ -61,6 +63,7 @@ typedef struct {
   int32_t problem;     /* max severity (ShellSystem.h:48-49) */
   int32_t n_rejected;  /* rejected step attempts so far */
   double dt;           /* dt for the next step */
+  double residual_deform; /* residual with the rigid-body inertial forces removed (least squares) */
 } orc_report;
 
 /* meshes */

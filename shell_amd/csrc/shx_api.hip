@@ -29,7 +29,7 @@ struct shx_ctx {
   DPtrs P{};
   std::vector<void*> allocs;
   std::vector<double> rawb;
-  int nblkE = 0, nblkN = 0, nblkC = 0, gridBP = 0;
+  int nblkE = 0, nblkN = 0, nblkC = 0;
   std::string err;
   bool prof = false;
   std::vector<hipEvent_t> evpool;
@@ -136,28 +136,33 @@ void prof_collect(shx_ctx* ctx)
   ctx->evclass.clear();
 }
 
+/* the broad phase: four kernels that return at once unless the device-side
+ * trigger (or a forced rebuild) set DShell::rebuild */
 int launch_broad(shx_ctx* ctx, int posmode)
 {
-  DPtrs P = ctx->P;
-  SConst C = ctx->C;
-  void* args[] = {&P, &C, &posmode};
+  const DPtrs& P = ctx->P;
+  const SConst& C = ctx->C;
+  const int nb = (C.ne + 255) / 256;
   prof_begin(ctx, 4);
-  HIPCHK(hipLaunchCooperativeKernel((const void*)k_broad, dim3(ctx->gridBP), dim3(256), args, 0, ctx->st));
+  k_bp_box<<<nb, 256, 0, ctx->st>>>(P, C, posmode);
+  k_bp_bin<<<nb, 256, 0, ctx->st>>>(P, C);
+  k_bp_pairs<<<nb, 256, 0, ctx->st>>>(P, C, posmode);
+  k_bp_emit<<<nb, 256, 0, ctx->st>>>(P, C);
   prof_end(ctx);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
-/* one step attempt: element force, predictor trigger, broad phase (gated),
- * narrow phase, node gather + Newmark, finalize, corrector trigger */
+/* one step attempt: element force (+ predictor trigger), broad phase,
+ * narrow phase, node gather + Newmark (+ finalize), corrector trigger */
 int enqueue_attempt(shx_ctx* ctx)
 {
   const DPtrs& P = ctx->P;
   const SConst& C = ctx->C;
   hipStream_t st = ctx->st;
   prof_begin(ctx, 0);
-  k_elem<<<ctx->nblkE, 256, 0, st>>>(P, C, M_STEP);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, st>>>(P, C, M_STEP);
   prof_end(ctx);
-  k_gate<<<1, 64, 0, st>>>(P, C, ctx->nblkE);
   if (C.contact) {
     if (int r = launch_broad(ctx, M_STEP)) return r;
     prof_begin(ctx, 2);
@@ -165,10 +170,7 @@ int enqueue_attempt(shx_ctx* ctx)
     prof_end(ctx);
   }
   prof_begin(ctx, 1);
-  k_node<<<ctx->nblkN, 256, 0, st>>>(P, C, M_STEP, 0.0, ctx->nblkE);
-  prof_end(ctx);
-  prof_begin(ctx, 3);
-  k_final<<<1, 64, 0, st>>>(P, C, M_STEP, ctx->nblkE, ctx->nblkN, C.contact ? ctx->nblkC : 0);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, st>>>(P, C, M_STEP, 0.0, C.contact ? ctx->nblkC : 0);
   prof_end(ctx);
   if (C.contact)
     if (int r = launch_broad(ctx, M_FORCES)) return r;
@@ -195,6 +197,7 @@ void fill_report(const DShell& s, shx_report* rep)
   rep->n_contacts = s.n_contacts; rep->n_pairs = s.n_pairs; rep->n_rebuilds = s.n_rebuilds;
   rep->problem = s.problem; rep->n_steps = s.n_steps; rep->n_rejected = s.n_rejected;
   rep->converged_step = s.converged_step; rep->reserved = 0;
+  rep->residual_deform = s.residual_deform;
 }
 
 } // namespace
@@ -265,6 +268,7 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
   C.quirk_fulltime = prm->quirk_fulltime; C.quirk_volume_schedule = prm->quirk_volume_schedule;
   C.conv_tol = 0; C.load_end_time = std::max(prm->pressure_ramp_time, prm->volume_ramp_time);
   C.stop_on_converged = 0;
+  C.volume_at_prediction = prm->volume_at_prediction ? 1 : 0;
   {
     unsigned m = (unsigned)std::ceil(std::sqrt((double)ne) / prm->cell_size_mult);
     C.n_cells_max = m * m * m;
@@ -287,8 +291,8 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
   }
   std::vector<double> Xh(X, X + (size_t)3 * nv);
   DPtrs& P = ctx->P;
-  ctx->nblkE = (ne + 255) / 256;
-  ctx->nblkN = (nv + 255) / 256;
+  ctx->nblkE = (ne + kElemBlock / kElemLanes - 1) / (kElemBlock / kElemLanes);
+  ctx->nblkN = (nv + kNodeBlock / kNodeLanes - 1) / (kNodeBlock / kNodeLanes);
   ctx->nblkC = 120;
   int r = 0;
   double* dX;
@@ -301,10 +305,11 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
     return fail(r);
   P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
   if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * ne)) || (r = dalloc(ctx, P.minv, nv)) ||
-      (r = dalloc(ctx, P.cdamp, nv)) || (r = dalloc(ctx, P.G, (size_t)G_ROWS * ne)) || (r = dalloc(ctx, P.area, ne)) ||
+      (r = dalloc(ctx, P.cdamp, nv)) || (r = dalloc(ctx, P.qe, (size_t)36 * ne)) || (r = dalloc(ctx, P.g0, (size_t)3 * ne)) || (r = dalloc(ctx, P.area, ne)) ||
+      (r = dalloc(ctx, P.ctr, 8)) ||
       (r = dalloc(ctx, P.fc, (size_t)3 * nv)) || (r = dalloc(ctx, P.fout, (size_t)3 * nv)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nv)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
-      (r = dalloc(ctx, P.npart, (size_t)6 * ctx->nblkN)) || (r = dalloc(ctx, P.cpart, (size_t)3 * ctx->nblkC)) ||
+      (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) || (r = dalloc(ctx, P.cpart, (size_t)3 * ctx->nblkC)) ||
       (r = dalloc(ctx, P.sh, 1)))
     return fail(r);
   for (int b = 0; b < 2; ++b)
@@ -318,21 +323,19 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
     int32_t *dcs, *dci;
     if ((r = upload(ctx, dcs, cs)) || (r = upload(ctx, dci, ci))) return fail(r);
     P.close_start = dcs; P.close_ids = dci;
-    int nbpm = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, (const void*)k_broad, 256, 0) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || nbpm < 1) {
-      ctx->err = "cooperative occupancy query failed";
-      return fail(SHX_E_HIP);
-    }
-    ctx->gridBP = std::max(1, std::min(nbpm * ncu, 1024));
-    P.cell_cap = (unsigned)std::min<double>((double)C.n_cells_max + 2, (double)(1u << 27));
-    P.ent_cap = 64 * ne + (1 << 20);
+    /* hashed cell grid: about four entries per bucket for the expected
+     * number of (cell, element) entries, (1/cell_size_mult + 2)^3 each */
+    const double kc = std::pow(1.0 / prm->cell_size_mult + 2.0, 3.0);
+    const double want = std::max(4096.0, (double)ne * kc / 4.0);
+    unsigned bits = 12;
+    while ((double)(1u << bits) < want && bits < 24) ++bits;
+    P.hb_bits = bits;
     P.pair_cap = 32 * ne + 4096;
     const int ntiles = (ne + 255) / 256;
     if ((r = dalloc(ctx, P.ebox, (size_t)6 * ne)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ntiles)) ||
-        (r = dalloc(ctx, P.ecell, (size_t)6 * ne)) || (r = dalloc(ctx, P.cell_cnt, (size_t)P.cell_cap + 1)) ||
-        (r = dalloc(ctx, P.cell_start, (size_t)P.cell_cap + 1)) || (r = dalloc(ctx, P.cell_elems, (size_t)P.ent_cap)) ||
-        (r = dalloc(ctx, P.scan_tmp, (size_t)ctx->gridBP + 1)) || (r = dalloc(ctx, P.found_cnt, (size_t)2 * (ne + 1))) ||
+        (r = dalloc(ctx, P.ecell, (size_t)6 * ne)) || (r = dalloc(ctx, P.bcount, (size_t)1 << bits)) ||
+        (r = dalloc(ctx, P.bent, ((size_t)1 << bits) * kBucketCap)) ||
+        (r = dalloc(ctx, P.found_cnt, (size_t)2 * (ne + 1))) ||
         (r = dalloc(ctx, P.found_e2, (size_t)kRowCap * ne)) || (r = dalloc(ctx, P.found_d2, (size_t)kRowCap * ne)))
       return fail(r);
     for (int b = 0; b < 2; ++b)
@@ -398,10 +401,9 @@ int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* 
   if (v) HIPCHK(hipMemcpy(ctx->P.V[s.par], v, bytes, hipMemcpyHostToDevice));
   const DPtrs& P = ctx->P;
   const SConst& C = ctx->C;
-  k_elem<<<ctx->nblkE, 256, 0, ctx->st>>>(P, C, M_FORCES);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, C, M_FORCES);
   if (C.contact) k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, C, M_FORCES);
-  k_node<<<ctx->nblkN, 256, 0, ctx->st>>>(P, C, M_FORCES, t, ctx->nblkE);
-  k_final<<<1, 64, 0, ctx->st>>>(P, C, M_FORCES, ctx->nblkE, ctx->nblkN, C.contact ? ctx->nblkC : 0);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, C, M_FORCES, t, C.contact ? ctx->nblkC : 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->st));
   if (f) HIPCHK(hipMemcpy(f, P.fout, bytes, hipMemcpyDeviceToHost));
@@ -438,8 +440,7 @@ int shx_init_dynamics(shx_ctx* ctx, int precision)
   if (int r = write_sh(ctx, s)) return r;
   const DPtrs& P = ctx->P;
   const SConst& C = ctx->C;
-  k_elem<<<ctx->nblkE, 256, 0, ctx->st>>>(P, C, M_INIT);
-  k_final<<<1, 64, 0, ctx->st>>>(P, C, M_INIT, ctx->nblkE, ctx->nblkN, 0);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, C, M_INIT);
   HIPCHK(hipGetLastError());
   if (C.contact) {
     if (int r = read_sh(ctx, s)) return r;
@@ -449,7 +450,7 @@ int shx_init_dynamics(shx_ctx* ctx, int precision)
     k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, C, M_INIT);
   }
   if (int r = read_sh(ctx, s)) return r;
-  k_node<<<ctx->nblkN, 256, 0, ctx->st>>>(P, C, M_INIT, s.t + s.dt, ctx->nblkE);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, C, M_INIT, s.t + s.dt, C.contact ? ctx->nblkC : 0);
   HIPCHK(hipGetLastError());
   if (int r = read_sh(ctx, s)) return r;
   prof_collect(ctx);
@@ -570,8 +571,8 @@ int shx_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
   if (int r = write_sh(ctx, s)) return r;
   auto fint = [&](const std::vector<double>& u, std::vector<double>& f) -> int {
     HIPCHK(hipMemcpy(ctx->P.U[s.par], u.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
-    k_elem<<<ctx->nblkE, 256, 0, ctx->st>>>(ctx->P, Ci, M_FORCES);
-    k_node<<<ctx->nblkN, 256, 0, ctx->st>>>(ctx->P, Ci, M_FORCES, 0.0, ctx->nblkE);
+    k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(ctx->P, Ci, M_FORCES);
+    k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(ctx->P, Ci, M_FORCES, 0.0, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(f.data(), ctx->P.fout, sizeof(double) * n3, hipMemcpyDeviceToHost));
     return 0;

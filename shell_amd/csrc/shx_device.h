@@ -10,9 +10,10 @@
  *    staged in LDS by the kernels that use it);
  *  - node state u, v, a twice (ping-pong; DShell::par selects the current
  *    copy), node-interleaved xyz FP64;
- *  - per-element generalised stresses G (18 rows) written by the element
- *    kernel and gathered by the node kernel through the node incidence list,
- *    so no force is ever scattered with atomics;
+ *  - per element and patch slot, the internal-force contribution (3 FP64,
+ *    slot-major) and per element JxW a3, written by the element kernel and
+ *    gathered by the node kernel through the node incidence list, so no
+ *    internal force is ever scattered with atomics;
  *  - the contact pair list (sorted by eid1 then eid2, ping-pong) and its
  *    per-pair friction state.
  */
@@ -30,9 +31,12 @@ enum {
   T_FF1 = 20, T_FF2 = 23, T_XYZ = 26, T_ROWS = 29
 };
 
-/* generalised stress rows: G0 = JxW a3 (pressure), G1, G2 (first-derivative
- * terms), G3, G4, G5 (second-derivative terms) */
-enum { G_ROWS = 18 };
+constexpr int kElemLanes = 4;       /* lanes cooperating on one element (3 patch slots each) */
+constexpr int kElemBlock = 256;     /* threads per element-kernel block: 64 elements */
+constexpr int kNodeLanes = 4;       /* lanes cooperating on one node */
+constexpr int kNodeBlock = 256;     /* threads per node-kernel block */
+constexpr int kNodeRec = 22;        /* node-kernel partial record length */
+constexpr int kBucketCap = 64;      /* (cell, element) entries per hash bucket */
 
 /* per-shell constants (kernel argument) */
 struct SConst {
@@ -49,7 +53,7 @@ struct SConst {
   double beta, gamma, dt_min, dt_max, rel_loc_err_min, rel_loc_err_max;
   double conv_tol, load_end_time;
   int adapt_dt_every, contact, self_friction, quirk_fulltime, quirk_volume_schedule;
-  int stop_on_converged;
+  int stop_on_converged, volume_at_prediction;
   unsigned n_cells_max;
 };
 
@@ -68,6 +72,9 @@ struct DShell {
   int nd[3], n_cells;
   int n_pairs, pair_buf, n_entries;
   int steps_target, done, attempts;
+  /* results of the element kernel of the current attempt */
+  double att_e_tens, att_e_bend, att_volume, att_area;
+  double fdeform, residual_deform;
 };
 
 /* error bits in DShell::err */
@@ -88,7 +95,9 @@ struct DPtrs {
   double* U[2];
   double* V[2];
   double* A[2];
-  double* G;                  /* G_ROWS*ne */
+  double* qe;                 /* 36*ne: per element and patch slot, internal-force contribution */
+  double* g0;                 /* 3*ne: JxW a3 per element (pressure per unit p) */
+  unsigned* ctr;              /* last-block counters: elem, node, bp_box, bp_pairs, bp_emit */
   double* area;               /* ne */
   double* fc;                 /* nv*3 contact force accumulator */
   double* fout;               /* nv*3 debug outputs (forces mode) */
@@ -103,11 +112,10 @@ struct DPtrs {
   double* ebox;               /* 6*ne, SoA */
   double* bpart;              /* broad-phase tile partials: 7 per tile */
   int32_t* ecell;             /* 6*ne: cell index ranges lo[3], hi[3] */
-  uint32_t* cell_cnt;         /* n_cells_max+1 */
-  uint32_t* cell_start;       /* n_cells_max+1 */
-  int32_t* cell_elems;        /* entry capacity */
-  uint32_t* scan_tmp;         /* block sums for scans */
-  int32_t* found_cnt;         /* ne+1 */
+  uint32_t* bcount;           /* hash buckets of the cell grid: fill counts */
+  uint64_t* bent;             /* bucket entries (cell << 32 | element) */
+  unsigned hb_bits;           /* log2 of the number of buckets */
+  int32_t* found_cnt;         /* 2*(ne+1): pair counts, new-pair counts */
   int32_t* found_e2;          /* ne*ROWCAP */
   double* found_d2;           /* ne*ROWCAP */
   int32_t* row_start[2];      /* ne+1, per pair buffer */
@@ -116,8 +124,7 @@ struct DPtrs {
   double* pd2[2];
   int32_t* pstick[2];
   double* pso[2];             /* 3 per pair */
-  int ent_cap, pair_cap;
-  unsigned cell_cap;
+  int pair_cap;
 };
 
 constexpr int kRowCap = 64;      /* new candidates per element and rebuild */

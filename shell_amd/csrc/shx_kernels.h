@@ -20,10 +20,11 @@ __global__ void k_precompute(DPtrs P, SConst C);
 __global__ void k_mass(DPtrs P, SConst C, double damping_alpha, double damping_beta, double damping_gamma);
 __global__ void k_elem(DPtrs P, SConst C, int mode);
 __global__ void k_contact(DPtrs P, SConst C, int mode);
-__global__ void k_node(DPtrs P, SConst C, int mode, double t_force, int nblkE);
-__global__ void k_gate(DPtrs P, SConst C, int nblkE);
-__global__ void k_final(DPtrs P, SConst C, int mode, int nblkE, int nblkN, int nblkC);
-__global__ void k_broad(DPtrs P, SConst C, int posmode);
+__global__ void k_node(DPtrs P, SConst C, int mode, double t_force, int nblkC);
+__global__ void k_bp_box(DPtrs P, SConst C, int posmode);
+__global__ void k_bp_bin(DPtrs P, SConst C);
+__global__ void k_bp_pairs(DPtrs P, SConst C, int posmode);
+__global__ void k_bp_emit(DPtrs P, SConst C);
 
 } // namespace shx
 #endif

@@ -3,19 +3,17 @@
  * precompute, lumped mass, element force (metrics, strains, stresses),
  * node gather fused with the Newmark corrector/predictor, contact narrow
  * phase with friction, per-step finalize (reductions, adaptive dt, rebuild
- * triggers) and the linked-cell broad phase (one cooperative kernel).
+ * triggers, in the node kernel's last block) and the linked-cell broad
+ * phase (four kernels over a hashed cell grid).
  *
  * All arithmetic is FP64 and compiled with -ffp-contract=off so that it
  * rounds like the CPU oracle it is checked against.
  */
 #include <hip/hip_runtime.h>
-#include <hip/hip_cooperative_groups.h>
 
 #include "shx_device.h"
 #include "shx_kernels.h"
 #include "shx_math.h"
-
-namespace cg = cooperative_groups;
 
 namespace shx {
 
@@ -186,27 +184,129 @@ __global__ void __launch_bounds__(256) k_mass(DPtrs P, SConst C, double damping_
   P.minv[n] = 1 / m;
 }
 
+
+/* ------------------------------------------------------- block utilities */
+
+/* every block of a kernel calls this once at its end; returns true in the
+ * last block to arrive, which then sees all other blocks' global writes
+ * (release/acquire fences around a device-scope counter).  The last block
+ * resets the counter for the next launch. */
+__device__ bool last_block(unsigned* counter)
+{
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = atomicAdd(counter, 1u);
+    s_last = (prev == gridDim.x - 1);
+    if (s_last) *counter = 0;
+  }
+  __syncthreads();
+  __threadfence();
+  return s_last != 0;
+}
+
+/* deterministic block reduction of NC components of nparts partial records
+ * (record length `stride`, components comp[0..NC), op 0: sum, 1: max):
+ * thread t folds records t, t+bs, ... in order (several loads in flight),
+ * then a fixed tree over the threads.  s holds NC*blockDim doubles.
+ * Results valid in every thread. */
+template <int NC>
+__device__ void reduce_records(const double* part, int nparts, int stride, const int* comp, const int* op,
+                               double* out, double* s)
+{
+  const int t = threadIdx.x, bs = blockDim.x;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0;
+#pragma unroll 4
+  for (int b = t; b < nparts; b += bs) {
+    const double* r = part + (size_t)b * stride;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double x = r[comp[c]];
+      acc[c] = op[c] ? fmax(acc[c], x) : acc[c] + x;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) s[c * bs + t] = acc[c];
+  __syncthreads();
+  for (int w = bs / 2; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        s[c * bs + t] = op[c] ? fmax(s[c * bs + t], s[c * bs + t + w]) : s[c * bs + t] + s[c * bs + t + w];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = s[c * bs];
+  __syncthreads();
+}
+
+__device__ void set_newmark_dev(DShell& s, const SConst& C, double dt)
+{
+  dt = fmax(dt, C.dt_min);
+  dt = fmin(dt, C.dt_max);
+  s.dt = dt;
+  const double b = C.beta, g = C.gamma;
+  s.nm[0] = b * dt * dt;
+  s.nm[1] = b != 0 ? g / (b * dt) : 0;
+  s.nm[2] = b != 0 ? 1 / (b * dt) : 0;
+  s.nm[3] = b != 0 ? 1 / (2 * b) - 1 : 0;
+  s.nm[4] = b != 0 ? g / b - 1 : 0;
+  s.nm[5] = b != 0 ? dt * (g / (2 * b) - 1) : 0;
+  s.nm[6] = (1 - g) * dt;
+  s.nm[7] = g * dt;
+  s.nm[8] = dt * dt * (0.5 - b);
+  s.nm[9] = dt * dt * fabs(b - 1. / 6);
+}
+
+/* check_increment (ShellSystem.C:2591-2652) on a max |du|^2 */
+__device__ void check_increment_dev(DShell& s, const SConst& C, double du2max, bool require_change)
+{
+  const double d = sqrt(du2max);
+  s.du_max = d;
+  if (isfinite(d)) {
+    s.du_sum += 2 * d;
+    if (require_change && d == 0) s.problem = max(s.problem, 1);
+    else if (C.contact && d > 0.5 * C.h) s.problem = max(s.problem, 2);
+  } else s.problem = max(s.problem, 7);
+}
+
+__device__ __forceinline__ bool rebuild_due(const DShell& s, const SConst& C)
+{
+  /* ShellSystem.C:1917-1919 */
+  return C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
+}
+
 /* --------------------------------------------------------- element force */
 
 /* One thread per element: gather u_pred over the patch, form the deformed
  * metric (compute_deformed_metrics, ShellSystem.C:1407-1460), membrane and
  * bending strains, orthotropic H, stress resultants and the p-vectors
  * (subtract_internal_force, ShellSystem.C:1164-1233), and write the element's
- * generalised stresses G.  The nodal force of patch slot i is then
+ * generalised stresses G and from them the force contribution of each of its
+ * 12 patch slots (written slot-major for the node gather).  The nodal force
+ * of patch slot i is
  *   f_i = -(phi1 G1 + phi2 G2 - phi11 G3 - phi22 G4 - phi12 G5) + p phi G0,
- * which is the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis
- * derivative (ShellSystem.C:1236-1282) plus the LOCAL pressure load
- * (ShellSystem.C:1570-1576); the node kernel forms it as a gather. */
-__global__ void __launch_bounds__(256) k_elem(DPtrs P, SConst C, int mode)
+ * the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis derivative
+ * (ShellSystem.C:1236-1282) plus the LOCAL pressure load (ShellSystem.C:
+ * 1570-1576).  The last block reduces the block partials: energies, the
+ * enclosed volume of the current state (compute_enclosed_volume,
+ * ShellSystem.C:2505-2541) and, in a step, the predictor increment and the
+ * broad-phase trigger (check_increment(false), ShellSystem.C:978, 993). */
+__global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, SConst C, int mode)
 {
   __shared__ double s_rb[4 * 72];
-  __shared__ double s_red[5][256];
+  __shared__ double s_red[5][kElemBlock];
   const DShell& sh = *P.sh;
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
   __syncthreads();
   const int ne = C.ne;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x;
+  const int lane = t % kElemLanes;
+  const int e = (blockIdx.x * kElemBlock + t) / kElemLanes;
   const bool valid = e < ne;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
@@ -214,16 +314,21 @@ __global__ void __launch_bounds__(256) k_elem(DPtrs P, SConst C, int mode)
   const double* U = P.U[par];
   const double* V = P.V[par];
   const double* Aa = P.A[par];
+  const double* T = P.tab;
   double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  /* each lane gathers patch slots lane, lane+4, lane+8 */
+  v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
+  v3 xa = du1, xb = du1, xc = du1;
+  double R00 = 0, R01 = 0, R10 = 0, R11 = 0;
+  int N = 6;
   if (valid) {
-    const double* T = P.tab;
-    const double R00 = T[(size_t)T_R00 * ne + e], R01 = T[(size_t)T_R01 * ne + e];
-    const double R10 = T[(size_t)T_R10 * ne + e], R11 = T[(size_t)T_R11 * ne + e];
-    const int N = P.vcls[e];
+    R00 = T[(size_t)T_R00 * ne + e]; R01 = T[(size_t)T_R01 * ne + e];
+    R10 = T[(size_t)T_R10 * ne + e]; R11 = T[(size_t)T_R11 * ne + e];
+    N = P.vcls[e];
     const double* rb = s_rb + 72 * (N - 3);
-    v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
-    v3 xv[3];
-    for (int i = 0; i < 12; ++i) {
+#pragma unroll
+    for (int k = 0; k < 12 / kElemLanes; ++k) {
+      const int i = lane + kElemLanes * k;
       const int n = P.pidx[(size_t)i * ne + e];
       const v3 u = ld3(U, n);
       v3 up = u;
@@ -234,15 +339,33 @@ __global__ void __launch_bounds__(256) k_elem(DPtrs P, SConst C, int mode)
       axpy(du11, up, b.d11);
       axpy(du22, up, b.d22);
       axpy(du12, up, b.d12);
-      const int vslot = (i == 0) ? 0 : (i == 1) ? 1 : (i == N) ? 2 : -1;
-      if (vslot >= 0) {
+      if (i == 0 || i == 1 || i == N) {
         if (predict) {
           const v3 d = sub(up, u);
           dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
         }
-        xv[vslot] = add(ld3(P.X, n), u);
+        const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
+        if (i == 0) xa = x;
+        else if (i == 1) xb = x;
+        else xc = x;
       }
     }
+  }
+  /* combine the lanes: fixed butterfly (the lanes of one element are
+   * adjacent and all valid or all invalid) */
+#pragma unroll
+  for (int off = 1; off < kElemLanes; off <<= 1) {
+    v3* vs[8] = {&du1, &du2, &du11, &du22, &du12, &xa, &xb, &xc};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      vs[q]->x += __shfl_xor(vs[q]->x, off);
+      vs[q]->y += __shfl_xor(vs[q]->y, off);
+      vs[q]->z += __shfl_xor(vs[q]->z, off);
+    }
+    dumax = fmax(dumax, __shfl_xor(dumax, off));
+  }
+  if (valid) {
+    const double* rb = s_rb + 72 * (N - 3);
     const v3 a1 = add(ldrow3(T, T_A1, ne, e), du1);
     const v3 a2 = add(ldrow3(T, T_A2, ne, e), du2);
     v3 a3 = crs(a1, a2);
@@ -288,24 +411,35 @@ __global__ void __launch_bounds__(256) k_elem(DPtrs P, SConst C, int mode)
     v3 g2 = add(add(add(add(scl(a2, nn[1]), scl(a1, nn[2])), scl(p21, mm[0])), scl(p22, mm[1])), scl(p23, m2x2));
     const v3 G[6] = {scl(a3, JxW), scl(g1, JxW), scl(g2, JxW), scl(a3, JxW * mm[0]), scl(a3, JxW * mm[1]),
                      scl(a3, JxW * m2x2)};
-    for (int r = 0; r < 6; ++r) {
-      P.G[(size_t)(3 * r) * ne + e] = G[r].x;
-      P.G[(size_t)(3 * r + 1) * ne + e] = G[r].y;
-      P.G[(size_t)(3 * r + 2) * ne + e] = G[r].z;
+    /* per patch slot: the internal-force contribution qe_i (f_i -= qe_i),
+     * slot-major so that each node gathers 24 contiguous bytes per slot;
+     * and the pressure vector JxW a3 of the element (f_i += p phi_i JxW a3) */
+    double* qrow = P.qe + (size_t)36 * e;
+#pragma unroll
+    for (int k = 0; k < 12 / kElemLanes; ++k) {
+      const int i = lane + kElemLanes * k;
+      const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
+      const v3 q = sub(sub(sub(add(scl(G[1], b.d1), scl(G[2], b.d2)), scl(G[3], b.d11)), scl(G[4], b.d22)),
+                       scl(G[5], b.d12));
+      qrow[3 * i] = q.x; qrow[3 * i + 1] = q.y; qrow[3 * i + 2] = q.z;
     }
-    Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
-    Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
-    const double new_area = 0.5 * jac;
-    oarea = (mode == M_INIT) ? new_area : P.area[e];
-    P.area[e] = new_area;
-    /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
-    const v3 a = xv[0], b = xv[1], c = xv[2];
-    det = a.x * (b.y * c.z - c.y * b.z) - b.x * (a.y * c.z - c.y * a.z) + c.x * (a.y * b.z - b.y * a.z);
+    if (lane == 0) {
+      P.g0[3 * e] = G[0].x; P.g0[3 * e + 1] = G[0].y; P.g0[3 * e + 2] = G[0].z;
+      Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
+      Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
+      const double new_area = 0.5 * jac;
+      oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
+      P.area[e] = new_area;
+      /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
+      const v3 a = xa, b = xb, c = xc;
+      det = a.x * (b.y * c.z - c.y * b.z) - b.x * (a.y * c.z - c.y * a.z) + c.x * (a.y * b.z - b.y * a.z);
+    } else {
+      dumax = 0;
+    }
   }
-  const int t = threadIdx.x;
   s_red[0][t] = Et; s_red[1][t] = Eb; s_red[2][t] = dumax; s_red[3][t] = det; s_red[4][t] = oarea;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kElemBlock / 2; s > 0; s >>= 1) {
     if (t < s) {
       s_red[0][t] += s_red[0][t + s];
       s_red[1][t] += s_red[1][t + s];
@@ -316,6 +450,21 @@ __global__ void __launch_bounds__(256) k_elem(DPtrs P, SConst C, int mode)
     __syncthreads();
   }
   if (t < 5) P.epart[(size_t)blockIdx.x * 5 + t] = s_red[t][0];
+  if (!last_block(P.ctr + 0)) return;
+  const int comp[5] = {0, 1, 2, 3, 4}, op[5] = {0, 0, 1, 0, 0};
+  double r[5];
+  reduce_records<5>(P.epart, gridDim.x, 5, comp, op, r, &s_red[0][0]);
+  if (t == 0) {
+    DShell& w = *P.sh;
+    w.att_e_tens = r[0];
+    w.att_e_bend = r[1];
+    w.att_volume = fabs(r[3] / 6) - r[4] * 0.5 * C.h;
+    w.att_area = r[4];
+    if (mode == M_STEP) {
+      check_increment_dev(w, C, r[2], false);
+      w.rebuild = rebuild_due(w, C);
+    }
+  }
 }
 
 /* --------------------------------------------------------------- contact */
@@ -332,7 +481,8 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
 /* add_shell_shell_contact_force, ShellSystem.C:1792-1907: per candidate
  * pair, distance, log-barrier normal force, normal damping and Margolis
  * friction; the force is split -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w)
- * onto element 2 and accumulated with FP64 atomics into fc. */
+ * onto element 2 and accumulated with FP64 atomics into fc (the node kernel
+ * reads and clears it). */
 __global__ void __launch_bounds__(256) k_contact(DPtrs P, SConst C, int mode)
 {
   __shared__ double s_red[3][256];
@@ -420,69 +570,183 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, SConst C, int mode)
   if (t < 3) P.cpart[(size_t)blockIdx.x * 3 + t] = s_red[t][0];
 }
 
-/* ------------------------------------------------------------ node force */
+/* ---------------------------------------------------- node gather + step */
 
-/* One thread per node: gather f from the generalised stresses of every
- * (element, slot) whose patch holds the node, add the contact force, then
- * the damped central-difference / Newmark update (ShellSystem.C:1006-1008,
- * 1068-1090): a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a,
- * v = v_pred + gamma dt a, written to the other state buffer.  Block
- * partials: |f|^2, |f_ext|^2, max|a-a_old|, max|v_pred|, max|du|^2, E_kin. */
-__global__ void __launch_bounds__(256) k_node(DPtrs P, SConst C, int mode, double t_force, int nblkE)
+/* ||f - B q|| for the least-squares rigid-body inertial force B q (node i:
+ * m_i [I | -[x_i]x], about the origin at the force-evaluation positions)
+ * from the sums b = B^T f (6) and B^T B (10 unique: m^2, m^2 x, m^2 (|x|^2 I
+ * - x x^T)): ||f||^2 - b^T G^-1 b, Cholesky on the 6x6 G. */
+__device__ double rigid_free_norm(double f2, const double* r)
+{
+  const double* b = r;
+  const double M2 = r[6], sx = r[7], sy = r[8], sz = r[9];
+  /* G = [[M2 I, -[s]x], [[s]x, J]], [s]x the cross matrix of (sx,sy,sz) */
+  double G[6][6] = {};
+  G[0][0] = G[1][1] = G[2][2] = M2;
+  const double S[3][3] = {{0, -sz, sy}, {sz, 0, -sx}, {-sy, sx, 0}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) { G[i][3 + j] = -S[i][j]; G[3 + i][j] = S[i][j]; }
+  G[3][3] = r[10]; G[4][4] = r[11]; G[5][5] = r[12];
+  G[3][4] = G[4][3] = r[13]; G[3][5] = G[5][3] = r[14]; G[4][5] = G[5][4] = r[15];
+  double L[6][6] = {};
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double sum = G[i][j];
+      for (int k = 0; k < j; ++k) sum -= L[i][k] * L[j][k];
+      if (i == j) L[i][i] = sqrt(fmax(sum, 1e-300));
+      else L[i][j] = sum / L[j][j];
+    }
+  double y[6], proj = 0;
+  for (int i = 0; i < 6; ++i) {
+    double sum = b[i];
+    for (int k = 0; k < i; ++k) sum -= L[i][k] * y[k];
+    y[i] = sum / L[i][i];
+    proj += y[i] * y[i];
+  }
+  return sqrt(fmax(f2 - proj, 0.0));
+}
+
+/* finalize of one step attempt (ShellSystem.C:1006-1117): residual and
+ * energies, local-error estimate, accept (state buffer flip, time advance,
+ * check_increment, corrector-time broad-phase trigger) or reject with a
+ * smaller dt.  Run by the last block of the node kernel. */
+__device__ void finalize(const DPtrs& P, const SConst& C, int mode, int nblkC, double* s)
+{
+  int cn[kNodeRec], on[kNodeRec];
+  for (int q = 0; q < kNodeRec; ++q) { cn[q] = q; on[q] = (q >= 2 && q <= 4); }
+  double rn[kNodeRec];
+  reduce_records<kNodeRec>(P.npart, gridDim.x, kNodeRec, cn, on, rn, s);
+  const int cc[3] = {0, 1, 2}, oc[3] = {0, 0, 0};
+  double rc[3];
+  reduce_records<3>(P.cpart, nblkC, 3, cc, oc, rc, s);
+  const double f2 = rn[0], fe2 = rn[1], err = rn[2], vmax = rn[3], du = rn[4], ek = rn[5];
+  const double nc = rc[0], es = rc[1], ef = rc[2];
+  if (threadIdx.x != 0) return;
+  DShell& w = *P.sh;
+  w.e_tens = w.att_e_tens; w.e_bend = w.att_e_bend; w.e_self = es; w.e_fric = ef;
+  w.n_contacts = (int)nc;
+  w.fnorm = sqrt(f2);
+  w.fext_norm = sqrt(fe2);
+  w.fdeform = rigid_free_norm(f2, rn + 6);
+  if (mode == M_INIT) {
+    w.volume = w.volume0 = w.att_volume;
+    w.area = w.att_area;
+    return;
+  }
+  if (mode == M_FORCES) return;
+  w.attempts++;
+  if (w.fresh) { w.volume = w.att_volume; w.area = w.att_area; }
+  const double dt = w.dt;
+  const double target = sqrt(C.rel_loc_err_min * C.rel_loc_err_max);
+  double rel = target, new_dt = dt;
+  const bool adapt_now = C.adapt_dt_every > 0 && ((w.n_steps + 1) % C.adapt_dt_every) == 0;
+  if (adapt_now) {
+    if (isfinite(err)) rel = w.nm[9] * err / C.h;
+    else w.problem = max(w.problem, 7);
+    if (rel == 0) new_dt = 2 * dt;
+    else if (rel > C.rel_loc_err_max || rel < C.rel_loc_err_min) new_dt = dt * pow(target / rel, 1. / 3);
+    if (C.contact) {
+      if (new_dt * vmax * 10 * 2 > C.h) new_dt = C.h / (2 * 10 * vmax);
+    }
+  }
+  if (new_dt >= dt) {
+    check_increment_dev(w, C, du, true);
+    w.par = 1 - w.par;
+    w.t += dt;
+    w.n_steps++;
+    w.fresh = 1;
+    w.residual = w.fext_norm > 0 ? w.fnorm / w.fext_norm : w.fnorm;
+    w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
+    w.e_kin = ek;
+    w.rebuild = rebuild_due(w, C);
+    set_newmark_dev(w, C, new_dt);
+    if (w.converged_step < 0 && w.t >= C.load_end_time && w.residual_deform <= C.conv_tol) {
+      w.converged_step = w.n_steps;
+      if (C.stop_on_converged) w.done = 1;
+    }
+    if (w.n_steps >= w.steps_target) w.done = 1;
+  } else {
+    w.n_rejected++;
+    w.fresh = 0;
+    w.rebuild = 0;
+    set_newmark_dev(w, C, new_dt);
+    if (w.dt <= 2.220446049250313e-16) { w.problem = max(w.problem, 5); w.done = 1; }
+  }
+  if (w.problem >= 7 || w.err) w.done = 1;
+}
+
+/* kNodeLanes lanes per node: each lane folds every kNodeLanes-th entry of the
+ * node's (element, slot) incidence list (the slot's internal-force
+ * contribution and phi_i times the element's JxW a3), the lanes combine by a
+ * fixed butterfly,
+ * then lane 0 adds the contact force and applies the damped
+ * central-difference / Newmark update (ShellSystem.C:1006-1008, 1068-1090):
+ * a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a, v = v_pred + gamma dt a,
+ * written to the other state buffer.  Block partials: |f|^2, |f_ext|^2,
+ * max|a-a_old|, max|v_pred|, max|du|^2, E_kin; the last block finalizes. */
+__global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, SConst C, int mode, double t_force, int nblkC)
 {
   __shared__ double s_rb[4 * 72];
-  __shared__ double s_red[6][256];
-  __shared__ double s_vol;
+  __shared__ double s_red[kNodeRec][kNodeBlock];
+  __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
   const DShell& sh = *P.sh;
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
-  const int t = threadIdx.x;
-  if (t == 0) {
-    double V = sh.volume;
-    if (C.bulk_modulus != 0 && ((mode == M_STEP && sh.fresh) || mode == M_INIT)) {
-      double d = 0, ar = 0;
-      for (int b = 0; b < nblkE; ++b) { d += P.epart[5 * b + 3]; ar += P.epart[5 * b + 4]; }
-      V = fabs(d / 6) - ar * 0.5 * C.h;
-    }
-    s_vol = V;
-  }
   __syncthreads();
-  const int ne = C.ne;
-  const int n = blockIdx.x * blockDim.x + t;
+  const int t = threadIdx.x;
+  const int lane = t % kNodeLanes;
+  const int n = blockIdx.x * (kNodeBlock / kNodeLanes) + t / kNodeLanes;
+  const bool valid = n < C.nv;
   const int par = sh.par;
   const double dt = sh.dt;
-  double r_f2 = 0, r_fe2 = 0, r_err = 0, r_vmax = 0, r_du = 0, r_ek = 0;
-  if (n < C.nv) {
+  v3 fint = mk(0, 0, 0), fp = fint;
+  if (valid) {
+    for (int q = P.inc_start[n] + lane; q < P.inc_start[n + 1]; q += kNodeLanes) {
+      const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
+      const double phi = s_rb[72 * (P.vcls[e] - 3) + i];
+      const double* qs = P.qe + (size_t)36 * e + 3 * i;
+      fint = sub(fint, mk(qs[0], qs[1], qs[2]));
+      axpy(fp, ld3(P.g0, e), phi);
+    }
+  }
+  for (int off = 1; off < kNodeLanes; off <<= 1) {
+    fint.x += __shfl_xor(fint.x, off); fint.y += __shfl_xor(fint.y, off); fint.z += __shfl_xor(fint.z, off);
+    fp.x += __shfl_xor(fp.x, off); fp.y += __shfl_xor(fp.y, off); fp.z += __shfl_xor(fp.z, off);
+  }
+  /* partial record: |f|^2, |f_ext|^2, max|a-a_old|, max|v_pred|, max|du|^2,
+   * E_kin, then the rigid-body least-squares sums B^T f (6: m f, m x*f) and
+   * B^T B (10: m^2, m^2 x, m^2 (|x|^2 - x_r x_c) for xx yy zz xy xz yz) */
+  double rec[kNodeRec];
+#pragma unroll
+  for (int q = 0; q < kNodeRec; ++q) rec[q] = 0;
+  if (valid && lane == 0) {
     const double t_eval = (mode == M_STEP) ? sh.t + dt : t_force;
     double ptot = pressure_at(C, t_eval);
     if (C.bulk_modulus != 0) {
-      const double V = s_vol;
-      const double V0 = (mode == M_INIT) ? V : sh.volume0;
-      ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / V - 1);
-    }
-    v3 fint = mk(0, 0, 0), fp = fint;
-    for (int q = P.inc_start[n]; q < P.inc_start[n + 1]; ++q) {
-      const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
-      const double* rb = s_rb + 72 * (P.vcls[e] - 3);
-      const double* T = P.tab;
-      const Basis6 b = basis_at(rb, i, T[(size_t)T_R00 * ne + e], T[(size_t)T_R01 * ne + e],
-                                T[(size_t)T_R10 * ne + e], T[(size_t)T_R11 * ne + e]);
-      const double* G = P.G;
-      v3 g[6];
-      for (int r = 0; r < 6; ++r)
-        g[r] = mk(G[(size_t)(3 * r) * ne + e], G[(size_t)(3 * r + 1) * ne + e], G[(size_t)(3 * r + 2) * ne + e]);
-      const v3 qe = sub(sub(sub(add(scl(g[1], b.d1), scl(g[2], b.d2)), scl(g[3], b.d11)), scl(g[4], b.d22)),
-                        scl(g[5], b.d12));
-      fint = sub(fint, qe);
-      axpy(fp, g[0], b.phi);
+      const double Vc = (C.volume_at_prediction || (mode == M_STEP && sh.fresh) || mode == M_INIT) ? sh.att_volume
+                                                                                                    : sh.volume;
+      const double V0 = (mode == M_INIT) ? Vc : sh.volume0;
+      ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / Vc - 1);
     }
     const v3 fcn = ld3(P.fc, n);
     st3(P.fc, n, mk(0, 0, 0));
     const v3 fext = add(scl(fp, ptot), fcn);
     const v3 f = add(fint, fext);
-    r_f2 = nrm2(f);
-    r_fe2 = nrm2(fext);
+    rec[0] = nrm2(f);
+    rec[1] = nrm2(fext);
     const double mi = P.minv[n];
+    {
+      const v3 u = ld3(P.U[par], n);
+      const v3 x = add(ld3(P.X, n), mode == M_STEP ? pred3(u, ld3(P.V[par], n), ld3(P.A[par], n), dt, sh.nm[8]) : u);
+      const double m = 1 / mi, m2 = m * m;
+      const v3 xf = crs(x, f);
+      rec[6] = m * f.x; rec[7] = m * f.y; rec[8] = m * f.z;
+      rec[9] = m * xf.x; rec[10] = m * xf.y; rec[11] = m * xf.z;
+      rec[12] = m2; rec[13] = m2 * x.x; rec[14] = m2 * x.y; rec[15] = m2 * x.z;
+      const double xx = dot(x, x);
+      rec[16] = m2 * (xx - x.x * x.x); rec[17] = m2 * (xx - x.y * x.y); rec[18] = m2 * (xx - x.z * x.z);
+      rec[19] = m2 * (-x.x * x.y); rec[20] = m2 * (-x.x * x.z); rec[21] = m2 * (-x.y * x.z);
+    }
     if (mode == M_FORCES) {
       st3(P.fout, n, f);
       st3(P.fext_out, n, fext);
@@ -495,207 +759,77 @@ __global__ void __launch_bounds__(256) k_node(DPtrs P, SConst C, int mode, doubl
       const v3 up = pred3(u, v, a, dt, a8);
       const v3 vp = mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
       const v3 an = mk(mi * (f.x - cd * vp.x), mi * (f.y - cd * vp.y), mi * (f.z - cd * vp.z));
-      r_err = fmax(fabs(an.x - a.x), fmax(fabs(an.y - a.y), fabs(an.z - a.z)));
-      r_vmax = fmax(fabs(vp.x), fmax(fabs(vp.y), fabs(vp.z)));
+      rec[2] = fmax(fabs(an.x - a.x), fmax(fabs(an.y - a.y), fabs(an.z - a.z)));
+      rec[3] = fmax(fabs(vp.x), fmax(fabs(vp.y), fabs(vp.z)));
       const v3 un = mk(a0 * an.x + up.x, a0 * an.y + up.y, a0 * an.z + up.z);
       const v3 vn = mk(a7 * an.x + vp.x, a7 * an.y + vp.y, a7 * an.z + vp.z);
       const v3 d = sub(un, u);
-      r_du = d.x * d.x + d.y * d.y + d.z * d.z;
-      r_ek = 0.5 * (vn.x * vn.x / mi + vn.y * vn.y / mi + vn.z * vn.z / mi);
+      rec[4] = d.x * d.x + d.y * d.y + d.z * d.z;
+      rec[5] = 0.5 * (vn.x * vn.x / mi + vn.y * vn.y / mi + vn.z * vn.z / mi);
       st3(P.U[1 - par], n, un);
       st3(P.V[1 - par], n, vn);
       st3(P.A[1 - par], n, an);
     }
   }
-  s_red[0][t] = r_f2; s_red[1][t] = r_fe2; s_red[2][t] = r_err; s_red[3][t] = r_vmax; s_red[4][t] = r_du;
-  s_red[5][t] = r_ek;
+  /* wave butterfly, then the block's waves in order */
+#pragma unroll
+  for (int q = 0; q < kNodeRec; ++q) {
+    const bool mx = (q >= 2 && q <= 4);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(rec[q], off);
+      rec[q] = mx ? fmax(rec[q], o) : rec[q] + o;
+    }
+  }
+  const int wave = t / 64;
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < kNodeRec; ++q) s_wave[q][wave] = rec[q];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) {
-      s_red[0][t] += s_red[0][t + s];
-      s_red[1][t] += s_red[1][t + s];
-      s_red[2][t] = fmax(s_red[2][t], s_red[2][t + s]);
-      s_red[3][t] = fmax(s_red[3][t], s_red[3][t + s]);
-      s_red[4][t] = fmax(s_red[4][t], s_red[4][t + s]);
-      s_red[5][t] += s_red[5][t + s];
-    }
-    __syncthreads();
+  if (t < kNodeRec) {
+    const bool mx = (t >= 2 && t <= 4);
+    double acc = s_wave[t][0];
+    for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
+    P.npart[(size_t)blockIdx.x * kNodeRec + t] = acc;
   }
-  if (t < 6) P.npart[(size_t)blockIdx.x * 6 + t] = s_red[t][0];
-}
-
-/* --------------------------------------------------------------- finalize */
-
-__device__ void set_newmark_dev(DShell& s, const SConst& C, double dt)
-{
-  dt = fmax(dt, C.dt_min);
-  dt = fmin(dt, C.dt_max);
-  s.dt = dt;
-  const double b = C.beta, g = C.gamma;
-  s.nm[0] = b * dt * dt;
-  s.nm[1] = b != 0 ? g / (b * dt) : 0;
-  s.nm[2] = b != 0 ? 1 / (b * dt) : 0;
-  s.nm[3] = b != 0 ? 1 / (2 * b) - 1 : 0;
-  s.nm[4] = b != 0 ? g / b - 1 : 0;
-  s.nm[5] = b != 0 ? dt * (g / (2 * b) - 1) : 0;
-  s.nm[6] = (1 - g) * dt;
-  s.nm[7] = g * dt;
-  s.nm[8] = dt * dt * (0.5 - b);
-  s.nm[9] = dt * dt * fabs(b - 1. / 6);
-}
-
-/* check_increment (ShellSystem.C:2591-2652) on a max |du|^2 */
-__device__ void check_increment_dev(DShell& s, const SConst& C, double du2max, bool require_change)
-{
-  const double d = sqrt(du2max);
-  s.du_max = d;
-  if (isfinite(d)) {
-    s.du_sum += 2 * d;
-    if (require_change && d == 0) s.problem = max(s.problem, 1);
-    else if (C.contact && d > 0.5 * C.h) s.problem = max(s.problem, 2);
-  } else s.problem = max(s.problem, 7);
-}
-
-/* broad-phase trigger at the predictor (ShellSystem.C:978, 993, 1917-1919) */
-__global__ void k_gate(DPtrs P, SConst C, int nblkE)
-{
-  if (threadIdx.x != 0) return;
-  DShell& s = *P.sh;
-  if (s.done) { s.rebuild = 0; return; }
-  double du = 0;
-  for (int b = 0; b < nblkE; ++b) du = fmax(du, P.epart[5 * b + 2]);
-  check_increment_dev(s, C, du, false);
-  s.rebuild = C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
-}
-
-/* reductions, local-error estimate and step acceptance
- * (ShellSystem.C:1010-1117) */
-__global__ void k_final(DPtrs P, SConst C, int mode, int nblkE, int nblkN, int nblkC)
-{
-  if (threadIdx.x != 0) return;
-  DShell& s = *P.sh;
-  if (s.done) return;
-  double et = 0, eb = 0, det = 0, ar = 0;
-  for (int b = 0; b < nblkE; ++b) {
-    et += P.epart[5 * b]; eb += P.epart[5 * b + 1]; det += P.epart[5 * b + 3]; ar += P.epart[5 * b + 4];
-  }
-  double f2 = 0, fe2 = 0, err = 0, vmax = 0, du = 0, ek = 0;
-  for (int b = 0; b < nblkN; ++b) {
-    const double* q = P.npart + 6 * b;
-    f2 += q[0]; fe2 += q[1]; err = fmax(err, q[2]); vmax = fmax(vmax, q[3]); du = fmax(du, q[4]); ek += q[5];
-  }
-  double nc = 0, es = 0, ef = 0;
-  for (int b = 0; b < nblkC; ++b) { nc += P.cpart[3 * b]; es += P.cpart[3 * b + 1]; ef += P.cpart[3 * b + 2]; }
-  const double V = fabs(det / 6) - ar * 0.5 * C.h;
-  s.e_tens = et; s.e_bend = eb; s.e_self = es; s.e_fric = ef;
-  s.n_contacts = (int)nc;
-  s.fnorm = sqrt(f2);
-  s.fext_norm = sqrt(fe2);
-  if (mode == M_INIT) {
-    s.volume = s.volume0 = V;
-    s.area = ar;
-    return;
-  }
-  if (mode == M_FORCES) return;
-  s.attempts++;
-  if (s.fresh) { s.volume = V; s.area = ar; }
-  const double dt = s.dt;
-  const double target = sqrt(C.rel_loc_err_min * C.rel_loc_err_max);
-  double rel = target, new_dt = dt;
-  const bool adapt_now = C.adapt_dt_every > 0 && ((s.n_steps + 1) % C.adapt_dt_every) == 0;
-  if (adapt_now) {
-    if (isfinite(err)) rel = s.nm[9] * err / C.h;
-    else s.problem = max(s.problem, 7);
-    if (rel == 0) new_dt = 2 * dt;
-    else if (rel > C.rel_loc_err_max || rel < C.rel_loc_err_min) new_dt = dt * pow(target / rel, 1. / 3);
-    if (C.contact) {
-      if (new_dt * vmax * 10 * 2 > C.h) new_dt = C.h / (2 * 10 * vmax);
-    }
-  }
-  if (new_dt >= dt) {
-    check_increment_dev(s, C, du, true);
-    s.par = 1 - s.par;
-    s.t += dt;
-    s.n_steps++;
-    s.fresh = 1;
-    s.residual = s.fext_norm > 0 ? s.fnorm / s.fext_norm : s.fnorm;
-    s.e_kin = ek;
-    s.rebuild = C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
-    set_newmark_dev(s, C, new_dt);
-    if (s.converged_step < 0 && s.t >= C.load_end_time && s.residual <= C.conv_tol) {
-      s.converged_step = s.n_steps;
-      if (C.stop_on_converged) s.done = 1;
-    }
-    if (s.n_steps >= s.steps_target) s.done = 1;
-  } else {
-    s.n_rejected++;
-    s.fresh = 0;
-    s.rebuild = 0;
-    set_newmark_dev(s, C, new_dt);
-    if (s.dt <= 2.220446049250313e-16) { s.problem = max(s.problem, 5); s.done = 1; }
-  }
-  if (s.problem >= 7 || s.err) s.done = 1;
+  if (!last_block(P.ctr + 1)) return;
+  finalize(P, C, mode, nblkC, &s_red[0][0]);
 }
 
 /* ------------------------------------------------------------ broad phase */
 
-__device__ __forceinline__ void vload(volatile int* p, int& out) { out = *p; }
+/* compute_shell_shell_contact_candidates after its trigger
+ * (ShellSystem.C:1922-2113) as four kernels that return at once unless
+ * DShell::rebuild is set:
+ *  k_bp_box   element AABBs at the positions, tile partials; last block:
+ *             global box, averaged element size (serial in 256-element
+ *             tiles, then serial over tiles), cell size, d_max, grid size
+ *             and the n_cells_max check (compute_bounding_box and
+ *             ShellSystem.C:1933-1964);
+ *  k_bp_bin   cell index ranges; (cell, element) entries appended with an
+ *             atomic cursor into hashed cell buckets (the reference's
+ *             cell_to_elem_map, ShellSystem.C:1967-2010);
+ *  k_bp_pairs per element: surviving old pairs (stick or d2 <= d_max^2,
+ *             ShellSystem.C:2016-2027) and new candidates from its cells
+ *             (counted in the first cell both ranges share, not close, not
+ *             listed, boxes within d_max, exact d2 <= d_max^2,
+ *             ShellSystem.C:2033-2086), counted; last block: prefix scan;
+ *  k_bp_emit  per element: sorted merge into the other pair buffer, buckets
+ *             cleared; last block publishes the new list.
+ * posmode M_STEP: positions X + u_pred, otherwise X + u. */
 
-/* grid barrier with release/acquire fences around it */
-__device__ __forceinline__ void gsync(cg::grid_group& grid)
+__device__ __forceinline__ bool bp_active(const DShell* sh)
 {
-  __threadfence();
-  grid.sync();
-  __threadfence();
+  return (*(volatile const int*)&sh->rebuild) != 0 && (*(volatile const int*)&sh->err) == 0;
 }
 
-/* exclusive scan of 256 values in a block; returns the prefix, total in *tot */
-__device__ uint32_t block_scan_u32(uint32_t v, uint32_t* s, uint32_t* tot)
-{
-  const int t = threadIdx.x;
-  s[t] = v;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const uint32_t add_ = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += add_;
-    __syncthreads();
-  }
-  const uint32_t incl = s[t];
-  *tot = s[255];
-  __syncthreads();
-  return incl - v;
-}
+__device__ __forceinline__ double ebox(const DPtrs& P, int ne, int r, int e) { return P.ebox[(size_t)r * ne + e]; }
 
-/* exclusive scan of cnt[0..n) into out[0..n], out[n] = total, over the whole
- * cooperative grid: contiguous chunk per block, block totals scanned by
- * block 0. */
-__device__ void grid_scan(cg::grid_group& grid, const uint32_t* cnt, uint32_t* out, int n, uint32_t* tmp,
-                          uint32_t* s_scan)
+/* BoundingBox::apart, BoundingBox.h:64-69 */
+__device__ bool boxes_apart(const DPtrs& P, int ne, int a, int b, double d)
 {
-  const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
-  const int chunk = (n + G - 1) / G;
-  const int lo = min(n, b * chunk), hi = min(n, lo + chunk);
-  uint32_t run = 0;
-  for (int base = lo; base < hi; base += 256) {
-    const int i = base + t;
-    const uint32_t v = i < hi ? cnt[i] : 0;
-    uint32_t tot;
-    const uint32_t pre = block_scan_u32(v, s_scan, &tot);
-    if (i < hi) out[i] = run + pre;
-    run += tot;
-  }
-  if (t == 0) tmp[b] = run;
-  gsync(grid);
-  if (b == 0 && t == 0) {
-    uint32_t acc = 0;
-    for (int q = 0; q < G; ++q) { const uint32_t x = tmp[q]; tmp[q] = acc; acc += x; }
-    out[n] = acc;
-  }
-  gsync(grid);
-  const uint32_t off = tmp[b];
-  for (int i = lo + t; i < hi; i += 256) out[i] += off;
-  gsync(grid);
+  for (int q = 0; q < 3; ++q)
+    if (ebox(P, ne, q, a) > ebox(P, ne, 3 + q, b) + d || ebox(P, ne, q, b) > ebox(P, ne, 3 + q, a) + d) return true;
+  return false;
 }
 
 __device__ bool close_pair(const DPtrs& P, int e1, int e2)
@@ -710,183 +844,185 @@ __device__ bool close_pair(const DPtrs& P, int e1, int e2)
   return false;
 }
 
-__device__ __forceinline__ double ebox(const DPtrs& P, int ne, int r, int e) { return P.ebox[(size_t)r * ne + e]; }
-
-/* BoundingBox::apart, BoundingBox.h:64-69 */
-__device__ bool boxes_apart(const DPtrs& P, int ne, int a, int b, double d)
+__device__ __forceinline__ unsigned bucket_of(unsigned cell, unsigned bits)
 {
-  for (int q = 0; q < 3; ++q)
-    if (ebox(P, ne, q, a) > ebox(P, ne, 3 + q, b) + d || ebox(P, ne, q, b) > ebox(P, ne, 3 + q, a) + d) return true;
-  return false;
+  return (cell * 0x9E3779B1u) >> (32 - bits);
 }
 
-/* compute_shell_shell_contact_candidates after the trigger
- * (ShellSystem.C:1922-2113), one cooperative launch:
- *  1 element AABBs and tile partials (compute_bounding_box);
- *  2 global box, averaged size, cell size, d_max, grid (block 0);
- *  3 cell index ranges, cell counters cleared;
- *  4 counting: one atomic increment per (element, cell);
- *  5 prefix scan of the cell counts;
- *  6 scatter of element ids into the cells;
- *  7 per element: surviving old pairs (stick or d2 <= d2_max) and new
- *    candidates of its cells (first common cell only, not close, boxes within
- *    d_max, exact d2 <= d2_max), counted;
- *  8 prefix scan of the counts;
- *  9 sorted merge into the other pair buffer.
- * posmode: M_STEP positions X + u_pred, otherwise X + u. */
-__global__ void __launch_bounds__(256) k_broad(DPtrs P, SConst C, int posmode)
+__global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
 {
-  cg::grid_group grid = cg::this_grid();
   __shared__ double s_box[6][256];
   __shared__ double s_size[256];
-  __shared__ uint32_t s_scan[256];
   DShell* sh = P.sh;
-  {
-    int rb;
-    vload(&sh->rebuild, rb);
-    if (!rb) return;
-  }
-  const int ne = C.ne, t = threadIdx.x, G = gridDim.x;
+  if (!bp_active(sh)) return;
+  const int ne = C.ne, t = threadIdx.x, tile = blockIdx.x;
   const bool predict = (posmode == M_STEP);
   const double dt = sh->dt, a8 = sh->nm[8];
   const int par = sh->par;
   const double* U = P.U[par];
   const double* V = P.V[par];
   const double* A = P.A[par];
-  /* 1 */
-  const int ntiles = (ne + 255) / 256;
-  for (int tile = blockIdx.x; tile < ntiles; tile += G) {
-    const int e = tile * 256 + t;
-    double bmn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
-    double bmx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
-    double size = 0;
-    if (e < ne) {
-      bool finite = true;
-      for (int k = 0; k < 3; ++k) {
-        const v3 x = node_pos(P, P.tri[(size_t)k * ne + e], U, V, A, predict, dt, a8);
-        const double xx[3] = {x.x, x.y, x.z};
-        finite = finite && isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
-        for (int q = 0; q < 3; ++q) {
-          if (xx[q] < bmn[q]) bmn[q] = xx[q];
-          if (xx[q] > bmx[q]) bmx[q] = xx[q];
-        }
+  const int e = tile * 256 + t;
+  double bmn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+  double bmx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+  double size = 0;
+  if (e < ne) {
+    bool finite = true;
+    for (int k = 0; k < 3; ++k) {
+      const v3 x = node_pos(P, P.tri[(size_t)k * ne + e], U, V, A, predict, dt, a8);
+      const double xx[3] = {x.x, x.y, x.z};
+      finite = finite && isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
+      for (int q = 0; q < 3; ++q) {
+        if (xx[q] < bmn[q]) bmn[q] = xx[q];
+        if (xx[q] > bmx[q]) bmx[q] = xx[q];
       }
-      for (int q = 0; q < 3; ++q) { P.ebox[(size_t)q * ne + e] = bmn[q]; P.ebox[(size_t)(3 + q) * ne + e] = bmx[q]; }
-      double dmax = bmx[0] - bmn[0];
-      const double d1 = bmx[1] - bmn[1], d2 = bmx[2] - bmn[2];
-      if (d1 > dmax) dmax = d1;
-      if (d2 > dmax) dmax = d2;
-      size = finite ? dmax : __builtin_nan(""); /* poisons the tile sum: the rebuild aborts */
     }
-    for (int q = 0; q < 3; ++q) { s_box[q][t] = bmn[q]; s_box[3 + q][t] = bmx[q]; }
-    s_size[t] = size;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s)
-        for (int q = 0; q < 3; ++q) {
-          s_box[q][t] = fmin(s_box[q][t], s_box[q][t + s]);
-          s_box[3 + q][t] = fmax(s_box[3 + q][t], s_box[3 + q][t + s]);
-        }
-      __syncthreads();
-    }
-    if (t == 0) {
-      double sum = 0;
-      const int m = min(256, ne - tile * 256);
-      for (int q = 0; q < m; ++q) sum += s_size[q];
-      for (int q = 0; q < 6; ++q) P.bpart[(size_t)tile * 7 + q] = s_box[q][0];
-      P.bpart[(size_t)tile * 7 + 6] = sum;
-    }
+    for (int q = 0; q < 3; ++q) { P.ebox[(size_t)q * ne + e] = bmn[q]; P.ebox[(size_t)(3 + q) * ne + e] = bmx[q]; }
+    double dmax = bmx[0] - bmn[0];
+    const double d1 = bmx[1] - bmn[1], d2 = bmx[2] - bmn[2];
+    if (d1 > dmax) dmax = d1;
+    if (d2 > dmax) dmax = d2;
+    size = finite ? dmax : __builtin_nan(""); /* poisons the sum: the rebuild aborts */
+  }
+  for (int q = 0; q < 3; ++q) { s_box[q][t] = bmn[q]; s_box[3 + q][t] = bmx[q]; }
+  s_size[t] = size;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int q = 0; q < 3; ++q) {
+        s_box[q][t] = fmin(s_box[q][t], s_box[q][t + s]);
+        s_box[3 + q][t] = fmax(s_box[3 + q][t], s_box[3 + q][t + s]);
+      }
     __syncthreads();
   }
-  gsync(grid);
-  /* 2 */
-  if (blockIdx.x == 0 && t == 0) {
-    double bmn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
-    double bmx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
-    double total = 0;
-    for (int tile = 0; tile < ntiles; ++tile) {
-      const double* q = P.bpart + (size_t)tile * 7;
-      for (int c = 0; c < 3; ++c) { bmn[c] = fmin(bmn[c], q[c]); bmx[c] = fmax(bmx[c], q[3 + c]); }
-      total += q[6];
+  if (t == 0) {
+    double sum = 0;
+    const int m = min(256, ne - tile * 256);
+    for (int q = 0; q < m; ++q) sum += s_size[q];
+    for (int q = 0; q < 6; ++q) P.bpart[(size_t)tile * 7 + q] = s_box[q][0];
+    P.bpart[(size_t)tile * 7 + 6] = sum;
+  }
+  if (!last_block(P.ctr + 2)) return;
+  /* global box: fixed-order tree of tile partials; size: serial over tiles */
+  const int ntiles = gridDim.x;
+  for (int q = 0; q < 6; ++q) {
+    double acc = q < 3 ? 1.7976931348623157e308 : -1.7976931348623157e308;
+    for (int b = t; b < ntiles; b += 256) {
+      const double x = P.bpart[(size_t)b * 7 + q];
+      acc = q < 3 ? fmin(acc, x) : fmax(acc, x);
     }
-    sh->du_sum = 0;
-    sh->n_rebuilds++;
-    sh->avg_elem_size = total / ne;
-    bool ok = true;
-    for (int c = 0; c < 3; ++c) {
-      sh->bmin[c] = bmn[c]; sh->bmax[c] = bmx[c];
-      if (!isfinite(bmn[c]) || !isfinite(bmx[c])) ok = false;
-    }
-    if (!isfinite(total)) ok = false;
-    if (!ok) { sh->problem = max(sh->problem, 7); sh->rebuild = 0; }
-    else {
-      double cell = C.cell_size_mult * sh->avg_elem_size;
-      const double tol = C.cell_margin * cell;
-      cell += tol;
-      const double dmx = C.h + tol;
-      sh->cell_size = cell;
-      sh->d_max = dmx;
-      sh->d2_max = dmx * dmx;
-      unsigned nd[3];
-      for (int c = 0; c < 3; ++c) nd[c] = (unsigned)(int)(((bmx[c] - bmn[c]) + dmx) / cell + 1);
-      const unsigned ncell = nd[0] * nd[1] * nd[2];
-      for (int c = 0; c < 3; ++c) sh->nd[c] = (int)nd[c];
-      sh->n_cells = (int)ncell;
-      if (ncell > C.n_cells_max) { sh->problem = max(sh->problem, 4); sh->rebuild = 0; }
-      else if (ncell + 1 > P.cell_cap) { sh->err |= ERR_CELLS; sh->rebuild = 0; }
-    }
+    s_box[q][t] = acc;
   }
-  gsync(grid);
-  {
-    int rb;
-    vload(&sh->rebuild, rb);
-    if (!rb) return;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int q = 0; q < 3; ++q) {
+        s_box[q][t] = fmin(s_box[q][t], s_box[q][t + s]);
+        s_box[3 + q][t] = fmax(s_box[3 + q][t], s_box[3 + q][t + s]);
+      }
+    __syncthreads();
   }
-  const double cell = sh->cell_size, dmx = sh->d_max, d2mx = sh->d2_max;
-  const double gmin[3] = {sh->bmin[0], sh->bmin[1], sh->bmin[2]};
-  const int nd0 = sh->nd[0], nd1 = sh->nd[1], nd2 = sh->nd[2];
-  const int ncell = sh->n_cells;
-  const int gid = blockIdx.x * 256 + t, gstride = G * 256;
-  /* 3 */
-  for (int c = gid; c <= ncell; c += gstride) P.cell_cnt[c] = 0;
-  for (int e = gid; e < ne; e += gstride)
-    for (int q = 0; q < 3; ++q) {
-      P.ecell[(size_t)q * ne + e] = (int)((ebox(P, ne, q, e) - gmin[q]) / cell);
-      P.ecell[(size_t)(3 + q) * ne + e] = (int)((ebox(P, ne, 3 + q, e) - gmin[q] + dmx) / cell);
-    }
-  gsync(grid);
-  /* 4 */
-  for (int e = gid; e < ne; e += gstride) {
-    const int* L = P.ecell;
-    for (int cx = L[e]; cx <= L[(size_t)3 * ne + e]; ++cx)
-      for (int cy = L[(size_t)ne + e]; cy <= L[(size_t)4 * ne + e]; ++cy)
-        for (int cz = L[(size_t)2 * ne + e]; cz <= L[(size_t)5 * ne + e]; ++cz)
-          atomicAdd(&P.cell_cnt[(size_t)cx * nd1 * nd2 + (size_t)cy * nd2 + cz], 1u);
+  double total = 0;
+  for (int base = 0; base < ntiles; base += 256) {
+    const int b = base + t;
+    s_size[t] = b < ntiles ? P.bpart[(size_t)b * 7 + 6] : 0.0;
+    __syncthreads();
+    if (t == 0)
+      for (int q = 0; q < min(256, ntiles - base); ++q) total += s_size[q];
+    __syncthreads();
   }
-  gsync(grid);
-  /* 5 */
-  grid_scan(grid, P.cell_cnt, P.cell_start, ncell, P.scan_tmp, s_scan);
-  if (P.cell_start[ncell] > (uint32_t)P.ent_cap) {
-    if (blockIdx.x == 0 && t == 0) { sh->err |= ERR_ENTRIES; sh->rebuild = 0; }
-    return; /* uniform: every block reads the same total */
+  if (t != 0) return;
+  sh->du_sum = 0;
+  sh->n_rebuilds++;
+  sh->avg_elem_size = total / ne;
+  bool ok = isfinite(total);
+  double bmn2[3], bmx2[3];
+  for (int c = 0; c < 3; ++c) {
+    bmn2[c] = s_box[c][0]; bmx2[c] = s_box[3 + c][0];
+    sh->bmin[c] = bmn2[c]; sh->bmax[c] = bmx2[c];
+    if (!isfinite(bmn2[c]) || !isfinite(bmx2[c])) ok = false;
   }
-  for (int c = gid; c < ncell; c += gstride) P.cell_cnt[c] = 0;
-  gsync(grid);
-  /* 6 */
-  for (int e = gid; e < ne; e += gstride) {
-    const int* L = P.ecell;
-    for (int cx = L[e]; cx <= L[(size_t)3 * ne + e]; ++cx)
-      for (int cy = L[(size_t)ne + e]; cy <= L[(size_t)4 * ne + e]; ++cy)
-        for (int cz = L[(size_t)2 * ne + e]; cz <= L[(size_t)5 * ne + e]; ++cz) {
-          const size_t c = (size_t)cx * nd1 * nd2 + (size_t)cy * nd2 + cz;
-          const uint32_t pos = P.cell_start[c] + atomicAdd(&P.cell_cnt[c], 1u);
-          P.cell_elems[pos] = e;
-        }
+  if (!ok) { sh->problem = max(sh->problem, 7); sh->rebuild = 0; return; }
+  double cell = C.cell_size_mult * sh->avg_elem_size;
+  const double tol = C.cell_margin * cell;
+  cell += tol;
+  const double dmx = C.h + tol;
+  sh->cell_size = cell;
+  sh->d_max = dmx;
+  sh->d2_max = dmx * dmx;
+  unsigned nd[3];
+  for (int c = 0; c < 3; ++c) nd[c] = (unsigned)(int)(((bmx2[c] - bmn2[c]) + dmx) / cell + 1);
+  const unsigned ncell = nd[0] * nd[1] * nd[2];
+  for (int c = 0; c < 3; ++c) sh->nd[c] = (int)nd[c];
+  sh->n_cells = (int)ncell;
+  if (ncell > C.n_cells_max) { sh->problem = max(sh->problem, 4); sh->rebuild = 0; }
+}
+
+__global__ void __launch_bounds__(256) k_bp_bin(DPtrs P, SConst C)
+{
+  const DShell* sh = P.sh;
+  if (!bp_active(sh)) return;
+  const int ne = C.ne;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  const double cell = sh->cell_size, dmx = sh->d_max;
+  const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
+  int lo[3], hi[3];
+  for (int q = 0; q < 3; ++q) {
+    lo[q] = (int)((ebox(P, ne, q, e) - sh->bmin[q]) / cell);
+    hi[q] = (int)((ebox(P, ne, 3 + q, e) - sh->bmin[q] + dmx) / cell);
+    P.ecell[(size_t)q * ne + e] = lo[q];
+    P.ecell[(size_t)(3 + q) * ne + e] = hi[q];
   }
-  gsync(grid);
-  /* 7 */
-  const int ob = sh->pair_buf, nb = 1 - ob;
-  for (int e1 = gid; e1 < ne; e1 += gstride) {
+  for (int cx = lo[0]; cx <= hi[0]; ++cx)
+    for (int cy = lo[1]; cy <= hi[1]; ++cy)
+      for (int cz = lo[2]; cz <= hi[2]; ++cz) {
+        const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+        const unsigned b = bucket_of(c, P.hb_bits);
+        const unsigned pos = atomicAdd(&P.bcount[b], 1u);
+        if (pos < (unsigned)kBucketCap) P.bent[(size_t)b * kBucketCap + pos] = ((uint64_t)c << 32) | (uint32_t)e;
+        else atomicOr(&P.sh->err, ERR_ENTRIES);
+      }
+}
+
+/* exclusive scan of cnt[0..n) into out[0..n], out[n] = total, by one block */
+__device__ void block_scan_i32(const int32_t* cnt, int32_t* out, int n, int32_t* s)
+{
+  const int t = threadIdx.x, bs = blockDim.x;
+  const int chunk = (n + bs - 1) / bs;
+  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
+  int32_t sum = 0;
+  for (int i = lo; i < hi; ++i) sum += cnt[i];
+  s[t] = sum;
+  __syncthreads();
+  if (t == 0) {
+    int32_t acc = 0;
+    for (int q = 0; q < bs; ++q) { const int32_t x = s[q]; s[q] = acc; acc += x; }
+    out[n] = acc;
+  }
+  __syncthreads();
+  int32_t run = s[t];
+  for (int i = lo; i < hi; ++i) { out[i] = run; run += cnt[i]; }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode)
+{
+  __shared__ int32_t s_scan[256];
+  DShell* sh = P.sh;
+  if (!bp_active(sh)) return;
+  const int ne = C.ne;
+  const int e1 = blockIdx.x * 256 + threadIdx.x;
+  const bool predict = (posmode == M_STEP);
+  const double dt = sh->dt, a8 = sh->nm[8];
+  const int par = sh->par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* A = P.A[par];
+  const double dmx = sh->d_max, d2mx = sh->d2_max;
+  const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
+  const int ob = sh->pair_buf;
+  if (e1 < ne) {
     const int* L = P.ecell;
     const int lo1[3] = {L[e1], L[(size_t)ne + e1], L[(size_t)2 * ne + e1]};
     const int hi1[3] = {L[(size_t)3 * ne + e1], L[(size_t)4 * ne + e1], L[(size_t)5 * ne + e1]};
@@ -900,9 +1036,13 @@ __global__ void __launch_bounds__(256) k_broad(DPtrs P, SConst C, int posmode)
     for (int cx = lo1[0]; cx <= hi1[0]; ++cx)
       for (int cy = lo1[1]; cy <= hi1[1]; ++cy)
         for (int cz = lo1[2]; cz <= hi1[2]; ++cz) {
-          const size_t c = (size_t)cx * nd1 * nd2 + (size_t)cy * nd2 + cz;
-          for (uint32_t q = P.cell_start[c]; q < P.cell_start[c + 1]; ++q) {
-            const int e2 = P.cell_elems[q];
+          const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+          const unsigned b = bucket_of(c, P.hb_bits);
+          const unsigned cnt = min(P.bcount[b], (unsigned)kBucketCap);
+          for (unsigned q = 0; q < cnt; ++q) {
+            const uint64_t ent = P.bent[(size_t)b * kBucketCap + q];
+            if ((unsigned)(ent >> 32) != c) continue;
+            const int e2 = (int)(uint32_t)ent;
             if (e2 <= e1) continue;
             /* count the pair once: in the first cell the two ranges share */
             if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)ne + e2]) ||
@@ -935,18 +1075,23 @@ __global__ void __launch_bounds__(256) k_broad(DPtrs P, SConst C, int posmode)
           }
         }
     P.found_cnt[e1] = nsurv + min(nfound, kRowCap);
-    P.found_cnt[(size_t)ne + 1 + e1] = min(nfound, kRowCap); /* second half: number of new pairs */
+    P.found_cnt[(size_t)ne + 1 + e1] = min(nfound, kRowCap);
   }
-  gsync(grid);
-  /* 8 */
-  grid_scan(grid, (const uint32_t*)P.found_cnt, (uint32_t*)P.row_start[nb], ne, P.scan_tmp, s_scan);
-  const int total = P.row_start[nb][ne];
-  if (total > P.pair_cap) {
-    if (blockIdx.x == 0 && t == 0) { sh->err |= ERR_PAIRS; sh->rebuild = 0; }
-    return;
-  }
-  /* 9 */
-  for (int e1 = gid; e1 < ne; e1 += gstride) {
+  if (!last_block(P.ctr + 3)) return;
+  block_scan_i32(P.found_cnt, P.row_start[1 - ob], ne, s_scan);
+  if (threadIdx.x == 0 && P.row_start[1 - ob][ne] > P.pair_cap) atomicOr(&sh->err, ERR_PAIRS);
+}
+
+__global__ void __launch_bounds__(256) k_bp_emit(DPtrs P, SConst C)
+{
+  DShell* sh = P.sh;
+  if (!bp_active(sh)) return;
+  const int ne = C.ne;
+  const int e1 = blockIdx.x * 256 + threadIdx.x;
+  const int ob = sh->pair_buf, nb = 1 - ob;
+  const double d2mx = sh->d2_max;
+  const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
+  if (e1 < ne) {
     const int nf = P.found_cnt[(size_t)ne + 1 + e1];
     int32_t* fe = P.found_e2 + (size_t)e1 * kRowCap;
     double* fd = P.found_d2 + (size_t)e1 * kRowCap;
@@ -977,11 +1122,17 @@ __global__ void __launch_bounds__(256) k_broad(DPtrs P, SConst C, int posmode)
       }
       ++w;
     }
+    /* clear the buckets this element filled */
+    const int* L = P.ecell;
+    for (int cx = L[e1]; cx <= L[(size_t)3 * ne + e1]; ++cx)
+      for (int cy = L[(size_t)ne + e1]; cy <= L[(size_t)4 * ne + e1]; ++cy)
+        for (int cz = L[(size_t)2 * ne + e1]; cz <= L[(size_t)5 * ne + e1]; ++cz)
+          P.bcount[bucket_of((unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz, P.hb_bits)] = 0;
   }
-  gsync(grid);
-  if (blockIdx.x == 0 && t == 0) {
+  if (!last_block(P.ctr + 4)) return;
+  if (threadIdx.x == 0) {
     sh->pair_buf = nb;
-    sh->n_pairs = total;
+    sh->n_pairs = P.row_start[nb][ne];
     sh->rebuild = 0;
   }
 }

@@ -166,3 +166,31 @@ def test_contact_dynamics_adaptive(icosphere):
         assert relerr(ug, uo) < 1e-9
     assert seen_pairs > 0
     g.close()
+
+
+def test_deflation_trajectory_and_residual(icosphere):
+    """volume-controlled deflation (bulk modulus, volume of the force
+    configuration), post-buckling: states, rigid-free residual and the
+    convergence step agree with the oracle"""
+    say("case deflation L3")
+    X, T = icosphere(3)
+    X = imperfect(X, 0.01 * H, seed=2024)
+    kw = dict(bulk_modulus=0.05, volume_final_ratio=0.9, volume_ramp_time=60.0, dt=0.1175, dt_max=0.1175,
+              damping_alpha=0.05, volume_at_prediction=1, contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0)
+    g, o = pair(X, T, **kw)
+    z = np.zeros(3 * len(X))
+    g.set_state(z)
+    o.set_state(z)
+    g.apply_initial_conditions()
+    o.init_dynamics()
+    for k in range(4):
+        rg = g.advance(150)
+        ro = o.step(150)
+        ug = g.get_state()[0]
+        uo = o.get_state()[0]
+        say("  steps", rg["n_steps"], "rel u", relerr(ug, uo), "res", rg["residual"], ro["residual"], "resdef",
+            rg["residual_deform"], ro["residual_deform"], "V", rg["volume"], ro["volume"])
+        assert relerr(ug, uo) < 1e-9
+        assert abs(rg["residual_deform"] - ro["residual_deform"]) <= 1e-6 * ro["residual_deform"] + 1e-12
+        assert abs(rg["volume"] - ro["volume"]) <= 1e-12 * ro["volume"]
+    g.close()
