@@ -143,10 +143,11 @@ int launch_broad(shx_ctx* ctx, int posmode)
   const DPtrs& P = ctx->P;
   const SConst& C = ctx->C;
   const int nb = (C.ne + 255) / 256;
+  const int nbl = (int)(((long long)C.ne * kBpLanes + 255) / 256);
   prof_begin(ctx, 4);
   k_bp_box<<<nb, 256, 0, ctx->st>>>(P, C, posmode);
-  k_bp_bin<<<nb, 256, 0, ctx->st>>>(P, C);
-  k_bp_pairs<<<nb, 256, 0, ctx->st>>>(P, C, posmode);
+  k_bp_bin<<<nbl, 256, 0, ctx->st>>>(P, C);
+  k_bp_pairs<<<nbl, 256, 0, ctx->st>>>(P, C, posmode);
   k_bp_emit<<<nb, 256, 0, ctx->st>>>(P, C);
   prof_end(ctx);
   HIPCHK(hipGetLastError());

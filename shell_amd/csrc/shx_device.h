@@ -37,6 +37,7 @@ constexpr int kNodeLanes = 4;       /* lanes cooperating on one node */
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block */
 constexpr int kNodeRec = 22;        /* node-kernel partial record length */
 constexpr int kBucketCap = 64;      /* (cell, element) entries per hash bucket */
+constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
 
 /* per-shell constants (kernel argument) */
 struct SConst {

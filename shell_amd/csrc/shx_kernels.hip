@@ -958,12 +958,24 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
   if (ncell > C.n_cells_max) { sh->problem = max(sh->problem, 4); sh->rebuild = 0; }
 }
 
+/* (element, cell) lane mapping of the binning and pair kernels: kBpLanes
+ * lanes per element, lane k handles the element's cells k, k + kBpLanes, ...
+ * of its index box in cx-major order */
+__device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz, int& cx, int& cy, int& cz)
+{
+  cz = lo[2] + k % nz;
+  const int r = k / nz;
+  cy = lo[1] + r % ny;
+  cx = lo[0] + r / ny;
+}
+
 __global__ void __launch_bounds__(256) k_bp_bin(DPtrs P, SConst C)
 {
   const DShell* sh = P.sh;
   if (!bp_active(sh)) return;
   const int ne = C.ne;
-  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int e = gid / kBpLanes, k0 = gid % kBpLanes;
   if (e >= ne) return;
   const double cell = sh->cell_size, dmx = sh->d_max;
   const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
@@ -971,28 +983,34 @@ __global__ void __launch_bounds__(256) k_bp_bin(DPtrs P, SConst C)
   for (int q = 0; q < 3; ++q) {
     lo[q] = (int)((ebox(P, ne, q, e) - sh->bmin[q]) / cell);
     hi[q] = (int)((ebox(P, ne, 3 + q, e) - sh->bmin[q] + dmx) / cell);
-    P.ecell[(size_t)q * ne + e] = lo[q];
-    P.ecell[(size_t)(3 + q) * ne + e] = hi[q];
   }
-  for (int cx = lo[0]; cx <= hi[0]; ++cx)
-    for (int cy = lo[1]; cy <= hi[1]; ++cy)
-      for (int cz = lo[2]; cz <= hi[2]; ++cz) {
-        const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-        const unsigned b = bucket_of(c, P.hb_bits);
-        const unsigned pos = atomicAdd(&P.bcount[b], 1u);
-        if (pos < (unsigned)kBucketCap) P.bent[(size_t)b * kBucketCap + pos] = ((uint64_t)c << 32) | (uint32_t)e;
-        else atomicOr(&P.sh->err, ERR_ENTRIES);
-      }
+  if (k0 == 0)
+    for (int q = 0; q < 3; ++q) {
+      P.ecell[(size_t)q * ne + e] = lo[q];
+      P.ecell[(size_t)(3 + q) * ne + e] = hi[q];
+    }
+  const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  const int total = (hi[0] - lo[0] + 1) * ny * nz;
+  for (int k = k0; k < total; k += kBpLanes) {
+    int cx, cy, cz;
+    bp_cell_of(k, lo, ny, nz, cx, cy, cz);
+    const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+    const unsigned b = bucket_of(c, P.hb_bits);
+    const unsigned pos = atomicAdd(&P.bcount[b], 1u);
+    if (pos < (unsigned)kBucketCap) P.bent[(size_t)b * kBucketCap + pos] = ((uint64_t)c << 32) | (uint32_t)e;
+    else atomicOr(&P.sh->err, ERR_ENTRIES);
+  }
 }
 
-/* exclusive scan of cnt[0..n) into out[0..n], out[n] = total, by one block */
-__device__ void block_scan_i32(const int32_t* cnt, int32_t* out, int n, int32_t* s)
+/* exclusive scan of the row lengths surv[e] + min(nnew[e], kRowCap),
+ * e < n, into out[0..n], out[n] = total, by one block */
+__device__ void block_scan_rows(const int32_t* surv, const int32_t* nnew, int32_t* out, int n, int32_t* s)
 {
   const int t = threadIdx.x, bs = blockDim.x;
   const int chunk = (n + bs - 1) / bs;
   const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
   int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += cnt[i];
+  for (int i = lo; i < hi; ++i) sum += surv[i] + min(nnew[i], kRowCap);
   s[t] = sum;
   __syncthreads();
   if (t == 0) {
@@ -1002,17 +1020,23 @@ __device__ void block_scan_i32(const int32_t* cnt, int32_t* out, int n, int32_t*
   }
   __syncthreads();
   int32_t run = s[t];
-  for (int i = lo; i < hi; ++i) { out[i] = run; run += cnt[i]; }
+  for (int i = lo; i < hi; ++i) { out[i] = run; run += surv[i] + min(nnew[i], kRowCap); }
   __syncthreads();
 }
 
+/* lane (e1, k) walks the bucket entries of its cells; a pair e1 < e2 is
+ * tested only in the first cell both index boxes share, so each pair is
+ * tested by exactly one lane.  New pairs are appended to e1's found row
+ * with an atomic cursor (the row is sorted in k_bp_emit, so the list does
+ * not depend on the order of arrival). */
 __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode)
 {
   __shared__ int32_t s_scan[256];
   DShell* sh = P.sh;
   if (!bp_active(sh)) return;
   const int ne = C.ne;
-  const int e1 = blockIdx.x * 256 + threadIdx.x;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int e1 = gid / kBpLanes, k0 = gid % kBpLanes;
   const bool predict = (posmode == M_STEP);
   const double dt = sh->dt, a8 = sh->nm[8];
   const int par = sh->par;
@@ -1022,63 +1046,70 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode
   const double dmx = sh->d_max, d2mx = sh->d2_max;
   const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
   const int ob = sh->pair_buf;
+  int32_t* nnew = P.found_cnt + ne + 1;
   if (e1 < ne) {
     const int* L = P.ecell;
     const int lo1[3] = {L[e1], L[(size_t)ne + e1], L[(size_t)2 * ne + e1]};
     const int hi1[3] = {L[(size_t)3 * ne + e1], L[(size_t)4 * ne + e1], L[(size_t)5 * ne + e1]};
     const int r0 = P.row_start[ob][e1], r1 = P.row_start[ob][e1 + 1];
-    int nsurv = 0;
-    for (int r = r0; r < r1; ++r)
-      if (P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx)) ++nsurv;
+    if (k0 == 0) {
+      int nsurv = 0;
+      for (int r = r0; r < r1; ++r)
+        if (P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx)) ++nsurv;
+      P.found_cnt[e1] = nsurv;
+    }
+    const int ny = hi1[1] - lo1[1] + 1, nz = hi1[2] - lo1[2] + 1;
+    const int total = (hi1[0] - lo1[0] + 1) * ny * nz;
+    bool have_x0 = false;
     v3 X0[3];
-    for (int k = 0; k < 3; ++k) X0[k] = node_pos(P, P.tri[(size_t)k * ne + e1], U, V, A, predict, dt, a8);
-    int nfound = 0;
-    for (int cx = lo1[0]; cx <= hi1[0]; ++cx)
-      for (int cy = lo1[1]; cy <= hi1[1]; ++cy)
-        for (int cz = lo1[2]; cz <= hi1[2]; ++cz) {
-          const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-          const unsigned b = bucket_of(c, P.hb_bits);
-          const unsigned cnt = min(P.bcount[b], (unsigned)kBucketCap);
-          for (unsigned q = 0; q < cnt; ++q) {
-            const uint64_t ent = P.bent[(size_t)b * kBucketCap + q];
-            if ((unsigned)(ent >> 32) != c) continue;
-            const int e2 = (int)(uint32_t)ent;
-            if (e2 <= e1) continue;
-            /* count the pair once: in the first cell the two ranges share */
-            if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)ne + e2]) ||
-                cz != max(lo1[2], L[(size_t)2 * ne + e2]))
-              continue;
-            if (close_pair(P, e1, e2)) continue;
-            bool listed = false;
-            {
-              int lo = r0, hi = r1;
-              while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                const int x = P.pe2[ob][mid];
-                if (x == e2) { listed = P.pstick[ob][mid] || !(P.pd2[ob][mid] > d2mx); break; }
-                if (x < e2) lo = mid + 1; else hi = mid;
-              }
-            }
-            if (listed) continue;
-            if (boxes_apart(P, ne, e1, e2, dmx)) continue;
-            v3 X1[3];
-            for (int k = 0; k < 3; ++k) X1[k] = node_pos(P, P.tri[(size_t)k * ne + e2], U, V, A, predict, dt, a8);
-            tri_hit hit;
-            tri_tri(X0, X1, hit);
-            if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
-            if (hit.d2 > d2mx) continue;
-            if (nfound < kRowCap) {
-              P.found_e2[(size_t)e1 * kRowCap + nfound] = e2;
-              P.found_d2[(size_t)e1 * kRowCap + nfound] = hit.d2;
-            } else atomicOr(&sh->err, ERR_ROW);
-            ++nfound;
+    for (int k = k0; k < total; k += kBpLanes) {
+      int cx, cy, cz;
+      bp_cell_of(k, lo1, ny, nz, cx, cy, cz);
+      const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+      const unsigned b = bucket_of(c, P.hb_bits);
+      const unsigned cnt = min(P.bcount[b], (unsigned)kBucketCap);
+      for (unsigned q = 0; q < cnt; ++q) {
+        const uint64_t ent = P.bent[(size_t)b * kBucketCap + q];
+        if ((unsigned)(ent >> 32) != c) continue;
+        const int e2 = (int)(uint32_t)ent;
+        if (e2 <= e1) continue;
+        /* count the pair once: in the first cell the two ranges share */
+        if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)ne + e2]) ||
+            cz != max(lo1[2], L[(size_t)2 * ne + e2]))
+          continue;
+        if (close_pair(P, e1, e2)) continue;
+        bool listed = false;
+        {
+          int lo = r0, hi = r1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const int x = P.pe2[ob][mid];
+            if (x == e2) { listed = P.pstick[ob][mid] || !(P.pd2[ob][mid] > d2mx); break; }
+            if (x < e2) lo = mid + 1; else hi = mid;
           }
         }
-    P.found_cnt[e1] = nsurv + min(nfound, kRowCap);
-    P.found_cnt[(size_t)ne + 1 + e1] = min(nfound, kRowCap);
+        if (listed) continue;
+        if (boxes_apart(P, ne, e1, e2, dmx)) continue;
+        if (!have_x0) {
+          for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * ne + e1], U, V, A, predict, dt, a8);
+          have_x0 = true;
+        }
+        v3 X1[3];
+        for (int j = 0; j < 3; ++j) X1[j] = node_pos(P, P.tri[(size_t)j * ne + e2], U, V, A, predict, dt, a8);
+        tri_hit hit;
+        tri_tri(X0, X1, hit);
+        if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
+        if (hit.d2 > d2mx) continue;
+        const int slot = atomicAdd(&nnew[e1], 1);
+        if (slot < kRowCap) {
+          P.found_e2[(size_t)e1 * kRowCap + slot] = e2;
+          P.found_d2[(size_t)e1 * kRowCap + slot] = hit.d2;
+        } else atomicOr(&sh->err, ERR_ROW);
+      }
+    }
   }
   if (!last_block(P.ctr + 3)) return;
-  block_scan_i32(P.found_cnt, P.row_start[1 - ob], ne, s_scan);
+  block_scan_rows(P.found_cnt, nnew, P.row_start[1 - ob], ne, s_scan);
   if (threadIdx.x == 0 && P.row_start[1 - ob][ne] > P.pair_cap) atomicOr(&sh->err, ERR_PAIRS);
 }
 
@@ -1090,9 +1121,9 @@ __global__ void __launch_bounds__(256) k_bp_emit(DPtrs P, SConst C)
   const int e1 = blockIdx.x * 256 + threadIdx.x;
   const int ob = sh->pair_buf, nb = 1 - ob;
   const double d2mx = sh->d2_max;
-  const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
   if (e1 < ne) {
-    const int nf = P.found_cnt[(size_t)ne + 1 + e1];
+    const int nf = min(P.found_cnt[(size_t)ne + 1 + e1], kRowCap);
+    P.found_cnt[(size_t)ne + 1 + e1] = 0;
     int32_t* fe = P.found_e2 + (size_t)e1 * kRowCap;
     double* fd = P.found_d2 + (size_t)e1 * kRowCap;
     for (int a = 1; a < nf; ++a) { /* insertion sort by e2 */
@@ -1122,13 +1153,11 @@ __global__ void __launch_bounds__(256) k_bp_emit(DPtrs P, SConst C)
       }
       ++w;
     }
-    /* clear the buckets this element filled */
-    const int* L = P.ecell;
-    for (int cx = L[e1]; cx <= L[(size_t)3 * ne + e1]; ++cx)
-      for (int cy = L[(size_t)ne + e1]; cy <= L[(size_t)4 * ne + e1]; ++cy)
-        for (int cz = L[(size_t)2 * ne + e1]; cz <= L[(size_t)5 * ne + e1]; ++cz)
-          P.bcount[bucket_of((unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz, P.hb_bits)] = 0;
   }
+  /* clear every bucket count (a coalesced sweep; cheaper than revisiting
+   * each element's cells) */
+  const unsigned nbk = 1u << P.hb_bits;
+  for (unsigned b = blockIdx.x * 256 + threadIdx.x; b < nbk; b += gridDim.x * 256) P.bcount[b] = 0;
   if (!last_block(P.ctr + 4)) return;
   if (threadIdx.x == 0) {
     sh->pair_buf = nb;
