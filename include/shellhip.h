@@ -151,6 +151,34 @@ int shx_cfl_omega_max(shx_ctx* ctx, int iters, double* omega);
 int shx_set_profiling(shx_ctx* ctx, int enable);
 int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 5 */, int64_t* counts /* 5 */);
 
+/* Batches of independent shells (no counterpart in the reference, which runs
+ * one ShellSystem per process; the batch replaces running many of them side
+ * by side).  Shell s has nv[s] nodes and ne[s] triangles; X and tri hold the
+ * shells' arrays one after another (tri with per-shell vertex ids); prm[s]
+ * its parameters, including its own dt.  Every step kernel runs once per
+ * phase over the whole batch; each shell keeps its own time, time step,
+ * rejections, contact lists and convergence.  The single-shell entry points
+ * above act on shell 0 of a batch context (shx_init_dynamics and shx_step
+ * advance every shell).  shx_create(...) == shx_batch_create(1, ...). */
+int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, const double* X,
+                     const int32_t* tri, const shx_params* prm, int device);
+int shx_batch_size(const shx_ctx* ctx);
+int shx_batch_set_state(shx_ctx* ctx, int shell, const double* u, const double* v, const double* a, double t);
+int shx_batch_get_state(shx_ctx* ctx, int shell, double* u, double* v, double* a, double* t);
+/* apply_initial_conditions (ShellSystem.C:855-905) for every shell */
+int shx_batch_init_dynamics(shx_ctx* ctx, int precision);
+/* n_steps accepted steps of every shell (ShellSystem::advance, ShellSystem.C:
+ * 952-1122), or until the shell converges if stop_on_converged; reps[nshell] */
+int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged,
+                   shx_report* reps);
+int shx_batch_reports(shx_ctx* ctx, shx_report* reps);
+/* sorted candidate list of one shell, element ids local to the shell */
+int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap);
+int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega /* nshell */);
+/* 1 (default): step attempts are replayed from a captured hipGraph of 64
+ * attempts; 0: launched one by one.  Profiling launches one by one. */
+int shx_set_graphs(shx_ctx* ctx, int enable);
+
 /* Primitives exposed for parity tests (host-compiled copies of the device
  * routines; no GPU needed). */
 void shx_tri_distance(const double* p /* 18 */, double* out /* d2, dc[3], v[2], w[2] */);

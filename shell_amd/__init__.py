@@ -6,8 +6,10 @@ all computation runs in libshellhip.so (HIP kernels).  `ShellSystem` mirrors
 the reference's public/protected interface of class ShellSystem
 (ShellSystem.h:264-306): advance(), compute_shell_shell_contact_candidates(),
 and a force evaluation equal to compute_deformed_metrics +
-compute_external_force + subtract_internal_force.  The library must be built
-(python shell_amd/build.py); there is no fallback.
+compute_external_force + subtract_internal_force.  `ShellBatch` runs many
+independent shells (own mesh, parameters and time step each) through the
+same kernels, one launch per phase for the whole batch.  The library must be
+built (python shell_amd/build.py); there is no fallback.
 """
 import ctypes
 import os
@@ -26,7 +28,10 @@ PROBLEMS = ["NO_PROBLEM", "NO_DISP_CHANGE", "LARGE_DISP", "STOP_SIM", "TOO_MANY_
 EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_get_state",
            "shx_forces", "shx_init_dynamics", "shx_step", "shx_contact_rebuild", "shx_pairs",
            "shx_elem_tables", "shx_mass", "shx_patches", "shx_cfl_omega_max", "shx_set_profiling",
-           "shx_kernel_times", "shx_tri_distance", "shx_raw_elem_basis"]
+           "shx_kernel_times", "shx_tri_distance", "shx_raw_elem_basis", "shx_batch_create",
+           "shx_batch_size", "shx_batch_set_state", "shx_batch_get_state", "shx_batch_init_dynamics",
+           "shx_batch_step", "shx_batch_reports", "shx_batch_pairs", "shx_batch_cfl_omega_max",
+           "shx_set_graphs"]
 
 _lib = None
 
@@ -95,6 +100,18 @@ def lib():
         L.shx_kernel_times.argtypes = [_V, _D, ctypes.POINTER(ctypes.c_int64)]
         L.shx_tri_distance.argtypes = [_D, _D]
         L.shx_raw_elem_basis.argtypes = [ctypes.c_int, _D]
+        L.shx_batch_create.argtypes = [ctypes.POINTER(_V), ctypes.c_int, _I, _I, _D, _I,
+                                       ctypes.POINTER(Params), ctypes.c_int]
+        L.shx_batch_size.argtypes = [_V]
+        L.shx_batch_set_state.argtypes = [_V, ctypes.c_int, _D, _D, _D, ctypes.c_double]
+        L.shx_batch_get_state.argtypes = [_V, ctypes.c_int, _D, _D, _D, _D]
+        L.shx_batch_init_dynamics.argtypes = [_V, ctypes.c_int]
+        L.shx_batch_step.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(Report)]
+        L.shx_batch_reports.argtypes = [_V, ctypes.POINTER(Report)]
+        L.shx_batch_pairs.argtypes = [_V, ctypes.c_int, _U, _U, ctypes.c_int]
+        L.shx_batch_cfl_omega_max.argtypes = [_V, ctypes.c_int, _D]
+        L.shx_set_graphs.argtypes = [_V, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -231,6 +248,9 @@ class ShellSystem:
     def set_profiling(self, enable=True):
         self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
 
+    def set_graphs(self, enable=True):
+        self._check(lib().shx_set_graphs(self.h, 1 if enable else 0), "shx_set_graphs")
+
     def kernel_times(self):
         ms = np.zeros(5)
         cnt = np.zeros(5, dtype=np.int64)
@@ -252,3 +272,99 @@ def raw_elem_basis(valence):
     if lib().shx_raw_elem_basis(valence, _dp(out)) != 0:
         raise ValueError(valence)
     return out.reshape(6, valence + 6)
+
+
+def _report_dict(rep):
+    return {k: getattr(rep, k) for k, _ in Report._fields_}
+
+
+class ShellBatch:
+    """A batch of independent shells on one GPU: every phase of a step is one
+    kernel launch over all shells (the per-shell step of
+    ShellSystem::advance, ShellSystem.C:952-1122, run side by side; each
+    shell keeps its own time step, step rejection and contact lists).
+
+    meshes: list of (X, T); params: one dict per shell (or one dict for all).
+    """
+
+    def __init__(self, meshes, params, device=0):
+        S = len(meshes)
+        if isinstance(params, dict):
+            params = [params] * S
+        if len(params) != S:
+            raise ValueError("one parameter set per shell")
+        self.meshes = [(np.ascontiguousarray(X, dtype=np.float64), np.ascontiguousarray(T, dtype=np.int32))
+                       for X, T in meshes]
+        self.S = S
+        self.nv = np.array([len(X) for X, _ in self.meshes], dtype=np.int32)
+        self.ne = np.array([len(T) for _, T in self.meshes], dtype=np.int32)
+        Xc = np.ascontiguousarray(np.concatenate([X.reshape(-1) for X, _ in self.meshes]))
+        Tc = np.ascontiguousarray(np.concatenate([T.reshape(-1) for _, T in self.meshes]))
+        self.params = (Params * S)(*[make_params(**p) for p in params])
+        h = _V()
+        r = lib().shx_batch_create(ctypes.byref(h), S, self.nv.ctypes.data_as(_I), self.ne.ctypes.data_as(_I),
+                                   _dp(Xc), Tc.ctypes.data_as(_I), self.params, device)
+        if r != 0:
+            raise ShellError(f"shx_batch_create: {r}: {lib().shx_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, r, what):
+        if r < 0:
+            raise ShellError(f"{what}: {r}: {lib().shx_last_error(self.h).decode()}")
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, shell, u, v=None, a=None, t=0.0):
+        n = 3 * int(self.nv[shell])
+        u = _vec(u, n)
+        v = None if v is None else _vec(v, n)
+        a = None if a is None else _vec(a, n)
+        self._check(lib().shx_batch_set_state(self.h, shell, _dp(u), _dp(v), _dp(a), t), "shx_batch_set_state")
+
+    def get_state(self, shell):
+        n = 3 * int(self.nv[shell])
+        u, v, a = (np.zeros(n) for _ in range(3))
+        t = ctypes.c_double()
+        self._check(lib().shx_batch_get_state(self.h, shell, _dp(u), _dp(v), _dp(a), ctypes.byref(t)),
+                    "shx_batch_get_state")
+        return u, v, a, t.value
+
+    def apply_initial_conditions(self, precision=FP64):
+        self._check(lib().shx_batch_init_dynamics(self.h, precision), "shx_batch_init_dynamics")
+
+    def advance(self, n_steps=1, tol=0.0, stop_on_converged=False, precision=FP64):
+        reps = (Report * self.S)()
+        self._check(lib().shx_batch_step(self.h, precision, n_steps, tol, 1 if stop_on_converged else 0, reps),
+                    "shx_batch_step")
+        return [_report_dict(r) for r in reps]
+
+    def reports(self):
+        reps = (Report * self.S)()
+        self._check(lib().shx_batch_reports(self.h, reps), "shx_batch_reports")
+        return [_report_dict(r) for r in reps]
+
+    def pairs(self, shell):
+        n = self._check(lib().shx_batch_pairs(self.h, shell, None, None, 0), "shx_batch_pairs")
+        e1 = np.zeros(max(n, 1), dtype=np.uint32)
+        e2 = np.zeros(max(n, 1), dtype=np.uint32)
+        self._check(lib().shx_batch_pairs(self.h, shell, e1.ctypes.data_as(_U), e2.ctypes.data_as(_U), n),
+                    "shx_batch_pairs")
+        return e1[:n], e2[:n]
+
+    def cfl_omega_max(self, iters=200):
+        w = np.zeros(self.S)
+        self._check(lib().shx_batch_cfl_omega_max(self.h, iters, _dp(w)), "shx_batch_cfl_omega_max")
+        return w
+
+    def set_graphs(self, enable=True):
+        self._check(lib().shx_set_graphs(self.h, 1 if enable else 0), "shx_set_graphs")

@@ -1,7 +1,9 @@
 /*
  * shx_api.hip -- host side of libshellhip: the extern "C" entry points of
- * include/shellhip.h, device memory ownership and the launch sequence of one
- * explicit step attempt.
+ * include/shellhip.h, placement of a batch of shells in one global
+ * numbering, device memory ownership, and the launch sequence of one
+ * explicit step attempt (captured once into a hipGraph of kGraphAttempts
+ * attempts and replayed).
  */
 #include <hip/hip_runtime.h>
 
@@ -20,22 +22,32 @@
 
 using namespace shx;
 
+namespace {
+constexpr int kGraphAttempts = 64;   /* step attempts per replayed graph */
+constexpr int kMaxContactBlocks = 120;
+}
+
 struct shx_ctx {
   int dev = 0;
   hipStream_t st = nullptr;
-  Topology T;
-  shx_params prm{};
-  SConst C{};
+  int S = 0;
+  std::vector<Topology> T;
+  std::vector<shx_params> prm;
+  std::vector<SConst> C;       /* host copy of the device array */
   DPtrs P{};
+  SConst* dC = nullptr;
+  int nblkE = 0, nblkN = 0, nblkC = 0, ntile = 0, nblkL = 0;
+  bool any_contact = false;
   std::vector<void*> allocs;
   std::vector<double> rawb;
-  int nblkE = 0, nblkN = 0, nblkC = 0;
   std::string err;
   bool prof = false;
   std::vector<hipEvent_t> evpool;
   std::vector<int> evclass;
   double tsum[5] = {0, 0, 0, 0, 0};
   int64_t tcnt[5] = {0, 0, 0, 0, 0};
+  hipGraphExec_t graph = nullptr;
+  bool use_graph = true;
 };
 
 namespace {
@@ -70,6 +82,8 @@ int upload(shx_ctx* ctx, T*& p, const std::vector<T>& h)
   return 0;
 }
 
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
 void set_newmark_host(DShell& s, const SConst& C, double dt)
 {
   dt = std::max(dt, C.dt_min);
@@ -88,16 +102,24 @@ void set_newmark_host(DShell& s, const SConst& C, double dt)
   s.nm[9] = dt * dt * std::fabs(b - 1. / 6);
 }
 
-int read_sh(shx_ctx* ctx, DShell& s)
+int read_sh(shx_ctx* ctx, std::vector<DShell>& s)
 {
-  HIPCHK(hipMemcpyAsync(&s, ctx->P.sh, sizeof(DShell), hipMemcpyDeviceToHost, ctx->st));
+  s.resize(ctx->S);
+  HIPCHK(hipMemcpyAsync(s.data(), ctx->P.sh, sizeof(DShell) * ctx->S, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
 }
 
-int write_sh(shx_ctx* ctx, const DShell& s)
+int write_sh(shx_ctx* ctx, const std::vector<DShell>& s)
 {
-  HIPCHK(hipMemcpyAsync(ctx->P.sh, &s, sizeof(DShell), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->P.sh, s.data(), sizeof(DShell) * ctx->S, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int write_consts(shx_ctx* ctx, const std::vector<SConst>& C)
+{
+  HIPCHK(hipMemcpyAsync(ctx->dC, C.data(), sizeof(SConst) * ctx->S, hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
 }
@@ -136,55 +158,70 @@ void prof_collect(shx_ctx* ctx)
   ctx->evclass.clear();
 }
 
-/* the broad phase: four kernels that return at once unless the device-side
- * trigger (or a forced rebuild) set DShell::rebuild */
-int launch_broad(shx_ctx* ctx, int posmode)
+/* the broad phase: four kernels whose blocks return at once unless the
+ * device-side trigger (or a forced rebuild) set DShell::rebuild of their
+ * shell */
+void launch_broad(shx_ctx* ctx, int posmode)
 {
   const DPtrs& P = ctx->P;
-  const SConst& C = ctx->C;
-  const int nb = (C.ne + 255) / 256;
-  const int nbl = (int)(((long long)C.ne * kBpLanes + 255) / 256);
   prof_begin(ctx, 4);
-  k_bp_box<<<nb, 256, 0, ctx->st>>>(P, C, posmode);
-  k_bp_bin<<<nbl, 256, 0, ctx->st>>>(P, C);
-  k_bp_pairs<<<nbl, 256, 0, ctx->st>>>(P, C, posmode);
-  k_bp_emit<<<nb, 256, 0, ctx->st>>>(P, C);
+  k_bp_box<<<ctx->ntile, 256, 0, ctx->st>>>(P, posmode);
+  k_bp_bin<<<ctx->nblkL, 256, 0, ctx->st>>>(P);
+  k_bp_pairs<<<ctx->nblkL, 256, 0, ctx->st>>>(P, posmode);
+  k_bp_emit<<<ctx->ntile, 256, 0, ctx->st>>>(P);
   prof_end(ctx);
-  HIPCHK(hipGetLastError());
-  return 0;
 }
 
-/* one step attempt: element force (+ predictor trigger), broad phase,
- * narrow phase, node gather + Newmark (+ finalize), corrector trigger */
-int enqueue_attempt(shx_ctx* ctx)
+/* one step attempt of every shell that is not done: element force
+ * (+ predictor trigger), broad phase, narrow phase, node gather + Newmark
+ * (+ finalize), corrector-time broad phase */
+void enqueue_attempt(shx_ctx* ctx)
 {
   const DPtrs& P = ctx->P;
-  const SConst& C = ctx->C;
   hipStream_t st = ctx->st;
   prof_begin(ctx, 0);
-  k_elem<<<ctx->nblkE, kElemBlock, 0, st>>>(P, C, M_STEP);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, st>>>(P, M_STEP);
   prof_end(ctx);
-  if (C.contact) {
-    if (int r = launch_broad(ctx, M_STEP)) return r;
+  if (ctx->any_contact) {
+    launch_broad(ctx, M_STEP);
     prof_begin(ctx, 2);
-    k_contact<<<ctx->nblkC, 256, 0, st>>>(P, C, M_STEP);
+    k_contact<<<ctx->nblkC, 256, 0, st>>>(P, M_STEP);
     prof_end(ctx);
   }
   prof_begin(ctx, 1);
-  k_node<<<ctx->nblkN, kNodeBlock, 0, st>>>(P, C, M_STEP, 0.0, C.contact ? ctx->nblkC : 0);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, st>>>(P, M_STEP);
   prof_end(ctx);
-  if (C.contact)
-    if (int r = launch_broad(ctx, M_FORCES)) return r;
-  HIPCHK(hipGetLastError());
+  if (ctx->any_contact) launch_broad(ctx, M_FORCES);
+}
+
+/* kGraphAttempts attempts captured once and replayed */
+int launch_attempts_graph(shx_ctx* ctx)
+{
+  if (!ctx->graph) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphAttempts; ++k) enqueue_attempt(ctx);
+    HIPCHK(hipStreamEndCapture(ctx->st, &g));
+    const hipError_t e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      ctx->graph = nullptr;
+      ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+      return SHX_E_HIP;
+    }
+  }
+  HIPCHK(hipGraphLaunch(ctx->graph, ctx->st));
   return 0;
 }
 
-int check_err(shx_ctx* ctx, const DShell& s)
+int check_err(shx_ctx* ctx, const std::vector<DShell>& s)
 {
-  if (s.err) {
-    ctx->err = "device list capacity exceeded (bits " + std::to_string(s.err) + ")";
-    return SHX_E_CAPACITY;
-  }
+  for (int i = 0; i < ctx->S; ++i)
+    if (s[i].err) {
+      ctx->err = "device list capacity exceeded in shell " + std::to_string(i) + " (bits " +
+                 std::to_string(s[i].err) + ")";
+      return SHX_E_CAPACITY;
+    }
   return 0;
 }
 
@@ -201,45 +238,11 @@ void fill_report(const DShell& s, shx_report* rep)
   rep->residual_deform = s.residual_deform;
 }
 
-} // namespace
-
-extern "C" {
-
-const char* shx_last_error(const shx_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
-
-int shx_destroy(shx_ctx* ctx)
+/* SConst of one shell from its parameters (update_derived,
+ * ShellSystem.C:421-487); placement fields are filled by the caller */
+SConst shell_consts(const shx_params* prm, int nv, int ne)
 {
-  if (!ctx) return SHX_OK;
-  (void)hipSetDevice(ctx->dev);
-  prof_collect(ctx);
-  for (void* p : ctx->allocs) (void)hipFree(p);
-  if (ctx->st) (void)hipStreamDestroy(ctx->st);
-  delete ctx;
-  return SHX_OK;
-}
-
-int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tri, const shx_params* prm,
-               int device)
-{
-  if (!out || nv <= 0 || ne <= 0 || !X || !tri || !prm) { g_create_err = "invalid argument"; return SHX_E_ARG; }
-  *out = nullptr;
-  if (prm->thickness <= 0 || prm->E1 <= 0 || prm->E2 <= 0 || prm->G12 <= 0 || prm->rho <= 0 ||
-      prm->nu < 0 || prm->nu >= 1 || prm->dt <= 0 || prm->newmark_beta < 0) {
-    g_create_err = "invalid parameters (ShellSystem.C:326-332 asserts)";
-    return SHX_E_ARG;
-  }
-  shx_ctx* ctx = new shx_ctx();
-  ctx->dev = device;
-  ctx->prm = *prm;
-  auto fail = [&](int code) { g_create_err = ctx->err; shx_destroy(ctx); return code; };
-  if (hipSetDevice(device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return fail(SHX_E_HIP); }
-  if (hipStreamCreate(&ctx->st) != hipSuccess) { ctx->err = "hipStreamCreate failed"; return fail(SHX_E_HIP); }
-  std::string terr;
-  if (!build_topology(nv, ne, tri, ctx->T, terr)) { ctx->err = terr; return fail(SHX_E_MESH); }
-  const Topology& T = ctx->T;
-
-  /* constants: update_derived (ShellSystem.C:421-487) */
-  SConst& C = ctx->C;
+  SConst C{};
   C.nv = nv; C.ne = ne;
   C.h = prm->thickness; C.rho = prm->rho;
   C.E1 = prm->E1; C.E2 = prm->E2; C.nu = prm->nu; C.G12 = prm->G12;
@@ -263,6 +266,8 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
   C.dt_min = std::max(prm->dt_min, DBL_EPSILON);
   C.dt_max = prm->dt_max > 0 ? prm->dt_max : DBL_MAX;
   C.rel_loc_err_min = prm->rel_loc_err_min; C.rel_loc_err_max = prm->rel_loc_err_max;
+  C.damping_alpha = prm->damping_alpha; C.damping_beta = prm->damping_beta;
+  C.damping_gamma = prm->damping_gamma;
   C.adapt_dt_every = prm->adapt_dt_every;
   C.contact = prm->contact ? 1 : 0;
   C.self_friction = (prm->mu_s != 0 || prm->mu_d != 0) ? 1 : 0;
@@ -274,118 +279,299 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
     unsigned m = (unsigned)std::ceil(std::sqrt((double)ne) / prm->cell_size_mult);
     C.n_cells_max = m * m * m;
   }
+  return C;
+}
+
+bool params_ok(const shx_params* prm)
+{
+  return !(prm->thickness <= 0 || prm->E1 <= 0 || prm->E2 <= 0 || prm->G12 <= 0 || prm->rho <= 0 ||
+           prm->nu < 0 || prm->nu >= 1 || prm->dt <= 0 || prm->newmark_beta < 0 ||
+           (prm->contact && prm->cell_size_mult <= 0));
+}
+
+int check_shell(shx_ctx* ctx, int shell)
+{
+  if (shell < 0 || shell >= ctx->S) { ctx->err = "shell index out of range"; return SHX_E_ARG; }
+  return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* shx_last_error(const shx_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int shx_destroy(shx_ctx* ctx)
+{
+  if (!ctx) return SHX_OK;
+  (void)hipSetDevice(ctx->dev);
+  prof_collect(ctx);
+  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+  for (void* p : ctx->allocs) (void)hipFree(p);
+  if (ctx->st) (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+  return SHX_OK;
+}
+
+int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, const double* X,
+                     const int32_t* tri, const shx_params* prm, int device)
+{
+  if (!out || nshell <= 0 || !nv || !ne || !X || !tri || !prm) { g_create_err = "invalid argument"; return SHX_E_ARG; }
+  *out = nullptr;
+  for (int s = 0; s < nshell; ++s) {
+    if (nv[s] <= 0 || ne[s] <= 0) { g_create_err = "invalid mesh size of shell " + std::to_string(s); return SHX_E_ARG; }
+    if (!params_ok(&prm[s])) {
+      g_create_err = "invalid parameters of shell " + std::to_string(s) + " (ShellSystem.C:326-332 asserts)";
+      return SHX_E_ARG;
+    }
+  }
+  shx_ctx* ctx = new shx_ctx();
+  ctx->dev = device;
+  ctx->S = nshell;
+  ctx->prm.assign(prm, prm + nshell);
+  auto fail = [&](int code) { g_create_err = ctx->err; shx_destroy(ctx); return code; };
+  if (hipSetDevice(device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return fail(SHX_E_HIP); }
+  if (hipStreamCreate(&ctx->st) != hipSuccess) { ctx->err = "hipStreamCreate failed"; return fail(SHX_E_HIP); }
+
+  /* topology per shell, placement in the batch */
+  ctx->T.resize(nshell);
+  ctx->C.resize(nshell);
+  std::vector<size_t> xoff(nshell + 1, 0), toff(nshell + 1, 0);
+  for (int s = 0; s < nshell; ++s) {
+    xoff[s + 1] = xoff[s] + (size_t)3 * nv[s];
+    toff[s + 1] = toff[s] + (size_t)3 * ne[s];
+  }
+  long long vb = 0, eb = 0, pb = 0;
+  int cb = 0;
+  unsigned long long bb = 0;
+  for (int s = 0; s < nshell; ++s) {
+    std::string terr;
+    if (!build_topology(nv[s], ne[s], tri + toff[s], ctx->T[s], terr)) {
+      ctx->err = "shell " + std::to_string(s) + ": " + terr;
+      return fail(SHX_E_MESH);
+    }
+    SConst C = shell_consts(&prm[s], nv[s], ne[s]);
+    const int nvp = round_up(nv[s], kAlign), nep = round_up(ne[s], kAlign);
+    C.vbase = (int)vb; C.ebase = (int)eb;
+    C.blkE = (int)(eb / kElemPerBlock); C.nbE = nep / kElemPerBlock;
+    C.blkN = (int)(vb / kNodePerBlock); C.nbN = nvp / kNodePerBlock;
+    C.tile0 = (int)(eb / 256); C.ntile = nep / 256;
+    C.nbL = nep / kBpPerBlock;
+    if (C.contact) {
+      C.blkC = cb;
+      C.nbC = std::max(1, std::min(kMaxContactBlocks, ne[s] / 64));
+      cb += C.nbC;
+      /* hashed cell grid: about four entries per bucket for the expected
+       * number of (cell, element) entries, (1/cell_size_mult + 2)^3 each */
+      const double kc = std::pow(1.0 / prm[s].cell_size_mult + 2.0, 3.0);
+      const double want = std::max(4096.0, (double)ne[s] * kc / 4.0);
+      unsigned bits = 12;
+      while ((double)(1u << bits) < want && bits < 24) ++bits;
+      C.hb_bits = bits;
+      C.bbase = (unsigned)bb;
+      bb += 1ull << bits;
+      C.pbase = (int)pb;
+      C.pair_cap = 16 * ne[s] + 4096;
+      pb += C.pair_cap;
+    }
+    ctx->C[s] = C;
+    vb += nvp;
+    eb += nep;
+    if (vb > (1ll << 26) || eb > (1ll << 26) || pb > (1ll << 30) || bb > (1ull << 31)) {
+      ctx->err = "batch too large for 32-bit global indices";
+      return fail(SHX_E_ARG);
+    }
+  }
+  const int nV = (int)vb, nE = (int)eb;
+  ctx->nblkE = nE / kElemPerBlock;
+  ctx->nblkN = nV / kNodePerBlock;
+  ctx->ntile = nE / 256;
+  ctx->nblkL = nE / kBpPerBlock;
+  ctx->nblkC = cb;
+  ctx->any_contact = cb > 0;
 
   /* Loop basis at the quadrature point, valences 3..6 */
   ctx->rawb.assign(4 * 72, 0.0);
   for (int val = 3; val <= 6; ++val) loop_basis_at_centroid(val, &ctx->rawb[(val - 3) * 72]);
 
-  /* structure-of-arrays copies of the topology */
-  std::vector<int32_t> pidx((size_t)12 * ne), triS((size_t)3 * ne);
-  std::vector<int8_t> vcls(ne);
-  for (int e = 0; e < ne; ++e) {
-    for (int i = 0; i < 12; ++i) {
-      const int n = T.patch[(size_t)e * 12 + i];
-      pidx[(size_t)i * ne + e] = n >= 0 ? n : T.patch[(size_t)e * 12]; /* padded slot: basis is zero */
+  /* global structure-of-arrays copies of the topology */
+  std::vector<int32_t> pidx((size_t)12 * nE), triS((size_t)3 * nE), eshell(nE), vshell(nV),
+      cshell(std::max(cb, 1), 0);
+  std::vector<int8_t> vcls(nE, 6);
+  std::vector<double> Xh((size_t)3 * nV, 0.0);
+  std::vector<int32_t> inc_start((size_t)nV + 1, 0), inc;
+  std::vector<int32_t> cstart((size_t)nE + 1, 0), cids;
+  for (int s = 0; s < nshell; ++s) {
+    const SConst& C = ctx->C[s];
+    const Topology& T = ctx->T[s];
+    const int nep = C.ntile * 256, nvp = C.nbN * kNodePerBlock;
+    for (int q = 0; q < nep; ++q) {
+      const int e = C.ebase + q;
+      eshell[e] = s;
+      for (int i = 0; i < 12; ++i) {
+        int n = C.vbase;
+        if (q < ne[s]) {
+          const int ln = T.patch[(size_t)q * 12 + i];
+          n = C.vbase + (ln >= 0 ? ln : T.patch[(size_t)q * 12]); /* padded slot: basis is zero */
+        }
+        pidx[(size_t)i * nE + e] = n;
+      }
+      for (int k = 0; k < 3; ++k) triS[(size_t)k * nE + e] = C.vbase + (q < ne[s] ? T.tri[(size_t)3 * q + k] : 0);
+      if (q < ne[s]) vcls[e] = (int8_t)T.valence[q];
     }
-    for (int k = 0; k < 3; ++k) triS[(size_t)k * ne + e] = T.tri[(size_t)3 * e + k];
-    vcls[e] = (int8_t)T.valence[e];
+    for (int q = 0; q < nvp; ++q) vshell[C.vbase + q] = s;
+    for (int q = 0; q < 3 * nv[s]; ++q) Xh[(size_t)3 * C.vbase + q] = X[xoff[s] + q];
+    for (int c = 0; c < C.nbC; ++c) cshell[C.blkC + c] = s;
   }
-  std::vector<double> Xh(X, X + (size_t)3 * nv);
+  /* node incidence and contact exclusion as global CSR lists */
+  for (int s = 0; s < nshell; ++s) {
+    const SConst& C = ctx->C[s];
+    const Topology& T = ctx->T[s];
+    for (int q = 0; q < nv[s]; ++q) {
+      const int n = C.vbase + q;
+      inc_start[n] = (int)inc.size();
+      for (int k = T.inc_start[q]; k < T.inc_start[q + 1]; ++k) {
+        const int ent = T.inc[k];
+        inc.push_back(((C.ebase + (ent >> 4)) << 4) | (ent & 15));
+      }
+    }
+    for (int q = nv[s]; q < C.nbN * kNodePerBlock; ++q) inc_start[C.vbase + q] = (int)inc.size();
+  }
+  inc_start[nV] = (int)inc.size();
+  for (int s = 0; s < nshell; ++s) {
+    const SConst& C = ctx->C[s];
+    std::vector<int32_t> cs, ci;
+    if (C.contact)
+      build_close_sets(ctx->T[s], X + xoff[s], (1 + prm[s].neighbor_margin) * prm[s].thickness, cs, ci);
+    for (int q = 0; q < C.ntile * 256; ++q) {
+      const int e = C.ebase + q;
+      cstart[e] = (int)cids.size();
+      if (C.contact && q < ne[s])
+        for (int k = cs[q]; k < cs[q + 1]; ++k) cids.push_back(C.ebase + ci[k]);
+    }
+  }
+  cstart[nE] = (int)cids.size();
+
   DPtrs& P = ctx->P;
-  ctx->nblkE = (ne + kElemBlock / kElemLanes - 1) / (kElemBlock / kElemLanes);
-  ctx->nblkN = (nv + kNodeBlock / kNodeLanes - 1) / (kNodeBlock / kNodeLanes);
-  ctx->nblkC = 120;
+  P.nE = nE; P.nV = nV;
   int r = 0;
-  double* dX;
-  int32_t *dp, *dt3, *dis, *di;
+  double *dX, *drb;
+  int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcsh, *dcs, *dci;
   int8_t* dv;
-  double* drb;
   if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
-      (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, T.inc_start)) ||
-      (r = upload(ctx, di, T.inc)))
+      (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, inc_start)) ||
+      (r = upload(ctx, di, inc)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell)) ||
+      (r = upload(ctx, dcsh, cshell)) || (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)))
     return fail(r);
   P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
-  if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * ne)) || (r = dalloc(ctx, P.minv, nv)) ||
-      (r = dalloc(ctx, P.cdamp, nv)) || (r = dalloc(ctx, P.qe, (size_t)36 * ne)) || (r = dalloc(ctx, P.g0, (size_t)3 * ne)) || (r = dalloc(ctx, P.area, ne)) ||
-      (r = dalloc(ctx, P.ctr, 8)) ||
-      (r = dalloc(ctx, P.fc, (size_t)3 * nv)) || (r = dalloc(ctx, P.fout, (size_t)3 * nv)) ||
-      (r = dalloc(ctx, P.fext_out, (size_t)3 * nv)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
-      (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) || (r = dalloc(ctx, P.cpart, (size_t)3 * ctx->nblkC)) ||
-      (r = dalloc(ctx, P.sh, 1)))
+  P.eshell = des; P.vshell = dvs; P.cshell = dcsh; P.close_start = dcs; P.close_ids = dci;
+  if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
+  P.cs = ctx->dC;
+  if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * nE)) || (r = dalloc(ctx, P.minv, nV)) ||
+      (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.qe, (size_t)36 * nE)) ||
+      (r = dalloc(ctx, P.g0, (size_t)3 * nE)) || (r = dalloc(ctx, P.area, nE)) ||
+      (r = dalloc(ctx, P.ctr, (size_t)kCtrPerShell * nshell)) ||
+      (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
+      (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
+      (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
+      (r = dalloc(ctx, P.cpart, (size_t)3 * std::max(cb, 1))) || (r = dalloc(ctx, P.sh, nshell)))
     return fail(r);
   for (int b = 0; b < 2; ++b)
-    if ((r = dalloc(ctx, P.U[b], (size_t)3 * nv)) || (r = dalloc(ctx, P.V[b], (size_t)3 * nv)) ||
-        (r = dalloc(ctx, P.A[b], (size_t)3 * nv)))
+    if ((r = dalloc(ctx, P.U[b], (size_t)3 * nV)) || (r = dalloc(ctx, P.V[b], (size_t)3 * nV)) ||
+        (r = dalloc(ctx, P.A[b], (size_t)3 * nV)))
       return fail(r);
-
-  if (C.contact) {
-    std::vector<int32_t> cs, ci;
-    build_close_sets(T, X, (1 + prm->neighbor_margin) * prm->thickness, cs, ci);
-    int32_t *dcs, *dci;
-    if ((r = upload(ctx, dcs, cs)) || (r = upload(ctx, dci, ci))) return fail(r);
-    P.close_start = dcs; P.close_ids = dci;
-    /* hashed cell grid: about four entries per bucket for the expected
-     * number of (cell, element) entries, (1/cell_size_mult + 2)^3 each */
-    const double kc = std::pow(1.0 / prm->cell_size_mult + 2.0, 3.0);
-    const double want = std::max(4096.0, (double)ne * kc / 4.0);
-    unsigned bits = 12;
-    while ((double)(1u << bits) < want && bits < 24) ++bits;
-    P.hb_bits = bits;
-    P.pair_cap = 32 * ne + 4096;
-    const int ntiles = (ne + 255) / 256;
-    if ((r = dalloc(ctx, P.ebox, (size_t)6 * ne)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ntiles)) ||
-        (r = dalloc(ctx, P.ecell, (size_t)6 * ne)) || (r = dalloc(ctx, P.bcount, (size_t)1 << bits)) ||
-        (r = dalloc(ctx, P.bent, ((size_t)1 << bits) * kBucketCap)) ||
-        (r = dalloc(ctx, P.found_cnt, (size_t)2 * (ne + 1))) ||
-        (r = dalloc(ctx, P.found_e2, (size_t)kRowCap * ne)) || (r = dalloc(ctx, P.found_d2, (size_t)kRowCap * ne)))
+  if (ctx->any_contact) {
+    if ((r = dalloc(ctx, P.ebox, (size_t)6 * nE)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ctx->ntile)) ||
+        (r = dalloc(ctx, P.ecell, (size_t)6 * nE)) || (r = dalloc(ctx, P.bcount, (size_t)bb)) ||
+        (r = dalloc(ctx, P.bent, (size_t)bb * kBucketCap)) ||
+        (r = dalloc(ctx, P.found_cnt, (size_t)2 * (nE + 1))) ||
+        (r = dalloc(ctx, P.found_e2, (size_t)kRowCap * nE)) || (r = dalloc(ctx, P.found_d2, (size_t)kRowCap * nE)))
       return fail(r);
     for (int b = 0; b < 2; ++b)
-      if ((r = dalloc(ctx, P.row_start[b], (size_t)ne + 1)) || (r = dalloc(ctx, P.pe1[b], (size_t)P.pair_cap)) ||
-          (r = dalloc(ctx, P.pe2[b], (size_t)P.pair_cap)) || (r = dalloc(ctx, P.pd2[b], (size_t)P.pair_cap)) ||
-          (r = dalloc(ctx, P.pstick[b], (size_t)P.pair_cap)) || (r = dalloc(ctx, P.pso[b], (size_t)3 * P.pair_cap)))
+      if ((r = dalloc(ctx, P.row_start[b], (size_t)nE + nshell)) || (r = dalloc(ctx, P.pe1[b], (size_t)pb)) ||
+          (r = dalloc(ctx, P.pe2[b], (size_t)pb)) || (r = dalloc(ctx, P.pd2[b], (size_t)pb)) ||
+          (r = dalloc(ctx, P.pstick[b], (size_t)pb)) || (r = dalloc(ctx, P.pso[b], (size_t)3 * pb)))
         return fail(r);
+    /* empty pair lists: every row of a shell starts at its region base */
+    std::vector<int32_t> rs0((size_t)nE + nshell, 0);
+    for (int s = 0; s < nshell; ++s) {
+      const SConst& C = ctx->C[s];
+      for (int q = 0; q <= ne[s]; ++q) rs0[(size_t)C.ebase + q + s] = C.pbase;
+    }
+    for (int b = 0; b < 2; ++b)
+      if (hipMemcpy(P.row_start[b], rs0.data(), sizeof(int32_t) * rs0.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        ctx->err = "row_start upload failed";
+        return fail(SHX_E_HIP);
+      }
   }
 
-  k_precompute<<<ctx->nblkE, 256, 0, ctx->st>>>(P, C);
-  k_mass<<<ctx->nblkN, 256, 0, ctx->st>>>(P, C, prm->damping_alpha, prm->damping_beta, prm->damping_gamma);
+  k_precompute<<<ctx->ntile, 256, 0, ctx->st>>>(P);
+  k_mass<<<nV / 256, 256, 0, ctx->st>>>(P);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->st) != hipSuccess) {
     ctx->err = "setup kernels failed";
     return fail(SHX_E_HIP);
   }
-  DShell s;
-  std::memset(&s, 0, sizeof(s));
-  set_newmark_host(s, C, prm->dt);
-  s.fresh = 1;
-  s.converged_step = -1;
-  if ((r = write_sh(ctx, s))) return fail(r);
+  std::vector<DShell> hs(nshell);
+  for (int s = 0; s < nshell; ++s) {
+    DShell& d = hs[s];
+    std::memset(&d, 0, sizeof(d));
+    set_newmark_host(d, ctx->C[s], prm[s].dt);
+    d.fresh = 1;
+    d.converged_step = -1;
+  }
+  if ((r = write_sh(ctx, hs))) return fail(r);
   *out = ctx;
+  return SHX_OK;
+}
+
+int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tri, const shx_params* prm,
+               int device)
+{
+  return shx_batch_create(out, 1, &nv, &ne, X, tri, prm, device);
+}
+
+int shx_batch_size(const shx_ctx* ctx) { return ctx ? ctx->S : SHX_E_ARG; }
+
+int shx_batch_set_state(shx_ctx* ctx, int shell, const double* u, const double* v, const double* a, double t)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  const SConst& C = ctx->C[shell];
+  const size_t off = (size_t)3 * C.vbase, bytes = sizeof(double) * 3 * C.nv;
+  const int par = s[shell].par;
+  if (u) HIPCHK(hipMemcpy(ctx->P.U[par] + off, u, bytes, hipMemcpyHostToDevice));
+  if (v) HIPCHK(hipMemcpy(ctx->P.V[par] + off, v, bytes, hipMemcpyHostToDevice));
+  if (a) HIPCHK(hipMemcpy(ctx->P.A[par] + off, a, bytes, hipMemcpyHostToDevice));
+  s[shell].t = t;
+  return write_sh(ctx, s);
+}
+
+int shx_batch_get_state(shx_ctx* ctx, int shell, double* u, double* v, double* a, double* t)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  const SConst& C = ctx->C[shell];
+  const size_t off = (size_t)3 * C.vbase, bytes = sizeof(double) * 3 * C.nv;
+  const int par = s[shell].par;
+  if (u) HIPCHK(hipMemcpy(u, ctx->P.U[par] + off, bytes, hipMemcpyDeviceToHost));
+  if (v) HIPCHK(hipMemcpy(v, ctx->P.V[par] + off, bytes, hipMemcpyDeviceToHost));
+  if (a) HIPCHK(hipMemcpy(a, ctx->P.A[par] + off, bytes, hipMemcpyDeviceToHost));
+  if (t) *t = s[shell].t;
   return SHX_OK;
 }
 
 int shx_set_state(shx_ctx* ctx, const double* u, const double* v, const double* a, double t)
 {
-  if (!ctx) return SHX_E_ARG;
-  DShell s;
-  if (int r = read_sh(ctx, s)) return r;
-  const size_t bytes = sizeof(double) * 3 * ctx->C.nv;
-  if (u) HIPCHK(hipMemcpy(ctx->P.U[s.par], u, bytes, hipMemcpyHostToDevice));
-  if (v) HIPCHK(hipMemcpy(ctx->P.V[s.par], v, bytes, hipMemcpyHostToDevice));
-  if (a) HIPCHK(hipMemcpy(ctx->P.A[s.par], a, bytes, hipMemcpyHostToDevice));
-  s.t = t;
-  return write_sh(ctx, s);
+  return shx_batch_set_state(ctx, 0, u, v, a, t);
 }
 
 int shx_get_state(shx_ctx* ctx, double* u, double* v, double* a, double* t)
 {
-  if (!ctx) return SHX_E_ARG;
-  DShell s;
-  if (int r = read_sh(ctx, s)) return r;
-  const size_t bytes = sizeof(double) * 3 * ctx->C.nv;
-  if (u) HIPCHK(hipMemcpy(u, ctx->P.U[s.par], bytes, hipMemcpyDeviceToHost));
-  if (v) HIPCHK(hipMemcpy(v, ctx->P.V[s.par], bytes, hipMemcpyDeviceToHost));
-  if (a) HIPCHK(hipMemcpy(a, ctx->P.A[s.par], bytes, hipMemcpyDeviceToHost));
-  if (t) *t = s.t;
-  return SHX_OK;
+  return shx_batch_get_state(ctx, 0, u, v, a, t);
 }
 
 int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* v, double t, double* f,
@@ -393,123 +579,177 @@ int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* 
 {
   if (!ctx || !u_pred) return SHX_E_ARG;
   if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
-  DShell s;
+  std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  s.done = 0;
+  for (auto& d : s) { d.done = 0; d.t_eval = t; }
   if (int r = write_sh(ctx, s)) return r;
-  const size_t nv = ctx->C.nv, ne = ctx->C.ne, bytes = sizeof(double) * 3 * nv;
-  HIPCHK(hipMemcpy(ctx->P.U[s.par], u_pred, bytes, hipMemcpyHostToDevice));
-  if (v) HIPCHK(hipMemcpy(ctx->P.V[s.par], v, bytes, hipMemcpyHostToDevice));
+  const SConst& C0 = ctx->C[0];
+  const size_t nv = C0.nv, ne = C0.ne, bytes = sizeof(double) * 3 * nv;
+  const int par = s[0].par;
+  HIPCHK(hipMemcpy(ctx->P.U[par], u_pred, bytes, hipMemcpyHostToDevice));
+  if (v) HIPCHK(hipMemcpy(ctx->P.V[par], v, bytes, hipMemcpyHostToDevice));
   const DPtrs& P = ctx->P;
-  const SConst& C = ctx->C;
-  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, C, M_FORCES);
-  if (C.contact) k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, C, M_FORCES);
-  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, C, M_FORCES, t, C.contact ? ctx->nblkC : 0);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, M_FORCES);
+  if (ctx->any_contact) k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, M_FORCES);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, M_FORCES);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->st));
   if (f) HIPCHK(hipMemcpy(f, P.fout, bytes, hipMemcpyDeviceToHost));
   if (f_ext) HIPCHK(hipMemcpy(f_ext, P.fext_out, bytes, hipMemcpyDeviceToHost));
   if (elem_area) HIPCHK(hipMemcpy(elem_area, P.area, sizeof(double) * ne, hipMemcpyDeviceToHost));
   if (int r = read_sh(ctx, s)) return r;
-  if (energies) { energies[0] = s.e_tens; energies[1] = s.e_bend; energies[2] = s.e_self; energies[3] = s.e_fric; }
+  if (energies) {
+    energies[0] = s[0].e_tens; energies[1] = s[0].e_bend; energies[2] = s[0].e_self; energies[3] = s[0].e_fric;
+  }
   return check_err(ctx, s);
 }
 
 int shx_contact_rebuild(shx_ctx* ctx, int force)
 {
   if (!ctx) return SHX_E_ARG;
-  if (!ctx->C.contact) return 0;
-  DShell s;
+  if (!ctx->C[0].contact) return 0;
+  std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  s.rebuild = force ? 1 : (s.du_sum >= ctx->C.cell_margin * (ctx->C.cell_size_mult * s.avg_elem_size));
-  s.done = 0;
+  for (int i = 0; i < ctx->S; ++i) {
+    const SConst& C = ctx->C[i];
+    s[i].rebuild = C.contact && (force ? 1 : (s[i].du_sum >= C.cell_margin * (C.cell_size_mult * s[i].avg_elem_size)));
+    s[i].done = 0;
+  }
   if (int r = write_sh(ctx, s)) return r;
-  if (int r = launch_broad(ctx, M_FORCES)) return r;
+  launch_broad(ctx, M_FORCES);
+  HIPCHK(hipGetLastError());
   if (int r = read_sh(ctx, s)) return r;
   prof_collect(ctx);
   if (int r = check_err(ctx, s)) return r;
-  return s.n_pairs;
+  return s[0].n_pairs;
 }
 
-int shx_init_dynamics(shx_ctx* ctx, int precision)
+int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
 {
   if (!ctx) return SHX_E_ARG;
   if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
-  DShell s;
+  std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  s.done = 0; s.n_steps = 0; s.n_rejected = 0; s.converged_step = -1; s.fresh = 1; s.attempts = 0;
+  for (int i = 0; i < ctx->S; ++i) {
+    DShell& d = s[i];
+    d.done = 0; d.n_steps = 0; d.n_rejected = 0; d.converged_step = -1; d.fresh = 1; d.attempts = 0;
+    d.rebuild = ctx->C[i].contact ? 1 : 0;
+  }
   if (int r = write_sh(ctx, s)) return r;
   const DPtrs& P = ctx->P;
-  const SConst& C = ctx->C;
-  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, C, M_INIT);
-  HIPCHK(hipGetLastError());
-  if (C.contact) {
-    if (int r = read_sh(ctx, s)) return r;
-    s.rebuild = 1;
-    if (int r = write_sh(ctx, s)) return r;
-    if (int r = launch_broad(ctx, M_INIT)) return r;
-    k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, C, M_INIT);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, M_INIT);
+  if (ctx->any_contact) {
+    launch_broad(ctx, M_INIT);
+    k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, M_INIT);
   }
-  if (int r = read_sh(ctx, s)) return r;
-  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, C, M_INIT, s.t + s.dt, C.contact ? ctx->nblkC : 0);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, M_INIT);
   HIPCHK(hipGetLastError());
   if (int r = read_sh(ctx, s)) return r;
   prof_collect(ctx);
+  return check_err(ctx, s);
+}
+
+int shx_init_dynamics(shx_ctx* ctx, int precision) { return shx_batch_init_dynamics(ctx, precision); }
+
+int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged, shx_report* reps)
+{
+  if (!ctx || n_steps < 0) return SHX_E_ARG;
+  if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  if (n_steps > 0) {
+    bool cchange = false;
+    for (auto& C : ctx->C)
+      if (C.conv_tol != tol || C.stop_on_converged != (stop_on_converged ? 1 : 0)) {
+        C.conv_tol = tol;
+        C.stop_on_converged = stop_on_converged ? 1 : 0;
+        cchange = true;
+      }
+    if (cchange)
+      if (int r = write_consts(ctx, ctx->C)) return r;
+    for (auto& d : s) { d.steps_target = d.n_steps + n_steps; d.done = (d.problem >= 7 || d.err) ? 1 : 0; }
+    if (int r = write_sh(ctx, s)) return r;
+    const long long max_attempts = 64LL * n_steps + 4096;
+    long long enq = 0;
+    while (true) {
+      if (ctx->use_graph && !ctx->prof) {
+        if (int r = launch_attempts_graph(ctx)) return r;
+        enq += kGraphAttempts;
+      } else {
+        int remaining = 0;
+        for (const auto& d : s) remaining = std::max(remaining, d.done ? 0 : d.steps_target - d.n_steps);
+        const int chunk = std::max(1, std::min(remaining, 256));
+        for (int k = 0; k < chunk; ++k) enqueue_attempt(ctx);
+        HIPCHK(hipGetLastError());
+        enq += chunk;
+      }
+      if (int r = read_sh(ctx, s)) return r;
+      prof_collect(ctx);
+      bool all_done = true;
+      for (const auto& d : s) all_done = all_done && (d.done || d.err);
+      if (all_done) break;
+      if (enq > max_attempts) { ctx->err = "step attempts exceeded the bound"; return SHX_E_STATE; }
+    }
+  }
+  if (reps)
+    for (int i = 0; i < ctx->S; ++i) fill_report(s[i], reps + i);
   return check_err(ctx, s);
 }
 
 int shx_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged, shx_report* rep)
 {
-  if (!ctx || n_steps < 0) return SHX_E_ARG;
-  if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
-  DShell s;
-  if (int r = read_sh(ctx, s)) return r;
-  if (n_steps == 0) { fill_report(s, rep); return SHX_OK; }
-  s.steps_target = s.n_steps + n_steps;
-  s.done = 0;
-  if (int r = write_sh(ctx, s)) return r;
-  ctx->C.conv_tol = tol;
-  ctx->C.stop_on_converged = stop_on_converged ? 1 : 0;
-  const long long max_attempts = 64LL * n_steps + 4096;
-  long long enq = 0;
-  while (true) {
-    const int remaining = s.steps_target - s.n_steps;
-    const int chunk = std::max(1, std::min(remaining, 256));
-    for (int k = 0; k < chunk; ++k)
-      if (int r = enqueue_attempt(ctx)) return r;
-    enq += chunk;
-    if (int r = read_sh(ctx, s)) return r;
-    prof_collect(ctx);
-    if (s.done || s.err) break;
-    if (enq > max_attempts) { ctx->err = "step attempts exceeded the bound"; return SHX_E_STATE; }
-  }
-  fill_report(s, rep);
-  return check_err(ctx, s);
+  if (!ctx) return SHX_E_ARG;
+  std::vector<shx_report> reps(ctx->S);
+  const int r = shx_batch_step(ctx, precision, n_steps, tol, stop_on_converged, reps.data());
+  if (rep) *rep = reps[0];
+  return r;
 }
 
-int shx_pairs(shx_ctx* ctx, uint32_t* e1, uint32_t* e2, int cap)
+int shx_batch_reports(shx_ctx* ctx, shx_report* reps)
+{
+  if (!ctx || !reps) return SHX_E_ARG;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  for (int i = 0; i < ctx->S; ++i) fill_report(s[i], reps + i);
+  return SHX_OK;
+}
+
+int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap)
 {
   if (!ctx) return SHX_E_ARG;
-  if (!ctx->C.contact) return 0;
-  DShell s;
+  if (int r = check_shell(ctx, shell)) return r;
+  const SConst& C = ctx->C[shell];
+  if (!C.contact) return 0;
+  std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  const int n = std::min(s.n_pairs, cap);
-  if (n > 0 && e1) HIPCHK(hipMemcpy(e1, ctx->P.pe1[s.pair_buf], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-  if (n > 0 && e2) HIPCHK(hipMemcpy(e2, ctx->P.pe2[s.pair_buf], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-  return s.n_pairs;
+  const int n = std::min(s[shell].n_pairs, cap), b = s[shell].pair_buf;
+  if (n > 0) {
+    std::vector<int32_t> tmp(n);
+    for (int k = 0; k < 2; ++k) {
+      uint32_t* dst = k ? e2 : e1;
+      if (!dst) continue;
+      HIPCHK(hipMemcpy(tmp.data(), (k ? ctx->P.pe2[b] : ctx->P.pe1[b]) + C.pbase, sizeof(int32_t) * n,
+                       hipMemcpyDeviceToHost));
+      for (int q = 0; q < n; ++q) dst[q] = (uint32_t)(tmp[q] - C.ebase);
+    }
+  }
+  return s[shell].n_pairs;
 }
+
+int shx_pairs(shx_ctx* ctx, uint32_t* e1, uint32_t* e2, int cap) { return shx_batch_pairs(ctx, 0, e1, e2, cap); }
 
 int shx_elem_tables(shx_ctx* ctx, double* out)
 {
   if (!ctx || !out) return SHX_E_ARG;
-  const int ne = ctx->C.ne;
-  std::vector<double> tab((size_t)T_ROWS * ne);
+  const SConst& C = ctx->C[0];
+  const int ne = C.ne, nE = ctx->P.nE;
+  std::vector<double> tab((size_t)T_ROWS * nE);
   HIPCHK(hipMemcpy(tab.data(), ctx->P.tab, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
   for (int e = 0; e < ne; ++e) {
     double* o = out + (size_t)101 * e;
-    auto g = [&](int row) { return tab[(size_t)row * ne + e]; };
+    auto g = [&](int row) { return tab[(size_t)row * nE + C.ebase + e]; };
     const double R00 = g(T_R00), R01 = g(T_R01), R10 = g(T_R10), R11 = g(T_R11);
-    const double* rb = &ctx->rawb[(ctx->T.valence[e] - 3) * 72];
+    const double* rb = &ctx->rawb[(ctx->T[0].valence[e] - 3) * 72];
     for (int i = 0; i < 12; ++i) {
       const double dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
       o[i] = rb[i];
@@ -530,10 +770,11 @@ int shx_elem_tables(shx_ctx* ctx, double* out)
 int shx_mass(shx_ctx* ctx, double* Minv, double* Cd)
 {
   if (!ctx) return SHX_E_ARG;
-  const int nv = ctx->C.nv;
+  const SConst& C = ctx->C[0];
+  const int nv = C.nv;
   std::vector<double> m(nv), c(nv);
-  HIPCHK(hipMemcpy(m.data(), ctx->P.minv, sizeof(double) * nv, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(c.data(), ctx->P.cdamp, sizeof(double) * nv, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(m.data(), ctx->P.minv + C.vbase, sizeof(double) * nv, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(c.data(), ctx->P.cdamp + C.vbase, sizeof(double) * nv, hipMemcpyDeviceToHost));
   for (int n = 0; n < nv; ++n)
     for (int q = 0; q < 3; ++q) {
       if (Minv) Minv[3 * n + q] = m[n];
@@ -545,59 +786,95 @@ int shx_mass(shx_ctx* ctx, double* Minv, double* Cd)
 int shx_patches(shx_ctx* ctx, int32_t* idx, int32_t* tri)
 {
   if (!ctx) return SHX_E_ARG;
-  if (idx) std::copy(ctx->T.patch.begin(), ctx->T.patch.end(), idx);
-  if (tri) std::copy(ctx->T.tri.begin(), ctx->T.tri.end(), tri);
+  if (idx) std::copy(ctx->T[0].patch.begin(), ctx->T[0].patch.end(), idx);
+  if (tri) std::copy(ctx->T[0].tri.begin(), ctx->T[0].tri.end(), tri);
+  return SHX_OK;
+}
+
+/* power iteration on M^-1 K for every shell at once: K x by central
+ * differences of the internal force (pressure, volume and contact off) at
+ * +-eps x about u = 0 on state buffer 0; the state is restored afterwards */
+int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
+{
+  if (!ctx || !omega || iters <= 0) return SHX_E_ARG;
+  const int nV = ctx->P.nV, S = ctx->S;
+  const size_t n3 = (size_t)3 * nV;
+  std::vector<DShell> s0, s;
+  if (int r = read_sh(ctx, s0)) return r;
+  std::vector<double> u0(n3), x(n3, 0.0), fp(n3), fm(n3), up(n3, 0.0), minv(nV);
+  HIPCHK(hipMemcpy(u0.data(), ctx->P.U[0], sizeof(double) * n3, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(minv.data(), ctx->P.minv, sizeof(double) * nV, hipMemcpyDeviceToHost));
+  std::vector<SConst> Ci = ctx->C;
+  for (auto& c : Ci) { c.pressure_final = 0; c.bulk_modulus = 0; c.contact = 0; }
+  if (int r = write_consts(ctx, Ci)) return r;
+  s = s0;
+  for (auto& d : s) { d.done = 0; d.par = 0; d.t_eval = 0; }
+  if (int r = write_sh(ctx, s)) return r;
+  uint64_t st = 1;
+  for (int q = 0; q < S; ++q) {
+    const SConst& C = ctx->C[q];
+    for (int i = 0; i < 3 * C.nv; ++i) {
+      st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+      x[(size_t)3 * C.vbase + i] = (double)(st >> 11) / 9007199254740992.0 - 0.5;
+    }
+  }
+  std::vector<double> lam(S, 0.0);
+  auto fint = [&](const std::vector<double>& u, std::vector<double>& f) -> int {
+    HIPCHK(hipMemcpy(ctx->P.U[0], u.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
+    k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(ctx->P, M_FORCES);
+    k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(ctx->P, M_FORCES);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(f.data(), ctx->P.fout, sizeof(double) * n3, hipMemcpyDeviceToHost));
+    return 0;
+  };
+  int rc = 0;
+  for (int it = 0; it < iters && !rc; ++it) {
+    for (int q = 0; q < S; ++q) {
+      const SConst& C = ctx->C[q];
+      double nx = 0;
+      for (int i = 0; i < 3 * C.nv; ++i) nx += x[(size_t)3 * C.vbase + i] * x[(size_t)3 * C.vbase + i];
+      nx = std::sqrt(nx);
+      const double eps = 1e-7 * C.h;
+      for (int i = 0; i < 3 * C.nv; ++i) {
+        double& xi = x[(size_t)3 * C.vbase + i];
+        xi /= nx;
+        up[(size_t)3 * C.vbase + i] = eps * xi;
+      }
+    }
+    if ((rc = fint(up, fp))) break;
+    for (double& q : up) q = -q;
+    if ((rc = fint(up, fm))) break;
+    for (int q = 0; q < S; ++q) {
+      const SConst& C = ctx->C[q];
+      const double eps = 1e-7 * C.h;
+      double num = 0;
+      for (int i = 0; i < 3 * C.nv; ++i) {
+        const size_t g = (size_t)3 * C.vbase + i;
+        const double y = minv[C.vbase + i / 3] * (-(fp[g] - fm[g]) / (2 * eps));
+        num += x[g] * y;
+        x[g] = y;
+      }
+      lam[q] = num;
+    }
+  }
+  if (hipMemcpy(ctx->P.U[0], u0.data(), sizeof(double) * n3, hipMemcpyHostToDevice) != hipSuccess && !rc) {
+    ctx->err = "state restore failed";
+    rc = SHX_E_HIP;
+  }
+  if (int r = write_consts(ctx, ctx->C)) return r;
+  if (int r = write_sh(ctx, s0)) return r;
+  if (rc) return rc;
+  for (int q = 0; q < S; ++q) omega[q] = std::sqrt(std::fabs(lam[q]));
   return SHX_OK;
 }
 
 int shx_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
 {
-  if (!ctx || !omega || iters <= 0) return SHX_E_ARG;
-  const int nv = ctx->C.nv, n3 = 3 * nv;
-  DShell s;
-  if (int r = read_sh(ctx, s)) return r;
-  std::vector<double> u0(n3), x(n3), fp(n3), fm(n3), y(n3), up(n3), minv(n3);
-  HIPCHK(hipMemcpy(u0.data(), ctx->P.U[s.par], sizeof(double) * n3, hipMemcpyDeviceToHost));
-  if (int r = shx_mass(ctx, minv.data(), nullptr)) return r;
-  SConst Ci = ctx->C;
-  Ci.pressure_final = 0; Ci.bulk_modulus = 0; Ci.contact = 0;
-  uint64_t st = 1;
-  for (int i = 0; i < n3; ++i) {
-    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
-    x[i] = (double)(st >> 11) / 9007199254740992.0 - 0.5;
-  }
-  const double eps = 1e-7 * ctx->C.h;
-  double lam = 0;
-  s.done = 0;
-  if (int r = write_sh(ctx, s)) return r;
-  auto fint = [&](const std::vector<double>& u, std::vector<double>& f) -> int {
-    HIPCHK(hipMemcpy(ctx->P.U[s.par], u.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
-    k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(ctx->P, Ci, M_FORCES);
-    k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(ctx->P, Ci, M_FORCES, 0.0, 0);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpy(f.data(), ctx->P.fout, sizeof(double) * n3, hipMemcpyDeviceToHost));
-    return 0;
-  };
-  for (int it = 0; it < iters; ++it) {
-    double nx = 0;
-    for (double q : x) nx += q * q;
-    nx = std::sqrt(nx);
-    for (double& q : x) q /= nx;
-    for (int i = 0; i < n3; ++i) up[i] = eps * x[i];
-    if (int r = fint(up, fp)) return r;
-    for (int i = 0; i < n3; ++i) up[i] = -eps * x[i];
-    if (int r = fint(up, fm)) return r;
-    double num = 0;
-    for (int i = 0; i < n3; ++i) {
-      y[i] = minv[i] * (-(fp[i] - fm[i]) / (2 * eps));
-      num += x[i] * y[i];
-    }
-    lam = num;
-    x = y;
-  }
-  HIPCHK(hipMemcpy(ctx->P.U[s.par], u0.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
-  *omega = std::sqrt(std::fabs(lam));
-  return SHX_OK;
+  if (!ctx || !omega) return SHX_E_ARG;
+  std::vector<double> om(ctx->S);
+  const int r = shx_batch_cfl_omega_max(ctx, iters, om.data());
+  if (r == SHX_OK) *omega = om[0];
+  return r;
 }
 
 int shx_set_profiling(shx_ctx* ctx, int enable)
@@ -606,6 +883,13 @@ int shx_set_profiling(shx_ctx* ctx, int enable)
   prof_collect(ctx);
   ctx->prof = enable != 0;
   for (int k = 0; k < 5; ++k) { ctx->tsum[k] = 0; ctx->tcnt[k] = 0; }
+  return SHX_OK;
+}
+
+int shx_set_graphs(shx_ctx* ctx, int enable)
+{
+  if (!ctx) return SHX_E_ARG;
+  ctx->use_graph = enable != 0;
   return SHX_OK;
 }
 

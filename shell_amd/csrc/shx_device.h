@@ -1,21 +1,28 @@
 /*
  * shx_device.h -- device-side data layout of libshellhip.
  *
- * One shell per context.  Everything lives in HBM:
- *  - element tables, structure of arrays [k][ne] so that one thread per
- *    element reads consecutive addresses: patch ids (12 x int32), valence
- *    (int8), and 29 FP64 rows (R00 R01 R10 R11, JxW, A1 A2 A11 A22 A12,
- *    ff1_ref, ff2_ref, xyz);
+ * A context holds a batch of S >= 1 independent shells.  Their nodes and
+ * elements are concatenated into one global numbering; each shell's range
+ * starts at a multiple of kAlign and is padded to a multiple of kAlign with
+ * inert entries, so that every thread block of every kernel covers exactly
+ * one shell (the block reads the shell id of its first element or node).
+ * Everything lives in HBM:
+ *  - element tables, structure of arrays [k][nE] over the global element
+ *    index: patch ids (12 x int32, global node ids), valence (int8), and 29
+ *    FP64 rows (R00 R01 R10 R11, JxW, A1 A2 A11 A22 A12, ff1_ref, ff2_ref,
+ *    xyz);
  *  - the Loop basis at the quadrature point for each valence (6 x 12 FP64,
  *    staged in LDS by the kernels that use it);
- *  - node state u, v, a twice (ping-pong; DShell::par selects the current
- *    copy), node-interleaved xyz FP64;
- *  - per element and patch slot, the internal-force contribution (3 FP64,
- *    slot-major) and per element JxW a3, written by the element kernel and
- *    gathered by the node kernel through the node incidence list, so no
- *    internal force is ever scattered with atomics;
- *  - the contact pair list (sorted by eid1 then eid2, ping-pong) and its
- *    per-pair friction state.
+ *  - node state u, v, a twice (ping-pong; DShell::par of the node's shell
+ *    selects the current copy), node-interleaved xyz FP64;
+ *  - per element and patch slot, the internal-force contribution (3 FP64)
+ *    and per element JxW a3, written by the element kernel and gathered by
+ *    the node kernel through the node incidence list, so no internal force
+ *    is ever scattered with atomics;
+ *  - per shell: constants (SConst), mutable scalars (DShell), block partial
+ *    records, last-block counters, a region of hashed cell buckets and a
+ *    region of the two contact pair buffers (pairs sorted by eid1 then eid2,
+ *    with per-pair friction state).
  */
 #ifndef SHX_DEVICE_H
 #define SHX_DEVICE_H
@@ -31,17 +38,33 @@ enum {
   T_FF1 = 20, T_FF2 = 23, T_XYZ = 26, T_ROWS = 29
 };
 
+constexpr int kAlign = 256;         /* shell ranges start at and are padded to multiples of this */
 constexpr int kElemLanes = 4;       /* lanes cooperating on one element (3 patch slots each) */
 constexpr int kElemBlock = 256;     /* threads per element-kernel block: 64 elements */
+constexpr int kElemPerBlock = kElemBlock / kElemLanes;
 constexpr int kNodeLanes = 4;       /* lanes cooperating on one node */
-constexpr int kNodeBlock = 256;     /* threads per node-kernel block */
+constexpr int kNodeBlock = 256;     /* threads per node-kernel block: 64 nodes */
+constexpr int kNodePerBlock = kNodeBlock / kNodeLanes;
 constexpr int kNodeRec = 22;        /* node-kernel partial record length */
 constexpr int kBucketCap = 64;      /* (cell, element) entries per hash bucket */
 constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
+constexpr int kBpPerBlock = 256 / kBpLanes;
+constexpr int kRowCap = 64;         /* new candidates per element and rebuild */
+constexpr int kCtrPerShell = 8;     /* last-block counters per shell */
 
-/* per-shell constants (kernel argument) */
+/* per-shell constants (device array, read-only in the step kernels) */
 struct SConst {
-  int nv, ne;
+  /* placement of the shell in the batch */
+  int nv, ne;                      /* real node and element counts */
+  int vbase, ebase;                /* first global node / element */
+  int blkE, nbE;                   /* element-kernel blocks (kElemPerBlock elements each) */
+  int blkN, nbN;                   /* node-kernel blocks (kNodePerBlock nodes each) */
+  int blkC, nbC;                   /* contact-kernel blocks (0 without contact) */
+  int tile0, ntile;                /* 256-element tiles (broad-phase box / emit blocks) */
+  int nbL;                         /* broad-phase lane blocks (kBpPerBlock elements each) */
+  int pbase, pair_cap;             /* pair-buffer region */
+  unsigned bbase, hb_bits;         /* hash-bucket region: 1 << hb_bits buckets from bbase */
+  /* physics (update_derived, ShellSystem.C:421-487) */
   double h, rho;
   double E1, E2, nu, G12, nu2;     /* nu2 = E1 nu / E2 (ShellSystem.C:1199) */
   double K, D;
@@ -52,16 +75,18 @@ struct SConst {
   double friction_stop_velocity;
   double cell_size_mult, cell_margin;
   double beta, gamma, dt_min, dt_max, rel_loc_err_min, rel_loc_err_max;
+  double damping_alpha, damping_beta, damping_gamma;
   double conv_tol, load_end_time;
   int adapt_dt_every, contact, self_friction, quirk_fulltime, quirk_volume_schedule;
   int stop_on_converged, volume_at_prediction;
   unsigned n_cells_max;
 };
 
-/* per-shell mutable scalars (device memory, written by the finalize and
+/* per-shell mutable scalars (device array, written by the finalize and
  * broad-phase kernels) */
 struct DShell {
   double t, dt, nm[10];
+  double t_eval;                   /* force time of a plain force evaluation (M_FORCES) */
   double du_sum, avg_elem_size;
   double volume, volume0, area;
   double bmin[3], bmax[3];
@@ -81,54 +106,55 @@ struct DShell {
 /* error bits in DShell::err */
 enum { ERR_ENTRIES = 1, ERR_PAIRS = 2, ERR_ROW = 4, ERR_CELLS = 8 };
 
-/* device pointers (kernel argument) */
+/* device pointers (kernel argument); element and node indices are global */
 struct DPtrs {
-  const double* X;            /* nv*3 */
-  const int32_t* pidx;        /* 12*ne */
-  const int8_t* vcls;         /* ne: valence */
-  const int32_t* tri;         /* 3*ne, SoA [k][ne] */
-  double* tab;                /* T_ROWS*ne */
+  int nE, nV;                 /* padded global element / node counts (SoA strides) */
+  const SConst* cs;           /* S */
+  DShell* sh;                 /* S */
+  const int32_t* eshell;      /* nE: shell of each (padded) element */
+  const int32_t* vshell;      /* nV: shell of each (padded) node */
+  const int32_t* cshell;      /* contact-kernel block -> shell */
+  const double* X;            /* nV*3 */
+  const int32_t* pidx;        /* 12*nE */
+  const int8_t* vcls;         /* nE: valence */
+  const int32_t* tri;         /* 3*nE, SoA [k][nE] */
+  double* tab;                /* T_ROWS*nE */
   const double* rawb;         /* [4 valences 3..6][6][12] */
-  const int32_t* inc_start;   /* nv+1 */
-  const int32_t* inc;         /* e*16+i */
-  double* minv;               /* nv */
-  double* cdamp;              /* nv */
+  const int32_t* inc_start;   /* nV+1 */
+  const int32_t* inc;         /* e*16+i, e global */
+  double* minv;               /* nV */
+  double* cdamp;              /* nV */
   double* U[2];
   double* V[2];
   double* A[2];
-  double* qe;                 /* 36*ne: per element and patch slot, internal-force contribution */
-  double* g0;                 /* 3*ne: JxW a3 per element (pressure per unit p) */
-  unsigned* ctr;              /* last-block counters: elem, node, bp_box, bp_pairs, bp_emit */
-  double* area;               /* ne */
-  double* fc;                 /* nv*3 contact force accumulator */
-  double* fout;               /* nv*3 debug outputs (forces mode) */
+  double* qe;                 /* 36*nE: per element and patch slot, internal-force contribution */
+  double* g0;                 /* 3*nE: JxW a3 per element (pressure per unit p) */
+  unsigned* ctr;              /* kCtrPerShell*S last-block counters: elem, node, bp_box, bp_pairs, bp_emit */
+  double* area;               /* nE */
+  double* fc;                 /* nV*3 contact force accumulator */
+  double* fout;               /* nV*3 outputs of a plain force evaluation */
   double* fext_out;
-  double* epart;              /* element kernel block partials */
-  double* npart;              /* node kernel block partials */
-  double* cpart;              /* contact kernel block partials */
-  DShell* sh;
+  double* epart;              /* element kernel block partials (5 per block) */
+  double* npart;              /* node kernel block partials (kNodeRec per block) */
+  double* cpart;              /* contact kernel block partials (3 per block) */
   /* contact */
-  const int32_t* close_start; /* ne+1 */
+  const int32_t* close_start; /* nE+1 */
   const int32_t* close_ids;
-  double* ebox;               /* 6*ne, SoA */
+  double* ebox;               /* 6*nE, SoA */
   double* bpart;              /* broad-phase tile partials: 7 per tile */
-  int32_t* ecell;             /* 6*ne: cell index ranges lo[3], hi[3] */
-  uint32_t* bcount;           /* hash buckets of the cell grid: fill counts */
-  uint64_t* bent;             /* bucket entries (cell << 32 | element) */
-  unsigned hb_bits;           /* log2 of the number of buckets */
-  int32_t* found_cnt;         /* 2*(ne+1): pair counts, new-pair counts */
-  int32_t* found_e2;          /* ne*ROWCAP */
-  double* found_d2;           /* ne*ROWCAP */
-  int32_t* row_start[2];      /* ne+1, per pair buffer */
+  int32_t* ecell;             /* 6*nE: cell index ranges lo[3], hi[3] */
+  uint32_t* bcount;           /* hash buckets of the cell grids: fill counts */
+  uint64_t* bent;             /* bucket entries (cell << 32 | global element) */
+  int32_t* found_cnt;         /* 2*(nE+1): surviving-pair counts, new-pair counts */
+  int32_t* found_e2;          /* nE*kRowCap */
+  double* found_d2;           /* nE*kRowCap */
+  int32_t* row_start[2];      /* nE+S, per pair buffer: rows of shell s at index e + s */
   int32_t* pe1[2];
   int32_t* pe2[2];
   double* pd2[2];
   int32_t* pstick[2];
   double* pso[2];             /* 3 per pair */
-  int pair_cap;
 };
-
-constexpr int kRowCap = 64;      /* new candidates per element and rebuild */
 
 } // namespace shx
 #endif

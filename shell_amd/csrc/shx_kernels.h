@@ -16,15 +16,15 @@ namespace shx {
  * acceleration a = M^-1 f (apply_initial_conditions) */
 enum { M_STEP = 0, M_FORCES = 1, M_INIT = 2 };
 
-__global__ void k_precompute(DPtrs P, SConst C);
-__global__ void k_mass(DPtrs P, SConst C, double damping_alpha, double damping_beta, double damping_gamma);
-__global__ void k_elem(DPtrs P, SConst C, int mode);
-__global__ void k_contact(DPtrs P, SConst C, int mode);
-__global__ void k_node(DPtrs P, SConst C, int mode, double t_force, int nblkC);
-__global__ void k_bp_box(DPtrs P, SConst C, int posmode);
-__global__ void k_bp_bin(DPtrs P, SConst C);
-__global__ void k_bp_pairs(DPtrs P, SConst C, int posmode);
-__global__ void k_bp_emit(DPtrs P, SConst C);
+__global__ void k_precompute(DPtrs P);
+__global__ void k_mass(DPtrs P);
+__global__ void k_elem(DPtrs P, int mode);
+__global__ void k_contact(DPtrs P, int mode);
+__global__ void k_node(DPtrs P, int mode);
+__global__ void k_bp_box(DPtrs P, int posmode);
+__global__ void k_bp_bin(DPtrs P);
+__global__ void k_bp_pairs(DPtrs P, int posmode);
+__global__ void k_bp_emit(DPtrs P);
 
 } // namespace shx
 #endif

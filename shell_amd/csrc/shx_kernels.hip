@@ -3,8 +3,15 @@
  * precompute, lumped mass, element force (metrics, strains, stresses),
  * node gather fused with the Newmark corrector/predictor, contact narrow
  * phase with friction, per-step finalize (reductions, adaptive dt, rebuild
- * triggers, in the node kernel's last block) and the linked-cell broad
- * phase (four kernels over a hashed cell grid).
+ * triggers, in the node kernel's last block of each shell) and the
+ * linked-cell broad phase (four kernels over hashed cell grids).
+ *
+ * Every kernel runs over all shells of the batch at once: a block belongs to
+ * one shell (shell ranges are kAlign-aligned and padded), reads that shell's
+ * id from its first element or node and its constants from SConst, and the
+ * per-shell reductions use per-shell partial ranges and last-block counters.
+ * Shells that are done (or, in the broad phase, not rebuilding) return at
+ * once.
  *
  * All arithmetic is FP64 and compiled with -ffp-contract=off so that it
  * rounds like the CPU oracle it is checked against.
@@ -17,11 +24,13 @@
 
 namespace shx {
 
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 __device__ __forceinline__ v3 ld3(const double* p, int n) { return mk(p[3 * n], p[3 * n + 1], p[3 * n + 2]); }
 __device__ __forceinline__ void st3(double* p, int n, v3 a) { p[3 * n] = a.x; p[3 * n + 1] = a.y; p[3 * n + 2] = a.z; }
-__device__ __forceinline__ v3 ldrow3(const double* t, int row, int ne, int e)
+__device__ __forceinline__ v3 ldrow3(const double* t, int row, int nE, int e)
 {
-  return mk(t[(size_t)row * ne + e], t[(size_t)(row + 1) * ne + e], t[(size_t)(row + 2) * ne + e]);
+  return mk(t[(size_t)row * nE + e], t[(size_t)(row + 1) * nE + e], t[(size_t)(row + 2) * nE + e]);
 }
 
 /* u_pred = u; u_pred.add(dt, v); u_pred.add(a8, a)  (ShellSystem.C:973-975) */
@@ -81,18 +90,19 @@ __device__ __forceinline__ void load_rawb(const double* g, double* s)
  * (ShellSystem.C:490-571, 3321-3499, 1317-1379).  The Rodrigues rotation
  * applied to both A and B in the reference (ShellSystem.C:3375-3421) cancels
  * in R = (Rn A)^-1 (Rn B) = A^-1 B and is not formed. */
-__global__ void __launch_bounds__(256) k_precompute(DPtrs P, SConst C)
+__global__ void __launch_bounds__(256) k_precompute(DPtrs P)
 {
   __shared__ double s_rb[4 * 72];
   load_rawb(P.rawb, s_rb);
   __syncthreads();
-  const int ne = C.ne;
+  const int nE = P.nE;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  const SConst& C = P.cs[uni(P.eshell[blockIdx.x * blockDim.x])];
+  if (e - C.ebase >= C.ne) return;
   const double* rb = s_rb + 72 * (P.vcls[e] - 3);
   v3 xyz = mk(0, 0, 0), b1 = xyz, b2 = xyz;
   for (int i = 0; i < 12; ++i) {
-    const v3 x = ld3(P.X, P.pidx[(size_t)i * ne + e]);
+    const v3 x = ld3(P.X, P.pidx[(size_t)i * nE + e]);
     axpy(xyz, x, rb[i]);
     axpy(b1, x, rb[12 + i]);
     axpy(b2, x, rb[24 + i]);
@@ -114,7 +124,7 @@ __global__ void __launch_bounds__(256) k_precompute(DPtrs P, SConst C)
   const double a10 = b1.y, a11 = b2.y, a12 = nu.y;
   const double a20 = b1.z, a21 = b2.z, a22 = nu.z;
   const double c00 = a11 * a22 - a12 * a21, c01 = a12 * a20 - a10 * a22, c02 = a10 * a21 - a11 * a20;
-  const double c10 = a02 * a21 - a01 * a22, c11 = a00 * a22 - a02 * a20, c12 = a01 * a20 - a00 * a21;
+  const double c10 = a02 * a21 - a01 * a22, c11 = a00 * a22 - a02 * a20;
   const double idet = 1 / (a00 * c00 + a01 * c01 + a02 * c02);
   /* rows 0 and 1 of A^-1: (c00 c10 c20), (c01 c11 c21) times idet */
   const double c20 = a01 * a12 - a02 * a11, c21 = a02 * a10 - a00 * a12;
@@ -126,7 +136,7 @@ __global__ void __launch_bounds__(256) k_precompute(DPtrs P, SConst C)
   const double R11 = i10 * j.x + i11 * j.y + i12 * j.z;
   v3 A1 = mk(0, 0, 0), A2 = A1, A11 = A1, A22 = A1, A12 = A1;
   for (int i = 0; i < 12; ++i) {
-    const v3 x = ld3(P.X, P.pidx[(size_t)i * ne + e]);
+    const v3 x = ld3(P.X, P.pidx[(size_t)i * nE + e]);
     const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
     axpy(A1, x, b.d1);
     axpy(A2, x, b.d2);
@@ -137,7 +147,7 @@ __global__ void __launch_bounds__(256) k_precompute(DPtrs P, SConst C)
   v3 A3 = crs(A1, A2);
   A3 = scl(A3, 1 / nrm(A3));
   double* T = P.tab;
-  auto put = [&](int row, double v) { T[(size_t)row * ne + e] = v; };
+  auto put = [&](int row, double v) { T[(size_t)row * nE + e] = v; };
   auto put3 = [&](int row, v3 v) { put(row, v.x); put(row + 1, v.y); put(row + 2, v.z); };
   put(T_R00, R00); put(T_R01, R01); put(T_R10, R10); put(T_R11, R11);
   put(T_JXW, JxW);
@@ -151,15 +161,15 @@ __global__ void __launch_bounds__(256) k_precompute(DPtrs P, SConst C)
 /* compute_mass_damping (ShellSystem.C:762-852) as a node gather: each node
  * sums its element contributions in element order, then C += gamma and
  * M is inverted. */
-__global__ void __launch_bounds__(256) k_mass(DPtrs P, SConst C, double damping_alpha, double damping_beta,
-                                              double damping_gamma)
+__global__ void __launch_bounds__(256) k_mass(DPtrs P)
 {
   __shared__ double s_rb[4 * 72];
   load_rawb(P.rawb, s_rb);
   __syncthreads();
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= C.nv) return;
-  const int ne = C.ne;
+  const SConst& C = P.cs[uni(P.vshell[blockIdx.x * blockDim.x])];
+  if (n - C.vbase >= C.nv) return;
+  const int nE = P.nE;
   double m = 0, c = 0;
   const double rh = C.rho * C.h;
   for (int q = P.inc_start[n]; q < P.inc_start[n + 1]; ++q) {
@@ -168,37 +178,38 @@ __global__ void __launch_bounds__(256) k_mass(DPtrs P, SConst C, double damping_
     const double* rb = s_rb + 72 * (P.vcls[e] - 3);
     double lumped = 0;
     for (int jj = 0; jj < N; ++jj) lumped += rb[jj];
-    lumped *= P.tab[(size_t)T_JXW * ne + e] * rb[i];
+    lumped *= P.tab[(size_t)T_JXW * nE + e] * rb[i];
     const double Me = lumped * rh;
     double hmin = 1e300, hmax = 0;
     for (int a = 0; a < 3; ++a)
       for (int b = a + 1; b < 3; ++b) {
-        const double d = nrm(sub(ld3(P.X, P.tri[(size_t)a * ne + e]), ld3(P.X, P.tri[(size_t)b * ne + e])));
+        const double d = nrm(sub(ld3(P.X, P.tri[(size_t)a * nE + e]), ld3(P.X, P.tri[(size_t)b * nE + e])));
         hmin = fmin(hmin, d); hmax = fmax(hmax, d);
       }
     const double avg_stiff = 2 * (C.K + C.D / (hmin * hmax)) / 3;
     m += Me;
-    c += damping_alpha * Me + damping_beta * avg_stiff;
+    c += C.damping_alpha * Me + C.damping_beta * avg_stiff;
   }
-  P.cdamp[n] = c + damping_gamma;
+  P.cdamp[n] = c + C.damping_gamma;
   P.minv[n] = 1 / m;
 }
 
 
 /* ------------------------------------------------------- block utilities */
 
-/* every block of a kernel calls this once at its end; returns true in the
- * last block to arrive, which then sees all other blocks' global writes
- * (release/acquire fences around a device-scope counter).  The last block
- * resets the counter for the next launch. */
-__device__ bool last_block(unsigned* counter)
+/* every block of a shell calls this once at the end of a kernel; returns
+ * true in the last of the shell's nblocks blocks to arrive, which then sees
+ * all other blocks' global writes (release/acquire fences around a
+ * device-scope counter).  The last block resets the counter for the next
+ * launch. */
+__device__ bool last_block(unsigned* counter, int nblocks)
 {
   __shared__ int s_last;
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned prev = atomicAdd(counter, 1u);
-    s_last = (prev == gridDim.x - 1);
+    s_last = (prev == (unsigned)nblocks - 1);
     if (s_last) *counter = 0;
   }
   __syncthreads();
@@ -281,33 +292,34 @@ __device__ __forceinline__ bool rebuild_due(const DShell& s, const SConst& C)
 
 /* --------------------------------------------------------- element force */
 
-/* One thread per element: gather u_pred over the patch, form the deformed
- * metric (compute_deformed_metrics, ShellSystem.C:1407-1460), membrane and
- * bending strains, orthotropic H, stress resultants and the p-vectors
- * (subtract_internal_force, ShellSystem.C:1164-1233), and write the element's
- * generalised stresses G and from them the force contribution of each of its
- * 12 patch slots (written slot-major for the node gather).  The nodal force
- * of patch slot i is
+/* kElemLanes lanes per element gather u_pred over the patch, form the
+ * deformed metric (compute_deformed_metrics, ShellSystem.C:1407-1460),
+ * membrane and bending strains, orthotropic H, stress resultants and the
+ * p-vectors (subtract_internal_force, ShellSystem.C:1164-1233), and write the
+ * element's generalised stresses G and from them the force contribution of
+ * each of its 12 patch slots.  The nodal force of patch slot i is
  *   f_i = -(phi1 G1 + phi2 G2 - phi11 G3 - phi22 G4 - phi12 G5) + p phi G0,
  * the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis derivative
  * (ShellSystem.C:1236-1282) plus the LOCAL pressure load (ShellSystem.C:
- * 1570-1576).  The last block reduces the block partials: energies, the
- * enclosed volume of the current state (compute_enclosed_volume,
+ * 1570-1576).  The last block of each shell reduces the shell's block
+ * partials: energies, the enclosed volume (compute_enclosed_volume,
  * ShellSystem.C:2505-2541) and, in a step, the predictor increment and the
  * broad-phase trigger (check_increment(false), ShellSystem.C:978, 993). */
-__global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, SConst C, int mode)
+__global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_red[5][kElemBlock];
-  const DShell& sh = *P.sh;
+  const int sid = uni(P.eshell[blockIdx.x * kElemPerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
   __syncthreads();
-  const int ne = C.ne;
+  const int nE = P.nE;
   const int t = threadIdx.x;
   const int lane = t % kElemLanes;
-  const int e = (blockIdx.x * kElemBlock + t) / kElemLanes;
-  const bool valid = e < ne;
+  const int e = blockIdx.x * kElemPerBlock + t / kElemLanes;
+  const bool valid = e - C.ebase < C.ne;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
   const int par = sh.par;
@@ -322,14 +334,14 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, SConst C, int mode
   double R00 = 0, R01 = 0, R10 = 0, R11 = 0;
   int N = 6;
   if (valid) {
-    R00 = T[(size_t)T_R00 * ne + e]; R01 = T[(size_t)T_R01 * ne + e];
-    R10 = T[(size_t)T_R10 * ne + e]; R11 = T[(size_t)T_R11 * ne + e];
+    R00 = T[(size_t)T_R00 * nE + e]; R01 = T[(size_t)T_R01 * nE + e];
+    R10 = T[(size_t)T_R10 * nE + e]; R11 = T[(size_t)T_R11 * nE + e];
     N = P.vcls[e];
     const double* rb = s_rb + 72 * (N - 3);
 #pragma unroll
     for (int k = 0; k < 12 / kElemLanes; ++k) {
       const int i = lane + kElemLanes * k;
-      const int n = P.pidx[(size_t)i * ne + e];
+      const int n = P.pidx[(size_t)i * nE + e];
       const v3 u = ld3(U, n);
       v3 up = u;
       if (predict) up = pred3(u, ld3(V, n), ld3(Aa, n), dt, a8);
@@ -366,18 +378,18 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, SConst C, int mode
   }
   if (valid) {
     const double* rb = s_rb + 72 * (N - 3);
-    const v3 a1 = add(ldrow3(T, T_A1, ne, e), du1);
-    const v3 a2 = add(ldrow3(T, T_A2, ne, e), du2);
+    const v3 a1 = add(ldrow3(T, T_A1, nE, e), du1);
+    const v3 a2 = add(ldrow3(T, T_A2, nE, e), du2);
     v3 a3 = crs(a1, a2);
     const double jac = nrm(a3);
     a3 = scl(a3, 1 / jac);
-    const v3 a11 = add(ldrow3(T, T_A11, ne, e), du11);
-    const v3 a22 = add(ldrow3(T, T_A22, ne, e), du22);
-    const v3 a12 = add(ldrow3(T, T_A12, ne, e), du12);
+    const v3 a11 = add(ldrow3(T, T_A11, nE, e), du11);
+    const v3 a22 = add(ldrow3(T, T_A22, nE, e), du22);
+    const v3 a12 = add(ldrow3(T, T_A12, nE, e), du12);
     const double ff1[3] = {dot(a1, a1), dot(a2, a2), dot(a1, a2)};
     const double ff2[3] = {dot(a11, a3), dot(a22, a3), dot(a12, a3)};
-    const v3 g = ldrow3(T, T_FF1, ne, e);
-    const v3 fr2 = ldrow3(T, T_FF2, ne, e);
+    const v3 g = ldrow3(T, T_FF1, nE, e);
+    const v3 fr2 = ldrow3(T, T_FF2, nE, e);
     double alpha[3] = {0.5 * (ff1[0] - g.x), 0.5 * (ff1[1] - g.y), 0.5 * (ff1[2] - g.z)};
     double beta[3] = {fr2.x - ff2[0], fr2.y - ff2[1], fr2.z - ff2[2]};
     alpha[2] *= 2;
@@ -405,15 +417,15 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, SConst C, int mode
     const v3 p21 = scl(add(crs(a1, a11), scl(a3xa1, ff2[0])), invjac);
     const v3 p22 = scl(add(crs(a1, a22), scl(a3xa1, ff2[1])), invjac);
     const v3 p23 = scl(add(crs(a1, a12), scl(a3xa1, ff2[2])), invjac);
-    const double JxW = T[(size_t)T_JXW * ne + e];
+    const double JxW = T[(size_t)T_JXW * nE + e];
     const double m2x2 = 2 * mm[2];
     v3 g1 = add(add(add(add(scl(a1, nn[0]), scl(a2, nn[2])), scl(p11, mm[0])), scl(p12, mm[1])), scl(p13, m2x2));
     v3 g2 = add(add(add(add(scl(a2, nn[1]), scl(a1, nn[2])), scl(p21, mm[0])), scl(p22, mm[1])), scl(p23, m2x2));
     const v3 G[6] = {scl(a3, JxW), scl(g1, JxW), scl(g2, JxW), scl(a3, JxW * mm[0]), scl(a3, JxW * mm[1]),
                      scl(a3, JxW * m2x2)};
     /* per patch slot: the internal-force contribution qe_i (f_i -= qe_i),
-     * slot-major so that each node gathers 24 contiguous bytes per slot;
-     * and the pressure vector JxW a3 of the element (f_i += p phi_i JxW a3) */
+     * 24 contiguous bytes per slot for the node gather; and the pressure
+     * vector JxW a3 of the element (f_i += p phi_i JxW a3) */
     double* qrow = P.qe + (size_t)36 * e;
 #pragma unroll
     for (int k = 0; k < 12 / kElemLanes; ++k) {
@@ -450,12 +462,12 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, SConst C, int mode
     __syncthreads();
   }
   if (t < 5) P.epart[(size_t)blockIdx.x * 5 + t] = s_red[t][0];
-  if (!last_block(P.ctr + 0)) return;
+  if (!last_block(P.ctr + kCtrPerShell * sid + 0, C.nbE)) return;
   const int comp[5] = {0, 1, 2, 3, 4}, op[5] = {0, 0, 1, 0, 0};
   double r[5];
-  reduce_records<5>(P.epart, gridDim.x, 5, comp, op, r, &s_red[0][0]);
+  reduce_records<5>(P.epart + (size_t)C.blkE * 5, C.nbE, 5, comp, op, r, &s_red[0][0]);
   if (t == 0) {
-    DShell& w = *P.sh;
+    DShell& w = P.sh[sid];
     w.att_e_tens = r[0];
     w.att_e_bend = r[1];
     w.att_volume = fabs(r[3] / 6) - r[4] * 0.5 * C.h;
@@ -482,11 +494,14 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
  * pair, distance, log-barrier normal force, normal damping and Margolis
  * friction; the force is split -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w)
  * onto element 2 and accumulated with FP64 atomics into fc (the node kernel
- * reads and clears it). */
-__global__ void __launch_bounds__(256) k_contact(DPtrs P, SConst C, int mode)
+ * reads and clears it).  Each shell owns C.nbC blocks that stride over its
+ * pairs. */
+__global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
 {
   __shared__ double s_red[3][256];
-  const DShell& sh = *P.sh;
+  const int sid = uni(P.cshell[blockIdx.x]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
   if (sh.done) return;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
@@ -495,23 +510,25 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, SConst C, int mode)
   const double* V = P.V[par];
   const double* A = P.A[par];
   const int np = sh.n_pairs, buf = sh.pair_buf;
-  const int ne = C.ne;
+  const int nE = P.nE;
   const double h = C.h;
+  const int lb = blockIdx.x - C.blkC;
   double nc = 0, es = 0, ef = 0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+  for (int pl = lb * blockDim.x + threadIdx.x; pl < np; pl += C.nbC * blockDim.x) {
+    const int p = C.pbase + pl;
     const int e[2] = {P.pe1[buf][p], P.pe2[buf][p]};
     v3 X0[3], X1[3];
     int nid[2][3];
     for (int k = 0; k < 3; ++k) {
-      nid[0][k] = P.tri[(size_t)k * ne + e[0]];
-      nid[1][k] = P.tri[(size_t)k * ne + e[1]];
+      nid[0][k] = P.tri[(size_t)k * nE + e[0]];
+      nid[1][k] = P.tri[(size_t)k * nE + e[1]];
       X0[k] = node_pos(P, nid[0][k], U, V, A, predict, dt, a8);
       X1[k] = node_pos(P, nid[1][k], U, V, A, predict, dt, a8);
     }
     tri_hit hit;
     tri_tri(X0, X1, hit);
     P.pd2[buf][p] = hit.d2;
-    if (hit.d2 <= 0) atomicMax(&P.sh->problem, 6); /* CONTACT_PROBLEM */
+    if (hit.d2 <= 0) atomicMax(&P.sh[sid].problem, 6); /* CONTACT_PROBLEM */
     int stick = P.pstick[buf][p];
     v3 so = ld3(P.pso[buf], p);
     if (hit.d2 < C.d2_friction) {
@@ -606,23 +623,23 @@ __device__ double rigid_free_norm(double f2, const double* r)
   return sqrt(fmax(f2 - proj, 0.0));
 }
 
-/* finalize of one step attempt (ShellSystem.C:1006-1117): residual and
- * energies, local-error estimate, accept (state buffer flip, time advance,
- * check_increment, corrector-time broad-phase trigger) or reject with a
- * smaller dt.  Run by the last block of the node kernel. */
-__device__ void finalize(const DPtrs& P, const SConst& C, int mode, int nblkC, double* s)
+/* finalize of one step attempt of a shell (ShellSystem.C:1006-1117):
+ * residual and energies, local-error estimate, accept (state buffer flip,
+ * time advance, check_increment, corrector-time broad-phase trigger) or
+ * reject with a smaller dt.  Run by the last node-kernel block of the shell. */
+__device__ void finalize(const DPtrs& P, const SConst& C, int sid, int mode, double* s)
 {
   int cn[kNodeRec], on[kNodeRec];
   for (int q = 0; q < kNodeRec; ++q) { cn[q] = q; on[q] = (q >= 2 && q <= 4); }
   double rn[kNodeRec];
-  reduce_records<kNodeRec>(P.npart, gridDim.x, kNodeRec, cn, on, rn, s);
+  reduce_records<kNodeRec>(P.npart + (size_t)C.blkN * kNodeRec, C.nbN, kNodeRec, cn, on, rn, s);
   const int cc[3] = {0, 1, 2}, oc[3] = {0, 0, 0};
   double rc[3];
-  reduce_records<3>(P.cpart, nblkC, 3, cc, oc, rc, s);
+  reduce_records<3>(P.cpart + (size_t)C.blkC * 3, C.contact ? C.nbC : 0, 3, cc, oc, rc, s);
   const double f2 = rn[0], fe2 = rn[1], err = rn[2], vmax = rn[3], du = rn[4], ek = rn[5];
   const double nc = rc[0], es = rc[1], ef = rc[2];
   if (threadIdx.x != 0) return;
-  DShell& w = *P.sh;
+  DShell& w = P.sh[sid];
   w.e_tens = w.att_e_tens; w.e_bend = w.att_e_bend; w.e_self = es; w.e_fric = ef;
   w.n_contacts = (int)nc;
   w.fnorm = sqrt(f2);
@@ -678,25 +695,28 @@ __device__ void finalize(const DPtrs& P, const SConst& C, int mode, int nblkC, d
 /* kNodeLanes lanes per node: each lane folds every kNodeLanes-th entry of the
  * node's (element, slot) incidence list (the slot's internal-force
  * contribution and phi_i times the element's JxW a3), the lanes combine by a
- * fixed butterfly,
- * then lane 0 adds the contact force and applies the damped
+ * fixed butterfly, then lane 0 adds the contact force and applies the damped
  * central-difference / Newmark update (ShellSystem.C:1006-1008, 1068-1090):
  * a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a, v = v_pred + gamma dt a,
  * written to the other state buffer.  Block partials: |f|^2, |f_ext|^2,
- * max|a-a_old|, max|v_pred|, max|du|^2, E_kin; the last block finalizes. */
-__global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, SConst C, int mode, double t_force, int nblkC)
+ * max|a-a_old|, max|v_pred|, max|du|^2, E_kin and the rigid-body sums; the
+ * last block of each shell finalizes it.  Force time: t + dt in a step,
+ * t + dt in M_INIT (apply_initial_conditions), DShell::t_eval in M_FORCES. */
+__global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_red[kNodeRec][kNodeBlock];
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
-  const DShell& sh = *P.sh;
+  const int sid = uni(P.vshell[blockIdx.x * kNodePerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
   __syncthreads();
   const int t = threadIdx.x;
   const int lane = t % kNodeLanes;
-  const int n = blockIdx.x * (kNodeBlock / kNodeLanes) + t / kNodeLanes;
-  const bool valid = n < C.nv;
+  const int n = blockIdx.x * kNodePerBlock + t / kNodeLanes;
+  const bool valid = n - C.vbase < C.nv;
   const int par = sh.par;
   const double dt = sh.dt;
   v3 fint = mk(0, 0, 0), fp = fint;
@@ -720,7 +740,7 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, SConst C, int mode
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) rec[q] = 0;
   if (valid && lane == 0) {
-    const double t_eval = (mode == M_STEP) ? sh.t + dt : t_force;
+    const double t_eval = (mode == M_FORCES) ? sh.t_eval : sh.t + dt;
     double ptot = pressure_at(C, t_eval);
     if (C.bulk_modulus != 0) {
       const double Vc = (C.volume_at_prediction || (mode == M_STEP && sh.fresh) || mode == M_INIT) ? sh.att_volume
@@ -791,30 +811,31 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, SConst C, int mode
     for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
     P.npart[(size_t)blockIdx.x * kNodeRec + t] = acc;
   }
-  if (!last_block(P.ctr + 1)) return;
-  finalize(P, C, mode, nblkC, &s_red[0][0]);
+  if (!last_block(P.ctr + kCtrPerShell * sid + 1, C.nbN)) return;
+  finalize(P, C, sid, mode, &s_red[0][0]);
 }
 
 /* ------------------------------------------------------------ broad phase */
 
 /* compute_shell_shell_contact_candidates after its trigger
- * (ShellSystem.C:1922-2113) as four kernels that return at once unless
- * DShell::rebuild is set:
- *  k_bp_box   element AABBs at the positions, tile partials; last block:
- *             global box, averaged element size (serial in 256-element
- *             tiles, then serial over tiles), cell size, d_max, grid size
- *             and the n_cells_max check (compute_bounding_box and
+ * (ShellSystem.C:1922-2113) as four kernels whose blocks return at once
+ * unless DShell::rebuild of their shell is set:
+ *  k_bp_box   element AABBs at the positions, tile partials; last block of
+ *             the shell: global box, averaged element size (serial in
+ *             256-element tiles, then serial over tiles), cell size, d_max,
+ *             grid size and the n_cells_max check (compute_bounding_box and
  *             ShellSystem.C:1933-1964);
- *  k_bp_bin   cell index ranges; (cell, element) entries appended with an
- *             atomic cursor into hashed cell buckets (the reference's
- *             cell_to_elem_map, ShellSystem.C:1967-2010);
- *  k_bp_pairs per element: surviving old pairs (stick or d2 <= d_max^2,
- *             ShellSystem.C:2016-2027) and new candidates from its cells
- *             (counted in the first cell both ranges share, not close, not
- *             listed, boxes within d_max, exact d2 <= d_max^2,
- *             ShellSystem.C:2033-2086), counted; last block: prefix scan;
- *  k_bp_emit  per element: sorted merge into the other pair buffer, buckets
- *             cleared; last block publishes the new list.
+ *  k_bp_bin   one lane per (element, cell): (cell, element) entries appended
+ *             with an atomic cursor into the shell's hashed cell buckets (the
+ *             reference's cell_to_elem_map, ShellSystem.C:1967-2010);
+ *  k_bp_pairs one lane per (element, cell): surviving old pairs (stick or
+ *             d2 <= d_max^2, ShellSystem.C:2016-2027) and new candidates from
+ *             the cell (counted in the first cell both ranges share, not
+ *             close, not listed, boxes within d_max, exact d2 <= d_max^2,
+ *             ShellSystem.C:2033-2086); last block of the shell: prefix scan;
+ *  k_bp_emit  per element: sorted merge into the shell's region of the other
+ *             pair buffer; the shell's buckets cleared; last block publishes
+ *             the new list.
  * posmode M_STEP: positions X + u_pred, otherwise X + u. */
 
 __device__ __forceinline__ bool bp_active(const DShell* sh)
@@ -822,13 +843,13 @@ __device__ __forceinline__ bool bp_active(const DShell* sh)
   return (*(volatile const int*)&sh->rebuild) != 0 && (*(volatile const int*)&sh->err) == 0;
 }
 
-__device__ __forceinline__ double ebox(const DPtrs& P, int ne, int r, int e) { return P.ebox[(size_t)r * ne + e]; }
+__device__ __forceinline__ double ebox(const DPtrs& P, int r, int e) { return P.ebox[(size_t)r * P.nE + e]; }
 
 /* BoundingBox::apart, BoundingBox.h:64-69 */
-__device__ bool boxes_apart(const DPtrs& P, int ne, int a, int b, double d)
+__device__ bool boxes_apart(const DPtrs& P, int a, int b, double d)
 {
   for (int q = 0; q < 3; ++q)
-    if (ebox(P, ne, q, a) > ebox(P, ne, 3 + q, b) + d || ebox(P, ne, q, b) > ebox(P, ne, 3 + q, a) + d) return true;
+    if (ebox(P, q, a) > ebox(P, 3 + q, b) + d || ebox(P, q, b) > ebox(P, 3 + q, a) + d) return true;
   return false;
 }
 
@@ -849,13 +870,15 @@ __device__ __forceinline__ unsigned bucket_of(unsigned cell, unsigned bits)
   return (cell * 0x9E3779B1u) >> (32 - bits);
 }
 
-__global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
+__global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode)
 {
   __shared__ double s_box[6][256];
   __shared__ double s_size[256];
-  DShell* sh = P.sh;
+  const int sid = uni(P.eshell[blockIdx.x * 256]);
+  const SConst& C = P.cs[sid];
+  DShell* sh = P.sh + sid;
   if (!bp_active(sh)) return;
-  const int ne = C.ne, t = threadIdx.x, tile = blockIdx.x;
+  const int nE = P.nE, t = threadIdx.x, tile = blockIdx.x;
   const bool predict = (posmode == M_STEP);
   const double dt = sh->dt, a8 = sh->nm[8];
   const int par = sh->par;
@@ -866,10 +889,10 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
   double bmn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
   double bmx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
   double size = 0;
-  if (e < ne) {
+  if (e - C.ebase < C.ne) {
     bool finite = true;
     for (int k = 0; k < 3; ++k) {
-      const v3 x = node_pos(P, P.tri[(size_t)k * ne + e], U, V, A, predict, dt, a8);
+      const v3 x = node_pos(P, P.tri[(size_t)k * nE + e], U, V, A, predict, dt, a8);
       const double xx[3] = {x.x, x.y, x.z};
       finite = finite && isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
       for (int q = 0; q < 3; ++q) {
@@ -877,7 +900,7 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
         if (xx[q] > bmx[q]) bmx[q] = xx[q];
       }
     }
-    for (int q = 0; q < 3; ++q) { P.ebox[(size_t)q * ne + e] = bmn[q]; P.ebox[(size_t)(3 + q) * ne + e] = bmx[q]; }
+    for (int q = 0; q < 3; ++q) { P.ebox[(size_t)q * nE + e] = bmn[q]; P.ebox[(size_t)(3 + q) * nE + e] = bmx[q]; }
     double dmax = bmx[0] - bmn[0];
     const double d1 = bmx[1] - bmn[1], d2 = bmx[2] - bmn[2];
     if (d1 > dmax) dmax = d1;
@@ -897,18 +920,19 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
   }
   if (t == 0) {
     double sum = 0;
-    const int m = min(256, ne - tile * 256);
+    const int m = min(256, C.ne - (tile - C.tile0) * 256);
     for (int q = 0; q < m; ++q) sum += s_size[q];
     for (int q = 0; q < 6; ++q) P.bpart[(size_t)tile * 7 + q] = s_box[q][0];
     P.bpart[(size_t)tile * 7 + 6] = sum;
   }
-  if (!last_block(P.ctr + 2)) return;
-  /* global box: fixed-order tree of tile partials; size: serial over tiles */
-  const int ntiles = gridDim.x;
+  if (!last_block(P.ctr + kCtrPerShell * sid + 2, C.ntile)) return;
+  /* the shell's box: fixed-order tree of tile partials; size: serial over tiles */
+  const int ntiles = C.ntile;
+  const double* bp = P.bpart + (size_t)C.tile0 * 7;
   for (int q = 0; q < 6; ++q) {
     double acc = q < 3 ? 1.7976931348623157e308 : -1.7976931348623157e308;
     for (int b = t; b < ntiles; b += 256) {
-      const double x = P.bpart[(size_t)b * 7 + q];
+      const double x = bp[(size_t)b * 7 + q];
       acc = q < 3 ? fmin(acc, x) : fmax(acc, x);
     }
     s_box[q][t] = acc;
@@ -925,7 +949,7 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
   double total = 0;
   for (int base = 0; base < ntiles; base += 256) {
     const int b = base + t;
-    s_size[t] = b < ntiles ? P.bpart[(size_t)b * 7 + 6] : 0.0;
+    s_size[t] = b < ntiles ? bp[(size_t)b * 7 + 6] : 0.0;
     __syncthreads();
     if (t == 0)
       for (int q = 0; q < min(256, ntiles - base); ++q) total += s_size[q];
@@ -934,7 +958,7 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, SConst C, int posmode)
   if (t != 0) return;
   sh->du_sum = 0;
   sh->n_rebuilds++;
-  sh->avg_elem_size = total / ne;
+  sh->avg_elem_size = total / C.ne;
   bool ok = isfinite(total);
   double bmn2[3], bmx2[3];
   for (int c = 0; c < 3; ++c) {
@@ -969,25 +993,27 @@ __device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz,
   cx = lo[0] + r / ny;
 }
 
-__global__ void __launch_bounds__(256) k_bp_bin(DPtrs P, SConst C)
+__global__ void __launch_bounds__(256) k_bp_bin(DPtrs P)
 {
-  const DShell* sh = P.sh;
+  const int sid = uni(P.eshell[blockIdx.x * kBpPerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell* sh = P.sh + sid;
   if (!bp_active(sh)) return;
-  const int ne = C.ne;
+  const int nE = P.nE;
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int e = gid / kBpLanes, k0 = gid % kBpLanes;
-  if (e >= ne) return;
+  if (e - C.ebase >= C.ne) return;
   const double cell = sh->cell_size, dmx = sh->d_max;
   const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
   int lo[3], hi[3];
   for (int q = 0; q < 3; ++q) {
-    lo[q] = (int)((ebox(P, ne, q, e) - sh->bmin[q]) / cell);
-    hi[q] = (int)((ebox(P, ne, 3 + q, e) - sh->bmin[q] + dmx) / cell);
+    lo[q] = (int)((ebox(P, q, e) - sh->bmin[q]) / cell);
+    hi[q] = (int)((ebox(P, 3 + q, e) - sh->bmin[q] + dmx) / cell);
   }
   if (k0 == 0)
     for (int q = 0; q < 3; ++q) {
-      P.ecell[(size_t)q * ne + e] = lo[q];
-      P.ecell[(size_t)(3 + q) * ne + e] = hi[q];
+      P.ecell[(size_t)q * nE + e] = lo[q];
+      P.ecell[(size_t)(3 + q) * nE + e] = hi[q];
     }
   const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
   const int total = (hi[0] - lo[0] + 1) * ny * nz;
@@ -995,16 +1021,17 @@ __global__ void __launch_bounds__(256) k_bp_bin(DPtrs P, SConst C)
     int cx, cy, cz;
     bp_cell_of(k, lo, ny, nz, cx, cy, cz);
     const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-    const unsigned b = bucket_of(c, P.hb_bits);
+    const size_t b = C.bbase + bucket_of(c, C.hb_bits);
     const unsigned pos = atomicAdd(&P.bcount[b], 1u);
-    if (pos < (unsigned)kBucketCap) P.bent[(size_t)b * kBucketCap + pos] = ((uint64_t)c << 32) | (uint32_t)e;
-    else atomicOr(&P.sh->err, ERR_ENTRIES);
+    if (pos < (unsigned)kBucketCap) P.bent[b * kBucketCap + pos] = ((uint64_t)c << 32) | (uint32_t)e;
+    else atomicOr(&P.sh[sid].err, ERR_ENTRIES);
   }
 }
 
-/* exclusive scan of the row lengths surv[e] + min(nnew[e], kRowCap),
- * e < n, into out[0..n], out[n] = total, by one block */
-__device__ void block_scan_rows(const int32_t* surv, const int32_t* nnew, int32_t* out, int n, int32_t* s)
+/* exclusive scan of the row lengths surv[i] + min(nnew[i], kRowCap), i < n,
+ * into out[0..n] offset by base, out[n] = base + total, by one block */
+__device__ void block_scan_rows(const int32_t* surv, const int32_t* nnew, int32_t* out, int n, int base,
+                                int32_t* s)
 {
   const int t = threadIdx.x, bs = blockDim.x;
   const int chunk = (n + bs - 1) / bs;
@@ -1014,7 +1041,7 @@ __device__ void block_scan_rows(const int32_t* surv, const int32_t* nnew, int32_
   s[t] = sum;
   __syncthreads();
   if (t == 0) {
-    int32_t acc = 0;
+    int32_t acc = base;
     for (int q = 0; q < bs; ++q) { const int32_t x = s[q]; s[q] = acc; acc += x; }
     out[n] = acc;
   }
@@ -1029,12 +1056,14 @@ __device__ void block_scan_rows(const int32_t* surv, const int32_t* nnew, int32_
  * tested by exactly one lane.  New pairs are appended to e1's found row
  * with an atomic cursor (the row is sorted in k_bp_emit, so the list does
  * not depend on the order of arrival). */
-__global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode)
+__global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode)
 {
   __shared__ int32_t s_scan[256];
-  DShell* sh = P.sh;
+  const int sid = uni(P.eshell[blockIdx.x * kBpPerBlock]);
+  const SConst& C = P.cs[sid];
+  DShell* sh = P.sh + sid;
   if (!bp_active(sh)) return;
-  const int ne = C.ne;
+  const int nE = P.nE;
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int e1 = gid / kBpLanes, k0 = gid % kBpLanes;
   const bool predict = (posmode == M_STEP);
@@ -1046,12 +1075,12 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode
   const double dmx = sh->d_max, d2mx = sh->d2_max;
   const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
   const int ob = sh->pair_buf;
-  int32_t* nnew = P.found_cnt + ne + 1;
-  if (e1 < ne) {
+  int32_t* nnew = P.found_cnt + nE + 1;
+  if (e1 - C.ebase < C.ne) {
     const int* L = P.ecell;
-    const int lo1[3] = {L[e1], L[(size_t)ne + e1], L[(size_t)2 * ne + e1]};
-    const int hi1[3] = {L[(size_t)3 * ne + e1], L[(size_t)4 * ne + e1], L[(size_t)5 * ne + e1]};
-    const int r0 = P.row_start[ob][e1], r1 = P.row_start[ob][e1 + 1];
+    const int lo1[3] = {L[e1], L[(size_t)nE + e1], L[(size_t)2 * nE + e1]};
+    const int hi1[3] = {L[(size_t)3 * nE + e1], L[(size_t)4 * nE + e1], L[(size_t)5 * nE + e1]};
+    const int r0 = P.row_start[ob][e1 + sid], r1 = P.row_start[ob][e1 + 1 + sid];
     if (k0 == 0) {
       int nsurv = 0;
       for (int r = r0; r < r1; ++r)
@@ -1066,16 +1095,16 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode
       int cx, cy, cz;
       bp_cell_of(k, lo1, ny, nz, cx, cy, cz);
       const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-      const unsigned b = bucket_of(c, P.hb_bits);
+      const size_t b = C.bbase + bucket_of(c, C.hb_bits);
       const unsigned cnt = min(P.bcount[b], (unsigned)kBucketCap);
       for (unsigned q = 0; q < cnt; ++q) {
-        const uint64_t ent = P.bent[(size_t)b * kBucketCap + q];
+        const uint64_t ent = P.bent[b * kBucketCap + q];
         if ((unsigned)(ent >> 32) != c) continue;
         const int e2 = (int)(uint32_t)ent;
         if (e2 <= e1) continue;
         /* count the pair once: in the first cell the two ranges share */
-        if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)ne + e2]) ||
-            cz != max(lo1[2], L[(size_t)2 * ne + e2]))
+        if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)nE + e2]) ||
+            cz != max(lo1[2], L[(size_t)2 * nE + e2]))
           continue;
         if (close_pair(P, e1, e2)) continue;
         bool listed = false;
@@ -1089,13 +1118,13 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode
           }
         }
         if (listed) continue;
-        if (boxes_apart(P, ne, e1, e2, dmx)) continue;
+        if (boxes_apart(P, e1, e2, dmx)) continue;
         if (!have_x0) {
-          for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * ne + e1], U, V, A, predict, dt, a8);
+          for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * nE + e1], U, V, A, predict, dt, a8);
           have_x0 = true;
         }
         v3 X1[3];
-        for (int j = 0; j < 3; ++j) X1[j] = node_pos(P, P.tri[(size_t)j * ne + e2], U, V, A, predict, dt, a8);
+        for (int j = 0; j < 3; ++j) X1[j] = node_pos(P, P.tri[(size_t)j * nE + e2], U, V, A, predict, dt, a8);
         tri_hit hit;
         tri_tri(X0, X1, hit);
         if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
@@ -1108,22 +1137,25 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, SConst C, int posmode
       }
     }
   }
-  if (!last_block(P.ctr + 3)) return;
-  block_scan_rows(P.found_cnt, nnew, P.row_start[1 - ob], ne, s_scan);
-  if (threadIdx.x == 0 && P.row_start[1 - ob][ne] > P.pair_cap) atomicOr(&sh->err, ERR_PAIRS);
+  if (!last_block(P.ctr + kCtrPerShell * sid + 3, C.nbL)) return;
+  int32_t* rs = P.row_start[1 - ob] + C.ebase + sid;
+  block_scan_rows(P.found_cnt + C.ebase, nnew + C.ebase, rs, C.ne, C.pbase, s_scan);
+  if (threadIdx.x == 0 && rs[C.ne] - C.pbase > C.pair_cap) atomicOr(&sh->err, ERR_PAIRS);
 }
 
-__global__ void __launch_bounds__(256) k_bp_emit(DPtrs P, SConst C)
+__global__ void __launch_bounds__(256) k_bp_emit(DPtrs P)
 {
-  DShell* sh = P.sh;
+  const int sid = uni(P.eshell[blockIdx.x * 256]);
+  const SConst& C = P.cs[sid];
+  DShell* sh = P.sh + sid;
   if (!bp_active(sh)) return;
-  const int ne = C.ne;
+  const int nE = P.nE;
   const int e1 = blockIdx.x * 256 + threadIdx.x;
   const int ob = sh->pair_buf, nb = 1 - ob;
   const double d2mx = sh->d2_max;
-  if (e1 < ne) {
-    const int nf = min(P.found_cnt[(size_t)ne + 1 + e1], kRowCap);
-    P.found_cnt[(size_t)ne + 1 + e1] = 0;
+  if (e1 - C.ebase < C.ne) {
+    const int nf = min(P.found_cnt[(size_t)nE + 1 + e1], kRowCap);
+    P.found_cnt[(size_t)nE + 1 + e1] = 0;
     int32_t* fe = P.found_e2 + (size_t)e1 * kRowCap;
     double* fd = P.found_d2 + (size_t)e1 * kRowCap;
     for (int a = 1; a < nf; ++a) { /* insertion sort by e2 */
@@ -1133,8 +1165,8 @@ __global__ void __launch_bounds__(256) k_bp_emit(DPtrs P, SConst C)
       while (b >= 0 && fe[b] > ke) { fe[b + 1] = fe[b]; fd[b + 1] = fd[b]; --b; }
       fe[b + 1] = ke; fd[b + 1] = kd;
     }
-    const int r0 = P.row_start[ob][e1], r1 = P.row_start[ob][e1 + 1];
-    int w = P.row_start[nb][e1];
+    const int r0 = P.row_start[ob][e1 + sid], r1 = P.row_start[ob][e1 + 1 + sid];
+    int w = P.row_start[nb][e1 + sid];
     int r = r0, f = 0;
     while (true) {
       while (r < r1 && !(P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx))) ++r;
@@ -1154,14 +1186,15 @@ __global__ void __launch_bounds__(256) k_bp_emit(DPtrs P, SConst C)
       ++w;
     }
   }
-  /* clear every bucket count (a coalesced sweep; cheaper than revisiting
-   * each element's cells) */
-  const unsigned nbk = 1u << P.hb_bits;
-  for (unsigned b = blockIdx.x * 256 + threadIdx.x; b < nbk; b += gridDim.x * 256) P.bcount[b] = 0;
-  if (!last_block(P.ctr + 4)) return;
+  /* clear the shell's bucket counts (a coalesced sweep; cheaper than
+   * revisiting each element's cells) */
+  const unsigned nbk = 1u << C.hb_bits;
+  for (unsigned b = (blockIdx.x - C.tile0) * 256 + threadIdx.x; b < nbk; b += C.ntile * 256)
+    P.bcount[C.bbase + b] = 0;
+  if (!last_block(P.ctr + kCtrPerShell * sid + 4, C.ntile)) return;
   if (threadIdx.x == 0) {
     sh->pair_buf = nb;
-    sh->n_pairs = P.row_start[nb][ne];
+    sh->n_pairs = P.row_start[nb][C.ebase + C.ne + sid] - C.pbase;
     sh->rebuild = 0;
   }
 }
