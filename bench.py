@@ -38,7 +38,7 @@ def workload(level):
     X = radial_imperfection(X, 0.01 * h, seed=2024)
     pcr = p_critical(E1, h, nu)
     params = dict(E1=E1, E2=E2, nu=nu, G12=G12, thickness=h, rho=1.0, damping_alpha=0.05,
-                  contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0)
+                  contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0)
     return X, T, params, pcr
 
 

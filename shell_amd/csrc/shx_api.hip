@@ -24,7 +24,6 @@ using namespace shx;
 
 namespace {
 constexpr int kGraphAttempts = 64;   /* step attempts per replayed graph */
-constexpr int kMaxContactBlocks = 120;
 }
 
 struct shx_ctx {
@@ -36,7 +35,7 @@ struct shx_ctx {
   std::vector<SConst> C;       /* host copy of the device array */
   DPtrs P{};
   SConst* dC = nullptr;
-  int nblkE = 0, nblkN = 0, nblkC = 0, ntile = 0, nblkL = 0;
+  int nblkE = 0, nblkN = 0, ntile = 0;
   bool any_contact = false;
   std::vector<void*> allocs;
   std::vector<double> rawb;
@@ -158,17 +157,47 @@ void prof_collect(shx_ctx* ctx)
   ctx->evclass.clear();
 }
 
-/* the broad phase: four kernels whose blocks return at once unless the
- * device-side trigger (or a forced rebuild) set DShell::rebuild of their
- * shell */
-void launch_broad(shx_ctx* ctx, int posmode)
+/* the broad phase of the shells on work list `list`: fixed grids that
+ * walk only the listed shells' work (an empty list costs five near-empty
+ * launches) */
+void launch_broad(shx_ctx* ctx, int posmode, int list)
 {
   const DPtrs& P = ctx->P;
   prof_begin(ctx, 4);
-  k_bp_box<<<ctx->ntile, 256, 0, ctx->st>>>(P, posmode);
-  k_bp_bin<<<ctx->nblkL, 256, 0, ctx->st>>>(P);
-  k_bp_pairs<<<ctx->nblkL, 256, 0, ctx->st>>>(P, posmode);
-  k_bp_emit<<<ctx->ntile, 256, 0, ctx->st>>>(P);
+  k_bp_box<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  k_bp_count<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
+  k_bp_scan<<<ctx->S, 256, 0, ctx->st>>>(P, list);
+  k_bp_scatter<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
+  k_bp_pairs<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  prof_end(ctx);
+}
+
+/* element force + per-shell epilogue (energies, volume, predictor trigger) */
+void launch_elem(shx_ctx* ctx, int mode)
+{
+  prof_begin(ctx, 0);
+  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(ctx->P, mode);
+  prof_end(ctx);
+  prof_begin(ctx, 3);
+  k_fin_elem<<<ctx->S, 256, 0, ctx->st>>>(ctx->P, mode);
+  prof_end(ctx);
+}
+
+/* node gather + Newmark + per-shell finalize (accept/reject, corrector trigger) */
+void launch_node(shx_ctx* ctx, int mode)
+{
+  prof_begin(ctx, 1);
+  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
+  prof_end(ctx);
+  prof_begin(ctx, 3);
+  k_fin_node<<<ctx->S, 256, 0, ctx->st>>>(ctx->P, mode);
+  prof_end(ctx);
+}
+
+void launch_contact(shx_ctx* ctx, int mode)
+{
+  prof_begin(ctx, 2);
+  k_contact<<<ctx->ntile, 256, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
 }
 
@@ -177,21 +206,13 @@ void launch_broad(shx_ctx* ctx, int posmode)
  * (+ finalize), corrector-time broad phase */
 void enqueue_attempt(shx_ctx* ctx)
 {
-  const DPtrs& P = ctx->P;
-  hipStream_t st = ctx->st;
-  prof_begin(ctx, 0);
-  k_elem<<<ctx->nblkE, kElemBlock, 0, st>>>(P, M_STEP);
-  prof_end(ctx);
+  launch_elem(ctx, M_STEP);
   if (ctx->any_contact) {
-    launch_broad(ctx, M_STEP);
-    prof_begin(ctx, 2);
-    k_contact<<<ctx->nblkC, 256, 0, st>>>(P, M_STEP);
-    prof_end(ctx);
+    launch_broad(ctx, M_STEP, 0);
+    launch_contact(ctx, M_STEP);
   }
-  prof_begin(ctx, 1);
-  k_node<<<ctx->nblkN, kNodeBlock, 0, st>>>(P, M_STEP);
-  prof_end(ctx);
-  if (ctx->any_contact) launch_broad(ctx, M_FORCES);
+  launch_node(ctx, M_STEP);
+  if (ctx->any_contact) launch_broad(ctx, M_FORCES, 1);
 }
 
 /* kGraphAttempts attempts captured once and replayed */
@@ -317,6 +338,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
                      const int32_t* tri, const shx_params* prm, int device)
 {
   if (!out || nshell <= 0 || !nv || !ne || !X || !tri || !prm) { g_create_err = "invalid argument"; return SHX_E_ARG; }
+  if (nshell > kMaxShells) { g_create_err = "more than " + std::to_string(kMaxShells) + " shells"; return SHX_E_ARG; }
   *out = nullptr;
   for (int s = 0; s < nshell; ++s) {
     if (nv[s] <= 0 || ne[s] <= 0) { g_create_err = "invalid mesh size of shell " + std::to_string(s); return SHX_E_ARG; }
@@ -341,9 +363,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     xoff[s + 1] = xoff[s] + (size_t)3 * nv[s];
     toff[s + 1] = toff[s] + (size_t)3 * ne[s];
   }
-  long long vb = 0, eb = 0, pb = 0;
-  int cb = 0;
-  unsigned long long bb = 0;
+  long long vb = 0, eb = 0;
+  unsigned long long bb = 0, entb = 0;
   for (int s = 0; s < nshell; ++s) {
     std::string terr;
     if (!build_topology(nv[s], ne[s], tri + toff[s], ctx->T[s], terr)) {
@@ -358,26 +379,23 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     C.tile0 = (int)(eb / 256); C.ntile = nep / 256;
     C.nbL = nep / kBpPerBlock;
     if (C.contact) {
-      C.blkC = cb;
-      C.nbC = std::max(1, std::min(kMaxContactBlocks, ne[s] / 64));
-      cb += C.nbC;
-      /* hashed cell grid: about four entries per bucket for the expected
+      /* hashed cell grid: about two entries per bucket for the expected
        * number of (cell, element) entries, (1/cell_size_mult + 2)^3 each */
       const double kc = std::pow(1.0 / prm[s].cell_size_mult + 2.0, 3.0);
-      const double want = std::max(4096.0, (double)ne[s] * kc / 4.0);
-      unsigned bits = 12;
+      const double want = std::max(1024.0, (double)ne[s] * kc / 2.0);
+      unsigned bits = 10;
       while ((double)(1u << bits) < want && bits < 24) ++bits;
       C.hb_bits = bits;
       C.bbase = (unsigned)bb;
       bb += 1ull << bits;
-      C.pbase = (int)pb;
-      C.pair_cap = 16 * ne[s] + 4096;
-      pb += C.pair_cap;
+      C.entbase = (unsigned)entb;
+      entb += (unsigned long long)ne[s] * kEntPerElem;
+      ctx->any_contact = true;
     }
     ctx->C[s] = C;
     vb += nvp;
     eb += nep;
-    if (vb > (1ll << 26) || eb > (1ll << 26) || pb > (1ll << 30) || bb > (1ull << 31)) {
+    if (vb > (1ll << 26) || eb > (1ll << 26) || bb > (1ull << 31) || entb > (1ull << 32) - 1) {
       ctx->err = "batch too large for 32-bit global indices";
       return fail(SHX_E_ARG);
     }
@@ -386,17 +404,13 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   ctx->nblkE = nE / kElemPerBlock;
   ctx->nblkN = nV / kNodePerBlock;
   ctx->ntile = nE / 256;
-  ctx->nblkL = nE / kBpPerBlock;
-  ctx->nblkC = cb;
-  ctx->any_contact = cb > 0;
 
   /* Loop basis at the quadrature point, valences 3..6 */
   ctx->rawb.assign(4 * 72, 0.0);
   for (int val = 3; val <= 6; ++val) loop_basis_at_centroid(val, &ctx->rawb[(val - 3) * 72]);
 
   /* global structure-of-arrays copies of the topology */
-  std::vector<int32_t> pidx((size_t)12 * nE), triS((size_t)3 * nE), eshell(nE), vshell(nV),
-      cshell(std::max(cb, 1), 0);
+  std::vector<int32_t> pidx((size_t)12 * nE), triS((size_t)3 * nE), eshell(nE), vshell(nV);
   std::vector<int8_t> vcls(nE, 6);
   std::vector<double> Xh((size_t)3 * nV, 0.0);
   std::vector<int32_t> inc_start((size_t)nV + 1, 0), inc;
@@ -421,7 +435,6 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     }
     for (int q = 0; q < nvp; ++q) vshell[C.vbase + q] = s;
     for (int q = 0; q < 3 * nv[s]; ++q) Xh[(size_t)3 * C.vbase + q] = X[xoff[s] + q];
-    for (int c = 0; c < C.nbC; ++c) cshell[C.blkC + c] = s;
   }
   /* node incidence and contact exclusion as global CSR lists */
   for (int s = 0; s < nshell; ++s) {
@@ -453,28 +466,28 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   cstart[nE] = (int)cids.size();
 
   DPtrs& P = ctx->P;
-  P.nE = nE; P.nV = nV;
+  P.nE = nE; P.nV = nV; P.S = nshell;
   int r = 0;
   double *dX, *drb;
-  int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcsh, *dcs, *dci;
+  int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci;
   int8_t* dv;
   if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
       (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, inc_start)) ||
       (r = upload(ctx, di, inc)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell)) ||
-      (r = upload(ctx, dcsh, cshell)) || (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)))
+      (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)))
     return fail(r);
   P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
-  P.eshell = des; P.vshell = dvs; P.cshell = dcsh; P.close_start = dcs; P.close_ids = dci;
+  P.eshell = des; P.vshell = dvs; P.close_start = dcs; P.close_ids = dci;
   if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
   P.cs = ctx->dC;
   if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * nE)) || (r = dalloc(ctx, P.minv, nV)) ||
       (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.qe, (size_t)36 * nE)) ||
       (r = dalloc(ctx, P.g0, (size_t)3 * nE)) || (r = dalloc(ctx, P.area, nE)) ||
-      (r = dalloc(ctx, P.ctr, (size_t)kCtrPerShell * nshell)) ||
       (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
-      (r = dalloc(ctx, P.cpart, (size_t)3 * std::max(cb, 1))) || (r = dalloc(ctx, P.sh, nshell)))
+      (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
+      (r = dalloc(ctx, P.wl[0], (size_t)nshell + 1)) || (r = dalloc(ctx, P.wl[1], (size_t)nshell + 1)))
     return fail(r);
   for (int b = 0; b < 2; ++b)
     if ((r = dalloc(ctx, P.U[b], (size_t)3 * nV)) || (r = dalloc(ctx, P.V[b], (size_t)3 * nV)) ||
@@ -482,27 +495,16 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       return fail(r);
   if (ctx->any_contact) {
     if ((r = dalloc(ctx, P.ebox, (size_t)6 * nE)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ctx->ntile)) ||
-        (r = dalloc(ctx, P.ecell, (size_t)6 * nE)) || (r = dalloc(ctx, P.bcount, (size_t)bb)) ||
-        (r = dalloc(ctx, P.bent, (size_t)bb * kBucketCap)) ||
-        (r = dalloc(ctx, P.found_cnt, (size_t)2 * (nE + 1))) ||
-        (r = dalloc(ctx, P.found_e2, (size_t)kRowCap * nE)) || (r = dalloc(ctx, P.found_d2, (size_t)kRowCap * nE)))
+        (r = dalloc(ctx, P.ecell, (size_t)6 * nE)) || (r = dalloc(ctx, P.bcnt, (size_t)bb)) ||
+        (r = dalloc(ctx, P.boff, (size_t)bb)) || (r = dalloc(ctx, P.bcur, (size_t)bb)) ||
+        (r = dalloc(ctx, P.bent, (size_t)entb)))
       return fail(r);
+    const size_t slots = (size_t)nE * kRowCap;
     for (int b = 0; b < 2; ++b)
-      if ((r = dalloc(ctx, P.row_start[b], (size_t)nE + nshell)) || (r = dalloc(ctx, P.pe1[b], (size_t)pb)) ||
-          (r = dalloc(ctx, P.pe2[b], (size_t)pb)) || (r = dalloc(ctx, P.pd2[b], (size_t)pb)) ||
-          (r = dalloc(ctx, P.pstick[b], (size_t)pb)) || (r = dalloc(ctx, P.pso[b], (size_t)3 * pb)))
+      if ((r = dalloc(ctx, P.rcnt[b], (size_t)nE)) || (r = dalloc(ctx, P.pe2[b], slots)) ||
+          (r = dalloc(ctx, P.pd2[b], slots)) || (r = dalloc(ctx, P.pstick[b], slots)) ||
+          (r = dalloc(ctx, P.pso[b], 3 * slots)))
         return fail(r);
-    /* empty pair lists: every row of a shell starts at its region base */
-    std::vector<int32_t> rs0((size_t)nE + nshell, 0);
-    for (int s = 0; s < nshell; ++s) {
-      const SConst& C = ctx->C[s];
-      for (int q = 0; q <= ne[s]; ++q) rs0[(size_t)C.ebase + q + s] = C.pbase;
-    }
-    for (int b = 0; b < 2; ++b)
-      if (hipMemcpy(P.row_start[b], rs0.data(), sizeof(int32_t) * rs0.size(), hipMemcpyHostToDevice) != hipSuccess) {
-        ctx->err = "row_start upload failed";
-        return fail(SHX_E_HIP);
-      }
   }
 
   k_precompute<<<ctx->ntile, 256, 0, ctx->st>>>(P);
@@ -589,9 +591,9 @@ int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* 
   HIPCHK(hipMemcpy(ctx->P.U[par], u_pred, bytes, hipMemcpyHostToDevice));
   if (v) HIPCHK(hipMemcpy(ctx->P.V[par], v, bytes, hipMemcpyHostToDevice));
   const DPtrs& P = ctx->P;
-  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, M_FORCES);
-  if (ctx->any_contact) k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, M_FORCES);
-  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, M_FORCES);
+  launch_elem(ctx, M_FORCES);
+  if (ctx->any_contact) launch_contact(ctx, M_FORCES);
+  launch_node(ctx, M_FORCES);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->st));
   if (f) HIPCHK(hipMemcpy(f, P.fout, bytes, hipMemcpyDeviceToHost));
@@ -616,12 +618,14 @@ int shx_contact_rebuild(shx_ctx* ctx, int force)
     s[i].done = 0;
   }
   if (int r = write_sh(ctx, s)) return r;
-  launch_broad(ctx, M_FORCES);
+  k_bp_host_plan<<<ctx->S, 256, 0, ctx->st>>>(ctx->P);
+  launch_broad(ctx, M_FORCES, 0);
+  k_bp_host_done<<<ctx->S, 256, 0, ctx->st>>>(ctx->P);
   HIPCHK(hipGetLastError());
   if (int r = read_sh(ctx, s)) return r;
   prof_collect(ctx);
   if (int r = check_err(ctx, s)) return r;
-  return s[0].n_pairs;
+  return shx_batch_pairs(ctx, 0, nullptr, nullptr, 0);
 }
 
 int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
@@ -636,13 +640,12 @@ int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
     d.rebuild = ctx->C[i].contact ? 1 : 0;
   }
   if (int r = write_sh(ctx, s)) return r;
-  const DPtrs& P = ctx->P;
-  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(P, M_INIT);
+  launch_elem(ctx, M_INIT);
   if (ctx->any_contact) {
-    launch_broad(ctx, M_INIT);
-    k_contact<<<ctx->nblkC, 256, 0, ctx->st>>>(P, M_INIT);
+    launch_broad(ctx, M_INIT, 0);
+    launch_contact(ctx, M_INIT);
   }
-  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(P, M_INIT);
+  launch_node(ctx, M_INIT);
   HIPCHK(hipGetLastError());
   if (int r = read_sh(ctx, s)) return r;
   prof_collect(ctx);
@@ -667,7 +670,10 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
       }
     if (cchange)
       if (int r = write_consts(ctx, ctx->C)) return r;
-    for (auto& d : s) { d.steps_target = d.n_steps + n_steps; d.done = (d.problem >= 7 || d.err) ? 1 : 0; }
+    for (auto& d : s) {
+      d.steps_target = d.n_steps + n_steps;
+      d.done = (d.problem >= 7 || d.err || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
+    }
     if (int r = write_sh(ctx, s)) return r;
     const long long max_attempts = 64LL * n_steps + 4096;
     long long enq = 0;
@@ -722,18 +728,27 @@ int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap
   if (!C.contact) return 0;
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  const int n = std::min(s[shell].n_pairs, cap), b = s[shell].pair_buf;
-  if (n > 0) {
-    std::vector<int32_t> tmp(n);
-    for (int k = 0; k < 2; ++k) {
-      uint32_t* dst = k ? e2 : e1;
-      if (!dst) continue;
-      HIPCHK(hipMemcpy(tmp.data(), (k ? ctx->P.pe2[b] : ctx->P.pe1[b]) + C.pbase, sizeof(int32_t) * n,
-                       hipMemcpyDeviceToHost));
-      for (int q = 0; q < n; ++q) dst[q] = (uint32_t)(tmp[q] - C.ebase);
+  const int b = s[shell].pair_buf;
+  std::vector<int32_t> cnt(C.ne);
+  HIPCHK(hipMemcpy(cnt.data(), ctx->P.rcnt[b] + C.ebase, sizeof(int32_t) * C.ne, hipMemcpyDeviceToHost));
+  int n = 0;
+  for (int& c : cnt) { c = std::min(c, kRowCap); n += c; }
+  if ((e1 || e2) && cap > 0 && n > 0) {
+    /* rows in element order, each sorted by e2: the list sorted by (eid1, eid2) */
+    std::vector<int32_t> rows((size_t)C.ne * kRowCap);
+    HIPCHK(hipMemcpy(rows.data(), ctx->P.pe2[b] + (size_t)C.ebase * kRowCap, sizeof(int32_t) * rows.size(),
+                     hipMemcpyDeviceToHost));
+    int w = 0;
+    for (int e = 0; e < C.ne && w < cap; ++e) {
+      int32_t* r = &rows[(size_t)e * kRowCap];
+      std::sort(r, r + cnt[e]);
+      for (int k = 0; k < cnt[e] && w < cap; ++k, ++w) {
+        if (e1) e1[w] = (uint32_t)e;
+        if (e2) e2[w] = (uint32_t)(r[k] - C.ebase);
+      }
     }
   }
-  return s[shell].n_pairs;
+  return n;
 }
 
 int shx_pairs(shx_ctx* ctx, uint32_t* e1, uint32_t* e2, int cap) { return shx_batch_pairs(ctx, 0, e1, e2, cap); }
@@ -821,8 +836,8 @@ int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
   std::vector<double> lam(S, 0.0);
   auto fint = [&](const std::vector<double>& u, std::vector<double>& f) -> int {
     HIPCHK(hipMemcpy(ctx->P.U[0], u.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
-    k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(ctx->P, M_FORCES);
-    k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(ctx->P, M_FORCES);
+    launch_elem(ctx, M_FORCES);
+    launch_node(ctx, M_FORCES);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(f.data(), ctx->P.fout, sizeof(double) * n3, hipMemcpyDeviceToHost));
     return 0;

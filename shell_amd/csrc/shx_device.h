@@ -20,9 +20,14 @@
  *    the node kernel through the node incidence list, so no internal force
  *    is ever scattered with atomics;
  *  - per shell: constants (SConst), mutable scalars (DShell), block partial
- *    records, last-block counters, a region of hashed cell buckets and a
- *    region of the two contact pair buffers (pairs sorted by eid1 then eid2,
- *    with per-pair friction state).
+ *    records (reduced by one-block-per-shell epilogue kernels: no kernel
+ *    synchronises across its own blocks) and a region of hashed cell
+ *    buckets;
+ *  - contact candidates as fixed-capacity rows per element e1 (the other
+ *    element e2 > e1, the last distance and the friction state of each
+ *    pair), twice (ping-pong; a rebuild writes the other copy);
+ *  - two work lists of the shells whose contact lists are being rebuilt
+ *    (predictor and corrector trigger), which size the broad-phase work.
  */
 #ifndef SHX_DEVICE_H
 #define SHX_DEVICE_H
@@ -46,11 +51,13 @@ constexpr int kNodeLanes = 4;       /* lanes cooperating on one node */
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block: 64 nodes */
 constexpr int kNodePerBlock = kNodeBlock / kNodeLanes;
 constexpr int kNodeRec = 22;        /* node-kernel partial record length */
-constexpr int kBucketCap = 64;      /* (cell, element) entries per hash bucket */
+constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
 constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
 constexpr int kBpPerBlock = 256 / kBpLanes;
-constexpr int kRowCap = 64;         /* new candidates per element and rebuild */
-constexpr int kCtrPerShell = 8;     /* last-block counters per shell */
+constexpr int kRowCap = 64;         /* contact candidates per element row */
+constexpr int kCRec = 4;            /* contact-kernel partial record length */
+constexpr int kMaxShells = 2048;    /* shells per batch (broad-phase work maps are staged in LDS) */
+constexpr int kBpGrid = 1024;       /* blocks of the broad-phase kernels (they loop over the work) */
 
 /* per-shell constants (device array, read-only in the step kernels) */
 struct SConst {
@@ -59,11 +66,10 @@ struct SConst {
   int vbase, ebase;                /* first global node / element */
   int blkE, nbE;                   /* element-kernel blocks (kElemPerBlock elements each) */
   int blkN, nbN;                   /* node-kernel blocks (kNodePerBlock nodes each) */
-  int blkC, nbC;                   /* contact-kernel blocks (0 without contact) */
-  int tile0, ntile;                /* 256-element tiles (broad-phase box / emit blocks) */
+  int tile0, ntile;                /* 256-element tiles (box and contact kernel blocks) */
   int nbL;                         /* broad-phase lane blocks (kBpPerBlock elements each) */
-  int pbase, pair_cap;             /* pair-buffer region */
   unsigned bbase, hb_bits;         /* hash-bucket region: 1 << hb_bits buckets from bbase */
+  unsigned entbase;                /* (cell, element) entry region: ne * kEntPerElem entries */
   /* physics (update_derived, ShellSystem.C:421-487) */
   double h, rho;
   double E1, E2, nu, G12, nu2;     /* nu2 = E1 nu / E2 (ShellSystem.C:1199) */
@@ -96,7 +102,7 @@ struct DShell {
   int problem, err;
   int n_steps, n_rejected, n_rebuilds, converged_step, n_contacts;
   int nd[3], n_cells;
-  int n_pairs, pair_buf, n_entries;
+  int n_pairs, pair_buf, pair_src, bclear;
   int steps_target, done, attempts;
   /* results of the element kernel of the current attempt */
   double att_e_tens, att_e_bend, att_volume, att_area;
@@ -109,11 +115,11 @@ enum { ERR_ENTRIES = 1, ERR_PAIRS = 2, ERR_ROW = 4, ERR_CELLS = 8 };
 /* device pointers (kernel argument); element and node indices are global */
 struct DPtrs {
   int nE, nV;                 /* padded global element / node counts (SoA strides) */
+  int S;                      /* shells in the batch (work-list capacity) */
   const SConst* cs;           /* S */
   DShell* sh;                 /* S */
   const int32_t* eshell;      /* nE: shell of each (padded) element */
   const int32_t* vshell;      /* nV: shell of each (padded) node */
-  const int32_t* cshell;      /* contact-kernel block -> shell */
   const double* X;            /* nV*3 */
   const int32_t* pidx;        /* 12*nE */
   const int8_t* vcls;         /* nE: valence */
@@ -129,31 +135,29 @@ struct DPtrs {
   double* A[2];
   double* qe;                 /* 36*nE: per element and patch slot, internal-force contribution */
   double* g0;                 /* 3*nE: JxW a3 per element (pressure per unit p) */
-  unsigned* ctr;              /* kCtrPerShell*S last-block counters: elem, node, bp_box, bp_pairs, bp_emit */
   double* area;               /* nE */
   double* fc;                 /* nV*3 contact force accumulator */
   double* fout;               /* nV*3 outputs of a plain force evaluation */
   double* fext_out;
   double* epart;              /* element kernel block partials (5 per block) */
   double* npart;              /* node kernel block partials (kNodeRec per block) */
-  double* cpart;              /* contact kernel block partials (3 per block) */
+  double* cpart;              /* contact kernel block partials (kCRec per 256-element tile) */
   /* contact */
   const int32_t* close_start; /* nE+1 */
   const int32_t* close_ids;
   double* ebox;               /* 6*nE, SoA */
   double* bpart;              /* broad-phase tile partials: 7 per tile */
   int32_t* ecell;             /* 6*nE: cell index ranges lo[3], hi[3] */
-  uint32_t* bcount;           /* hash buckets of the cell grids: fill counts */
-  uint64_t* bent;             /* bucket entries (cell << 32 | global element) */
-  int32_t* found_cnt;         /* 2*(nE+1): surviving-pair counts, new-pair counts */
-  int32_t* found_e2;          /* nE*kRowCap */
-  double* found_d2;           /* nE*kRowCap */
-  int32_t* row_start[2];      /* nE+S, per pair buffer: rows of shell s at index e + s */
-  int32_t* pe1[2];
-  int32_t* pe2[2];
+  uint32_t* bcnt;             /* hash buckets of the cell grids: entry counts (zero between rebuilds) */
+  uint32_t* boff;             /* bucket start in the shell's entry region (counting sort) */
+  uint32_t* bcur;             /* bucket fill cursor; bucket end after the scatter */
+  uint64_t* bent;             /* entries sorted by bucket (cell << 32 | global element) */
+  int32_t* wl[2];             /* S+1 each: count, then the shells rebuilding */
+  int32_t* rcnt[2];           /* nE: candidates in each element row */
+  int32_t* pe2[2];            /* nE*kRowCap: slot e*kRowCap + k */
   double* pd2[2];
   int32_t* pstick[2];
-  double* pso[2];             /* 3 per pair */
+  double* pso[2];             /* 3 per slot */
 };
 
 } // namespace shx

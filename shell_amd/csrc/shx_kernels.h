@@ -19,12 +19,17 @@ enum { M_STEP = 0, M_FORCES = 1, M_INIT = 2 };
 __global__ void k_precompute(DPtrs P);
 __global__ void k_mass(DPtrs P);
 __global__ void k_elem(DPtrs P, int mode);
+__global__ void k_fin_elem(DPtrs P, int mode);
 __global__ void k_contact(DPtrs P, int mode);
 __global__ void k_node(DPtrs P, int mode);
-__global__ void k_bp_box(DPtrs P, int posmode);
-__global__ void k_bp_bin(DPtrs P);
-__global__ void k_bp_pairs(DPtrs P, int posmode);
-__global__ void k_bp_emit(DPtrs P);
+__global__ void k_fin_node(DPtrs P, int mode);
+__global__ void k_bp_box(DPtrs P, int posmode, int list);
+__global__ void k_bp_count(DPtrs P, int list);
+__global__ void k_bp_scan(DPtrs P, int list);
+__global__ void k_bp_scatter(DPtrs P, int list);
+__global__ void k_bp_pairs(DPtrs P, int posmode, int list);
+__global__ void k_bp_host_plan(DPtrs P);
+__global__ void k_bp_host_done(DPtrs P);
 
 } // namespace shx
 #endif

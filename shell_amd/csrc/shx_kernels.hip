@@ -197,26 +197,6 @@ __global__ void __launch_bounds__(256) k_mass(DPtrs P)
 
 /* ------------------------------------------------------- block utilities */
 
-/* every block of a shell calls this once at the end of a kernel; returns
- * true in the last of the shell's nblocks blocks to arrive, which then sees
- * all other blocks' global writes (release/acquire fences around a
- * device-scope counter).  The last block resets the counter for the next
- * launch. */
-__device__ bool last_block(unsigned* counter, int nblocks)
-{
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(counter, 1u);
-    s_last = (prev == (unsigned)nblocks - 1);
-    if (s_last) *counter = 0;
-  }
-  __syncthreads();
-  __threadfence();
-  return s_last != 0;
-}
-
 /* deterministic block reduction of NC components of nparts partial records
  * (record length `stride`, components comp[0..NC), op 0: sum, 1: max):
  * thread t folds records t, t+bs, ... in order (several loads in flight),
@@ -290,6 +270,27 @@ __device__ __forceinline__ bool rebuild_due(const DShell& s, const SConst& C)
   return C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
 }
 
+/* clear the bucket counts of one shell's cell grid: the nb blocks lb =
+ * 0..nb-1 of a kernel's range for the shell each take a strided share */
+__device__ void clear_buckets(const DPtrs& P, const SConst& C, int lb, int nb)
+{
+  const unsigned nbk = 1u << C.hb_bits;
+  for (unsigned b = (unsigned)lb * blockDim.x + threadIdx.x; b < nbk; b += (unsigned)nb * blockDim.x)
+    P.bcnt[C.bbase + b] = 0;
+}
+
+/* start of a contact-list rebuild of shell sid (one thread): the rebuild
+ * writes the other pair buffer, the shell joins work list `list` of the
+ * broad-phase kernels, and the next step kernel clears its buckets */
+__device__ void plan_rebuild(const DPtrs& P, DShell& w, int sid, int list)
+{
+  w.pair_src = w.pair_buf;
+  w.pair_buf = 1 - w.pair_buf;
+  w.bclear = 1;
+  const int k = atomicAdd(P.wl[list], 1);
+  if (k < P.S) P.wl[list][1 + k] = sid;
+}
+
 /* --------------------------------------------------------- element force */
 
 /* kElemLanes lanes per element gather u_pred over the patch, form the
@@ -301,10 +302,9 @@ __device__ __forceinline__ bool rebuild_due(const DShell& s, const SConst& C)
  *   f_i = -(phi1 G1 + phi2 G2 - phi11 G3 - phi22 G4 - phi12 G5) + p phi G0,
  * the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis derivative
  * (ShellSystem.C:1236-1282) plus the LOCAL pressure load (ShellSystem.C:
- * 1570-1576).  The last block of each shell reduces the shell's block
- * partials: energies, the enclosed volume (compute_enclosed_volume,
- * ShellSystem.C:2505-2541) and, in a step, the predictor increment and the
- * broad-phase trigger (check_increment(false), ShellSystem.C:978, 993). */
+ * 1570-1576).  Block partials (energies, volume terms, max |du|^2) go to
+ * epart for k_fin_elem.  The blocks of a shell whose contact list was
+ * rebuilt at the end of the previous step clear its cell buckets. */
 __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
@@ -312,6 +312,8 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
   const int sid = uni(P.eshell[blockIdx.x * kElemPerBlock]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && P.wl[1]) P.wl[1][0] = 0;
+  if (sh.bclear && C.contact) clear_buckets(P, C, blockIdx.x - C.blkE, C.nbE);
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
   __syncthreads();
@@ -462,21 +464,37 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
     __syncthreads();
   }
   if (t < 5) P.epart[(size_t)blockIdx.x * 5 + t] = s_red[t][0];
-  if (!last_block(P.ctr + kCtrPerShell * sid + 0, C.nbE)) return;
+}
+
+/* per-shell epilogue of the element kernel, one block per shell: reduces the
+ * shell's block partials in a fixed order: energies, the enclosed volume
+ * (compute_enclosed_volume, ShellSystem.C:2505-2541) and, in a step, the
+ * predictor increment and the broad-phase trigger (check_increment(false),
+ * ShellSystem.C:978, 993; ShellSystem.C:1917-1919) */
+__global__ void __launch_bounds__(256) k_fin_elem(DPtrs P, int mode)
+{
+  __shared__ double s_red[5 * 256];
+  const int sid = blockIdx.x;
+  const SConst& C = P.cs[sid];
+  DShell& w = P.sh[sid];
+  if (w.done) return;
   const int comp[5] = {0, 1, 2, 3, 4}, op[5] = {0, 0, 1, 0, 0};
   double r[5];
-  reduce_records<5>(P.epart + (size_t)C.blkE * 5, C.nbE, 5, comp, op, r, &s_red[0][0]);
-  if (t == 0) {
-    DShell& w = P.sh[sid];
-    w.att_e_tens = r[0];
-    w.att_e_bend = r[1];
-    w.att_volume = fabs(r[3] / 6) - r[4] * 0.5 * C.h;
-    w.att_area = r[4];
-    if (mode == M_STEP) {
-      check_increment_dev(w, C, r[2], false);
-      w.rebuild = rebuild_due(w, C);
-    }
+  reduce_records<5>(P.epart + (size_t)C.blkE * 5, C.nbE, 5, comp, op, r, s_red);
+  if (threadIdx.x != 0) return;
+  w.att_e_tens = r[0];
+  w.att_e_bend = r[1];
+  w.att_volume = fabs(r[3] / 6) - r[4] * 0.5 * C.h;
+  w.att_area = r[4];
+  w.bclear = 0;
+  if (mode == M_STEP) {
+    check_increment_dev(w, C, r[2], false);
+    w.rebuild = rebuild_due(w, C);
+  } else if (mode != M_INIT) {
+    w.rebuild = 0;
   }
+  if (w.rebuild && C.contact) plan_rebuild(P, w, sid, 0);
+  else w.rebuild = 0;
 }
 
 /* --------------------------------------------------------------- contact */
@@ -490,18 +508,22 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
   return add(x, u);
 }
 
-/* add_shell_shell_contact_force, ShellSystem.C:1792-1907: per candidate
- * pair, distance, log-barrier normal force, normal damping and Margolis
- * friction; the force is split -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w)
- * onto element 2 and accumulated with FP64 atomics into fc (the node kernel
- * reads and clears it).  Each shell owns C.nbC blocks that stride over its
- * pairs. */
+/* add_shell_shell_contact_force, ShellSystem.C:1792-1907: one thread per
+ * element e1 walks its candidate row (e2 > e1): distance, log-barrier normal
+ * force, normal damping and Margolis friction; the force is split
+ * -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w) onto element 2 and
+ * accumulated with FP64 atomics into fc (the node kernel reads and clears
+ * it).  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
+ * a shell whose list was rebuilt in this step clear its cell buckets. */
 __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
 {
-  __shared__ double s_red[3][256];
-  const int sid = uni(P.cshell[blockIdx.x]);
+  __shared__ double s_red[kCRec][256];
+  const int sid = uni(P.eshell[blockIdx.x * 256]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.wl[0][0] = 0;
+  if (!C.contact) return;
+  if (sh.bclear) clear_buckets(P, C, blockIdx.x - C.tile0, C.ntile);
   if (sh.done) return;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
@@ -509,82 +531,93 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
   const double* U = P.U[par];
   const double* V = P.V[par];
   const double* A = P.A[par];
-  const int np = sh.n_pairs, buf = sh.pair_buf;
+  const int buf = sh.pair_buf;
   const int nE = P.nE;
   const double h = C.h;
-  const int lb = blockIdx.x - C.blkC;
-  double nc = 0, es = 0, ef = 0;
-  for (int pl = lb * blockDim.x + threadIdx.x; pl < np; pl += C.nbC * blockDim.x) {
-    const int p = C.pbase + pl;
-    const int e[2] = {P.pe1[buf][p], P.pe2[buf][p]};
-    v3 X0[3], X1[3];
-    int nid[2][3];
-    for (int k = 0; k < 3; ++k) {
-      nid[0][k] = P.tri[(size_t)k * nE + e[0]];
-      nid[1][k] = P.tri[(size_t)k * nE + e[1]];
-      X0[k] = node_pos(P, nid[0][k], U, V, A, predict, dt, a8);
-      X1[k] = node_pos(P, nid[1][k], U, V, A, predict, dt, a8);
-    }
-    tri_hit hit;
-    tri_tri(X0, X1, hit);
-    P.pd2[buf][p] = hit.d2;
-    if (hit.d2 <= 0) atomicMax(&P.sh[sid].problem, 6); /* CONTACT_PROBLEM */
-    int stick = P.pstick[buf][p];
-    v3 so = ld3(P.pso[buf], p);
-    if (hit.d2 < C.d2_friction) {
-      const double amin = fmin(P.area[e[0]], P.area[e[1]]);
-      const double adc = sqrt(hit.d2);
-      const double depth = h - adc;
-      const v3 n = mk(hit.dc.x / adc, hit.dc.y / adc, hit.dc.z / adc);
-      const double k = C.contact_stiffness > 0 ? C.contact_stiffness : C.stiff_density * amin;
-      double fn = 0, Es = 0, Ef = 0;
-      if (depth > 0) {
-        nc += 1;
-        fn = contact_force(k, h, adc);
-        Es = contact_energy(k, h, adc);
+  const int e0 = blockIdx.x * 256 + threadIdx.x;
+  double nc = 0, es = 0, ef = 0, npair = 0;
+  if (e0 - C.ebase < C.ne) {
+    const int np = min(P.rcnt[buf][e0], kRowCap);
+    npair = np;
+    int nid0[3];
+    v3 X0[3];
+    if (np > 0)
+      for (int k = 0; k < 3; ++k) {
+        nid0[k] = P.tri[(size_t)k * nE + e0];
+        X0[k] = node_pos(P, nid0[k], U, V, A, predict, dt, a8);
       }
-      es += Es;
-      v3 vv[2][3];
-      for (int q = 0; q < 3; ++q) { vv[0][q] = ld3(V, nid[0][q]); vv[1][q] = ld3(V, nid[1][q]); }
-      const double u0 = 1 - hit.v0 - hit.w0, u1 = 1 - hit.v1 - hit.w1;
-      const v3 vr = sub(add(add(scl(vv[1][0], u1), scl(vv[1][1], hit.v1)), scl(vv[1][2], hit.w1)),
-                        add(add(scl(vv[0][0], u0), scl(vv[0][1], hit.v0)), scl(vv[0][2], hit.w0)));
-      const double vn = dot(vr, n);
-      v3 fcon = mk(0, 0, 0);
-      if (C.self_friction) {
-        const v3 cp = add(add(add(scl(X0[0], u0), scl(X0[1], hit.v0)), scl(X0[2], hit.w0)), scl(hit.dc, 0.5));
-        friction(Ef, fcon, fn, n, vr, vn, C.friction_stop_velocity, C.stiff_static_friction, C.mu_s, C.mu_d,
-                 C.self_friction_damping, stick, so, cp);
-        ef += Ef;
+    for (int q = 0; q < np; ++q) {
+      const size_t p = (size_t)e0 * kRowCap + q;
+      const int e1 = P.pe2[buf][p];
+      v3 X1[3];
+      int nid[2][3];
+      for (int k = 0; k < 3; ++k) {
+        nid[0][k] = nid0[k];
+        nid[1][k] = P.tri[(size_t)k * nE + e1];
+        X1[k] = node_pos(P, nid[1][k], U, V, A, predict, dt, a8);
       }
-      fn -= C.self_damping * k * vn;
-      fcon = add(fcon, scl(n, fn));
-      const double wv[2] = {hit.v0, hit.v1}, ww[2] = {hit.w0, hit.w1};
-      for (int s = 0; s < 2; ++s) {
-        const double vb = wv[s], wb = ww[s], ub = 1 - vb - wb;
-        const double pre = s == 0 ? -0.5 : 0.5;
-        const double fq[3] = {pre * fcon.x, pre * fcon.y, pre * fcon.z};
-        for (int q = 0; q < 3; ++q) {
-          atomicAdd(&P.fc[3 * nid[s][0] + q], ub * fq[q]);
-          atomicAdd(&P.fc[3 * nid[s][1] + q], vb * fq[q]);
-          atomicAdd(&P.fc[3 * nid[s][2] + q], wb * fq[q]);
+      tri_hit hit;
+      tri_tri(X0, X1, hit);
+      P.pd2[buf][p] = hit.d2;
+      if (hit.d2 <= 0) atomicMax(&P.sh[sid].problem, 6); /* CONTACT_PROBLEM */
+      int stick = P.pstick[buf][p];
+      v3 so = ld3(P.pso[buf], p);
+      if (hit.d2 < C.d2_friction) {
+        const double amin = fmin(P.area[e0], P.area[e1]);
+        const double adc = sqrt(hit.d2);
+        const double depth = h - adc;
+        const v3 n = mk(hit.dc.x / adc, hit.dc.y / adc, hit.dc.z / adc);
+        const double k = C.contact_stiffness > 0 ? C.contact_stiffness : C.stiff_density * amin;
+        double fn = 0, Es = 0, Ef = 0;
+        if (depth > 0) {
+          nc += 1;
+          fn = contact_force(k, h, adc);
+          Es = contact_energy(k, h, adc);
         }
+        es += Es;
+        v3 vv[2][3];
+        for (int j = 0; j < 3; ++j) { vv[0][j] = ld3(V, nid[0][j]); vv[1][j] = ld3(V, nid[1][j]); }
+        const double u0 = 1 - hit.v0 - hit.w0, u1 = 1 - hit.v1 - hit.w1;
+        const v3 vr = sub(add(add(scl(vv[1][0], u1), scl(vv[1][1], hit.v1)), scl(vv[1][2], hit.w1)),
+                          add(add(scl(vv[0][0], u0), scl(vv[0][1], hit.v0)), scl(vv[0][2], hit.w0)));
+        const double vn = dot(vr, n);
+        v3 fcon = mk(0, 0, 0);
+        if (C.self_friction) {
+          const v3 cp = add(add(add(scl(X0[0], u0), scl(X0[1], hit.v0)), scl(X0[2], hit.w0)), scl(hit.dc, 0.5));
+          friction(Ef, fcon, fn, n, vr, vn, C.friction_stop_velocity, C.stiff_static_friction, C.mu_s, C.mu_d,
+                   C.self_friction_damping, stick, so, cp);
+          ef += Ef;
+        }
+        fn -= C.self_damping * k * vn;
+        fcon = add(fcon, scl(n, fn));
+        const double wv[2] = {hit.v0, hit.v1}, ww[2] = {hit.w0, hit.w1};
+        for (int s = 0; s < 2; ++s) {
+          const double vb = wv[s], wb = ww[s], ub = 1 - vb - wb;
+          const double pre = s == 0 ? -0.5 : 0.5;
+          const double fq[3] = {pre * fcon.x, pre * fcon.y, pre * fcon.z};
+          for (int j = 0; j < 3; ++j) {
+            atomicAdd(&P.fc[3 * nid[s][0] + j], ub * fq[j]);
+            atomicAdd(&P.fc[3 * nid[s][1] + j], vb * fq[j]);
+            atomicAdd(&P.fc[3 * nid[s][2] + j], wb * fq[j]);
+          }
+        }
+      } else {
+        so = mk(0, 0, 0);
+        stick = 0;
       }
-    } else {
-      so = mk(0, 0, 0);
-      stick = 0;
+      P.pstick[buf][p] = stick;
+      st3(P.pso[buf], p, so);
     }
-    P.pstick[buf][p] = stick;
-    st3(P.pso[buf], p, so);
   }
   const int t = threadIdx.x;
-  s_red[0][t] = nc; s_red[1][t] = es; s_red[2][t] = ef;
+  s_red[0][t] = nc; s_red[1][t] = es; s_red[2][t] = ef; s_red[3][t] = npair;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) { s_red[0][t] += s_red[0][t + s]; s_red[1][t] += s_red[1][t + s]; s_red[2][t] += s_red[2][t + s]; }
+    if (t < s)
+      for (int c = 0; c < kCRec; ++c) s_red[c][t] += s_red[c][t + s];
     __syncthreads();
   }
-  if (t < 3) P.cpart[(size_t)blockIdx.x * 3 + t] = s_red[t][0];
+  if (t < kCRec) P.cpart[(size_t)blockIdx.x * kCRec + t] = s_red[t][0];
 }
 
 /* ---------------------------------------------------- node gather + step */
@@ -625,23 +658,32 @@ __device__ double rigid_free_norm(double f2, const double* r)
 
 /* finalize of one step attempt of a shell (ShellSystem.C:1006-1117):
  * residual and energies, local-error estimate, accept (state buffer flip,
- * time advance, check_increment, corrector-time broad-phase trigger) or
- * reject with a smaller dt.  Run by the last node-kernel block of the shell. */
-__device__ void finalize(const DPtrs& P, const SConst& C, int sid, int mode, double* s)
+ * time advance, check_increment, corrector-time broad-phase trigger,
+ * ShellSystem.C:1068-1093) or reject with a smaller dt.  One block per
+ * shell, after the node kernel; the shell's block partials are reduced in a
+ * fixed order. */
+__global__ void __launch_bounds__(256) k_fin_node(DPtrs P, int mode)
 {
+  __shared__ double s[kNodeRec * 256];
+  const int sid = blockIdx.x;
+  const SConst& C = P.cs[sid];
+  if (P.sh[sid].done) return;
   int cn[kNodeRec], on[kNodeRec];
   for (int q = 0; q < kNodeRec; ++q) { cn[q] = q; on[q] = (q >= 2 && q <= 4); }
   double rn[kNodeRec];
   reduce_records<kNodeRec>(P.npart + (size_t)C.blkN * kNodeRec, C.nbN, kNodeRec, cn, on, rn, s);
-  const int cc[3] = {0, 1, 2}, oc[3] = {0, 0, 0};
-  double rc[3];
-  reduce_records<3>(P.cpart + (size_t)C.blkC * 3, C.contact ? C.nbC : 0, 3, cc, oc, rc, s);
+  const int cc[kCRec] = {0, 1, 2, 3}, oc[kCRec] = {0, 0, 0, 0};
+  double rc[kCRec];
+  reduce_records<kCRec>(P.cpart + (size_t)C.tile0 * kCRec, C.contact ? C.ntile : 0, kCRec, cc, oc, rc, s);
   const double f2 = rn[0], fe2 = rn[1], err = rn[2], vmax = rn[3], du = rn[4], ek = rn[5];
   const double nc = rc[0], es = rc[1], ef = rc[2];
   if (threadIdx.x != 0) return;
   DShell& w = P.sh[sid];
   w.e_tens = w.att_e_tens; w.e_bend = w.att_e_bend; w.e_self = es; w.e_fric = ef;
   w.n_contacts = (int)nc;
+  w.n_pairs = (int)rc[3];
+  w.bclear = 0;
+  w.rebuild = 0;
   w.fnorm = sqrt(f2);
   w.fext_norm = sqrt(fe2);
   w.fdeform = rigid_free_norm(f2, rn + 6);
@@ -676,6 +718,7 @@ __device__ void finalize(const DPtrs& P, const SConst& C, int sid, int mode, dou
     w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
     w.e_kin = ek;
     w.rebuild = rebuild_due(w, C);
+    if (w.rebuild) plan_rebuild(P, w, sid, 1);
     set_newmark_dev(w, C, new_dt);
     if (w.converged_step < 0 && w.t >= C.load_end_time && w.residual_deform <= C.conv_tol) {
       w.converged_step = w.n_steps;
@@ -699,13 +742,12 @@ __device__ void finalize(const DPtrs& P, const SConst& C, int sid, int mode, dou
  * central-difference / Newmark update (ShellSystem.C:1006-1008, 1068-1090):
  * a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a, v = v_pred + gamma dt a,
  * written to the other state buffer.  Block partials: |f|^2, |f_ext|^2,
- * max|a-a_old|, max|v_pred|, max|du|^2, E_kin and the rigid-body sums; the
- * last block of each shell finalizes it.  Force time: t + dt in a step,
+ * max|a-a_old|, max|v_pred|, max|du|^2, E_kin and the rigid-body sums, for
+ * k_fin_node.  Force time: t + dt in a step,
  * t + dt in M_INIT (apply_initial_conditions), DShell::t_eval in M_FORCES. */
 __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
-  __shared__ double s_red[kNodeRec][kNodeBlock];
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
   const int sid = uni(P.vshell[blockIdx.x * kNodePerBlock]);
   const SConst& C = P.cs[sid];
@@ -811,31 +853,29 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
     for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
     P.npart[(size_t)blockIdx.x * kNodeRec + t] = acc;
   }
-  if (!last_block(P.ctr + kCtrPerShell * sid + 1, C.nbN)) return;
-  finalize(P, C, sid, mode, &s_red[0][0]);
 }
 
 /* ------------------------------------------------------------ broad phase */
 
 /* compute_shell_shell_contact_candidates after its trigger
- * (ShellSystem.C:1922-2113) as four kernels whose blocks return at once
- * unless DShell::rebuild of their shell is set:
- *  k_bp_box   element AABBs at the positions, tile partials; last block of
- *             the shell: global box, averaged element size (serial in
- *             256-element tiles, then serial over tiles), cell size, d_max,
- *             grid size and the n_cells_max check (compute_bounding_box and
- *             ShellSystem.C:1933-1964);
- *  k_bp_bin   one lane per (element, cell): (cell, element) entries appended
- *             with an atomic cursor into the shell's hashed cell buckets (the
- *             reference's cell_to_elem_map, ShellSystem.C:1967-2010);
- *  k_bp_pairs one lane per (element, cell): surviving old pairs (stick or
- *             d2 <= d_max^2, ShellSystem.C:2016-2027) and new candidates from
- *             the cell (counted in the first cell both ranges share, not
- *             close, not listed, boxes within d_max, exact d2 <= d_max^2,
- *             ShellSystem.C:2033-2086); last block of the shell: prefix scan;
- *  k_bp_emit  per element: sorted merge into the shell's region of the other
- *             pair buffer; the shell's buckets cleared; last block publishes
- *             the new list.
+ * (ShellSystem.C:1922-2113).  The step epilogues put the shells whose
+ * trigger fired on a work list; each broad-phase kernel runs a fixed grid
+ * that walks the blocks of the listed shells only:
+ *  k_bp_box   element AABBs at the positions and 256-element tile partials
+ *             (compute_bounding_box, ShellSystem.C:2544-2587);
+ *  k_bp_count  the shell's cell grid (box, averaged element size, cell size,
+ *             d_max, grid size, n_cells_max check, ShellSystem.C:1933-1964)
+ *             and one count per (element, cell) in the cell's hash bucket;
+ *  k_bp_scan  one block per listed shell: bucket offsets (prefix scan);
+ *  k_bp_scatter (element, cell) entries into their buckets: together a
+ *             counting sort of the entries by cell hash, the reference's
+ *             cell_to_elem_map (ShellSystem.C:1967-2010);
+ *  k_bp_pairs lanes per (element, cell): the surviving pairs of the old
+ *             list (stick or d2 <= d_max^2, ShellSystem.C:2016-2027) and the
+ *             new candidates of the cell (tested in the first cell both index
+ *             ranges share, not close, not listed, boxes within d_max, exact
+ *             d2 <= d_max^2, ShellSystem.C:2033-2086) appended to the
+ *             element's row of the new pair buffer.
  * posmode M_STEP: positions X + u_pred, otherwise X + u. */
 
 __device__ __forceinline__ bool bp_active(const DShell* sh)
@@ -870,15 +910,64 @@ __device__ __forceinline__ unsigned bucket_of(unsigned cell, unsigned bits)
   return (cell * 0x9E3779B1u) >> (32 - bits);
 }
 
-__global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode)
+/* the work of the shells on a work list as virtual blocks: `kind` 0 counts
+ * the shell's 256-element tiles, 1 its lane blocks.  s_sh[i] is the i-th
+ * listed shell, s_pref the exclusive prefix of their block counts; returns
+ * the number n of listed shells (s_pref[n] = total). */
+__device__ int vmap_build(const DPtrs& P, const int32_t* wl, int kind, int* s_sh, int* s_pref, int* s_tmp)
+{
+  const int n = min(wl[0], P.S);
+  const int t = threadIdx.x, bs = blockDim.x;
+  const int chunk = (n + bs - 1) / bs;
+  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
+  int sum = 0;
+  for (int i = lo; i < hi; ++i) {
+    const int s = wl[1 + i];
+    s_sh[i] = s;
+    sum += kind ? P.cs[s].nbL : P.cs[s].ntile;
+  }
+  s_tmp[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < bs; off <<= 1) {
+    const int x = t >= off ? s_tmp[t - off] : 0;
+    __syncthreads();
+    s_tmp[t] += x;
+    __syncthreads();
+  }
+  int run = s_tmp[t] - sum;
+  for (int i = lo; i < hi; ++i) {
+    s_pref[i] = run;
+    run += kind ? P.cs[s_sh[i]].nbL : P.cs[s_sh[i]].ntile;
+  }
+  if (t == bs - 1) s_pref[n] = s_tmp[t];
+  __syncthreads();
+  return n;
+}
+
+/* listed shell of virtual block vb (binary search in s_pref) */
+__device__ __forceinline__ int vmap_find(int vb, int n, const int* s_pref)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_pref[mid] <= vb) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode, int list)
 {
   __shared__ double s_box[6][256];
   __shared__ double s_size[256];
-  const int sid = uni(P.eshell[blockIdx.x * 256]);
+  __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  const int nl = vmap_build(P, P.wl[list], 0, s_sh, s_pref, s_tmp);
+  for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
+  const int li = vmap_find(vb, nl, s_pref);
+  const int sid = uni(s_sh[li]);
   const SConst& C = P.cs[sid];
   DShell* sh = P.sh + sid;
-  if (!bp_active(sh)) return;
-  const int nE = P.nE, t = threadIdx.x, tile = blockIdx.x;
+  if (!bp_active(sh)) continue;
+  const int nE = P.nE, t = threadIdx.x, tile = C.tile0 + (vb - s_pref[li]);
   const bool predict = (posmode == M_STEP);
   const double dt = sh->dt, a8 = sh->nm[8];
   const int par = sh->par;
@@ -925,8 +1014,27 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode)
     for (int q = 0; q < 6; ++q) P.bpart[(size_t)tile * 7 + q] = s_box[q][0];
     P.bpart[(size_t)tile * 7 + 6] = sum;
   }
-  if (!last_block(P.ctr + kCtrPerShell * sid + 2, C.ntile)) return;
-  /* the shell's box: fixed-order tree of tile partials; size: serial over tiles */
+  __syncthreads();
+  }
+}
+
+/* cell grid of a rebuilding shell from its tile partials (compute_bounding_box
+ * and ShellSystem.C:1933-1964): the shell's box (fixed-order tree of tile
+ * partials), averaged element size (serial in 256-element tiles, then
+ * serial over tiles), cell size, d_max, grid dimensions.  Every thread of
+ * the block returns the same Grid; false if the rebuild aborts (non-finite
+ * positions: NOT_FINITE; more than n_cells_max cells: TOO_MANY_CELLS). */
+struct Grid {
+  double bmin[3], bmax[3], avg, cell, dmx;
+  unsigned nd[3], ncell;
+  int problem;
+};
+
+__device__ bool shell_grid(const DPtrs& P, const SConst& C, Grid& g)
+{
+  __shared__ double s_box[6][256];
+  __shared__ double s_size[256];
+  const int t = threadIdx.x;
   const int ntiles = C.ntile;
   const double* bp = P.bpart + (size_t)C.tile0 * 7;
   for (int q = 0; q < 6; ++q) {
@@ -955,36 +1063,33 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode)
       for (int q = 0; q < min(256, ntiles - base); ++q) total += s_size[q];
     __syncthreads();
   }
-  if (t != 0) return;
-  sh->du_sum = 0;
-  sh->n_rebuilds++;
-  sh->avg_elem_size = total / C.ne;
+  if (t == 0) s_size[0] = total;
+  __syncthreads();
+  total = s_size[0];
+  g.avg = total / C.ne;
+  g.problem = 0;
   bool ok = isfinite(total);
-  double bmn2[3], bmx2[3];
   for (int c = 0; c < 3; ++c) {
-    bmn2[c] = s_box[c][0]; bmx2[c] = s_box[3 + c][0];
-    sh->bmin[c] = bmn2[c]; sh->bmax[c] = bmx2[c];
-    if (!isfinite(bmn2[c]) || !isfinite(bmx2[c])) ok = false;
+    g.bmin[c] = s_box[c][0];
+    g.bmax[c] = s_box[3 + c][0];
+    if (!isfinite(g.bmin[c]) || !isfinite(g.bmax[c])) ok = false;
   }
-  if (!ok) { sh->problem = max(sh->problem, 7); sh->rebuild = 0; return; }
-  double cell = C.cell_size_mult * sh->avg_elem_size;
+  __syncthreads();
+  if (!ok) { g.problem = 7; return false; }
+  double cell = C.cell_size_mult * g.avg;
   const double tol = C.cell_margin * cell;
   cell += tol;
-  const double dmx = C.h + tol;
-  sh->cell_size = cell;
-  sh->d_max = dmx;
-  sh->d2_max = dmx * dmx;
-  unsigned nd[3];
-  for (int c = 0; c < 3; ++c) nd[c] = (unsigned)(int)(((bmx2[c] - bmn2[c]) + dmx) / cell + 1);
-  const unsigned ncell = nd[0] * nd[1] * nd[2];
-  for (int c = 0; c < 3; ++c) sh->nd[c] = (int)nd[c];
-  sh->n_cells = (int)ncell;
-  if (ncell > C.n_cells_max) { sh->problem = max(sh->problem, 4); sh->rebuild = 0; }
+  g.cell = cell;
+  g.dmx = C.h + tol;
+  for (int c = 0; c < 3; ++c) g.nd[c] = (unsigned)(int)(((g.bmax[c] - g.bmin[c]) + g.dmx) / cell + 1);
+  g.ncell = g.nd[0] * g.nd[1] * g.nd[2];
+  if (g.ncell > C.n_cells_max) { g.problem = 4; return false; }
+  return true;
 }
 
-/* (element, cell) lane mapping of the binning and pair kernels: kBpLanes
- * lanes per element, lane k handles the element's cells k, k + kBpLanes, ...
- * of its index box in cx-major order */
+/* (element, cell) lane mapping of the counting and scatter kernels:
+ * kBpLanes lanes per element, lane k handles the element's cells k,
+ * k + kBpLanes, ... of its index box in cx-major order */
 __device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz, int& cx, int& cy, int& cz)
 {
   cz = lo[2] + k % nz;
@@ -993,112 +1098,194 @@ __device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz,
   cx = lo[0] + r / ny;
 }
 
-__global__ void __launch_bounds__(256) k_bp_bin(DPtrs P)
+/* count pass of the counting sort: the cell grid of the shell (each block
+ * forms it from the tile partials; the shell's first block records it),
+ * each element's cell index range (its AABB widened by d_max, ShellSystem.C:
+ * 1967-2010) and one count per (element, cell) in the cell's hash bucket */
+__global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
 {
-  const int sid = uni(P.eshell[blockIdx.x * kBpPerBlock]);
-  const SConst& C = P.cs[sid];
-  const DShell* sh = P.sh + sid;
-  if (!bp_active(sh)) return;
-  const int nE = P.nE;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int e = gid / kBpLanes, k0 = gid % kBpLanes;
-  if (e - C.ebase >= C.ne) return;
-  const double cell = sh->cell_size, dmx = sh->d_max;
-  const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
-  int lo[3], hi[3];
-  for (int q = 0; q < 3; ++q) {
-    lo[q] = (int)((ebox(P, q, e) - sh->bmin[q]) / cell);
-    hi[q] = (int)((ebox(P, 3 + q, e) - sh->bmin[q] + dmx) / cell);
-  }
-  if (k0 == 0)
-    for (int q = 0; q < 3; ++q) {
-      P.ecell[(size_t)q * nE + e] = lo[q];
-      P.ecell[(size_t)(3 + q) * nE + e] = hi[q];
+  __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  const int nl = vmap_build(P, P.wl[list], 1, s_sh, s_pref, s_tmp);
+  for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
+    const int li = vmap_find(vb, nl, s_pref);
+    const int sid = uni(s_sh[li]);
+    const SConst& C = P.cs[sid];
+    DShell* sh = P.sh + sid;
+    if (!bp_active(sh)) continue;
+    const int local = vb - s_pref[li];
+    Grid g;
+    const bool ok = shell_grid(P, C, g);
+    if (local == 0 && threadIdx.x == 0) {
+      sh->du_sum = 0;
+      sh->n_rebuilds++;
+      sh->avg_elem_size = g.avg;
+      for (int c = 0; c < 3; ++c) { sh->bmin[c] = g.bmin[c]; sh->bmax[c] = g.bmax[c]; }
+      if (ok) {
+        sh->cell_size = g.cell;
+        sh->d_max = g.dmx;
+        sh->d2_max = g.dmx * g.dmx;
+        for (int c = 0; c < 3; ++c) sh->nd[c] = (int)g.nd[c];
+        sh->n_cells = (int)g.ncell;
+      } else {
+        if (g.problem == 4) sh->n_cells = (int)g.ncell;
+        sh->problem = max(sh->problem, g.problem);
+        sh->rebuild = 0;
+        sh->pair_buf = sh->pair_src;
+      }
     }
-  const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-  const int total = (hi[0] - lo[0] + 1) * ny * nz;
-  for (int k = k0; k < total; k += kBpLanes) {
-    int cx, cy, cz;
-    bp_cell_of(k, lo, ny, nz, cx, cy, cz);
-    const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-    const size_t b = C.bbase + bucket_of(c, C.hb_bits);
-    const unsigned pos = atomicAdd(&P.bcount[b], 1u);
-    if (pos < (unsigned)kBucketCap) P.bent[b * kBucketCap + pos] = ((uint64_t)c << 32) | (uint32_t)e;
-    else atomicOr(&P.sh[sid].err, ERR_ENTRIES);
+    if (!ok) continue;
+    const int nE = P.nE;
+    const int e = C.ebase + local * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
+    if (e - C.ebase >= C.ne) continue;
+    int lo[3], hi[3];
+    for (int q = 0; q < 3; ++q) {
+      lo[q] = (int)((ebox(P, q, e) - g.bmin[q]) / g.cell);
+      hi[q] = (int)((ebox(P, 3 + q, e) - g.bmin[q] + g.dmx) / g.cell);
+    }
+    if (k0 == 0) {
+      for (int q = 0; q < 3; ++q) {
+        P.ecell[(size_t)q * nE + e] = lo[q];
+        P.ecell[(size_t)(3 + q) * nE + e] = hi[q];
+      }
+      P.rcnt[sh->pair_buf][e] = 0;
+    }
+    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    const int total = (hi[0] - lo[0] + 1) * ny * nz;
+    for (int k = k0; k < total; k += kBpLanes) {
+      int cx, cy, cz;
+      bp_cell_of(k, lo, ny, nz, cx, cy, cz);
+      const unsigned c = (unsigned)cx * g.nd[1] * g.nd[2] + (unsigned)cy * g.nd[2] + (unsigned)cz;
+      atomicAdd(&P.bcnt[C.bbase + bucket_of(c, C.hb_bits)], 1u);
+    }
   }
 }
 
-/* exclusive scan of the row lengths surv[i] + min(nnew[i], kRowCap), i < n,
- * into out[0..n] offset by base, out[n] = base + total, by one block */
-__device__ void block_scan_rows(const int32_t* surv, const int32_t* nnew, int32_t* out, int n, int base,
-                                int32_t* s)
+/* prefix scan of the bucket counts of each listed shell (one block per
+ * shell): bucket start offsets and fill cursors in the shell's entry region */
+__global__ void __launch_bounds__(256) k_bp_scan(DPtrs P, int list)
 {
-  const int t = threadIdx.x, bs = blockDim.x;
-  const int chunk = (n + bs - 1) / bs;
-  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
-  int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += surv[i] + min(nnew[i], kRowCap);
-  s[t] = sum;
-  __syncthreads();
-  if (t == 0) {
-    int32_t acc = base;
-    for (int q = 0; q < bs; ++q) { const int32_t x = s[q]; s[q] = acc; acc += x; }
-    out[n] = acc;
-  }
-  __syncthreads();
-  int32_t run = s[t];
-  for (int i = lo; i < hi; ++i) { out[i] = run; run += surv[i] + min(nnew[i], kRowCap); }
-  __syncthreads();
-}
-
-/* lane (e1, k) walks the bucket entries of its cells; a pair e1 < e2 is
- * tested only in the first cell both index boxes share, so each pair is
- * tested by exactly one lane.  New pairs are appended to e1's found row
- * with an atomic cursor (the row is sorted in k_bp_emit, so the list does
- * not depend on the order of arrival). */
-__global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode)
-{
-  __shared__ int32_t s_scan[256];
-  const int sid = uni(P.eshell[blockIdx.x * kBpPerBlock]);
+  __shared__ uint32_t s[256];
+  if ((int)blockIdx.x >= min(P.wl[list][0], P.S)) return;
+  const int sid = P.wl[list][1 + blockIdx.x];
   const SConst& C = P.cs[sid];
   DShell* sh = P.sh + sid;
   if (!bp_active(sh)) return;
-  const int nE = P.nE;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int e1 = gid / kBpLanes, k0 = gid % kBpLanes;
-  const bool predict = (posmode == M_STEP);
-  const double dt = sh->dt, a8 = sh->nm[8];
-  const int par = sh->par;
-  const double* U = P.U[par];
-  const double* V = P.V[par];
-  const double* A = P.A[par];
-  const double dmx = sh->d_max, d2mx = sh->d2_max;
-  const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
-  const int ob = sh->pair_buf;
-  int32_t* nnew = P.found_cnt + nE + 1;
-  if (e1 - C.ebase < C.ne) {
+  const unsigned nbk = 1u << C.hb_bits;
+  const uint32_t* cnt = P.bcnt + C.bbase;
+  const int t = threadIdx.x;
+  const unsigned chunk = (nbk + 255) / 256;
+  const unsigned lo = min(nbk, t * chunk), hi = min(nbk, lo + chunk);
+  uint32_t sum = 0;
+  for (unsigned i = lo; i < hi; ++i) sum += cnt[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t x = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  uint32_t run = s[t] - sum;
+  for (unsigned i = lo; i < hi; ++i) {
+    P.boff[C.bbase + i] = run;
+    P.bcur[C.bbase + i] = run;
+    run += cnt[i];
+  }
+  if (t == 255 && s[255] > (uint32_t)C.ne * kEntPerElem) atomicOr(&sh->err, ERR_ENTRIES);
+}
+
+/* scatter pass of the counting sort: each (element, cell) entry to its
+ * bucket's next free slot */
+__global__ void __launch_bounds__(256) k_bp_scatter(DPtrs P, int list)
+{
+  __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  const int nl = vmap_build(P, P.wl[list], 1, s_sh, s_pref, s_tmp);
+  for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
+    const int li = vmap_find(vb, nl, s_pref);
+    const int sid = uni(s_sh[li]);
+    const SConst& C = P.cs[sid];
+    DShell* sh = P.sh + sid;
+    if (!bp_active(sh) || sh->err) continue;
+    const int nE = P.nE;
+    const int e = C.ebase + (vb - s_pref[li]) * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
+    if (e - C.ebase >= C.ne) continue;
+    const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
+    const int* L = P.ecell;
+    const int lo[3] = {L[e], L[(size_t)nE + e], L[(size_t)2 * nE + e]};
+    const int hi[3] = {L[(size_t)3 * nE + e], L[(size_t)4 * nE + e], L[(size_t)5 * nE + e]};
+    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    const int total = (hi[0] - lo[0] + 1) * ny * nz;
+    for (int k = k0; k < total; k += kBpLanes) {
+      int cx, cy, cz;
+      bp_cell_of(k, lo, ny, nz, cx, cy, cz);
+      const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+      const unsigned pos = atomicAdd(&P.bcur[C.bbase + bucket_of(c, C.hb_bits)], 1u);
+      P.bent[(size_t)C.entbase + pos] = ((uint64_t)c << 32) | (uint32_t)e;
+    }
+  }
+}
+
+/* lane k of element e1 takes cell k / lpc of the element's index box and
+ * every lpc-th entry of that cell's bucket, lpc = max(1, kBpLanes / cells)
+ * (cells beyond kBpLanes / lpc are taken in turn).  A pair e1 < e2 is tested
+ * only in the first cell both index boxes share, so each pair is tested
+ * exactly once.  Lane 0 of each element first copies the surviving pairs of
+ * the element's old row.  Entries are appended to the element's row of the
+ * new buffer with an atomic cursor, so the order within a row depends on
+ * arrival; the pair set does not. */
+__global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list)
+{
+  __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  const int nl = vmap_build(P, P.wl[list], 1, s_sh, s_pref, s_tmp);
+  for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
+    const int li = vmap_find(vb, nl, s_pref);
+    const int sid = uni(s_sh[li]);
+    const SConst& C = P.cs[sid];
+    DShell* sh = P.sh + sid;
+    if (!bp_active(sh) || sh->err) continue;
+    const int nE = P.nE;
+    const int e1 = C.ebase + (vb - s_pref[li]) * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
+    if (e1 - C.ebase >= C.ne) continue;
+    const bool predict = (posmode == M_STEP);
+    const double dt = sh->dt, a8 = sh->nm[8];
+    const int par = sh->par;
+    const double* U = P.U[par];
+    const double* V = P.V[par];
+    const double* A = P.A[par];
+    const double dmx = sh->d_max, d2mx = sh->d2_max;
+    const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
+    const int ob = sh->pair_src, nb = sh->pair_buf;
     const int* L = P.ecell;
     const int lo1[3] = {L[e1], L[(size_t)nE + e1], L[(size_t)2 * nE + e1]};
     const int hi1[3] = {L[(size_t)3 * nE + e1], L[(size_t)4 * nE + e1], L[(size_t)5 * nE + e1]};
-    const int r0 = P.row_start[ob][e1 + sid], r1 = P.row_start[ob][e1 + 1 + sid];
-    if (k0 == 0) {
-      int nsurv = 0;
-      for (int r = r0; r < r1; ++r)
-        if (P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx)) ++nsurv;
-      P.found_cnt[e1] = nsurv;
-    }
+    const size_t row = (size_t)e1 * kRowCap;
+    const int nold = min(P.rcnt[ob][e1], kRowCap);
+    if (k0 == 0)
+      for (int r = 0; r < nold; ++r) {
+        const size_t po = row + r;
+        if (!(P.pstick[ob][po] || !(P.pd2[ob][po] > d2mx))) continue;
+        const int slot = atomicAdd(&P.rcnt[nb][e1], 1);
+        if (slot >= kRowCap) { atomicOr(&sh->err, ERR_ROW); continue; }
+        const size_t pn = row + slot;
+        P.pe2[nb][pn] = P.pe2[ob][po];
+        P.pd2[nb][pn] = P.pd2[ob][po];
+        P.pstick[nb][pn] = P.pstick[ob][po];
+        st3(P.pso[nb], pn, ld3(P.pso[ob], po));
+      }
     const int ny = hi1[1] - lo1[1] + 1, nz = hi1[2] - lo1[2] + 1;
-    const int total = (hi1[0] - lo1[0] + 1) * ny * nz;
+    const int ncells = (hi1[0] - lo1[0] + 1) * ny * nz;
+    const int lpc = max(1, kBpLanes / ncells), groups = kBpLanes / lpc;
+    const int sub = k0 % lpc;
     bool have_x0 = false;
     v3 X0[3];
-    for (int k = k0; k < total; k += kBpLanes) {
+    for (int ci = k0 / lpc; ci < ncells; ci += groups) {
       int cx, cy, cz;
-      bp_cell_of(k, lo1, ny, nz, cx, cy, cz);
+      bp_cell_of(ci, lo1, ny, nz, cx, cy, cz);
       const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-      const size_t b = C.bbase + bucket_of(c, C.hb_bits);
-      const unsigned cnt = min(P.bcount[b], (unsigned)kBucketCap);
-      for (unsigned q = 0; q < cnt; ++q) {
-        const uint64_t ent = P.bent[b * kBucketCap + q];
+      const unsigned b = C.bbase + bucket_of(c, C.hb_bits);
+      const unsigned qend = P.bcur[b];
+      for (unsigned q = P.boff[b] + sub; q < qend; q += lpc) {
+        const uint64_t ent = P.bent[(size_t)C.entbase + q];
         if ((unsigned)(ent >> 32) != c) continue;
         const int e2 = (int)(uint32_t)ent;
         if (e2 <= e1) continue;
@@ -1108,15 +1295,11 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode)
           continue;
         if (close_pair(P, e1, e2)) continue;
         bool listed = false;
-        {
-          int lo = r0, hi = r1;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const int x = P.pe2[ob][mid];
-            if (x == e2) { listed = P.pstick[ob][mid] || !(P.pd2[ob][mid] > d2mx); break; }
-            if (x < e2) lo = mid + 1; else hi = mid;
+        for (int r = 0; r < nold; ++r)
+          if (P.pe2[ob][row + r] == e2) {
+            listed = P.pstick[ob][row + r] || !(P.pd2[ob][row + r] > d2mx);
+            break;
           }
-        }
         if (listed) continue;
         if (boxes_apart(P, e1, e2, dmx)) continue;
         if (!have_x0) {
@@ -1129,74 +1312,43 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode)
         tri_tri(X0, X1, hit);
         if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
         if (hit.d2 > d2mx) continue;
-        const int slot = atomicAdd(&nnew[e1], 1);
-        if (slot < kRowCap) {
-          P.found_e2[(size_t)e1 * kRowCap + slot] = e2;
-          P.found_d2[(size_t)e1 * kRowCap + slot] = hit.d2;
-        } else atomicOr(&sh->err, ERR_ROW);
+        const int slot = atomicAdd(&P.rcnt[nb][e1], 1);
+        if (slot >= kRowCap) { atomicOr(&sh->err, ERR_ROW); continue; }
+        const size_t pn = row + slot;
+        P.pe2[nb][pn] = e2;
+        P.pd2[nb][pn] = hit.d2;
+        P.pstick[nb][pn] = 0;
+        st3(P.pso[nb], pn, mk(0, 0, 0));
       }
     }
   }
-  if (!last_block(P.ctr + kCtrPerShell * sid + 3, C.nbL)) return;
-  int32_t* rs = P.row_start[1 - ob] + C.ebase + sid;
-  block_scan_rows(P.found_cnt + C.ebase, nnew + C.ebase, rs, C.ne, C.pbase, s_scan);
-  if (threadIdx.x == 0 && rs[C.ne] - C.pbase > C.pair_cap) atomicOr(&sh->err, ERR_PAIRS);
 }
 
-__global__ void __launch_bounds__(256) k_bp_emit(DPtrs P)
+/* host-forced rebuilds (shx_contact_rebuild, apply_initial_conditions): one
+ * block per shell; a shell whose rebuild flag the host set clears its
+ * buckets if needed and joins work list 0 */
+__global__ void __launch_bounds__(256) k_bp_host_plan(DPtrs P)
 {
-  const int sid = uni(P.eshell[blockIdx.x * 256]);
+  const int sid = blockIdx.x;
   const SConst& C = P.cs[sid];
-  DShell* sh = P.sh + sid;
-  if (!bp_active(sh)) return;
-  const int nE = P.nE;
-  const int e1 = blockIdx.x * 256 + threadIdx.x;
-  const int ob = sh->pair_buf, nb = 1 - ob;
-  const double d2mx = sh->d2_max;
-  if (e1 - C.ebase < C.ne) {
-    const int nf = min(P.found_cnt[(size_t)nE + 1 + e1], kRowCap);
-    P.found_cnt[(size_t)nE + 1 + e1] = 0;
-    int32_t* fe = P.found_e2 + (size_t)e1 * kRowCap;
-    double* fd = P.found_d2 + (size_t)e1 * kRowCap;
-    for (int a = 1; a < nf; ++a) { /* insertion sort by e2 */
-      const int ke = fe[a];
-      const double kd = fd[a];
-      int b = a - 1;
-      while (b >= 0 && fe[b] > ke) { fe[b + 1] = fe[b]; fd[b + 1] = fd[b]; --b; }
-      fe[b + 1] = ke; fd[b + 1] = kd;
-    }
-    const int r0 = P.row_start[ob][e1 + sid], r1 = P.row_start[ob][e1 + 1 + sid];
-    int w = P.row_start[nb][e1 + sid];
-    int r = r0, f = 0;
-    while (true) {
-      while (r < r1 && !(P.pstick[ob][r] || !(P.pd2[ob][r] > d2mx))) ++r;
-      const bool hr = r < r1, hf = f < nf;
-      if (!hr && !hf) break;
-      if (hr && (!hf || P.pe2[ob][r] < fe[f])) {
-        P.pe1[nb][w] = e1; P.pe2[nb][w] = P.pe2[ob][r]; P.pd2[nb][w] = P.pd2[ob][r];
-        P.pstick[nb][w] = P.pstick[ob][r];
-        st3(P.pso[nb], w, ld3(P.pso[ob], r));
-        ++r;
-      } else {
-        P.pe1[nb][w] = e1; P.pe2[nb][w] = fe[f]; P.pd2[nb][w] = fd[f];
-        P.pstick[nb][w] = 0;
-        st3(P.pso[nb], w, mk(0, 0, 0));
-        ++f;
-      }
-      ++w;
-    }
-  }
-  /* clear the shell's bucket counts (a coalesced sweep; cheaper than
-   * revisiting each element's cells) */
-  const unsigned nbk = 1u << C.hb_bits;
-  for (unsigned b = (blockIdx.x - C.tile0) * 256 + threadIdx.x; b < nbk; b += C.ntile * 256)
-    P.bcount[C.bbase + b] = 0;
-  if (!last_block(P.ctr + kCtrPerShell * sid + 4, C.ntile)) return;
-  if (threadIdx.x == 0) {
-    sh->pair_buf = nb;
-    sh->n_pairs = P.row_start[nb][C.ebase + C.ne + sid] - C.pbase;
-    sh->rebuild = 0;
-  }
+  DShell& w = P.sh[sid];
+  if (!C.contact || !w.rebuild) return;
+  if (w.bclear) clear_buckets(P, C, 0, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) plan_rebuild(P, w, sid, 0);
+}
+
+/* after a host-forced rebuild: buckets cleared, flags and work list 0 reset */
+__global__ void __launch_bounds__(256) k_bp_host_done(DPtrs P)
+{
+  const int sid = blockIdx.x;
+  const SConst& C = P.cs[sid];
+  DShell& w = P.sh[sid];
+  if (sid == 0 && threadIdx.x == 0) P.wl[0][0] = 0;
+  if (!C.contact) return;
+  if (w.bclear) clear_buckets(P, C, 0, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) { w.bclear = 0; w.rebuild = 0; }
 }
 
 } // namespace shx

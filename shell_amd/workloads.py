@@ -55,3 +55,51 @@ def radial_imperfection(X, std, seed):
 def p_critical(E1, h, nu, R=1.0):
     """classical buckling pressure of a sphere, 2 E h^2 / (R^2 sqrt(3(1-nu^2)))"""
     return 2 * E1 * h * h / (R * R * np.sqrt(3 * (1 - nu * nu)))
+
+
+# level-3/4/5 mix of the batched sweep and the deflation range; the ramp is
+# given in time units so that shells with different stable steps see the
+# same loading history
+SWEEP_LEVELS = (3, 4, 5)
+SWEEP_MIX = (2, 2, 1)
+SWEEP_RAMP_TIME = 280.0
+
+
+def sweep_counts(n_shells, mix=SWEEP_MIX):
+    """shell counts per level for a mix ratio (largest remainder)"""
+    w = np.asarray(mix, dtype=float) / sum(mix)
+    c = np.floor(w * n_shells).astype(int)
+    rest = n_shells - c.sum()
+    order = np.argsort(-(w * n_shells - c))
+    c[order[:rest]] += 1
+    return [int(x) for x in c]
+
+
+def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX):
+    """Seeded batch of independent post-buckling problems.
+
+    Per shell: icosphere level by the mix (2:2:1 of 3/4/5 by default),
+    E2/E1 ~ U[0.2, 1], h/R ~ logU[0.005, 0.02], mu ~ U[0, 0.5], Gaussian
+    radial imperfection 0.01 h with its own seed, and deflation (bulk modulus
+    0.05, enclosed volume of the force configuration) to V/V0 ~ U[0.85, 0.92]
+    over SWEEP_RAMP_TIME, self-contact with friction, mass damping 0.05.
+    Returns a list of dicts with keys level, X, T, params (dt left to the
+    caller: it depends on the CFL bound of each shell)."""
+    rng = np.random.default_rng(seed)
+    counts = sweep_counts(n_shells, mix)
+    lv = np.concatenate([[L] * c for L, c in zip(levels, counts)])
+    meshes = {L: icosphere(L) for L in set(lv.tolist())}
+    out = []
+    for k, L in enumerate(lv):
+        e2 = rng.uniform(0.2, 1.0)
+        h = float(np.exp(rng.uniform(np.log(0.005), np.log(0.02))))
+        mu = rng.uniform(0.0, 0.5)
+        ratio = rng.uniform(0.85, 0.92)
+        iseed = int(rng.integers(1 << 30))
+        X, T = meshes[int(L)]
+        params = dict(E1=1.0, E2=e2, nu=0.3, G12=0.35, thickness=h, rho=1.0, damping_alpha=0.05,
+                      bulk_modulus=0.05, volume_final_ratio=ratio, volume_ramp_time=SWEEP_RAMP_TIME,
+                      volume_at_prediction=1, contact=1, mu_s=mu, mu_d=mu, cell_size_mult=4.0)
+        out.append(dict(level=int(L), X=radial_imperfection(X, 0.01 * h, iseed), T=T, params=params,
+                        imperfection_seed=iseed))
+    return out
