@@ -3,16 +3,22 @@
 force-and-step path on one MI355X, with the element-force kernel's HBM
 roofline and the CPU oracle timed on the host beside it.
 
-A benchmark "step" is one complete equilibrium solve of the workload: a
-level-4 icosphere shell (2562 nodes, 5120 Loop elements) deflated by a
-prescribed enclosed-volume reduction to 90% (deep in the post-buckled
-range: the sphere buckles after about 1.3% volume loss), relaxed by the
-damped explicit method until the residual of the deformation
-||f - M a_rigid|| / ||f_ext|| <= 1e-6.  Inputs (mesh, tables, state)
-are resident in HBM before the timed region; each solve restarts from the
-same initial state.  value = converged solves / wall seconds.
+Default workload (--mode batch): the batched sweep, 512 independent shells
+(levels 3/4/5 icospheres in ratio 2:2:1, 205/205/102 shells, 3.40 M
+elements), each with seeded E2/E1 ~ U[0.2,1], h/R ~ logU[0.005,0.02],
+mu ~ U[0,0.5], its own radial imperfection (0.01 h) and deflation to a seeded
+V/V0 ~ U[0.85,0.92] (deep in the post-buckled range), self-contact with
+friction, mass damping 0.05, dt = 0.5 x its own CFL bound.  A benchmark
+"step" is one solve of the whole batch: every shell relaxed by the damped
+explicit method until its residual of the deformation
+||f - M a_rigid|| / ||f_ext|| <= 1e-6 (or --max-steps).  value = converged
+shells / wall seconds of the solve; inputs (meshes, tables, state) are
+resident in HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+--mode single: one level-4 shell (E2/E1 = 0.5, h = 0.01, V/V0 = 0.9), the
+same method, one shell per launch.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode batch|single]
 """
 import argparse
 import json
@@ -26,60 +32,179 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import shell_amd as S  # noqa: E402
-from shell_amd.workloads import icosphere, p_critical, radial_imperfection  # noqa: E402
+from shell_amd.workloads import batch_sweep, icosphere, radial_imperfection  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "post-buckled equilibria/s at rel residual 1e-6 (plain FP64 GPU method); force GB/s"
 
 
-def workload(level):
-    """the single-GPU workload (level-4 single shell) and its parameters"""
-    h, E1, E2, nu, G12 = 0.01, 1.0, 0.5, 0.3, 0.35
-    X, T = icosphere(level)
-    X = radial_imperfection(X, 0.01 * h, seed=2024)
-    pcr = p_critical(E1, h, nu)
-    params = dict(E1=E1, E2=E2, nu=nu, G12=G12, thickness=h, rho=1.0, damping_alpha=0.05,
-                  contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0)
-    return X, T, params, pcr
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def element_kernel_bytes(ne, nv):
-    """algorithmic HBM bytes of one element-force launch (FP64):
+    """algorithmic HBM bytes of one element-force launch over a shell (FP64):
     per element: 12 int32 patch ids + 1 valence byte + 26 FP64 table rows
     (R 2x2, JxW, A1 A2 A11 A22 A12, ff1, ff2) read; 12 slot force
     contributions (36 FP64) and JxW a3 (3 FP64) written; element area read
-    and written;
-    per node: u, v, a (72 B) and the mesh position (24 B) read once."""
+    and written (585 B); per node: u, v, a and the mesh position read once
+    (96 B)."""
     per_elem = 12 * 4 + 1 + 26 * 8 + 39 * 8 + 2 * 8
     per_node = 4 * 3 * 8
     return ne * per_elem + nv * per_node
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2, help="timed equilibrium solves")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--level", type=int, default=4)
-    ap.add_argument("--max-steps", type=int, default=400000)
-    ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def single_workload(level=4):
+    h, E1, E2, nu, G12 = 0.01, 1.0, 0.5, 0.3, 0.35
+    X, T = icosphere(level)
+    X = radial_imperfection(X, 0.01 * h, seed=2024)
+    params = dict(E1=E1, E2=E2, nu=nu, G12=G12, thickness=h, rho=1.0, damping_alpha=0.05,
+                  contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0, cell_margin=0.1)
+    return X, T, params
 
-    X, T, prm, pcr = workload(args.level)
-    # stable step: half the central-difference limit 2/omega_max of M^-1 K
-    probe = S.ShellSystem(X, T, **prm)
-    omega = probe.cfl_omega_max(200)
+
+def with_cfl(batch, params, iters):
+    """dt = 0.5 * 2 / omega_max of M^-1 K per shell (power iteration on the GPU force kernel)"""
+    omega = batch.cfl_omega_max(iters)
+    return [dict(p, dt=1.0 / w, dt_max=1.0 / w) for p, w in zip(params, omega)]
+
+
+def roofline(b, meshes, steps=200):
+    """element-force kernel: algorithmic bytes / average launch time, HIP
+    events on the library's stream, over `steps` steps from the initial state"""
+    for i, (X, _) in enumerate(meshes):
+        b.set_state(i, np.zeros(X.size))
+    b.apply_initial_conditions()
+    b.set_profiling(True)
+    b.advance(steps)
+    kt = b.kernel_times()
+    b.set_profiling(False)
+    elem_ms = kt["elem_force"][0]
+    nbytes = sum(element_kernel_bytes(len(T), len(X)) for X, T in meshes)
+    achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_elem (element force)",
+            "bytes_per_launch": nbytes, "avg_launch_ms": elem_ms,
+            "kernel_ms": {k: v[0] for k, v in kt.items()}}
+
+
+def cpu_baseline(levels, steps_by_shell, runs_by_level, seconds):
+    """the CPU oracle (reference-faithful OpenMP mode) on the host: steps/s per
+    mesh level on a bounded sample of the same workload, converted to
+    equilibria/s with the per-shell step counts measured on the GPU"""
+    import oracle  # CPU baseline leg only
+    threads = os.cpu_count() or 1
+    rate = {}
+    parts = []
+    per = seconds / max(len(levels), 1)
+    for L in levels:
+        X, T = icosphere(L)
+        p = runs_by_level[L]
+        o = oracle.Oracle(X, T, **dict(p, omp_faithful=1))
+        o.set_state(np.zeros(X.size))
+        o.init_dynamics()
+        n, c0 = 0, time.perf_counter()
+        while time.perf_counter() - c0 < per:
+            o.step(20)
+            n += 20
+        el = time.perf_counter() - c0
+        rate[L] = n / el
+        parts.append(f"L{L}: {n} steps in {el:.1f} s ({rate[L]:.1f} steps/s)")
+    total_s = sum(steps / rate[L] for L, steps in steps_by_shell)
+    n_eq = len(steps_by_shell)
+    return {"value": n_eq / total_s, "unit": "equilibria/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle, OpenMP element loops with critical-section assembly, on the host; "
+                      + "; ".join(parts) + f"; equilibria/s = {n_eq} equilibria / sum over shells of "
+                      f"(GPU-measured steps to converge / steps per second at the shell's level)",
+            "steps_per_s_by_level": {f"L{k}": v for k, v in rate.items()}}
+
+
+def run_batch(args):
+    shells = batch_sweep(args.shells, seed=args.seed)
+    meshes = [(s["X"], s["T"]) for s in shells]
+    params = [s["params"] for s in shells]
+    t0 = time.perf_counter()
+    probe = S.ShellBatch(meshes, params)
+    params = with_cfl(probe, params, args.cfl_iters)
     probe.close()
-    dt = 0.5 * 2.0 / omega
+    b = S.ShellBatch(meshes, params)
+    if args.no_graph:
+        b.set_graphs(False)
+    log(f"setup {time.perf_counter() - t0:.1f} s: {len(shells)} shells, {sum(len(T) for _, T in meshes)} elements")
+
+    def solve():
+        for i, (X, _) in enumerate(meshes):
+            b.set_state(i, np.zeros(X.size))
+        b.apply_initial_conditions()
+        t1 = time.perf_counter()
+        done = 0
+        reps = None
+        while done < args.max_steps:
+            n = min(args.chunk, args.max_steps - done)
+            reps = b.advance(n, tol=args.tol, stop_on_converged=True)
+            done += n
+            nconv = sum(r["converged_step"] >= 0 for r in reps)
+            log(f"  {done} steps {time.perf_counter() - t1:.1f} s converged {nconv}/{len(reps)}")
+            if all(r["converged_step"] >= 0 or r["problem"] >= 7 for r in reps):
+                break
+        return reps
+
+    for _ in range(args.warmup):
+        solve()
+    walls, reps_all = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        reps = solve()
+        walls.append(time.perf_counter() - t1)
+        reps_all.append(reps)
+    wall = sum(walls)
+    converged = sum(sum(r["converged_step"] >= 0 for r in reps) for reps in reps_all)
+    last = reps_all[-1]
+    steps = [r["converged_step"] if r["converged_step"] >= 0 else r["n_steps"] for r in last]
+    levels = [s["level"] for s in shells]
+    by_level = {}
+    for L, n in zip(levels, steps):
+        by_level.setdefault(L, []).append(int(n))
+    rl = roofline(b, meshes)
+    cpu = None
+    if not args.no_cpu:
+        first = {}
+        for s, p in zip(shells, params):
+            first.setdefault(s["level"], p)
+        cpu = cpu_baseline(sorted(by_level), list(zip(levels, steps)), first, args.cpu_seconds)
+    counts = {f"L{L}": len(v) for L, v in sorted(by_level.items())}
+    out = {
+        "metric": METRIC, "value": converged / wall if wall > 0 else 0.0, "unit": "equilibria/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / max(args.steps, 1) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded icosphere shells with radial imperfections and seeded parameters",
+        "config": {"workload": f"batched sweep of {len(shells)} shells ({counts}), E2/E1~U[0.2,1], "
+                               f"h/R~logU[0.005,0.02], mu~U[0,0.5], imperfection 0.01h, deflation to "
+                               f"V/V0~U[0.85,0.92] over 280 time units (bulk modulus 0.05), self-contact with "
+                               f"friction, mass damping 0.05, dt=0.5*CFL per shell, relaxation to deformation "
+                               f"residual {args.tol:g}; one step = the whole batch solved",
+                   "shells": len(shells), "elements": int(sum(len(T) for _, T in meshes)),
+                   "nodes": int(sum(len(X) for X, _ in meshes)), "converged": int(converged),
+                   "solver_steps_to_converge_min_median_max": {
+                       f"L{L}": [int(np.min(v)), int(np.median(v)), int(np.max(v))] for L, v in sorted(by_level.items())},
+                   "max_steps": args.max_steps},
+        "roofline": rl, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_single(args):
+    X, T, prm = single_workload()
+    probe = S.ShellSystem(X, T, **prm)
+    omega = probe.cfl_omega_max(args.cfl_iters)
+    probe.close()
+    dt = 1.0 / omega
     ramp_steps = 5000
     run = dict(prm, bulk_modulus=0.05, volume_final_ratio=0.9, volume_ramp_time=ramp_steps * dt,
                volume_at_prediction=1, dt=dt, dt_max=dt)
     g = S.ShellSystem(X, T, **run)
+    if args.no_graph:
+        g.set_graphs(False)
     z = np.zeros(3 * len(X))
 
     def solve():
@@ -93,13 +218,10 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         reps.append(solve())
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    wall = time.perf_counter() - t0
     converged = sum(1 for r in reps if r["converged_step"] >= 0)
     steps_per_solve = float(np.mean([r["n_steps"] for r in reps]))
-    attempts_per_solve = steps_per_solve + float(np.mean([r["n_rejected"] for r in reps]))
-
-    # roofline of the element-force kernel: HIP events on the library's stream
+    attempts = steps_per_solve + float(np.mean([r["n_rejected"] for r in reps]))
     g.set_state(z, z, z, 0.0)
     g.apply_initial_conditions()
     g.set_profiling(True)
@@ -109,57 +231,54 @@ def main():
     elem_ms = kt["elem_force"][0]
     nbytes = element_kernel_bytes(len(T), len(X))
     achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
-
     cpu = None
     if not args.no_cpu:
-        import oracle  # CPU baseline leg only
-        threads = os.cpu_count() or 1
-        o = oracle.Oracle(X, T, **dict(run, omp_faithful=1))
-        o.set_state(z)
-        o.init_dynamics()
-        n, c0 = 0, time.perf_counter()
-        while time.perf_counter() - c0 < args.cpu_seconds:
-            o.step(100)
-            n += 100
-        cpu_wall = time.perf_counter() - c0
-        cpu_steps_s = n / cpu_wall
-        cpu = {"value": cpu_steps_s / attempts_per_solve, "unit": "equilibria/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{n} steps of the same workload from the same initial state on the CPU oracle "
-                         f"(OpenMP element loops, critical-section assembly), {cpu_wall:.1f} s; "
-                         f"equilibria/s = steps/s / {attempts_per_solve:.0f} step attempts per equilibrium "
-                         f"measured on the GPU",
-               "steps_per_s": cpu_steps_s}
-
-    ms_per_solve = wall / max(args.steps, 1) * 1e3
+        cpu = cpu_baseline([4], [(4, steps_per_solve)], {4: run}, args.cpu_seconds)
     out = {
-        "metric": "post-buckled equilibria/s at rel residual 1e-6 (plain FP64 GPU method); force GB/s",
-        "value": converged / wall if wall > 0 else 0.0,
-        "unit": "equilibria/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_solve,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
+        "metric": METRIC, "value": converged / wall if wall > 0 else 0.0, "unit": "equilibria/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / max(args.steps, 1) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: level-4 icosphere, seeded radial imperfection 0.01h",
-        "config": {"workload": f"level-{args.level} icosphere shell, h=0.01, E1=1, E2=0.5, nu12=0.3, G12=0.35, "
-                               f"imperfection 0.01h, deflation to V/V0=0.9 over {ramp_steps} steps (bulk modulus "
-                               f"0.05), self-contact mu=0.3, dt=0.5*CFL={dt:.4g}, mass damping 0.05, relaxation to "
-                               f"deformation residual {args.tol:g}",
+        "config": {"workload": f"one level-4 icosphere shell, h=0.01, E1=1, E2=0.5, nu12=0.3, G12=0.35, "
+                               f"imperfection 0.01h, deflation to V/V0=0.9 over {ramp_steps} steps (bulk "
+                               f"modulus 0.05), self-contact mu=0.3, dt=0.5*CFL={dt:.4g}, mass damping 0.05, "
+                               f"relaxation to deformation residual {args.tol:g}",
                    "nodes": len(X), "elements": len(T), "converged": converged,
-                   "solver_steps_per_equilibrium": steps_per_solve,
-                   "step_attempts_per_equilibrium": attempts_per_solve,
-                   "solver_steps_per_s": sum(r["n_steps"] + r["n_rejected"] for r in reps) / wall},
+                   "solver_steps_per_equilibrium": steps_per_solve, "step_attempts_per_equilibrium": attempts,
+                   "solver_steps_per_s": attempts * args.steps / wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_elem (element force)", "bytes_per_launch": nbytes,
-                     "avg_launch_ms": elem_ms, "kernel_ms": {k: v[0] for k, v in kt.items()}},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_elem (element force)",
+                     "bytes_per_launch": nbytes, "avg_launch_ms": elem_ms,
+                     "kernel_ms": {k: v[0] for k, v in kt.items()}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed solves")
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--mode", choices=["batch", "single"], default="batch")
+    ap.add_argument("--shells", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--max-steps", type=int, default=60000)
+    ap.add_argument("--chunk", type=int, default=2000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--cfl-iters", type=int, default=60)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch step attempts one by one (profilers)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.mode == "batch":
+        run_batch(args)
+    else:
+        run_single(args)
 
 
 if __name__ == "__main__":

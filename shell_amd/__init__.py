@@ -35,6 +35,8 @@ EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_
 
 _lib = None
 
+KERNEL_CLASSES = ["elem_force", "node_newmark", "contact_narrow", "shell_epilogues", "broad_phase"]
+
 
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
@@ -251,13 +253,22 @@ class ShellSystem:
     def set_graphs(self, enable=True):
         self._check(lib().shx_set_graphs(self.h, 1 if enable else 0), "shx_set_graphs")
 
+    def set_profiling(self, enable=True):
+        self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
+
     def kernel_times(self):
         ms = np.zeros(5)
         cnt = np.zeros(5, dtype=np.int64)
         self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
                     "shx_kernel_times")
-        names = ["elem_force", "node_newmark", "contact_narrow", "finalize", "broad_phase"]
-        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(names)}
+        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}
+
+    def kernel_times(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, dtype=np.int64)
+        self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+                    "shx_kernel_times")
+        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}
 
 
 def tri_distance(P):
@@ -368,3 +379,13 @@ class ShellBatch:
 
     def set_graphs(self, enable=True):
         self._check(lib().shx_set_graphs(self.h, 1 if enable else 0), "shx_set_graphs")
+
+    def set_profiling(self, enable=True):
+        self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
+
+    def kernel_times(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, dtype=np.int64)
+        self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+                    "shx_kernel_times")
+        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}

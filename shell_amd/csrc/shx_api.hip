@@ -47,6 +47,12 @@ struct shx_ctx {
   int64_t tcnt[5] = {0, 0, 0, 0, 0};
   hipGraphExec_t graph = nullptr;
   bool use_graph = true;
+  /* launch block -> global block maps of the active shells (tiles, -,
+   * element blocks, node blocks), host copies and device buffers, and the
+   * grid sizes they give */
+  std::vector<int32_t> bmap_h[4];
+  int32_t* bmap_d[4] = {nullptr, nullptr, nullptr, nullptr};
+  int grid[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -166,20 +172,50 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   prof_begin(ctx, 4);
   k_bp_box<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
   k_bp_count<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
-  k_bp_scan<<<ctx->S, 256, 0, ctx->st>>>(P, list);
+  k_bp_scan<<<ctx->S, 1024, 0, ctx->st>>>(P, list);
   k_bp_scatter<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
   k_bp_pairs<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
   prof_end(ctx);
+}
+
+/* the blocks of the active (not done) shells as the launch grids of the
+ * element, node and contact kernels; a captured graph is dropped when the
+ * grids change */
+int set_active(shx_ctx* ctx, const std::vector<DShell>& s)
+{
+  std::vector<int32_t> m[4];
+  for (int i = 0; i < ctx->S; ++i) {
+    if (s[i].done) continue;
+    const SConst& C = ctx->C[i];
+    for (int b = 0; b < C.ntile; ++b) m[0].push_back(C.tile0 + b);
+    for (int b = 0; b < C.nbE; ++b) m[2].push_back(C.blkE + b);
+    for (int b = 0; b < C.nbN; ++b) m[3].push_back(C.blkN + b);
+  }
+  bool changed = false;
+  for (int k : {0, 2, 3}) {
+    if (m[k] == ctx->bmap_h[k]) continue;
+    changed = true;
+    ctx->bmap_h[k].swap(m[k]);
+    ctx->grid[k] = (int)ctx->bmap_h[k].size();
+    if (ctx->grid[k] > 0)
+      HIPCHK(hipMemcpy(ctx->bmap_d[k], ctx->bmap_h[k].data(), sizeof(int32_t) * ctx->grid[k],
+                       hipMemcpyHostToDevice));
+  }
+  if (changed && ctx->graph) {
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  return 0;
 }
 
 /* element force + per-shell epilogue (energies, volume, predictor trigger) */
 void launch_elem(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 0);
-  k_elem<<<ctx->nblkE, kElemBlock, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[2]) k_elem<<<ctx->grid[2], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
   prof_begin(ctx, 3);
-  k_fin_elem<<<ctx->S, 256, 0, ctx->st>>>(ctx->P, mode);
+  k_fin_elem<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
 }
 
@@ -187,17 +223,17 @@ void launch_elem(shx_ctx* ctx, int mode)
 void launch_node(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 1);
-  k_node<<<ctx->nblkN, kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[3]) k_node<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
   prof_begin(ctx, 3);
-  k_fin_node<<<ctx->S, 256, 0, ctx->st>>>(ctx->P, mode);
+  k_fin_node<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
 }
 
 void launch_contact(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 2);
-  k_contact<<<ctx->ntile, 256, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[0]) k_contact<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
 }
 
@@ -379,10 +415,11 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     C.tile0 = (int)(eb / 256); C.ntile = nep / 256;
     C.nbL = nep / kBpPerBlock;
     if (C.contact) {
-      /* hashed cell grid: about two entries per bucket for the expected
-       * number of (cell, element) entries, (1/cell_size_mult + 2)^3 each */
+      /* hashed cell grid: a sixteenth of the bound (1/cell_size_mult + 2)^3
+       * on the (cell, element) entries per element; a shell's surface
+       * occupies far fewer cells than its grid box holds */
       const double kc = std::pow(1.0 / prm[s].cell_size_mult + 2.0, 3.0);
-      const double want = std::max(1024.0, (double)ne[s] * kc / 2.0);
+      const double want = std::max(1024.0, (double)ne[s] * kc / 16.0);
       unsigned bits = 10;
       while ((double)(1u << bits) < want && bits < 24) ++bits;
       C.hb_bits = bits;
@@ -487,15 +524,18 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
-      (r = dalloc(ctx, P.wl[0], (size_t)nshell + 1)) || (r = dalloc(ctx, P.wl[1], (size_t)nshell + 1)))
+      (r = dalloc(ctx, P.wl[0], (size_t)nshell + 1)) || (r = dalloc(ctx, P.wl[1], (size_t)nshell + 1)) ||
+      (r = dalloc(ctx, ctx->bmap_d[0], (size_t)ctx->ntile)) || (r = dalloc(ctx, ctx->bmap_d[2], (size_t)ctx->nblkE)) ||
+      (r = dalloc(ctx, ctx->bmap_d[3], (size_t)ctx->nblkN)))
     return fail(r);
+  for (int k = 0; k < 4; ++k) P.bmap[k] = ctx->bmap_d[k];
   for (int b = 0; b < 2; ++b)
     if ((r = dalloc(ctx, P.U[b], (size_t)3 * nV)) || (r = dalloc(ctx, P.V[b], (size_t)3 * nV)) ||
         (r = dalloc(ctx, P.A[b], (size_t)3 * nV)))
       return fail(r);
   if (ctx->any_contact) {
     if ((r = dalloc(ctx, P.ebox, (size_t)6 * nE)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ctx->ntile)) ||
-        (r = dalloc(ctx, P.ecell, (size_t)6 * nE)) || (r = dalloc(ctx, P.bcnt, (size_t)bb)) ||
+        (r = dalloc(ctx, P.ecell, (size_t)8 * nE)) || (r = dalloc(ctx, P.bcnt, (size_t)bb)) ||
         (r = dalloc(ctx, P.boff, (size_t)bb)) || (r = dalloc(ctx, P.bcur, (size_t)bb)) ||
         (r = dalloc(ctx, P.bent, (size_t)entb)))
       return fail(r);
@@ -585,6 +625,7 @@ int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* 
   if (int r = read_sh(ctx, s)) return r;
   for (auto& d : s) { d.done = 0; d.t_eval = t; }
   if (int r = write_sh(ctx, s)) return r;
+  if (int r = set_active(ctx, s)) return r;
   const SConst& C0 = ctx->C[0];
   const size_t nv = C0.nv, ne = C0.ne, bytes = sizeof(double) * 3 * nv;
   const int par = s[0].par;
@@ -640,6 +681,7 @@ int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
     d.rebuild = ctx->C[i].contact ? 1 : 0;
   }
   if (int r = write_sh(ctx, s)) return r;
+  if (int r = set_active(ctx, s)) return r;
   launch_elem(ctx, M_INIT);
   if (ctx->any_contact) {
     launch_broad(ctx, M_INIT, 0);
@@ -678,6 +720,7 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
     const long long max_attempts = 64LL * n_steps + 4096;
     long long enq = 0;
     while (true) {
+      if (int r = set_active(ctx, s)) return r;
       if (ctx->use_graph && !ctx->prof) {
         if (int r = launch_attempts_graph(ctx)) return r;
         enq += kGraphAttempts;
@@ -825,6 +868,7 @@ int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
   s = s0;
   for (auto& d : s) { d.done = 0; d.par = 0; d.t_eval = 0; }
   if (int r = write_sh(ctx, s)) return r;
+  if (int r = set_active(ctx, s)) return r;
   uint64_t st = 1;
   for (int q = 0; q < S; ++q) {
     const SConst& C = ctx->C[q];

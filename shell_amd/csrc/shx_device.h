@@ -26,8 +26,10 @@
  *  - contact candidates as fixed-capacity rows per element e1 (the other
  *    element e2 > e1, the last distance and the friction state of each
  *    pair), twice (ping-pong; a rebuild writes the other copy);
- *  - two work lists of the shells whose contact lists are being rebuilt
- *    (predictor and corrector trigger), which size the broad-phase work.
+ *  - work lists of the shells whose contact lists are being rebuilt
+ *    (predictor and corrector trigger), which size the broad-phase work,
+ *    and host-built maps from launch blocks to the blocks of the active
+ *    (not done) shells, which size the element, node and contact work.
  */
 #ifndef SHX_DEVICE_H
 #define SHX_DEVICE_H
@@ -47,8 +49,8 @@ constexpr int kAlign = 256;         /* shell ranges start at and are padded to m
 constexpr int kElemLanes = 4;       /* lanes cooperating on one element (3 patch slots each) */
 constexpr int kElemBlock = 256;     /* threads per element-kernel block: 64 elements */
 constexpr int kElemPerBlock = kElemBlock / kElemLanes;
-constexpr int kNodeLanes = 4;       /* lanes cooperating on one node */
-constexpr int kNodeBlock = 256;     /* threads per node-kernel block: 64 nodes */
+constexpr int kNodeLanes = 8;       /* lanes cooperating on one node */
+constexpr int kNodeBlock = 256;     /* threads per node-kernel block: 32 nodes */
 constexpr int kNodePerBlock = kNodeBlock / kNodeLanes;
 constexpr int kNodeRec = 22;        /* node-kernel partial record length */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
@@ -56,7 +58,7 @@ constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell e
 constexpr int kBpPerBlock = 256 / kBpLanes;
 constexpr int kRowCap = 64;         /* contact candidates per element row */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
-constexpr int kMaxShells = 2048;    /* shells per batch (broad-phase work maps are staged in LDS) */
+constexpr int kMaxShells = 1024;    /* shells per batch (work maps of the list-driven kernels are staged in LDS) */
 constexpr int kBpGrid = 1024;       /* blocks of the broad-phase kernels (they loop over the work) */
 
 /* per-shell constants (device array, read-only in the step kernels) */
@@ -145,14 +147,17 @@ struct DPtrs {
   /* contact */
   const int32_t* close_start; /* nE+1 */
   const int32_t* close_ids;
-  double* ebox;               /* 6*nE, SoA */
+  double* ebox;               /* 6*nE: per element min xyz, max xyz */
   double* bpart;              /* broad-phase tile partials: 7 per tile */
-  int32_t* ecell;             /* 6*nE: cell index ranges lo[3], hi[3] */
+  int32_t* ecell;             /* 8*nE: per element cell index range lo xyz, 0, hi xyz, 0 */
   uint32_t* bcnt;             /* hash buckets of the cell grids: entry counts (zero between rebuilds) */
   uint32_t* boff;             /* bucket start in the shell's entry region (counting sort) */
   uint32_t* bcur;             /* bucket fill cursor; bucket end after the scatter */
   uint64_t* bent;             /* entries sorted by bucket (cell << 32 | global element) */
-  int32_t* wl[2];             /* S+1 each: count, then the shells rebuilding */
+  int32_t* wl[2];             /* S+1 each: count, then the shells rebuilding (predictor,
+                                 corrector trigger) */
+  const int32_t* bmap[4];     /* per kernel family (tiles, -, element blocks, node blocks): launch
+                                 block -> global block of the active shells, set by the host */
   int32_t* rcnt[2];           /* nE: candidates in each element row */
   int32_t* pe2[2];            /* nE*kRowCap: slot e*kRowCap + k */
   double* pd2[2];

@@ -197,41 +197,31 @@ __global__ void __launch_bounds__(256) k_mass(DPtrs P)
 
 /* ------------------------------------------------------- block utilities */
 
-/* deterministic block reduction of NC components of nparts partial records
- * (record length `stride`, components comp[0..NC), op 0: sum, 1: max):
- * thread t folds records t, t+bs, ... in order (several loads in flight),
- * then a fixed tree over the threads.  s holds NC*blockDim doubles.
- * Results valid in every thread. */
+/* deterministic reduction of NC components of nparts partial records by
+ * one wave (64 lanes): lane l folds records l, l + 64, ... in order, then a
+ * fixed xor butterfly (each step adds the same two values in every lane, so
+ * all lanes end with bit-identical totals).  op[c]: 0 sum, 1 max. */
 template <int NC>
-__device__ void reduce_records(const double* part, int nparts, int stride, const int* comp, const int* op,
-                               double* out, double* s)
+__device__ void wave_reduce_records(const double* part, int nparts, int stride, const int* op, double* out)
 {
-  const int t = threadIdx.x, bs = blockDim.x;
+  const int l = threadIdx.x & 63;
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
-#pragma unroll 4
-  for (int b = t; b < nparts; b += bs) {
+  for (int b = l; b < nparts; b += 64) {
     const double* r = part + (size_t)b * stride;
 #pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = op[c] ? fmax(acc[c], r[c]) : acc[c] + r[c];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const double x = r[comp[c]];
-      acc[c] = op[c] ? fmax(acc[c], x) : acc[c] + x;
+      const double o = __shfl_xor(acc[c], off);
+      acc[c] = op[c] ? fmax(acc[c], o) : acc[c] + o;
     }
-  }
 #pragma unroll
-  for (int c = 0; c < NC; ++c) s[c * bs + t] = acc[c];
-  __syncthreads();
-  for (int w = bs / 2; w > 0; w >>= 1) {
-    if (t < w)
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-        s[c * bs + t] = op[c] ? fmax(s[c * bs + t], s[c * bs + t + w]) : s[c * bs + t] + s[c * bs + t + w];
-    __syncthreads();
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) out[c] = s[c * bs];
-  __syncthreads();
+  for (int c = 0; c < NC; ++c) out[c] = acc[c];
 }
 
 __device__ void set_newmark_dev(DShell& s, const SConst& C, double dt)
@@ -291,6 +281,61 @@ __device__ void plan_rebuild(const DPtrs& P, DShell& w, int sid, int list)
   if (k < P.S) P.wl[list][1 + k] = sid;
 }
 
+/* blocks of one shell for a kernel family: V_TILE 256-element tiles,
+ * V_LANE broad-phase lane blocks, V_ELEM element-kernel blocks, V_NODE
+ * node-kernel blocks */
+enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3 };
+__device__ __forceinline__ int vcount(const SConst& C, int kind)
+{
+  return kind == V_TILE ? C.ntile : kind == V_LANE ? C.nbL : kind == V_ELEM ? C.nbE : C.nbN;
+}
+
+/* the work of the shells on a work list as virtual blocks of a kernel
+ * family: s_sh[i] is the i-th listed shell, s_pref the exclusive prefix of
+ * their block counts; returns the number n of listed shells (s_pref[n] =
+ * total).  Kernels run a fixed grid whose blocks take virtual blocks
+ * blockIdx.x, blockIdx.x + gridDim.x, ... */
+__device__ int vmap_build(const DPtrs& P, const int32_t* wl, int kind, int* s_sh, int* s_pref, int* s_tmp)
+{
+  const int n = min(wl[0], P.S);
+  const int t = threadIdx.x, bs = blockDim.x;
+  const int chunk = (n + bs - 1) / bs;
+  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
+  int sum = 0;
+  for (int i = lo; i < hi; ++i) {
+    const int s = wl[1 + i];
+    s_sh[i] = s;
+    sum += vcount(P.cs[s], kind);
+  }
+  s_tmp[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < bs; off <<= 1) {
+    const int x = t >= off ? s_tmp[t - off] : 0;
+    __syncthreads();
+    s_tmp[t] += x;
+    __syncthreads();
+  }
+  int run = s_tmp[t] - sum;
+  for (int i = lo; i < hi; ++i) {
+    s_pref[i] = run;
+    run += vcount(P.cs[s_sh[i]], kind);
+  }
+  if (t == bs - 1) s_pref[n] = s_tmp[t];
+  __syncthreads();
+  return n;
+}
+
+/* listed shell of virtual block vb (binary search in s_pref) */
+__device__ __forceinline__ int vmap_find(int vb, int n, const int* s_pref)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_pref[mid] <= vb) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 /* --------------------------------------------------------- element force */
 
 /* kElemLanes lanes per element gather u_pred over the patch, form the
@@ -304,23 +349,26 @@ __device__ void plan_rebuild(const DPtrs& P, DShell& w, int sid, int list)
  * (ShellSystem.C:1236-1282) plus the LOCAL pressure load (ShellSystem.C:
  * 1570-1576).  Block partials (energies, volume terms, max |du|^2) go to
  * epart for k_fin_elem.  The blocks of a shell whose contact list was
- * rebuilt at the end of the previous step clear its cell buckets. */
+ * rebuilt at the end of the previous step clear its cell buckets.  Launch
+ * block b computes element block bmap[V_ELEM][b] (the blocks of the active
+ * shells, set by the host). */
 __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
-  __shared__ double s_red[5][kElemBlock];
-  const int sid = uni(P.eshell[blockIdx.x * kElemPerBlock]);
+  __shared__ double s_w[5][kElemBlock / 64];
+  const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
+  const int sid = uni(P.eshell[blk * kElemPerBlock]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
-  if (blockIdx.x == 0 && threadIdx.x == 0 && P.wl[1]) P.wl[1][0] = 0;
-  if (sh.bclear && C.contact) clear_buckets(P, C, blockIdx.x - C.blkE, C.nbE);
+  if (sh.bclear && C.contact) clear_buckets(P, C, blk - C.blkE, C.nbE);
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
   __syncthreads();
+  {
   const int nE = P.nE;
   const int t = threadIdx.x;
   const int lane = t % kElemLanes;
-  const int e = blockIdx.x * kElemPerBlock + t / kElemLanes;
+  const int e = blk * kElemPerBlock + t / kElemLanes;
   const bool valid = e - C.ebase < C.ne;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
@@ -451,19 +499,25 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
       dumax = 0;
     }
   }
-  s_red[0][t] = Et; s_red[1][t] = Eb; s_red[2][t] = dumax; s_red[3][t] = det; s_red[4][t] = oarea;
-  __syncthreads();
-  for (int s = kElemBlock / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      s_red[0][t] += s_red[0][t + s];
-      s_red[1][t] += s_red[1][t + s];
-      s_red[2][t] = fmax(s_red[2][t], s_red[2][t + s]);
-      s_red[3][t] += s_red[3][t + s];
-      s_red[4][t] += s_red[4][t + s];
+  /* block partial: wave butterflies, then the block's waves in order */
+  double rr[5] = {Et, Eb, dumax, det, oarea};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double o = __shfl_xor(rr[c], off);
+      rr[c] = c == 2 ? fmax(rr[c], o) : rr[c] + o;
     }
-    __syncthreads();
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s_w[c][t / 64] = rr[c];
+  __syncthreads();
+  if (t < 5) {
+    double acc = s_w[t][0];
+    for (int w = 1; w < kElemBlock / 64; ++w) acc = t == 2 ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
+    P.epart[(size_t)blk * 5 + t] = acc;
   }
-  if (t < 5) P.epart[(size_t)blockIdx.x * 5 + t] = s_red[t][0];
+  }
 }
 
 /* per-shell epilogue of the element kernel, one block per shell: reduces the
@@ -471,16 +525,17 @@ __global__ void __launch_bounds__(kElemBlock) k_elem(DPtrs P, int mode)
  * (compute_enclosed_volume, ShellSystem.C:2505-2541) and, in a step, the
  * predictor increment and the broad-phase trigger (check_increment(false),
  * ShellSystem.C:978, 993; ShellSystem.C:1917-1919) */
-__global__ void __launch_bounds__(256) k_fin_elem(DPtrs P, int mode)
+__global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
 {
-  __shared__ double s_red[5 * 256];
   const int sid = blockIdx.x;
   const SConst& C = P.cs[sid];
   DShell& w = P.sh[sid];
+  /* the corrector list of the previous attempt has been consumed */
+  if (sid == 0 && threadIdx.x == 0) P.wl[1][0] = 0;
   if (w.done) return;
-  const int comp[5] = {0, 1, 2, 3, 4}, op[5] = {0, 0, 1, 0, 0};
+  const int op[5] = {0, 0, 1, 0, 0};
   double r[5];
-  reduce_records<5>(P.epart + (size_t)C.blkE * 5, C.nbE, 5, comp, op, r, s_red);
+  wave_reduce_records<5>(P.epart + (size_t)C.blkE * 5, C.nbE, 5, op, r);
   if (threadIdx.x != 0) return;
   w.att_e_tens = r[0];
   w.att_e_bend = r[1];
@@ -514,17 +569,19 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
  * -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w) onto element 2 and
  * accumulated with FP64 atomics into fc (the node kernel reads and clears
  * it).  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
- * a shell whose list was rebuilt in this step clear its cell buckets. */
+ * a shell whose list was rebuilt in this step clear its cell buckets.
+ * Launch block b takes 256-element tile bmap[V_TILE][b]. */
 __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
 {
-  __shared__ double s_red[kCRec][256];
-  const int sid = uni(P.eshell[blockIdx.x * 256]);
+  __shared__ double s_w[kCRec][4];
+  const int tile = uni(P.bmap[V_TILE][blockIdx.x]);
+  const int sid = uni(P.eshell[tile * 256]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.wl[0][0] = 0;
   if (!C.contact) return;
-  if (sh.bclear) clear_buckets(P, C, blockIdx.x - C.tile0, C.ntile);
+  if (sh.bclear) clear_buckets(P, C, tile - C.tile0, C.ntile);
   if (sh.done) return;
+  {
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
   const int par = sh.par;
@@ -534,7 +591,7 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
   const int buf = sh.pair_buf;
   const int nE = P.nE;
   const double h = C.h;
-  const int e0 = blockIdx.x * 256 + threadIdx.x;
+  const int e0 = tile * 256 + threadIdx.x;
   double nc = 0, es = 0, ef = 0, npair = 0;
   if (e0 - C.ebase < C.ne) {
     const int np = min(P.rcnt[buf][e0], kRowCap);
@@ -610,14 +667,17 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
     }
   }
   const int t = threadIdx.x;
-  s_red[0][t] = nc; s_red[1][t] = es; s_red[2][t] = ef; s_red[3][t] = npair;
+  double rr[kCRec] = {nc, es, ef, npair};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < kCRec; ++c) rr[c] += __shfl_xor(rr[c], off);
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < kCRec; ++c) s_w[c][t / 64] = rr[c];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s)
-      for (int c = 0; c < kCRec; ++c) s_red[c][t] += s_red[c][t + s];
-    __syncthreads();
+  if (t < kCRec) P.cpart[(size_t)tile * kCRec + t] = ((s_w[t][0] + s_w[t][1]) + s_w[t][2]) + s_w[t][3];
   }
-  if (t < kCRec) P.cpart[(size_t)blockIdx.x * kCRec + t] = s_red[t][0];
 }
 
 /* ---------------------------------------------------- node gather + step */
@@ -662,19 +722,20 @@ __device__ double rigid_free_norm(double f2, const double* r)
  * ShellSystem.C:1068-1093) or reject with a smaller dt.  One block per
  * shell, after the node kernel; the shell's block partials are reduced in a
  * fixed order. */
-__global__ void __launch_bounds__(256) k_fin_node(DPtrs P, int mode)
+__global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
 {
-  __shared__ double s[kNodeRec * 256];
   const int sid = blockIdx.x;
   const SConst& C = P.cs[sid];
+  /* the predictor list of this attempt has been consumed */
+  if (sid == 0 && threadIdx.x == 0) P.wl[0][0] = 0;
   if (P.sh[sid].done) return;
-  int cn[kNodeRec], on[kNodeRec];
-  for (int q = 0; q < kNodeRec; ++q) { cn[q] = q; on[q] = (q >= 2 && q <= 4); }
+  int on[kNodeRec];
+  for (int q = 0; q < kNodeRec; ++q) on[q] = (q >= 2 && q <= 4);
   double rn[kNodeRec];
-  reduce_records<kNodeRec>(P.npart + (size_t)C.blkN * kNodeRec, C.nbN, kNodeRec, cn, on, rn, s);
-  const int cc[kCRec] = {0, 1, 2, 3}, oc[kCRec] = {0, 0, 0, 0};
+  wave_reduce_records<kNodeRec>(P.npart + (size_t)C.blkN * kNodeRec, C.nbN, kNodeRec, on, rn);
+  const int oc[kCRec] = {0, 0, 0, 0};
   double rc[kCRec];
-  reduce_records<kCRec>(P.cpart + (size_t)C.tile0 * kCRec, C.contact ? C.ntile : 0, kCRec, cc, oc, rc, s);
+  wave_reduce_records<kCRec>(P.cpart + (size_t)C.tile0 * kCRec, C.contact ? C.ntile : 0, kCRec, oc, rc);
   const double f2 = rn[0], fe2 = rn[1], err = rn[2], vmax = rn[3], du = rn[4], ek = rn[5];
   const double nc = rc[0], es = rc[1], ef = rc[2];
   if (threadIdx.x != 0) return;
@@ -743,21 +804,24 @@ __global__ void __launch_bounds__(256) k_fin_node(DPtrs P, int mode)
  * a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a, v = v_pred + gamma dt a,
  * written to the other state buffer.  Block partials: |f|^2, |f_ext|^2,
  * max|a-a_old|, max|v_pred|, max|du|^2, E_kin and the rigid-body sums, for
- * k_fin_node.  Force time: t + dt in a step,
+ * k_fin_node.  Launch block b computes node block bmap[V_NODE][b].  Force
+ * time: t + dt in a step,
  * t + dt in M_INIT (apply_initial_conditions), DShell::t_eval in M_FORCES. */
 __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
-  const int sid = uni(P.vshell[blockIdx.x * kNodePerBlock]);
+  const int blk = uni(P.bmap[V_NODE][blockIdx.x]);
+  const int sid = uni(P.vshell[blk * kNodePerBlock]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
   __syncthreads();
+  {
   const int t = threadIdx.x;
   const int lane = t % kNodeLanes;
-  const int n = blockIdx.x * kNodePerBlock + t / kNodeLanes;
+  const int n = blk * kNodePerBlock + t / kNodeLanes;
   const bool valid = n - C.vbase < C.nv;
   const int par = sh.par;
   const double dt = sh.dt;
@@ -851,7 +915,8 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
     const bool mx = (t >= 2 && t <= 4);
     double acc = s_wave[t][0];
     for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
-    P.npart[(size_t)blockIdx.x * kNodeRec + t] = acc;
+    P.npart[(size_t)blk * kNodeRec + t] = acc;
+  }
   }
 }
 
@@ -883,14 +948,35 @@ __device__ __forceinline__ bool bp_active(const DShell* sh)
   return (*(volatile const int*)&sh->rebuild) != 0 && (*(volatile const int*)&sh->err) == 0;
 }
 
-__device__ __forceinline__ double ebox(const DPtrs& P, int r, int e) { return P.ebox[(size_t)r * P.nE + e]; }
+/* element AABB records: 6 doubles per element (min xyz, max xyz), read
+ * with three 16-byte loads */
+struct Box { double lo[3], hi[3]; };
+__device__ __forceinline__ Box load_box(const DPtrs& P, int e)
+{
+  const double2* p = reinterpret_cast<const double2*>(P.ebox + (size_t)6 * e);
+  const double2 a = p[0], b = p[1], c = p[2];
+  Box r;
+  r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = b.x;
+  r.hi[0] = b.y; r.hi[1] = c.x; r.hi[2] = c.y;
+  return r;
+}
 
 /* BoundingBox::apart, BoundingBox.h:64-69 */
-__device__ bool boxes_apart(const DPtrs& P, int a, int b, double d)
+__device__ __forceinline__ bool boxes_apart(const Box& a, const Box& b, double d)
 {
   for (int q = 0; q < 3; ++q)
-    if (ebox(P, q, a) > ebox(P, 3 + q, b) + d || ebox(P, q, b) > ebox(P, 3 + q, a) + d) return true;
+    if (a.lo[q] > b.hi[q] + d || b.lo[q] > a.hi[q] + d) return true;
   return false;
+}
+
+/* element cell-index ranges: 8 ints per element (lo xyz, -, hi xyz, -) */
+__device__ __forceinline__ int4 load_cell_lo(const DPtrs& P, int e)
+{
+  return reinterpret_cast<const int4*>(P.ecell)[2 * (size_t)e];
+}
+__device__ __forceinline__ int4 load_cell_hi(const DPtrs& P, int e)
+{
+  return reinterpret_cast<const int4*>(P.ecell)[2 * (size_t)e + 1];
 }
 
 __device__ bool close_pair(const DPtrs& P, int e1, int e2)
@@ -910,57 +996,13 @@ __device__ __forceinline__ unsigned bucket_of(unsigned cell, unsigned bits)
   return (cell * 0x9E3779B1u) >> (32 - bits);
 }
 
-/* the work of the shells on a work list as virtual blocks: `kind` 0 counts
- * the shell's 256-element tiles, 1 its lane blocks.  s_sh[i] is the i-th
- * listed shell, s_pref the exclusive prefix of their block counts; returns
- * the number n of listed shells (s_pref[n] = total). */
-__device__ int vmap_build(const DPtrs& P, const int32_t* wl, int kind, int* s_sh, int* s_pref, int* s_tmp)
-{
-  const int n = min(wl[0], P.S);
-  const int t = threadIdx.x, bs = blockDim.x;
-  const int chunk = (n + bs - 1) / bs;
-  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
-  int sum = 0;
-  for (int i = lo; i < hi; ++i) {
-    const int s = wl[1 + i];
-    s_sh[i] = s;
-    sum += kind ? P.cs[s].nbL : P.cs[s].ntile;
-  }
-  s_tmp[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < bs; off <<= 1) {
-    const int x = t >= off ? s_tmp[t - off] : 0;
-    __syncthreads();
-    s_tmp[t] += x;
-    __syncthreads();
-  }
-  int run = s_tmp[t] - sum;
-  for (int i = lo; i < hi; ++i) {
-    s_pref[i] = run;
-    run += kind ? P.cs[s_sh[i]].nbL : P.cs[s_sh[i]].ntile;
-  }
-  if (t == bs - 1) s_pref[n] = s_tmp[t];
-  __syncthreads();
-  return n;
-}
-
-/* listed shell of virtual block vb (binary search in s_pref) */
-__device__ __forceinline__ int vmap_find(int vb, int n, const int* s_pref)
-{
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s_pref[mid] <= vb) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
 __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode, int list)
 {
+  if (P.wl[list][0] == 0) return;
   __shared__ double s_box[6][256];
   __shared__ double s_size[256];
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
-  const int nl = vmap_build(P, P.wl[list], 0, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], V_TILE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
   const int li = vmap_find(vb, nl, s_pref);
   const int sid = uni(s_sh[li]);
@@ -989,7 +1031,12 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode, int list)
         if (xx[q] > bmx[q]) bmx[q] = xx[q];
       }
     }
-    for (int q = 0; q < 3; ++q) { P.ebox[(size_t)q * nE + e] = bmn[q]; P.ebox[(size_t)(3 + q) * nE + e] = bmx[q]; }
+    {
+      double2* pb = reinterpret_cast<double2*>(P.ebox + (size_t)6 * e);
+      pb[0] = make_double2(bmn[0], bmn[1]);
+      pb[1] = make_double2(bmn[2], bmx[0]);
+      pb[2] = make_double2(bmx[1], bmx[2]);
+    }
     double dmax = bmx[0] - bmn[0];
     const double d1 = bmx[1] - bmn[1], d2 = bmx[2] - bmn[2];
     if (d1 > dmax) dmax = d1;
@@ -1104,8 +1151,9 @@ __device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz,
  * 1967-2010) and one count per (element, cell) in the cell's hash bucket */
 __global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
 {
+  if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
-  const int nl = vmap_build(P, P.wl[list], 1, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
@@ -1137,16 +1185,16 @@ __global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
     const int nE = P.nE;
     const int e = C.ebase + local * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
     if (e - C.ebase >= C.ne) continue;
+    const Box bx = load_box(P, e);
     int lo[3], hi[3];
     for (int q = 0; q < 3; ++q) {
-      lo[q] = (int)((ebox(P, q, e) - g.bmin[q]) / g.cell);
-      hi[q] = (int)((ebox(P, 3 + q, e) - g.bmin[q] + g.dmx) / g.cell);
+      lo[q] = (int)((bx.lo[q] - g.bmin[q]) / g.cell);
+      hi[q] = (int)((bx.hi[q] - g.bmin[q] + g.dmx) / g.cell);
     }
     if (k0 == 0) {
-      for (int q = 0; q < 3; ++q) {
-        P.ecell[(size_t)q * nE + e] = lo[q];
-        P.ecell[(size_t)(3 + q) * nE + e] = hi[q];
-      }
+      int4* pc = reinterpret_cast<int4*>(P.ecell) + 2 * (size_t)e;
+      pc[0] = make_int4(lo[0], lo[1], lo[2], 0);
+      pc[1] = make_int4(hi[0], hi[1], hi[2], 0);
       P.rcnt[sh->pair_buf][e] = 0;
     }
     const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
@@ -1160,11 +1208,13 @@ __global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
   }
 }
 
-/* prefix scan of the bucket counts of each listed shell (one block per
- * shell): bucket start offsets and fill cursors in the shell's entry region */
-__global__ void __launch_bounds__(256) k_bp_scan(DPtrs P, int list)
+/* prefix scan of the bucket counts of each listed shell (one block of
+ * 1024 threads per shell, tiles of 4096 buckets: 4 per thread, a wave scan
+ * and a scan of the 16 wave totals): bucket start offsets and fill cursors
+ * in the shell's entry region */
+__global__ void __launch_bounds__(1024) k_bp_scan(DPtrs P, int list)
 {
-  __shared__ uint32_t s[256];
+  __shared__ uint32_t s_wave[16];
   if ((int)blockIdx.x >= min(P.wl[list][0], P.S)) return;
   const int sid = P.wl[list][1 + blockIdx.x];
   const SConst& C = P.cs[sid];
@@ -1172,34 +1222,46 @@ __global__ void __launch_bounds__(256) k_bp_scan(DPtrs P, int list)
   if (!bp_active(sh)) return;
   const unsigned nbk = 1u << C.hb_bits;
   const uint32_t* cnt = P.bcnt + C.bbase;
-  const int t = threadIdx.x;
-  const unsigned chunk = (nbk + 255) / 256;
-  const unsigned lo = min(nbk, t * chunk), hi = min(nbk, lo + chunk);
-  uint32_t sum = 0;
-  for (unsigned i = lo; i < hi; ++i) sum += cnt[i];
-  s[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const uint32_t x = t >= off ? s[t - off] : 0;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t base = 0;
+  for (unsigned tile = 0; tile < nbk; tile += 4096) {
+    const unsigned i0 = tile + 4 * t;
+    uint32_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = i0 + k < nbk ? cnt[i0 + k] : 0;
+    const uint32_t sum = c[0] + c[1] + c[2] + c[3];
+    uint32_t x = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
     __syncthreads();
-    s[t] += x;
+    if (t == 0)
+      for (int q = 1; q < 16; ++q) s_wave[q] += s_wave[q - 1];
+    __syncthreads();
+    uint32_t run = base + (w ? s_wave[w - 1] : 0) + x - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i0 + k < nbk) {
+        P.boff[C.bbase + i0 + k] = run;
+        P.bcur[C.bbase + i0 + k] = run;
+        run += c[k];
+      }
+    base += s_wave[15];
     __syncthreads();
   }
-  uint32_t run = s[t] - sum;
-  for (unsigned i = lo; i < hi; ++i) {
-    P.boff[C.bbase + i] = run;
-    P.bcur[C.bbase + i] = run;
-    run += cnt[i];
-  }
-  if (t == 255 && s[255] > (uint32_t)C.ne * kEntPerElem) atomicOr(&sh->err, ERR_ENTRIES);
+  if (t == 0 && base > (uint32_t)C.ne * kEntPerElem) atomicOr(&sh->err, ERR_ENTRIES);
 }
 
 /* scatter pass of the counting sort: each (element, cell) entry to its
  * bucket's next free slot */
 __global__ void __launch_bounds__(256) k_bp_scatter(DPtrs P, int list)
 {
+  if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
-  const int nl = vmap_build(P, P.wl[list], 1, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
@@ -1210,9 +1272,9 @@ __global__ void __launch_bounds__(256) k_bp_scatter(DPtrs P, int list)
     const int e = C.ebase + (vb - s_pref[li]) * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
     if (e - C.ebase >= C.ne) continue;
     const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
-    const int* L = P.ecell;
-    const int lo[3] = {L[e], L[(size_t)nE + e], L[(size_t)2 * nE + e]};
-    const int hi[3] = {L[(size_t)3 * nE + e], L[(size_t)4 * nE + e], L[(size_t)5 * nE + e]};
+    const int4 l4 = load_cell_lo(P, e), h4 = load_cell_hi(P, e);
+    const int lo[3] = {l4.x, l4.y, l4.z};
+    const int hi[3] = {h4.x, h4.y, h4.z};
     const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
     const int total = (hi[0] - lo[0] + 1) * ny * nz;
     for (int k = k0; k < total; k += kBpLanes) {
@@ -1235,8 +1297,9 @@ __global__ void __launch_bounds__(256) k_bp_scatter(DPtrs P, int list)
  * arrival; the pair set does not. */
 __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list)
 {
+  if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
-  const int nl = vmap_build(P, P.wl[list], 1, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
@@ -1255,9 +1318,10 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
     const double dmx = sh->d_max, d2mx = sh->d2_max;
     const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
     const int ob = sh->pair_src, nb = sh->pair_buf;
-    const int* L = P.ecell;
-    const int lo1[3] = {L[e1], L[(size_t)nE + e1], L[(size_t)2 * nE + e1]};
-    const int hi1[3] = {L[(size_t)3 * nE + e1], L[(size_t)4 * nE + e1], L[(size_t)5 * nE + e1]};
+    const int4 l4 = load_cell_lo(P, e1), h4 = load_cell_hi(P, e1);
+    const int lo1[3] = {l4.x, l4.y, l4.z};
+    const int hi1[3] = {h4.x, h4.y, h4.z};
+    const Box b1 = load_box(P, e1);
     const size_t row = (size_t)e1 * kRowCap;
     const int nold = min(P.rcnt[ob][e1], kRowCap);
     if (k0 == 0)
@@ -1290,9 +1354,9 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
         const int e2 = (int)(uint32_t)ent;
         if (e2 <= e1) continue;
         /* count the pair once: in the first cell the two ranges share */
-        if (cx != max(lo1[0], L[e2]) || cy != max(lo1[1], L[(size_t)nE + e2]) ||
-            cz != max(lo1[2], L[(size_t)2 * nE + e2]))
-          continue;
+        const int4 l2 = load_cell_lo(P, e2);
+        if (cx != max(lo1[0], l2.x) || cy != max(lo1[1], l2.y) || cz != max(lo1[2], l2.z)) continue;
+        if (boxes_apart(b1, load_box(P, e2), dmx)) continue;
         if (close_pair(P, e1, e2)) continue;
         bool listed = false;
         for (int r = 0; r < nold; ++r)
@@ -1301,7 +1365,6 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
             break;
           }
         if (listed) continue;
-        if (boxes_apart(P, e1, e2, dmx)) continue;
         if (!have_x0) {
           for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * nE + e1], U, V, A, predict, dt, a8);
           have_x0 = true;

@@ -99,7 +99,8 @@ def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX):
         X, T = meshes[int(L)]
         params = dict(E1=1.0, E2=e2, nu=0.3, G12=0.35, thickness=h, rho=1.0, damping_alpha=0.05,
                       bulk_modulus=0.05, volume_final_ratio=ratio, volume_ramp_time=SWEEP_RAMP_TIME,
-                      volume_at_prediction=1, contact=1, mu_s=mu, mu_d=mu, cell_size_mult=4.0)
+                      volume_at_prediction=1, contact=1, mu_s=mu, mu_d=mu, cell_size_mult=4.0,
+                      cell_margin=0.1)
         out.append(dict(level=int(L), X=radial_imperfection(X, 0.01 * h, iseed), T=T, params=params,
                         imperfection_seed=iseed))
     return out
