@@ -143,13 +143,15 @@ int shx_patches(shx_ctx* ctx, int32_t* idx /* ne*12, -1 padded */, int32_t* tri 
  * force kernel); the stable central-difference step is 2/omega. */
 int shx_cfl_omega_max(shx_ctx* ctx, int iters, double* omega);
 
-/* Device-resident timing hooks for benchmarks: the HIP stream the context
- * launches on (hipStream_t) and the average duration in ms of each kernel
- * class measured with HIP events over the last shx_step call when
- * profiling is enabled (shx_set_profiling). Classes: 0 element force,
- * 1 node gather + Newmark, 2 contact narrow phase, 3 finalize, 4 broad phase. */
+/* Timing hooks for benchmarks: with profiling enabled (shx_set_profiling;
+ * step attempts are then launched one by one), HIP events on the context's
+ * stream bracket every launch, and shx_kernel_times gives the average
+ * duration in ms and the launch count of each kernel class since profiling
+ * was enabled.  Classes: 0 element force, 1 node gather + Newmark, 2 contact
+ * narrow phase, 3 per-shell epilogues, 4..8 broad phase box, count, scan,
+ * scatter, pairs. */
 int shx_set_profiling(shx_ctx* ctx, int enable);
-int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 5 */, int64_t* counts /* 5 */);
+int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 9 */, int64_t* counts /* 9 */);
 
 /* Batches of independent shells (no counterpart in the reference, which runs
  * one ShellSystem per process; the batch replaces running many of them side

@@ -35,7 +35,8 @@ EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_
 
 _lib = None
 
-KERNEL_CLASSES = ["elem_force", "node_newmark", "contact_narrow", "shell_epilogues", "broad_phase"]
+KERNEL_CLASSES = ["elem_force", "node_newmark", "contact_narrow", "shell_epilogues", "bp_box", "bp_count",
+                  "bp_scan", "bp_scatter", "bp_pairs"]
 
 
 class Params(ctypes.Structure):
@@ -257,15 +258,15 @@ class ShellSystem:
         self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
 
     def kernel_times(self):
-        ms = np.zeros(5)
-        cnt = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
         self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
                     "shx_kernel_times")
         return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}
 
     def kernel_times(self):
-        ms = np.zeros(5)
-        cnt = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
         self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
                     "shx_kernel_times")
         return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}
@@ -384,8 +385,8 @@ class ShellBatch:
         self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
 
     def kernel_times(self):
-        ms = np.zeros(5)
-        cnt = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
         self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
                     "shx_kernel_times")
         return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}

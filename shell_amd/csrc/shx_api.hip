@@ -24,6 +24,10 @@ using namespace shx;
 
 namespace {
 constexpr int kGraphAttempts = 64;   /* step attempts per replayed graph */
+/* profiled kernel classes: element force, node + Newmark, contact narrow
+ * phase, per-shell epilogues, and the broad-phase box, count, scan,
+ * scatter and pair kernels */
+constexpr int kProfClasses = 9;
 }
 
 struct shx_ctx {
@@ -43,8 +47,8 @@ struct shx_ctx {
   bool prof = false;
   std::vector<hipEvent_t> evpool;
   std::vector<int> evclass;
-  double tsum[5] = {0, 0, 0, 0, 0};
-  int64_t tcnt[5] = {0, 0, 0, 0, 0};
+  double tsum[kProfClasses] = {};
+  int64_t tcnt[kProfClasses] = {};
   hipGraphExec_t graph = nullptr;
   bool use_graph = true;
   /* launch block -> global block maps of the active shells (tiles, -,
@@ -171,9 +175,17 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   const DPtrs& P = ctx->P;
   prof_begin(ctx, 4);
   k_bp_box<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  prof_end(ctx);
+  prof_begin(ctx, 5);
   k_bp_count<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
+  prof_end(ctx);
+  prof_begin(ctx, 6);
   k_bp_scan<<<ctx->S, 1024, 0, ctx->st>>>(P, list);
+  prof_end(ctx);
+  prof_begin(ctx, 7);
   k_bp_scatter<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
+  prof_end(ctx);
+  prof_begin(ctx, 8);
   k_bp_pairs<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
   prof_end(ctx);
 }
@@ -941,7 +953,7 @@ int shx_set_profiling(shx_ctx* ctx, int enable)
   if (!ctx) return SHX_E_ARG;
   prof_collect(ctx);
   ctx->prof = enable != 0;
-  for (int k = 0; k < 5; ++k) { ctx->tsum[k] = 0; ctx->tcnt[k] = 0; }
+  for (int k = 0; k < kProfClasses; ++k) { ctx->tsum[k] = 0; ctx->tcnt[k] = 0; }
   return SHX_OK;
 }
 
@@ -956,7 +968,7 @@ int shx_kernel_times(shx_ctx* ctx, double* avg_ms, int64_t* counts)
 {
   if (!ctx) return SHX_E_ARG;
   prof_collect(ctx);
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < kProfClasses; ++k) {
     if (avg_ms) avg_ms[k] = ctx->tcnt[k] ? ctx->tsum[k] / ctx->tcnt[k] : 0.0;
     if (counts) counts[k] = ctx->tcnt[k];
   }

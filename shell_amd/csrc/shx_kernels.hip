@@ -1348,14 +1348,34 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
       const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
       const unsigned b = C.bbase + bucket_of(c, C.hb_bits);
       const unsigned qend = P.bcur[b];
-      for (unsigned q = P.boff[b] + sub; q < qend; q += lpc) {
-        const uint64_t ent = P.bent[(size_t)C.entbase + q];
-        if ((unsigned)(ent >> 32) != c) continue;
-        const int e2 = (int)(uint32_t)ent;
-        if (e2 <= e1) continue;
-        /* count the pair once: in the first cell the two ranges share */
-        const int4 l2 = load_cell_lo(P, e2);
-        if (cx != max(lo1[0], l2.x) || cy != max(lo1[1], l2.y) || cz != max(lo1[2], l2.z)) continue;
+      const uint64_t* ents = P.bent + C.entbase;
+      /* entries four at a time: the loads of each stage are issued together
+       * (cell id and e2 > e1 from the entry, then the first-common-cell test
+       * on e2's index range); the survivors go on one by one */
+      for (unsigned q0 = P.boff[b] + sub; q0 < qend; q0 += 4 * lpc) {
+        int cand[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned q = q0 + j * lpc;
+          cand[j] = -1;
+          if (q < qend) {
+            const uint64_t ent = ents[q];
+            const int e2 = (int)(uint32_t)ent;
+            if ((unsigned)(ent >> 32) == c && e2 > e1) cand[j] = e2;
+          }
+        }
+        int4 l2[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (cand[j] >= 0) l2[j] = load_cell_lo(P, cand[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          /* count the pair once: in the first cell the two ranges share */
+          if (cand[j] >= 0 && (cx != max(lo1[0], l2[j].x) || cy != max(lo1[1], l2[j].y) || cz != max(lo1[2], l2[j].z)))
+            cand[j] = -1;
+        for (int j = 0; j < 4; ++j) {
+        const int e2 = cand[j];
+        if (e2 < 0) continue;
         if (boxes_apart(b1, load_box(P, e2), dmx)) continue;
         if (close_pair(P, e1, e2)) continue;
         bool listed = false;
@@ -1382,6 +1402,7 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
         P.pd2[nb][pn] = hit.d2;
         P.pstick[nb][pn] = 0;
         st3(P.pso[nb], pn, mk(0, 0, 0));
+        }
       }
     }
   }
