@@ -43,6 +43,10 @@ enum {
   SHX_SMALL_TIMESTEP, SHX_CONTACT_PROBLEM, SHX_NOT_FINITE
 };
 
+/* Element-force precision.  SHX_FP32_SPLIT: metric and curvature changes
+ * formed in FP32 from the displacement gradients relative to the reference
+ * surface, FP32 element forces and node gather; predictor, volume, contact,
+ * Newmark update and all reductions FP64. */
 enum { SHX_FP64 = 0, SHX_FP32_SPLIT = 1 };
 
 /* Parameters: the config.cfg keys of ShellSystem::set_parameters
@@ -86,7 +90,8 @@ typedef struct {
   int32_t n_contacts, n_pairs, n_rebuilds, problem;
   int32_t n_steps, n_rejected;     /* accepted steps and rejected attempts since shx_init_dynamics */
   int32_t converged_step;          /* first accepted step with t >= load end and residual_deform <= tol, or -1 */
-  int32_t reserved;
+  int32_t n_steps_fp32;            /* accepted steps taken with FP32 split forces (SHX_FP32_SPLIT runs them until
+                                      residual_deform <= 1000 tol after the load ramp, then FP64 finishes) */
   double residual_deform;          /* ||f - M a_rigid|| / ||f_ext||: residual with the least-squares rigid-body
                                       inertial force removed (a free-floating shell drifts under the follower load) */
 } shx_report;

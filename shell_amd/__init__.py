@@ -56,7 +56,7 @@ class Report(ctypes.Structure):
         "t", "dt", "residual", "fnorm", "fext_norm", "e_tens", "e_bend", "e_self", "e_fric", "e_kin",
         "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
         "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_steps", "n_rejected", "converged_step",
-        "reserved")] + [("residual_deform", ctypes.c_double)]
+        "n_steps_fp32")] + [("residual_deform", ctypes.c_double)]
 
 
 DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_stiff_mult=1.0,

@@ -52,11 +52,11 @@ struct shx_ctx {
   hipGraphExec_t graph = nullptr;
   bool use_graph = true;
   /* launch block -> global block maps of the active shells (tiles, -,
-   * element blocks, node blocks), host copies and device buffers, and the
-   * grid sizes they give */
-  std::vector<int32_t> bmap_h[4];
-  int32_t* bmap_d[4] = {nullptr, nullptr, nullptr, nullptr};
-  int grid[4] = {0, 0, 0, 0};
+   * FP64 element blocks, node blocks, FP32 element blocks), host copies and
+   * device buffers, and the grid sizes they give */
+  std::vector<int32_t> bmap_h[5];
+  int32_t* bmap_d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int grid[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -195,16 +195,16 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
  * grids change */
 int set_active(shx_ctx* ctx, const std::vector<DShell>& s)
 {
-  std::vector<int32_t> m[4];
+  std::vector<int32_t> m[5];
   for (int i = 0; i < ctx->S; ++i) {
     if (s[i].done) continue;
     const SConst& C = ctx->C[i];
     for (int b = 0; b < C.ntile; ++b) m[0].push_back(C.tile0 + b);
-    for (int b = 0; b < C.nbE; ++b) m[2].push_back(C.blkE + b);
+    for (int b = 0; b < C.nbE; ++b) m[s[i].eprec ? 4 : 2].push_back(C.blkE + b);
     for (int b = 0; b < C.nbN; ++b) m[3].push_back(C.blkN + b);
   }
   bool changed = false;
-  for (int k : {0, 2, 3}) {
+  for (int k : {0, 2, 3, 4}) {
     if (m[k] == ctx->bmap_h[k]) continue;
     changed = true;
     ctx->bmap_h[k].swap(m[k]);
@@ -225,6 +225,7 @@ void launch_elem(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 0);
   if (ctx->grid[2]) k_elem<<<ctx->grid[2], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[4]) k_elem32<<<ctx->grid[4], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
   prof_begin(ctx, 3);
   k_fin_elem<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
@@ -283,6 +284,20 @@ int launch_attempts_graph(shx_ctx* ctx)
   return 0;
 }
 
+int check_precision(shx_ctx* ctx, int precision)
+{
+  if (precision != SHX_FP64 && precision != SHX_FP32_SPLIT) {
+    ctx->err = "unknown precision";
+    return SHX_E_ARG;
+  }
+  return 0;
+}
+
+/* FP32 split steps of a shell go on until its residual of the deformation
+ * is within this factor of the tolerance after the load ramp; FP64 finishes
+ * (FP32 forces have a residual floor of order 1e-4 on stiff shells) */
+constexpr double kFp32Switch = 1000.0;
+
 int check_err(shx_ctx* ctx, const std::vector<DShell>& s)
 {
   for (int i = 0; i < ctx->S; ++i)
@@ -303,7 +318,7 @@ void fill_report(const DShell& s, shx_report* rep)
   rep->e_kin = s.e_kin; rep->volume = s.volume; rep->area = s.area; rep->du_max = s.du_max;
   rep->n_contacts = s.n_contacts; rep->n_pairs = s.n_pairs; rep->n_rebuilds = s.n_rebuilds;
   rep->problem = s.problem; rep->n_steps = s.n_steps; rep->n_rejected = s.n_rejected;
-  rep->converged_step = s.converged_step; rep->reserved = 0;
+  rep->converged_step = s.converged_step; rep->n_steps_fp32 = s.n_fp32;
   rep->residual_deform = s.residual_deform;
 }
 
@@ -532,15 +547,17 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * nE)) || (r = dalloc(ctx, P.minv, nV)) ||
       (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.qe, (size_t)36 * nE)) ||
       (r = dalloc(ctx, P.g0, (size_t)3 * nE)) || (r = dalloc(ctx, P.area, nE)) ||
+      (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) || (r = dalloc(ctx, P.qe32, (size_t)36 * nE)) ||
+      (r = dalloc(ctx, P.g032, (size_t)3 * nE)) ||
       (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
       (r = dalloc(ctx, P.wl[0], (size_t)nshell + 1)) || (r = dalloc(ctx, P.wl[1], (size_t)nshell + 1)) ||
       (r = dalloc(ctx, ctx->bmap_d[0], (size_t)ctx->ntile)) || (r = dalloc(ctx, ctx->bmap_d[2], (size_t)ctx->nblkE)) ||
-      (r = dalloc(ctx, ctx->bmap_d[3], (size_t)ctx->nblkN)))
+      (r = dalloc(ctx, ctx->bmap_d[3], (size_t)ctx->nblkN)) || (r = dalloc(ctx, ctx->bmap_d[4], (size_t)ctx->nblkE)))
     return fail(r);
-  for (int k = 0; k < 4; ++k) P.bmap[k] = ctx->bmap_d[k];
+  for (int k = 0; k < 5; ++k) P.bmap[k] = ctx->bmap_d[k];
   for (int b = 0; b < 2; ++b)
     if ((r = dalloc(ctx, P.U[b], (size_t)3 * nV)) || (r = dalloc(ctx, P.V[b], (size_t)3 * nV)) ||
         (r = dalloc(ctx, P.A[b], (size_t)3 * nV)))
@@ -560,6 +577,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   }
 
   k_precompute<<<ctx->ntile, 256, 0, ctx->st>>>(P);
+  k_precompute32<<<ctx->ntile, 256, 0, ctx->st>>>(P);
   k_mass<<<nV / 256, 256, 0, ctx->st>>>(P);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->st) != hipSuccess) {
     ctx->err = "setup kernels failed";
@@ -632,10 +650,10 @@ int shx_forces(shx_ctx* ctx, int precision, const double* u_pred, const double* 
                double* f_ext, double* energies, double* elem_area)
 {
   if (!ctx || !u_pred) return SHX_E_ARG;
-  if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
+  if (int r = check_precision(ctx, precision)) return r;
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  for (auto& d : s) { d.done = 0; d.t_eval = t; }
+  for (auto& d : s) { d.done = 0; d.t_eval = t; d.eprec = precision; }
   if (int r = write_sh(ctx, s)) return r;
   if (int r = set_active(ctx, s)) return r;
   const SConst& C0 = ctx->C[0];
@@ -684,12 +702,13 @@ int shx_contact_rebuild(shx_ctx* ctx, int force)
 int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
 {
   if (!ctx) return SHX_E_ARG;
-  if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
+  if (int r = check_precision(ctx, precision)) return r;
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
   for (int i = 0; i < ctx->S; ++i) {
     DShell& d = s[i];
     d.done = 0; d.n_steps = 0; d.n_rejected = 0; d.converged_step = -1; d.fresh = 1; d.attempts = 0;
+    d.eprec = precision; d.n_fp32 = 0;
     d.rebuild = ctx->C[i].contact ? 1 : 0;
   }
   if (int r = write_sh(ctx, s)) return r;
@@ -711,7 +730,7 @@ int shx_init_dynamics(shx_ctx* ctx, int precision) { return shx_batch_init_dynam
 int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged, shx_report* reps)
 {
   if (!ctx || n_steps < 0) return SHX_E_ARG;
-  if (precision != SHX_FP64) { ctx->err = "precision not available"; return SHX_E_ARG; }
+  if (int r = check_precision(ctx, precision)) return r;
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
   if (n_steps > 0) {
@@ -724,9 +743,14 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
       }
     if (cchange)
       if (int r = write_consts(ctx, ctx->C)) return r;
-    for (auto& d : s) {
+    auto fp32_done = [&](const DShell& d, const SConst& C) {
+      return d.t >= C.load_end_time && d.n_steps > 0 && d.residual_deform <= kFp32Switch * tol;
+    };
+    for (int i = 0; i < ctx->S; ++i) {
+      DShell& d = s[i];
       d.steps_target = d.n_steps + n_steps;
       d.done = (d.problem >= 7 || d.err || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
+      d.eprec = (precision == SHX_FP32_SPLIT && !fp32_done(d, ctx->C[i])) ? SHX_FP32_SPLIT : SHX_FP64;
     }
     if (int r = write_sh(ctx, s)) return r;
     const long long max_attempts = 64LL * n_steps + 4096;
@@ -746,6 +770,13 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
       }
       if (int r = read_sh(ctx, s)) return r;
       prof_collect(ctx);
+      if (precision == SHX_FP32_SPLIT) {
+        bool sw = false;
+        for (int i = 0; i < ctx->S; ++i)
+          if (s[i].eprec == SHX_FP32_SPLIT && fp32_done(s[i], ctx->C[i])) { s[i].eprec = SHX_FP64; sw = true; }
+        if (sw)
+          if (int r = write_sh(ctx, s)) return r;
+      }
       bool all_done = true;
       for (const auto& d : s) all_done = all_done && (d.done || d.err);
       if (all_done) break;
@@ -878,7 +909,7 @@ int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
   for (auto& c : Ci) { c.pressure_final = 0; c.bulk_modulus = 0; c.contact = 0; }
   if (int r = write_consts(ctx, Ci)) return r;
   s = s0;
-  for (auto& d : s) { d.done = 0; d.par = 0; d.t_eval = 0; }
+  for (auto& d : s) { d.done = 0; d.par = 0; d.t_eval = 0; d.eprec = SHX_FP64; }
   if (int r = write_sh(ctx, s)) return r;
   if (int r = set_active(ctx, s)) return r;
   uint64_t st = 1;

@@ -38,11 +38,21 @@
 
 namespace shx {
 
-/* element table rows */
+/* element table rows (FP64) */
 enum {
   T_R00 = 0, T_R01, T_R10, T_R11, T_JXW,
   T_A1 = 5, T_A2 = 8, T_A11 = 11, T_A22 = 14, T_A12 = 17,
   T_FF1 = 20, T_FF2 = 23, T_XYZ = 26, T_ROWS = 29
+};
+
+/* element table rows of the FP32 split path: R, JxW, the reference
+ * tangents and second derivatives, the reference unit normal A3 and
+ * J = |A1 x A2|, and the orthotropic H (ShellSystem.C:1199-1213, with
+ * 1/det(ff1_ref)^2 applied; unique entries H00 H01 H02 H11 H12 H22) */
+enum {
+  F_R00 = 0, F_R01, F_R10, F_R11, F_JXW,
+  F_A1 = 5, F_A2 = 8, F_A11 = 11, F_A22 = 14, F_A12 = 17,
+  F_A3 = 20, F_J = 23, F_H = 24, F_ROWS = 30
 };
 
 constexpr int kAlign = 256;         /* shell ranges start at and are padded to multiples of this */
@@ -106,6 +116,8 @@ struct DShell {
   int nd[3], n_cells;
   int n_pairs, pair_buf, pair_src, bclear;
   int steps_target, done, attempts;
+  int eprec;                       /* element-force precision of the shell (set by the host): 0 FP64, 1 FP32 split */
+  int n_fp32;                      /* accepted steps taken with FP32 split forces */
   /* results of the element kernel of the current attempt */
   double att_e_tens, att_e_bend, att_volume, att_area;
   double fdeform, residual_deform;
@@ -137,6 +149,9 @@ struct DPtrs {
   double* A[2];
   double* qe;                 /* 36*nE: per element and patch slot, internal-force contribution */
   double* g0;                 /* 3*nE: JxW a3 per element (pressure per unit p) */
+  float* tab32;               /* F_ROWS*nE: tables of the FP32 split path */
+  float* qe32;                /* 36*nE, g032 3*nE: FP32 split path counterparts of qe, g0 */
+  float* g032;
   double* area;               /* nE */
   double* fc;                 /* nV*3 contact force accumulator */
   double* fout;               /* nV*3 outputs of a plain force evaluation */
@@ -156,8 +171,9 @@ struct DPtrs {
   uint64_t* bent;             /* entries sorted by bucket (cell << 32 | global element) */
   int32_t* wl[2];             /* S+1 each: count, then the shells rebuilding (predictor,
                                  corrector trigger) */
-  const int32_t* bmap[4];     /* per kernel family (tiles, -, element blocks, node blocks): launch
-                                 block -> global block of the active shells, set by the host */
+  const int32_t* bmap[5];     /* per kernel family (tiles, -, FP64 element blocks, node blocks, FP32
+                                 element blocks): launch block -> global block of the active shells,
+                                 set by the host */
   int32_t* rcnt[2];           /* nE: candidates in each element row */
   int32_t* pe2[2];            /* nE*kRowCap: slot e*kRowCap + k */
   double* pd2[2];

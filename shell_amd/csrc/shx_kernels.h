@@ -16,12 +16,17 @@ namespace shx {
  * acceleration a = M^-1 f (apply_initial_conditions) */
 enum { M_STEP = 0, M_FORCES = 1, M_INIT = 2 };
 
+/* element-force precision (values of SHX_FP64 / SHX_FP32_SPLIT) */
+enum { SHX_PREC_FP64 = 0, SHX_PREC_FP32 = 1 };
+
 __global__ void k_precompute(DPtrs P);
 __global__ void k_mass(DPtrs P);
 __global__ void k_elem(DPtrs P, int mode);
 __global__ void k_fin_elem(DPtrs P, int mode);
 __global__ void k_contact(DPtrs P, int mode);
 __global__ void k_node(DPtrs P, int mode);
+__global__ void k_precompute32(DPtrs P);
+__global__ void k_elem32(DPtrs P, int mode);
 __global__ void k_fin_node(DPtrs P, int mode);
 __global__ void k_bp_box(DPtrs P, int posmode, int list);
 __global__ void k_bp_count(DPtrs P, int list);

@@ -284,10 +284,10 @@ __device__ void plan_rebuild(const DPtrs& P, DShell& w, int sid, int list)
 /* blocks of one shell for a kernel family: V_TILE 256-element tiles,
  * V_LANE broad-phase lane blocks, V_ELEM element-kernel blocks, V_NODE
  * node-kernel blocks */
-enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3 };
+enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3, V_ELEM32 = 4 };
 __device__ __forceinline__ int vcount(const SConst& C, int kind)
 {
-  return kind == V_TILE ? C.ntile : kind == V_LANE ? C.nbL : kind == V_ELEM ? C.nbE : C.nbN;
+  return kind == V_TILE ? C.ntile : kind == V_LANE ? C.nbL : kind == V_NODE ? C.nbN : C.nbE;
 }
 
 /* the work of the shells on a work list as virtual blocks of a kernel
@@ -552,6 +552,249 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   else w.rebuild = 0;
 }
 
+/* ------------------------------------------------ FP32 split element force */
+
+/* FP32 small-vector helpers (device only) */
+struct v3f { float x, y, z; };
+__device__ __forceinline__ v3f mkf(float x, float y, float z) { v3f r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ v3f addf(v3f a, v3f b) { return mkf(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3f subf(v3f a, v3f b) { return mkf(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3f sclf(v3f a, float s) { return mkf(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dotf(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3f crsf(v3f a, v3f b)
+{
+  return mkf(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ void axpyf(v3f& a, v3f b, float s) { a.x += b.x * s; a.y += b.y * s; a.z += b.z * s; }
+__device__ __forceinline__ v3f ldrow3f(const float* t, int row, int nE, int e)
+{
+  return mkf(t[(size_t)row * nE + e], t[(size_t)(row + 1) * nE + e], t[(size_t)(row + 2) * nE + e]);
+}
+
+/* tables of the FP32 split path from the FP64 reference geometry: rounded
+ * once, after every reference quantity (A3, J, H) is formed in FP64 */
+__global__ void __launch_bounds__(256) k_precompute32(DPtrs P)
+{
+  const int nE = P.nE;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const SConst& C = P.cs[uni(P.eshell[blockIdx.x * blockDim.x])];
+  if (e - C.ebase >= C.ne) return;
+  const double* T = P.tab;
+  float* F = P.tab32;
+  auto put = [&](int row, double v) { F[(size_t)row * nE + e] = (float)v; };
+  for (int r = 0; r < 20; ++r) put(r, T[(size_t)r * nE + e]); /* R, JxW, A1 A2 A11 A22 A12 */
+  const v3 A1 = ldrow3(T, T_A1, nE, e), A2 = ldrow3(T, T_A2, nE, e);
+  const v3 N = crs(A1, A2);
+  const double J = nrm(N);
+  put(F_A3, N.x / J); put(F_A3 + 1, N.y / J); put(F_A3 + 2, N.z / J);
+  put(F_J, J);
+  const v3 g = ldrow3(T, T_FF1, nE, e);
+  /* orthotropic H, ShellSystem.C:1199-1213 */
+  double H[6];
+  H[0] = C.E1 * g.y * g.y;
+  H[1] = C.E1 * (C.nu * g.y * g.x + (1 - C.nu) * g.z * g.z);
+  H[2] = C.E1 * -g.y * g.z;
+  H[3] = C.E2 * g.x * g.x;
+  H[4] = C.E2 * -g.x * g.z;
+  H[5] = C.G12 * ((1 - C.nu * C.nu2) * g.y * g.x + (1 - C.nu * C.nu2) * ((1 + C.nu) / (1 - C.nu)) * g.z * g.z);
+  const double invdet = 1 / (g.x * g.y - g.z * g.z);
+  for (int q = 0; q < 6; ++q) put(F_H + q, H[q] * invdet * invdet);
+}
+
+/* FP32 split element force: the same method as k_elem, with the metric and
+ * curvature changes formed in FP32 from the displacement gradients relative
+ * to the reference surface, so that no difference of nearly equal numbers
+ * is rounded:
+ *   a_a.a_b - A_a.A_b = A_a.du_b + du_a.A_b + du_a.du_b,
+ *   n = a1 x a2 = N + dn,  dn = A1 x du2 + du1 x A2 + du1 x du2,
+ *   |n| - |N| = (2 N.dn + dn.dn) / (|n| + |N|),
+ *   a3 - A3 = dn / |n| - N (|n| - |N|) / (|n| |N|),
+ *   ff2_ref - ff2 = -(A_ab.(a3 - A3) + du_ab.a3).
+ * The displacements enter the FP32 gradient sums relative to the patch's
+ * first node (formed in FP64).  The predictor, the element vertex positions
+ * of the enclosed volume and all reductions stay FP64.  Slot contributions
+ * and JxW a3 are written in FP32 (qe32, g032). */
+__global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
+{
+  __shared__ float s_rb[4 * 72];
+  __shared__ double s_w[5][kElemBlock / 64];
+  const int blk = uni(P.bmap[V_ELEM32][blockIdx.x]);
+  const int sid = uni(P.eshell[blk * kElemPerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
+  if (sh.bclear && C.contact) clear_buckets(P, C, blk - C.blkE, C.nbE);
+  if (sh.done) return;
+  for (int q = threadIdx.x; q < 4 * 72; q += blockDim.x) s_rb[q] = (float)P.rawb[q];
+  __syncthreads();
+  const int nE = P.nE;
+  const int t = threadIdx.x;
+  const int lane = t % kElemLanes;
+  const int e = blk * kElemPerBlock + t / kElemLanes;
+  const bool valid = e - C.ebase < C.ne;
+  const bool predict = (mode == M_STEP);
+  const double dt = sh.dt, a8 = sh.nm[8];
+  const int par = sh.par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* Aa = P.A[par];
+  const float* T = P.tab32;
+  double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  v3f du1 = mkf(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
+  v3 xa = mk(0, 0, 0), xb = xa, xc = xa;
+  float R00 = 0, R01 = 0, R10 = 0, R11 = 0;
+  int N = 6;
+  if (valid) {
+    R00 = T[(size_t)F_R00 * nE + e]; R01 = T[(size_t)F_R01 * nE + e];
+    R10 = T[(size_t)F_R10 * nE + e]; R11 = T[(size_t)F_R11 * nE + e];
+    N = P.vcls[e];
+    const float* rb = s_rb + 72 * (N - 3);
+    /* displacements relative to the patch's first node, formed in FP64 and
+     * then rounded: the gradient sums are unchanged (the basis derivatives
+     * sum to zero over the patch) and the rounding acts on |u_i - u_0| */
+    v3 u0;
+    {
+      const int n0 = P.pidx[e];
+      u0 = ld3(U, n0);
+      if (predict) u0 = pred3(u0, ld3(V, n0), ld3(Aa, n0), dt, a8);
+    }
+#pragma unroll
+    for (int k = 0; k < 12 / kElemLanes; ++k) {
+      const int i = lane + kElemLanes * k;
+      const int n = P.pidx[(size_t)i * nE + e];
+      const v3 u = ld3(U, n);
+      v3 up = u;
+      if (predict) up = pred3(u, ld3(V, n), ld3(Aa, n), dt, a8);
+      const v3 ur = sub(up, u0);
+      const v3f uf = mkf((float)ur.x, (float)ur.y, (float)ur.z);
+      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+      axpyf(du1, uf, dx * R00 + de * R10);
+      axpyf(du2, uf, dx * R01 + de * R11);
+      axpyf(du11, uf, dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10);
+      axpyf(du22, uf, dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11);
+      axpyf(du12, uf, dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11);
+      if (i == 0 || i == 1 || i == N) {
+        if (predict) {
+          const v3 d = sub(up, u);
+          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+        }
+        const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
+        if (i == 0) xa = x;
+        else if (i == 1) xb = x;
+        else xc = x;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < kElemLanes; off <<= 1) {
+    v3f* vs[5] = {&du1, &du2, &du11, &du22, &du12};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      vs[q]->x += __shfl_xor(vs[q]->x, off);
+      vs[q]->y += __shfl_xor(vs[q]->y, off);
+      vs[q]->z += __shfl_xor(vs[q]->z, off);
+    }
+    v3* xs[3] = {&xa, &xb, &xc};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      xs[q]->x += __shfl_xor(xs[q]->x, off);
+      xs[q]->y += __shfl_xor(xs[q]->y, off);
+      xs[q]->z += __shfl_xor(xs[q]->z, off);
+    }
+    dumax = fmax(dumax, __shfl_xor(dumax, off));
+  }
+  if (valid) {
+    const float* rb = s_rb + 72 * (N - 3);
+    const v3f A1 = ldrow3f(T, F_A1, nE, e), A2 = ldrow3f(T, F_A2, nE, e);
+    const v3f A11 = ldrow3f(T, F_A11, nE, e), A22 = ldrow3f(T, F_A22, nE, e), A12 = ldrow3f(T, F_A12, nE, e);
+    const v3f A3 = ldrow3f(T, F_A3, nE, e);
+    const float J = T[(size_t)F_J * nE + e];
+    const v3f Nn = sclf(A3, J);
+    const v3f a1 = addf(A1, du1), a2 = addf(A2, du2);
+    /* membrane strain from the differences, alpha_3 doubled */
+    float alpha[3] = {dotf(A1, du1) + 0.5f * dotf(du1, du1), dotf(A2, du2) + 0.5f * dotf(du2, du2),
+                      dotf(A1, du2) + dotf(du1, A2) + dotf(du1, du2)};
+    const v3f dn = addf(addf(crsf(A1, du2), crsf(du1, A2)), crsf(du1, du2));
+    const v3f nvec = addf(Nn, dn);
+    const float jac = sqrtf(dotf(nvec, nvec));
+    const float djac = (2 * dotf(Nn, dn) + dotf(dn, dn)) / (jac + J);
+    const v3f d3 = subf(sclf(dn, 1 / jac), sclf(Nn, djac / (jac * J)));
+    const v3f a3 = addf(A3, d3);
+    const v3f a11 = addf(A11, du11), a22 = addf(A22, du22), a12 = addf(A12, du12);
+    /* bending strain ff2_ref - ff2, beta_3 doubled */
+    float beta[3] = {-(dotf(A11, d3) + dotf(du11, a3)), -(dotf(A22, d3) + dotf(du22, a3)),
+                     -2 * (dotf(A12, d3) + dotf(du12, a3))};
+    const float ff2[3] = {dotf(a11, a3), dotf(a22, a3), dotf(a12, a3)};
+    const float* Hs = T + (size_t)F_H * nE + e;
+    const float H00 = Hs[0], H01 = Hs[(size_t)nE], H02 = Hs[(size_t)2 * nE], H11 = Hs[(size_t)3 * nE],
+                H12 = Hs[(size_t)4 * nE], H22 = Hs[(size_t)5 * nE];
+    const float Hm[9] = {H00, H01, H02, H01, H11, H12, H02, H12, H22};
+    const float K = (float)C.K, D = (float)C.D;
+    float nn[3], mm[3];
+    for (int r = 0; r < 3; ++r) {
+      nn[r] = K * (Hm[r * 3] * alpha[0] + Hm[r * 3 + 1] * alpha[1] + Hm[r * 3 + 2] * alpha[2]);
+      mm[r] = D * (Hm[r * 3] * beta[0] + Hm[r * 3 + 1] * beta[1] + Hm[r * 3 + 2] * beta[2]);
+    }
+    const v3f a2xa3 = crsf(a2, a3), a3xa1 = crsf(a3, a1);
+    const float invjac = 1 / jac;
+    const v3f p11 = sclf(addf(crsf(a11, a2), sclf(a2xa3, ff2[0])), invjac);
+    const v3f p12 = sclf(addf(crsf(a22, a2), sclf(a2xa3, ff2[1])), invjac);
+    const v3f p13 = sclf(addf(crsf(a12, a2), sclf(a2xa3, ff2[2])), invjac);
+    const v3f p21 = sclf(addf(crsf(a1, a11), sclf(a3xa1, ff2[0])), invjac);
+    const v3f p22 = sclf(addf(crsf(a1, a22), sclf(a3xa1, ff2[1])), invjac);
+    const v3f p23 = sclf(addf(crsf(a1, a12), sclf(a3xa1, ff2[2])), invjac);
+    const float JxW = T[(size_t)F_JXW * nE + e];
+    const float m2x2 = 2 * mm[2];
+    const v3f g1 = addf(addf(addf(addf(sclf(a1, nn[0]), sclf(a2, nn[2])), sclf(p11, mm[0])), sclf(p12, mm[1])),
+                        sclf(p13, m2x2));
+    const v3f g2 = addf(addf(addf(addf(sclf(a2, nn[1]), sclf(a1, nn[2])), sclf(p21, mm[0])), sclf(p22, mm[1])),
+                        sclf(p23, m2x2));
+    const v3f G[6] = {sclf(a3, JxW), sclf(g1, JxW), sclf(g2, JxW), sclf(a3, JxW * mm[0]), sclf(a3, JxW * mm[1]),
+                      sclf(a3, JxW * m2x2)};
+    float* qrow = P.qe32 + (size_t)36 * e;
+#pragma unroll
+    for (int k = 0; k < 12 / kElemLanes; ++k) {
+      const int i = lane + kElemLanes * k;
+      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+      const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
+      const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
+      const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
+      const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
+      const v3f q = subf(subf(subf(addf(sclf(G[1], b1), sclf(G[2], b2)), sclf(G[3], b11)), sclf(G[4], b22)),
+                         sclf(G[5], b12));
+      qrow[3 * i] = q.x; qrow[3 * i + 1] = q.y; qrow[3 * i + 2] = q.z;
+    }
+    if (lane == 0) {
+      P.g032[3 * e] = G[0].x; P.g032[3 * e + 1] = G[0].y; P.g032[3 * e + 2] = G[0].z;
+      Et = 0.5 * (double)JxW * ((double)nn[0] * alpha[0] + (double)nn[1] * alpha[1] + (double)nn[2] * alpha[2]);
+      Eb = 0.5 * (double)JxW * ((double)mm[0] * beta[0] + (double)mm[1] * beta[1] + (double)mm[2] * beta[2]);
+      const double new_area = 0.5 * (double)jac;
+      oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
+      P.area[e] = new_area;
+      const v3 a = xa, b = xb, c = xc;
+      det = a.x * (b.y * c.z - c.y * b.z) - b.x * (a.y * c.z - c.y * a.z) + c.x * (a.y * b.z - b.y * a.z);
+    } else {
+      dumax = 0;
+    }
+  }
+  double rr[5] = {Et, Eb, dumax, det, oarea};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double o = __shfl_xor(rr[c], off);
+      rr[c] = c == 2 ? fmax(rr[c], o) : rr[c] + o;
+    }
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s_w[c][t / 64] = rr[c];
+  __syncthreads();
+  if (t < 5) {
+    double acc = s_w[t][0];
+    for (int w = 1; w < kElemBlock / 64; ++w) acc = t == 2 ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
+    P.epart[(size_t)blk * 5 + t] = acc;
+  }
+}
+
 /* --------------------------------------------------------------- contact */
 
 __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, const double* V, const double* A,
@@ -774,6 +1017,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     w.par = 1 - w.par;
     w.t += dt;
     w.n_steps++;
+    if (w.eprec == SHX_PREC_FP32) w.n_fp32++;
     w.fresh = 1;
     w.residual = w.fext_norm > 0 ? w.fnorm / w.fext_norm : w.fnorm;
     w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
@@ -826,18 +1070,36 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
   const int par = sh.par;
   const double dt = sh.dt;
   v3 fint = mk(0, 0, 0), fp = fint;
-  if (valid) {
-    for (int q = P.inc_start[n] + lane; q < P.inc_start[n + 1]; q += kNodeLanes) {
-      const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
-      const double phi = s_rb[72 * (P.vcls[e] - 3) + i];
-      const double* qs = P.qe + (size_t)36 * e + 3 * i;
-      fint = sub(fint, mk(qs[0], qs[1], qs[2]));
-      axpy(fp, ld3(P.g0, e), phi);
+  if (sh.eprec == SHX_PREC_FP32) {
+    /* FP32 split path: FP32 contributions, FP32 accumulation */
+    v3f fi = mkf(0, 0, 0), fq = fi;
+    if (valid)
+      for (int q = P.inc_start[n] + lane; q < P.inc_start[n + 1]; q += kNodeLanes) {
+        const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
+        const float phi = (float)s_rb[72 * (P.vcls[e] - 3) + i];
+        const float* qs = P.qe32 + (size_t)36 * e + 3 * i;
+        fi = subf(fi, mkf(qs[0], qs[1], qs[2]));
+        axpyf(fq, mkf(P.g032[3 * e], P.g032[3 * e + 1], P.g032[3 * e + 2]), phi);
+      }
+    for (int off = 1; off < kNodeLanes; off <<= 1) {
+      fi.x += __shfl_xor(fi.x, off); fi.y += __shfl_xor(fi.y, off); fi.z += __shfl_xor(fi.z, off);
+      fq.x += __shfl_xor(fq.x, off); fq.y += __shfl_xor(fq.y, off); fq.z += __shfl_xor(fq.z, off);
     }
-  }
-  for (int off = 1; off < kNodeLanes; off <<= 1) {
-    fint.x += __shfl_xor(fint.x, off); fint.y += __shfl_xor(fint.y, off); fint.z += __shfl_xor(fint.z, off);
-    fp.x += __shfl_xor(fp.x, off); fp.y += __shfl_xor(fp.y, off); fp.z += __shfl_xor(fp.z, off);
+    fint = mk(fi.x, fi.y, fi.z);
+    fp = mk(fq.x, fq.y, fq.z);
+  } else {
+    if (valid)
+      for (int q = P.inc_start[n] + lane; q < P.inc_start[n + 1]; q += kNodeLanes) {
+        const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
+        const double phi = s_rb[72 * (P.vcls[e] - 3) + i];
+        const double* qs = P.qe + (size_t)36 * e + 3 * i;
+        fint = sub(fint, mk(qs[0], qs[1], qs[2]));
+        axpy(fp, ld3(P.g0, e), phi);
+      }
+    for (int off = 1; off < kNodeLanes; off <<= 1) {
+      fint.x += __shfl_xor(fint.x, off); fint.y += __shfl_xor(fint.y, off); fint.z += __shfl_xor(fint.z, off);
+      fp.x += __shfl_xor(fp.x, off); fp.y += __shfl_xor(fp.y, off); fp.z += __shfl_xor(fp.z, off);
+    }
   }
   /* partial record: |f|^2, |f_ext|^2, max|a-a_old|, max|v_pred|, max|du|^2,
    * E_kin, then the rigid-body least-squares sums B^T f (6: m f, m x*f) and
