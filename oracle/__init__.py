@@ -4,8 +4,9 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 import this package, and only as the checker or as the timed CPU baseline.
 The product library never loads it.
 
-Parity status of each oracle function (what pins it) is listed in
-oracle/PINNING.md.
+Each oracle function is pinned by tests/test_oracle_pins.py (closed forms,
+invariants, brute force, and the SHA-256 of the reference's mesh files);
+tests/test_golden.py keeps it reproducing the committed fixtures.
 """
 import ctypes
 import os

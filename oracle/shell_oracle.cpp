@@ -15,8 +15,12 @@
  *    which pair forces are summed;
  *  - the averaged element size is summed in blocks of 256 elements (serial
  *    inside a block, then serial over blocks) so that a GPU can reproduce it
- *    bit for bit;
- *  - constant dt only (adapt_dt_every = 0 in every workload used here).
+ *    bit for bit.
+ * Parity unpinned (no closed form or reference data to check against):
+ *  - the adaptive time-step control of advance (ShellSystem.C:1027-1047),
+ *    held only by agreement with the GPU path;
+ *  - the residual of the deformation (rigid-free residual), a build
+ *    extension with no counterpart in the reference.
  */
 #include "shell_oracle.h"
 #include "oracle_geom.h"

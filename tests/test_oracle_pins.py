@@ -420,3 +420,26 @@ def test_energy_conservation_undamped(icosphere):
         E.append(r["e_kin"] + r["e_tens"] + r["e_bend"])
     E = np.array(E)
     assert (E.max() - E.min()) / E.mean() < 2e-2
+
+
+def test_enclosed_volume_is_the_polyhedral_volume(icosphere):
+    """compute_enclosed_volume (ShellSystem.C:2505-2541): |sum of det(x0,x1,x2)|/6
+    over the control triangles minus half the thickness layer; the first term
+    equals the polyhedron volume from the divergence theorem, close to 4 pi/3"""
+    X, T = icosphere(3)
+    h = 0.01
+    o = O.Oracle(X, T, thickness=h, dt=1e-6, dt_max=1e-6)
+    o.set_state(np.zeros(X.size))
+    o.init_dynamics()
+    r = o.step(1)
+    a, b, c = X[T[:, 0]], X[T[:, 1]], X[T[:, 2]]
+    poly = np.sum(np.einsum("ij,ij->i", a, np.cross(b, c))) / 6.0
+    # the same volume as a sum over faces of (centroid . area vector) / 3
+    cen = (a + b + c) / 3.0
+    poly2 = np.sum(np.einsum("ij,ij->i", cen, 0.5 * np.cross(b - a, c - a))) / 3.0
+    assert abs(poly - poly2) < 1e-12
+    assert abs(r["volume"] + r["area"] * h / 2 - poly) < 1e-12 * poly
+    assert 0.98 * 4 * np.pi / 3 < poly < 4 * np.pi / 3
+    # area: one-point quadrature of the limit surface, which lies inside the
+    # control net (about 1% below 4 pi at level 3)
+    assert abs(r["area"] - 4 * np.pi) < 0.02 * 4 * np.pi
