@@ -1,0 +1,45 @@
+"""Run a fixed number of explicit step attempts of the benchmark workloads,
+for profilers (rocprofv3 --kernel-trace / --pmc): the level-4 single shell
+or the seeded batched sweep, FP64 or FP32 split, attempts launched one by
+one (no graph).
+
+usage: python -m shell_amd.profile_steps single|batch STEPS [fp32]
+"""
+import sys
+
+import numpy as np
+
+import shell_amd as S
+from shell_amd.workloads import batch_sweep, icosphere, radial_imperfection
+
+# stable steps of the sweep's meshes (half the power-iteration CFL bound of
+# the unit icosphere with E1 = 1, rho = 1: omega_max 8.6 / 17.9 / 37.0)
+DT = {3: 0.5 * 2.0 / 8.6, 4: 0.5 * 2.0 / 17.9, 5: 0.5 * 2.0 / 37.0}
+
+
+def main():
+    what, steps = sys.argv[1], int(sys.argv[2])
+    prec = S.FP32_SPLIT if len(sys.argv) > 3 and sys.argv[3] == "fp32" else S.FP64
+    if what == "single":
+        X, T = icosphere(4)
+        X = radial_imperfection(X, 1e-4, seed=2024)
+        p = dict(thickness=0.01, E2=0.5, damping_alpha=0.05, contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0,
+                 cell_margin=0.1, bulk_modulus=0.05, volume_final_ratio=0.9, volume_ramp_time=5000 * DT[4],
+                 volume_at_prediction=1, dt=DT[4], dt_max=DT[4])
+        meshes, params = [(X, T)], [p]
+    else:
+        shells = batch_sweep(512, seed=7)
+        meshes = [(s["X"], s["T"]) for s in shells]
+        params = [dict(s["params"], dt=DT[s["level"]], dt_max=DT[s["level"]]) for s in shells]
+    b = S.ShellBatch(meshes, params)
+    b.set_graphs(False)
+    for i, (X, _) in enumerate(meshes):
+        b.set_state(i, np.zeros(X.size))
+    b.apply_initial_conditions(precision=prec)
+    reps = b.advance(steps, precision=prec)
+    print(f"{what}: {steps} steps, {sum(r['n_rebuilds'] for r in reps)} rebuilds, "
+          f"{sum(len(T) for _, T in meshes)} elements", flush=True)
+
+
+if __name__ == "__main__":
+    main()
