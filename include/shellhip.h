@@ -186,6 +186,33 @@ int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega /* nshell */)
  * attempts; 0: launched one by one.  Profiling launches one by one. */
 int shx_set_graphs(shx_ctx* ctx, int enable);
 
+/* Learned warm start (no counterpart in the reference): a multilayer
+ * perceptron width[0] -> width[1] -> ... -> width[n_layers] with GELU on the
+ * hidden layers, trained and evaluated on the GPU by the library's own
+ * kernels (BF16 matrix-core GEMMs with FP32 accumulation, Adam on FP32
+ * master weights).  Host arrays are row-major FP32, one sample per row.
+ * Parameters are laid out layer by layer: W (width[l] x width[l+1], row
+ * major) then b (width[l+1]). */
+typedef struct shx_mlp shx_mlp;
+int shx_mlp_create(shx_mlp** out, int n_layers, const int* width /* n_layers+1 */, uint64_t seed, int device);
+int shx_mlp_destroy(shx_mlp* m);
+const char* shx_mlp_last_error(const shx_mlp* m);
+int shx_mlp_num_params(const shx_mlp* m);
+int shx_mlp_get_params(shx_mlp* m, float* out);
+int shx_mlp_set_params(shx_mlp* m, const float* in);   /* also resets the optimiser state */
+int shx_mlp_forward(shx_mlp* m, int batch, const float* x, float* y);
+/* mean squared error of the outputs for x against t and its gradient with
+ * respect to the parameters (grad may be NULL) */
+int shx_mlp_gradient(shx_mlp* m, int batch, const float* x, const float* t, float* grad, double* loss);
+/* epochs of minibatch Adam over n samples (shuffled per epoch from seed);
+ * loss: mean minibatch loss of the last epoch */
+int shx_mlp_train(shx_mlp* m, int n, const float* x, const float* t, int batch, int epochs, float lr,
+                  uint64_t seed, double* loss);
+/* C (M x N) = op(A) op(B) on the matrix cores (BF16 operands, FP32
+ * accumulation), host arrays; op = transpose when ta / tb != 0 (A is M x K
+ * or, transposed, K x M; B is K x N or N x K) */
+int shx_mlp_gemm(int M, int N, int K, const float* A, int ta, const float* B, int tb, float* C);
+
 /* Primitives exposed for parity tests (host-compiled copies of the device
  * routines; no GPU needed). */
 void shx_tri_distance(const double* p /* 18 */, double* out /* d2, dc[3], v[2], w[2] */);
