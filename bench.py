@@ -35,6 +35,7 @@ import shell_amd as S  # noqa: E402
 from shell_amd.workloads import batch_sweep, icosphere, radial_imperfection  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+BASELINE_METRIC = "post-buckled equilibria/s at rel residual 1e-6, surrogate vs plain; force GB/s"
 METRIC = "post-buckled equilibria/s at rel residual 1e-6 (plain FP64 GPU method); force GB/s"
 
 
@@ -72,8 +73,8 @@ def with_cfl(batch, params, iters):
 def roofline(b, meshes, steps=200):
     """element-force kernel: algorithmic bytes / average launch time, HIP
     events on the library's stream, over `steps` steps from the initial state"""
-    for i, (X, _) in enumerate(meshes):
-        b.set_state(i, np.zeros(X.size))
+    z = np.zeros(3 * int(b.nv.sum()))
+    b.set_state(S.ALL_SHELLS, z, v=z, a=z, t=0.0)
     b.apply_initial_conditions()
     b.set_profiling(True)
     b.advance(steps)
@@ -133,8 +134,8 @@ def run_batch(args):
     log(f"setup {time.perf_counter() - t0:.1f} s: {len(shells)} shells, {sum(len(T) for _, T in meshes)} elements")
 
     def solve():
-        for i, (X, _) in enumerate(meshes):
-            b.set_state(i, np.zeros(X.size))
+        z = np.zeros(3 * int(b.nv.sum()))
+        b.set_state(S.ALL_SHELLS, z, v=z, a=z, t=0.0)
         b.apply_initial_conditions()
         t1 = time.perf_counter()
         done = 0
@@ -255,14 +256,111 @@ def run_single(args):
     print(json.dumps(out), flush=True)
 
 
+def run_surrogate(args):
+    """learned warm start vs plain start on a test batch of level-4 shells"""
+    from shell_amd import warmstart as W
+    from shell_amd.surrogate import Surrogate
+    d = np.load(args.warm_data)
+    conv = d["converged"]
+    k = args.components
+    P_train, C_train = d["params"][conv], d["coeffs"][:, :k].astype(np.float64)   # coeffs: converged rows
+    t0 = time.perf_counter()
+    sg = Surrogate(d["umean"], d["basis"][:k].astype(np.float64), hidden=(256, 256, 256), seed=1)
+    losses = []
+    for ep, lr in ((args.epochs // 2, 1e-3), (args.epochs // 4, 3e-4), (args.epochs - args.epochs // 2 - args.epochs // 4, 1e-4)):
+        losses.append(sg.fit(P_train, C_train, epochs=ep, batch=256, lr=lr))
+    train_s = time.perf_counter() - t0
+    log(f"surrogate trained on {len(P_train)} equilibria in {train_s:.1f} s, loss {losses}")
+    P = W.draw_params(args.shells, args.seed)
+    t0 = time.perf_counter()
+    b, meshes = W.build(P, args.cfl_iters)
+    if args.no_graph:
+        b.set_graphs(False)
+    vref = W.reference_volumes(b)
+    log(f"setup {time.perf_counter() - t0:.1f} s: {b.S} level-4 shells")
+    n_dof = 3 * int(b.nv[0])
+    results = {}
+    for _ in range(args.warmup):
+        W.warm_solve(b, sg, P, vref, args.tol, args.max_steps, args.chunk,
+                     precision=S.FP32_SPLIT if args.precision == "fp32" else S.FP64)
+    prec = S.FP32_SPLIT if args.precision == "fp32" else S.FP64
+    walls_w, conv_w = [], 0
+    for _ in range(args.steps):
+        reps_w, wall_w, info = W.warm_solve(b, sg, P, vref, args.tol, args.max_steps, args.chunk, precision=prec)
+        walls_w.append(wall_w)
+        conv_w += sum(r["converged_step"] >= 0 for r in reps_w)
+    Uw = b.get_state(S.ALL_SHELLS)[0].reshape(b.S, n_dof)
+    reps_p, wall_p = W.plain_solve(b, args.tol, args.max_steps, args.chunk)
+    conv_p = sum(r["converged_step"] >= 0 for r in reps_p)
+    Up = b.get_state(S.ALL_SHELLS)[0].reshape(b.S, n_dof)
+    Xs = [X for X, _ in meshes]
+    Up_al, Uw_al = W.align(Xs, Up), W.align(Xs, Uw)
+    U0 = sg.predict(P)
+    ok = np.array([r["converged_step"] >= 0 for r in reps_p])
+    scale = np.linalg.norm(Up_al, axis=1)
+    pred_err = np.linalg.norm(U0 - Up_al, axis=1)[ok] / scale[ok]
+    same = np.linalg.norm(Uw_al - Up_al, axis=1)[ok] / scale[ok]
+    steps_w = [r["n_steps"] for r in reps_w]
+    steps_p = [r["converged_step"] if r["converged_step"] >= 0 else r["n_steps"] for r in reps_p]
+    wall_w = float(np.mean(walls_w))
+    value = conv_w / sum(walls_w)
+    plain = conv_p / wall_p
+    rl = roofline(b, meshes)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline([4], [(4, s) for s in steps_p], {4: b_params_first(meshes, P)}, args.cpu_seconds)
+    out = {
+        "metric": BASELINE_METRIC, "value": value, "unit": "equilibria/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall_w * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == S.FP64 else "f32 element forces, f64 accumulation/finish",
+        "data": "synthetic: seeded level-4 shells (fixed imperfection pattern, seeded parameters); surrogate "
+                "trained in this run on equilibria this solver generated (shell_amd/data)",
+        "config": {"workload": f"learned warm start on {b.S} level-4 shells: E2/E1~U[0.3,1], nu12~U[0.25,0.35], "
+                               f"h/R~logU[0.008,0.016], V/V0~U[0.86,0.92], mu~U[0,0.5], imperfection/h~"
+                               f"U[0.005,0.02]; MLP 6-256-256-256-{k} (GELU, BF16 MFMA GEMMs, FP32 accumulation) "
+                               f"-> {k} PCA coefficients of the displacement; damped relaxation (mass damping 0.05, "
+                               f"dt=0.5*CFL) to deformation residual {args.tol:g}; plain-start fallback on "
+                               f"interpenetration or residual growth; one step = the whole batch solved",
+                   "shells": b.S, "elements": int(sum(len(T) for _, T in meshes)),
+                   "surrogate_precision": args.precision, "plain_precision": "fp64",
+                   "surrogate_equilibria_per_s": value, "plain_equilibria_per_s": plain,
+                   "surrogate_vs_plain": value / plain if plain > 0 else None,
+                   "surrogate_wall_s": wall_w, "plain_wall_s": wall_p,
+                   "converged_surrogate": int(conv_w / max(args.steps, 1)), "converged_plain": int(conv_p),
+                   "fallback_interpenetrating": info["fallback_interpenetrating"],
+                   "fallback_residual_growth": info["fallback_residual_growth"],
+                   "solver_steps_median_surrogate_plain": [int(np.median(steps_w)), int(np.median(steps_p))],
+                   "prediction_rel_err_median": float(np.median(pred_err)),
+                   "equilibrium_rel_diff_surrogate_vs_plain_median": float(np.median(same)),
+                   "initial_residual_median": info["r0_median"],
+                   "training": {"equilibria": int(len(P_train)), "seconds": train_s, "epochs": args.epochs,
+                                "final_loss": losses[-1], "pca_variance_kept": float(d["explained"])},
+                   "max_steps": args.max_steps},
+        "roofline": rl, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def b_params_first(meshes, P):
+    """parameters of the first test shell for the CPU leg (same mesh level and physics family)"""
+    from shell_amd import warmstart as W
+    _, prms = W.shells(P[:1])
+    return prms[0]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed solves")
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--mode", choices=["batch", "single"], default="batch")
-    ap.add_argument("--shells", type=int, default=512)
-    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--mode", choices=["surrogate", "batch", "single"], default="surrogate")
+    ap.add_argument("--warm-data", default=os.path.join(ROOT, "shell_amd", "data", "warm_L4_4096.npz"))
+    ap.add_argument("--components", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="element forces of the warm-started relaxation (the plain leg is FP64)")
+    ap.add_argument("--shells", type=int, default=None, help="default 1024 (surrogate), 512 (batch)")
+    ap.add_argument("--seed", type=int, default=None, help="default 12 (surrogate test set), 7 (batch)")
     ap.add_argument("--max-steps", type=int, default=60000)
     ap.add_argument("--chunk", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -275,7 +373,13 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    if args.mode == "batch":
+    if args.mode == "surrogate":
+        args.shells = args.shells or 1024
+        args.seed = 12 if args.seed is None else args.seed
+        run_surrogate(args)
+    elif args.mode == "batch":
+        args.shells = args.shells or 512
+        args.seed = 7 if args.seed is None else args.seed
         run_batch(args)
     else:
         run_single(args)

@@ -46,7 +46,10 @@ enum {
 /* Element-force precision.  SHX_FP32_SPLIT: metric and curvature changes
  * formed in FP32 from the displacement gradients relative to the reference
  * surface, FP32 element forces and node gather; predictor, volume, contact,
- * Newmark update and all reductions FP64. */
+ * Newmark update and all reductions FP64.  In shx_step / shx_batch_step a
+ * shell leaves FP32 for FP64, for the rest of the solve (until the next
+ * init), once after the load ramp its residual of the deformation is within
+ * 1000 tol or has not halved within 1500 accepted steps (the FP32 floor). */
 enum { SHX_FP64 = 0, SHX_FP32_SPLIT = 1 };
 
 /* Parameters: the config.cfg keys of ShellSystem::set_parameters
@@ -94,6 +97,10 @@ typedef struct {
                                       residual_deform <= 1000 tol after the load ramp, then FP64 finishes) */
   double residual_deform;          /* ||f - M a_rigid|| / ||f_ext||: residual with the least-squares rigid-body
                                       inertial force removed (a free-floating shell drifts under the follower load) */
+  int32_t capacity_err;            /* nonzero: a device list of this shell overflowed (bits: 1 bucket entries,
+                                      2 pairs, 4 contact row, 8 cells); the shell stopped and the call returned
+                                      SHX_E_CAPACITY; shx_batch_init_shells clears it */
+  int32_t reserved;
 } shx_report;
 
 /* Replaces: ShellSystem constructor + set_parameters + build_cache +
@@ -170,10 +177,34 @@ int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 9 */, int64_t* counts /* 9 
 int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, const double* X,
                      const int32_t* tri, const shx_params* prm, int device);
 int shx_batch_size(const shx_ctx* ctx);
+/* shx_set_state / shx_get_state of one shell; shell == SHX_ALL_SHELLS moves
+ * every shell's state at once, the arrays holding the shells one after
+ * another (3 * sum nv doubles each; t is set for all, or read from shell 0) */
+#define SHX_ALL_SHELLS (-1)
 int shx_batch_set_state(shx_ctx* ctx, int shell, const double* u, const double* v, const double* a, double t);
 int shx_batch_get_state(shx_ctx* ctx, int shell, double* u, double* v, double* a, double* t);
 /* apply_initial_conditions (ShellSystem.C:855-905) for every shell */
 int shx_batch_init_dynamics(shx_ctx* ctx, int precision);
+/* apply_initial_conditions (ShellSystem.C:855-905) for the n listed shells
+ * only (shells == NULL: n == nshell, all in order), from their current state;
+ * the other shells keep their state and progress.  Resets the listed shells'
+ * step counters, problem flag and contact (stick) state.  volume0 (n values,
+ * or NULL): a value > 0 fixes the shell's reference enclosed volume, the V0
+ * of the volume constraint (ShellSystem.C:1476), for a start that is not
+ * undeformed (a warm start); 0 takes the state's own volume as V0, as the
+ * reference does (:884-885).  No counterpart in the reference: it serves the
+ * learned warm start and its fallback to a plain start. */
+int shx_batch_init_shells(shx_ctx* ctx, int precision, int n, const int* shells, const double* volume0);
+/* Extra mass-proportional damping alpha[s] >= 0 of every shell (nshell
+ * values; NULL clears it), added to the configured damping
+ * (ShellSystem.C:762-852) in every following step until changed:
+ * c_i = c_i(config) + alpha m_i.  No counterpart in the reference: the
+ * learned warm start uses it for a short quench of the predicted state's
+ * high-frequency error (alpha of the order of the highest frequency; the
+ * central-difference step stays stable for alpha dt <= 1.5 at dt = 0.5 x the
+ * CFL bound).  Equilibria do not depend on it: the residual is formed from
+ * the undamped force. */
+int shx_batch_set_extra_damping(shx_ctx* ctx, const double* alpha);
 /* n_steps accepted steps of every shell (ShellSystem::advance, ShellSystem.C:
  * 952-1122), or until the shell converges if stop_on_converged; reps[nshell] */
 int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged,

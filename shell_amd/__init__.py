@@ -30,10 +30,13 @@ EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_
            "shx_elem_tables", "shx_mass", "shx_patches", "shx_cfl_omega_max", "shx_set_profiling",
            "shx_kernel_times", "shx_tri_distance", "shx_raw_elem_basis", "shx_batch_create",
            "shx_batch_size", "shx_batch_set_state", "shx_batch_get_state", "shx_batch_init_dynamics",
+           "shx_batch_init_shells", "shx_batch_set_extra_damping",
            "shx_batch_step", "shx_batch_reports", "shx_batch_pairs", "shx_batch_cfl_omega_max",
            "shx_set_graphs"]
 
 _lib = None
+
+ALL_SHELLS = -1
 
 KERNEL_CLASSES = ["elem_force", "node_newmark", "contact_narrow", "shell_epilogues", "bp_box", "bp_count",
                   "bp_scan", "bp_scatter", "bp_pairs"]
@@ -56,7 +59,8 @@ class Report(ctypes.Structure):
         "t", "dt", "residual", "fnorm", "fext_norm", "e_tens", "e_bend", "e_self", "e_fric", "e_kin",
         "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
         "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_steps", "n_rejected", "converged_step",
-        "n_steps_fp32")] + [("residual_deform", ctypes.c_double)]
+        "n_steps_fp32")] + [("residual_deform", ctypes.c_double)] + [
+        (n, ctypes.c_int32) for n in ("capacity_err", "reserved")]
 
 
 DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_stiff_mult=1.0,
@@ -109,6 +113,8 @@ def lib():
         L.shx_batch_set_state.argtypes = [_V, ctypes.c_int, _D, _D, _D, ctypes.c_double]
         L.shx_batch_get_state.argtypes = [_V, ctypes.c_int, _D, _D, _D, _D]
         L.shx_batch_init_dynamics.argtypes = [_V, ctypes.c_int]
+        L.shx_batch_init_shells.argtypes = [_V, ctypes.c_int, ctypes.c_int, _I, _D]
+        L.shx_batch_set_extra_damping.argtypes = [_V, _D]
         L.shx_batch_step.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(Report)]
         L.shx_batch_reports.argtypes = [_V, ctypes.POINTER(Report)]
@@ -143,7 +149,14 @@ def make_params(**kw):
 
 
 class ShellError(RuntimeError):
-    pass
+    """a failed call; code is the SHX_E_* status"""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+E_CAPACITY = -4
 
 
 class ShellSystem:
@@ -163,7 +176,7 @@ class ShellSystem:
 
     def _check(self, r, what):
         if r < 0:
-            raise ShellError(f"{what}: {r}: {lib().shx_last_error(self.h).decode()}")
+            raise ShellError(f"{what}: {r}: {lib().shx_last_error(self.h).decode()}", r)
         return r
 
     def close(self):
@@ -322,7 +335,7 @@ class ShellBatch:
 
     def _check(self, r, what):
         if r < 0:
-            raise ShellError(f"{what}: {r}: {lib().shx_last_error(self.h).decode()}")
+            raise ShellError(f"{what}: {r}: {lib().shx_last_error(self.h).decode()}", r)
         return r
 
     def close(self):
@@ -336,15 +349,20 @@ class ShellBatch:
         except Exception:
             pass
 
+    def _ndof(self, shell):
+        return 3 * int(self.nv.sum() if shell == ALL_SHELLS else self.nv[shell])
+
     def set_state(self, shell, u, v=None, a=None, t=0.0):
-        n = 3 * int(self.nv[shell])
+        """one shell's state, or every shell's (shell=ALL_SHELLS, arrays of the
+        shells one after another)"""
+        n = self._ndof(shell)
         u = _vec(u, n)
         v = None if v is None else _vec(v, n)
         a = None if a is None else _vec(a, n)
         self._check(lib().shx_batch_set_state(self.h, shell, _dp(u), _dp(v), _dp(a), t), "shx_batch_set_state")
 
     def get_state(self, shell):
-        n = 3 * int(self.nv[shell])
+        n = self._ndof(shell)
         u, v, a = (np.zeros(n) for _ in range(3))
         t = ctypes.c_double()
         self._check(lib().shx_batch_get_state(self.h, shell, _dp(u), _dp(v), _dp(a), ctypes.byref(t)),
@@ -353,6 +371,22 @@ class ShellBatch:
 
     def apply_initial_conditions(self, precision=FP64):
         self._check(lib().shx_batch_init_dynamics(self.h, precision), "shx_batch_init_dynamics")
+
+    def init_shells(self, shells, volume0=None, precision=FP64):
+        """apply_initial_conditions for the listed shells only; volume0[k] > 0
+        fixes shell k's reference volume V0 (a warm start), 0 takes the
+        state's own volume"""
+        idx = np.ascontiguousarray(shells, dtype=np.int32)
+        v0 = None if volume0 is None else np.ascontiguousarray(volume0, dtype=np.float64)
+        if v0 is not None and v0.shape != idx.shape:
+            raise ValueError("one volume0 per listed shell")
+        self._check(lib().shx_batch_init_shells(self.h, precision, len(idx), idx.ctypes.data_as(_I), _dp(v0)),
+                    "shx_batch_init_shells")
+
+    def set_extra_damping(self, alpha=None):
+        """extra mass-proportional damping per shell (S values; None clears)"""
+        a = None if alpha is None else _vec(alpha, self.S)
+        self._check(lib().shx_batch_set_extra_damping(self.h, _dp(a)), "shx_batch_set_extra_damping")
 
     def advance(self, n_steps=1, tol=0.0, stop_on_converged=False, precision=FP64):
         reps = (Report * self.S)()

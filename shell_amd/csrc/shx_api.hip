@@ -43,6 +43,10 @@ struct shx_ctx {
   bool any_contact = false;
   std::vector<void*> allocs;
   std::vector<double> rawb;
+  /* per shell, for the FP32 -> FP64 switch: the lowest residual of the
+   * deformation after the load ramp (halvings only) and its step */
+  std::vector<double> fp32_best;
+  std::vector<int> fp32_best_step;
   std::string err;
   bool prof = false;
   std::vector<hipEvent_t> evpool;
@@ -294,9 +298,12 @@ int check_precision(shx_ctx* ctx, int precision)
 }
 
 /* FP32 split steps of a shell go on until its residual of the deformation
- * is within this factor of the tolerance after the load ramp; FP64 finishes
- * (FP32 forces have a residual floor of order 1e-4 on stiff shells) */
+ * is within kFp32Switch of the tolerance after the load ramp, or until the
+ * residual has stopped falling (not halved within kFp32Stall accepted steps
+ * after the ramp: the FP32 floor, which depends on the shell's stiffness to
+ * load ratio, lies above kFp32Switch tol); FP64 finishes */
 constexpr double kFp32Switch = 1000.0;
+constexpr int kFp32Stall = 1500;
 
 int check_err(shx_ctx* ctx, const std::vector<DShell>& s)
 {
@@ -320,6 +327,8 @@ void fill_report(const DShell& s, shx_report* rep)
   rep->problem = s.problem; rep->n_steps = s.n_steps; rep->n_rejected = s.n_rejected;
   rep->converged_step = s.converged_step; rep->n_steps_fp32 = s.n_fp32;
   rep->residual_deform = s.residual_deform;
+  rep->capacity_err = s.err;
+  rep->reserved = 0;
 }
 
 /* SConst of one shell from its parameters (update_derived,
@@ -381,6 +390,38 @@ int check_shell(shx_ctx* ctx, int shell)
 
 } // namespace
 
+namespace {
+/* host <-> device state of one shell, or of all shells one after another
+ * (shell == SHX_ALL_SHELLS) */
+int xfer_state(shx_ctx* ctx, int shell, double* u, double* v, double* a, double* t, bool to_dev)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (shell != SHX_ALL_SHELLS)
+    if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  const int i0 = shell == SHX_ALL_SHELLS ? 0 : shell, i1 = shell == SHX_ALL_SHELLS ? ctx->S : shell + 1;
+  size_t host = 0;
+  for (int i = i0; i < i1; ++i) {
+    const SConst& C = ctx->C[i];
+    const size_t off = (size_t)3 * C.vbase, n = (size_t)3 * C.nv, bytes = sizeof(double) * n;
+    const int par = s[i].par;
+    double* dev[3] = {ctx->P.U[par] + off, ctx->P.V[par] + off, ctx->P.A[par] + off};
+    double* hst[3] = {u, v, a};
+    for (int q = 0; q < 3; ++q) {
+      if (!hst[q]) continue;
+      if (to_dev) HIPCHK(hipMemcpy(dev[q], hst[q] + host, bytes, hipMemcpyHostToDevice));
+      else HIPCHK(hipMemcpy(hst[q] + host, dev[q], bytes, hipMemcpyDeviceToHost));
+    }
+    host += n;
+    if (to_dev) s[i].t = *t;
+  }
+  if (to_dev) return write_sh(ctx, s);
+  if (t) *t = s[i0].t;
+  return SHX_OK;
+}
+} // namespace
+
 extern "C" {
 
 const char* shx_last_error(const shx_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
@@ -413,6 +454,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   shx_ctx* ctx = new shx_ctx();
   ctx->dev = device;
   ctx->S = nshell;
+  ctx->fp32_best.assign(nshell, HUGE_VAL);
+  ctx->fp32_best_step.assign(nshell, 0);
   ctx->prm.assign(prm, prm + nshell);
   auto fail = [&](int code) { g_create_err = ctx->err; shx_destroy(ctx); return code; };
   if (hipSetDevice(device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return fail(SHX_E_HIP); }
@@ -606,34 +649,12 @@ int shx_batch_size(const shx_ctx* ctx) { return ctx ? ctx->S : SHX_E_ARG; }
 
 int shx_batch_set_state(shx_ctx* ctx, int shell, const double* u, const double* v, const double* a, double t)
 {
-  if (!ctx) return SHX_E_ARG;
-  if (int r = check_shell(ctx, shell)) return r;
-  std::vector<DShell> s;
-  if (int r = read_sh(ctx, s)) return r;
-  const SConst& C = ctx->C[shell];
-  const size_t off = (size_t)3 * C.vbase, bytes = sizeof(double) * 3 * C.nv;
-  const int par = s[shell].par;
-  if (u) HIPCHK(hipMemcpy(ctx->P.U[par] + off, u, bytes, hipMemcpyHostToDevice));
-  if (v) HIPCHK(hipMemcpy(ctx->P.V[par] + off, v, bytes, hipMemcpyHostToDevice));
-  if (a) HIPCHK(hipMemcpy(ctx->P.A[par] + off, a, bytes, hipMemcpyHostToDevice));
-  s[shell].t = t;
-  return write_sh(ctx, s);
+  return xfer_state(ctx, shell, const_cast<double*>(u), const_cast<double*>(v), const_cast<double*>(a), &t, true);
 }
 
 int shx_batch_get_state(shx_ctx* ctx, int shell, double* u, double* v, double* a, double* t)
 {
-  if (!ctx) return SHX_E_ARG;
-  if (int r = check_shell(ctx, shell)) return r;
-  std::vector<DShell> s;
-  if (int r = read_sh(ctx, s)) return r;
-  const SConst& C = ctx->C[shell];
-  const size_t off = (size_t)3 * C.vbase, bytes = sizeof(double) * 3 * C.nv;
-  const int par = s[shell].par;
-  if (u) HIPCHK(hipMemcpy(u, ctx->P.U[par] + off, bytes, hipMemcpyDeviceToHost));
-  if (v) HIPCHK(hipMemcpy(v, ctx->P.V[par] + off, bytes, hipMemcpyDeviceToHost));
-  if (a) HIPCHK(hipMemcpy(a, ctx->P.A[par] + off, bytes, hipMemcpyDeviceToHost));
-  if (t) *t = s[shell].t;
-  return SHX_OK;
+  return xfer_state(ctx, shell, u, v, a, t, false);
 }
 
 int shx_set_state(shx_ctx* ctx, const double* u, const double* v, const double* a, double t)
@@ -699,18 +720,39 @@ int shx_contact_rebuild(shx_ctx* ctx, int force)
   return shx_batch_pairs(ctx, 0, nullptr, nullptr, 0);
 }
 
-int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
+int shx_batch_init_shells(shx_ctx* ctx, int precision, int n, const int* shells, const double* volume0)
 {
-  if (!ctx) return SHX_E_ARG;
+  if (!ctx || n < 0 || n > ctx->S || (!shells && n != ctx->S)) return SHX_E_ARG;
   if (int r = check_precision(ctx, precision)) return r;
+  std::vector<char> pick(ctx->S, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = shells ? shells[k] : k;
+    if (int r = check_shell(ctx, i)) return r;
+    if (pick[i]) { ctx->err = "shell listed twice"; return SHX_E_ARG; }
+    if (volume0 && !(volume0[k] >= 0)) { ctx->err = "volume0 must be >= 0"; return SHX_E_ARG; }
+    pick[i] = 1;
+  }
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  for (int i = 0; i < ctx->S; ++i) {
+  for (int k = 0; k < n; ++k) {
+    const int i = shells ? shells[k] : k;
     DShell& d = s[i];
     d.done = 0; d.n_steps = 0; d.n_rejected = 0; d.converged_step = -1; d.fresh = 1; d.attempts = 0;
+    d.problem = 0; d.err = 0; d.n_rebuilds = 0; d.du_sum = 0;
     d.eprec = precision; d.n_fp32 = 0;
+    ctx->fp32_best[i] = HUGE_VAL;
+    ctx->fp32_best_step[i] = 0;
+    d.volume_ref = volume0 ? volume0[k] : 0.0;
     d.rebuild = ctx->C[i].contact ? 1 : 0;
+    /* a fresh start carries no contact (stick) state over from an earlier run */
+    if (ctx->C[i].contact)
+      for (int b = 0; b < 2; ++b)
+        HIPCHK(hipMemsetAsync(ctx->P.rcnt[b] + ctx->C[i].ebase, 0, sizeof(int32_t) * ctx->C[i].ne, ctx->st));
   }
+  /* the shells not listed keep their state and progress: they sit out the
+   * init kernels as done shells; shx_batch_step recomputes done for all */
+  for (int i = 0; i < ctx->S; ++i)
+    if (!pick[i]) s[i].done = 1;
   if (int r = write_sh(ctx, s)) return r;
   if (int r = set_active(ctx, s)) return r;
   launch_elem(ctx, M_INIT);
@@ -723,6 +765,24 @@ int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
   if (int r = read_sh(ctx, s)) return r;
   prof_collect(ctx);
   return check_err(ctx, s);
+}
+
+int shx_batch_set_extra_damping(shx_ctx* ctx, const double* alpha)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (alpha)
+    for (int i = 0; i < ctx->S; ++i)
+      if (!(alpha[i] >= 0)) { ctx->err = "extra damping must be >= 0"; return SHX_E_ARG; }
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  for (int i = 0; i < ctx->S; ++i) s[i].alpha_extra = alpha ? alpha[i] : 0.0;
+  return write_sh(ctx, s);
+}
+
+int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
+{
+  if (!ctx) return SHX_E_ARG;
+  return shx_batch_init_shells(ctx, precision, ctx->S, nullptr, nullptr);
 }
 
 int shx_init_dynamics(shx_ctx* ctx, int precision) { return shx_batch_init_dynamics(ctx, precision); }
@@ -743,14 +803,20 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
       }
     if (cchange)
       if (int r = write_consts(ctx, ctx->C)) return r;
-    auto fp32_done = [&](const DShell& d, const SConst& C) {
-      return d.t >= C.load_end_time && d.n_steps > 0 && d.residual_deform <= kFp32Switch * tol;
+    std::vector<double>& best = ctx->fp32_best;
+    std::vector<int>& best_step = ctx->fp32_best_step;
+    auto fp32_done = [&](int i, const DShell& d, const SConst& C) {
+      if (d.t < C.load_end_time || d.n_steps == 0) return false;
+      if (d.residual_deform <= kFp32Switch * tol) return true;
+      if (d.residual_deform < 0.5 * best[i]) { best[i] = d.residual_deform; best_step[i] = d.n_steps; }
+      return d.n_steps - best_step[i] > kFp32Stall;
     };
     for (int i = 0; i < ctx->S; ++i) {
       DShell& d = s[i];
       d.steps_target = d.n_steps + n_steps;
       d.done = (d.problem >= 7 || d.err || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
-      d.eprec = (precision == SHX_FP32_SPLIT && !fp32_done(d, ctx->C[i])) ? SHX_FP32_SPLIT : SHX_FP64;
+      d.eprec = (precision == SHX_FP32_SPLIT && d.eprec == SHX_FP32_SPLIT && !fp32_done(i, d, ctx->C[i]))
+                    ? SHX_FP32_SPLIT : SHX_FP64;
     }
     if (int r = write_sh(ctx, s)) return r;
     const long long max_attempts = 64LL * n_steps + 4096;
@@ -773,7 +839,7 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
       if (precision == SHX_FP32_SPLIT) {
         bool sw = false;
         for (int i = 0; i < ctx->S; ++i)
-          if (s[i].eprec == SHX_FP32_SPLIT && fp32_done(s[i], ctx->C[i])) { s[i].eprec = SHX_FP64; sw = true; }
+          if (s[i].eprec == SHX_FP32_SPLIT && fp32_done(i, s[i], ctx->C[i])) { s[i].eprec = SHX_FP64; sw = true; }
         if (sw)
           if (int r = write_sh(ctx, s)) return r;
       }

@@ -107,6 +107,8 @@ struct DShell {
   double t_eval;                   /* force time of a plain force evaluation (M_FORCES) */
   double du_sum, avg_elem_size;
   double volume, volume0, area;
+  double volume_ref;               /* > 0: V0 of the volume constraint given by the host at init (warm start) */
+  double alpha_extra;              /* extra mass-proportional damping set by the host (warm-start quench) */
   double bmin[3], bmax[3];
   double cell_size, d_max, d2_max;
   double residual, fnorm, fext_norm, e_tens, e_bend, e_self, e_fric, e_kin, du_max;

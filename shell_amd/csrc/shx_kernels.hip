@@ -992,8 +992,12 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   w.fext_norm = sqrt(fe2);
   w.fdeform = rigid_free_norm(f2, rn + 6);
   if (mode == M_INIT) {
-    w.volume = w.volume0 = w.att_volume;
+    w.volume = w.att_volume;
+    w.volume0 = w.volume_ref > 0 ? w.volume_ref : w.att_volume;
     w.area = w.att_area;
+    /* residuals of the initial state (with its own V0 when volume_ref is 0) */
+    w.residual = w.fext_norm > 0 ? w.fnorm / w.fext_norm : w.fnorm;
+    w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
     return;
   }
   if (mode == M_FORCES) return;
@@ -1113,7 +1117,7 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
     if (C.bulk_modulus != 0) {
       const double Vc = (C.volume_at_prediction || (mode == M_STEP && sh.fresh) || mode == M_INIT) ? sh.att_volume
                                                                                                     : sh.volume;
-      const double V0 = (mode == M_INIT) ? Vc : sh.volume0;
+      const double V0 = (mode == M_INIT) ? (sh.volume_ref > 0 ? sh.volume_ref : Vc) : sh.volume0;
       ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / Vc - 1);
     }
     const v3 fcn = ld3(P.fc, n);
@@ -1141,7 +1145,7 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
     } else if (mode == M_INIT) {
       st3(P.A[par], n, scl(f, mi));
     } else {
-      const double cd = P.cdamp[n];
+      const double cd = P.cdamp[n] + sh.alpha_extra / mi;   /* + alpha_extra m */
       const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
       const double a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];
       const v3 up = pred3(u, v, a, dt, a8);

@@ -126,32 +126,44 @@ class MLP:
         return loss.value
 
 
+def pca(U, k):
+    """principal components of the rows of U (samples x DOF): mean, the
+    first k right singular vectors (k x DOF, orthonormal rows), the
+    coefficients of every sample and the fraction of the variance kept"""
+    U = np.asarray(U, np.float64)
+    umean = U.mean(0)
+    _, s, vt = np.linalg.svd(U - umean, full_matrices=False)
+    k = min(k, vt.shape[0])
+    basis = vt[:k]
+    return umean, basis, (U - umean) @ basis.T, float(np.sum(s[:k] ** 2) / np.sum(s ** 2))
+
+
 class Surrogate:
     """parameters -> principal-component coefficients of the equilibrium
-    displacement -> displacement field"""
+    displacement -> displacement field.  Inputs and coefficients are
+    standardised (zero mean, unit variance over the training set) around the
+    network."""
 
-    def __init__(self, n_components=64, hidden=(256, 256, 256), seed=1, device=0):
-        self.k = n_components
+    def __init__(self, umean, basis, hidden=(256, 256, 256), seed=1, device=0):
+        self.umean = np.asarray(umean, np.float64)
+        self.basis = np.asarray(basis, np.float64)
+        self.k = self.basis.shape[0]
         self.hidden = tuple(hidden)
         self.seed = seed
         self.device = device
         self.net = None
 
-    def fit(self, P, U, epochs=400, batch=128, lr=1e-3):
-        """P: samples x parameters; U: samples x DOF equilibrium displacements"""
+    def fit(self, P, C, epochs=400, batch=128, lr=1e-3):
+        """P: samples x parameters; C: samples x k coefficients.  Returns the
+        final training loss (standardised units)"""
         P = np.asarray(P, np.float64)
-        U = np.asarray(U, np.float64)
+        C = np.asarray(C, np.float64)
         self.pmean, self.pstd = P.mean(0), P.std(0) + 1e-12
-        self.umean = U.mean(0)
-        _, s, vt = np.linalg.svd(U - self.umean, full_matrices=False)
-        k = min(self.k, vt.shape[0])
-        self.basis = vt[:k]                                     # k x DOF, orthonormal rows
-        self.explained = float(np.sum(s[:k] ** 2) / np.sum(s ** 2))
-        C = (U - self.umean) @ self.basis.T
         self.cmean, self.cstd = C.mean(0), C.std(0) + 1e-12
         X = (P - self.pmean) / self.pstd
         T = (C - self.cmean) / self.cstd
-        self.net = MLP([P.shape[1], *self.hidden, k], seed=self.seed, device=self.device)
+        if self.net is None:
+            self.net = MLP([P.shape[1], *self.hidden, self.k], seed=self.seed, device=self.device)
         return self.net.train(X, T, batch=min(batch, len(X)), epochs=epochs, lr=lr)
 
     def coefficients(self, P):
