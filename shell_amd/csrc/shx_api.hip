@@ -521,6 +521,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   std::vector<int8_t> vcls(nE, 6);
   std::vector<double> Xh((size_t)3 * nV, 0.0);
   std::vector<int32_t> inc_start((size_t)nV + 1, 0), inc;
+  std::vector<double> incphi;
   std::vector<int32_t> cstart((size_t)nE + 1, 0), cids;
   for (int s = 0; s < nshell; ++s) {
     const SConst& C = ctx->C[s];
@@ -551,8 +552,9 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       const int n = C.vbase + q;
       inc_start[n] = (int)inc.size();
       for (int k = T.inc_start[q]; k < T.inc_start[q + 1]; ++k) {
-        const int ent = T.inc[k];
-        inc.push_back(((C.ebase + (ent >> 4)) << 4) | (ent & 15));
+        const int ent = T.inc[k], eg = C.ebase + (ent >> 4);
+        inc.push_back((eg << 4) | (ent & 15));
+        incphi.push_back(ctx->rawb[(size_t)72 * (vcls[eg] - 3) + (ent & 15)]);
       }
     }
     for (int q = nv[s]; q < C.nbN * kNodePerBlock; ++q) inc_start[C.vbase + q] = (int)inc.size();
@@ -575,15 +577,15 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   DPtrs& P = ctx->P;
   P.nE = nE; P.nV = nV; P.S = nshell;
   int r = 0;
-  double *dX, *drb;
+  double *dX, *drb, *dip;
   int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci;
   int8_t* dv;
   if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
       (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, inc_start)) ||
       (r = upload(ctx, di, inc)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell)) ||
-      (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)))
+      (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)) || (r = upload(ctx, dip, incphi)))
     return fail(r);
-  P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
+  P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di; P.incphi = dip;
   P.eshell = des; P.vshell = dvs; P.close_start = dcs; P.close_ids = dci;
   if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
   P.cs = ctx->dC;

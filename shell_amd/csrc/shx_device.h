@@ -56,12 +56,14 @@ enum {
 };
 
 constexpr int kAlign = 256;         /* shell ranges start at and are padded to multiples of this */
-constexpr int kElemLanes = 4;       /* lanes cooperating on one element (3 patch slots each) */
-constexpr int kElemBlock = 256;     /* threads per element-kernel block: 64 elements */
-constexpr int kElemPerBlock = kElemBlock / kElemLanes;
+constexpr int kElemLanes = 4;       /* lanes cooperating on one element's patch gather / slot writes (3 slots each) */
+constexpr int kElemBlock = 256;     /* threads per element-kernel block */
+constexpr int kElemPerBlock = kElemBlock;   /* elements per block: one per thread in the stress phase */
+constexpr int kGRec = 19;           /* per element in LDS: G1..G5 (15) and R (4) */
 constexpr int kNodeLanes = 8;       /* lanes cooperating on one node */
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block: 32 nodes */
 constexpr int kNodePerBlock = kNodeBlock / kNodeLanes;
+constexpr int kIncUnroll = 4;       /* incidence entries per node lane whose loads are issued together */
 constexpr int kNodeRec = 22;        /* node-kernel partial record length */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
 constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
@@ -144,6 +146,7 @@ struct DPtrs {
   const double* rawb;         /* [4 valences 3..6][6][12] */
   const int32_t* inc_start;   /* nV+1 */
   const int32_t* inc;         /* e*16+i, e global */
+  const double* incphi;       /* per incidence entry: phi_i at the element's quadrature point */
   double* minv;               /* nV */
   double* cdamp;              /* nV */
   double* U[2];
