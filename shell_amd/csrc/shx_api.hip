@@ -521,7 +521,6 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   std::vector<int8_t> vcls(nE, 6);
   std::vector<double> Xh((size_t)3 * nV, 0.0);
   std::vector<int32_t> inc_start((size_t)nV + 1, 0), inc;
-  std::vector<double> incphi;
   std::vector<int32_t> cstart((size_t)nE + 1, 0), cids;
   for (int s = 0; s < nshell; ++s) {
     const SConst& C = ctx->C[s];
@@ -552,14 +551,64 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       const int n = C.vbase + q;
       inc_start[n] = (int)inc.size();
       for (int k = T.inc_start[q]; k < T.inc_start[q + 1]; ++k) {
-        const int ent = T.inc[k], eg = C.ebase + (ent >> 4);
-        inc.push_back((eg << 4) | (ent & 15));
-        incphi.push_back(ctx->rawb[(size_t)72 * (vcls[eg] - 3) + (ent & 15)]);
+        const int ent = T.inc[k];
+        inc.push_back(((C.ebase + (ent >> 4)) << 4) | (ent & 15));
       }
     }
     for (int q = nv[s]; q < C.nbN * kNodePerBlock; ++q) inc_start[C.vbase + q] = (int)inc.size();
   }
   inc_start[nV] = (int)inc.size();
+  /* element blocks: their patch nodes (ascending global ids), each patch
+   * slot's index into them, per block node its (element, slot) entries, and
+   * per node the block records that hold its partial forces */
+  std::vector<int32_t> bn_start((size_t)ctx->nblkE + 1, 0), bn_ids, rent_start(1, 0);
+  std::vector<uint16_t> lidx((size_t)12 * nE, 0), rent;
+  for (int s = 0; s < nshell; ++s) {
+    const SConst& C = ctx->C[s];
+    const Topology& T = ctx->T[s];
+    for (int bl = 0; bl < C.nbE; ++bl) {
+      const int gb = C.blkE + bl, first = gb * kElemPerBlock;
+      std::vector<int32_t> ids;
+      for (int le = 0; le < kElemPerBlock; ++le) {
+        const int q = first + le - C.ebase;
+        if (q < ne[s])
+          for (int i = 0; i < 12; ++i) ids.push_back(pidx[(size_t)i * nE + first + le]);
+      }
+      std::sort(ids.begin(), ids.end());
+      ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+      if ((int)ids.size() > kMaxBlockNodes) {
+        ctx->err = "shell " + std::to_string(s) + ": element block " + std::to_string(bl) + " touches " +
+                   std::to_string(ids.size()) + " patch nodes (limit " + std::to_string(kMaxBlockNodes) +
+                   "): the element order is not spatially coherent";
+        return fail(SHX_E_MESH);
+      }
+      bn_start[gb] = (int)bn_ids.size();
+      std::vector<std::vector<uint16_t>> ent(ids.size());
+      for (int le = 0; le < kElemPerBlock; ++le) {
+        const int e = first + le, q = e - C.ebase;
+        if (q >= ne[s]) continue;
+        for (int i = 0; i < 12; ++i) {
+          const int j = (int)(std::lower_bound(ids.begin(), ids.end(), pidx[(size_t)i * nE + e]) - ids.begin());
+          lidx[(size_t)i * nE + e] = (uint16_t)j;
+          if (T.patch[(size_t)q * 12 + i] >= 0) ent[j].push_back((uint16_t)(le * 16 + i));
+        }
+      }
+      for (size_t j = 0; j < ids.size(); ++j) {
+        bn_ids.push_back(ids[j]);
+        rent.insert(rent.end(), ent[j].begin(), ent[j].end());
+        rent_start.push_back((int)rent.size());
+      }
+    }
+  }
+  const int nrecs = (int)bn_ids.size();
+  bn_start[ctx->nblkE] = nrecs;
+  std::vector<int32_t> nrec_start((size_t)nV + 1, 0), nrec(nrecs);
+  for (int r = 0; r < nrecs; ++r) nrec_start[bn_ids[r] + 1]++;
+  for (int n = 0; n < nV; ++n) nrec_start[n + 1] += nrec_start[n];
+  {
+    std::vector<int32_t> fill(nrec_start.begin(), nrec_start.end() - 1);
+    for (int r = 0; r < nrecs; ++r) nrec[fill[bn_ids[r]]++] = r;   /* ascending block order per node */
+  }
   for (int s = 0; s < nshell; ++s) {
     const SConst& C = ctx->C[s];
     std::vector<int32_t> cs, ci;
@@ -577,23 +626,27 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   DPtrs& P = ctx->P;
   P.nE = nE; P.nV = nV; P.S = nshell;
   int r = 0;
-  double *dX, *drb, *dip;
+  double *dX, *drb;
+  int32_t *dbs, *dbi, *drs, *dns, *dnr;
+  uint16_t *dli, *dre;
   int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci;
   int8_t* dv;
   if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
       (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, inc_start)) ||
       (r = upload(ctx, di, inc)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell)) ||
-      (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)) || (r = upload(ctx, dip, incphi)))
+      (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)) || (r = upload(ctx, dbs, bn_start)) ||
+      (r = upload(ctx, dbi, bn_ids)) || (r = upload(ctx, dli, lidx)) || (r = upload(ctx, drs, rent_start)) ||
+      (r = upload(ctx, dre, rent)) || (r = upload(ctx, dns, nrec_start)) || (r = upload(ctx, dnr, nrec)))
     return fail(r);
-  P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di; P.incphi = dip;
+  P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
+  P.bn_start = dbs; P.bn_ids = dbi; P.lidx = dli; P.rent_start = drs; P.rent = dre; P.nrec_start = dns; P.nrec = dnr;
   P.eshell = des; P.vshell = dvs; P.close_start = dcs; P.close_ids = dci;
   if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
   P.cs = ctx->dC;
   if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * nE)) || (r = dalloc(ctx, P.minv, nV)) ||
-      (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.qe, (size_t)36 * nE)) ||
-      (r = dalloc(ctx, P.g0, (size_t)3 * nE)) || (r = dalloc(ctx, P.area, nE)) ||
-      (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) || (r = dalloc(ctx, P.qe32, (size_t)36 * nE)) ||
-      (r = dalloc(ctx, P.g032, (size_t)3 * nE)) ||
+      (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.brec, (size_t)kBRec * std::max(nrecs, 1))) ||
+      (r = dalloc(ctx, P.area, nE)) || (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) ||
+      (r = dalloc(ctx, P.brec32, (size_t)kBRec * std::max(nrecs, 1))) ||
       (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||

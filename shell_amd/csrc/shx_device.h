@@ -56,14 +56,16 @@ enum {
 };
 
 constexpr int kAlign = 256;         /* shell ranges start at and are padded to multiples of this */
-constexpr int kElemLanes = 4;       /* lanes cooperating on one element's patch gather / slot writes (3 slots each) */
 constexpr int kElemBlock = 256;     /* threads per element-kernel block */
-constexpr int kElemPerBlock = kElemBlock;   /* elements per block: one per thread in the stress phase */
-constexpr int kGRec = 19;           /* per element in LDS: G1..G5 (15) and R (4) */
-constexpr int kNodeLanes = 8;       /* lanes cooperating on one node */
-constexpr int kNodeBlock = 256;     /* threads per node-kernel block: 32 nodes */
-constexpr int kNodePerBlock = kNodeBlock / kNodeLanes;
-constexpr int kIncUnroll = 4;       /* incidence entries per node lane whose loads are issued together */
+constexpr int kElemPerBlock = kElemBlock;   /* elements per block, one per thread */
+constexpr int kMaxBlockNodes = 512; /* patch nodes of one element block (icospheres: at most 267) */
+constexpr int kNodeStage = 9;       /* per staged node in LDS: u_pred, u, X */
+constexpr int kGRec = 22;           /* per element in LDS: G0..G5 (18) and R (4) */
+constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kElemPerBlock * kGRec) ? kMaxBlockNodes * kNodeStage
+                                                                               : kElemPerBlock * kGRec;
+constexpr int kNodeBlock = 256;     /* threads per node-kernel block, one node each */
+constexpr int kNodePerBlock = kNodeBlock;
+constexpr int kBRec = 6;            /* per block-node record: sum of slot contributions q, sum of phi JxW a3 */
 constexpr int kNodeRec = 22;        /* node-kernel partial record length */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
 constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
@@ -146,17 +148,27 @@ struct DPtrs {
   const double* rawb;         /* [4 valences 3..6][6][12] */
   const int32_t* inc_start;   /* nV+1 */
   const int32_t* inc;         /* e*16+i, e global */
-  const double* incphi;       /* per incidence entry: phi_i at the element's quadrature point */
+  /* element blocks and their patch nodes: block b's nodes are records
+   * bn_start[b] .. bn_start[b+1] (global ids bn_ids, ascending); lidx is each
+   * patch slot's index into its block's nodes; record r sums the block's
+   * (element, slot) entries rent[rent_start[r] .. rent_start[r+1]]
+   * (local element * 16 + slot); node n gathers the records
+   * nrec[nrec_start[n] .. nrec_start[n+1]] */
+  const int32_t* bn_start;    /* blocks+1 */
+  const int32_t* bn_ids;      /* records */
+  const uint16_t* lidx;       /* 12*nE, SoA [slot][nE] */
+  const int32_t* rent_start;  /* records+1 */
+  const uint16_t* rent;
+  const int32_t* nrec_start;  /* nV+1 */
+  const int32_t* nrec;
   double* minv;               /* nV */
   double* cdamp;              /* nV */
   double* U[2];
   double* V[2];
   double* A[2];
-  double* qe;                 /* 36*nE: per element and patch slot, internal-force contribution */
-  double* g0;                 /* 3*nE: JxW a3 per element (pressure per unit p) */
+  double* brec;               /* kBRec per record: sum of q_i (f_i -= q_i), sum of phi_i JxW a3 (f_i += p ...) */
   float* tab32;               /* F_ROWS*nE: tables of the FP32 split path */
-  float* qe32;                /* 36*nE, g032 3*nE: FP32 split path counterparts of qe, g0 */
-  float* g032;
+  float* brec32;              /* kBRec per record: FP32 split path counterpart of brec */
   double* area;               /* nE */
   double* fc;                 /* nV*3 contact force accumulator */
   double* fout;               /* nV*3 outputs of a plain force evaluation */

@@ -341,32 +341,30 @@ __device__ __forceinline__ int vmap_find(int vb, int n, const int* s_pref)
 /* The element force.  Per element: gather u_pred over the patch, form the
  * deformed metric (compute_deformed_metrics, ShellSystem.C:1407-1460),
  * membrane and bending strains, orthotropic H, stress resultants and the
- * p-vectors (subtract_internal_force, ShellSystem.C:1164-1233), and write the
- * element's generalised stresses G and from them the force contribution of
- * each of its 12 patch slots.  The nodal force of patch slot i is
+ * p-vectors (subtract_internal_force, ShellSystem.C:1164-1233), and the
+ * element's generalised stresses G; the force on patch slot i is
  *   f_i = -(phi1 G1 + phi2 G2 - phi11 G3 - phi22 G4 - phi12 G5) + p phi G0,
  * the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis derivative
  * (ShellSystem.C:1236-1282) plus the LOCAL pressure load (ShellSystem.C:
- * 1570-1576).
+ * 1570-1576), G0 = JxW a3.
  *
- * Work split of a block (kElemPerBlock elements, one per thread): the patch
- * gather runs in kElemLanes passes over 64 elements with kElemLanes lanes per
- * element (3 patch slots each, combined by a fixed butterfly); after pass s
- * the lane with index s keeps the sums, so that every thread then owns one
- * element, e = first + 64 lane + t / kElemLanes, and forms its metric,
- * strains and stresses alone (no lane repeats another's arithmetic).  The
- * generalised stresses pass through LDS and the slot contributions are
- * written with kElemLanes lanes per element again (24 contiguous bytes per
- * lane).  Block partials (energies, volume terms, max |du|^2) go to epart for
- * k_fin_elem.  The blocks of a shell whose contact list was rebuilt at the
- * end of the previous step clear its cell buckets.  Launch block b computes
- * element block bmap[V_ELEM][b] (the blocks of the active shells, set by the
- * host). */
+ * A block covers kElemPerBlock elements, one per thread.  It first stages
+ * its patch nodes (bn_ids: u_pred, u, X) in LDS with coalesced loads, so
+ * the patch gather reads LDS only; after the stress phase the LDS holds each
+ * element's G and R, and one thread per block node sums the block's slot
+ * contributions to that node (its entries in fixed order) into the node's
+ * block record: sum q_i and sum phi_i G0.  The node kernel adds a node's
+ * records (one to three), so no contribution is written per (element, slot)
+ * and no atomics are used.  Block partials (energies, volume terms, max
+ * |du|^2) go to epart for k_fin_elem.  The blocks of a shell whose contact
+ * list was rebuilt at the end of the previous step clear its cell buckets.
+ * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
+ * active shells, set by the host). */
 __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_w[5][kElemBlock / 64];
-  __shared__ double s_g[kElemPerBlock][kGRec];   /* G1..G5, then R00 R01 R10 R11 */
+  __shared__ double s_buf[kElemLds];     /* staged nodes, then per element G0..G5 and R */
   __shared__ int s_n[kElemPerBlock];
   const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
@@ -375,15 +373,10 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
   if (sh.bclear && C.contact) clear_buckets(P, C, blk - C.blkE, C.nbE);
   if (sh.done) return;
   load_rawb(P.rawb, s_rb);
-  __syncthreads();
   {
   const int nE = P.nE;
   const int t = threadIdx.x;
-  const int lane = t % kElemLanes;
-  const int grp = t / kElemLanes;
-  const int first = blk * kElemPerBlock;
-  const int own = 64 * lane + grp;                 /* the element this thread owns, in the block */
-  const int e = first + own;
+  const int e = blk * kElemPerBlock + t;
   const bool valid = e - C.ebase < C.ne;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
@@ -392,69 +385,56 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
   const double* V = P.V[par];
   const double* Aa = P.A[par];
   const double* T = P.tab;
+  const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int n = P.bn_ids[b0 + j];
+    const v3 u = ld3(U, n);
+    const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
+    const v3 x = ld3(P.X, n);
+    double* sn = s_buf + kNodeStage * j;
+    sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
+    sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
+    sn[6] = x.x; sn[7] = x.y; sn[8] = x.z;
+  }
+  __syncthreads();
   double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
   v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
-  /* patch gather: pass s covers elements first + 64 s + [0, 64); each lane
-   * gathers patch slots lane, lane+4, lane+8 */
-#pragma unroll 1
-  for (int s = 0; s < kElemLanes; ++s) {
-    const int es = first + 64 * s + grp;
-    v3 p1 = mk(0, 0, 0), p2 = p1, p11 = p1, p22 = p1, p12 = p1, pa = p1, pb = p1, pc = p1;
-    double pd = 0;
-    if (es - C.ebase < C.ne) {
-      const double R00 = T[(size_t)T_R00 * nE + es], R01 = T[(size_t)T_R01 * nE + es];
-      const double R10 = T[(size_t)T_R10 * nE + es], R11 = T[(size_t)T_R11 * nE + es];
-      const int N = P.vcls[es];
-      const double* rb = s_rb + 72 * (N - 3);
-#pragma unroll
-      for (int k = 0; k < 12 / kElemLanes; ++k) {
-        const int i = lane + kElemLanes * k;
-        const int n = P.pidx[(size_t)i * nE + es];
-        const v3 u = ld3(U, n);
-        v3 up = u;
-        if (predict) up = pred3(u, ld3(V, n), ld3(Aa, n), dt, a8);
-        const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
-        axpy(p1, up, b.d1);
-        axpy(p2, up, b.d2);
-        axpy(p11, up, b.d11);
-        axpy(p22, up, b.d22);
-        axpy(p12, up, b.d12);
-        if (i == 0 || i == 1 || i == N) {
-          if (predict) {
-            const v3 d = sub(up, u);
-            pd = fmax(pd, d.x * d.x + d.y * d.y + d.z * d.z);
-          }
-          const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
-          if (i == 0) pa = x;
-          else if (i == 1) pb = x;
-          else pc = x;
-        }
-      }
-    }
-    /* combine the lanes: fixed butterfly (the lanes of one element are
-     * adjacent and all valid or all invalid) */
-#pragma unroll
-    for (int off = 1; off < kElemLanes; off <<= 1) {
-      v3* vs[8] = {&p1, &p2, &p11, &p22, &p12, &pa, &pb, &pc};
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        vs[q]->x += __shfl_xor(vs[q]->x, off);
-        vs[q]->y += __shfl_xor(vs[q]->y, off);
-        vs[q]->z += __shfl_xor(vs[q]->z, off);
-      }
-      pd = fmax(pd, __shfl_xor(pd, off));
-    }
-    if (lane == s) {
-      du1 = p1; du2 = p2; du11 = p11; du22 = p22; du12 = p12;
-      dumax = pd;
-      /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
-      det = pa.x * (pb.y * pc.z - pc.y * pb.z) - pb.x * (pa.y * pc.z - pc.y * pa.z) +
-            pc.x * (pa.y * pb.z - pb.y * pa.z);
-    }
-  }
+  double R00 = 0, R01 = 0, R10 = 0, R11 = 0;
+  int N = 6;
   if (valid) {
-    const double R00 = T[(size_t)T_R00 * nE + e], R01 = T[(size_t)T_R01 * nE + e];
-    const double R10 = T[(size_t)T_R10 * nE + e], R11 = T[(size_t)T_R11 * nE + e];
+    R00 = T[(size_t)T_R00 * nE + e]; R01 = T[(size_t)T_R01 * nE + e];
+    R10 = T[(size_t)T_R10 * nE + e]; R11 = T[(size_t)T_R11 * nE + e];
+    N = P.vcls[e];
+    const double* rb = s_rb + 72 * (N - 3);
+    v3 xa = du1, xb = du1, xc = du1;
+#pragma unroll 2
+    for (int i = 0; i < 12; ++i) {
+      const double* sn = s_buf + kNodeStage * P.lidx[(size_t)i * nE + e];
+      const v3 up = mk(sn[0], sn[1], sn[2]);
+      const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
+      axpy(du1, up, b.d1);
+      axpy(du2, up, b.d2);
+      axpy(du11, up, b.d11);
+      axpy(du22, up, b.d22);
+      axpy(du12, up, b.d12);
+      if (i == 0 || i == 1 || i == N) {
+        const v3 u = mk(sn[3], sn[4], sn[5]);
+        if (predict) {
+          const v3 d = sub(up, u);
+          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+        }
+        const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
+        if (i == 0) xa = x;
+        else if (i == 1) xb = x;
+        else xc = x;
+      }
+    }
+    /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
+    det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
+          xc.x * (xa.y * xb.z - xb.y * xa.z);
+  }
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes G and R */
+  if (valid) {
     const v3 a1 = add(ldrow3(T, T_A1, nE, e), du1);
     const v3 a2 = add(ldrow3(T, T_A2, nE, e), du2);
     v3 a3 = crs(a1, a2);
@@ -500,15 +480,11 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
     v3 g2 = add(add(add(add(scl(a2, nn[1]), scl(a1, nn[2])), scl(p21, mm[0])), scl(p22, mm[1])), scl(p23, m2x2));
     const v3 G[6] = {scl(a3, JxW), scl(g1, JxW), scl(g2, JxW), scl(a3, JxW * mm[0]), scl(a3, JxW * mm[1]),
                      scl(a3, JxW * m2x2)};
-    double* gs = s_g[own];
-    gs[0] = G[1].x; gs[1] = G[1].y; gs[2] = G[1].z;
-    gs[3] = G[2].x; gs[4] = G[2].y; gs[5] = G[2].z;
-    gs[6] = G[3].x; gs[7] = G[3].y; gs[8] = G[3].z;
-    gs[9] = G[4].x; gs[10] = G[4].y; gs[11] = G[4].z;
-    gs[12] = G[5].x; gs[13] = G[5].y; gs[14] = G[5].z;
-    gs[15] = R00; gs[16] = R01; gs[17] = R10; gs[18] = R11;
-    s_n[own] = P.vcls[e];
-    P.g0[3 * e] = G[0].x; P.g0[3 * e + 1] = G[0].y; P.g0[3 * e + 2] = G[0].z;
+    double* gs = s_buf + kGRec * t;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { gs[3 * c] = G[c].x; gs[3 * c + 1] = G[c].y; gs[3 * c + 2] = G[c].z; }
+    gs[18] = R00; gs[19] = R01; gs[20] = R10; gs[21] = R11;
+    s_n[t] = N;
     Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
     Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
     const double new_area = 0.5 * jac;
@@ -516,28 +492,24 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
     P.area[e] = new_area;
   }
   __syncthreads();
-  /* per patch slot: the internal-force contribution qe_i (f_i -= qe_i), 24
-   * contiguous bytes per slot for the node gather; the pressure vector
-   * JxW a3 of the element (f_i += p phi_i JxW a3) was written above */
-#pragma unroll 1
-  for (int s = 0; s < kElemLanes; ++s) {
-    const int le = 64 * s + grp;
-    const int es = first + le;
-    if (es - C.ebase < C.ne) {
-      const double* gs = s_g[le];
-      const v3 G1 = mk(gs[0], gs[1], gs[2]), G2 = mk(gs[3], gs[4], gs[5]), G3 = mk(gs[6], gs[7], gs[8]);
-      const v3 G4 = mk(gs[9], gs[10], gs[11]), G5 = mk(gs[12], gs[13], gs[14]);
-      const double* rb = s_rb + 72 * (s_n[le] - 3);
-      double* qrow = P.qe + (size_t)36 * es;
-#pragma unroll
-      for (int k = 0; k < 12 / kElemLanes; ++k) {
-        const int i = lane + kElemLanes * k;
-        const Basis6 b = basis_at(rb, i, gs[15], gs[16], gs[17], gs[18]);
-        const v3 q = sub(sub(sub(add(scl(G1, b.d1), scl(G2, b.d2)), scl(G3, b.d11)), scl(G4, b.d22)),
-                         scl(G5, b.d12));
-        qrow[3 * i] = q.x; qrow[3 * i + 1] = q.y; qrow[3 * i + 2] = q.z;
-      }
+  /* block records: per block node, its slot contributions summed in entry
+   * order (q_i for f_i -= q_i, phi_i G0 for f_i += p phi_i G0) */
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int r = b0 + j;
+    v3 qs = mk(0, 0, 0), ps = qs;
+    for (int k = P.rent_start[r]; k < P.rent_start[r + 1]; ++k) {
+      const int ent = P.rent[k], le = ent >> 4, i = ent & 15;
+      const double* gs = s_buf + kGRec * le;
+      const Basis6 b = basis_at(s_rb + 72 * (s_n[le] - 3), i, gs[18], gs[19], gs[20], gs[21]);
+      const v3 G1 = mk(gs[3], gs[4], gs[5]), G2 = mk(gs[6], gs[7], gs[8]), G3 = mk(gs[9], gs[10], gs[11]);
+      const v3 G4 = mk(gs[12], gs[13], gs[14]), G5 = mk(gs[15], gs[16], gs[17]);
+      const v3 q = sub(sub(sub(add(scl(G1, b.d1), scl(G2, b.d2)), scl(G3, b.d11)), scl(G4, b.d22)),
+                       scl(G5, b.d12));
+      qs = add(qs, q);
+      axpy(ps, mk(gs[0], gs[1], gs[2]), b.phi);
     }
+    double* rec = P.brec + (size_t)kBRec * r;
+    rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
   }
   /* block partial: wave butterflies, then the block's waves in order */
   double rr[5] = {Et, Eb, dumax, det, oarea};
@@ -653,12 +625,12 @@ __global__ void __launch_bounds__(256) k_precompute32(DPtrs P)
  * The displacements enter the FP32 gradient sums relative to the patch's
  * first node (formed in FP64).  The predictor, the element vertex positions
  * of the enclosed volume and all reductions stay FP64.  Slot contributions
- * and JxW a3 are written in FP32 (qe32, g032). */
+ * and JxW a3 enter the block records in FP32 (brec32). */
 __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
 {
   __shared__ float s_rb[4 * 72];
   __shared__ double s_w[5][kElemBlock / 64];
-  __shared__ float s_g[kElemPerBlock][kGRec];    /* G1..G5, then R00 R01 R10 R11 */
+  __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element G0..G5 and R (FP32) */
   __shared__ int s_n[kElemPerBlock];
   const int blk = uni(P.bmap[V_ELEM32][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
@@ -667,14 +639,9 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   if (sh.bclear && C.contact) clear_buckets(P, C, blk - C.blkE, C.nbE);
   if (sh.done) return;
   for (int q = threadIdx.x; q < 4 * 72; q += blockDim.x) s_rb[q] = (float)P.rawb[q];
-  __syncthreads();
   const int nE = P.nE;
   const int t = threadIdx.x;
-  const int lane = t % kElemLanes;
-  const int grp = t / kElemLanes;
-  const int first = blk * kElemPerBlock;
-  const int own = 64 * lane + grp;                 /* the element this thread owns, in the block */
-  const int e = first + own;
+  const int e = blk * kElemPerBlock + t;
   const bool valid = e - C.ebase < C.ne;
   const bool predict = (mode == M_STEP);
   const double dt = sh.dt, a8 = sh.nm[8];
@@ -683,84 +650,63 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   const double* V = P.V[par];
   const double* Aa = P.A[par];
   const float* T = P.tab32;
+  const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int n = P.bn_ids[b0 + j];
+    const v3 u = ld3(U, n);
+    const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
+    const v3 x = ld3(P.X, n);
+    double* sn = s_buf + kNodeStage * j;
+    sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
+    sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
+    sn[6] = x.x; sn[7] = x.y; sn[8] = x.z;
+  }
+  __syncthreads();
   double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
   v3f du1 = mkf(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
-  /* patch gather in kElemLanes passes of 64 elements (as k_elem) */
-#pragma unroll 1
-  for (int s = 0; s < kElemLanes; ++s) {
-    const int es = first + 64 * s + grp;
-    v3f p1 = mkf(0, 0, 0), p2 = p1, p11 = p1, p22 = p1, p12 = p1;
-    v3 pa = mk(0, 0, 0), pb = pa, pc = pa;
-    double pd = 0;
-    if (es - C.ebase < C.ne) {
-      const float R00 = T[(size_t)F_R00 * nE + es], R01 = T[(size_t)F_R01 * nE + es];
-      const float R10 = T[(size_t)F_R10 * nE + es], R11 = T[(size_t)F_R11 * nE + es];
-      const int N = P.vcls[es];
-      const float* rb = s_rb + 72 * (N - 3);
-      /* displacements relative to the patch's first node, formed in FP64 and
-       * then rounded: the gradient sums are unchanged (the basis derivatives
-       * sum to zero over the patch) and the rounding acts on |u_i - u_0| */
-      v3 u0;
-      {
-        const int n0 = P.pidx[es];
-        u0 = ld3(U, n0);
-        if (predict) u0 = pred3(u0, ld3(V, n0), ld3(Aa, n0), dt, a8);
-      }
-#pragma unroll
-      for (int k = 0; k < 12 / kElemLanes; ++k) {
-        const int i = lane + kElemLanes * k;
-        const int n = P.pidx[(size_t)i * nE + es];
-        const v3 u = ld3(U, n);
-        v3 up = u;
-        if (predict) up = pred3(u, ld3(V, n), ld3(Aa, n), dt, a8);
-        const v3 ur = sub(up, u0);
-        const v3f uf = mkf((float)ur.x, (float)ur.y, (float)ur.z);
-        const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
-        axpyf(p1, uf, dx * R00 + de * R10);
-        axpyf(p2, uf, dx * R01 + de * R11);
-        axpyf(p11, uf, dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10);
-        axpyf(p22, uf, dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11);
-        axpyf(p12, uf, dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11);
-        if (i == 0 || i == 1 || i == N) {
-          if (predict) {
-            const v3 d = sub(up, u);
-            pd = fmax(pd, d.x * d.x + d.y * d.y + d.z * d.z);
-          }
-          const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
-          if (i == 0) pa = x;
-          else if (i == 1) pb = x;
-          else pc = x;
-        }
-      }
-    }
-#pragma unroll
-    for (int off = 1; off < kElemLanes; off <<= 1) {
-      v3f* vs[5] = {&p1, &p2, &p11, &p22, &p12};
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        vs[q]->x += __shfl_xor(vs[q]->x, off);
-        vs[q]->y += __shfl_xor(vs[q]->y, off);
-        vs[q]->z += __shfl_xor(vs[q]->z, off);
-      }
-      v3* xs[3] = {&pa, &pb, &pc};
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        xs[q]->x += __shfl_xor(xs[q]->x, off);
-        xs[q]->y += __shfl_xor(xs[q]->y, off);
-        xs[q]->z += __shfl_xor(xs[q]->z, off);
-      }
-      pd = fmax(pd, __shfl_xor(pd, off));
-    }
-    if (lane == s) {
-      du1 = p1; du2 = p2; du11 = p11; du22 = p22; du12 = p12;
-      dumax = pd;
-      det = pa.x * (pb.y * pc.z - pc.y * pb.z) - pb.x * (pa.y * pc.z - pc.y * pa.z) +
-            pc.x * (pa.y * pb.z - pb.y * pa.z);
-    }
-  }
+  float R00 = 0, R01 = 0, R10 = 0, R11 = 0;
+  int N = 6;
   if (valid) {
-    const float R00 = T[(size_t)F_R00 * nE + e], R01 = T[(size_t)F_R01 * nE + e];
-    const float R10 = T[(size_t)F_R10 * nE + e], R11 = T[(size_t)F_R11 * nE + e];
+    R00 = T[(size_t)F_R00 * nE + e]; R01 = T[(size_t)F_R01 * nE + e];
+    R10 = T[(size_t)F_R10 * nE + e]; R11 = T[(size_t)F_R11 * nE + e];
+    N = P.vcls[e];
+    const float* rb = s_rb + 72 * (N - 3);
+    /* displacements relative to the patch's first node, formed in FP64 and
+     * then rounded: the gradient sums are unchanged (the basis derivatives
+     * sum to zero over the patch) and the rounding acts on |u_i - u_0| */
+    const double* s0 = s_buf + kNodeStage * P.lidx[e];
+    const v3 u0 = mk(s0[0], s0[1], s0[2]);
+    v3 xa = mk(0, 0, 0), xb = xa, xc = xa;
+#pragma unroll 2
+    for (int i = 0; i < 12; ++i) {
+      const double* sn = s_buf + kNodeStage * P.lidx[(size_t)i * nE + e];
+      const v3 up = mk(sn[0], sn[1], sn[2]);
+      const v3 ur = sub(up, u0);
+      const v3f uf = mkf((float)ur.x, (float)ur.y, (float)ur.z);
+      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+      axpyf(du1, uf, dx * R00 + de * R10);
+      axpyf(du2, uf, dx * R01 + de * R11);
+      axpyf(du11, uf, dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10);
+      axpyf(du22, uf, dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11);
+      axpyf(du12, uf, dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11);
+      if (i == 0 || i == 1 || i == N) {
+        const v3 u = mk(sn[3], sn[4], sn[5]);
+        if (predict) {
+          const v3 d = sub(up, u);
+          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+        }
+        const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
+        if (i == 0) xa = x;
+        else if (i == 1) xb = x;
+        else xc = x;
+      }
+    }
+    det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
+          xc.x * (xa.y * xb.z - xb.y * xa.z);
+  }
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes G and R */
+  float* s_g = reinterpret_cast<float*>(s_buf);
+  if (valid) {
     const v3f A1 = ldrow3f(T, F_A1, nE, e), A2 = ldrow3f(T, F_A2, nE, e);
     const v3f A11 = ldrow3f(T, F_A11, nE, e), A22 = ldrow3f(T, F_A22, nE, e), A12 = ldrow3f(T, F_A12, nE, e);
     const v3f A3 = ldrow3f(T, F_A3, nE, e);
@@ -807,15 +753,11 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
                         sclf(p23, m2x2));
     const v3f G[6] = {sclf(a3, JxW), sclf(g1, JxW), sclf(g2, JxW), sclf(a3, JxW * mm[0]), sclf(a3, JxW * mm[1]),
                       sclf(a3, JxW * m2x2)};
-    float* gs = s_g[own];
-    gs[0] = G[1].x; gs[1] = G[1].y; gs[2] = G[1].z;
-    gs[3] = G[2].x; gs[4] = G[2].y; gs[5] = G[2].z;
-    gs[6] = G[3].x; gs[7] = G[3].y; gs[8] = G[3].z;
-    gs[9] = G[4].x; gs[10] = G[4].y; gs[11] = G[4].z;
-    gs[12] = G[5].x; gs[13] = G[5].y; gs[14] = G[5].z;
-    gs[15] = R00; gs[16] = R01; gs[17] = R10; gs[18] = R11;
-    s_n[own] = P.vcls[e];
-    P.g032[3 * e] = G[0].x; P.g032[3 * e + 1] = G[0].y; P.g032[3 * e + 2] = G[0].z;
+    float* gs = s_g + kGRec * t;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { gs[3 * c] = G[c].x; gs[3 * c + 1] = G[c].y; gs[3 * c + 2] = G[c].z; }
+    gs[18] = R00; gs[19] = R01; gs[20] = R10; gs[21] = R11;
+    s_n[t] = N;
     Et = 0.5 * (double)JxW * ((double)nn[0] * alpha[0] + (double)nn[1] * alpha[1] + (double)nn[2] * alpha[2]);
     Eb = 0.5 * (double)JxW * ((double)mm[0] * beta[0] + (double)mm[1] * beta[1] + (double)mm[2] * beta[2]);
     const double new_area = 0.5 * (double)jac;
@@ -823,30 +765,28 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     P.area[e] = new_area;
   }
   __syncthreads();
-#pragma unroll 1
-  for (int s = 0; s < kElemLanes; ++s) {
-    const int le = 64 * s + grp;
-    const int es = first + le;
-    if (es - C.ebase < C.ne) {
-      const float* gs = s_g[le];
-      const v3f G1 = mkf(gs[0], gs[1], gs[2]), G2 = mkf(gs[3], gs[4], gs[5]), G3 = mkf(gs[6], gs[7], gs[8]);
-      const v3f G4 = mkf(gs[9], gs[10], gs[11]), G5 = mkf(gs[12], gs[13], gs[14]);
-      const float R00 = gs[15], R01 = gs[16], R10 = gs[17], R11 = gs[18];
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int r = b0 + j;
+    v3f qs = mkf(0, 0, 0), ps = qs;
+    for (int k = P.rent_start[r]; k < P.rent_start[r + 1]; ++k) {
+      const int ent = P.rent[k], le = ent >> 4, i = ent & 15;
+      const float* gs = s_g + kGRec * le;
       const float* rb = s_rb + 72 * (s_n[le] - 3);
-      float* qrow = P.qe32 + (size_t)36 * es;
-#pragma unroll
-      for (int k = 0; k < 12 / kElemLanes; ++k) {
-        const int i = lane + kElemLanes * k;
-        const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
-        const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
-        const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
-        const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
-        const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
-        const v3f q = subf(subf(subf(addf(sclf(G1, b1), sclf(G2, b2)), sclf(G3, b11)), sclf(G4, b22)),
-                           sclf(G5, b12));
-        qrow[3 * i] = q.x; qrow[3 * i + 1] = q.y; qrow[3 * i + 2] = q.z;
-      }
+      const float R00 = gs[18], R01 = gs[19], R10 = gs[20], R11 = gs[21];
+      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+      const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
+      const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
+      const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
+      const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
+      const v3f G1 = mkf(gs[3], gs[4], gs[5]), G2 = mkf(gs[6], gs[7], gs[8]), G3 = mkf(gs[9], gs[10], gs[11]);
+      const v3f G4 = mkf(gs[12], gs[13], gs[14]), G5 = mkf(gs[15], gs[16], gs[17]);
+      const v3f q = subf(subf(subf(addf(sclf(G1, b1), sclf(G2, b2)), sclf(G3, b11)), sclf(G4, b22)),
+                         sclf(G5, b12));
+      qs = addf(qs, q);
+      axpyf(ps, mkf(gs[0], gs[1], gs[2]), rb[i]);
     }
+    float* rec = P.brec32 + (size_t)kBRec * r;
+    rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
   }
   double rr[5] = {Et, Eb, dumax, det, oarea};
 #pragma unroll
@@ -1116,17 +1056,16 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   if (w.problem >= 7 || w.err) w.done = 1;
 }
 
-/* kNodeLanes lanes per node: each lane folds every kNodeLanes-th entry of the
- * node's (element, slot) incidence list (the slot's internal-force
- * contribution and phi_i times the element's JxW a3), the lanes combine by a
- * fixed butterfly, then lane 0 adds the contact force and applies the damped
- * central-difference / Newmark update (ShellSystem.C:1006-1008, 1068-1090):
- * a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a, v = v_pred + gamma dt a,
- * written to the other state buffer.  Block partials: |f|^2, |f_ext|^2,
- * max|a-a_old|, max|v_pred|, max|du|^2, E_kin and the rigid-body sums, for
- * k_fin_node.  Launch block b computes node block bmap[V_NODE][b].  Force
- * time: t + dt in a step,
- * t + dt in M_INIT (apply_initial_conditions), DShell::t_eval in M_FORCES. */
+/* One thread per node: the node's internal force and pressure direction are
+ * the sums of its element-block records (written by the element kernel, in
+ * ascending block order), then the contact force is added and the damped
+ * central-difference / Newmark update applied (ShellSystem.C:1006-1008,
+ * 1068-1090): a = M^-1 (f - C v_pred), u = u_pred + beta dt^2 a,
+ * v = v_pred + gamma dt a, written to the other state buffer.  Block
+ * partials: |f|^2, |f_ext|^2, max|a-a_old|, max|v_pred|, max|du|^2, E_kin and
+ * the rigid-body sums, for k_fin_node.  Launch block b computes node block
+ * bmap[V_NODE][b].  Force time: t + dt in a step, t + dt in M_INIT
+ * (apply_initial_conditions), DShell::t_eval in M_FORCES. */
 __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
 {
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
@@ -1137,60 +1076,29 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
   if (sh.done) return;
   {
   const int t = threadIdx.x;
-  const int lane = t % kNodeLanes;
-  const int n = blk * kNodePerBlock + t / kNodeLanes;
+  const int n = blk * kNodePerBlock + t;
   const bool valid = n - C.vbase < C.nv;
   const int par = sh.par;
   const double dt = sh.dt;
-  /* the node's own state, loaded before the gather so that its latency
-   * overlaps the gather's */
-  v3 su = mk(0, 0, 0), sv = su, sa = su, sx = su, sfc = su;
-  double smi = 1, scd = 0;
-  if (valid && lane == 0) {
-    su = ld3(P.U[par], n); sv = ld3(P.V[par], n); sa = ld3(P.A[par], n);
-    sx = ld3(P.X, n); sfc = ld3(P.fc, n);
-    smi = P.minv[n]; scd = P.cdamp[n];
-  }
   v3 fint = mk(0, 0, 0), fp = fint;
-  const int q0 = valid ? P.inc_start[n] : 0, q1 = valid ? P.inc_start[n + 1] : 0;
-  /* each lane folds entries q0 + lane + kNodeLanes j in order of j; the
-   * first kIncUnroll are unrolled so that their loads are in flight
-   * together */
-  if (sh.eprec == SHX_PREC_FP32) {
-    /* FP32 split path: FP32 contributions, FP32 accumulation */
-    v3f fi = mkf(0, 0, 0), fq = fi;
-    auto fold = [&](int q) {
-      const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
-      const float phi = (float)P.incphi[q];
-      const float* qs = P.qe32 + (size_t)36 * e + 3 * i;
-      fi = subf(fi, mkf(qs[0], qs[1], qs[2]));
-      axpyf(fq, mkf(P.g032[3 * e], P.g032[3 * e + 1], P.g032[3 * e + 2]), phi);
-    };
-#pragma unroll
-    for (int j = 0; j < kIncUnroll; ++j)
-      if (q0 + lane + kNodeLanes * j < q1) fold(q0 + lane + kNodeLanes * j);
-    for (int q = q0 + lane + kNodeLanes * kIncUnroll; q < q1; q += kNodeLanes) fold(q);
-    for (int off = 1; off < kNodeLanes; off <<= 1) {
-      fi.x += __shfl_xor(fi.x, off); fi.y += __shfl_xor(fi.y, off); fi.z += __shfl_xor(fi.z, off);
-      fq.x += __shfl_xor(fq.x, off); fq.y += __shfl_xor(fq.y, off); fq.z += __shfl_xor(fq.z, off);
-    }
-    fint = mk(fi.x, fi.y, fi.z);
-    fp = mk(fq.x, fq.y, fq.z);
-  } else {
-    auto fold = [&](int q) {
-      const int ent = P.inc[q], e = ent >> 4, i = ent & 15;
-      const double phi = P.incphi[q];
-      const double* qs = P.qe + (size_t)36 * e + 3 * i;
-      fint = sub(fint, mk(qs[0], qs[1], qs[2]));
-      axpy(fp, ld3(P.g0, e), phi);
-    };
-#pragma unroll
-    for (int j = 0; j < kIncUnroll; ++j)
-      if (q0 + lane + kNodeLanes * j < q1) fold(q0 + lane + kNodeLanes * j);
-    for (int q = q0 + lane + kNodeLanes * kIncUnroll; q < q1; q += kNodeLanes) fold(q);
-    for (int off = 1; off < kNodeLanes; off <<= 1) {
-      fint.x += __shfl_xor(fint.x, off); fint.y += __shfl_xor(fint.y, off); fint.z += __shfl_xor(fint.z, off);
-      fp.x += __shfl_xor(fp.x, off); fp.y += __shfl_xor(fp.y, off); fp.z += __shfl_xor(fp.z, off);
+  if (valid) {
+    const int r0 = P.nrec_start[n], r1 = P.nrec_start[n + 1];
+    if (sh.eprec == SHX_PREC_FP32) {
+      /* FP32 split path: FP32 records, FP32 accumulation */
+      v3f fi = mkf(0, 0, 0), fq = fi;
+      for (int k = r0; k < r1; ++k) {
+        const float* rec = P.brec32 + (size_t)kBRec * P.nrec[k];
+        fi = subf(fi, mkf(rec[0], rec[1], rec[2]));
+        fq = addf(fq, mkf(rec[3], rec[4], rec[5]));
+      }
+      fint = mk(fi.x, fi.y, fi.z);
+      fp = mk(fq.x, fq.y, fq.z);
+    } else {
+      for (int k = r0; k < r1; ++k) {
+        const double* rec = P.brec + (size_t)kBRec * P.nrec[k];
+        fint = sub(fint, mk(rec[0], rec[1], rec[2]));
+        fp = add(fp, mk(rec[3], rec[4], rec[5]));
+      }
     }
   }
   /* partial record: |f|^2, |f_ext|^2, max|a-a_old|, max|v_pred|, max|du|^2,
@@ -1199,7 +1107,7 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
   double rec[kNodeRec];
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) rec[q] = 0;
-  if (valid && lane == 0) {
+  if (valid) {
     const double t_eval = (mode == M_FORCES) ? sh.t_eval : sh.t + dt;
     double ptot = pressure_at(C, t_eval);
     if (C.bulk_modulus != 0) {
@@ -1208,16 +1116,16 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
       const double V0 = (mode == M_INIT) ? (sh.volume_ref > 0 ? sh.volume_ref : Vc) : sh.volume0;
       ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / Vc - 1);
     }
-    const v3 fcn = sfc;
+    const v3 fcn = ld3(P.fc, n);
     st3(P.fc, n, mk(0, 0, 0));
     const v3 fext = add(scl(fp, ptot), fcn);
     const v3 f = add(fint, fext);
     rec[0] = nrm2(f);
     rec[1] = nrm2(fext);
-    const double mi = smi;
+    const double mi = P.minv[n];
+    const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
     {
-      const v3 u = su;
-      const v3 x = add(sx, mode == M_STEP ? pred3(u, sv, sa, dt, sh.nm[8]) : u);
+      const v3 x = add(ld3(P.X, n), mode == M_STEP ? pred3(u, v, a, dt, sh.nm[8]) : u);
       const double m = 1 / mi, m2 = m * m;
       const v3 xf = crs(x, f);
       rec[6] = m * f.x; rec[7] = m * f.y; rec[8] = m * f.z;
@@ -1233,8 +1141,7 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
     } else if (mode == M_INIT) {
       st3(P.A[par], n, scl(f, mi));
     } else {
-      const double cd = scd + sh.alpha_extra / mi;   /* + alpha_extra m */
-      const v3 u = su, v = sv, a = sa;
+      const double cd = P.cdamp[n] + sh.alpha_extra / mi;   /* + alpha_extra m */
       const double a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];
       const v3 up = pred3(u, v, a, dt, a8);
       const v3 vp = mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
@@ -1251,12 +1158,11 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
       st3(P.A[1 - par], n, an);
     }
   }
-  /* wave butterfly over the nodes' lane 0 (the other lanes hold zeros),
-   * then the block's waves in order */
+  /* wave butterfly, then the block's waves in order */
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) {
     const bool mx = (q >= 2 && q <= 4);
-    for (int off = 32; off >= kNodeLanes; off >>= 1) {
+    for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_xor(rec[q], off);
       rec[q] = mx ? fmax(rec[q], o) : rec[q] + o;
     }
