@@ -43,16 +43,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def element_kernel_bytes(ne, nv):
-    """algorithmic HBM bytes of one element-force launch over a shell (FP64):
-    per element: 12 int32 patch ids + 1 valence byte + 26 FP64 table rows
-    (R 2x2, JxW, A1 A2 A11 A22 A12, ff1, ff2) read; 12 slot force
-    contributions (36 FP64) and JxW a3 (3 FP64) written; element area read
-    and written (585 B); per node: u, v, a and the mesh position read once
-    (96 B)."""
-    per_elem = 12 * 4 + 1 + 26 * 8 + 39 * 8 + 2 * 8
-    per_node = 4 * 3 * 8
-    return ne * per_elem + nv * per_node
+def force_blocks(T, nv, block=256):
+    """block-node records and (element, slot) entries of a mesh's element
+    blocks: the patch of an element is the one-ring of its triangle (the
+    vertices of every triangle sharing a vertex with it: 12 nodes, 11 next to
+    a valence-5 vertex)"""
+    import scipy.sparse as sp
+    ne = len(T)
+    inc = sp.csr_matrix((np.ones(3 * ne), (np.repeat(np.arange(ne), 3), np.asarray(T).reshape(-1))),
+                        shape=(ne, nv))
+    patch = ((inc @ inc.T) @ inc) > 0                   # element x node: patch membership
+    entries = int(patch.nnz)
+    records = 0
+    for b0 in range(0, ne, block):
+        records += int((patch[b0:b0 + block].sum(axis=0) > 0).sum())
+    return records, entries
+
+
+_BLOCKS = {}
+
+
+def element_kernel_bytes(T, nv):
+    """algorithmic HBM bytes of one element-force launch over a shell (FP64,
+    the benchmark's volume-at-prediction mode): per element 26 FP64 table rows
+    (R 2x2, JxW, A1 A2 A11 A22 A12, ff1, ff2), 12 uint16 patch-node indices, the
+    valence byte, the element area written (241 B); per (element, slot)
+    entry its uint16 id; per block-node record its node id, entry offset and
+    the 6 FP64 sums written (56 B); per node u, v, a and X read once (96 B;
+    the re-read of nodes shared by two blocks is not counted)."""
+    key = (len(T), nv, hash(np.asarray(T).tobytes()))
+    if key not in _BLOCKS:
+        _BLOCKS[key] = force_blocks(T, nv)
+    records, entries = _BLOCKS[key]
+    return len(T) * (26 * 8 + 12 * 2 + 1 + 8) + entries * 2 + records * (4 + 4 + 6 * 8) + nv * 96
 
 
 def single_workload(level=4):
@@ -81,7 +104,7 @@ def roofline(b, meshes, steps=200):
     kt = b.kernel_times()
     b.set_profiling(False)
     elem_ms = kt["elem_force"][0]
-    nbytes = sum(element_kernel_bytes(len(T), len(X)) for X, T in meshes)
+    nbytes = sum(element_kernel_bytes(T, len(X)) for X, T in meshes)
     achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_elem (element force)",
@@ -230,7 +253,7 @@ def run_single(args):
     kt = g.kernel_times()
     g.set_profiling(False)
     elem_ms = kt["elem_force"][0]
-    nbytes = element_kernel_bytes(len(T), len(X))
+    nbytes = element_kernel_bytes(T, len(X))
     achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
     cpu = None
     if not args.no_cpu:

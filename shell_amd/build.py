@@ -7,13 +7,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libshellhip.so")
-SOURCES = ["shx_api.hip", "shx_kernels.hip", "shx_mlp.hip", "shx_setup.cpp"]
+SOURCES = ["shx_api.hip", "shx_kernels.hip", "shx_elem.hip", "shx_mlp.hip", "shx_setup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # every product and sum rounded once, in source order: the kernels are
-    # compared bit for bit (pair lists) and to 1e-10 (forces) with the CPU oracle
+    # compared bit for bit (pair lists) and to 1e-10 (forces) with the CPU
+    # oracle; shx_elem.hip (element force only) enables contraction itself
     "-ffp-contract=off",
     # no packed FP32 instructions (v_pk_*_f32) in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

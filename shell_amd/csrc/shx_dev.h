@@ -1,0 +1,94 @@
+/*
+ * shx_dev.h -- device helpers shared by the kernel translation units of
+ * libshellhip (shx_kernels.hip, shx_elem.hip).
+ */
+#ifndef SHX_DEV_H
+#define SHX_DEV_H
+
+#include <hip/hip_runtime.h>
+
+#include "shx_device.h"
+#include "shx_math.h"
+
+namespace shx {
+
+/* blocks of one shell for a kernel family: V_TILE 256-element tiles,
+ * V_LANE broad-phase lane blocks, V_ELEM element-kernel blocks, V_NODE
+ * node-kernel blocks */
+enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3, V_ELEM32 = 4 };
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ v3 ld3(const double* p, int n) { return mk(p[3 * n], p[3 * n + 1], p[3 * n + 2]); }
+__device__ __forceinline__ void st3(double* p, int n, v3 a) { p[3 * n] = a.x; p[3 * n + 1] = a.y; p[3 * n + 2] = a.z; }
+__device__ __forceinline__ v3 ldrow3(const double* t, int row, int nE, int e)
+{
+  return mk(t[(size_t)row * nE + e], t[(size_t)(row + 1) * nE + e], t[(size_t)(row + 2) * nE + e]);
+}
+
+/* u_pred = u; u_pred.add(dt, v); u_pred.add(a8, a)  (ShellSystem.C:973-975),
+ * each product and sum rounded once in every translation unit (the element
+ * kernels are compiled with contraction, the others without) */
+__device__ __forceinline__ double pred1(double u, double v, double a, double dt, double a8)
+{
+  double x = u;
+  x = __dadd_rn(__dmul_rn(dt, v), x);
+  x = __dadd_rn(__dmul_rn(a8, a), x);
+  return x;
+}
+__device__ __forceinline__ v3 pred3(v3 u, v3 v, v3 a, double dt, double a8)
+{
+  return mk(pred1(u.x, v.x, a.x, dt, a8), pred1(u.y, v.y, a.y, dt, a8), pred1(u.z, v.z, a.z, dt, a8));
+}
+
+/* reparametrised basis of slot i of an element (reinit_mod,
+ * ShellSystem.C:3466-3478) from the raw basis rb (6 x 12) and R */
+struct Basis6 { double phi, d1, d2, d11, d22, d12; };
+__device__ __forceinline__ Basis6 basis_at(const double* rb, int i, double R00, double R01, double R10,
+                                           double R11)
+{
+  Basis6 b;
+  const double dx = rb[12 + i], de = rb[24 + i];
+  const double dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+  b.phi = rb[i];
+  b.d1 = dx * R00 + de * R10;
+  b.d2 = dx * R01 + de * R11;
+  b.d11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
+  b.d22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
+  b.d12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
+  return b;
+}
+
+__device__ __forceinline__ void load_rawb(const double* g, double* s)
+{
+  for (int q = threadIdx.x; q < 4 * 72; q += blockDim.x) s[q] = g[q];
+}
+
+/* clear the bucket counts of one shell's cell grid: the nb blocks lb =
+ * 0..nb-1 of a kernel's range for the shell each take a strided share */
+__device__ __forceinline__ void clear_buckets(const DPtrs& P, const SConst& C, int lb, int nb)
+{
+  const unsigned nbk = 1u << C.hb_bits;
+  for (unsigned b = (unsigned)lb * blockDim.x + threadIdx.x; b < nbk; b += (unsigned)nb * blockDim.x)
+    P.bcnt[C.bbase + b] = 0;
+}
+
+/* FP32 small-vector helpers (device only) */
+struct v3f { float x, y, z; };
+__device__ __forceinline__ v3f mkf(float x, float y, float z) { v3f r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ v3f addf(v3f a, v3f b) { return mkf(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3f subf(v3f a, v3f b) { return mkf(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3f sclf(v3f a, float s) { return mkf(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dotf(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3f crsf(v3f a, v3f b)
+{
+  return mkf(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ void axpyf(v3f& a, v3f b, float s) { a.x += b.x * s; a.y += b.y * s; a.z += b.z * s; }
+__device__ __forceinline__ v3f ldrow3f(const float* t, int row, int nE, int e)
+{
+  return mkf(t[(size_t)row * nE + e], t[(size_t)(row + 1) * nE + e], t[(size_t)(row + 2) * nE + e]);
+}
+
+} // namespace shx
+#endif

@@ -58,9 +58,9 @@ enum {
 constexpr int kAlign = 256;         /* shell ranges start at and are padded to multiples of this */
 constexpr int kElemBlock = 256;     /* threads per element-kernel block */
 constexpr int kElemPerBlock = kElemBlock;   /* elements per block, one per thread */
-constexpr int kMaxBlockNodes = 512; /* patch nodes of one element block (icospheres: at most 267) */
+constexpr int kMaxBlockNodes = 480; /* patch nodes of one element block (icospheres: at most 267) */
 constexpr int kNodeStage = 9;       /* per staged node in LDS: u_pred, u, X */
-constexpr int kGRec = 22;           /* per element in LDS: G0..G5 (18) and R (4) */
+constexpr int kGRec = 17;           /* per element in LDS: a3, JxW (mm0, mm1, 2 mm2), G1, G2, R */
 constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kElemPerBlock * kGRec) ? kMaxBlockNodes * kNodeStage
                                                                                : kElemPerBlock * kGRec;
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block, one node each */

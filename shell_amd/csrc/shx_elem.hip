@@ -1,0 +1,414 @@
+/*
+ * shx_elem.hip -- the element-force kernels of libshellhip (FP64 and FP32
+ * split): patch gather, metrics, strains, stresses and the per-block node
+ * records the node kernel sums.
+ *
+ * This translation unit is compiled with floating-point contraction
+ * (fused multiply-add): the element force is checked against the CPU oracle
+ * to a relative 1e-10, not bit for bit.  The predictor (pred3) rounds each
+ * operation as the other kernels do.
+ */
+#pragma clang fp contract(fast)
+#include <hip/hip_runtime.h>
+
+#include "shx_device.h"
+#include "shx_kernels.h"
+#include "shx_math.h"
+#include "shx_dev.h"
+
+namespace shx {
+
+/* The element force.  Per element: gather u_pred over the patch, form the
+ * deformed metric (compute_deformed_metrics, ShellSystem.C:1407-1460),
+ * membrane and bending strains, orthotropic H, stress resultants and the
+ * p-vectors (subtract_internal_force, ShellSystem.C:1164-1233), and the
+ * element's generalised stresses G; the force on patch slot i is
+ *   f_i = -(phi1 G1 + phi2 G2 - phi11 G3 - phi22 G4 - phi12 G5) + p phi G0,
+ * the reference's f -= JxW (Bm^T n + Bb^T m) regrouped by basis derivative
+ * (ShellSystem.C:1236-1282) plus the LOCAL pressure load (ShellSystem.C:
+ * 1570-1576), G0 = JxW a3.
+ *
+ * A block covers kElemPerBlock elements, one per thread.  It first stages
+ * its patch nodes (bn_ids: u_pred, u, X) in LDS with coalesced loads, so
+ * the patch gather reads LDS only; after the stress phase the LDS holds each
+ * element's G and R, and one thread per block node sums the block's slot
+ * contributions to that node (its entries in fixed order) into the node's
+ * block record: sum q_i and sum phi_i G0.  The node kernel adds a node's
+ * records (one to three), so no contribution is written per (element, slot)
+ * and no atomics are used.  Block partials (energies, volume terms, max
+ * |du|^2) go to epart for k_fin_elem.  The blocks of a shell whose contact
+ * list was rebuilt at the end of the previous step clear its cell buckets.
+ * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
+ * active shells, set by the host). */
+__global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
+{
+  __shared__ double s_rb[4 * 72];
+  __shared__ double s_w[5][kElemBlock / 64];
+  __shared__ double s_buf[kElemLds];     /* staged nodes, then per element a3, its scales, G1, G2 and R */
+  __shared__ int s_n[kElemPerBlock];
+  const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
+  const int sid = uni(P.eshell[blk * kElemPerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
+  if (sh.bclear && C.contact) clear_buckets(P, C, blk - C.blkE, C.nbE);
+  if (sh.done) return;
+  load_rawb(P.rawb, s_rb);
+  {
+  const int nE = P.nE;
+  const int t = threadIdx.x;
+  const int e = blk * kElemPerBlock + t;
+  const bool valid = e - C.ebase < C.ne;
+  const bool predict = (mode == M_STEP);
+  const double dt = sh.dt, a8 = sh.nm[8];
+  const int par = sh.par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* Aa = P.A[par];
+  const double* T = P.tab;
+  const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int n = P.bn_ids[b0 + j];
+    const v3 u = ld3(U, n);
+    const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
+    const v3 x = ld3(P.X, n);
+    double* sn = s_buf + kNodeStage * j;
+    sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
+    sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
+    sn[6] = x.x; sn[7] = x.y; sn[8] = x.z;
+  }
+  __syncthreads();
+  double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
+  double R00 = 0, R01 = 0, R10 = 0, R11 = 0;
+  int N = 6;
+  if (valid) {
+    R00 = T[(size_t)T_R00 * nE + e]; R01 = T[(size_t)T_R01 * nE + e];
+    R10 = T[(size_t)T_R10 * nE + e]; R11 = T[(size_t)T_R11 * nE + e];
+    N = P.vcls[e];
+    const double* rb = s_rb + 72 * (N - 3);
+    v3 xa = du1, xb = du1, xc = du1;
+#pragma unroll 2
+    for (int i = 0; i < 12; ++i) {
+      const double* sn = s_buf + kNodeStage * P.lidx[(size_t)i * nE + e];
+      const v3 up = mk(sn[0], sn[1], sn[2]);
+      const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
+      axpy(du1, up, b.d1);
+      axpy(du2, up, b.d2);
+      axpy(du11, up, b.d11);
+      axpy(du22, up, b.d22);
+      axpy(du12, up, b.d12);
+      if (i == 0 || i == 1 || i == N) {
+        const v3 u = mk(sn[3], sn[4], sn[5]);
+        if (predict) {
+          const v3 d = sub(up, u);
+          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+        }
+        const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
+        if (i == 0) xa = x;
+        else if (i == 1) xb = x;
+        else xc = x;
+      }
+    }
+    /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
+    det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
+          xc.x * (xa.y * xb.z - xb.y * xa.z);
+  }
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes G and R */
+  if (valid) {
+    const v3 a1 = add(ldrow3(T, T_A1, nE, e), du1);
+    const v3 a2 = add(ldrow3(T, T_A2, nE, e), du2);
+    v3 a3 = crs(a1, a2);
+    const double jac = nrm(a3);
+    a3 = scl(a3, 1 / jac);
+    const v3 a11 = add(ldrow3(T, T_A11, nE, e), du11);
+    const v3 a22 = add(ldrow3(T, T_A22, nE, e), du22);
+    const v3 a12 = add(ldrow3(T, T_A12, nE, e), du12);
+    const double ff1[3] = {dot(a1, a1), dot(a2, a2), dot(a1, a2)};
+    const double ff2[3] = {dot(a11, a3), dot(a22, a3), dot(a12, a3)};
+    const v3 g = ldrow3(T, T_FF1, nE, e);
+    const v3 fr2 = ldrow3(T, T_FF2, nE, e);
+    double alpha[3] = {0.5 * (ff1[0] - g.x), 0.5 * (ff1[1] - g.y), 0.5 * (ff1[2] - g.z)};
+    double beta[3] = {fr2.x - ff2[0], fr2.y - ff2[1], fr2.z - ff2[2]};
+    alpha[2] *= 2;
+    beta[2] *= 2;
+    /* orthotropic H, ShellSystem.C:1199-1213 */
+    double H[9];
+    H[0] = C.E1 * g.y * g.y;
+    H[1] = H[3] = C.E1 * (C.nu * g.y * g.x + (1 - C.nu) * g.z * g.z);
+    H[2] = H[6] = C.E1 * -g.y * g.z;
+    H[4] = C.E2 * g.x * g.x;
+    H[5] = H[7] = C.E2 * -g.x * g.z;
+    H[8] = C.G12 * ((1 - C.nu * C.nu2) * g.y * g.x + (1 - C.nu * C.nu2) * ((1 + C.nu) / (1 - C.nu)) * g.z * g.z);
+    const double invdet = 1 / (g.x * g.y - g.z * g.z);
+    for (int q = 0; q < 9; ++q) H[q] *= invdet * invdet;
+    double nn[3], mm[3];
+    for (int r = 0; r < 3; ++r) {
+      nn[r] = C.K * (H[r * 3] * alpha[0] + H[r * 3 + 1] * alpha[1] + H[r * 3 + 2] * alpha[2]);
+      mm[r] = C.D * (H[r * 3] * beta[0] + H[r * 3 + 1] * beta[1] + H[r * 3 + 2] * beta[2]);
+    }
+    const v3 a2xa3 = crs(a2, a3), a3xa1 = crs(a3, a1);
+    const double invjac = 1 / jac;
+    const v3 p11 = scl(add(crs(a11, a2), scl(a2xa3, ff2[0])), invjac);
+    const v3 p12 = scl(add(crs(a22, a2), scl(a2xa3, ff2[1])), invjac);
+    const v3 p13 = scl(add(crs(a12, a2), scl(a2xa3, ff2[2])), invjac);
+    const v3 p21 = scl(add(crs(a1, a11), scl(a3xa1, ff2[0])), invjac);
+    const v3 p22 = scl(add(crs(a1, a22), scl(a3xa1, ff2[1])), invjac);
+    const v3 p23 = scl(add(crs(a1, a12), scl(a3xa1, ff2[2])), invjac);
+    const double JxW = T[(size_t)T_JXW * nE + e];
+    const double m2x2 = 2 * mm[2];
+    v3 g1 = add(add(add(add(scl(a1, nn[0]), scl(a2, nn[2])), scl(p11, mm[0])), scl(p12, mm[1])), scl(p13, m2x2));
+    v3 g2 = add(add(add(add(scl(a2, nn[1]), scl(a1, nn[2])), scl(p21, mm[0])), scl(p22, mm[1])), scl(p23, m2x2));
+    const v3 G[6] = {scl(a3, JxW), scl(g1, JxW), scl(g2, JxW), scl(a3, JxW * mm[0]), scl(a3, JxW * mm[1]),
+                     scl(a3, JxW * m2x2)};
+    /* G0 and G3..G5 are a3 scaled: kept as a3 and the four scales */
+    double* gs = s_buf + kGRec * t;
+    gs[0] = a3.x; gs[1] = a3.y; gs[2] = a3.z;
+    gs[3] = JxW; gs[4] = JxW * mm[0]; gs[5] = JxW * mm[1]; gs[6] = JxW * m2x2;
+    gs[7] = G[1].x; gs[8] = G[1].y; gs[9] = G[1].z; gs[10] = G[2].x; gs[11] = G[2].y; gs[12] = G[2].z;
+    gs[13] = R00; gs[14] = R01; gs[15] = R10; gs[16] = R11;
+    s_n[t] = N;
+    Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
+    Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
+    const double new_area = 0.5 * jac;
+    oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
+    P.area[e] = new_area;
+  }
+  __syncthreads();
+  /* block records: per block node, its slot contributions summed in entry
+   * order (q_i for f_i -= q_i, phi_i G0 for f_i += p phi_i G0) */
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int r = b0 + j;
+    v3 qs = mk(0, 0, 0), ps = qs;
+    for (int k = P.rent_start[r]; k < P.rent_start[r + 1]; ++k) {
+      const int ent = P.rent[k], le = ent >> 4, i = ent & 15;
+      const double* gs = s_buf + kGRec * le;
+      const Basis6 b = basis_at(s_rb + 72 * (s_n[le] - 3), i, gs[13], gs[14], gs[15], gs[16]);
+      const v3 a3 = mk(gs[0], gs[1], gs[2]);
+      const v3 G1 = mk(gs[7], gs[8], gs[9]), G2 = mk(gs[10], gs[11], gs[12]);
+      const v3 q = sub(sub(sub(add(scl(G1, b.d1), scl(G2, b.d2)), scl(scl(a3, gs[4]), b.d11)),
+                           scl(scl(a3, gs[5]), b.d22)),
+                       scl(scl(a3, gs[6]), b.d12));
+      qs = add(qs, q);
+      axpy(ps, scl(a3, gs[3]), b.phi);
+    }
+    double* rec = P.brec + (size_t)kBRec * r;
+    rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
+  }
+  /* block partial: wave butterflies, then the block's waves in order */
+  double rr[5] = {Et, Eb, dumax, det, oarea};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double o = __shfl_xor(rr[c], off);
+      rr[c] = c == 2 ? fmax(rr[c], o) : rr[c] + o;
+    }
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s_w[c][t / 64] = rr[c];
+  __syncthreads();
+  if (t < 5) {
+    double acc = s_w[t][0];
+    for (int w = 1; w < kElemBlock / 64; ++w) acc = t == 2 ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
+    P.epart[(size_t)blk * 5 + t] = acc;
+  }
+  }
+}
+
+/* FP32 split element force: the same method as k_elem, with the metric and
+ * curvature changes formed in FP32 from the displacement gradients relative
+ * to the reference surface, so that no difference of nearly equal numbers
+ * is rounded:
+ *   a_a.a_b - A_a.A_b = A_a.du_b + du_a.A_b + du_a.du_b,
+ *   n = a1 x a2 = N + dn,  dn = A1 x du2 + du1 x A2 + du1 x du2,
+ *   |n| - |N| = (2 N.dn + dn.dn) / (|n| + |N|),
+ *   a3 - A3 = dn / |n| - N (|n| - |N|) / (|n| |N|),
+ *   ff2_ref - ff2 = -(A_ab.(a3 - A3) + du_ab.a3).
+ * The displacements enter the FP32 gradient sums relative to the patch's
+ * first node (formed in FP64).  The predictor, the element vertex positions
+ * of the enclosed volume and all reductions stay FP64.  Slot contributions
+ * and JxW a3 enter the block records in FP32 (brec32). */
+__global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
+{
+  __shared__ float s_rb[4 * 72];
+  __shared__ double s_w[5][kElemBlock / 64];
+  __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element a3, its scales, G1, G2, R (FP32) */
+  __shared__ int s_n[kElemPerBlock];
+  const int blk = uni(P.bmap[V_ELEM32][blockIdx.x]);
+  const int sid = uni(P.eshell[blk * kElemPerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
+  if (sh.bclear && C.contact) clear_buckets(P, C, blk - C.blkE, C.nbE);
+  if (sh.done) return;
+  for (int q = threadIdx.x; q < 4 * 72; q += blockDim.x) s_rb[q] = (float)P.rawb[q];
+  const int nE = P.nE;
+  const int t = threadIdx.x;
+  const int e = blk * kElemPerBlock + t;
+  const bool valid = e - C.ebase < C.ne;
+  const bool predict = (mode == M_STEP);
+  const double dt = sh.dt, a8 = sh.nm[8];
+  const int par = sh.par;
+  const double* U = P.U[par];
+  const double* V = P.V[par];
+  const double* Aa = P.A[par];
+  const float* T = P.tab32;
+  const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int n = P.bn_ids[b0 + j];
+    const v3 u = ld3(U, n);
+    const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
+    const v3 x = ld3(P.X, n);
+    double* sn = s_buf + kNodeStage * j;
+    sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
+    sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
+    sn[6] = x.x; sn[7] = x.y; sn[8] = x.z;
+  }
+  __syncthreads();
+  double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  v3f du1 = mkf(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
+  float R00 = 0, R01 = 0, R10 = 0, R11 = 0;
+  int N = 6;
+  if (valid) {
+    R00 = T[(size_t)F_R00 * nE + e]; R01 = T[(size_t)F_R01 * nE + e];
+    R10 = T[(size_t)F_R10 * nE + e]; R11 = T[(size_t)F_R11 * nE + e];
+    N = P.vcls[e];
+    const float* rb = s_rb + 72 * (N - 3);
+    /* displacements relative to the patch's first node, formed in FP64 and
+     * then rounded: the gradient sums are unchanged (the basis derivatives
+     * sum to zero over the patch) and the rounding acts on |u_i - u_0| */
+    const double* s0 = s_buf + kNodeStage * P.lidx[e];
+    const v3 u0 = mk(s0[0], s0[1], s0[2]);
+    v3 xa = mk(0, 0, 0), xb = xa, xc = xa;
+#pragma unroll 2
+    for (int i = 0; i < 12; ++i) {
+      const double* sn = s_buf + kNodeStage * P.lidx[(size_t)i * nE + e];
+      const v3 up = mk(sn[0], sn[1], sn[2]);
+      const v3 ur = sub(up, u0);
+      const v3f uf = mkf((float)ur.x, (float)ur.y, (float)ur.z);
+      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+      axpyf(du1, uf, dx * R00 + de * R10);
+      axpyf(du2, uf, dx * R01 + de * R11);
+      axpyf(du11, uf, dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10);
+      axpyf(du22, uf, dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11);
+      axpyf(du12, uf, dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11);
+      if (i == 0 || i == 1 || i == N) {
+        const v3 u = mk(sn[3], sn[4], sn[5]);
+        if (predict) {
+          const v3 d = sub(up, u);
+          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+        }
+        const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
+        if (i == 0) xa = x;
+        else if (i == 1) xb = x;
+        else xc = x;
+      }
+    }
+    det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
+          xc.x * (xa.y * xb.z - xb.y * xa.z);
+  }
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes G and R */
+  float* s_g = reinterpret_cast<float*>(s_buf);
+  if (valid) {
+    const v3f A1 = ldrow3f(T, F_A1, nE, e), A2 = ldrow3f(T, F_A2, nE, e);
+    const v3f A11 = ldrow3f(T, F_A11, nE, e), A22 = ldrow3f(T, F_A22, nE, e), A12 = ldrow3f(T, F_A12, nE, e);
+    const v3f A3 = ldrow3f(T, F_A3, nE, e);
+    const float J = T[(size_t)F_J * nE + e];
+    const v3f Nn = sclf(A3, J);
+    const v3f a1 = addf(A1, du1), a2 = addf(A2, du2);
+    /* membrane strain from the differences, alpha_3 doubled */
+    float alpha[3] = {dotf(A1, du1) + 0.5f * dotf(du1, du1), dotf(A2, du2) + 0.5f * dotf(du2, du2),
+                      dotf(A1, du2) + dotf(du1, A2) + dotf(du1, du2)};
+    const v3f dn = addf(addf(crsf(A1, du2), crsf(du1, A2)), crsf(du1, du2));
+    const v3f nvec = addf(Nn, dn);
+    const float jac = sqrtf(dotf(nvec, nvec));
+    const float djac = (2 * dotf(Nn, dn) + dotf(dn, dn)) / (jac + J);
+    const v3f d3 = subf(sclf(dn, 1 / jac), sclf(Nn, djac / (jac * J)));
+    const v3f a3 = addf(A3, d3);
+    const v3f a11 = addf(A11, du11), a22 = addf(A22, du22), a12 = addf(A12, du12);
+    /* bending strain ff2_ref - ff2, beta_3 doubled */
+    float beta[3] = {-(dotf(A11, d3) + dotf(du11, a3)), -(dotf(A22, d3) + dotf(du22, a3)),
+                     -2 * (dotf(A12, d3) + dotf(du12, a3))};
+    const float ff2[3] = {dotf(a11, a3), dotf(a22, a3), dotf(a12, a3)};
+    const float* Hs = T + (size_t)F_H * nE + e;
+    const float H00 = Hs[0], H01 = Hs[(size_t)nE], H02 = Hs[(size_t)2 * nE], H11 = Hs[(size_t)3 * nE],
+                H12 = Hs[(size_t)4 * nE], H22 = Hs[(size_t)5 * nE];
+    const float Hm[9] = {H00, H01, H02, H01, H11, H12, H02, H12, H22};
+    const float K = (float)C.K, D = (float)C.D;
+    float nn[3], mm[3];
+    for (int r = 0; r < 3; ++r) {
+      nn[r] = K * (Hm[r * 3] * alpha[0] + Hm[r * 3 + 1] * alpha[1] + Hm[r * 3 + 2] * alpha[2]);
+      mm[r] = D * (Hm[r * 3] * beta[0] + Hm[r * 3 + 1] * beta[1] + Hm[r * 3 + 2] * beta[2]);
+    }
+    const v3f a2xa3 = crsf(a2, a3), a3xa1 = crsf(a3, a1);
+    const float invjac = 1 / jac;
+    const v3f p11 = sclf(addf(crsf(a11, a2), sclf(a2xa3, ff2[0])), invjac);
+    const v3f p12 = sclf(addf(crsf(a22, a2), sclf(a2xa3, ff2[1])), invjac);
+    const v3f p13 = sclf(addf(crsf(a12, a2), sclf(a2xa3, ff2[2])), invjac);
+    const v3f p21 = sclf(addf(crsf(a1, a11), sclf(a3xa1, ff2[0])), invjac);
+    const v3f p22 = sclf(addf(crsf(a1, a22), sclf(a3xa1, ff2[1])), invjac);
+    const v3f p23 = sclf(addf(crsf(a1, a12), sclf(a3xa1, ff2[2])), invjac);
+    const float JxW = T[(size_t)F_JXW * nE + e];
+    const float m2x2 = 2 * mm[2];
+    const v3f g1 = addf(addf(addf(addf(sclf(a1, nn[0]), sclf(a2, nn[2])), sclf(p11, mm[0])), sclf(p12, mm[1])),
+                        sclf(p13, m2x2));
+    const v3f g2 = addf(addf(addf(addf(sclf(a2, nn[1]), sclf(a1, nn[2])), sclf(p21, mm[0])), sclf(p22, mm[1])),
+                        sclf(p23, m2x2));
+    const v3f G[6] = {sclf(a3, JxW), sclf(g1, JxW), sclf(g2, JxW), sclf(a3, JxW * mm[0]), sclf(a3, JxW * mm[1]),
+                      sclf(a3, JxW * m2x2)};
+    float* gs = s_g + kGRec * t;
+    gs[0] = a3.x; gs[1] = a3.y; gs[2] = a3.z;
+    gs[3] = JxW; gs[4] = JxW * mm[0]; gs[5] = JxW * mm[1]; gs[6] = JxW * m2x2;
+    gs[7] = G[1].x; gs[8] = G[1].y; gs[9] = G[1].z; gs[10] = G[2].x; gs[11] = G[2].y; gs[12] = G[2].z;
+    gs[13] = R00; gs[14] = R01; gs[15] = R10; gs[16] = R11;
+    s_n[t] = N;
+    Et = 0.5 * (double)JxW * ((double)nn[0] * alpha[0] + (double)nn[1] * alpha[1] + (double)nn[2] * alpha[2]);
+    Eb = 0.5 * (double)JxW * ((double)mm[0] * beta[0] + (double)mm[1] * beta[1] + (double)mm[2] * beta[2]);
+    const double new_area = 0.5 * (double)jac;
+    oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
+    P.area[e] = new_area;
+  }
+  __syncthreads();
+  for (int j = t; j < nbn; j += kElemBlock) {
+    const int r = b0 + j;
+    v3f qs = mkf(0, 0, 0), ps = qs;
+    for (int k = P.rent_start[r]; k < P.rent_start[r + 1]; ++k) {
+      const int ent = P.rent[k], le = ent >> 4, i = ent & 15;
+      const float* gs = s_g + kGRec * le;
+      const float* rb = s_rb + 72 * (s_n[le] - 3);
+      const float R00 = gs[13], R01 = gs[14], R10 = gs[15], R11 = gs[16];
+      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+      const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
+      const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
+      const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
+      const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
+      const v3f a3 = mkf(gs[0], gs[1], gs[2]);
+      const v3f G1 = mkf(gs[7], gs[8], gs[9]), G2 = mkf(gs[10], gs[11], gs[12]);
+      const v3f q = subf(subf(subf(addf(sclf(G1, b1), sclf(G2, b2)), sclf(sclf(a3, gs[4]), b11)),
+                              sclf(sclf(a3, gs[5]), b22)),
+                         sclf(sclf(a3, gs[6]), b12));
+      qs = addf(qs, q);
+      axpyf(ps, sclf(a3, gs[3]), rb[i]);
+    }
+    float* rec = P.brec32 + (size_t)kBRec * r;
+    rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
+  }
+  double rr[5] = {Et, Eb, dumax, det, oarea};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double o = __shfl_xor(rr[c], off);
+      rr[c] = c == 2 ? fmax(rr[c], o) : rr[c] + o;
+    }
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s_w[c][t / 64] = rr[c];
+  __syncthreads();
+  if (t < 5) {
+    double acc = s_w[t][0];
+    for (int w = 1; w < kElemBlock / 64; ++w) acc = t == 2 ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
+    P.epart[(size_t)blk * 5 + t] = acc;
+  }
+}
+
+} // namespace shx
