@@ -1,24 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark: post-buckled equilibria per second of the explicit
-force-and-step path on one MI355X, with the element-force kernel's HBM
-roofline and the CPU oracle timed on the host beside it.
+force-and-step path on one MI355X, surrogate warm start vs plain start,
+with the element-force kernel's HBM roofline and the CPU oracle timed on the
+host beside it.
 
-Default workload (--mode batch): the batched sweep, 512 independent shells
-(levels 3/4/5 icospheres in ratio 2:2:1, 205/205/102 shells, 3.40 M
-elements), each with seeded E2/E1 ~ U[0.2,1], h/R ~ logU[0.005,0.02],
-mu ~ U[0,0.5], its own radial imperfection (0.01 h) and deflation to a seeded
-V/V0 ~ U[0.85,0.92] (deep in the post-buckled range), self-contact with
-friction, mass damping 0.05, dt = 0.5 x its own CFL bound.  A benchmark
-"step" is one solve of the whole batch: every shell relaxed by the damped
-explicit method until its residual of the deformation
-||f - M a_rigid|| / ||f_ext|| <= 1e-6 (or --max-steps).  value = converged
-shells / wall seconds of the solve; inputs (meshes, tables, state) are
-resident in HBM before the timed region.
+Default workload (--mode surrogate): 1024 level-4 shells (5.24 M elements)
+of the warm-start family (seeded E2/E1, nu12, h/R, V/V0, mu and
+imperfection amplitude; shell_amd/warmstart.py), unseen by the surrogate.
+The MLP (6-256-256-256-64, the library's BF16 matrix-core kernels) is
+trained in this run on the committed equilibria this solver generated
+(shell_amd/data/warm_L4_4096.npz); its predicted displacement seeds the
+damped explicit relaxation, which judges it (quench, probe, plain-start
+fallback).  A benchmark "step" is one solve of the whole batch: every shell
+relaxed until its residual of the deformation ||f - M a_rigid|| / ||f_ext||
+<= 1e-6 (or --max-steps).  value = converged shells / wall seconds of the
+warm-started solve (prediction included, training excluded); the same batch
+solved from the plain start is reported beside it.  Inputs (meshes, tables)
+are resident in HBM before the timed region.
 
---mode single: one level-4 shell (E2/E1 = 0.5, h = 0.01, V/V0 = 0.9), the
-same method, one shell per launch.
+--mode batch: the 512-shell sweep (levels 3/4/5 in ratio 2:2:1, seeded
+E2/E1, h/R, mu, imperfections and deflation), plain start.
+--mode single: one level-4 shell, plain start, one shell per launch.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode batch|single]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode surrogate|batch|single]
 """
 import argparse
 import json
@@ -35,6 +39,10 @@ import shell_amd as S  # noqa: E402
 from shell_amd.workloads import batch_sweep, icosphere, radial_imperfection  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per element-kernel launch, from rocprofv3 PMC counters
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain copy; median
+# KiB per dispatch) on the same workload: profiles/hbm_traffic_pmc.md
+MEASURED_TRAFFIC = {"surrogate": (851310.9 * 2.0 + 238296.1) * 1024}
 BASELINE_METRIC = "post-buckled equilibria/s at rel residual 1e-6, surrogate vs plain; force GB/s"
 METRIC = "post-buckled equilibria/s at rel residual 1e-6 (plain FP64 GPU method); force GB/s"
 
@@ -93,7 +101,7 @@ def with_cfl(batch, params, iters):
     return [dict(p, dt=1.0 / w, dt_max=1.0 / w) for p, w in zip(params, omega)]
 
 
-def roofline(b, meshes, steps=200):
+def roofline(b, meshes, steps=200, traffic=None):
     """element-force kernel: algorithmic bytes / average launch time, HIP
     events on the library's stream, over `steps` steps from the initial state"""
     z = np.zeros(3 * int(b.nv.sum()))
@@ -107,7 +115,7 @@ def roofline(b, meshes, steps=200):
     nbytes = sum(element_kernel_bytes(T, len(X)) for X, T in meshes)
     achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_elem (element force)",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_elem (element force)",
             "bytes_per_launch": nbytes, "avg_launch_ms": elem_ms,
             "kernel_ms": {k: v[0] for k, v in kt.items()}}
 
@@ -328,7 +336,7 @@ def run_surrogate(args):
     wall_w = float(np.mean(walls_w))
     value = conv_w / sum(walls_w)
     plain = conv_p / wall_p
-    rl = roofline(b, meshes)
+    rl = roofline(b, meshes, traffic=MEASURED_TRAFFIC["surrogate"] if b.S == 1024 and args.seed == 12 else None)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline([4], [(4, s) for s in steps_p], {4: b_params_first(meshes, P)}, args.cpu_seconds)

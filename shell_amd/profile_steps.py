@@ -1,9 +1,10 @@
 """Run a fixed number of explicit step attempts of the benchmark workloads,
-for profilers (rocprofv3 --kernel-trace / --pmc): the level-4 single shell
-or the seeded batched sweep, FP64 or FP32 split, attempts launched one by
-one (no graph).
+for profilers (rocprofv3 --kernel-trace / --pmc): the level-4 single shell,
+the seeded batched sweep, or the 1024 level-4 shells of the warm-start test
+set (plain start, dt = 0.5 x their CFL bound), FP64 or FP32 split, attempts
+launched one by one (no graph).
 
-usage: python -m shell_amd.profile_steps single|batch STEPS [fp32]
+usage: python -m shell_amd.profile_steps single|batch|surrogate STEPS [fp32]
 """
 import sys
 
@@ -27,14 +28,19 @@ def main():
                  cell_margin=0.1, bulk_modulus=0.05, volume_final_ratio=0.9, volume_ramp_time=5000 * DT[4],
                  volume_at_prediction=1, dt=DT[4], dt_max=DT[4])
         meshes, params = [(X, T)], [p]
+    elif what == "surrogate":
+        from shell_amd import warmstart as W
+        b, meshes = W.build(W.draw_params(1024, 12))
+        params = None
     else:
         shells = batch_sweep(512, seed=7)
         meshes = [(s["X"], s["T"]) for s in shells]
         params = [dict(s["params"], dt=DT[s["level"]], dt_max=DT[s["level"]]) for s in shells]
-    b = S.ShellBatch(meshes, params)
+    if params is not None:
+        b = S.ShellBatch(meshes, params)
     b.set_graphs(False)
-    for i, (X, _) in enumerate(meshes):
-        b.set_state(i, np.zeros(X.size))
+    z = np.zeros(3 * int(b.nv.sum()))
+    b.set_state(S.ALL_SHELLS, z, v=z, a=z, t=0.0)
     b.apply_initial_conditions(precision=prec)
     reps = b.advance(steps, precision=prec)
     print(f"{what}: {steps} steps, {sum(r['n_rebuilds'] for r in reps)} rebuilds, "
