@@ -91,7 +91,8 @@ def single_workload(level=4):
     X, T = icosphere(level)
     X = radial_imperfection(X, 0.01 * h, seed=2024)
     params = dict(E1=E1, E2=E2, nu=nu, G12=G12, thickness=h, rho=1.0, damping_alpha=0.05,
-                  contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0, cell_margin=0.1)
+                  contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0, cell_margin=0.1,
+                  rebuild_trigger=1)
     return X, T, params
 
 
@@ -349,7 +350,8 @@ def run_surrogate(args):
         "config": {"workload": f"learned warm start on {b.S} level-4 shells: E2/E1~U[0.3,1], nu12~U[0.25,0.35], "
                                f"h/R~logU[0.008,0.016], V/V0~U[0.86,0.92], mu~U[0,0.5], imperfection/h~"
                                f"U[0.005,0.02]; MLP 6-256-256-256-{k} (GELU, BF16 MFMA GEMMs, FP32 accumulation) "
-                               f"-> {k} PCA coefficients of the displacement; damped relaxation (mass damping 0.05, "
+                               f"-> {k} PCA coefficients of the displacement; self-contact with friction (displacement rebuild "
+                               f"trigger); damped relaxation (mass damping 0.05, "
                                f"dt=0.5*CFL) to deformation residual {args.tol:g}; plain-start fallback on "
                                f"interpenetration or residual growth; one step = the whole batch solved",
                    "shells": b.S, "elements": int(sum(len(T) for _, T in meshes)),

@@ -81,6 +81,11 @@ typedef struct {
   int32_t volume_at_prediction;    /* ext: 1: volume pressure from the enclosed volume of the configuration
                                       the forces are evaluated at; 0: the previous step's volume, as the
                                       reference (ShellSystem.C:1097, 1476) */
+  int32_t rebuild_trigger;         /* ext: contact-list rebuild trigger.  0: the reference's, du_sum (2 x the
+                                      largest node increment, summed over the predictor and corrector checks,
+                                      ShellSystem.C:2634) >= cell_margin x cell_size (:1917-1919); 1: 2 x the
+                                      largest node displacement since the displacement the list was built at
+                                      >= the same tolerance (the bound du_sum only overestimates) */
 } shx_params;
 
 typedef struct {

@@ -42,7 +42,8 @@ class Params(ctypes.Structure):
         "static_friction_tol", "cell_size_mult", "cell_margin", "neighbor_margin", "newmark_beta",
         "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
         (n, ctypes.c_int32) for n in ("adapt_dt_every",
-        "contact", "quirk_fulltime", "quirk_volume_schedule", "omp_faithful", "volume_at_prediction")]
+        "contact", "quirk_fulltime", "quirk_volume_schedule", "omp_faithful", "volume_at_prediction",
+        "rebuild_trigger")]
 
 
 class Report(ctypes.Structure):
@@ -137,7 +138,8 @@ def default_params():
                 static_friction_tol=0.1, cell_size_mult=4.0, cell_margin=0.02, neighbor_margin=0.4,
                 newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0, rel_loc_err_min=5e-5,
                 rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0, quirk_fulltime=0,
-                quirk_volume_schedule=0, omp_faithful=0, volume_at_prediction=0)
+                quirk_volume_schedule=0, omp_faithful=0, volume_at_prediction=0,
+                rebuild_trigger=0)
 
 
 class Oracle:

@@ -289,6 +289,7 @@ struct System {
   std::vector<ElemCache> ec;
   std::vector<std::vector<int>> node_elems;
   std::vector<double> u, v, a, f, fext, Minv, C, u_pred, v_pred, a_pred;
+  std::vector<double> u_rb;   /* displacement the contact list was built at (rebuild trigger 1) */
   std::vector<Pt> p;          /* node_cache[].p */
   double t = 0, dt = 0;
   double K = 0, D = 0, stiff_density = 0, stiff_static_friction = 0, d2_friction = 0;
@@ -850,15 +851,32 @@ bool elems_are_close(const System& s, int e1, int e2)
   return std::binary_search(v.begin(), v.end(), e2);
 }
 
-/* compute_shell_shell_contact_candidates, ShellSystem.C:1910-2135 */
-void contact_candidates(System& s, bool force)
+/* compute_shell_shell_contact_candidates, ShellSystem.C:1910-2135; ucur is
+ * the displacement of the positions s.p.  Trigger 0 is the reference's
+ * (du_sum >= tolerance, :1917-1919); trigger 1 (extension) rebuilds when
+ * twice the largest node displacement since the build reaches it */
+void contact_candidates(System& s, bool force, const std::vector<double>& ucur)
 {
   const orc_params& P = s.prm;
   if (!P.contact) return;
   double cell_size = P.cell_size_mult * s.avg_elem_size;
   double tolerance = P.cell_margin * cell_size;
-  if (!(force || s.du_sum >= tolerance)) return;
+  bool due = force || (P.rebuild_trigger && s.u_rb.size() != ucur.size());
+  if (!due && P.rebuild_trigger) {
+    double dmax = 0;
+    for (int n = 0; n < s.nv; ++n) {
+      const double dx = ucur[3 * n] - s.u_rb[3 * n];
+      const double dy = ucur[3 * n + 1] - s.u_rb[3 * n + 1];
+      const double dz = ucur[3 * n + 2] - s.u_rb[3 * n + 2];
+      dmax = std::max(dmax, dx * dx + dy * dy + dz * dz);
+    }
+    due = 2 * std::sqrt(dmax) >= tolerance;
+  } else if (!due) {
+    due = s.du_sum >= tolerance;
+  }
+  if (!due) return;
   s.du_sum = 0;
+  s.u_rb = ucur;
   ++s.n_rebuilds;
   compute_bounding_box(s);
   for (int q = 0; q < 3; ++q)
@@ -1073,7 +1091,7 @@ void advance(System& s, orc_report* rep)
     check_increment(s, false);
     compute_position(s, s.u_pred);
     for (int i = 0; i < n3; ++i) s.v_pred[i] = A[6] * s.a[i] + s.v[i];
-    contact_candidates(s, false);
+    contact_candidates(s, false, s.u_pred);
     evaluate_forces(s, s.t);
     fnorm = l2(s.f);
     fext_norm = l2(s.fext);
@@ -1108,7 +1126,7 @@ void advance(System& s, orc_report* rep)
       for (int i = 0; i < n3; ++i) s.v_pred[i] = A[7] * s.a_pred[i] + s.v_pred[i];
       s.v.swap(s.v_pred);
       s.a.swap(s.a_pred);
-      contact_candidates(s, false);
+      contact_candidates(s, false, s.u);
       compute_enclosed_volume(s);
       set_newmark(s, new_dt);
       break;
@@ -1299,7 +1317,7 @@ int orc_init_dynamics(void* h)
   for (int e = 0; e < s.ne; ++e) deformed_metrics_elem(s, e, s.u_pred.data());
   compute_enclosed_volume(s);
   s.volume0 = s.volume;
-  contact_candidates(s, true);
+  contact_candidates(s, true, s.u_pred);
   evaluate_forces(s, s.t + s.dt);
   for (int i = 0; i < 3 * s.nv; ++i) s.a[i] = s.Minv[i] * s.f[i];
   return 0;
@@ -1316,7 +1334,7 @@ int orc_contact_rebuild(void* h, int force)
 {
   System& s = *static_cast<System*>(h);
   compute_position(s, s.u);
-  contact_candidates(s, force != 0);
+  contact_candidates(s, force != 0, s.u);
   return (int)s.pairs.size();
 }
 

@@ -48,6 +48,8 @@ typedef struct {
   int32_t omp_faithful;            /* 1: OpenMP loops with critical assembly (CPU baseline timing) */
   int32_t volume_at_prediction;    /* ext: 1: enclosed volume of the force-evaluation configuration
                                       instead of the previous step's (ShellSystem.C:1097, 2505) */
+  int32_t rebuild_trigger;         /* ext: 0: du_sum >= tolerance (ShellSystem.C:1917-1919); 1: 2 x the
+                                      largest node displacement since the list's build >= tolerance */
 } orc_params;
 
 typedef struct {

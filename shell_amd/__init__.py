@@ -51,7 +51,7 @@ class Params(ctypes.Structure):
         "static_friction_tol", "cell_size_mult", "cell_margin", "neighbor_margin", "newmark_beta",
         "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
         (n, ctypes.c_int32) for n in ("adapt_dt_every", "contact", "quirk_fulltime",
-                                      "quirk_volume_schedule", "volume_at_prediction")]
+                                      "quirk_volume_schedule", "volume_at_prediction", "rebuild_trigger")]
 
 
 class Report(ctypes.Structure):
@@ -71,7 +71,8 @@ DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_
                 static_friction_tol=0.1, cell_size_mult=4.0, cell_margin=0.02, neighbor_margin=0.4,
                 newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0,
                 rel_loc_err_min=5e-5, rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0,
-                quirk_fulltime=0, quirk_volume_schedule=0, volume_at_prediction=0)
+                quirk_fulltime=0, quirk_volume_schedule=0, volume_at_prediction=0,
+                rebuild_trigger=0)
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)

@@ -191,6 +191,7 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   prof_end(ctx);
   prof_begin(ctx, 8);
   k_bp_pairs<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  k_bp_exact<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
   prof_end(ctx);
 }
 
@@ -368,6 +369,7 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
   C.conv_tol = 0; C.load_end_time = std::max(prm->pressure_ramp_time, prm->volume_ramp_time);
   C.stop_on_converged = 0;
   C.volume_at_prediction = prm->volume_at_prediction ? 1 : 0;
+  C.rebuild_trigger = prm->rebuild_trigger ? 1 : 0;
   {
     unsigned m = (unsigned)std::ceil(std::sqrt((double)ne) / prm->cell_size_mult);
     C.n_cells_max = m * m * m;
@@ -647,8 +649,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.brec, (size_t)kBRec * std::max(nrecs, 1))) ||
       (r = dalloc(ctx, P.area, nE)) || (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) ||
       (r = dalloc(ctx, P.brec32, (size_t)kBRec * std::max(nrecs, 1))) ||
-      (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
-      (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)5 * ctx->nblkE)) ||
+      (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.urb, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
+      (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)kElemRec * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
       (r = dalloc(ctx, P.wl[0], (size_t)nshell + 1)) || (r = dalloc(ctx, P.wl[1], (size_t)nshell + 1)) ||

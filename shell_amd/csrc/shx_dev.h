@@ -41,6 +41,13 @@ __device__ __forceinline__ v3 pred3(v3 u, v3 v, v3 a, double dt, double a8)
   return mk(pred1(u.x, v.x, a.x, dt, a8), pred1(u.y, v.y, a.y, dt, a8), pred1(u.z, v.z, a.z, dt, a8));
 }
 
+/* |a|^2 with each product and sum rounded once (as the CPU oracle forms
+ * it), also in a translation unit compiled with contraction */
+__device__ __forceinline__ double nrm2_rn(v3 a)
+{
+  return __dadd_rn(__dadd_rn(__dmul_rn(a.x, a.x), __dmul_rn(a.y, a.y)), __dmul_rn(a.z, a.z));
+}
+
 /* reparametrised basis of slot i of an element (reinit_mod,
  * ShellSystem.C:3466-3478) from the raw basis rb (6 x 12) and R */
 struct Basis6 { double phi, d1, d2, d11, d22, d12; };

@@ -66,7 +66,9 @@ constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kElemPerBlock * kGRec) ?
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block, one node each */
 constexpr int kNodePerBlock = kNodeBlock;
 constexpr int kBRec = 6;            /* per block-node record: sum of slot contributions q, sum of phi JxW a3 */
-constexpr int kNodeRec = 22;        /* node-kernel partial record length */
+constexpr int kNodeRec = 23;        /* node-kernel partial record length */
+constexpr int kElemRec = 6;         /* element-kernel partial record length */
+constexpr double kTentative = -1.0;  /* d2 of a candidate pair whose exact distance is not yet formed */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
 constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
 constexpr int kBpPerBlock = 256 / kBpLanes;
@@ -100,7 +102,7 @@ struct SConst {
   double damping_alpha, damping_beta, damping_gamma;
   double conv_tol, load_end_time;
   int adapt_dt_every, contact, self_friction, quirk_fulltime, quirk_volume_schedule;
-  int stop_on_converged, volume_at_prediction;
+  int stop_on_converged, volume_at_prediction, rebuild_trigger;
   unsigned n_cells_max;
 };
 
@@ -113,6 +115,7 @@ struct DShell {
   double volume, volume0, area;
   double volume_ref;               /* > 0: V0 of the volume constraint given by the host at init (warm start) */
   double alpha_extra;              /* extra mass-proportional damping set by the host (warm-start quench) */
+  double dev_rb;                   /* largest node displacement since the list's build displacement (trigger 1) */
   double bmin[3], bmax[3];
   double cell_size, d_max, d2_max;
   double residual, fnorm, fext_norm, e_tens, e_bend, e_self, e_fric, e_kin, du_max;
@@ -171,6 +174,7 @@ struct DPtrs {
   float* brec32;              /* kBRec per record: FP32 split path counterpart of brec */
   double* area;               /* nE */
   double* fc;                 /* nV*3 contact force accumulator */
+  double* urb;                /* nV*3 displacement each node's contact list was built at (rebuild trigger 1) */
   double* fout;               /* nV*3 outputs of a plain force evaluation */
   double* fext_out;
   double* epart;              /* element kernel block partials (5 per block) */

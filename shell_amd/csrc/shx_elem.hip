@@ -36,14 +36,14 @@ namespace shx {
  * block record: sum q_i and sum phi_i G0.  The node kernel adds a node's
  * records (one to three), so no contribution is written per (element, slot)
  * and no atomics are used.  Block partials (energies, volume terms, max
- * |du|^2) go to epart for k_fin_elem.  The blocks of a shell whose contact
+ * |du|^2, max |u_pred - u_build|^2) go to epart for k_fin_elem.  The blocks of a shell whose contact
  * list was rebuilt at the end of the previous step clear its cell buckets.
  * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
  * active shells, set by the host). */
 __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
-  __shared__ double s_w[5][kElemBlock / 64];
+  __shared__ double s_w[kElemRec][kElemBlock / 64];
   __shared__ double s_buf[kElemLds];     /* staged nodes, then per element a3, its scales, G1, G2 and R */
   __shared__ int s_n[kElemPerBlock];
   const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
@@ -66,11 +66,16 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
   const double* Aa = P.A[par];
   const double* T = P.tab;
   const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
+  const bool track = predict && C.contact && C.rebuild_trigger;
+  double dev = 0;                        /* max |u_pred - u_build|^2 of the block's nodes (trigger 1) */
   for (int j = t; j < nbn; j += kElemBlock) {
     const int n = P.bn_ids[b0 + j];
     const v3 u = ld3(U, n);
     const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
     const v3 x = ld3(P.X, n);
+    if (track) {
+      dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
+    }
     double* sn = s_buf + kNodeStage * j;
     sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
     sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
@@ -100,8 +105,7 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
       if (i == 0 || i == 1 || i == N) {
         const v3 u = mk(sn[3], sn[4], sn[5]);
         if (predict) {
-          const v3 d = sub(up, u);
-          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+          dumax = fmax(dumax, nrm2_rn(sub(up, u)));
         }
         const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
         if (i == 0) xa = x;
@@ -195,22 +199,23 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
     rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
   }
   /* block partial: wave butterflies, then the block's waves in order */
-  double rr[5] = {Et, Eb, dumax, det, oarea};
+  double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < kElemRec; ++c) {
       const double o = __shfl_xor(rr[c], off);
-      rr[c] = c == 2 ? fmax(rr[c], o) : rr[c] + o;
+      rr[c] = (c == 2 || c == 5) ? fmax(rr[c], o) : rr[c] + o;
     }
   if ((t & 63) == 0)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) s_w[c][t / 64] = rr[c];
+    for (int c = 0; c < kElemRec; ++c) s_w[c][t / 64] = rr[c];
   __syncthreads();
-  if (t < 5) {
+  if (t < kElemRec) {
     double acc = s_w[t][0];
-    for (int w = 1; w < kElemBlock / 64; ++w) acc = t == 2 ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
-    P.epart[(size_t)blk * 5 + t] = acc;
+    for (int w = 1; w < kElemBlock / 64; ++w)
+      acc = (t == 2 || t == 5) ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
+    P.epart[(size_t)blk * kElemRec + t] = acc;
   }
   }
 }
@@ -231,7 +236,7 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
 __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
 {
   __shared__ float s_rb[4 * 72];
-  __shared__ double s_w[5][kElemBlock / 64];
+  __shared__ double s_w[kElemRec][kElemBlock / 64];
   __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element a3, its scales, G1, G2, R (FP32) */
   __shared__ int s_n[kElemPerBlock];
   const int blk = uni(P.bmap[V_ELEM32][blockIdx.x]);
@@ -253,11 +258,16 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   const double* Aa = P.A[par];
   const float* T = P.tab32;
   const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
+  const bool track = predict && C.contact && C.rebuild_trigger;
+  double dev = 0;                        /* max |u_pred - u_build|^2 of the block's nodes (trigger 1) */
   for (int j = t; j < nbn; j += kElemBlock) {
     const int n = P.bn_ids[b0 + j];
     const v3 u = ld3(U, n);
     const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
     const v3 x = ld3(P.X, n);
+    if (track) {
+      dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
+    }
     double* sn = s_buf + kNodeStage * j;
     sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
     sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
@@ -294,8 +304,7 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
       if (i == 0 || i == 1 || i == N) {
         const v3 u = mk(sn[3], sn[4], sn[5]);
         if (predict) {
-          const v3 d = sub(up, u);
-          dumax = fmax(dumax, d.x * d.x + d.y * d.y + d.z * d.z);
+          dumax = fmax(dumax, nrm2_rn(sub(up, u)));
         }
         const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
         if (i == 0) xa = x;
@@ -392,22 +401,23 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     float* rec = P.brec32 + (size_t)kBRec * r;
     rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
   }
-  double rr[5] = {Et, Eb, dumax, det, oarea};
+  double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < kElemRec; ++c) {
       const double o = __shfl_xor(rr[c], off);
-      rr[c] = c == 2 ? fmax(rr[c], o) : rr[c] + o;
+      rr[c] = (c == 2 || c == 5) ? fmax(rr[c], o) : rr[c] + o;
     }
   if ((t & 63) == 0)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) s_w[c][t / 64] = rr[c];
+    for (int c = 0; c < kElemRec; ++c) s_w[c][t / 64] = rr[c];
   __syncthreads();
-  if (t < 5) {
+  if (t < kElemRec) {
     double acc = s_w[t][0];
-    for (int w = 1; w < kElemBlock / 64; ++w) acc = t == 2 ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
-    P.epart[(size_t)blk * 5 + t] = acc;
+    for (int w = 1; w < kElemBlock / 64; ++w)
+      acc = (t == 2 || t == 5) ? fmax(acc, s_w[t][w]) : acc + s_w[t][w];
+    P.epart[(size_t)blk * kElemRec + t] = acc;
   }
 }
 

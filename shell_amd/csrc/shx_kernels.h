@@ -33,6 +33,7 @@ __global__ void k_bp_count(DPtrs P, int list);
 __global__ void k_bp_scan(DPtrs P, int list);
 __global__ void k_bp_scatter(DPtrs P, int list);
 __global__ void k_bp_pairs(DPtrs P, int posmode, int list);
+__global__ void k_bp_exact(DPtrs P, int posmode, int list);
 __global__ void k_bp_host_plan(DPtrs P);
 __global__ void k_bp_host_done(DPtrs P);
 

@@ -212,8 +212,9 @@ __device__ void check_increment_dev(DShell& s, const SConst& C, double du2max, b
 
 __device__ __forceinline__ bool rebuild_due(const DShell& s, const SConst& C)
 {
-  /* ShellSystem.C:1917-1919 */
-  return C.contact && (s.du_sum >= C.cell_margin * (C.cell_size_mult * s.avg_elem_size));
+  /* ShellSystem.C:1917-1919; trigger 1: the displacement since the build */
+  const double tol = C.cell_margin * (C.cell_size_mult * s.avg_elem_size);
+  return C.contact && (C.rebuild_trigger ? 2 * s.dev_rb >= tol : s.du_sum >= tol);
 }
 
 /* start of a contact-list rebuild of shell sid (one thread): the rebuild
@@ -294,9 +295,9 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   /* the corrector list of the previous attempt has been consumed */
   if (sid == 0 && threadIdx.x == 0) P.wl[1][0] = 0;
   if (w.done) return;
-  const int op[5] = {0, 0, 1, 0, 0};
-  double r[5];
-  wave_reduce_records<5>(P.epart + (size_t)C.blkE * 5, C.nbE, 5, op, r);
+  const int op[kElemRec] = {0, 0, 1, 0, 0, 1};
+  double r[kElemRec];
+  wave_reduce_records<kElemRec>(P.epart + (size_t)C.blkE * kElemRec, C.nbE, kElemRec, op, r);
   if (threadIdx.x != 0) return;
   w.att_e_tens = r[0];
   w.att_e_bend = r[1];
@@ -305,6 +306,7 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   w.bclear = 0;
   if (mode == M_STEP) {
     check_increment_dev(w, C, r[2], false);
+    w.dev_rb = sqrt(r[5]);
     w.rebuild = rebuild_due(w, C);
   } else if (mode != M_INIT) {
     w.rebuild = 0;
@@ -523,7 +525,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   if (sid == 0 && threadIdx.x == 0) P.wl[0][0] = 0;
   if (P.sh[sid].done) return;
   int on[kNodeRec];
-  for (int q = 0; q < kNodeRec; ++q) on[q] = (q >= 2 && q <= 4);
+  for (int q = 0; q < kNodeRec; ++q) on[q] = (q >= 2 && q <= 4) || q == 22;
   double rn[kNodeRec];
   wave_reduce_records<kNodeRec>(P.npart + (size_t)C.blkN * kNodeRec, C.nbN, kNodeRec, on, rn);
   const int oc[kCRec] = {0, 0, 0, 0};
@@ -568,6 +570,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   }
   if (new_dt >= dt) {
     check_increment_dev(w, C, du, true);
+    w.dev_rb = sqrt(rn[22]);
     w.par = 1 - w.par;
     w.t += dt;
     w.n_steps++;
@@ -694,12 +697,16 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
       st3(P.U[1 - par], n, un);
       st3(P.V[1 - par], n, vn);
       st3(P.A[1 - par], n, an);
+      if (C.contact && C.rebuild_trigger) {
+        const v3 g = sub(un, ld3(P.urb, n));
+        rec[22] = g.x * g.x + g.y * g.y + g.z * g.z;
+      }
     }
   }
   /* wave butterfly, then the block's waves in order */
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) {
-    const bool mx = (q >= 2 && q <= 4);
+    const bool mx = (q >= 2 && q <= 4) || q == 22;
     for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_xor(rec[q], off);
       rec[q] = mx ? fmax(rec[q], o) : rec[q] + o;
@@ -711,7 +718,7 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
     for (int q = 0; q < kNodeRec; ++q) s_wave[q][wave] = rec[q];
   __syncthreads();
   if (t < kNodeRec) {
-    const bool mx = (t >= 2 && t <= 4);
+    const bool mx = (t >= 2 && t <= 4) || t == 22;
     double acc = s_wave[t][0];
     for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
     P.npart[(size_t)blk * kNodeRec + t] = acc;
@@ -822,7 +829,11 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode, int list)
   if (e - C.ebase < C.ne) {
     bool finite = true;
     for (int k = 0; k < 3; ++k) {
-      const v3 x = node_pos(P, P.tri[(size_t)k * nE + e], U, V, A, predict, dt, a8);
+      const int n = P.tri[(size_t)k * nE + e];
+      v3 u = ld3(U, n);
+      if (predict) u = pred3(u, ld3(V, n), ld3(A, n), dt, a8);
+      if (C.rebuild_trigger) st3(P.urb, n, u);     /* the build displacement (same value from every element) */
+      const v3 x = add(ld3(P.X, n), u);
       const double xx[3] = {x.x, x.y, x.z};
       finite = finite && isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
       for (int q = 0; q < 3; ++q) {
@@ -1108,12 +1119,6 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
     const int nE = P.nE;
     const int e1 = C.ebase + (vb - s_pref[li]) * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
     if (e1 - C.ebase >= C.ne) continue;
-    const bool predict = (posmode == M_STEP);
-    const double dt = sh->dt, a8 = sh->nm[8];
-    const int par = sh->par;
-    const double* U = P.U[par];
-    const double* V = P.V[par];
-    const double* A = P.A[par];
     const double dmx = sh->d_max, d2mx = sh->d2_max;
     const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
     const int ob = sh->pair_src, nb = sh->pair_buf;
@@ -1139,8 +1144,6 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
     const int ncells = (hi1[0] - lo1[0] + 1) * ny * nz;
     const int lpc = max(1, kBpLanes / ncells), groups = kBpLanes / lpc;
     const int sub = k0 % lpc;
-    bool have_x0 = false;
-    v3 X0[3];
     for (int ci = k0 / lpc; ci < ncells; ci += groups) {
       int cx, cy, cz;
       bp_cell_of(ci, lo1, ny, nz, cx, cy, cz);
@@ -1184,6 +1187,59 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
             break;
           }
         if (listed) continue;
+        /* a candidate: k_bp_exact forms its distance and keeps it if d2 <= d_max^2 */
+        const int slot = atomicAdd(&P.rcnt[nb][e1], 1);
+        if (slot >= kRowCap) { atomicOr(&sh->err, ERR_ROW); continue; }
+        const size_t pn = row + slot;
+        P.pe2[nb][pn] = e2;
+        P.pd2[nb][pn] = kTentative;
+        P.pstick[nb][pn] = 0;
+        st3(P.pso[nb], pn, mk(0, 0, 0));
+        }
+      }
+    }
+  }
+}
+
+/* exact distance of the candidates k_bp_pairs appended to the listed
+ * shells' new rows (ShellShellContact::compute, ShellShellContact.C:35-145,
+ * kept if d2 <= d_max^2, ShellSystem.C:2096-2101; d2 <= 0 registers
+ * CONTACT_PROBLEM): one thread per element row, the row compacted in place
+ * in its order.  Kept apart from the pair search so that the search runs at
+ * the occupancy of its own (small) register footprint. */
+__global__ void __launch_bounds__(256) k_bp_exact(DPtrs P, int posmode, int list)
+{
+  if (P.wl[list][0] == 0) return;
+  __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  const int nl = vmap_build(P, P.wl[list], V_TILE, s_sh, s_pref, s_tmp);
+  for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
+    const int li = vmap_find(vb, nl, s_pref);
+    const int sid = uni(s_sh[li]);
+    const SConst& C = P.cs[sid];
+    DShell* sh = P.sh + sid;
+    if (!bp_active(sh)) continue;
+    const int e1 = C.ebase + (vb - s_pref[li]) * 256 + threadIdx.x;
+    if (e1 - C.ebase >= C.ne) continue;
+    const int nE = P.nE;
+    const bool predict = (posmode == M_STEP);
+    const double dt = sh->dt, a8 = sh->nm[8];
+    const int par = sh->par;
+    const double* U = P.U[par];
+    const double* V = P.V[par];
+    const double* A = P.A[par];
+    const double d2mx = sh->d2_max;
+    const int nb = sh->pair_buf;
+    const size_t row = (size_t)e1 * kRowCap;
+    const int n = P.rcnt[nb][e1];
+    if (n > kRowCap) continue;             /* the row overflowed: the shell reports ERR_ROW */
+    bool have_x0 = false;
+    v3 X0[3];
+    int w = 0;
+    for (int r = 0; r < n; ++r) {
+      const size_t p = row + r;
+      double d2 = P.pd2[nb][p];
+      if (d2 == kTentative) {
+        const int e2 = P.pe2[nb][p];
         if (!have_x0) {
           for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * nE + e1], U, V, A, predict, dt, a8);
           have_x0 = true;
@@ -1194,16 +1250,18 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
         tri_tri(X0, X1, hit);
         if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
         if (hit.d2 > d2mx) continue;
-        const int slot = atomicAdd(&P.rcnt[nb][e1], 1);
-        if (slot >= kRowCap) { atomicOr(&sh->err, ERR_ROW); continue; }
-        const size_t pn = row + slot;
-        P.pe2[nb][pn] = e2;
-        P.pd2[nb][pn] = hit.d2;
-        P.pstick[nb][pn] = 0;
-        st3(P.pso[nb], pn, mk(0, 0, 0));
-        }
+        d2 = hit.d2;
       }
+      const size_t q = row + w;
+      if (q != p) {
+        P.pe2[nb][q] = P.pe2[nb][p];
+        P.pstick[nb][q] = P.pstick[nb][p];
+        st3(P.pso[nb], q, ld3(P.pso[nb], p));
+      }
+      P.pd2[nb][q] = d2;
+      ++w;
     }
+    if (w != n) P.rcnt[nb][e1] = w;
   }
 }
 

@@ -25,7 +25,8 @@ def main():
         X, T = icosphere(4)
         X = radial_imperfection(X, 1e-4, seed=2024)
         p = dict(thickness=0.01, E2=0.5, damping_alpha=0.05, contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=4.0,
-                 cell_margin=0.1, bulk_modulus=0.05, volume_final_ratio=0.9, volume_ramp_time=5000 * DT[4],
+                 cell_margin=0.1, rebuild_trigger=1, bulk_modulus=0.05, volume_final_ratio=0.9,
+                 volume_ramp_time=5000 * DT[4],
                  volume_at_prediction=1, dt=DT[4], dt_max=DT[4])
         meshes, params = [(X, T)], [p]
     elif what == "surrogate":

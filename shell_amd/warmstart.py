@@ -4,8 +4,9 @@ Workload family: level-4 icosphere shells (2562 nodes, 5120 triangles), one
 fixed radial imperfection pattern (unit Gaussian per vertex, seed
 PATTERN_SEED; an inward Gaussian dimple is the alternative pattern) scaled
 per shell, deflation by the volume constraint to V/V0 at
-the end of a RAMP_TIME ramp, self-contact with Coulomb friction, mass damping
-0.05, dt = 0.5 x the shell's CFL bound.  Six parameters per shell (PARAMS):
+the end of a RAMP_TIME ramp, self-contact with Coulomb friction (contact
+lists rebuilt on the displacement trigger), mass damping 0.05, dt = 0.5 x
+the shell's CFL bound.  Six parameters per shell (PARAMS):
 
     E2/E1 ~ U[0.3, 1], nu12 ~ U[0.25, 0.35], h/R ~ logU[0.008, 0.016],
     V/V0 ~ U[0.86, 0.92], mu ~ U[0, 0.5], imperfection amplitude / h ~ U[0.005, 0.02]
@@ -89,7 +90,7 @@ def shells(P, kind=None):
         prms.append(dict(E1=1.0, E2=float(e2), nu=float(nu), G12=0.35, thickness=float(h), rho=1.0,
                          damping_alpha=0.05, bulk_modulus=0.05, volume_final_ratio=float(ratio),
                          volume_ramp_time=RAMP_TIME, volume_at_prediction=1, contact=1, mu_s=float(mu),
-                         mu_d=float(mu), cell_size_mult=4.0, cell_margin=0.1))
+                         mu_d=float(mu), cell_size_mult=4.0, cell_margin=0.1, rebuild_trigger=1))
     return meshes, prms
 
 

@@ -82,7 +82,8 @@ def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX):
     E2/E1 ~ U[0.2, 1], h/R ~ logU[0.005, 0.02], mu ~ U[0, 0.5], Gaussian
     radial imperfection 0.01 h with its own seed, and deflation (bulk modulus
     0.05, enclosed volume of the force configuration) to V/V0 ~ U[0.85, 0.92]
-    over SWEEP_RAMP_TIME, self-contact with friction, mass damping 0.05.
+    over SWEEP_RAMP_TIME, self-contact with friction (contact lists rebuilt on
+    the displacement trigger), mass damping 0.05.
     Returns a list of dicts with keys level, X, T, params (dt left to the
     caller: it depends on the CFL bound of each shell)."""
     rng = np.random.default_rng(seed)
@@ -100,7 +101,7 @@ def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX):
         params = dict(E1=1.0, E2=e2, nu=0.3, G12=0.35, thickness=h, rho=1.0, damping_alpha=0.05,
                       bulk_modulus=0.05, volume_final_ratio=ratio, volume_ramp_time=SWEEP_RAMP_TIME,
                       volume_at_prediction=1, contact=1, mu_s=mu, mu_d=mu, cell_size_mult=4.0,
-                      cell_margin=0.1)
+                      cell_margin=0.1, rebuild_trigger=1)
         out.append(dict(level=int(L), X=radial_imperfection(X, 0.01 * h, iseed), T=T, params=params,
                         imperfection_seed=iseed))
     return out

@@ -129,16 +129,18 @@ def test_contact_pairs_bitexact(icosphere, level, squash):
     g.close()
 
 
-def test_contact_dynamics_adaptive(icosphere):
+@pytest.mark.parametrize("trigger", [0, 1])
+def test_contact_dynamics_adaptive(icosphere, trigger):
     """Squashed L3 shell whose two sheets close at 0.04 per unit time, with
     self-contact, friction and the reference's adaptive dt: rebuild counts,
     rejected attempts and sorted pair lists identical after every chunk,
-    states within tolerance over the window."""
-    say("case contact dynamics L3")
+    states within tolerance over the window; with the reference's rebuild
+    trigger and with the displacement trigger (extension)."""
+    say("case contact dynamics L3, rebuild trigger", trigger)
     X, T = icosphere(3)
     X = imperfect(X, 0.01 * H, seed=7)
     kw = dict(dt=0.02, dt_max=0.02, damping_alpha=0.05, adapt_dt_every=1, contact=1, mu_s=0.3, mu_d=0.3,
-              cell_size_mult=1.0)
+              cell_size_mult=1.0, rebuild_trigger=trigger)
     g, o = pair(X, T, **kw)
     u = _pancake(X, 0.03)
     v = np.zeros_like(u).reshape(-1, 3)

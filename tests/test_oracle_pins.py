@@ -443,3 +443,53 @@ def test_enclosed_volume_is_the_polyhedral_volume(icosphere):
     # area: one-point quadrature of the limit surface, which lies inside the
     # control net (about 1% below 4 pi at level 3)
     assert abs(r["area"] - 4 * np.pi) < 0.02 * 4 * np.pi
+
+
+def test_displacement_trigger_is_safe_and_rebuilds_less(icosphere):
+    """rebuild trigger 1 (twice the largest node displacement since the list's
+    build reaches cell_margin x cell_size) against the reference's du_sum
+    trigger, on a squashed level-3 shell whose sheets close (no friction):
+    every pair within h of the current positions (brute force) is listed at
+    every checkpoint, the trajectories are identical (pairs beyond h carry no
+    force) and trigger 1 rebuilds less often"""
+    X, T = icosphere(3)
+    X = imperfect(X, 0.01 * H, seed=7)
+    kw = dict(dt=0.02, dt_max=0.02, damping_alpha=0.05, adapt_dt_every=1, contact=1, cell_size_mult=1.0,
+              mu_s=0.0, mu_d=0.0)
+    Y = X.copy()
+    Y[:, 2] *= 0.03
+    u = (Y - X).reshape(-1)
+    v = np.zeros_like(X)
+    v[:, 2] = -0.02 * np.sign(X[:, 2])
+    runs = {}
+    for trig in (0, 1):
+        o = O.Oracle(X, T, rebuild_trigger=trig, **kw)
+        o.set_state(u, v.reshape(-1))
+        o.init_dynamics()
+        _, tri, _ = o.patches()
+        start, ids = o.close_sets()
+        close = set()
+        for e in range(len(T)):
+            for c in ids[start[e]:start[e + 1]]:
+                close.add((e, int(c)))
+        contacts = 0
+        for k in range(8):
+            r = o.step(25)
+            if trig == 1 and k % 2 == 1:
+                P = X + o.get_state()[0].reshape(-1, 3)
+                bmin, bmax = P[tri].min(1), P[tri].max(1)
+                listed = set(zip(*[a.tolist() for a in o.pairs()]))
+                for a in range(len(T)):
+                    b = np.arange(a + 1, len(T))
+                    apart = np.any((bmin[a] > bmax[b] + H) | (bmin[b] > bmax[a] + H), axis=1)
+                    for bb in b[~apart]:
+                        if (a, int(bb)) in close:
+                            continue
+                        d2 = O.tri_distance(np.concatenate([P[tri[a]].reshape(-1), P[tri[bb]].reshape(-1)]))[0]
+                        if d2 <= H * H:
+                            contacts += 1
+                            assert (a, int(bb)) in listed, (k, a, int(bb), d2)
+        runs[trig] = (o.get_state()[0], r["n_rebuilds"], contacts)
+    assert runs[1][2] > 0 and np.all(np.isfinite(runs[1][0]))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert 0 < runs[1][1] < runs[0][1], (runs[0][1], runs[1][1])
