@@ -564,6 +564,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
    * slot's index into them, per block node its (element, slot) entries, and
    * per node the block records that hold its partial forces */
   std::vector<int32_t> bn_start((size_t)ctx->nblkE + 1, 0), bn_ids, rent_start(1, 0);
+  std::vector<uint32_t> rent_gcnt;
   std::vector<uint16_t> lidx((size_t)12 * nE, 0), rent;
   for (int s = 0; s < nshell; ++s) {
     const SConst& C = ctx->C[s];
@@ -597,7 +598,18 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       }
       for (size_t j = 0; j < ids.size(); ++j) {
         bn_ids.push_back(ids[j]);
-        rent.insert(rent.end(), ent[j].begin(), ent[j].end());
+        uint32_t packed = 0;
+        for (int g = 0; g < 4; ++g) {          /* slot groups 0..3 (slots 3g .. 3g+2), entry order within */
+          int c = 0;
+          for (uint16_t en : ent[j])
+            if ((en & 15) / 3 == g) { rent.push_back(en); ++c; }
+          if (c > 255) {
+            ctx->err = "a patch node has more than 255 entries in one element block";
+            return fail(SHX_E_MESH);
+          }
+          packed |= (uint32_t)c << (8 * g);
+        }
+        rent_gcnt.push_back(packed);
         rent_start.push_back((int)rent.size());
       }
     }
@@ -631,6 +643,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   double *dX, *drb;
   int32_t *dbs, *dbi, *drs, *dns, *dnr;
   uint16_t *dli, *dre;
+  uint32_t* drg;
   int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci;
   int8_t* dv;
   if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
@@ -638,10 +651,11 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = upload(ctx, di, inc)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell)) ||
       (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)) || (r = upload(ctx, dbs, bn_start)) ||
       (r = upload(ctx, dbi, bn_ids)) || (r = upload(ctx, dli, lidx)) || (r = upload(ctx, drs, rent_start)) ||
-      (r = upload(ctx, dre, rent)) || (r = upload(ctx, dns, nrec_start)) || (r = upload(ctx, dnr, nrec)))
+      (r = upload(ctx, dre, rent)) || (r = upload(ctx, dns, nrec_start)) || (r = upload(ctx, dnr, nrec)) ||
+      (r = upload(ctx, drg, rent_gcnt)))
     return fail(r);
   P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
-  P.bn_start = dbs; P.bn_ids = dbi; P.lidx = dli; P.rent_start = drs; P.rent = dre; P.nrec_start = dns; P.nrec = dnr;
+  P.bn_start = dbs; P.bn_ids = dbi; P.lidx = dli; P.rent_start = drs; P.rent = dre; P.rent_gcnt = drg; P.nrec_start = dns; P.nrec = dnr;
   P.eshell = des; P.vshell = dvs; P.close_start = dcs; P.close_ids = dci;
   if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
   P.cs = ctx->dC;

@@ -60,9 +60,10 @@ constexpr int kElemBlock = 256;     /* threads per element-kernel block */
 constexpr int kElemPerBlock = kElemBlock;   /* elements per block, one per thread */
 constexpr int kMaxBlockNodes = 480; /* patch nodes of one element block (icospheres: at most 267) */
 constexpr int kNodeStage = 9;       /* per staged node in LDS: u_pred, u, X */
-constexpr int kGRec = 17;           /* per element in LDS: a3, JxW (mm0, mm1, 2 mm2), G1, G2, R */
-constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kElemPerBlock * kGRec) ? kMaxBlockNodes * kNodeStage
-                                                                               : kElemPerBlock * kGRec;
+constexpr int kQRec = 18;           /* per element in LDS, per pass: 3 slots x (q_i, phi_i JxW a3) */
+constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kElemPerBlock * kQRec) ? kMaxBlockNodes * kNodeStage
+                                                                               : kElemPerBlock * kQRec;
+static_assert(kMaxBlockNodes <= 2 * kElemBlock, "a thread sums at most two block records");
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block, one node each */
 constexpr int kNodePerBlock = kNodeBlock;
 constexpr int kBRec = 6;            /* per block-node record: sum of slot contributions q, sum of phi JxW a3 */
@@ -72,6 +73,7 @@ constexpr double kTentative = -1.0;  /* d2 of a candidate pair whose exact dista
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
 constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
 constexpr int kBpPerBlock = 256 / kBpLanes;
+constexpr int kCloseLds = 32;       /* close-set entries of an element staged in LDS by the pair search */
 constexpr int kRowCap = 64;         /* contact candidates per element row */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
 constexpr int kMaxShells = 1024;    /* shells per batch (work maps of the list-driven kernels are staged in LDS) */
@@ -161,7 +163,8 @@ struct DPtrs {
   const int32_t* bn_ids;      /* records */
   const uint16_t* lidx;       /* 12*nE, SoA [slot][nE] */
   const int32_t* rent_start;  /* records+1 */
-  const uint16_t* rent;
+  const uint16_t* rent;       /* per record sorted by slot group (slot / 3), then element, then slot */
+  const uint32_t* rent_gcnt;  /* per record: entries of slot groups 0..3, 8 bits each */
   const int32_t* nrec_start;  /* nV+1 */
   const int32_t* nrec;
   double* minv;               /* nV */

@@ -40,12 +40,11 @@ namespace shx {
  * list was rebuilt at the end of the previous step clear its cell buckets.
  * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
  * active shells, set by the host). */
-__global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
+__global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_w[kElemRec][kElemBlock / 64];
-  __shared__ double s_buf[kElemLds];     /* staged nodes, then per element a3, its scales, G1, G2 and R */
-  __shared__ int s_n[kElemPerBlock];
+  __shared__ double s_buf[kElemLds];     /* staged nodes, then per element three slot contributions */
   const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
   const SConst& C = P.cs[sid];
@@ -117,7 +116,9 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
     det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
           xc.x * (xa.y * xb.z - xb.y * xa.z);
   }
-  __syncthreads();   /* the staged nodes are consumed; s_buf now takes G and R */
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes the slot contributions */
+  v3 ka3 = mk(0, 0, 0), kG1 = ka3, kG2 = ka3;
+  double kJ = 0, ks3 = 0, ks4 = 0, ks5 = 0;
   if (valid) {
     const v3 a1 = add(ldrow3(T, T_A1, nE, e), du1);
     const v3 a2 = add(ldrow3(T, T_A2, nE, e), du2);
@@ -165,38 +166,63 @@ __global__ void __launch_bounds__(kElemBlock, 3) k_elem(DPtrs P, int mode)
     const v3 G[6] = {scl(a3, JxW), scl(g1, JxW), scl(g2, JxW), scl(a3, JxW * mm[0]), scl(a3, JxW * mm[1]),
                      scl(a3, JxW * m2x2)};
     /* G0 and G3..G5 are a3 scaled: kept as a3 and the four scales */
-    double* gs = s_buf + kGRec * t;
-    gs[0] = a3.x; gs[1] = a3.y; gs[2] = a3.z;
-    gs[3] = JxW; gs[4] = JxW * mm[0]; gs[5] = JxW * mm[1]; gs[6] = JxW * m2x2;
-    gs[7] = G[1].x; gs[8] = G[1].y; gs[9] = G[1].z; gs[10] = G[2].x; gs[11] = G[2].y; gs[12] = G[2].z;
-    gs[13] = R00; gs[14] = R01; gs[15] = R10; gs[16] = R11;
-    s_n[t] = N;
+    ka3 = a3; kG1 = G[1]; kG2 = G[2];
+    kJ = JxW; ks3 = JxW * mm[0]; ks4 = JxW * mm[1]; ks5 = JxW * m2x2;
     Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
     Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
     const double new_area = 0.5 * jac;
     oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
     P.area[e] = new_area;
   }
-  __syncthreads();
-  /* block records: per block node, its slot contributions summed in entry
-   * order (q_i for f_i -= q_i, phi_i G0 for f_i += p phi_i G0) */
-  for (int j = t; j < nbn; j += kElemBlock) {
-    const int r = b0 + j;
-    v3 qs = mk(0, 0, 0), ps = qs;
-    for (int k = P.rent_start[r]; k < P.rent_start[r + 1]; ++k) {
-      const int ent = P.rent[k], le = ent >> 4, i = ent & 15;
-      const double* gs = s_buf + kGRec * le;
-      const Basis6 b = basis_at(s_rb + 72 * (s_n[le] - 3), i, gs[13], gs[14], gs[15], gs[16]);
-      const v3 a3 = mk(gs[0], gs[1], gs[2]);
-      const v3 G1 = mk(gs[7], gs[8], gs[9]), G2 = mk(gs[10], gs[11], gs[12]);
-      const v3 q = sub(sub(sub(add(scl(G1, b.d1), scl(G2, b.d2)), scl(scl(a3, gs[4]), b.d11)),
-                           scl(scl(a3, gs[5]), b.d22)),
-                       scl(scl(a3, gs[6]), b.d12));
-      qs = add(qs, q);
-      axpy(ps, scl(a3, gs[3]), b.phi);
+  /* block records in four passes of three patch slots: each element thread
+   * writes q_i (f_i -= q_i) and phi_i G0 (f_i += p phi_i G0) of its slots
+   * 3g .. 3g+2 to LDS, then each block-node thread adds its record's entries
+   * of those slots (in entry order) */
+  const double* rb = s_rb + 72 * (N - 3);
+  const v3 kG0 = scl(ka3, kJ);
+  v3 rq[2] = {mk(0, 0, 0), mk(0, 0, 0)}, rp[2] = {mk(0, 0, 0), mk(0, 0, 0)};
+#pragma unroll 1
+  for (int g = 0; g < 4; ++g) {
+    if (valid) {
+      double* qs = s_buf + kQRec * t;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const Basis6 b = basis_at(rb, 3 * g + k, R00, R01, R10, R11);
+        const v3 q = sub(sub(sub(add(scl(kG1, b.d1), scl(kG2, b.d2)), scl(scl(ka3, ks3), b.d11)),
+                             scl(scl(ka3, ks4), b.d22)),
+                         scl(scl(ka3, ks5), b.d12));
+        const v3 ps = scl(kG0, b.phi);
+        qs[6 * k] = q.x; qs[6 * k + 1] = q.y; qs[6 * k + 2] = q.z;
+        qs[6 * k + 3] = ps.x; qs[6 * k + 4] = ps.y; qs[6 * k + 5] = ps.z;
+      }
     }
-    double* rec = P.brec + (size_t)kBRec * r;
-    rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int j = t + m * kElemBlock;
+      if (j < nbn) {
+        const int r = b0 + j;
+        const uint32_t cnt = P.rent_gcnt[r];
+        int k0 = P.rent_start[r];
+        for (int gg = 0; gg < g; ++gg) k0 += (cnt >> (8 * gg)) & 255;
+        const int k1 = k0 + ((cnt >> (8 * g)) & 255);
+        for (int k = k0; k < k1; ++k) {
+          const int ent = P.rent[k];
+          const double* qs = s_buf + kQRec * (ent >> 4) + 6 * ((ent & 15) - 3 * g);
+          rq[m] = add(rq[m], mk(qs[0], qs[1], qs[2]));
+          rp[m] = add(rp[m], mk(qs[3], qs[4], qs[5]));
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int j = t + m * kElemBlock;
+    if (j < nbn) {
+      double* rec = P.brec + (size_t)kBRec * (b0 + j);
+      rec[0] = rq[m].x; rec[1] = rq[m].y; rec[2] = rq[m].z; rec[3] = rp[m].x; rec[4] = rp[m].y; rec[5] = rp[m].z;
+    }
   }
   /* block partial: wave butterflies, then the block's waves in order */
   double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
@@ -237,8 +263,7 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
 {
   __shared__ float s_rb[4 * 72];
   __shared__ double s_w[kElemRec][kElemBlock / 64];
-  __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element a3, its scales, G1, G2, R (FP32) */
-  __shared__ int s_n[kElemPerBlock];
+  __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element three slot contributions (FP32) */
   const int blk = uni(P.bmap[V_ELEM32][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
   const SConst& C = P.cs[sid];
@@ -315,8 +340,10 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
           xc.x * (xa.y * xb.z - xb.y * xa.z);
   }
-  __syncthreads();   /* the staged nodes are consumed; s_buf now takes G and R */
-  float* s_g = reinterpret_cast<float*>(s_buf);
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes the slot contributions */
+  float* s_q = reinterpret_cast<float*>(s_buf);
+  v3f ka3 = mkf(0, 0, 0), kG1 = ka3, kG2 = ka3;
+  float kJ = 0, ks3 = 0, ks4 = 0, ks5 = 0;
   if (valid) {
     const v3f A1 = ldrow3f(T, F_A1, nE, e), A2 = ldrow3f(T, F_A2, nE, e);
     const v3f A11 = ldrow3f(T, F_A11, nE, e), A22 = ldrow3f(T, F_A22, nE, e), A12 = ldrow3f(T, F_A12, nE, e);
@@ -364,42 +391,65 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
                         sclf(p23, m2x2));
     const v3f G[6] = {sclf(a3, JxW), sclf(g1, JxW), sclf(g2, JxW), sclf(a3, JxW * mm[0]), sclf(a3, JxW * mm[1]),
                       sclf(a3, JxW * m2x2)};
-    float* gs = s_g + kGRec * t;
-    gs[0] = a3.x; gs[1] = a3.y; gs[2] = a3.z;
-    gs[3] = JxW; gs[4] = JxW * mm[0]; gs[5] = JxW * mm[1]; gs[6] = JxW * m2x2;
-    gs[7] = G[1].x; gs[8] = G[1].y; gs[9] = G[1].z; gs[10] = G[2].x; gs[11] = G[2].y; gs[12] = G[2].z;
-    gs[13] = R00; gs[14] = R01; gs[15] = R10; gs[16] = R11;
-    s_n[t] = N;
+    ka3 = a3; kG1 = G[1]; kG2 = G[2];
+    kJ = JxW; ks3 = JxW * mm[0]; ks4 = JxW * mm[1]; ks5 = JxW * m2x2;
     Et = 0.5 * (double)JxW * ((double)nn[0] * alpha[0] + (double)nn[1] * alpha[1] + (double)nn[2] * alpha[2]);
     Eb = 0.5 * (double)JxW * ((double)mm[0] * beta[0] + (double)mm[1] * beta[1] + (double)mm[2] * beta[2]);
     const double new_area = 0.5 * (double)jac;
     oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
     P.area[e] = new_area;
   }
-  __syncthreads();
-  for (int j = t; j < nbn; j += kElemBlock) {
-    const int r = b0 + j;
-    v3f qs = mkf(0, 0, 0), ps = qs;
-    for (int k = P.rent_start[r]; k < P.rent_start[r + 1]; ++k) {
-      const int ent = P.rent[k], le = ent >> 4, i = ent & 15;
-      const float* gs = s_g + kGRec * le;
-      const float* rb = s_rb + 72 * (s_n[le] - 3);
-      const float R00 = gs[13], R01 = gs[14], R10 = gs[15], R11 = gs[16];
-      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
-      const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
-      const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
-      const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
-      const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
-      const v3f a3 = mkf(gs[0], gs[1], gs[2]);
-      const v3f G1 = mkf(gs[7], gs[8], gs[9]), G2 = mkf(gs[10], gs[11], gs[12]);
-      const v3f q = subf(subf(subf(addf(sclf(G1, b1), sclf(G2, b2)), sclf(sclf(a3, gs[4]), b11)),
-                              sclf(sclf(a3, gs[5]), b22)),
-                         sclf(sclf(a3, gs[6]), b12));
-      qs = addf(qs, q);
-      axpyf(ps, sclf(a3, gs[3]), rb[i]);
+  /* block records in four passes of three patch slots (as k_elem) */
+  const float* rb = s_rb + 72 * (N - 3);
+  const v3f kG0 = sclf(ka3, kJ);
+  v3f rq[2] = {mkf(0, 0, 0), mkf(0, 0, 0)}, rp[2] = {mkf(0, 0, 0), mkf(0, 0, 0)};
+#pragma unroll 1
+  for (int g = 0; g < 4; ++g) {
+    if (valid) {
+      float* qs = s_q + kQRec * t;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int i = 3 * g + k;
+        const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
+        const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
+        const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
+        const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
+        const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
+        const v3f q = subf(subf(subf(addf(sclf(kG1, b1), sclf(kG2, b2)), sclf(sclf(ka3, ks3), b11)),
+                                sclf(sclf(ka3, ks4), b22)),
+                           sclf(sclf(ka3, ks5), b12));
+        const v3f ps = sclf(kG0, rb[i]);
+        qs[6 * k] = q.x; qs[6 * k + 1] = q.y; qs[6 * k + 2] = q.z;
+        qs[6 * k + 3] = ps.x; qs[6 * k + 4] = ps.y; qs[6 * k + 5] = ps.z;
+      }
     }
-    float* rec = P.brec32 + (size_t)kBRec * r;
-    rec[0] = qs.x; rec[1] = qs.y; rec[2] = qs.z; rec[3] = ps.x; rec[4] = ps.y; rec[5] = ps.z;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int j = t + m * kElemBlock;
+      if (j < nbn) {
+        const int r = b0 + j;
+        const uint32_t cnt = P.rent_gcnt[r];
+        int k0 = P.rent_start[r];
+        for (int gg = 0; gg < g; ++gg) k0 += (cnt >> (8 * gg)) & 255;
+        const int k1 = k0 + ((cnt >> (8 * g)) & 255);
+        for (int k = k0; k < k1; ++k) {
+          const int ent = P.rent[k];
+          const float* qs = s_q + kQRec * (ent >> 4) + 6 * ((ent & 15) - 3 * g);
+          rq[m] = addf(rq[m], mkf(qs[0], qs[1], qs[2]));
+          rp[m] = addf(rp[m], mkf(qs[3], qs[4], qs[5]));
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int j = t + m * kElemBlock;
+    if (j < nbn) {
+      float* rec = P.brec32 + (size_t)kBRec * (b0 + j);
+      rec[0] = rq[m].x; rec[1] = rq[m].y; rec[2] = rq[m].z; rec[3] = rp[m].x; rec[4] = rp[m].y; rec[5] = rp[m].z;
+    }
   }
   double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
 #pragma unroll

@@ -797,6 +797,19 @@ __device__ bool close_pair(const DPtrs& P, int e1, int e2)
   return false;
 }
 
+/* e2 in the ascending list s[0 .. n) (an element's close set staged in LDS) */
+__device__ __forceinline__ bool sorted_has(const int* s, int n, int e2)
+{
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int x = s[mid];
+    if (x == e2) return true;
+    if (x < e2) lo = mid + 1; else hi = mid;
+  }
+  return false;
+}
+
 __device__ __forceinline__ unsigned bucket_of(unsigned cell, unsigned bits)
 {
   return (cell * 0x9E3779B1u) >> (32 - bits);
@@ -1109,25 +1122,44 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
 {
   if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  /* per element of the block: its close set (when it has at most kCloseLds
+   * entries) and the element ids of its old row, for the candidate tests */
+  __shared__ int s_close[kBpPerBlock][kCloseLds];
+  __shared__ int s_old[kBpPerBlock][kRowCap];
+  __shared__ int s_go;
   const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
     const SConst& C = P.cs[sid];
     DShell* sh = P.sh + sid;
-    if (!bp_active(sh) || sh->err) continue;
+    __syncthreads();                   /* the previous element group is done with the LDS lists */
+    if (threadIdx.x == 0) s_go = bp_active(sh) && !sh->err;
+    __syncthreads();
+    if (!s_go) continue;               /* block-uniform */
     const int nE = P.nE;
-    const int e1 = C.ebase + (vb - s_pref[li]) * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
-    if (e1 - C.ebase >= C.ne) continue;
+    const int el = threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
+    const int e1 = C.ebase + (vb - s_pref[li]) * kBpPerBlock + el;
+    const bool valid = e1 - C.ebase < C.ne;
+    const int ob = sh->pair_src, nb = sh->pair_buf;
+    const size_t row = (size_t)e1 * kRowCap;
+    int nold = 0, cl0 = 0, ncl = 0;
+    if (valid) {
+      nold = min(P.rcnt[ob][e1], kRowCap);
+      cl0 = P.close_start[e1];
+      ncl = P.close_start[e1 + 1] - cl0;
+      for (int r = k0; r < nold; r += kBpLanes) s_old[el][r] = P.pe2[ob][row + r];
+      if (ncl <= kCloseLds)
+        for (int r = k0; r < ncl; r += kBpLanes) s_close[el][r] = P.close_ids[cl0 + r];
+    }
+    __syncthreads();
+    if (!valid) continue;
     const double dmx = sh->d_max, d2mx = sh->d2_max;
     const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
-    const int ob = sh->pair_src, nb = sh->pair_buf;
     const int4 l4 = load_cell_lo(P, e1), h4 = load_cell_hi(P, e1);
     const int lo1[3] = {l4.x, l4.y, l4.z};
     const int hi1[3] = {h4.x, h4.y, h4.z};
     const Box b1 = load_box(P, e1);
-    const size_t row = (size_t)e1 * kRowCap;
-    const int nold = min(P.rcnt[ob][e1], kRowCap);
     if (k0 == 0)
       for (int r = 0; r < nold; ++r) {
         const size_t po = row + r;
@@ -1179,10 +1211,10 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
         const int e2 = cand[j];
         if (e2 < 0) continue;
         if (boxes_apart(b1, load_box(P, e2), dmx)) continue;
-        if (close_pair(P, e1, e2)) continue;
+        if (ncl <= kCloseLds ? sorted_has(s_close[el], ncl, e2) : close_pair(P, e1, e2)) continue;
         bool listed = false;
         for (int r = 0; r < nold; ++r)
-          if (P.pe2[ob][row + r] == e2) {
+          if (s_old[el][r] == e2) {
             listed = P.pstick[ob][row + r] || !(P.pd2[ob][row + r] > d2mx);
             break;
           }
