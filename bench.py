@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per element-kernel launch, from rocprofv3 PMC counters
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain copy; median
 # KiB per dispatch) on the same workload: profiles/hbm_traffic_pmc.md
-MEASURED_TRAFFIC = {"surrogate": (851310.9 * 2.0 + 238296.1) * 1024}
+MEASURED_TRAFFIC = {"surrogate": (916304.9 * 2.0 + 381440.0) * 1024}
 BASELINE_METRIC = "post-buckled equilibria/s at rel residual 1e-6, surrogate vs plain; force GB/s"
 METRIC = "post-buckled equilibria/s at rel residual 1e-6 (plain FP64 GPU method); force GB/s"
 
