@@ -6,7 +6,10 @@ The product library never loads it.
 
 Each oracle function is pinned by tests/test_oracle_pins.py (closed forms,
 invariants, brute force, and the SHA-256 of the reference's mesh files);
-tests/test_golden.py keeps it reproducing the committed fixtures.
+tests/test_golden.py keeps it reproducing the committed fixtures.  The
+displacement rebuild trigger (an extension) is pinned by brute force: every
+pair within h is listed at every checkpoint, and the trajectory equals the
+reference trigger's.
 """
 import ctypes
 import os
