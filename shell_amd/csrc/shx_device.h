@@ -15,10 +15,11 @@
  *    staged in LDS by the kernels that use it);
  *  - node state u, v, a twice (ping-pong; DShell::par of the node's shell
  *    selects the current copy), node-interleaved xyz FP64;
- *  - per element and patch slot, the internal-force contribution (3 FP64)
- *    and per element JxW a3, written by the element kernel and gathered by
- *    the node kernel through the node incidence list, so no internal force
- *    is ever scattered with atomics;
+ *  - per element block (256 elements), its patch nodes and for each of
+ *    them a record: the block's internal-force contributions to the node and
+ *    its phi JxW a3 sums (6 FP64), written by the element kernel; the node
+ *    kernel adds a node's records (one to three, in block order), so no
+ *    internal force is ever scattered with atomics;
  *  - per shell: constants (SConst), mutable scalars (DShell), block partial
  *    records (reduced by one-block-per-shell epilogue kernels: no kernel
  *    synchronises across its own blocks) and a region of hashed cell

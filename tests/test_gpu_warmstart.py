@@ -183,3 +183,42 @@ def test_trained_surrogate_gains_over_plain_start():
     say(f"  plain steps {sum(sp)} ({wall_p:.1f} s), warm {sum(sw)} ({wall_w:.1f} s), fallbacks {len(info['fallback'])}")
     assert all(s >= 0 for s in sp) and all(r["converged_step"] >= 0 for r in rw)
     assert sum(sw) < sum(sp)
+
+
+def test_capacity_error_stops_only_its_shell(icosphere):
+    """a shell whose contact rows overflow (a squashed level-3 shell with a
+    candidate radius that lists up to 80 pairs per row; rows hold 64) stops
+    with capacity_err set while the other shell of the batch steps on; the
+    call reports SHX_E_CAPACITY; re-initialising the shell from a state whose
+    rows fit clears the error"""
+    X, T = icosphere(3)
+    X0 = imperfect(X, 0.01 * H, seed=3)
+    pa = dict(E2=0.5, bulk_modulus=0.05, volume_final_ratio=0.95, volume_ramp_time=30.0, dt=0.1, dt_max=0.1,
+              damping_alpha=0.05, volume_at_prediction=1, contact=1, mu_s=0.2, mu_d=0.2, cell_size_mult=4.0,
+              cell_margin=0.1)
+    pb = dict(dt=0.02, dt_max=0.02, damping_alpha=0.05, contact=1, cell_size_mult=1.0, cell_margin=1.0)
+    say("case row overflow in shell 1 of 2")
+    b = S.ShellBatch([(X0, T), (X, T)], [pa, pb])
+    ua = np.zeros(3 * len(X))
+    Y = X.copy()
+    Y[:, 2] *= 0.01
+    ub = (Y - X).reshape(-1)
+    z = np.zeros(3 * len(X))
+    b.set_state(0, ua, v=z, a=z, t=0.0)
+    b.set_state(1, ub, v=z, a=z, t=0.0)
+    with pytest.raises(S.ShellError) as ei:
+        b.apply_initial_conditions()
+    assert ei.value.code == S.E_CAPACITY
+    r = b.reports()
+    say("  capacity_err", r[0]["capacity_err"], r[1]["capacity_err"])
+    assert r[0]["capacity_err"] == 0 and r[1]["capacity_err"] != 0
+    with pytest.raises(S.ShellError) as ei:
+        b.advance(40)
+    assert ei.value.code == S.E_CAPACITY
+    r = b.reports()
+    assert r[0]["n_steps"] == 40 and r[1]["n_steps"] == 0
+    b.set_state(1, z, v=z, a=z, t=0.0)
+    b.init_shells([1])
+    r = b.advance(40)
+    say("  after re-init: steps", r[0]["n_steps"], r[1]["n_steps"], "capacity_err", r[1]["capacity_err"])
+    assert r[1]["capacity_err"] == 0 and r[1]["n_steps"] == 40 and r[0]["n_steps"] == 80
