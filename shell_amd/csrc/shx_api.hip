@@ -181,6 +181,7 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   k_bp_box<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
   prof_end(ctx);
   prof_begin(ctx, 5);
+  k_bp_grid<<<ctx->S, 256, 0, ctx->st>>>(P, list);
   k_bp_count<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
   prof_end(ctx);
   prof_begin(ctx, 6);

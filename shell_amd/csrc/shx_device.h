@@ -72,7 +72,7 @@ constexpr int kNodeRec = 23;        /* node-kernel partial record length */
 constexpr int kElemRec = 6;         /* element-kernel partial record length */
 constexpr double kTentative = -1.0;  /* d2 of a candidate pair whose exact distance is not yet formed */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
-constexpr int kBpLanes = 32;        /* broad-phase lanes per element (one cell each) */
+constexpr int kBpLanes = 16;        /* pair-search lanes per element (cells, then bucket entries) */
 constexpr int kBpPerBlock = 256 / kBpLanes;
 constexpr int kCloseLds = 32;       /* close-set entries of an element staged in LDS by the pair search */
 constexpr int kRowCap = 64;         /* contact candidates per element row */

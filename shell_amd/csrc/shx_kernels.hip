@@ -957,9 +957,8 @@ __device__ bool shell_grid(const DPtrs& P, const SConst& C, Grid& g)
   return true;
 }
 
-/* (element, cell) lane mapping of the counting and scatter kernels:
- * kBpLanes lanes per element, lane k handles the element's cells k,
- * k + kBpLanes, ... of its index box in cx-major order */
+/* (element, cell) lane mapping of the pair search: lane k handles the
+ * element's cells k, k + step, ... of its index box in cx-major order */
 __device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz, int& cx, int& cy, int& cz)
 {
   cz = lo[2] + k % nz;
@@ -968,66 +967,73 @@ __device__ __forceinline__ void bp_cell_of(int k, const int* lo, int ny, int nz,
   cx = lo[0] + r / ny;
 }
 
-/* count pass of the counting sort: the cell grid of the shell (each block
- * forms it from the tile partials; the shell's first block records it),
- * each element's cell index range (its AABB widened by d_max, ShellSystem.C:
- * 1967-2010) and one count per (element, cell) in the cell's hash bucket */
+/* the cell grid of each listed shell, once per rebuild (one block per
+ * shell): box, averaged element size, cell size, d_max, grid size and the
+ * n_cells_max check (ShellSystem.C:1933-1964) from the tile partials; an
+ * aborted rebuild keeps the old list */
+__global__ void __launch_bounds__(256) k_bp_grid(DPtrs P, int list)
+{
+  if ((int)blockIdx.x >= min(P.wl[list][0], P.S)) return;
+  const int sid = P.wl[list][1 + blockIdx.x];
+  const SConst& C = P.cs[sid];
+  DShell* sh = P.sh + sid;
+  if (!bp_active(sh)) return;
+  Grid g;
+  const bool ok = shell_grid(P, C, g);
+  if (threadIdx.x != 0) return;
+  sh->du_sum = 0;
+  sh->n_rebuilds++;
+  sh->avg_elem_size = g.avg;
+  for (int c = 0; c < 3; ++c) { sh->bmin[c] = g.bmin[c]; sh->bmax[c] = g.bmax[c]; }
+  if (ok) {
+    sh->cell_size = g.cell;
+    sh->d_max = g.dmx;
+    sh->d2_max = g.dmx * g.dmx;
+    for (int c = 0; c < 3; ++c) sh->nd[c] = (int)g.nd[c];
+    sh->n_cells = (int)g.ncell;
+  } else {
+    if (g.problem == 4) sh->n_cells = (int)g.ncell;
+    sh->problem = max(sh->problem, g.problem);
+    sh->rebuild = 0;
+    sh->pair_buf = sh->pair_src;
+  }
+}
+
+/* count pass of the counting sort, one thread per element: the element's
+ * cell index range (its AABB widened by d_max, ShellSystem.C:1967-2010) and
+ * one count per (element, cell) in the cell's hash bucket */
 __global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
 {
   if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
-  const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], V_TILE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
     const SConst& C = P.cs[sid];
     DShell* sh = P.sh + sid;
     if (!bp_active(sh)) continue;
-    const int local = vb - s_pref[li];
-    Grid g;
-    const bool ok = shell_grid(P, C, g);
-    if (local == 0 && threadIdx.x == 0) {
-      sh->du_sum = 0;
-      sh->n_rebuilds++;
-      sh->avg_elem_size = g.avg;
-      for (int c = 0; c < 3; ++c) { sh->bmin[c] = g.bmin[c]; sh->bmax[c] = g.bmax[c]; }
-      if (ok) {
-        sh->cell_size = g.cell;
-        sh->d_max = g.dmx;
-        sh->d2_max = g.dmx * g.dmx;
-        for (int c = 0; c < 3; ++c) sh->nd[c] = (int)g.nd[c];
-        sh->n_cells = (int)g.ncell;
-      } else {
-        if (g.problem == 4) sh->n_cells = (int)g.ncell;
-        sh->problem = max(sh->problem, g.problem);
-        sh->rebuild = 0;
-        sh->pair_buf = sh->pair_src;
-      }
-    }
-    if (!ok) continue;
-    const int nE = P.nE;
-    const int e = C.ebase + local * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
+    const int e = C.ebase + (vb - s_pref[li]) * 256 + threadIdx.x;
     if (e - C.ebase >= C.ne) continue;
+    const double cell = sh->cell_size, dmx = sh->d_max;
+    const double bmin[3] = {sh->bmin[0], sh->bmin[1], sh->bmin[2]};
+    const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
     const Box bx = load_box(P, e);
     int lo[3], hi[3];
     for (int q = 0; q < 3; ++q) {
-      lo[q] = (int)((bx.lo[q] - g.bmin[q]) / g.cell);
-      hi[q] = (int)((bx.hi[q] - g.bmin[q] + g.dmx) / g.cell);
+      lo[q] = (int)((bx.lo[q] - bmin[q]) / cell);
+      hi[q] = (int)((bx.hi[q] - bmin[q] + dmx) / cell);
     }
-    if (k0 == 0) {
-      int4* pc = reinterpret_cast<int4*>(P.ecell) + 2 * (size_t)e;
-      pc[0] = make_int4(lo[0], lo[1], lo[2], 0);
-      pc[1] = make_int4(hi[0], hi[1], hi[2], 0);
-      P.rcnt[sh->pair_buf][e] = 0;
-    }
-    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-    const int total = (hi[0] - lo[0] + 1) * ny * nz;
-    for (int k = k0; k < total; k += kBpLanes) {
-      int cx, cy, cz;
-      bp_cell_of(k, lo, ny, nz, cx, cy, cz);
-      const unsigned c = (unsigned)cx * g.nd[1] * g.nd[2] + (unsigned)cy * g.nd[2] + (unsigned)cz;
-      atomicAdd(&P.bcnt[C.bbase + bucket_of(c, C.hb_bits)], 1u);
-    }
+    int4* pc = reinterpret_cast<int4*>(P.ecell) + 2 * (size_t)e;
+    pc[0] = make_int4(lo[0], lo[1], lo[2], 0);
+    pc[1] = make_int4(hi[0], hi[1], hi[2], 0);
+    P.rcnt[sh->pair_buf][e] = 0;
+    for (int cx = lo[0]; cx <= hi[0]; ++cx)
+      for (int cy = lo[1]; cy <= hi[1]; ++cy)
+        for (int cz = lo[2]; cz <= hi[2]; ++cz) {
+          const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+          atomicAdd(&P.bcnt[C.bbase + bucket_of(c, C.hb_bits)], 1u);
+        }
   }
 }
 
@@ -1084,29 +1090,24 @@ __global__ void __launch_bounds__(256) k_bp_scatter(DPtrs P, int list)
 {
   if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
-  const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], V_TILE, s_sh, s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
     const SConst& C = P.cs[sid];
     DShell* sh = P.sh + sid;
     if (!bp_active(sh) || sh->err) continue;
-    const int nE = P.nE;
-    const int e = C.ebase + (vb - s_pref[li]) * kBpPerBlock + threadIdx.x / kBpLanes, k0 = threadIdx.x % kBpLanes;
+    const int e = C.ebase + (vb - s_pref[li]) * 256 + threadIdx.x;
     if (e - C.ebase >= C.ne) continue;
     const unsigned nd1 = sh->nd[1], nd2 = sh->nd[2];
     const int4 l4 = load_cell_lo(P, e), h4 = load_cell_hi(P, e);
-    const int lo[3] = {l4.x, l4.y, l4.z};
-    const int hi[3] = {h4.x, h4.y, h4.z};
-    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-    const int total = (hi[0] - lo[0] + 1) * ny * nz;
-    for (int k = k0; k < total; k += kBpLanes) {
-      int cx, cy, cz;
-      bp_cell_of(k, lo, ny, nz, cx, cy, cz);
-      const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
-      const unsigned pos = atomicAdd(&P.bcur[C.bbase + bucket_of(c, C.hb_bits)], 1u);
-      P.bent[(size_t)C.entbase + pos] = ((uint64_t)c << 32) | (uint32_t)e;
-    }
+    for (int cx = l4.x; cx <= h4.x; ++cx)
+      for (int cy = l4.y; cy <= h4.y; ++cy)
+        for (int cz = l4.z; cz <= h4.z; ++cz) {
+          const unsigned c = (unsigned)cx * nd1 * nd2 + (unsigned)cy * nd2 + (unsigned)cz;
+          const unsigned pos = atomicAdd(&P.bcur[C.bbase + bucket_of(c, C.hb_bits)], 1u);
+          P.bent[(size_t)C.entbase + pos] = ((uint64_t)c << 32) | (uint32_t)e;
+        }
   }
 }
 
