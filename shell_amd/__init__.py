@@ -268,16 +268,6 @@ class ShellSystem:
     def set_graphs(self, enable=True):
         self._check(lib().shx_set_graphs(self.h, 1 if enable else 0), "shx_set_graphs")
 
-    def set_profiling(self, enable=True):
-        self._check(lib().shx_set_profiling(self.h, 1 if enable else 0), "shx_set_profiling")
-
-    def kernel_times(self):
-        ms = np.zeros(len(KERNEL_CLASSES))
-        cnt = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
-        self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
-                    "shx_kernel_times")
-        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}
-
     def kernel_times(self):
         ms = np.zeros(len(KERNEL_CLASSES))
         cnt = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)

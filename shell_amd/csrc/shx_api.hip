@@ -561,7 +561,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     for (int q = nv[s]; q < C.nbN * kNodePerBlock; ++q) inc_start[C.vbase + q] = (int)inc.size();
   }
   inc_start[nV] = (int)inc.size();
-  /* element blocks: their patch nodes (ascending global ids), each patch
+  /* element blocks: their patch nodes (in descending entry count), each patch
    * slot's index into them, per block node its (element, slot) entries, and
    * per node the block records that hold its partial forces */
   std::vector<int32_t> bn_start((size_t)ctx->nblkE + 1, 0), bn_ids, rent_start(1, 0);
@@ -587,17 +587,32 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
         return fail(SHX_E_MESH);
       }
       bn_start[gb] = (int)bn_ids.size();
+      /* entries: local element * 16 + slot, valence class - 3 in bits 12-13 */
       std::vector<std::vector<uint16_t>> ent(ids.size());
       for (int le = 0; le < kElemPerBlock; ++le) {
         const int e = first + le, q = e - C.ebase;
         if (q >= ne[s]) continue;
+        const int vc = vcls[(size_t)e] - 3;       /* 0 .. 3 */
         for (int i = 0; i < 12; ++i) {
           const int j = (int)(std::lower_bound(ids.begin(), ids.end(), pidx[(size_t)i * nE + e]) - ids.begin());
           lidx[(size_t)i * nE + e] = (uint16_t)j;
-          if (T.patch[(size_t)q * 12 + i] >= 0) ent[j].push_back((uint16_t)(le * 16 + i));
+          if (T.patch[(size_t)q * 12 + i] >= 0) ent[j].push_back((uint16_t)((vc << 12) | (le * 16 + i)));
         }
       }
-      for (size_t j = 0; j < ids.size(); ++j) {
+      /* block nodes in descending entry count (ties: ascending id): thread t
+       * of the element kernel sums records t and 511 - t, a full one with a
+       * sparse one */
+      std::vector<int> ord(ids.size()), pos(ids.size());
+      for (size_t j = 0; j < ids.size(); ++j) ord[j] = (int)j;
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ent[a].size() > ent[b].size(); });
+      for (size_t j = 0; j < ids.size(); ++j) pos[ord[j]] = (int)j;
+      for (int le = 0; le < kElemPerBlock; ++le) {
+        const int e = first + le;
+        if (e - C.ebase >= ne[s]) continue;
+        for (int i = 0; i < 12; ++i) lidx[(size_t)i * nE + e] = (uint16_t)pos[lidx[(size_t)i * nE + e]];
+      }
+      for (size_t jj = 0; jj < ids.size(); ++jj) {
+        const int j = ord[jj];
         bn_ids.push_back(ids[j]);
         uint32_t packed = 0;
         for (int g = 0; g < 4; ++g) {          /* slot groups 0..3 (slots 3g .. 3g+2), entry order within */

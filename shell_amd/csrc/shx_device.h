@@ -60,11 +60,14 @@ constexpr int kAlign = 256;         /* shell ranges start at and are padded to m
 constexpr int kElemBlock = 256;     /* threads per element-kernel block */
 constexpr int kElemPerBlock = kElemBlock;   /* elements per block, one per thread */
 constexpr int kMaxBlockNodes = 480; /* patch nodes of one element block (icospheres: at most 267) */
-constexpr int kNodeStage = 9;       /* per staged node in LDS: u_pred, u, X */
-constexpr int kQRec = 18;           /* per element in LDS, per pass: 3 slots x (q_i, phi_i JxW a3) */
-constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kElemPerBlock * kQRec) ? kMaxBlockNodes * kNodeStage
-                                                                               : kElemPerBlock * kQRec;
+constexpr int kNodeStage = 6;       /* per staged node in LDS: u_pred, vertex position for the volume */
+constexpr int kQRec = 9;            /* per element in LDS, per pass: 3 slots x q_i */
+constexpr int kQArea = kElemPerBlock * kQRec;        /* then per element JxW a3 (3 values) */
+constexpr int kElemLds = (kMaxBlockNodes * kNodeStage > kQArea + 3 * kElemPerBlock) ? kMaxBlockNodes * kNodeStage
+                                                                                     : kQArea + 3 * kElemPerBlock;
+constexpr int kMaxBlockEnt = 12 * kElemPerBlock;     /* (element, slot) entries of one element block */
 static_assert(kMaxBlockNodes <= 2 * kElemBlock, "a thread sums at most two block records");
+static_assert(kMaxBlockEnt <= 65535, "block-relative entry offsets are 16-bit");
 constexpr int kNodeBlock = 256;     /* threads per node-kernel block, one node each */
 constexpr int kNodePerBlock = kNodeBlock;
 constexpr int kBRec = 6;            /* per block-node record: sum of slot contributions q, sum of phi JxW a3 */
@@ -155,11 +158,12 @@ struct DPtrs {
   const int32_t* inc_start;   /* nV+1 */
   const int32_t* inc;         /* e*16+i, e global */
   /* element blocks and their patch nodes: block b's nodes are records
-   * bn_start[b] .. bn_start[b+1] (global ids bn_ids, ascending); lidx is each
-   * patch slot's index into its block's nodes; record r sums the block's
-   * (element, slot) entries rent[rent_start[r] .. rent_start[r+1]]
-   * (local element * 16 + slot); node n gathers the records
-   * nrec[nrec_start[n] .. nrec_start[n+1]] */
+   * bn_start[b] .. bn_start[b+1] (global ids bn_ids, in descending entry
+   * count, ties by ascending id); lidx is each patch slot's index into its
+   * block's nodes; record r sums the block's (element, slot) entries
+   * rent[rent_start[r] .. rent_start[r+1]] (local element * 16 + slot,
+   * valence class - 3 in bits 12-13); node n gathers the records
+   * nrec[nrec_start[n] .. nrec_start[n+1]] (ascending, i.e. in block order) */
   const int32_t* bn_start;    /* blocks+1 */
   const int32_t* bn_ids;      /* records */
   const uint16_t* lidx;       /* 12*nE, SoA [slot][nE] */

@@ -18,6 +18,24 @@
 
 namespace shx {
 
+/* The block's (element, slot) entries (local element * 16 + slot, the
+ * element's valence class - 3 in bits 12-13; at most kMaxBlockEnt) and per
+ * block node its first entry (block-relative) and slot-group counts, staged
+ * in LDS for the record passes.  Block nodes come in descending entry count:
+ * thread t sums records t and 511 - t (rec_of). */
+__device__ __forceinline__ void stage_entries(const DPtrs& P, int b0, int nbn, uint16_t* s_ent, uint16_t* s_nst,
+                                              uint32_t* s_ngc)
+{
+  const int k0 = P.rent_start[b0], nent = P.rent_start[b0 + nbn] - k0;
+  for (int k = threadIdx.x; k < nent; k += blockDim.x) s_ent[k] = P.rent[k0 + k];
+  for (int j = threadIdx.x; j < nbn; j += blockDim.x) {
+    s_nst[j] = (uint16_t)(P.rent_start[b0 + j] - k0);
+    s_ngc[j] = P.rent_gcnt[b0 + j];
+  }
+}
+
+__device__ __forceinline__ int rec_of(int t, int m) { return m == 0 ? t : 2 * kElemBlock - 1 - t; }
+
 /* The element force.  Per element: gather u_pred over the patch, form the
  * deformed metric (compute_deformed_metrics, ShellSystem.C:1407-1460),
  * membrane and bending strains, orthotropic H, stress resultants and the
@@ -29,13 +47,16 @@ namespace shx {
  * 1570-1576), G0 = JxW a3.
  *
  * A block covers kElemPerBlock elements, one per thread.  It first stages
- * its patch nodes (bn_ids: u_pred, u, X) in LDS with coalesced loads, so
- * the patch gather reads LDS only; after the stress phase the LDS holds each
- * element's G and R, and one thread per block node sums the block's slot
- * contributions to that node (its entries in fixed order) into the node's
- * block record: sum q_i and sum phi_i G0.  The node kernel adds a node's
- * records (one to three), so no contribution is written per (element, slot)
- * and no atomics are used.  Block partials (energies, volume terms, max
+ * its patch nodes (bn_ids: u_pred and the position the enclosed volume
+ * uses) and its (element, slot) entry lists in LDS with coalesced loads, so
+ * the patch gather and the record sums read LDS only; max |u_pred - u| is
+ * taken over the staged nodes (every node of the shell is a patch node of
+ * some block).  After the stress phase, in four passes of three slots, the
+ * element threads put q_i in LDS and one thread per block node sums its
+ * entries (fixed order) into the node's block record: sum q_i and
+ * sum phi_i G0 (G0 = JxW a3 of the element, in LDS once).  The node kernel
+ * adds a node's records (one to three), so no contribution is written per
+ * (element, slot) and no atomics are used.  Block partials (energies, volume terms, max
  * |du|^2, max |u_pred - u_build|^2) go to epart for k_fin_elem.  The blocks of a shell whose contact
  * list was rebuilt at the end of the previous step clear its cell buckets.
  * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
@@ -44,7 +65,9 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_w[kElemRec][kElemBlock / 64];
-  __shared__ double s_buf[kElemLds];     /* staged nodes, then per element three slot contributions */
+  __shared__ double s_buf[kElemLds];     /* staged nodes, then per element three q_i and JxW a3 */
+  __shared__ uint16_t s_ent[kMaxBlockEnt], s_nst[kMaxBlockNodes];
+  __shared__ uint32_t s_ngc[kMaxBlockNodes];
   const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
   const SConst& C = P.cs[sid];
@@ -67,21 +90,21 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
   const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
   const bool track = predict && C.contact && C.rebuild_trigger;
   double dev = 0;                        /* max |u_pred - u_build|^2 of the block's nodes (trigger 1) */
+  double dumax = 0;                      /* max |u_pred - u|^2 of the block's nodes */
+  stage_entries(P, b0, nbn, s_ent, s_nst, s_ngc);
   for (int j = t; j < nbn; j += kElemBlock) {
     const int n = P.bn_ids[b0 + j];
     const v3 u = ld3(U, n);
     const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
-    const v3 x = ld3(P.X, n);
-    if (track) {
-      dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
-    }
+    const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
+    if (predict) dumax = fmax(dumax, nrm2_rn(sub(up, u)));
+    if (track) dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
     double* sn = s_buf + kNodeStage * j;
     sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
-    sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
-    sn[6] = x.x; sn[7] = x.y; sn[8] = x.z;
+    sn[3] = x.x; sn[4] = x.y; sn[5] = x.z;
   }
   __syncthreads();
-  double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  double Et = 0, Eb = 0, det = 0, oarea = 0;
   v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
   double R00 = 0, R01 = 0, R10 = 0, R11 = 0;
   int N = 6;
@@ -91,32 +114,35 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
     N = P.vcls[e];
     const double* rb = s_rb + 72 * (N - 3);
     v3 xa = du1, xb = du1, xc = du1;
+    /* the rotation to the element frame is linear: the patch sums are
+     * formed with the reference-coordinate derivatives and rotated once,
+     * phi_{i,1} = R00 phi_{i,xi} + R10 phi_{i,eta}, etc. (basis_at) */
+    v3 sx = du1, se = du1, sxx = du1, see = du1, sxe = du1;
 #pragma unroll 2
     for (int i = 0; i < 12; ++i) {
       const double* sn = s_buf + kNodeStage * P.lidx[(size_t)i * nE + e];
       const v3 up = mk(sn[0], sn[1], sn[2]);
-      const Basis6 b = basis_at(rb, i, R00, R01, R10, R11);
-      axpy(du1, up, b.d1);
-      axpy(du2, up, b.d2);
-      axpy(du11, up, b.d11);
-      axpy(du22, up, b.d22);
-      axpy(du12, up, b.d12);
+      axpy(sx, up, rb[12 + i]);
+      axpy(se, up, rb[24 + i]);
+      axpy(sxx, up, rb[36 + i]);
+      axpy(see, up, rb[48 + i]);
+      axpy(sxe, up, rb[60 + i]);
       if (i == 0 || i == 1 || i == N) {
-        const v3 u = mk(sn[3], sn[4], sn[5]);
-        if (predict) {
-          dumax = fmax(dumax, nrm2_rn(sub(up, u)));
-        }
-        const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
+        const v3 x = mk(sn[3], sn[4], sn[5]);
         if (i == 0) xa = x;
         else if (i == 1) xb = x;
         else xc = x;
       }
     }
+    du1 = add(scl(sx, R00), scl(se, R10));
+    du2 = add(scl(sx, R01), scl(se, R11));
+    du11 = add(add(scl(sxx, R00 * R00), scl(see, R10 * R10)), scl(sxe, 2 * R00 * R10));
+    du22 = add(add(scl(sxx, R01 * R01), scl(see, R11 * R11)), scl(sxe, 2 * R01 * R11));
+    du12 = add(add(scl(sxx, R00 * R01), scl(see, R10 * R11)), scl(sxe, R01 * R10 + R00 * R11));
     /* triad(var, n) = p_n(var), TypeTensor::det (ShellSystem.C:2523-2531) */
     det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
           xc.x * (xa.y * xb.z - xb.y * xa.z);
   }
-  __syncthreads();   /* the staged nodes are consumed; s_buf now takes the slot contributions */
   v3 ka3 = mk(0, 0, 0), kG1 = ka3, kG2 = ka3;
   double kJ = 0, ks3 = 0, ks4 = 0, ks5 = 0;
   if (valid) {
@@ -175,11 +201,27 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
     P.area[e] = new_area;
   }
   /* block records in four passes of three patch slots: each element thread
-   * writes q_i (f_i -= q_i) and phi_i G0 (f_i += p phi_i G0) of its slots
-   * 3g .. 3g+2 to LDS, then each block-node thread adds its record's entries
-   * of those slots (in entry order) */
+   * writes q_i (f_i -= q_i) of its slots 3g .. 3g+2 to LDS, then each
+   * block-node thread adds its record's entries of those slots (in entry
+   * order) to sum q_i and sum phi_i G0 (f_i += p phi_i G0).  With the
+   * rotation folded into per-element vectors,
+   *   q_i = phi_{i,xi} Vx + phi_{i,eta} Ve
+   *         - a3 (phi_{i,xixi} cxx + phi_{i,etaeta} cee + phi_{i,xieta} cxe),
+   * the same sum as G1 phi_{i,1} + G2 phi_{i,2} - G3 phi_{i,11} - ... */
   const double* rb = s_rb + 72 * (N - 3);
-  const v3 kG0 = scl(ka3, kJ);
+  double* s_g0 = s_buf + kQArea;
+  v3 kVx = ka3, kVe = ka3;
+  double cxx = 0, cee = 0, cxe = 0;
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes q_i and JxW a3 */
+  if (valid) {
+    const v3 kG0 = scl(ka3, kJ);
+    s_g0[3 * t] = kG0.x; s_g0[3 * t + 1] = kG0.y; s_g0[3 * t + 2] = kG0.z;
+    kVx = add(scl(kG1, R00), scl(kG2, R01));
+    kVe = add(scl(kG1, R10), scl(kG2, R11));
+    cxx = ks3 * (R00 * R00) + ks4 * (R01 * R01) + ks5 * (R00 * R01);
+    cee = ks3 * (R10 * R10) + ks4 * (R11 * R11) + ks5 * (R10 * R11);
+    cxe = ks3 * (2 * R00 * R10) + ks4 * (2 * R01 * R11) + ks5 * (R01 * R10 + R00 * R11);
+  }
   v3 rq[2] = {mk(0, 0, 0), mk(0, 0, 0)}, rp[2] = {mk(0, 0, 0), mk(0, 0, 0)};
 #pragma unroll 1
   for (int g = 0; g < 4; ++g) {
@@ -187,30 +229,28 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
       double* qs = s_buf + kQRec * t;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const Basis6 b = basis_at(rb, 3 * g + k, R00, R01, R10, R11);
-        const v3 q = sub(sub(sub(add(scl(kG1, b.d1), scl(kG2, b.d2)), scl(scl(ka3, ks3), b.d11)),
-                             scl(scl(ka3, ks4), b.d22)),
-                         scl(scl(ka3, ks5), b.d12));
-        const v3 ps = scl(kG0, b.phi);
-        qs[6 * k] = q.x; qs[6 * k + 1] = q.y; qs[6 * k + 2] = q.z;
-        qs[6 * k + 3] = ps.x; qs[6 * k + 4] = ps.y; qs[6 * k + 5] = ps.z;
+        const int i = 3 * g + k;
+        const double c = rb[36 + i] * cxx + rb[48 + i] * cee + rb[60 + i] * cxe;
+        const v3 q = sub(add(scl(kVx, rb[12 + i]), scl(kVe, rb[24 + i])), scl(ka3, c));
+        qs[3 * k] = q.x; qs[3 * k + 1] = q.y; qs[3 * k + 2] = q.z;
       }
     }
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const int j = t + m * kElemBlock;
+      const int j = rec_of(t, m);
       if (j < nbn) {
-        const int r = b0 + j;
-        const uint32_t cnt = P.rent_gcnt[r];
-        int k0 = P.rent_start[r];
+        const uint32_t cnt = s_ngc[j];
+        int k0 = s_nst[j];
         for (int gg = 0; gg < g; ++gg) k0 += (cnt >> (8 * gg)) & 255;
         const int k1 = k0 + ((cnt >> (8 * g)) & 255);
         for (int k = k0; k < k1; ++k) {
-          const int ent = P.rent[k];
-          const double* qs = s_buf + kQRec * (ent >> 4) + 6 * ((ent & 15) - 3 * g);
+          const int ent = s_ent[k], le = (ent >> 4) & 255, i = ent & 15;
+          const double* qs = s_buf + kQRec * le + 3 * (i - 3 * g);
+          const double* g0 = s_g0 + 3 * le;
+          const double phi = s_rb[72 * (ent >> 12) + i];
           rq[m] = add(rq[m], mk(qs[0], qs[1], qs[2]));
-          rp[m] = add(rp[m], mk(qs[3], qs[4], qs[5]));
+          rp[m] = add(rp[m], scl(mk(g0[0], g0[1], g0[2]), phi));
         }
       }
     }
@@ -218,7 +258,7 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
   }
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
-    const int j = t + m * kElemBlock;
+    const int j = rec_of(t, m);
     if (j < nbn) {
       double* rec = P.brec + (size_t)kBRec * (b0 + j);
       rec[0] = rq[m].x; rec[1] = rq[m].y; rec[2] = rq[m].z; rec[3] = rp[m].x; rec[4] = rp[m].y; rec[5] = rp[m].z;
@@ -263,7 +303,9 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
 {
   __shared__ float s_rb[4 * 72];
   __shared__ double s_w[kElemRec][kElemBlock / 64];
-  __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element three slot contributions (FP32) */
+  __shared__ double s_buf[kElemLds];     /* staged nodes (FP64), then per element three q_i and JxW a3 (FP32) */
+  __shared__ uint16_t s_ent[kMaxBlockEnt], s_nst[kMaxBlockNodes];
+  __shared__ uint32_t s_ngc[kMaxBlockNodes];
   const int blk = uni(P.bmap[V_ELEM32][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
   const SConst& C = P.cs[sid];
@@ -285,21 +327,21 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   const int b0 = P.bn_start[blk], nbn = P.bn_start[blk + 1] - b0;
   const bool track = predict && C.contact && C.rebuild_trigger;
   double dev = 0;                        /* max |u_pred - u_build|^2 of the block's nodes (trigger 1) */
+  double dumax = 0;                      /* max |u_pred - u|^2 of the block's nodes */
+  stage_entries(P, b0, nbn, s_ent, s_nst, s_ngc);
   for (int j = t; j < nbn; j += kElemBlock) {
     const int n = P.bn_ids[b0 + j];
     const v3 u = ld3(U, n);
     const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
-    const v3 x = ld3(P.X, n);
-    if (track) {
-      dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
-    }
+    const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
+    if (predict) dumax = fmax(dumax, nrm2_rn(sub(up, u)));
+    if (track) dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
     double* sn = s_buf + kNodeStage * j;
     sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
-    sn[3] = u.x; sn[4] = u.y; sn[5] = u.z;
-    sn[6] = x.x; sn[7] = x.y; sn[8] = x.z;
+    sn[3] = x.x; sn[4] = x.y; sn[5] = x.z;
   }
   __syncthreads();
-  double Et = 0, Eb = 0, dumax = 0, det = 0, oarea = 0;
+  double Et = 0, Eb = 0, det = 0, oarea = 0;
   v3f du1 = mkf(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
   float R00 = 0, R01 = 0, R10 = 0, R11 = 0;
   int N = 6;
@@ -314,33 +356,34 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     const double* s0 = s_buf + kNodeStage * P.lidx[e];
     const v3 u0 = mk(s0[0], s0[1], s0[2]);
     v3 xa = mk(0, 0, 0), xb = xa, xc = xa;
+    /* reference-coordinate sums, rotated once (as k_elem) */
+    v3f sx = du1, se = du1, sxx = du1, see = du1, sxe = du1;
 #pragma unroll 2
     for (int i = 0; i < 12; ++i) {
       const double* sn = s_buf + kNodeStage * P.lidx[(size_t)i * nE + e];
       const v3 up = mk(sn[0], sn[1], sn[2]);
       const v3 ur = sub(up, u0);
       const v3f uf = mkf((float)ur.x, (float)ur.y, (float)ur.z);
-      const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
-      axpyf(du1, uf, dx * R00 + de * R10);
-      axpyf(du2, uf, dx * R01 + de * R11);
-      axpyf(du11, uf, dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10);
-      axpyf(du22, uf, dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11);
-      axpyf(du12, uf, dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11);
+      axpyf(sx, uf, rb[12 + i]);
+      axpyf(se, uf, rb[24 + i]);
+      axpyf(sxx, uf, rb[36 + i]);
+      axpyf(see, uf, rb[48 + i]);
+      axpyf(sxe, uf, rb[60 + i]);
       if (i == 0 || i == 1 || i == N) {
-        const v3 u = mk(sn[3], sn[4], sn[5]);
-        if (predict) {
-          dumax = fmax(dumax, nrm2_rn(sub(up, u)));
-        }
-        const v3 x = add(mk(sn[6], sn[7], sn[8]), C.volume_at_prediction ? up : u);
+        const v3 x = mk(sn[3], sn[4], sn[5]);
         if (i == 0) xa = x;
         else if (i == 1) xb = x;
         else xc = x;
       }
     }
+    du1 = addf(sclf(sx, R00), sclf(se, R10));
+    du2 = addf(sclf(sx, R01), sclf(se, R11));
+    du11 = addf(addf(sclf(sxx, R00 * R00), sclf(see, R10 * R10)), sclf(sxe, 2 * R00 * R10));
+    du22 = addf(addf(sclf(sxx, R01 * R01), sclf(see, R11 * R11)), sclf(sxe, 2 * R01 * R11));
+    du12 = addf(addf(sclf(sxx, R00 * R01), sclf(see, R10 * R11)), sclf(sxe, R01 * R10 + R00 * R11));
     det = xa.x * (xb.y * xc.z - xc.y * xb.z) - xb.x * (xa.y * xc.z - xc.y * xa.z) +
           xc.x * (xa.y * xb.z - xb.y * xa.z);
   }
-  __syncthreads();   /* the staged nodes are consumed; s_buf now takes the slot contributions */
   float* s_q = reinterpret_cast<float*>(s_buf);
   v3f ka3 = mkf(0, 0, 0), kG1 = ka3, kG2 = ka3;
   float kJ = 0, ks3 = 0, ks4 = 0, ks5 = 0;
@@ -401,7 +444,19 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   }
   /* block records in four passes of three patch slots (as k_elem) */
   const float* rb = s_rb + 72 * (N - 3);
-  const v3f kG0 = sclf(ka3, kJ);
+  float* s_g0 = s_q + kQArea;
+  v3f kVx = ka3, kVe = ka3;
+  float cxx = 0, cee = 0, cxe = 0;
+  __syncthreads();   /* the staged nodes are consumed; s_buf now takes q_i and JxW a3 */
+  if (valid) {
+    const v3f kG0 = sclf(ka3, kJ);
+    s_g0[3 * t] = kG0.x; s_g0[3 * t + 1] = kG0.y; s_g0[3 * t + 2] = kG0.z;
+    kVx = addf(sclf(kG1, R00), sclf(kG2, R01));
+    kVe = addf(sclf(kG1, R10), sclf(kG2, R11));
+    cxx = ks3 * (R00 * R00) + ks4 * (R01 * R01) + ks5 * (R00 * R01);
+    cee = ks3 * (R10 * R10) + ks4 * (R11 * R11) + ks5 * (R10 * R11);
+    cxe = ks3 * (2 * R00 * R10) + ks4 * (2 * R01 * R11) + ks5 * (R01 * R10 + R00 * R11);
+  }
   v3f rq[2] = {mkf(0, 0, 0), mkf(0, 0, 0)}, rp[2] = {mkf(0, 0, 0), mkf(0, 0, 0)};
 #pragma unroll 1
   for (int g = 0; g < 4; ++g) {
@@ -410,34 +465,27 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const int i = 3 * g + k;
-        const float dx = rb[12 + i], de = rb[24 + i], dxx = rb[36 + i], dee = rb[48 + i], dxe = rb[60 + i];
-        const float b1 = dx * R00 + de * R10, b2 = dx * R01 + de * R11;
-        const float b11 = dxx * R00 * R00 + dee * R10 * R10 + 2 * dxe * R00 * R10;
-        const float b22 = dxx * R01 * R01 + dee * R11 * R11 + 2 * dxe * R01 * R11;
-        const float b12 = dxx * R00 * R01 + dee * R10 * R11 + dxe * R01 * R10 + dxe * R00 * R11;
-        const v3f q = subf(subf(subf(addf(sclf(kG1, b1), sclf(kG2, b2)), sclf(sclf(ka3, ks3), b11)),
-                                sclf(sclf(ka3, ks4), b22)),
-                           sclf(sclf(ka3, ks5), b12));
-        const v3f ps = sclf(kG0, rb[i]);
-        qs[6 * k] = q.x; qs[6 * k + 1] = q.y; qs[6 * k + 2] = q.z;
-        qs[6 * k + 3] = ps.x; qs[6 * k + 4] = ps.y; qs[6 * k + 5] = ps.z;
+        const float c = rb[36 + i] * cxx + rb[48 + i] * cee + rb[60 + i] * cxe;
+        const v3f q = subf(addf(sclf(kVx, rb[12 + i]), sclf(kVe, rb[24 + i])), sclf(ka3, c));
+        qs[3 * k] = q.x; qs[3 * k + 1] = q.y; qs[3 * k + 2] = q.z;
       }
     }
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const int j = t + m * kElemBlock;
+      const int j = rec_of(t, m);
       if (j < nbn) {
-        const int r = b0 + j;
-        const uint32_t cnt = P.rent_gcnt[r];
-        int k0 = P.rent_start[r];
+        const uint32_t cnt = s_ngc[j];
+        int k0 = s_nst[j];
         for (int gg = 0; gg < g; ++gg) k0 += (cnt >> (8 * gg)) & 255;
         const int k1 = k0 + ((cnt >> (8 * g)) & 255);
         for (int k = k0; k < k1; ++k) {
-          const int ent = P.rent[k];
-          const float* qs = s_q + kQRec * (ent >> 4) + 6 * ((ent & 15) - 3 * g);
+          const int ent = s_ent[k], le = (ent >> 4) & 255, i = ent & 15;
+          const float* qs = s_q + kQRec * le + 3 * (i - 3 * g);
+          const float* g0 = s_g0 + 3 * le;
+          const float phi = s_rb[72 * (ent >> 12) + i];
           rq[m] = addf(rq[m], mkf(qs[0], qs[1], qs[2]));
-          rp[m] = addf(rp[m], mkf(qs[3], qs[4], qs[5]));
+          rp[m] = addf(rp[m], sclf(mkf(g0[0], g0[1], g0[2]), phi));
         }
       }
     }
@@ -445,7 +493,7 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   }
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
-    const int j = t + m * kElemBlock;
+    const int j = rec_of(t, m);
     if (j < nbn) {
       float* rec = P.brec32 + (size_t)kBRec * (b0 + j);
       rec[0] = rq[m].x; rec[1] = rq[m].y; rec[2] = rq[m].z; rec[3] = rp[m].x; rec[4] = rp[m].y; rec[5] = rp[m].z;
