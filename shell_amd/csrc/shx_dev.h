@@ -19,6 +19,38 @@ enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3, V_ELEM32 = 4 };
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+/* x of another lane by a DPP move (both 32-bit halves); lanes of rows
+ * outside RMASK get 0 */
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RMASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+/* sum (MAX false) or max (MAX true, of values >= 0) over the 64 lanes of a
+ * wave with every lane active, in a fixed order, without LDS: lane pairs and
+ * quads (quad_perm), half rows and rows (row_half_mirror, row_mirror), then
+ * rows 1 and 3 add the last lane of rows 0 and 2 (row_bcast:15) and rows 2
+ * and 3 that of row 1 (row_bcast:31); lane 63 holds the result, which every
+ * lane returns */
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double x)
+{
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  x = op(x, dpp_f64<0xB1, 0xF>(x));    /* quad_perm [1,0,3,2] */
+  x = op(x, dpp_f64<0x4E, 0xF>(x));    /* quad_perm [2,3,0,1] */
+  x = op(x, dpp_f64<0x141, 0xF>(x));   /* row_half_mirror */
+  x = op(x, dpp_f64<0x140, 0xF>(x));   /* row_mirror */
+  x = op(x, dpp_f64<0x142, 0xA>(x));   /* row_bcast:15 */
+  x = op(x, dpp_f64<0x143, 0xC>(x));   /* row_bcast:31 */
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ v3 ld3(const double* p, int n) { return mk(p[3 * n], p[3 * n + 1], p[3 * n + 2]); }
 __device__ __forceinline__ void st3(double* p, int n, v3 a) { p[3 * n] = a.x; p[3 * n + 1] = a.y; p[3 * n + 2] = a.z; }
 __device__ __forceinline__ v3 ldrow3(const double* t, int row, int nE, int e)

@@ -264,15 +264,10 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
       rec[0] = rq[m].x; rec[1] = rq[m].y; rec[2] = rq[m].z; rec[3] = rp[m].x; rec[4] = rp[m].y; rec[5] = rp[m].z;
     }
   }
-  /* block partial: wave butterflies, then the block's waves in order */
+  /* block partial: wave reductions (DPP), then the block's waves in order */
   double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int c = 0; c < kElemRec; ++c) {
-      const double o = __shfl_xor(rr[c], off);
-      rr[c] = (c == 2 || c == 5) ? fmax(rr[c], o) : rr[c] + o;
-    }
+  for (int c = 0; c < kElemRec; ++c) rr[c] = (c == 2 || c == 5) ? wave_reduce<true>(rr[c]) : wave_reduce<false>(rr[c]);
   if ((t & 63) == 0)
 #pragma unroll
     for (int c = 0; c < kElemRec; ++c) s_w[c][t / 64] = rr[c];
@@ -501,12 +496,7 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
   }
   double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int c = 0; c < kElemRec; ++c) {
-      const double o = __shfl_xor(rr[c], off);
-      rr[c] = (c == 2 || c == 5) ? fmax(rr[c], o) : rr[c] + o;
-    }
+  for (int c = 0; c < kElemRec; ++c) rr[c] = (c == 2 || c == 5) ? wave_reduce<true>(rr[c]) : wave_reduce<false>(rr[c]);
   if ((t & 63) == 0)
 #pragma unroll
     for (int c = 0; c < kElemRec; ++c) s_w[c][t / 64] = rr[c];

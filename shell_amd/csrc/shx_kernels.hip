@@ -464,9 +464,7 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
   const int t = threadIdx.x;
   double rr[kCRec] = {nc, es, ef, npair};
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int c = 0; c < kCRec; ++c) rr[c] += __shfl_xor(rr[c], off);
+  for (int c = 0; c < kCRec; ++c) rr[c] = wave_reduce<false>(rr[c]);
   if ((t & 63) == 0)
 #pragma unroll
     for (int c = 0; c < kCRec; ++c) s_w[c][t / 64] = rr[c];
@@ -703,14 +701,11 @@ __global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
       }
     }
   }
-  /* wave butterfly, then the block's waves in order */
+  /* wave reductions (DPP), then the block's waves in order */
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) {
     const bool mx = (q >= 2 && q <= 4) || q == 22;
-    for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_xor(rec[q], off);
-      rec[q] = mx ? fmax(rec[q], o) : rec[q] + o;
-    }
+    rec[q] = mx ? wave_reduce<true>(rec[q]) : wave_reduce<false>(rec[q]);
   }
   const int wave = t / 64;
   if ((t & 63) == 0)
