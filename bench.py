@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per element-kernel launch, from rocprofv3 PMC counters
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain copy; median
 # KiB per dispatch) on the same workload: profiles/hbm_traffic_pmc.md
-MEASURED_TRAFFIC = {"surrogate": (916304.9 * 2.0 + 381440.0) * 1024}
+MEASURED_TRAFFIC = {"surrogate": (732876.75 * 2.0 + 238080.0) * 1024}
 BASELINE_METRIC = "post-buckled equilibria/s at rel residual 1e-6, surrogate vs plain; force GB/s"
 METRIC = "post-buckled equilibria/s at rel residual 1e-6 (plain FP64 GPU method); force GB/s"
 
@@ -71,19 +71,22 @@ def force_blocks(T, nv, block=256):
 _BLOCKS = {}
 
 
-def element_kernel_bytes(T, nv):
-    """algorithmic HBM bytes of one element-force launch over a shell (FP64,
-    the benchmark's volume-at-prediction mode): per element 26 FP64 table rows
-    (R 2x2, JxW, A1 A2 A11 A22 A12, ff1, ff2), 12 uint16 patch-node indices, the
-    valence byte, the element area written (241 B); per (element, slot)
-    entry its uint16 id; per block-node record its node id, entry offset and
-    the 6 FP64 sums written (56 B); per node u, v, a and X read once (96 B;
-    the re-read of nodes shared by two blocks is not counted)."""
+def element_kernel_bytes(T, nv, trigger=False):
+    """algorithmic HBM bytes of one element-force step launch over a shell
+    (FP64, the benchmark's volume-at-prediction mode): per element 20 FP64
+    table rows (R 2x2, JxW, A1 A2 A11 A22 A12), 12 uint16 patch-node
+    indices, the valence byte, the element area written (193 B); per
+    (element, slot) entry its uint16 id; per block-node record its node id,
+    entry offset, slot-group counts and the 6 FP64 sums written (60 B); per
+    node u, v, a and X read once (96 B; the re-read of nodes shared by two
+    blocks is not counted), and with the displacement rebuild trigger of a
+    contact shell its build displacement (24 B)."""
     key = (len(T), nv, hash(np.asarray(T).tobytes()))
     if key not in _BLOCKS:
         _BLOCKS[key] = force_blocks(T, nv)
     records, entries = _BLOCKS[key]
-    return len(T) * (26 * 8 + 12 * 2 + 1 + 8) + entries * 2 + records * (4 + 4 + 6 * 8) + nv * 96
+    return (len(T) * (20 * 8 + 12 * 2 + 1 + 8) + entries * 2 + records * (4 + 4 + 4 + 6 * 8)
+            + nv * (96 + (24 if trigger else 0)))
 
 
 def single_workload(level=4):
@@ -113,7 +116,8 @@ def roofline(b, meshes, steps=200, traffic=None):
     kt = b.kernel_times()
     b.set_profiling(False)
     elem_ms = kt["elem_force"][0]
-    nbytes = sum(element_kernel_bytes(T, len(X)) for X, T in meshes)
+    nbytes = sum(element_kernel_bytes(T, len(X), bool(p.contact and p.rebuild_trigger))
+                 for (X, T), p in zip(meshes, b.params))
     achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_elem (element force)",
@@ -262,7 +266,7 @@ def run_single(args):
     kt = g.kernel_times()
     g.set_profiling(False)
     elem_ms = kt["elem_force"][0]
-    nbytes = element_kernel_bytes(T, len(X))
+    nbytes = element_kernel_bytes(T, len(X), bool(run.get("contact") and run.get("rebuild_trigger")))
     achieved = nbytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
     cpu = None
     if not args.no_cpu:

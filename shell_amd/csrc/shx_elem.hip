@@ -146,18 +146,25 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
   v3 ka3 = mk(0, 0, 0), kG1 = ka3, kG2 = ka3;
   double kJ = 0, ks3 = 0, ks4 = 0, ks5 = 0;
   if (valid) {
-    const v3 a1 = add(ldrow3(T, T_A1, nE, e), du1);
-    const v3 a2 = add(ldrow3(T, T_A2, nE, e), du2);
+    const v3 A1 = ldrow3(T, T_A1, nE, e), A2 = ldrow3(T, T_A2, nE, e);
+    const v3 A11 = ldrow3(T, T_A11, nE, e), A22 = ldrow3(T, T_A22, nE, e), A12 = ldrow3(T, T_A12, nE, e);
+    /* the reference metrics ff1_ref, ff2_ref (compute_reference_metrics,
+     * ShellSystem.C:1317-1379) are formed here from the reference tangents
+     * rather than read from the table: 48 bytes less per element */
+    v3 A3 = crs(A1, A2);
+    A3 = scl(A3, 1 / nrm(A3));
+    const v3 g = mk(dot(A1, A1), dot(A2, A2), dot(A1, A2));
+    const v3 fr2 = mk(dot(A11, A3), dot(A22, A3), dot(A12, A3));
+    const v3 a1 = add(A1, du1);
+    const v3 a2 = add(A2, du2);
     v3 a3 = crs(a1, a2);
     const double jac = nrm(a3);
     a3 = scl(a3, 1 / jac);
-    const v3 a11 = add(ldrow3(T, T_A11, nE, e), du11);
-    const v3 a22 = add(ldrow3(T, T_A22, nE, e), du22);
-    const v3 a12 = add(ldrow3(T, T_A12, nE, e), du12);
+    const v3 a11 = add(A11, du11);
+    const v3 a22 = add(A22, du22);
+    const v3 a12 = add(A12, du12);
     const double ff1[3] = {dot(a1, a1), dot(a2, a2), dot(a1, a2)};
     const double ff2[3] = {dot(a11, a3), dot(a22, a3), dot(a12, a3)};
-    const v3 g = ldrow3(T, T_FF1, nE, e);
-    const v3 fr2 = ldrow3(T, T_FF2, nE, e);
     double alpha[3] = {0.5 * (ff1[0] - g.x), 0.5 * (ff1[1] - g.y), 0.5 * (ff1[2] - g.z)};
     double beta[3] = {fr2.x - ff2[0], fr2.y - ff2[1], fr2.z - ff2[2]};
     alpha[2] *= 2;
