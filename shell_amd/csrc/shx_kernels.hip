@@ -154,9 +154,9 @@ __global__ void __launch_bounds__(256) k_mass(DPtrs P)
 /* ------------------------------------------------------- block utilities */
 
 /* deterministic reduction of NC components of nparts partial records by
- * one wave (64 lanes): lane l folds records l, l + 64, ... in order, then a
- * fixed xor butterfly (each step adds the same two values in every lane, so
- * all lanes end with bit-identical totals).  op[c]: 0 sum, 1 max. */
+ * one wave (64 lanes, all active): lane l folds records l, l + 64, ... in
+ * order, then the DPP wave reduction (fixed order; every lane returns the
+ * same total).  op[c]: 0 sum, 1 max (of values >= 0). */
 template <int NC>
 __device__ void wave_reduce_records(const double* part, int nparts, int stride, const int* op, double* out)
 {
@@ -170,14 +170,7 @@ __device__ void wave_reduce_records(const double* part, int nparts, int stride, 
     for (int c = 0; c < NC; ++c) acc[c] = op[c] ? fmax(acc[c], r[c]) : acc[c] + r[c];
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const double o = __shfl_xor(acc[c], off);
-      acc[c] = op[c] ? fmax(acc[c], o) : acc[c] + o;
-    }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) out[c] = acc[c];
+  for (int c = 0; c < NC; ++c) out[c] = op[c] ? wave_reduce<true>(acc[c]) : wave_reduce<false>(acc[c]);
 }
 
 __device__ void set_newmark_dev(DShell& s, const SConst& C, double dt)
@@ -358,10 +351,12 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
   return add(x, u);
 }
 
-/* add_shell_shell_contact_force, ShellSystem.C:1792-1907: one thread per
- * element e1 walks its candidate row (e2 > e1): distance, log-barrier normal
- * force, normal damping and Margolis friction; the force is split
- * -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w) onto element 2 and
+/* add_shell_shell_contact_force, ShellSystem.C:1792-1907: the candidate
+ * pairs (e1, e2 > e1) of a 256-element tile, its rows laid end to end, are
+ * split into 256 equal runs, one per thread (so a long row does not hold up
+ * its wave; a run mostly stays within one or two rows): distance,
+ * log-barrier normal force, normal damping and Margolis friction; the force
+ * is split -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w) onto element 2 and
  * accumulated with FP64 atomics into fc (the node kernel reads and clears
  * it).  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
  * a shell whose list was rebuilt in this step clear its cell buckets.
@@ -369,6 +364,9 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
 __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
 {
   __shared__ double s_w[kCRec][4];
+  __shared__ int s_pre[257];             /* exclusive prefix of the tile's row lengths */
+  __shared__ double s_x0[9][256];        /* per row with pairs: its element's vertex positions */
+  __shared__ int s_n0[3][256];           /* and vertex ids */
   const int tile = uni(P.bmap[V_TILE][blockIdx.x]);
   const int sid = uni(P.eshell[tile * 256]);
   const SConst& C = P.cs[sid];
@@ -386,19 +384,59 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
   const int buf = sh.pair_buf;
   const int nE = P.nE;
   const double h = C.h;
-  const int e0 = tile * 256 + threadIdx.x;
+  const int t = threadIdx.x;
   double nc = 0, es = 0, ef = 0, npair = 0;
-  if (e0 - C.ebase < C.ne) {
-    const int np = min(P.rcnt[buf][e0], kRowCap);
+  {
+    const int e = tile * 256 + t;
+    const int np = e - C.ebase < C.ne ? min(P.rcnt[buf][e], kRowCap) : 0;
     npair = np;
-    int nid0[3];
-    v3 X0[3];
     if (np > 0)
       for (int k = 0; k < 3; ++k) {
-        nid0[k] = P.tri[(size_t)k * nE + e0];
-        X0[k] = node_pos(P, nid0[k], U, V, A, predict, dt, a8);
+        const int n = P.tri[(size_t)k * nE + e];
+        const v3 x = node_pos(P, n, U, V, A, predict, dt, a8);
+        s_n0[k][t] = n;
+        s_x0[3 * k][t] = x.x; s_x0[3 * k + 1][t] = x.y; s_x0[3 * k + 2][t] = x.z;
       }
-    for (int q = 0; q < np; ++q) {
+    /* exclusive scan of the row lengths: wave scans, then the 4 wave totals */
+    int x = np;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if ((t & 63) >= off) x += y;
+    }
+    s_pre[t + 1] = x;
+    __syncthreads();
+    if (t == 0) {
+      s_pre[0] = 0;
+      for (int w = 1; w < 4; ++w) s_pre[64 * w + 64] += s_pre[64 * w];
+    }
+    __syncthreads();
+    if (t >= 64) s_pre[t + 1] += (t & 63) == 63 ? 0 : s_pre[64 * (t >> 6)];
+    __syncthreads();
+  }
+  const int total = s_pre[256];
+  const int run = (total + 255) / 256;
+  const int k0 = t * run, k1 = min(total, k0 + run);
+  int lo = 0;
+  if (k0 < k1) {
+    /* row of pair k0: the last r with s_pre[r] <= k0 */
+    int hi = 255;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= k0) lo = mid; else hi = mid - 1;
+    }
+  }
+  for (int kp = k0; kp < k1; ++kp) {
+    while (kp >= s_pre[lo + 1]) ++lo;    /* kp < total = s_pre[256]: lo stays below 256 */
+    const int q = kp - s_pre[lo];
+    const int e0 = tile * 256 + lo;
+    int nid0[3];
+    v3 X0[3];
+    for (int k = 0; k < 3; ++k) {
+      nid0[k] = s_n0[k][lo];
+      X0[k] = mk(s_x0[3 * k][lo], s_x0[3 * k + 1][lo], s_x0[3 * k + 2][lo]);
+    }
+    {
       const size_t p = (size_t)e0 * kRowCap + q;
       const int e1 = P.pe2[buf][p];
       v3 X1[3];
@@ -461,7 +499,6 @@ __global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
       st3(P.pso[buf], p, so);
     }
   }
-  const int t = threadIdx.x;
   double rr[kCRec] = {nc, es, ef, npair};
 #pragma unroll
   for (int c = 0; c < kCRec; ++c) rr[c] = wave_reduce<false>(rr[c]);
