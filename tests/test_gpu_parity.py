@@ -107,6 +107,9 @@ def _pancake(X, s):
 
 @pytest.mark.parametrize("level,squash", [(3, 0.006), (4, 0.008)])
 def test_contact_pairs_bitexact(icosphere, level, squash):
+    """pair lists of a pancaked shell bit-exact, contact and total force to
+    RTOL; its 256-element tiles hold up to 2173 (L3) and 3345 (L4) pairs, so
+    every thread of the contact kernel walks a run of pairs across rows"""
     say("case contact pairs level", level)
     X, T = icosphere(level)
     kw = dict(contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0)
