@@ -642,7 +642,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
  * the rigid-body sums, for k_fin_node.  Launch block b computes node block
  * bmap[V_NODE][b].  Force time: t + dt in a step, t + dt in M_INIT
  * (apply_initial_conditions), DShell::t_eval in M_FORCES. */
-__global__ void __launch_bounds__(kNodeBlock) k_node(DPtrs P, int mode)
+__global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
 {
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
   const int blk = uni(P.bmap[V_NODE][blockIdx.x]);
