@@ -16,11 +16,12 @@
  *  - the averaged element size is summed in blocks of 256 elements (serial
  *    inside a block, then serial over blocks) so that a GPU can reproduce it
  *    bit for bit.
- * Parity unpinned (no closed form or reference data to check against):
- *  - the adaptive time-step control of advance (ShellSystem.C:1027-1047),
- *    held only by agreement with the GPU path;
- *  - the residual of the deformation (rigid-free residual), a build
- *    extension with no counterpart in the reference.
+ * Pins without reference data: the adaptive time-step control of advance
+ * (ShellSystem.C:1027-1047) by a known answer (a damped rigid translation,
+ * whose local-error estimate has a closed form), the residual of the
+ * deformation (rigid-free residual, a build extension with no counterpart in
+ * the reference) by an independent least-squares solve; both in
+ * tests/test_oracle_pins.py.
  */
 #include "shell_oracle.h"
 #include "oracle_geom.h"

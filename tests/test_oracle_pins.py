@@ -493,3 +493,74 @@ def test_displacement_trigger_is_safe_and_rebuilds_less(icosphere):
     assert runs[1][2] > 0 and np.all(np.isfinite(runs[1][0]))
     assert np.array_equal(runs[0][0], runs[1][0])
     assert 0 < runs[1][1] < runs[0][1], (runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("band", ["above", "below"])
+def test_adaptive_dt_known_answer(icosphere, band):
+    """the step control of advance (ShellSystem.C:1027-1047) on a motion
+    whose local-error estimate is known in closed form: a rigid translation
+    v0 under mass-proportional damping, from the consistent acceleration
+    a0 = -c v0 (c = M^-1 C per DOF, no elastic or external force).  One step
+    of the scheme gives v_pred = v0 + (1 - gamma) dt a0 and a1 = -c v_pred, so
+    ||a1 - a0||_inf = max c^2 (1 - gamma) dt |v0| and the estimate
+    rel = dt^2 |beta - 1/6| ||a1 - a0||_inf / h grows as dt^3.  Hence: above
+    the band the attempt is rejected once, and the retry at
+    dt1 = dt (sqrt(min max) / rel)^(1/3) lands on the band's centre and is
+    accepted; below the band the step is accepted at once and dt grows to the
+    same formula.  A wrong power of dt, exponent, factor or the target would
+    change the rejection count or dt1."""
+    X, T = icosphere(2)
+    alpha, beta, gamma, dt0 = 0.5, 0.0, 0.5, 0.1
+    v0 = np.zeros_like(X)
+    v0[:, 2] = 0.01                       # dt |v0| < h / 2: no large-increment problem
+    v0 = v0.reshape(-1)
+    probe = O.Oracle(X, T, damping_alpha=alpha)
+    Minv, C = probe.mass()
+    probe.close()
+    c = Minv * C
+    rel0 = dt0 ** 2 * abs(beta - 1 / 6) * np.max(c ** 2 * (1 - gamma) * dt0 * np.abs(v0)) / H
+    lo, hi = (rel0 / 20, rel0 / 10) if band == "above" else (10 * rel0, 20 * rel0)
+    o = O.Oracle(X, T, damping_alpha=alpha, newmark_beta=beta, newmark_gamma=gamma, dt=dt0,
+                 adapt_dt_every=1, rel_loc_err_min=lo, rel_loc_err_max=hi)
+    o.set_state(np.zeros_like(v0), v0, -c * v0, 0.0)
+    r = o.step(1)
+    dt1 = dt0 * (np.sqrt(lo * hi) / rel0) ** (1 / 3)
+    assert r["n_rejected"] == (1 if band == "above" else 0)
+    assert abs(r["dt"] - dt1) <= 1e-9 * dt1
+    assert abs(r["t"] - (dt1 if band == "above" else dt0)) <= 1e-9 * dt0
+    assert r["problem"] == 0
+    o.close()
+
+
+def test_rigid_free_residual_is_the_least_squares_remainder(icosphere):
+    """the deformation residual (the force minus its least-squares fit by the
+    rigid-body inertial forces m_i (t + w x x_i) at the force-evaluation
+    positions; an extension with no counterpart in the reference) equals the
+    remainder of a dense least-squares solve of the same force, on a strongly
+    imperfect shell under pressure whose force has a rigid part (about 1 %
+    of its norm, from the discrete pressure resultant)"""
+    X, T = icosphere(2)
+    X = imperfect(X, 0.2, seed=3)
+    dt = 0.01
+    o = O.Oracle(X, T, pressure_final=-1.5 * PCR, dt=dt)
+    u = np.random.default_rng(0).normal(size=X.size) * 1e-3
+    z = np.zeros(X.size)
+    o.set_state(u, z, z, 0.0)
+    r = o.step(1)                          # v = a = 0: the force is evaluated at u, t = dt
+    f, fext, _, _ = o.forces(u, t=dt)
+    Minv, _ = o.mass()
+    o.close()
+    m = 1 / Minv.reshape(-1, 3)[:, 0]
+    x = X + u.reshape(-1, 3)
+    B = np.zeros((X.size, 6))
+    for i in range(len(X)):
+        xi = x[i]
+        cross = np.array([[0, -xi[2], xi[1]], [xi[2], 0, -xi[0]], [-xi[1], xi[0], 0]])
+        B[3 * i:3 * i + 3, :3] = m[i] * np.eye(3)
+        B[3 * i:3 * i + 3, 3:] = -m[i] * cross   # w x x_i = -[x_i]x w
+    q, *_ = np.linalg.lstsq(B, f, rcond=None)
+    res = np.linalg.norm(f - B @ q)
+    fn = np.linalg.norm(f)
+    assert fn - res > 1e-5 * fn            # the fit removes a material rigid part
+    assert abs(r["fext_norm"] - np.linalg.norm(fext)) <= 1e-12 * r["fext_norm"]
+    assert abs(r["residual_deform"] * r["fext_norm"] - res) <= 1e-12 * fn
