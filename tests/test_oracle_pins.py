@@ -495,7 +495,7 @@ def test_displacement_trigger_is_safe_and_rebuilds_less(icosphere):
     assert 0 < runs[1][1] < runs[0][1], (runs[0][1], runs[1][1])
 
 
-@pytest.mark.parametrize("band", ["above", "below"])
+@pytest.mark.parametrize("band", ["above", "below", "clamp"])
 def test_adaptive_dt_known_answer(icosphere, band):
     """the step control of advance (ShellSystem.C:1027-1047) on a motion
     whose local-error estimate is known in closed form: a rigid translation
@@ -508,7 +508,8 @@ def test_adaptive_dt_known_answer(icosphere, band):
     dt1 = dt (sqrt(min max) / rel)^(1/3) lands on the band's centre and is
     accepted; below the band the step is accepted at once and dt grows to the
     same formula.  A wrong power of dt, exponent, factor or the target would
-    change the rejection count or dt1."""
+    change the rejection count or dt1.  With contact, the velocity clamp of
+    the step (dt <= h / (20 v_max)) is checked the same way."""
     X, T = icosphere(2)
     alpha, beta, gamma, dt0 = 0.5, 0.0, 0.5, 0.1
     v0 = np.zeros_like(X)
@@ -519,6 +520,19 @@ def test_adaptive_dt_known_answer(icosphere, band):
     probe.close()
     c = Minv * C
     rel0 = dt0 ** 2 * abs(beta - 1 / 6) * np.max(c ** 2 * (1 - gamma) * dt0 * np.abs(v0)) / H
+    if band == "clamp":
+        # with contact the step is also bounded by dt v_max 2 * 10 <= h (contact_resolution
+        # 10, v_max = max |v_pred|); with gamma = 1, v_pred = v0 whatever dt, so the first
+        # attempt is rejected and the retry at h / (20 max |v0|) is accepted with that dt
+        o = O.Oracle(X, T, damping_alpha=alpha, newmark_beta=beta, newmark_gamma=1.0, dt=dt0,
+                     adapt_dt_every=1, rel_loc_err_min=rel0 / 20, rel_loc_err_max=rel0 / 10, contact=1)
+        o.set_state(np.zeros_like(v0), v0, -c * v0, 0.0)
+        r = o.step(1)
+        o.close()
+        dt1 = H / (20 * np.max(np.abs(v0)))
+        assert r["n_rejected"] == 1 and r["problem"] == 0
+        assert abs(r["t"] - dt1) <= 1e-12 * dt1 and abs(r["dt"] - dt1) <= 1e-12 * dt1
+        return
     lo, hi = (rel0 / 20, rel0 / 10) if band == "above" else (10 * rel0, 20 * rel0)
     o = O.Oracle(X, T, damping_alpha=alpha, newmark_beta=beta, newmark_gamma=gamma, dt=dt0,
                  adapt_dt_every=1, rel_loc_err_min=lo, rel_loc_err_max=hi)
