@@ -578,3 +578,40 @@ def test_rigid_free_residual_is_the_least_squares_remainder(icosphere):
     assert fn - res > 1e-5 * fn            # the fit removes a material rigid part
     assert abs(r["fext_norm"] - np.linalg.norm(fext)) <= 1e-12 * r["fext_norm"]
     assert abs(r["residual_deform"] * r["fext_norm"] - res) <= 1e-12 * fn
+
+
+def test_close_sets_are_the_cutoff_closure(icosphere):
+    """close sets (build_topological_neighborhood, ShellSystem.C:684-741) by
+    their defining properties, on a coarse mesh whose cutoff (1 + margin) h
+    reaches several rings out (so the search must iterate): each holds every element sharing a node and
+    not the element itself; every other member lies within the cutoff of the
+    element (triangle distance on the undeformed mesh) and shares a node
+    with another member; and no element that shares a node with a member and
+    lies within the cutoff is missing"""
+    X, T = icosphere(2)
+    h, margin = 0.4, 0.4
+    o = O.Oracle(X, T, thickness=h, neighbor_margin=margin, contact=1)
+    start, ids = o.close_sets()
+    o.close()
+    cutoff = (1 + margin) * h
+    node_elems = [[] for _ in range(len(X))]
+    for e, t in enumerate(T):
+        for n in t:
+            node_elems[n].append(e)
+    nbr = [set(x for n in t for x in node_elems[n]) - {e} for e, t in enumerate(T)]
+
+    def dist(a, b):
+        return np.sqrt(max(O.tri_distance(np.concatenate([X[T[a]], X[T[b]]]).reshape(-1))[0], 0.0))
+
+    beyond = 0
+    for e in range(len(T)):
+        close = set(ids[start[e]:start[e + 1]].tolist())
+        assert e not in close and nbr[e] <= close
+        for c in close - nbr[e]:
+            beyond += 1
+            assert dist(e, c) < cutoff
+            assert nbr[c] & (close | {e})
+        frontier = set().union(*(nbr[c] for c in close)) - close - {e}
+        for c in frontier:
+            assert dist(e, c) >= cutoff
+    assert beyond > 0                      # the closure reaches past the one-ring
