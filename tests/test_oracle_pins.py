@@ -615,3 +615,27 @@ def test_close_sets_are_the_cutoff_closure(icosphere):
         for c in frontier:
             assert dist(e, c) >= cutoff
     assert beyond > 0                      # the closure reaches past the one-ring
+
+
+def test_damping_is_alpha_mass_plus_gamma_and_linear(icosphere):
+    """compute_mass_damping (ShellSystem.C:762-852): the damping vector is
+    alpha times the lumped mass plus gamma per DOF (rho h != 1, so a
+    damping formed from the unscaled element mass fails), and linear in
+    (alpha, beta, gamma)"""
+    X, T = icosphere(2)
+    base = dict(rho=2.5, thickness=0.02)
+
+    def damping(a, b, g):
+        o = O.Oracle(X, T, damping_alpha=a, damping_beta=b, damping_gamma=g, **base)
+        Minv, C = o.mass()
+        o.close()
+        return Minv, C
+
+    Minv, C = damping(0.3, 0.0, 0.0)
+    assert np.allclose(C * Minv, 0.3, rtol=1e-14, atol=0)
+    _, Cg = damping(0.0, 0.0, 0.7)
+    assert np.array_equal(Cg, np.full_like(Cg, 0.7))
+    _, Cb = damping(0.0, 0.2, 0.0)
+    assert np.all(Cb > 0)
+    _, Call = damping(0.3, 0.2, 0.7)
+    assert np.allclose(Call, C + Cb + Cg, rtol=1e-14, atol=0)
