@@ -22,6 +22,8 @@
  * deformation (rigid-free residual, a build extension with no counterpart in
  * the reference) by an independent least-squares solve; both in
  * tests/test_oracle_pins.py.
+ * Parity unpinned: the stiffness-proportional (beta) term of the damping in
+ * compute_mass_damping, which no workload or test sets.
  */
 #include "shell_oracle.h"
 #include "oracle_geom.h"
