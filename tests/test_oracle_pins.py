@@ -639,3 +639,28 @@ def test_damping_is_alpha_mass_plus_gamma_and_linear(icosphere):
     assert np.all(Cb > 0)
     _, Call = damping(0.3, 0.2, 0.7)
     assert np.allclose(Call, C + Cb + Cg, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("E1,E2", [(1.0, 0.5), (0.7, 1.3), (1.0, 1.0)])
+def test_dilation_energies_closed_form(icosphere, E1, E2):
+    """a uniform dilation u = eps X maps every tangent and second derivative
+    to (1 + eps) times itself (the basis reproduces X), so the membrane
+    strain is the equibiaxial Green strain E = eps + eps^2 / 2 times the
+    reference metric and a3 is unchanged.  In the element's orthonormal
+    frame the reference's H (ShellSystem.C:1199-1213) is [[E1, nu E1, .],
+    [nu E1, E2, .], [., ., G12 ...]], so with K = h / (1 - nu nu21),
+    nu21 = E2 nu / E1 (update_derived, ShellSystem.C:421-487):
+      E_tens = 1/2 K E^2 (E1 + E2 + 2 nu E1) * area   (exactly),
+      E_bend = 1/2 D eps^2 (E1 + E2 + 2 nu E1) / R^2 * area, D = h^3 / 12 ...
+    with R = 1 up to the limit surface's deviation from the unit sphere."""
+    X, T = icosphere(3)
+    h, nu = 0.01, 0.3
+    o = O.Oracle(X, T, E1=E1, E2=E2, nu=nu, G12=0.35, thickness=h)
+    area = o.forces(np.zeros(X.size))[3].sum()
+    c = (E1 + E2 + 2 * nu * E1) / (1 - nu * (E2 * nu / E1))
+    for eps in (1e-3, 2e-2):
+        en = o.forces((eps * X).reshape(-1))[2]
+        G = eps + eps * eps / 2
+        assert abs(en[0] - 0.5 * h * G ** 2 * c * area) <= 1e-12 * en[0]
+        assert abs(en[1] - 0.5 * h ** 3 / 12 * eps ** 2 * c * area) <= 0.03 * en[1]
+    o.close()
