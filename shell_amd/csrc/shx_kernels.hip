@@ -361,7 +361,7 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
  * it).  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
  * a shell whose list was rebuilt in this step clear its cell buckets.
  * Launch block b takes 256-element tile bmap[V_TILE][b]. */
-__global__ void __launch_bounds__(256) k_contact(DPtrs P, int mode)
+__global__ void __launch_bounds__(256, 3) k_contact(DPtrs P, int mode)
 {
   __shared__ double s_w[kCRec][4];
   __shared__ int s_pre[257];             /* exclusive prefix of the tile's row lengths */
