@@ -185,7 +185,7 @@ struct DPtrs {
   double* urb;                /* nV*3 displacement each node's contact list was built at (rebuild trigger 1) */
   double* fout;               /* nV*3 outputs of a plain force evaluation */
   double* fext_out;
-  double* epart;              /* element kernel block partials (5 per block) */
+  double* epart;              /* element kernel block partials (kElemRec per block) */
   double* npart;              /* node kernel block partials (kNodeRec per block) */
   double* cpart;              /* contact kernel block partials (kCRec per 256-element tile) */
   /* contact */
