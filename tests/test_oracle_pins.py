@@ -664,3 +664,25 @@ def test_dilation_energies_closed_form(icosphere, E1, E2):
         assert abs(en[0] - 0.5 * h * G ** 2 * c * area) <= 1e-12 * en[0]
         assert abs(en[1] - 0.5 * h ** 3 / 12 * eps ** 2 * c * area) <= 0.03 * en[1]
     o.close()
+
+
+def test_volume_constraint_is_a_uniform_pressure_at_init(icosphere):
+    """the volume term of compute_external_force (ShellSystem.C:1467-1585)
+    is the pressure kappa (V0 r / V - 1) on the same follower load path as
+    the applied pressure: right after initialisation V = V0, so a shell with
+    bulk modulus kappa and target ratio r loads exactly like one under the
+    pressure kappa (r - 1)"""
+    X, T = icosphere(3)
+    X = imperfect(X, 0.01 * H, seed=5)
+    u = np.random.default_rng(2).normal(size=X.size) * 1e-3
+    z = np.zeros(X.size)
+    kappa, r = 0.05, 0.9
+    out = []
+    for kw in (dict(bulk_modulus=kappa, volume_final_ratio=r), dict(pressure_final=kappa * (r - 1))):
+        o = O.Oracle(X, T, **kw)
+        o.set_state(u, z, z, 0.0)
+        o.init_dynamics()
+        out.append(o.forces(u, t=0.0)[1])
+        o.close()
+    assert np.linalg.norm(out[0]) > 0
+    assert np.linalg.norm(out[0] - out[1]) <= 1e-13 * np.linalg.norm(out[1])
