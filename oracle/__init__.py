@@ -26,6 +26,14 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def load(path):
+    """a handle on an oracle library at `path` (the tests load copies built
+    with a planted slip through this, to show that a pin catches it)"""
+    L = ctypes.CDLL(path)
+    _declare(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -148,18 +156,19 @@ def default_params():
 class Oracle:
     """Handle on one oracle shell system (FP64, CPU)."""
 
-    def __init__(self, X, T, **params):
+    def __init__(self, X, T, lib_handle=None, **params):
+        self.L = lib_handle if lib_handle is not None else lib()
         self.X = np.ascontiguousarray(X, dtype=np.float64)
         self.T = np.ascontiguousarray(T, dtype=np.int32)
         self.nv, self.ne = len(self.X), len(self.T)
         self.params = make_params(**params)
-        self.h = lib().orc_create(self.nv, self.ne, dp(self.X), ip(self.T), ctypes.byref(self.params))
+        self.h = self.L.orc_create(self.nv, self.ne, dp(self.X), ip(self.T), ctypes.byref(self.params))
         if not self.h:
-            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+            raise RuntimeError("oracle: " + self.L.orc_last_error().decode())
 
     def close(self):
         if self.h:
-            lib().orc_destroy(self.h)
+            self.L.orc_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -171,20 +180,20 @@ class Oracle:
     def patches(self):
         idx = np.zeros((self.ne, 12), dtype=np.int32)
         tri = np.zeros((self.ne, 3), dtype=np.int32)
-        lib().orc_patches(self.h, ip(idx), ip(tri))
+        self.L.orc_patches(self.h, ip(idx), ip(tri))
         N = np.zeros(self.ne, dtype=np.int32)
-        lib().orc_patch_size(self.h, ip(N))
+        self.L.orc_patch_size(self.h, ip(N))
         return idx, tri, N
 
     def elem_tables(self):
         out = np.zeros((self.ne, 101))
-        lib().orc_elem_tables(self.h, dp(out))
+        self.L.orc_elem_tables(self.h, dp(out))
         return out
 
     def mass(self):
         Minv = np.zeros(3 * self.nv)
         C = np.zeros(3 * self.nv)
-        lib().orc_mass(self.h, dp(Minv), dp(C))
+        self.L.orc_mass(self.h, dp(Minv), dp(C))
         return Minv, C
 
     def set_state(self, u, v=None, a=None, t=0.0):
@@ -192,12 +201,12 @@ class Oracle:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
         v = z if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         a = z if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
-        lib().orc_set_state(self.h, dp(u), dp(v), dp(a), t)
+        self.L.orc_set_state(self.h, dp(u), dp(v), dp(a), t)
 
     def get_state(self):
         u, v, a = (np.zeros(3 * self.nv) for _ in range(3))
         t = ctypes.c_double()
-        lib().orc_get_state(self.h, dp(u), dp(v), dp(a), ctypes.byref(t))
+        self.L.orc_get_state(self.h, dp(u), dp(v), dp(a), ctypes.byref(t))
         return u, v, a, t.value
 
     def forces(self, u_pred, v=None, t=0.0):
@@ -207,51 +216,51 @@ class Oracle:
         fext = np.zeros(3 * self.nv)
         en = np.zeros(4)
         area = np.zeros(self.ne)
-        lib().orc_forces(self.h, dp(u_pred), dp(vv) if vv is not None else None, t, dp(f), dp(fext),
+        self.L.orc_forces(self.h, dp(u_pred), dp(vv) if vv is not None else None, t, dp(f), dp(fext),
                          dp(en), dp(area))
         return f, fext, en, area
 
     def internal_force(self, u_pred):
         u_pred = np.ascontiguousarray(u_pred, dtype=np.float64).reshape(-1)
         f = np.zeros(3 * self.nv)
-        lib().orc_internal_force(self.h, dp(u_pred), dp(f))
+        self.L.orc_internal_force(self.h, dp(u_pred), dp(f))
         return f
 
     def init_dynamics(self):
-        lib().orc_init_dynamics(self.h)
+        self.L.orc_init_dynamics(self.h)
 
     def step(self, n=1):
         rep = Report()
-        lib().orc_step(self.h, n, ctypes.byref(rep))
+        self.L.orc_step(self.h, n, ctypes.byref(rep))
         return {k: getattr(rep, k) for k, _ in Report._fields_}
 
     def rebuild(self, force=True):
-        return lib().orc_contact_rebuild(self.h, 1 if force else 0)
+        return self.L.orc_contact_rebuild(self.h, 1 if force else 0)
 
     def pairs(self):
-        n = lib().orc_pairs(self.h, None, None, 0)
+        n = self.L.orc_pairs(self.h, None, None, 0)
         e1 = np.zeros(max(n, 1), dtype=np.uint32)
         e2 = np.zeros(max(n, 1), dtype=np.uint32)
-        lib().orc_pairs(self.h, up_(e1), up_(e2), n)
+        self.L.orc_pairs(self.h, up_(e1), up_(e2), n)
         return e1[:n], e2[:n]
 
     def pair_state(self):
-        n = lib().orc_pairs(self.h, None, None, 0)
+        n = self.L.orc_pairs(self.h, None, None, 0)
         d2 = np.zeros(max(n, 1))
         st = np.zeros(max(n, 1), dtype=np.int32)
         so = np.zeros((max(n, 1), 3))
-        lib().orc_pair_state(self.h, dp(d2), ip(st), dp(so), n)
+        self.L.orc_pair_state(self.h, dp(d2), ip(st), dp(so), n)
         return d2[:n], st[:n], so[:n]
 
     def close_sets(self):
         start = np.zeros(self.ne + 1, dtype=np.int32)
-        n = lib().orc_close_sets(self.h, ip(start), None, 0)
+        n = self.L.orc_close_sets(self.h, ip(start), None, 0)
         ids = np.zeros(max(n, 1), dtype=np.int32)
-        lib().orc_close_sets(self.h, ip(start), ip(ids), n)
+        self.L.orc_close_sets(self.h, ip(start), ip(ids), n)
         return start, ids[:n]
 
     def cfl_omega_max(self, iters=60, seed=1):
-        return lib().orc_cfl_omega_max(self.h, iters, seed)
+        return self.L.orc_cfl_omega_max(self.h, iters, seed)
 
 
 def stam_basis(v, w):

@@ -22,8 +22,11 @@
  * deformation (rigid-free residual, a build extension with no counterpart in
  * the reference) by an independent least-squares solve; both in
  * tests/test_oracle_pins.py.
- * Parity unpinned: the stiffness-proportional (beta) term of the damping in
- * compute_mass_damping, which no workload or test sets.
+ * The stiffness-proportional (beta) term of the damping in
+ * compute_mass_damping is pinned by an independent brute-force
+ * patch-incidence sum, and the orthotropic shear stiffness (H22) by the
+ * closed-form membrane energy of a torsion about the polar axis; copies of
+ * this file with a planted slip in either fail those pins.
  */
 #include "shell_oracle.h"
 #include "oracle_geom.h"

@@ -617,28 +617,132 @@ def test_close_sets_are_the_cutoff_closure(icosphere):
     assert beyond > 0                      # the closure reaches past the one-ring
 
 
-def test_damping_is_alpha_mass_plus_gamma_and_linear(icosphere):
+def _damping(X, T, a, b, g, handle=None, **base):
+    o = O.Oracle(X, T, lib_handle=handle, damping_alpha=a, damping_beta=b, damping_gamma=g, **base)
+    Minv, C = o.mass()
+    o.close()
+    return Minv, C
+
+
+def _beta_damping_brute(X, T, h, nu, E1, E2, beta):
+    """the stiffness-proportional damping of compute_mass_damping
+    (ShellSystem.C:828-833) by brute force: every element adds
+    beta * 2 (K + D / (hmin hmax)) / 3 to each node of its one-ring patch
+    (the vertices of all triangles that touch one of its three vertices),
+    hmin / hmax its shortest and longest edge (libMesh Elem::hmin/hmax of a
+    triangle: the extreme vertex distances), K = h / (1 - nu nu21),
+    D = h^3 / (12 (1 - nu nu21)), nu21 = E2 nu / E1 (ShellSystem.C:473-476)"""
+    nu21 = E2 * nu / E1
+    K = h / (1 - nu * nu21)
+    D = h ** 3 / (12 * (1 - nu * nu21))
+    node_tris = [[] for _ in range(len(X))]
+    for e, t in enumerate(T):
+        for n in t:
+            node_tris[n].append(e)
+    C = np.zeros(len(X))
+    for e, t in enumerate(T):
+        patch = set()
+        for n in t:
+            for f in node_tris[n]:
+                patch.update(T[f].tolist())
+        edges = [np.linalg.norm(X[t[i]] - X[t[j]]) for i, j in ((0, 1), (0, 2), (1, 2))]
+        stiff = 2 * (K + D / (min(edges) * max(edges))) / 3
+        for n in patch:
+            C[n] += beta * stiff
+    return np.repeat(C, 3)
+
+
+def test_damping_alpha_beta_gamma(icosphere):
     """compute_mass_damping (ShellSystem.C:762-852): the damping vector is
-    alpha times the lumped mass plus gamma per DOF (rho h != 1, so a
-    damping formed from the unscaled element mass fails), and linear in
-    (alpha, beta, gamma)"""
+    alpha times the lumped mass (rho h != 1, so a damping formed from the
+    unscaled element mass fails) plus the beta term formed by brute force
+    (an independent patch-incidence sum) plus gamma per DOF, and linear in
+    (alpha, beta, gamma); an imperfect mesh, so that the edge lengths differ
+    from element to element"""
     X, T = icosphere(2)
-    base = dict(rho=2.5, thickness=0.02)
-
-    def damping(a, b, g):
-        o = O.Oracle(X, T, damping_alpha=a, damping_beta=b, damping_gamma=g, **base)
-        Minv, C = o.mass()
-        o.close()
-        return Minv, C
-
-    Minv, C = damping(0.3, 0.0, 0.0)
+    X = imperfect(X, 0.05, seed=3)
+    base = dict(rho=2.5, thickness=0.02, nu=0.3, E1=1.0, E2=0.5)
+    Minv, C = _damping(X, T, 0.3, 0.0, 0.0, **base)
     assert np.allclose(C * Minv, 0.3, rtol=1e-14, atol=0)
-    _, Cg = damping(0.0, 0.0, 0.7)
+    _, Cg = _damping(X, T, 0.0, 0.0, 0.7, **base)
     assert np.array_equal(Cg, np.full_like(Cg, 0.7))
-    _, Cb = damping(0.0, 0.2, 0.0)
-    assert np.all(Cb > 0)
-    _, Call = damping(0.3, 0.2, 0.7)
+    _, Cb = _damping(X, T, 0.0, 0.2, 0.0, **base)
+    brute = _beta_damping_brute(X, T, 0.02, 0.3, 1.0, 0.5, 0.2)
+    assert np.allclose(Cb, brute, rtol=1e-13, atol=0)
+    _, Call = _damping(X, T, 0.3, 0.2, 0.7, **base)
     assert np.allclose(Call, C + Cb + Cg, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("slip", ["hmin*hmin", "hmax only", "K without 2/3"])
+def test_damping_beta_pin_catches_planted_slips(icosphere, slip):
+    """the brute-force beta pin fails for an oracle copy with a plausible
+    slip in the beta term (planted in a copy, on the CPU)"""
+    from oracle_mutants import build
+    old = "const double avg_stiff = 2 * (s.K + s.D / (hmin * hmax)) / 3;"
+    new = {"hmin*hmin": "const double avg_stiff = 2 * (s.K + s.D / (hmin * hmin)) / 3;",
+           "hmax only": "const double avg_stiff = 2 * (s.K + s.D / (hmax * hmax)) / 3;",
+           "K without 2/3": "const double avg_stiff = (s.K + s.D / (hmin * hmax));"}[slip]
+    L = build(old, new, slip.replace(" ", "_").replace("*", "x").replace("/", "_"))
+    X, T = icosphere(2)
+    X = imperfect(X, 0.05, seed=3)
+    _, Cb = _damping(X, T, 0.0, 0.2, 0.0, handle=L, rho=2.5, thickness=0.02, nu=0.3, E1=1.0, E2=0.5)
+    brute = _beta_damping_brute(X, T, 0.02, 0.3, 1.0, 0.5, 0.2)
+    assert not np.allclose(Cb, brute, rtol=1e-6, atol=0)
+
+
+def _torsion_energy_ratio(icosphere, level, handle=None, E1=1.0, E2=0.5, nu=0.3, G12=0.35, h=0.01, eps=1e-3):
+    """E_tens of the torsion u = eps z (-y, x, 0) over its closed form
+    1/2 K G12 (1 - nu nu') * int gamma^2 dA with gamma = -eps sin^2(theta)
+    (int over the unit sphere: 32 pi / 15 eps^2), nu' = E1 nu / E2 (the
+    reference's H22, ShellSystem.C:1199, 1208-1209), K = h / (1 - nu nu21),
+    nu21 = E2 nu / E1 (ShellSystem.C:473-476)"""
+    X, T = icosphere(level)
+    o = O.Oracle(X, T, lib_handle=handle, E1=E1, E2=E2, nu=nu, G12=G12, thickness=h)
+    u = np.zeros_like(X)
+    u[:, 0] = -eps * X[:, 2] * X[:, 1]
+    u[:, 1] = eps * X[:, 2] * X[:, 0]
+    en = o.forces(u.reshape(-1))[2]
+    o.close()
+    K = h / (1 - nu * (E2 * nu / E1))
+    closed = 0.5 * K * G12 * (1 - nu * (E1 * nu / E2)) * 32 * np.pi / 15 * eps ** 2
+    return en[0] / closed
+
+
+@pytest.mark.parametrize("E1,E2,nu", [(1.0, 0.5, 0.3), (0.8, 1.2, 0.25)])
+def test_torsion_pins_the_orthotropic_shear_stiffness(icosphere, E1, E2, nu):
+    """a torsion about the polar axis, u = eps z (-y, x, 0), is pure in-plane
+    shear in the lat/long frame of reinit_mod (A1 east, A2 north), with
+    engineering shear gamma = -eps sin^2(theta) on the unit sphere and no
+    stretch to first order.  Its membrane energy is therefore the shear term
+    of the reference's orthotropic H alone (H22, ShellSystem.C:1208-1209):
+      E_tens = 1/2 K G12 (1 - nu nu') int gamma^2 dA,  nu' = E1 nu / E2.
+    The discrete energy converges at second order in the mesh size (3.3 %
+    low at level 3, 0.84 % at level 4); its Richardson extrapolation from
+    levels 3 and 4 meets the closed form to 0.1 %.  With E2/E1 = 0.5,
+    nu nu' = 0.18: a dropped (1 - nu nu') is 22 % off, nu' taken as
+    nu21 = E2 nu / E1 is 16 % off."""
+    r3 = _torsion_energy_ratio(icosphere, 3, E1=E1, E2=E2, nu=nu)
+    r4 = _torsion_energy_ratio(icosphere, 4, E1=E1, E2=E2, nu=nu)
+    assert 0.95 < r3 < r4 < 1.0
+    assert abs(r4 + (r4 - r3) / 3 - 1) < 1e-3
+
+
+@pytest.mark.parametrize("slip", ["drop (1 - nu nu')", "nu' -> nu21", "G12 -> E2/2(1+nu)"])
+def test_torsion_pin_catches_planted_slips(icosphere, slip):
+    """the torsion pin fails for oracle copies with a slip in the shear
+    stiffness (planted in a copy, on the CPU)"""
+    from oracle_mutants import build
+    h8 = "H[8] = G12 * ((1 - nu * nu2) * g[1] * g[0] + (1 - nu * nu2) * ((1 + nu) / (1 - nu)) * g[2] * g[2]);"
+    if slip == "drop (1 - nu nu')":
+        old, new = h8, "H[8] = G12 * (g[1] * g[0] + ((1 + nu) / (1 - nu)) * g[2] * g[2]);"
+    elif slip == "nu' -> nu21":
+        old, new = "const double nu2 = (E1 * nu) / E2; /* ShellSystem.C:1199 */", "const double nu2 = (E2 * nu) / E1;"
+    else:
+        old, new = h8, h8.replace("H[8] = G12 *", "H[8] = (E2 / (2 * (1 + nu))) *")
+    L = build(old, new, "torsion_%d" % ["drop (1 - nu nu')", "nu' -> nu21", "G12 -> E2/2(1+nu)"].index(slip))
+    r3 = _torsion_energy_ratio(icosphere, 3, handle=L)
+    r4 = _torsion_energy_ratio(icosphere, 4, handle=L)
+    assert abs(r4 + (r4 - r3) / 3 - 1) > 0.05
 
 
 @pytest.mark.parametrize("E1,E2", [(1.0, 0.5), (0.7, 1.3), (1.0, 1.0)])
