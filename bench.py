@@ -182,7 +182,7 @@ def run_batch(args):
             done += n
             nconv = sum(r["converged_step"] >= 0 for r in reps)
             log(f"  {done} steps {time.perf_counter() - t1:.1f} s converged {nconv}/{len(reps)}")
-            if all(r["converged_step"] >= 0 or r["problem"] >= 7 for r in reps):
+            if all(r["converged_step"] >= 0 or S.stopped(r) for r in reps):
                 break
         return reps
 

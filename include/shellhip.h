@@ -86,6 +86,8 @@ typedef struct {
                                       ShellSystem.C:2634) >= cell_margin x cell_size (:1917-1919); 1: 2 x the
                                       largest node displacement since the displacement the list was built at
                                       >= the same tolerance (the bound du_sum only overestimates) */
+  int32_t stop_on_problem;         /* stop_on_problem: 1 stops a shell on any problem; 0 only on problems
+                                      >= SHX_STOP_SIM, as check_for_problems (ShellSystem.C:2952) */
 } shx_params;
 
 typedef struct {
@@ -95,9 +97,13 @@ typedef struct {
   double e_tens, e_bend, e_self, e_fric, e_kin;
   double volume, area;
   double du_max;
-  int32_t n_contacts, n_pairs, n_rebuilds, problem;
+  int32_t n_contacts, n_pairs, n_rebuilds;
+  int32_t problem;                 /* worst problem since the shell's init (severities above); a problem
+                                      >= SHX_STOP_SIM (any problem with stop_on_problem) stops the shell
+                                      in the step it occurs, and that step and later ones never converge */
   int32_t n_steps, n_rejected;     /* accepted steps and rejected attempts since shx_init_dynamics */
-  int32_t converged_step;          /* first accepted step with t >= load end and residual_deform <= tol, or -1 */
+  int32_t converged_step;          /* first accepted step with t >= load end and residual_deform <= tol,
+                                      taken with FP64 forces and without a stopping problem; or -1 */
   int32_t n_steps_fp32;            /* accepted steps taken with FP32 split forces (SHX_FP32_SPLIT runs them until
                                       residual_deform <= 1000 tol after the load ramp, then FP64 finishes) */
   double residual_deform;          /* ||f - M a_rigid|| / ||f_ext||: residual with the least-squares rigid-body
@@ -116,6 +122,9 @@ int shx_create(shx_ctx** out, int nv, int ne, const double* X, const int32_t* tr
                const shx_params* prm, int device);
 int shx_destroy(shx_ctx* ctx);
 const char* shx_last_error(const shx_ctx* ctx);
+/* number of HIP devices visible to the process (0 when there is none); the
+ * reference runs on the CPU only and has no counterpart */
+int shx_device_count(void);
 
 /* Host<->device state, FP64 node-interleaved; replaces direct access to the
  * ScalarVectors u, v, a (ShellSystem.h:116) and `time`. */
@@ -241,7 +250,7 @@ int shx_mlp_forward(shx_mlp* m, int batch, const float* x, float* y);
  * respect to the parameters (grad may be NULL) */
 int shx_mlp_gradient(shx_mlp* m, int batch, const float* x, const float* t, float* grad, double* loss);
 /* epochs of minibatch Adam over n samples (shuffled per epoch from seed);
- * loss: mean minibatch loss of the last epoch */
+ * loss: mean minibatch loss of the last epoch; batch > n is SHX_E_ARG */
 int shx_mlp_train(shx_mlp* m, int n, const float* x, const float* t, int batch, int epochs, float lr,
                   uint64_t seed, double* loss);
 /* C (M x N) = op(A) op(B) on the matrix cores (BF16 operands, FP32

@@ -24,6 +24,13 @@ FP32_SPLIT = 1
 
 PROBLEMS = ["NO_PROBLEM", "NO_DISP_CHANGE", "LARGE_DISP", "STOP_SIM", "TOO_MANY_CELLS",
             "SMALL_TIMESTEP", "CONTACT_PROBLEM", "NOT_FINITE"]
+STOP_SIM = 3   # problems of this severity and above stop a shell (ShellSystem.C:2952)
+
+
+def stopped(rep):
+    """the shell's run has ended without an equilibrium: a stopping problem
+    or a device-list overflow"""
+    return rep["problem"] >= STOP_SIM or rep["capacity_err"] != 0
 
 EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_get_state",
            "shx_forces", "shx_init_dynamics", "shx_step", "shx_contact_rebuild", "shx_pairs",
@@ -32,7 +39,7 @@ EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_
            "shx_batch_size", "shx_batch_set_state", "shx_batch_get_state", "shx_batch_init_dynamics",
            "shx_batch_init_shells", "shx_batch_set_extra_damping",
            "shx_batch_step", "shx_batch_reports", "shx_batch_pairs", "shx_batch_cfl_omega_max",
-           "shx_set_graphs"]
+           "shx_set_graphs", "shx_device_count"]
 
 _lib = None
 
@@ -51,7 +58,8 @@ class Params(ctypes.Structure):
         "static_friction_tol", "cell_size_mult", "cell_margin", "neighbor_margin", "newmark_beta",
         "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
         (n, ctypes.c_int32) for n in ("adapt_dt_every", "contact", "quirk_fulltime",
-                                      "quirk_volume_schedule", "volume_at_prediction", "rebuild_trigger")]
+                                      "quirk_volume_schedule", "volume_at_prediction", "rebuild_trigger",
+                                      "stop_on_problem")]
 
 
 class Report(ctypes.Structure):
@@ -72,7 +80,7 @@ DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_
                 newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0,
                 rel_loc_err_min=5e-5, rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0,
                 quirk_fulltime=0, quirk_volume_schedule=0, volume_at_prediction=0,
-                rebuild_trigger=0)
+                rebuild_trigger=0, stop_on_problem=0)
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -92,6 +100,7 @@ def lib():
         L.shx_destroy.argtypes = [_V]
         L.shx_last_error.restype = ctypes.c_char_p
         L.shx_last_error.argtypes = [_V]
+        L.shx_device_count.argtypes = []
         L.shx_set_state.argtypes = [_V, _D, _D, _D, ctypes.c_double]
         L.shx_get_state.argtypes = [_V, _D, _D, _D, _D]
         L.shx_forces.argtypes = [_V, ctypes.c_int, _D, _D, ctypes.c_double, _D, _D, _D, _D]

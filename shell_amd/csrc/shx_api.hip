@@ -182,7 +182,7 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   prof_end(ctx);
   prof_begin(ctx, 5);
   k_bp_grid<<<ctx->S, 256, 0, ctx->st>>>(P, list);
-  k_bp_count<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
+  k_bp_count<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
   prof_end(ctx);
   prof_begin(ctx, 6);
   k_bp_scan<<<ctx->S, 1024, 0, ctx->st>>>(P, list);
@@ -371,11 +371,32 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
   C.stop_on_converged = 0;
   C.volume_at_prediction = prm->volume_at_prediction ? 1 : 0;
   C.rebuild_trigger = prm->rebuild_trigger ? 1 : 0;
+  C.stop_on_problem = prm->stop_on_problem ? 1 : 0;
   {
     unsigned m = (unsigned)std::ceil(std::sqrt((double)ne) / prm->cell_size_mult);
     C.n_cells_max = m * m * m;
   }
   return C;
+}
+
+/* scale of the exact contact-force sums: the characteristic penalty force
+ * k h (k the contact stiffness, or stiff_density times the mean element
+ * area, ShellSystem.C:1820) becomes about 2^60 units, so a unit is 2^-60 of
+ * it and the accumulators hold up to 2^66 times it */
+int contact_fx_shift(const shx_params* prm, const SConst& C, int ne, const double* X, const int32_t* tri)
+{
+  double area = 0;
+  for (int e = 0; e < ne; ++e) {
+    const double* a = X + 3 * tri[3 * e];
+    const double* b = X + 3 * tri[3 * e + 1];
+    const double* c = X + 3 * tri[3 * e + 2];
+    const v3 ab = mk(b[0] - a[0], b[1] - a[1], b[2] - a[2]), ac = mk(c[0] - a[0], c[1] - a[1], c[2] - a[2]);
+    area += 0.5 * nrm(crs(ab, ac));
+  }
+  const double k = prm->contact_stiffness > 0 ? prm->contact_stiffness : C.stiff_density * area / ne;
+  const double f = k * prm->thickness;
+  if (!(f > 0) || !std::isfinite(f)) return 60;
+  return std::max(-900, std::min(900, 60 - std::ilogb(f)));
 }
 
 bool params_ok(const shx_params* prm)
@@ -429,6 +450,12 @@ extern "C" {
 
 const char* shx_last_error(const shx_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
+int shx_device_count(void)
+{
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess ? n : 0;
+}
+
 int shx_destroy(shx_ctx* ctx)
 {
   if (!ctx) return SHX_OK;
@@ -481,6 +508,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       return fail(SHX_E_MESH);
     }
     SConst C = shell_consts(&prm[s], nv[s], ne[s]);
+    C.fx_shift = contact_fx_shift(&prm[s], C, ne[s], X + xoff[s], tri + toff[s]);
     const int nvp = round_up(nv[s], kAlign), nep = round_up(ne[s], kAlign);
     C.vbase = (int)vb; C.ebase = (int)eb;
     C.blkE = (int)(eb / kElemPerBlock); C.nbE = nep / kElemPerBlock;
@@ -679,7 +707,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.brec, (size_t)kBRec * std::max(nrecs, 1))) ||
       (r = dalloc(ctx, P.area, nE)) || (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) ||
       (r = dalloc(ctx, P.brec32, (size_t)kBRec * std::max(nrecs, 1))) ||
-      (r = dalloc(ctx, P.fc, (size_t)3 * nV)) || (r = dalloc(ctx, P.urb, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
+      (r = dalloc(ctx, P.fcq, (size_t)6 * nV)) || (r = dalloc(ctx, P.urb, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)kElemRec * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
@@ -901,7 +929,8 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
     for (int i = 0; i < ctx->S; ++i) {
       DShell& d = s[i];
       d.steps_target = d.n_steps + n_steps;
-      d.done = (d.problem >= 7 || d.err || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
+      const bool stopped = d.problem >= SHX_STOP_SIM || (ctx->C[i].stop_on_problem && d.problem > SHX_NO_PROBLEM);
+      d.done = (stopped || d.err || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
       d.eprec = (precision == SHX_FP32_SPLIT && d.eprec == SHX_FP32_SPLIT && !fp32_done(i, d, ctx->C[i]))
                     ? SHX_FP32_SPLIT : SHX_FP64;
     }

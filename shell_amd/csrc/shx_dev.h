@@ -112,6 +112,53 @@ __device__ __forceinline__ void clear_buckets(const DPtrs& P, const SConst& C, i
     P.bcnt[C.bbase + b] = 0;
 }
 
+/* Exact, order-independent sums of contact forces.  A contribution x is
+ * rounded once to the nearest multiple of 2^-S and added to a 128-bit two's
+ * complement accumulator (w[0] the low word, w[1] the high word): an atomic
+ * add on the low word returns its old value, from which the carry out of
+ * that 64-bit add follows, and the high word receives x's high word plus
+ * the carry (skipped when that is 0).  Integer addition is associative, so
+ * the total is the exact sum of the rounded contributions whatever the
+ * order of the adds.  Returns false (nothing added) for a non-finite x or
+ * |x| >= 2^(126-S). */
+__device__ __forceinline__ bool fx_add(unsigned long long* w, double x, int S)
+{
+  const double y = ldexp(x, S);
+  unsigned long long lo;
+  long long hi;
+  if (fabs(y) < 9.2e18) {                     /* below 2^63: one 64-bit integer */
+    const long long q = __double2ll_rn(y);
+    lo = (unsigned long long)q;
+    hi = q < 0 ? -1 : 0;
+  } else if (fabs(y) < 8.5e37) {              /* below 2^126: y is a multiple of 2^11, split exactly */
+    const double hd = floor(y * 0x1p-64);
+    lo = (unsigned long long)(y - hd * 0x1p64);
+    hi = (long long)hd;
+  } else {
+    return false;                             /* also NaN */
+  }
+  if (lo != 0) {
+    const unsigned long long old = atomicAdd(w, lo);
+    hi += (old + lo < lo) ? 1 : 0;            /* carry out of the low word */
+  }
+  if (hi != 0) atomicAdd(w + 1, (unsigned long long)hi);
+  return true;
+}
+
+/* the accumulator's value in FP64: magnitude (no cancellation), then sign */
+__device__ __forceinline__ double fx_get(unsigned long long lo, unsigned long long hi, int S)
+{
+  const bool neg = (long long)hi < 0;
+  if (neg) {                                  /* two's complement negation of the 128-bit value */
+    lo = ~lo + 1ull;
+    hi = ~hi + (lo == 0 ? 1ull : 0ull);
+  }
+  double m = (double)lo;
+  if (hi) m = ldexp((double)hi, 64) + m;
+  m = ldexp(m, -S);
+  return neg ? -m : m;
+}
+
 /* FP32 small-vector helpers (device only) */
 struct v3f { float x, y, z; };
 __device__ __forceinline__ v3f mkf(float x, float y, float z) { v3f r; r.x = x; r.y = y; r.z = z; return r; }

@@ -25,8 +25,10 @@
  *    synchronises across its own blocks) and a region of hashed cell
  *    buckets;
  *  - contact candidates as fixed-capacity rows per element e1 (the other
- *    element e2 > e1, the last distance and the friction state of each
- *    pair), twice (ping-pong; a rebuild writes the other copy);
+ *    element e2 > e1 in ascending order, the last distance and the friction
+ *    state of each pair), twice (ping-pong; a rebuild writes the other
+ *    copy); contact forces summed exactly per node component in 128-bit
+ *    fixed point, so that runs with contact are reproducible bit for bit;
  *  - work lists of the shells whose contact lists are being rebuilt
  *    (predictor and corrector trigger), which size the broad-phase work,
  *    and host-built maps from launch blocks to the blocks of the active
@@ -109,6 +111,8 @@ struct SConst {
   double conv_tol, load_end_time;
   int adapt_dt_every, contact, self_friction, quirk_fulltime, quirk_volume_schedule;
   int stop_on_converged, volume_at_prediction, rebuild_trigger;
+  int stop_on_problem;             /* stop the shell on any problem, not only on those >= STOP_SIM */
+  int fx_shift;                    /* contact forces are summed as integer multiples of 2^-fx_shift */
   unsigned n_cells_max;
 };
 
@@ -181,7 +185,10 @@ struct DPtrs {
   float* tab32;               /* F_ROWS*nE: tables of the FP32 split path */
   float* brec32;              /* kBRec per record: FP32 split path counterpart of brec */
   double* area;               /* nE */
-  double* fc;                 /* nV*3 contact force accumulator */
+  unsigned long long* fcq;    /* nV*6: contact force of each node component as a 128-bit two's
+                                 complement multiple of 2^-fx_shift (lo word, hi word): the sum of
+                                 the pairs' contributions is exact, so it does not depend on the
+                                 order of their atomic adds */
   double* urb;                /* nV*3 displacement each node's contact list was built at (rebuild trigger 1) */
   double* fout;               /* nV*3 outputs of a plain force evaluation */
   double* fext_out;

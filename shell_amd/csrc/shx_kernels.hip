@@ -191,6 +191,22 @@ __device__ void set_newmark_dev(DShell& s, const SConst& C, double dt)
   s.nm[9] = dt * dt * fabs(b - 1. / 6);
 }
 
+/* a problem that ends the shell's run: severity >= STOP_SIM, or any problem
+ * with stop_on_problem (check_for_problems, ShellSystem.C:2952) */
+__device__ __forceinline__ bool stops(const DShell& s, const SConst& C)
+{
+  return s.problem >= 3 || (C.stop_on_problem && s.problem > 0);
+}
+
+/* a problem registered by the corrector-time broad phase belongs to the
+ * step just accepted (the reference rebuilds inside advance, before
+ * check_for_problems): that step stops the shell and is no equilibrium */
+__device__ __forceinline__ void stop_after_corrector(DShell* sh)
+{
+  atomicCAS(&sh->converged_step, sh->n_steps, -1);
+  sh->done = 1;
+}
+
 /* check_increment (ShellSystem.C:2591-2652) on a max |du|^2 */
 __device__ void check_increment_dev(DShell& s, const SConst& C, double du2max, bool require_change)
 {
@@ -357,8 +373,9 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
  * its wave; a run mostly stays within one or two rows): distance,
  * log-barrier normal force, normal damping and Margolis friction; the force
  * is split -1/2 (u,v,w) onto element 1 and +1/2 (u,v,w) onto element 2 and
- * accumulated with FP64 atomics into fc (the node kernel reads and clears
- * it).  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
+ * summed exactly into the nodes' 128-bit fixed-point accumulators fcq (the
+ * node kernel reads and clears them), so the node forces do not depend on
+ * the order in which the pairs are processed.  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
  * a shell whose list was rebuilt in this step clear its cell buckets.
  * Launch block b takes 256-element tile bmap[V_TILE][b]. */
 __global__ void __launch_bounds__(256, 3) k_contact(DPtrs P, int mode)
@@ -481,16 +498,18 @@ __global__ void __launch_bounds__(256, 3) k_contact(DPtrs P, int mode)
         fn -= C.self_damping * k * vn;
         fcon = add(fcon, scl(n, fn));
         const double wv[2] = {hit.v0, hit.v1}, ww[2] = {hit.w0, hit.w1};
+        bool fin = true;
         for (int s = 0; s < 2; ++s) {
           const double vb = wv[s], wb = ww[s], ub = 1 - vb - wb;
           const double pre = s == 0 ? -0.5 : 0.5;
           const double fq[3] = {pre * fcon.x, pre * fcon.y, pre * fcon.z};
           for (int j = 0; j < 3; ++j) {
-            atomicAdd(&P.fc[3 * nid[s][0] + j], ub * fq[j]);
-            atomicAdd(&P.fc[3 * nid[s][1] + j], vb * fq[j]);
-            atomicAdd(&P.fc[3 * nid[s][2] + j], wb * fq[j]);
+            fin &= fx_add(&P.fcq[6 * nid[s][0] + 2 * j], ub * fq[j], C.fx_shift);
+            fin &= fx_add(&P.fcq[6 * nid[s][1] + 2 * j], vb * fq[j], C.fx_shift);
+            fin &= fx_add(&P.fcq[6 * nid[s][2] + 2 * j], wb * fq[j], C.fx_shift);
           }
         }
+        if (!fin) atomicMax(&P.sh[sid].problem, 7); /* NOT_FINITE */
       } else {
         so = mk(0, 0, 0);
         stick = 0;
@@ -617,7 +636,9 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     w.rebuild = rebuild_due(w, C);
     if (w.rebuild) plan_rebuild(P, w, sid, 1);
     set_newmark_dev(w, C, new_dt);
-    if (w.converged_step < 0 && w.t >= C.load_end_time && w.residual_deform <= C.conv_tol) {
+    /* a shell converges only in FP64 and with no problem that stops it */
+    if (w.converged_step < 0 && w.eprec == SHX_PREC_FP64 && !stops(w, C) && w.t >= C.load_end_time &&
+        w.residual_deform <= C.conv_tol) {
       w.converged_step = w.n_steps;
       if (C.stop_on_converged) w.done = 1;
     }
@@ -629,7 +650,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     set_newmark_dev(w, C, new_dt);
     if (w.dt <= 2.220446049250313e-16) { w.problem = max(w.problem, 5); w.done = 1; }
   }
-  if (w.problem >= 7 || w.err) w.done = 1;
+  if (stops(w, C) || w.err) w.done = 1;
 }
 
 /* One thread per node: the node's internal force and pressure direction are
@@ -692,8 +713,15 @@ __global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
       const double V0 = (mode == M_INIT) ? (sh.volume_ref > 0 ? sh.volume_ref : Vc) : sh.volume0;
       ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / Vc - 1);
     }
-    const v3 fcn = ld3(P.fc, n);
-    st3(P.fc, n, mk(0, 0, 0));
+    v3 fcn = mk(0, 0, 0);
+    if (C.contact) {
+      const ulonglong2* q = reinterpret_cast<const ulonglong2*>(P.fcq + 6 * (size_t)n);
+      const ulonglong2 qx = q[0], qy = q[1], qz = q[2];
+      fcn = mk(fx_get(qx.x, qx.y, C.fx_shift), fx_get(qy.x, qy.y, C.fx_shift), fx_get(qz.x, qz.y, C.fx_shift));
+      ulonglong2* qw = reinterpret_cast<ulonglong2*>(P.fcq + 6 * (size_t)n);
+      const ulonglong2 z = make_ulonglong2(0ull, 0ull);
+      qw[0] = z; qw[1] = z; qw[2] = z;
+    }
     const v3 fext = add(scl(fp, ptot), fcn);
     const v3 f = add(fint, fext);
     rec[0] = nrm2(f);
@@ -766,9 +794,10 @@ __global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
  * that walks the blocks of the listed shells only:
  *  k_bp_box   element AABBs at the positions and 256-element tile partials
  *             (compute_bounding_box, ShellSystem.C:2544-2587);
- *  k_bp_count  the shell's cell grid (box, averaged element size, cell size,
- *             d_max, grid size, n_cells_max check, ShellSystem.C:1933-1964)
- *             and one count per (element, cell) in the cell's hash bucket;
+ *  k_bp_grid  the shell's cell grid (box, averaged element size, cell size,
+ *             d_max, grid size, n_cells_max check, ShellSystem.C:1933-1964);
+ *  k_bp_count one count per (element, cell) in the cell's hash bucket, and
+ *             the build displacement of the element's nodes (trigger 1);
  *  k_bp_scan  one block per listed shell: bucket offsets (prefix scan);
  *  k_bp_scatter (element, cell) entries into their buckets: together a
  *             counting sort of the entries by cell hash, the reference's
@@ -877,7 +906,6 @@ __global__ void __launch_bounds__(256) k_bp_box(DPtrs P, int posmode, int list)
       const int n = P.tri[(size_t)k * nE + e];
       v3 u = ld3(U, n);
       if (predict) u = pred3(u, ld3(V, n), ld3(A, n), dt, a8);
-      if (C.rebuild_trigger) st3(P.urb, n, u);     /* the build displacement (same value from every element) */
       const v3 x = add(ld3(P.X, n), u);
       const double xx[3] = {x.x, x.y, x.z};
       finite = finite && isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
@@ -1013,11 +1041,13 @@ __global__ void __launch_bounds__(256) k_bp_grid(DPtrs P, int list)
   Grid g;
   const bool ok = shell_grid(P, C, g);
   if (threadIdx.x != 0) return;
-  sh->du_sum = 0;
-  sh->n_rebuilds++;
-  sh->avg_elem_size = g.avg;
-  for (int c = 0; c < 3; ++c) { sh->bmin[c] = g.bmin[c]; sh->bmax[c] = g.bmax[c]; }
   if (ok) {
+    /* the trigger baseline restarts only with a list that is built: an
+     * aborted rebuild keeps the old list and stops the shell */
+    sh->du_sum = 0;
+    sh->n_rebuilds++;
+    sh->avg_elem_size = g.avg;
+    for (int c = 0; c < 3; ++c) { sh->bmin[c] = g.bmin[c]; sh->bmax[c] = g.bmax[c]; }
     sh->cell_size = g.cell;
     sh->d_max = g.dmx;
     sh->d2_max = g.dmx * g.dmx;
@@ -1028,13 +1058,15 @@ __global__ void __launch_bounds__(256) k_bp_grid(DPtrs P, int list)
     sh->problem = max(sh->problem, g.problem);
     sh->rebuild = 0;
     sh->pair_buf = sh->pair_src;
+    sh->done = 1;
+    if (list == 1) stop_after_corrector(sh);
   }
 }
 
 /* count pass of the counting sort, one thread per element: the element's
  * cell index range (its AABB widened by d_max, ShellSystem.C:1967-2010) and
  * one count per (element, cell) in the cell's hash bucket */
-__global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
+__global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int posmode, int list)
 {
   if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
@@ -1060,6 +1092,17 @@ __global__ void __launch_bounds__(256) k_bp_count(DPtrs P, int list)
     pc[0] = make_int4(lo[0], lo[1], lo[2], 0);
     pc[1] = make_int4(hi[0], hi[1], hi[2], 0);
     P.rcnt[sh->pair_buf][e] = 0;
+    if (C.rebuild_trigger) {
+      /* the displacement the new list is built at (trigger 1): the same
+       * value from every element of the node */
+      const int par = sh->par;
+      for (int k = 0; k < 3; ++k) {
+        const int n = P.tri[(size_t)k * P.nE + e];
+        v3 u = ld3(P.U[par], n);
+        if (posmode == M_STEP) u = pred3(u, ld3(P.V[par], n), ld3(P.A[par], n), sh->dt, sh->nm[8]);
+        st3(P.urb, n, u);
+      }
+    }
     for (int cx = lo[0]; cx <= hi[0]; ++cx)
       for (int cy = lo[1]; cy <= hi[1]; ++cy)
         for (int cz = lo[2]; cz <= hi[2]; ++cz) {
@@ -1270,7 +1313,7 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
  * shells' new rows (ShellShellContact::compute, ShellShellContact.C:35-145,
  * kept if d2 <= d_max^2, ShellSystem.C:2096-2101; d2 <= 0 registers
  * CONTACT_PROBLEM): one thread per element row, the row compacted in place
- * in its order.  Kept apart from the pair search so that the search runs at
+ * and sorted by e2.  Kept apart from the pair search so that the search runs at
  * the occupancy of its own (small) register footprint. */
 __global__ void __launch_bounds__(256) k_bp_exact(DPtrs P, int posmode, int list)
 {
@@ -1313,7 +1356,10 @@ __global__ void __launch_bounds__(256) k_bp_exact(DPtrs P, int posmode, int list
         for (int j = 0; j < 3; ++j) X1[j] = node_pos(P, P.tri[(size_t)j * nE + e2], U, V, A, predict, dt, a8);
         tri_hit hit;
         tri_tri(X0, X1, hit);
-        if (hit.d2 <= 0) atomicMax(&sh->problem, 6);
+        if (hit.d2 <= 0) {
+          atomicMax(&sh->problem, 6);
+          if (list == 1) stop_after_corrector(sh);
+        }
         if (hit.d2 > d2mx) continue;
         d2 = hit.d2;
       }
@@ -1327,6 +1373,31 @@ __global__ void __launch_bounds__(256) k_bp_exact(DPtrs P, int posmode, int list
       ++w;
     }
     if (w != n) P.rcnt[nb][e1] = w;
+    /* the row in ascending e2 (insertion sort in place; rows are short):
+     * survivors and new pairs arrive in any order, the sorted row does not
+     * depend on it, nor do the sums the contact kernel forms over it */
+    for (int i = 1; i < w; ++i) {
+      const size_t pi = row + i;
+      const int e2 = P.pe2[nb][pi];
+      const double d2 = P.pd2[nb][pi];
+      const int stk = P.pstick[nb][pi];
+      const v3 so = ld3(P.pso[nb], pi);
+      int j = i - 1;
+      for (; j >= 0 && P.pe2[nb][row + j] > e2; --j) {
+        const size_t a = row + j, b = row + j + 1;
+        P.pe2[nb][b] = P.pe2[nb][a];
+        P.pd2[nb][b] = P.pd2[nb][a];
+        P.pstick[nb][b] = P.pstick[nb][a];
+        st3(P.pso[nb], b, ld3(P.pso[nb], a));
+      }
+      if (j + 1 != i) {
+        const size_t b = row + j + 1;
+        P.pe2[nb][b] = e2;
+        P.pd2[nb][b] = d2;
+        P.pstick[nb][b] = stk;
+        st3(P.pso[nb], b, so);
+      }
+    }
   }
 }
 

@@ -361,6 +361,11 @@ int shx_mlp_train(shx_mlp* m, int n, const float* x, const float* t, int batch, 
                   uint64_t seed, double* loss_out)
 {
   if (!m || n <= 0 || !x || !t || batch <= 0 || epochs <= 0) return SHX_E_ARG;
+  if (batch > n) {
+    m->err = "minibatch larger than the training set (batch " + std::to_string(batch) + " > n " +
+             std::to_string(n) + "): no optimizer step would run";
+    return SHX_E_ARG;
+  }
   const int nin = m->width[0], nout = m->width[m->nl];
   if (int r = ensure_batch(m, batch)) return r;
   float *dx = nullptr, *dt = nullptr;

@@ -32,7 +32,7 @@ import time
 
 import numpy as np
 
-from . import ALL_SHELLS, E_CAPACITY, FP64, ShellBatch, ShellError
+from . import ALL_SHELLS, E_CAPACITY, FP64, ShellBatch, ShellError, stopped
 from .workloads import icosphere
 
 LEVEL = 4
@@ -105,7 +105,7 @@ def build(P, cfl_iters=60, kind=None):
 
 
 def _converged_or_dead(reps):
-    return all(r["converged_step"] >= 0 or r["problem"] >= 7 for r in reps)
+    return all(r["converged_step"] >= 0 or stopped(r) for r in reps)
 
 
 def _advance(b, n, tol, precision, tolerant):
