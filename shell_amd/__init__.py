@@ -166,6 +166,7 @@ class ShellError(RuntimeError):
         self.code = code
 
 
+E_ARG = -1
 E_CAPACITY = -4
 
 

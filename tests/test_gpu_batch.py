@@ -1,9 +1,8 @@
 """Batched shells: every phase of a step is one launch over all shells of
-the batch.  Each shell of a batch must evolve exactly as it does alone:
-bit-exact where no atomics are involved (no contact: same arithmetic, same
-per-shell reduction order), within the FP64 bar with contact (the contact
-force is accumulated with atomics), with pair lists identical to the CPU
-oracle's.  Graph replay of step attempts must equal launching them one by
+the batch.  Each shell of a batch must evolve exactly as it does alone,
+bit for bit, with contact too (the same arithmetic, the same per-shell
+reduction orders, contact rows sorted and contact forces summed exactly in
+fixed point), with pair lists identical to the CPU oracle's.  Graph replay of step attempts must equal launching them one by
 one.
 """
 import numpy as np
@@ -65,11 +64,10 @@ def test_batch_equals_single_shells(icosphere):
             assert rb[s]["n_steps"] == rs["n_steps"] and rb[s]["n_rejected"] == rs["n_rejected"]
             assert rb[s]["n_rebuilds"] == rs["n_rebuilds"] and rb[s]["n_pairs"] == rs["n_pairs"]
             assert tb == ts
-            if not params[s].get("contact"):
-                assert np.array_equal(ub, us) and np.array_equal(vb, vs)
-                assert rb[s]["residual"] == rs["residual"]
-            else:
-                assert relerr(ub, us) < 1e-10 and relerr(vb, vs) < 1e-9
+            assert np.array_equal(ub, us) and np.array_equal(vb, vs)
+            assert rb[s]["residual"] == rs["residual"]
+            assert rb[s]["e_self"] == rs["e_self"] and rb[s]["e_fric"] == rs["e_fric"]
+            if params[s].get("contact"):
                 e1b, e2b = b.pairs(s)
                 e1s, e2s = g.pairs()
                 assert np.array_equal(e1b, e1s) and np.array_equal(e2b, e2s)
@@ -106,9 +104,13 @@ def test_batch_shell_against_oracle(icosphere):
     b.close()
 
 
-def test_graph_replay_equals_launches(icosphere):
+@pytest.mark.parametrize("contact", [False, True])
+def test_graph_replay_equals_launches(icosphere, contact):
     meshes, params = _shells(icosphere)
-    meshes, params = [meshes[0], meshes[0]], [params[0], dict(params[0], dt=0.07)]
+    if contact:
+        meshes, params = [meshes[2], meshes[1]], [params[2], params[1]]
+    else:
+        meshes, params = [meshes[0], meshes[0]], [params[0], dict(params[0], dt=0.07)]
     say("case graph replay vs launches")
     runs = []
     for graphs in (True, False):
@@ -123,3 +125,4 @@ def test_graph_replay_equals_launches(icosphere):
     for s in range(len(meshes)):
         assert np.array_equal(runs[0][0][s], runs[1][0][s])
         assert runs[0][1][s]["n_steps"] == runs[1][1][s]["n_steps"] == 150
+        assert runs[0][1][s]["e_self"] == runs[1][1][s]["e_self"]

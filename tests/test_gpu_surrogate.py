@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import mlp_ref as R
+import shell_amd as S
 from shell_amd import surrogate as SG
 
 pytestmark = pytest.mark.gpu
@@ -64,4 +65,17 @@ def test_mlp_learns_a_smooth_map():
     l1 = net.train(x, t, batch=128, epochs=300, lr=3e-3)
     say(f"  loss {l0:.4f} -> {l1:.5f}")
     assert l1 < l0 / 50
+    net.close()
+
+
+def test_mlp_train_refuses_a_minibatch_larger_than_the_set():
+    """batch > n would run no optimizer step: refused with SHX_E_ARG, the
+    parameters untouched"""
+    net = SG.MLP([6, 32, 8], seed=5)
+    p0 = net.params()
+    x = np.zeros((10, 6), np.float32)
+    t = np.zeros((10, 8), np.float32)
+    with pytest.raises(S.ShellError, match="minibatch larger"):
+        net.train(x, t, batch=16, epochs=3)
+    assert np.array_equal(net.params(), p0)
     net.close()
