@@ -246,6 +246,16 @@ int shx_mlp_num_params(const shx_mlp* m);
 int shx_mlp_get_params(shx_mlp* m, float* out);
 int shx_mlp_set_params(shx_mlp* m, const float* in);   /* also resets the optimiser state */
 int shx_mlp_forward(shx_mlp* m, int batch, const float* x, float* y);
+/* shx_mlp_forward without any device allocation: x_dev (batch x width[0])
+ * and y_dev (batch x width[n_layers]) are caller-owned device arrays on the
+ * network's GPU; the hidden activations live in a file-scope device scratch
+ * array of 4 MiB per GPU (so batch <= 2^19 / widest hidden layer, 2048 rows
+ * at width 256; larger is SHX_E_ARG), set up by shx_mlp_create.  Results
+ * equal shx_mlp_forward's.  Calls that overlap on one GPU share the scratch
+ * and race.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops of the warm start). */
+int shx_mlp_forward_noalloc(shx_mlp* m, int batch, const float* x_dev, float* y_dev);
 /* mean squared error of the outputs for x against t and its gradient with
  * respect to the parameters (grad may be NULL) */
 int shx_mlp_gradient(shx_mlp* m, int batch, const float* x, const float* t, float* grad, double* loss);

@@ -9,8 +9,10 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,8 +41,8 @@ __device__ __forceinline__ float gelu_grad(float x)
 }
 
 /* C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[N]) with the epilogue
- * `epi`: 0 plain, 1 store the sum in Z and GELU(sum) in C, 2 accumulate
- * into C.  op(A)[m][k] = A[m * lda + k] (ta = 0) or A[k * lda + m] (ta = 1);
+ * `epi`: 0 plain, 1 store the sum in Z (unless Z is null) and GELU(sum) in
+ * C, 2 accumulate into C.  op(A)[m][k] = A[m * lda + k] (ta = 0) or A[k * lda + m] (ta = 1);
  * op(B)[k][n] = B[k * ldb + n] (tb = 0) or B[n * ldb + k] (tb = 1).
  * One wave per 16 x 16 tile of C, four waves per block (32 x 32); the K
  * loop takes 16 at a time.  MFMA operand layout (16x16x16): lane l holds
@@ -75,7 +77,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const fl
     float v = acc[i] + (bias ? bias[n] : 0.f);
     const size_t o = (size_t)m * ldc + n;
     if (epi == 1) {
-      Z[o] = v;
+      if (Z) Z[o] = v;
       C[o] = gelu(v);
     } else if (epi == 2) {
       C[o] += v;
@@ -148,10 +150,19 @@ __global__ void k_adam(int n, float* w, const float* g, float* m1, float* m2, fl
 
 }  // namespace
 
+/* Scratch of the allocation-free forward pass (shx_mlp_forward_noalloc):
+ * the hidden activations, ping-ponged between its two halves.  One copy per
+ * GPU, held until the process ends; host code never passes this name to a
+ * kernel, only the per-device address hipGetSymbolAddress gives for it. */
+constexpr size_t kNoAllocFloats = (size_t)1 << 20;   /* 4 MiB */
+__device__ __align__(256) float scratch[kNoAllocFloats];
+
 /* ------------------------------------------------------------------ host */
 
 struct shx_mlp {
   int nl = 0;                  /* layers */
+  int dev = 0;                 /* the GPU that holds the network */
+  float* scratch = nullptr;    /* that GPU's copy of `scratch` (device address) */
   std::vector<int> width;      /* nl + 1 */
   std::vector<size_t> woff, boff;
   size_t nparam = 0;
@@ -176,6 +187,26 @@ namespace {
   } while (0)
 
 thread_local std::string g_mlp_err;
+
+/* device address of `scratch` on each GPU, looked up once per device with
+ * that device current (each GPU holds its own copy at its own address) */
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+hipError_t scratch_address(int dev, float** out)
+{
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
 
 void gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb, int tb,
           const float* bias, float* C, int ldc, float* Z, int epi)
@@ -289,7 +320,9 @@ int shx_mlp_create(shx_mlp** out, int n_layers, const int* width, uint64_t seed,
     shx_mlp_destroy(m);
     return SHX_E_HIP;
   };
+  m->dev = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->st) != hipSuccess) return fail("hip setup failed");
+  if (scratch_address(device, &m->scratch) != hipSuccess) return fail("hipGetSymbolAddress(scratch) failed");
   if (hipMalloc(&m->p, off * sizeof(float)) != hipSuccess || hipMalloc(&m->g, off * sizeof(float)) != hipSuccess ||
       hipMalloc(&m->m1, off * sizeof(float)) != hipSuccess || hipMalloc(&m->m2, off * sizeof(float)) != hipSuccess)
     return fail("hipMalloc failed");
@@ -340,6 +373,33 @@ int shx_mlp_forward(shx_mlp* m, int B, const float* x, float* y)
   MCHK(hipGetLastError());
   MCHK(hipMemcpyAsync(y, m->act[m->nl], (size_t)B * m->width[m->nl] * sizeof(float), hipMemcpyDeviceToHost,
                       m->st));
+  MCHK(hipStreamSynchronize(m->st));
+  return SHX_OK;
+}
+
+int shx_mlp_forward_noalloc(shx_mlp* m, int B, const float* x_dev, float* y_dev)
+{
+  if (!m || B <= 0 || !x_dev || !y_dev) return SHX_E_ARG;
+  if (!m->scratch) { m->err = "no scratch address for this device"; return SHX_E_STATE; }
+  int wh = 0;                                   /* widest hidden layer */
+  for (int l = 1; l < m->nl; ++l) wh = std::max(wh, m->width[l]);
+  if ((size_t)2 * B * wh > kNoAllocFloats) {
+    m->err = "shx_mlp_forward_noalloc: batch " + std::to_string(B) + " exceeds the scratch limit of " +
+             std::to_string(wh ? kNoAllocFloats / (2 * (size_t)wh) : 0) + " rows for hidden width " +
+             std::to_string(wh);
+    return SHX_E_ARG;
+  }
+  MCHK(hipSetDevice(m->dev));
+  float* half[2] = {m->scratch, m->scratch + kNoAllocFloats / 2};
+  const float* in = x_dev;
+  for (int l = 0; l < m->nl; ++l) {
+    const int K = m->width[l], N = m->width[l + 1];
+    const bool last = l == m->nl - 1;
+    float* out = last ? y_dev : half[l & 1];
+    gemm(m->st, B, N, K, in, K, 0, m->p + m->woff[l], N, 0, m->p + m->boff[l], out, N, nullptr, last ? 0 : 1);
+    in = out;
+  }
+  MCHK(hipGetLastError());
   MCHK(hipStreamSynchronize(m->st));
   return SHX_OK;
 }
