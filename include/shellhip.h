@@ -88,7 +88,47 @@ typedef struct {
                                       >= the same tolerance (the bound du_sum only overestimates) */
   int32_t stop_on_problem;         /* stop_on_problem: 1 stops a shell on any problem; 0 only on problems
                                       >= SHX_STOP_SIM, as check_for_problems (ShellSystem.C:2952) */
+  double load_end_time;            /* ext: convergence is checked once t >= this; 0: the later of
+                                      pressure_ramp_time and volume_ramp_time */
+  char pressure_fn[160];           /* pressure: a muparser expression in t (config.cfg [function] key,
+                                      Function.h); non-empty replaces the linear pressure ramp */
+  char volume_growth_fn[160];      /* volume_growth: target enclosed volume over V0 as an expression in t;
+                                      non-empty replaces the linear volume ramp (the reference parses it but
+                                      uses the fixed schedule of quirk_volume_schedule, ShellSystem.C:1474) */
 } shx_params;
+
+/* The reference's defaults for every field (ShellSystem constructor and
+ * set_parameters defaults, ShellSystem.C:23-142, 170-418). */
+void shx_params_default(shx_params* p);
+
+/* Replaces: the driver's and ShellSystem::set_parameters' reading of
+ * config.cfg with GetPot and muparser (shell.C:80-144, ShellSystem.C:
+ * 170-418, macros.h:15-78).  `text` holds the config file (GetPot
+ * `key = value` lines, `#` comments, `[section]` prefixes); argv holds
+ * `-key value` overrides, which win (argv[0] is not skipped: pass the
+ * arguments only).  Keys get the reference's defaults when absent (a
+ * warning, as macros.h:16-17), the reference's ASSERTs are checked, and
+ * the [function] keys are muparser expressions: constant for the material
+ * and contact keys, in t for pressure and volume_growth (compiled for the
+ * GPU).  Keys of features outside this path (cavity contact, growth,
+ * gravity, perturbation, smooth contact, non-local pressure, extra
+ * quadrature points) are refused when set to values that turn them on.
+ * Returns SHX_OK (message: warnings) or SHX_E_ARG (message: the error). */
+typedef struct {
+  shx_params prm;
+  int32_t time_steps, refinements, random_seed, reserved;   /* time_steps 0: no step limit */
+  double walltime_limit, data_time, vtk_time, state_time;   /* shell.C:114-121 */
+  double randomness, distortion;
+  double project[3], scale[3], rotate[3], translate[3];    /* mesh transforms, shell.C:190-225 */
+  char stopping_criterion[160];                            /* expression in t; non-zero stops the run */
+  char message[4096];
+} shx_config;
+int shx_config_parse(const char* text, int argc, const char* const* argv, shx_config* out);
+/* value at time t of an expression of the config dialect (muparser subset:
+ * + - * / ^ ?: || && comparisons, t, pi, _e, sin cos tan asin acos atan
+ * sinh cosh tanh asinh acosh atanh exp ln log log10 log2 sqrt abs sign
+ * rint min max); SHX_E_ARG for an expression that does not compile */
+int shx_expr_eval(const char* expr, double t, double* value);
 
 typedef struct {
   double t, dt;                    /* time and next dt after the last accepted step */

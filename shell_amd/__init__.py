@@ -59,7 +59,9 @@ class Params(ctypes.Structure):
         "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
         (n, ctypes.c_int32) for n in ("adapt_dt_every", "contact", "quirk_fulltime",
                                       "quirk_volume_schedule", "volume_at_prediction", "rebuild_trigger",
-                                      "stop_on_problem")]
+                                      "stop_on_problem")] + [
+        ("load_end_time", ctypes.c_double), ("pressure_fn", ctypes.c_char * 160),
+        ("volume_growth_fn", ctypes.c_char * 160)]
 
 
 class Report(ctypes.Structure):
@@ -80,7 +82,7 @@ DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_
                 newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0,
                 rel_loc_err_min=5e-5, rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0,
                 quirk_fulltime=0, quirk_volume_schedule=0, volume_at_prediction=0,
-                rebuild_trigger=0, stop_on_problem=0)
+                rebuild_trigger=0, stop_on_problem=0, load_end_time=0.0, pressure_fn="", volume_growth_fn="")
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -154,7 +156,7 @@ def make_params(**kw):
     d = dict(DEFAULTS)
     d.update(kw)
     for k, v in d.items():
-        setattr(p, k, v)
+        setattr(p, k, v.encode() if isinstance(v, str) else v)
     return p
 
 
@@ -425,3 +427,62 @@ class ShellBatch:
         self._check(lib().shx_kernel_times(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
                     "shx_kernel_times")
         return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(KERNEL_CLASSES)}
+
+
+class Config(ctypes.Structure):
+    """shx_config: parameters and driver settings read from a config.cfg"""
+    _fields_ = [("prm", Params)] + [(n, ctypes.c_int32) for n in (
+        "time_steps", "refinements", "random_seed", "reserved")] + [(n, ctypes.c_double) for n in (
+        "walltime_limit", "data_time", "vtk_time", "state_time", "randomness", "distortion")] + [
+        (n, ctypes.c_double * 3) for n in ("project", "scale", "rotate", "translate")] + [
+        ("stopping_criterion", ctypes.c_char * 160), ("message", ctypes.c_char * 4096)]
+
+
+def _declare_config(L):
+    if getattr(L, "_cfg_declared", False):
+        return
+    L.shx_config_parse.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                   ctypes.POINTER(Config)]
+    L.shx_expr_eval.argtypes = [ctypes.c_char_p, ctypes.c_double, _D]
+    L.shx_params_default.argtypes = [ctypes.POINTER(Params)]
+    L._cfg_declared = True
+
+
+def params_dict(p):
+    """the fields of a Params structure as a dict (strings decoded)"""
+    out = {}
+    for name, _ in Params._fields_:
+        v = getattr(p, name)
+        out[name] = v.decode() if isinstance(v, bytes) else v
+    return out
+
+
+def read_config(text, args=()):
+    """config.cfg text plus `-key value` overrides -> (params dict for
+    ShellSystem / ShellBatch, driver settings dict, warnings); raises
+    ShellError with the reference's ASSERT or the refusal on a bad config"""
+    L = lib()
+    _declare_config(L)
+    argv = (ctypes.c_char_p * max(len(args), 1))(*[a.encode() for a in args])
+    cfg = Config()
+    r = L.shx_config_parse(text.encode(), len(args), argv, ctypes.byref(cfg))
+    msg = cfg.message.decode()
+    if r != 0:
+        raise ShellError(f"shx_config_parse: {msg}", r)
+    driver = {n: getattr(cfg, n) for n in ("time_steps", "refinements", "random_seed", "walltime_limit",
+                                           "data_time", "vtk_time", "state_time", "randomness", "distortion")}
+    for n in ("project", "scale", "rotate", "translate"):
+        driver[n] = tuple(getattr(cfg, n))
+    driver["stopping_criterion"] = cfg.stopping_criterion.decode()
+    return params_dict(cfg.prm), driver, msg
+
+
+def expr_eval(text, t=0.0):
+    """value of a config time-function expression at time t"""
+    L = lib()
+    _declare_config(L)
+    v = ctypes.c_double()
+    r = L.shx_expr_eval(text.encode(), t, ctypes.byref(v))
+    if r != 0:
+        raise ShellError(f"shx_expr_eval: cannot compile '{text}'", r)
+    return v.value

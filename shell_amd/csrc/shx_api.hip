@@ -16,6 +16,7 @@
 
 #include "shellhip.h"
 #include "shx_device.h"
+#include "shx_expr.h"
 #include "shx_kernels.h"
 #include "shx_math.h"
 #include "shx_setup.h"
@@ -367,7 +368,9 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
   C.contact = prm->contact ? 1 : 0;
   C.self_friction = (prm->mu_s != 0 || prm->mu_d != 0) ? 1 : 0;
   C.quirk_fulltime = prm->quirk_fulltime; C.quirk_volume_schedule = prm->quirk_volume_schedule;
-  C.conv_tol = 0; C.load_end_time = std::max(prm->pressure_ramp_time, prm->volume_ramp_time);
+  C.conv_tol = 0;
+  C.load_end_time = prm->load_end_time > 0 ? prm->load_end_time
+                                           : std::max(prm->pressure_ramp_time, prm->volume_ramp_time);
   C.stop_on_converged = 0;
   C.volume_at_prediction = prm->volume_at_prediction ? 1 : 0;
   C.rebuild_trigger = prm->rebuild_trigger ? 1 : 0;
@@ -474,11 +477,27 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   if (!out || nshell <= 0 || !nv || !ne || !X || !tri || !prm) { g_create_err = "invalid argument"; return SHX_E_ARG; }
   if (nshell > kMaxShells) { g_create_err = "more than " + std::to_string(kMaxShells) + " shells"; return SHX_E_ARG; }
   *out = nullptr;
+  std::vector<ExprProg> pprog(nshell), vprog(nshell);
   for (int s = 0; s < nshell; ++s) {
     if (nv[s] <= 0 || ne[s] <= 0) { g_create_err = "invalid mesh size of shell " + std::to_string(s); return SHX_E_ARG; }
     if (!params_ok(&prm[s])) {
       g_create_err = "invalid parameters of shell " + std::to_string(s) + " (ShellSystem.C:326-332 asserts)";
       return SHX_E_ARG;
+    }
+    /* time functions of the config (muparser expressions in t) */
+    const char* fns[2] = {prm[s].pressure_fn, prm[s].volume_growth_fn};
+    ExprProg* progs[2] = {&pprog[s], &vprog[s]};
+    for (int q = 0; q < 2; ++q) {
+      std::memset(progs[q], 0, sizeof(ExprProg));
+      if (strnlen(fns[q], sizeof(prm[s].pressure_fn)) == sizeof(prm[s].pressure_fn)) {
+        g_create_err = "shell " + std::to_string(s) + ": time function not NUL-terminated";
+        return SHX_E_ARG;
+      }
+      char e[256];
+      if (fns[q][0] && !expr_compile(fns[q], *progs[q], e, sizeof(e))) {
+        g_create_err = "shell " + std::to_string(s) + (q ? ": volume_growth: " : ": pressure: ") + e;
+        return SHX_E_ARG;
+      }
     }
   }
   shx_ctx* ctx = new shx_ctx();
@@ -509,6 +528,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     }
     SConst C = shell_consts(&prm[s], nv[s], ne[s]);
     C.fx_shift = contact_fx_shift(&prm[s], C, ne[s], X + xoff[s], tri + toff[s]);
+    C.pressure_prog = pprog[s];
+    C.volume_prog = vprog[s];
     const int nvp = round_up(nv[s], kAlign), nep = round_up(ne[s], kAlign);
     C.vbase = (int)vb; C.ebase = (int)eb;
     C.blkE = (int)(eb / kElemPerBlock); C.nbE = nep / kElemPerBlock;
@@ -1088,7 +1109,7 @@ int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega)
   HIPCHK(hipMemcpy(u0.data(), ctx->P.U[0], sizeof(double) * n3, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(minv.data(), ctx->P.minv, sizeof(double) * nV, hipMemcpyDeviceToHost));
   std::vector<SConst> Ci = ctx->C;
-  for (auto& c : Ci) { c.pressure_final = 0; c.bulk_modulus = 0; c.contact = 0; }
+  for (auto& c : Ci) { c.pressure_final = 0; c.pressure_prog.n = 0; c.bulk_modulus = 0; c.contact = 0; }
   if (int r = write_consts(ctx, Ci)) return r;
   s = s0;
   for (auto& d : s) { d.done = 0; d.par = 0; d.t_eval = 0; d.eprec = SHX_FP64; }

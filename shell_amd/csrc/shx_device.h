@@ -39,6 +39,8 @@
 
 #include <stdint.h>
 
+#include "shx_expr.h"
+
 namespace shx {
 
 /* element table rows (FP64) */
@@ -114,6 +116,8 @@ struct SConst {
   int stop_on_problem;             /* stop the shell on any problem, not only on those >= STOP_SIM */
   int fx_shift;                    /* contact forces are summed as integer multiples of 2^-fx_shift */
   unsigned n_cells_max;
+  ExprProg pressure_prog;          /* pressure(t) of the config (n == 0: the linear ramp) */
+  ExprProg volume_prog;            /* volume_growth(t), the target ratio (n == 0: the linear ramp) */
 };
 
 /* per-shell mutable scalars (device array, written by the finalize and
@@ -137,6 +141,8 @@ struct DShell {
   int steps_target, done, attempts;
   int eprec;                       /* element-force precision of the shell (set by the host): 0 FP64, 1 FP32 split */
   int n_fp32;                      /* accepted steps taken with FP32 split forces */
+  double load_p, load_r;           /* pressure and target volume ratio at the force time of the current
+                                      attempt (the element epilogue evaluates them, the node kernel applies) */
   /* results of the element kernel of the current attempt */
   double att_e_tens, att_e_bend, att_volume, att_area;
   double fdeform, residual_deform;

@@ -25,17 +25,29 @@
 
 namespace shx {
 
+/* the pressure time function (MAKE_FUNCTION(pressure), ShellSystem.C:222;
+ * pressure *= fulltime with the quirk, :1480): the config's expression, or
+ * the linear ramp pressure_final * min(t / pressure_ramp_time, 1) */
 __device__ double pressure_at(const SConst& C, double t)
 {
-  double p = C.pressure_final;
-  if (C.pressure_ramp_time > 0) p *= fmin(t / C.pressure_ramp_time, 1.0);
+  double p;
+  if (C.pressure_prog.n) {
+    p = expr_eval(C.pressure_prog, t);
+  } else {
+    p = C.pressure_final;
+    if (C.pressure_ramp_time > 0) p *= fmin(t / C.pressure_ramp_time, 1.0);
+  }
   if (C.quirk_fulltime) p *= 0.0;
   return p;
 }
 
+/* target ratio of the enclosed volume to V0: the reference's fixed
+ * schedule (quirk, ShellSystem.C:1474), the config's volume_growth
+ * expression, or the linear ramp to volume_final_ratio */
 __device__ double volume_ratio_at(const SConst& C, double t)
 {
   if (C.quirk_volume_schedule) return fmax((1.0 - (double)((int)(t * 5.0)) * 0.01), 0.5);
+  if (C.volume_prog.n) return expr_eval(C.volume_prog, t);
   if (C.volume_ramp_time <= 0) return C.volume_final_ratio > 0 ? C.volume_final_ratio : 1.0;
   return 1.0 - (1.0 - C.volume_final_ratio) * fmin(t / C.volume_ramp_time, 1.0);
 }
@@ -308,6 +320,12 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   double r[kElemRec];
   wave_reduce_records<kElemRec>(P.epart + (size_t)C.blkE * kElemRec, C.nbE, kElemRec, op, r);
   if (threadIdx.x != 0) return;
+  {
+    /* the loads at the force time of this attempt (the node kernel's t_eval) */
+    const double t_eval = (mode == M_FORCES) ? w.t_eval : w.t + w.dt;
+    w.load_p = pressure_at(C, t_eval);
+    w.load_r = volume_ratio_at(C, t_eval);
+  }
   w.att_e_tens = r[0];
   w.att_e_bend = r[1];
   w.att_volume = fabs(r[3] / 6) - r[4] * 0.5 * C.h;
@@ -705,13 +723,12 @@ __global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) rec[q] = 0;
   if (valid) {
-    const double t_eval = (mode == M_FORCES) ? sh.t_eval : sh.t + dt;
-    double ptot = pressure_at(C, t_eval);
+    double ptot = sh.load_p;
     if (C.bulk_modulus != 0) {
       const double Vc = (C.volume_at_prediction || (mode == M_STEP && sh.fresh) || mode == M_INIT) ? sh.att_volume
                                                                                                     : sh.volume;
       const double V0 = (mode == M_INIT) ? (sh.volume_ref > 0 ? sh.volume_ref : Vc) : sh.volume0;
-      ptot = ptot + C.bulk_modulus * (V0 * volume_ratio_at(C, t_eval) / Vc - 1);
+      ptot = ptot + C.bulk_modulus * (V0 * sh.load_r / Vc - 1);
     }
     v3 fcn = mk(0, 0, 0);
     if (C.contact) {

@@ -37,17 +37,19 @@ def test_struct_layout_matches_header(tmp_path):
     src = tmp_path / "probe.c"
     fields_p = [f for f, _ in S.Params._fields_]
     fields_r = [f for f, _ in S.Report._fields_]
+    fields_c = [f for f, _ in S.Config._fields_]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "shellhip.h"', 'int main(void) {',
-             'printf("%zu %zu\\n", sizeof(shx_params), sizeof(shx_report));']
+             'printf("%zu %zu %zu\\n", sizeof(shx_params), sizeof(shx_report), sizeof(shx_config));']
     lines += [f'printf("p {f} %zu\\n", offsetof(shx_params, {f}));' for f in fields_p]
     lines += [f'printf("r {f} %zu\\n", offsetof(shx_report, {f}));' for f in fields_r]
+    lines += [f'printf("c {f} %zu\\n", offsetof(shx_config, {f}));' for f in fields_c]
     lines += ['return 0; }']
     src.write_text("\n".join(lines))
     exe = tmp_path / "probe"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
-    sp, sr = (int(x) for x in out[0].split())
-    assert sp == ctypes.sizeof(S.Params) and sr == ctypes.sizeof(S.Report)
+    sp, sr, sc = (int(x) for x in out[0].split())
+    assert sp == ctypes.sizeof(S.Params) and sr == ctypes.sizeof(S.Report) and sc == ctypes.sizeof(S.Config)
     off = {}
     for ln in out[1:]:
         if ln:
@@ -57,6 +59,8 @@ def test_struct_layout_matches_header(tmp_path):
         assert off[("p", f)] == getattr(S.Params, f).offset, f
     for f in fields_r:
         assert off[("r", f)] == getattr(S.Report, f).offset, f
+    for f in fields_c:
+        assert off[("c", f)] == getattr(S.Config, f).offset, f
 
 
 def test_host_triangle_distance_bitexact_with_oracle():
