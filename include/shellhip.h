@@ -264,6 +264,26 @@ int shx_batch_set_extra_damping(shx_ctx* ctx, const double* alpha);
 int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged,
                    shx_report* reps);
 int shx_batch_reports(shx_ctx* ctx, shx_report* reps);
+/* measure_properties (ShellSystem.C:2714-2846) of every shell's current
+ * configuration: one sample, weighted with the current dt, added to the
+ * moving averages (each accepted step adds its own sample on the GPU; the
+ * driver calls this once for the initial configuration, shell.C:302-307).
+ * Area, enclosed volume and the integrated Gaussian and mean curvatures
+ * are those of the configuration of the step's force evaluation. */
+int shx_batch_measure(shx_ctx* ctx);
+/* print_properties (ShellSystem.C:2849-2898) of one shell: the dt-weighted
+ * averages since the last call, as the reference forms them (MovingAverage:
+ * mean of the samples over the mean dt), then reset.  out[16]: dt, time,
+ * area, volume, density, pressure, Nc_cav, Nc_self, E_kin, E_tens, E_bend,
+ * E_cav, E_self, E_fric, kGauss, kMean (the cavity columns are 0). */
+int shx_batch_properties(shx_ctx* ctx, int shell, double* out);
+/* The measurement fields the reference writes to its .vtu files
+ * (ShellSystem.C:3156-3320), from a force evaluation at the current state:
+ * node_out[nv*5] = ecav (0 on this path), efric, ekin (over the node's mean
+ * element area), eself (the contacts' energy shares, ShellSystem.C:
+ * 1862-1877), mass; elem_out[ne*4] = ebend, etens, kGauss, kMean, each over
+ * the element's area.  The state is unchanged.  Either output may be NULL. */
+int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out);
 /* sorted candidate list of one shell, element ids local to the shell */
 int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap);
 int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega /* nshell */);

@@ -87,6 +87,9 @@ def _declare(L):
     L.orc_internal_force.argtypes = [_V, _D, _D]
     L.orc_init_dynamics.argtypes = [_V]
     L.orc_step.argtypes = [_V, ctypes.c_int, ctypes.POINTER(Report)]
+    L.orc_measure.argtypes = [_V]
+    L.orc_properties.argtypes = [_V, _D]
+    L.orc_elem_measure.argtypes = [_V, _D]
     L.orc_contact_rebuild.argtypes = [_V, ctypes.c_int]
     L.orc_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
     L.orc_pair_state.argtypes = [_V, _D, _I, _D, ctypes.c_int]
@@ -233,6 +236,19 @@ class Oracle:
         rep = Report()
         self.L.orc_step(self.h, n, ctypes.byref(rep))
         return {k: getattr(rep, k) for k, _ in Report._fields_}
+
+    def measure(self):
+        self.L.orc_measure(self.h)
+
+    def properties(self):
+        out = np.zeros(16)
+        self.L.orc_properties(self.h, dp(out))
+        return out
+
+    def elem_measure(self):
+        out = np.zeros((self.ne, 10))
+        self.L.orc_elem_measure(self.h, dp(out))
+        return out
 
     def rebuild(self, force=True):
         return self.L.orc_contact_rebuild(self.h, 1 if force else 0)

@@ -310,6 +310,11 @@ struct System {
   int Nc = 0;
   unsigned time_step = 0;
   double last_du_max = 0;
+  /* moving averages of measure_properties (MovingAverage samples summed in
+   * order): dt, area, volume, Nc, E_kin, E_tens, E_bend, E_self, E_fric,
+   * kGauss, kMean */
+  double avg[11] = {};
+  int avg_n = 0;
 
   void reg(int pr) { if (pr > problem) problem = pr; }
 };
@@ -1074,6 +1079,56 @@ double rigid_free_norm(const System& s, const std::vector<double>& f)
   return std::sqrt(std::max(f2 - proj, 0.0));
 }
 
+/* the curvature and area part of measure_properties (ShellSystem.C:
+ * 2728-2788): per element, from the deformed metrics of the last force
+ * evaluation, w = jac * qw (qw = 1/2), kGauss += w det(ff2) / det(ff1),
+ * kMean += w (ff2_0 ff1_1 + ff2_1 ff1_0 - 2 ff2_2 ff1_2) / (2 det(ff1)),
+ * area = sum of the element areas; summed in element order */
+void elem_curvature(const SurfaceMetric& ad, double& kG, double& kM)
+{
+  const double w = ad.jac * 0.5;
+  const double invdet = 1 / (ad.ff1[0] * ad.ff1[1] - ad.ff1[2] * ad.ff1[2]);
+  kG = w * invdet * (ad.ff2[0] * ad.ff2[1] - ad.ff2[2] * ad.ff2[2]);
+  kM = w * invdet * 0.5 * (ad.ff2[0] * ad.ff1[1] + ad.ff2[1] * ad.ff1[0] - 2 * ad.ff2[2] * ad.ff1[2]);
+}
+
+void measure_curvature(const System& s, double& kG, double& kM, double& area)
+{
+  kG = 0; kM = 0; area = 0;
+  for (int e = 0; e < s.ne; ++e) {
+    double g, m;
+    elem_curvature(s.ec[e].a_def, g, m);
+    kG += g;
+    kM += m;
+    area += s.ec[e].area;
+  }
+}
+
+double kinetic_energy(const System& s)
+{
+  double ekin = 0;
+  for (int n = 0; n < s.nv; ++n) {
+    double en = 0;
+    for (int q = 0; q < 3; ++q) en += s.v[3 * n + q] * s.v[3 * n + q] / s.Minv[3 * n + q];
+    ekin += 0.5 * en;
+  }
+  return ekin;
+}
+
+/* one sample of the moving averages, weighted with dt (ShellSystem.C:
+ * 2830-2843); the volume is the enclosed volume the force evaluation used
+ * (the GPU path measures it there; the reference re-forms it from the
+ * corrected positions) */
+void add_sample(System& s, double dt, double volume)
+{
+  double kG, kM, area;
+  measure_curvature(s, kG, kM, area);
+  const double smp[11] = {1.0, area, volume, (double)s.Nc, kinetic_energy(s), s.Etens, s.Ebend, s.Eself, s.Efric,
+                          kG, kM};
+  for (int q = 0; q < 11; ++q) s.avg[q] += dt * smp[q];
+  s.avg_n++;
+}
+
 /* ShellSystem::advance, ShellSystem.C:952-1122: step attempts repeat until
  * the local-error estimate accepts dt (adapt_dt_every > 0), as in the
  * reference; with adapt_dt_every = 0 every attempt is accepted. */
@@ -1099,6 +1154,7 @@ void advance(System& s, orc_report* rep)
     for (int i = 0; i < n3; ++i) s.v_pred[i] = A[6] * s.a[i] + s.v[i];
     contact_candidates(s, false, s.u_pred);
     evaluate_forces(s, s.t);
+    const double volume_used = s.volume;
     fnorm = l2(s.f);
     fext_norm = l2(s.fext);
     frigid = rigid_free_norm(s, s.f);
@@ -1133,6 +1189,7 @@ void advance(System& s, orc_report* rep)
       s.v.swap(s.v_pred);
       s.a.swap(s.a_pred);
       contact_candidates(s, false, s.u);
+      add_sample(s, s.dt, volume_used);
       compute_enclosed_volume(s);
       set_newmark(s, new_dt);
       break;
@@ -1326,6 +1383,53 @@ int orc_init_dynamics(void* h)
   contact_candidates(s, true, s.u_pred);
   evaluate_forces(s, s.t + s.dt);
   for (int i = 0; i < 3 * s.nv; ++i) s.a[i] = s.Minv[i] * s.f[i];
+  /* reset_properties (ShellSystem.C:876) */
+  for (double& x : s.avg) x = 0;
+  s.avg_n = 0;
+  return 0;
+}
+
+/* per element, from the last force evaluation: ff1[3], ff2[3], jac, the
+ * element's integrated Gaussian and mean curvature, its area */
+int orc_elem_measure(void* h, double* out /* ne*10 */)
+{
+  System& s = *static_cast<System*>(h);
+  for (int e = 0; e < s.ne; ++e) {
+    const SurfaceMetric& ad = s.ec[e].a_def;
+    double* o = out + 10 * (size_t)e;
+    for (int q = 0; q < 3; ++q) { o[q] = ad.ff1[q]; o[3 + q] = ad.ff2[q]; }
+    o[6] = ad.jac;
+    elem_curvature(ad, o[7], o[8]);
+    o[9] = s.ec[e].area;
+  }
+  return 0;
+}
+
+/* measure_properties of the current configuration (the driver's initial
+ * measurement, shell.C:302-307) */
+int orc_measure(void* h)
+{
+  System& s = *static_cast<System*>(h);
+  add_sample(s, s.dt, s.volume);
+  return 0;
+}
+
+/* print_properties (ShellSystem.C:2849-2898): averages since the last call
+ * (mean sample over mean dt, MovingAverage.h:25-35), then reset; out[16]:
+ * dt, time, area, volume, density, pressure, Nc_cav, Nc_self, E_kin,
+ * E_tens, E_bend, E_cav, E_self, E_fric, kGauss, kMean */
+int orc_properties(void* h, double* out)
+{
+  System& s = *static_cast<System*>(h);
+  const int n = s.avg_n;
+  auto mavg = [&](double sum) { return n == 0 ? 0.0 : sum / n; };
+  const double adt = mavg(s.avg[0]);
+  auto a = [&](int q) { return mavg(s.avg[q]) / adt; };
+  const double z = mavg(0.0) / adt;
+  const double vals[16] = {adt, s.t, a(1), a(2), 0.0, z, z, a(3), a(4), a(5), a(6), z, a(7), a(8), a(9), a(10)};
+  for (int q = 0; q < 16; ++q) out[q] = vals[q];
+  for (double& x : s.avg) x = 0;
+  s.avg_n = 0;
   return 0;
 }
 

@@ -100,6 +100,9 @@ int  orc_internal_force(void* h, const double* u_pred, double* f_int);
 
 int  orc_init_dynamics(void* h);       /* apply_initial_conditions without distortion */
 int  orc_step(void* h, int n_steps, orc_report* rep);
+int  orc_measure(void* h);
+int  orc_properties(void* h, double* out /* 16 */);
+int  orc_elem_measure(void* h, double* out /* ne*10: ff1, ff2, jac, kGauss, kMean, area */);
 int  orc_contact_rebuild(void* h, int force);
 int  orc_pairs(void* h, uint32_t* e1, uint32_t* e2, int cap);   /* returns count, sorted */
 int  orc_pair_state(void* h, double* d2, int32_t* stick, double* spring_origin, int cap);

@@ -133,6 +133,9 @@ def lib():
         L.shx_batch_pairs.argtypes = [_V, ctypes.c_int, _U, _U, ctypes.c_int]
         L.shx_batch_cfl_omega_max.argtypes = [_V, ctypes.c_int, _D]
         L.shx_set_graphs.argtypes = [_V, ctypes.c_int]
+        L.shx_batch_measure.argtypes = [_V]
+        L.shx_batch_properties.argtypes = [_V, ctypes.c_int, _D]
+        L.shx_batch_fields.argtypes = [_V, ctypes.c_int, _D, _D]
         _lib = L
     return _lib
 
@@ -172,7 +175,36 @@ E_ARG = -1
 E_CAPACITY = -4
 
 
-class ShellSystem:
+PROPERTY_COLUMNS = ["dt", "time", "area", "volume", "density", "pressure", "Nc_cav", "Nc_self", "E_kin", "E_tens",
+                    "E_bend", "E_cav", "E_self", "E_fric", "kGauss", "kMean"]
+NODE_FIELDS = ["ecav", "efric", "ekin", "eself", "mass"]
+ELEM_FIELDS = ["ebend", "etens", "kGauss", "kMean"]
+
+
+class _Measurements:
+    """measure_properties / print_properties and the .vtu fields
+    (ShellSystem.C:2714-2898, 3156-3320) through the C-ABI"""
+
+    def measure(self):
+        """one dt-weighted sample of every shell's current configuration"""
+        self._check(lib().shx_batch_measure(self.h), "shx_batch_measure")
+
+    def properties(self, shell=0):
+        """the averages since the last call (dict of PROPERTY_COLUMNS), then reset"""
+        out = np.zeros(16)
+        self._check(lib().shx_batch_properties(self.h, shell, _dp(out)), "shx_batch_properties")
+        return dict(zip(PROPERTY_COLUMNS, out.tolist()))
+
+    def fields(self, shell=0):
+        """node fields (nv x 5, NODE_FIELDS) and element fields (ne x 4, ELEM_FIELDS)"""
+        nv, ne = self._sizes(shell)
+        nf = np.zeros((nv, 5))
+        ef = np.zeros((ne, 4))
+        self._check(lib().shx_batch_fields(self.h, shell, _dp(nf), _dp(ef)), "shx_batch_fields")
+        return nf, ef
+
+
+class ShellSystem(_Measurements):
     """One shell on one GPU (mirrors class ShellSystem, ShellSystem.h:28)."""
 
     def __init__(self, X, T, device=0, **params):
@@ -210,6 +242,9 @@ class ShellSystem:
         self.close()
 
     # state
+    def _sizes(self, shell):
+        return self.nv, self.ne
+
     def set_state(self, u, v=None, a=None, t=0.0):
         n = 3 * self.nv
         u = _vec(u, n)
@@ -306,7 +341,7 @@ def _report_dict(rep):
     return {k: getattr(rep, k) for k, _ in Report._fields_}
 
 
-class ShellBatch:
+class ShellBatch(_Measurements):
     """A batch of independent shells on one GPU: every phase of a step is one
     kernel launch over all shells (the per-shell step of
     ShellSystem::advance, ShellSystem.C:952-1122, run side by side; each
@@ -351,6 +386,9 @@ class ShellBatch:
             self.close()
         except Exception:
             pass
+
+    def _sizes(self, shell):
+        return int(self.nv[shell]), int(self.ne[shell])
 
     def _ndof(self, shell):
         return 3 * int(self.nv.sum() if shell == ALL_SHELLS else self.nv[shell])

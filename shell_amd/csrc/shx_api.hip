@@ -728,7 +728,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.brec, (size_t)kBRec * std::max(nrecs, 1))) ||
       (r = dalloc(ctx, P.area, nE)) || (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) ||
       (r = dalloc(ctx, P.brec32, (size_t)kBRec * std::max(nrecs, 1))) ||
-      (r = dalloc(ctx, P.fcq, (size_t)6 * nV)) || (r = dalloc(ctx, P.urb, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) ||
+      (r = dalloc(ctx, P.fcq, (size_t)6 * nV)) || (r = dalloc(ctx, P.urb, (size_t)3 * nV)) || (r = dalloc(ctx, P.fout, (size_t)3 * nV)) || (r = dalloc(ctx, P.emeas, (size_t)4 * nE)) ||
+      (r = dalloc(ctx, P.nmeasq, (size_t)4 * nV)) || (r = dalloc(ctx, P.nmeas, (size_t)2 * nV)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)kElemRec * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
@@ -1007,6 +1008,105 @@ int shx_batch_reports(shx_ctx* ctx, shx_report* reps)
   if (int r = read_sh(ctx, s)) return r;
   for (int i = 0; i < ctx->S; ++i) fill_report(s[i], reps + i);
   return SHX_OK;
+}
+
+namespace {
+/* MovingAverage::average (MovingAverage.h:25-35): the samples' sum over
+ * their count (0 without samples) */
+double moving_average(double sum, int n) { return n == 0 ? 0.0 : sum / n; }
+}
+
+int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s0, s;
+  if (int r = read_sh(ctx, s0)) return r;
+  /* a plain force evaluation of every shell at its current state and time */
+  s = s0;
+  for (auto& d : s) { d.done = 0; d.t_eval = d.t; d.measure_only = 1; }
+  /* the element areas the next step's volume term reads are kept */
+  std::vector<double> area_keep(ctx->P.nE);
+  HIPCHK(hipMemcpy(area_keep.data(), ctx->P.area, sizeof(double) * ctx->P.nE, hipMemcpyDeviceToHost));
+  if (int r = write_sh(ctx, s)) return r;
+  if (int r = set_active(ctx, s)) return r;
+  launch_elem(ctx, M_FORCES);
+  if (ctx->any_contact) launch_contact(ctx, M_FORCES);
+  launch_node(ctx, M_FORCES);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const SConst& C = ctx->C[shell];
+  const int nv = C.nv, ne = C.ne, par = s0[shell].par;
+  std::vector<double> area(ne), em((size_t)4 * ne), nm((size_t)2 * nv, 0.0), v((size_t)3 * nv), minv(nv);
+  HIPCHK(hipMemcpy(area.data(), ctx->P.area + C.ebase, sizeof(double) * ne, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(em.data(), ctx->P.emeas + (size_t)4 * C.ebase, sizeof(double) * 4 * ne, hipMemcpyDeviceToHost));
+  if (C.contact)
+    HIPCHK(hipMemcpy(nm.data(), ctx->P.nmeas + (size_t)2 * C.vbase, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(v.data(), ctx->P.V[par] + (size_t)3 * C.vbase, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(minv.data(), ctx->P.minv + C.vbase, sizeof(double) * nv, hipMemcpyDeviceToHost));
+  /* the evaluation leaves the shells' state as it was */
+  HIPCHK(hipMemcpy(ctx->P.area, area_keep.data(), sizeof(double) * ctx->P.nE, hipMemcpyHostToDevice));
+  if (int r = write_sh(ctx, s0)) return r;
+  if (elem_out) std::copy(em.begin(), em.end(), elem_out);
+  if (node_out) {
+    /* avg_area_node (ShellSystem.C:2462-2480): mean area of the node's triangles */
+    const Topology& T = ctx->T[shell];
+    std::vector<double> asum(nv, 0.0);
+    std::vector<int> acnt(nv, 0);
+    for (int e = 0; e < ne; ++e)
+      for (int k = 0; k < 3; ++k) { asum[T.tri[3 * e + k]] += area[e]; acnt[T.tri[3 * e + k]]++; }
+    for (int n = 0; n < nv; ++n) {
+      double ek = 0;
+      for (int q = 0; q < 3; ++q) ek += v[3 * n + q] * v[3 * n + q] / minv[n];
+      ek *= 0.5;
+      double* o = node_out + 5 * (size_t)n;
+      o[0] = 0.0;                                      /* ecav */
+      o[1] = nm[2 * n + 1];                            /* efric */
+      o[2] = ek / (asum[n] / acnt[n]);                 /* ekin */
+      o[3] = nm[2 * n];                                /* eself */
+      o[4] = 1 / minv[n];                              /* mass */
+    }
+  }
+  return SHX_OK;
+}
+
+int shx_batch_measure(shx_ctx* ctx)
+{
+  if (!ctx) return SHX_E_ARG;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  for (auto& w : s) {
+    /* measure_properties (ShellSystem.C:2714-2846) of the current
+     * configuration, weighted with the current dt */
+    const double smp[11] = {1.0, w.area, w.volume, (double)w.n_contacts, w.e_kin, w.e_tens, w.e_bend,
+                            w.e_self, w.e_fric, w.kgauss, w.kmean};
+    for (int q = 0; q < 11; ++q) w.avg[q] += w.dt * smp[q];
+    w.avg_n++;
+  }
+  return write_sh(ctx, s);
+}
+
+int shx_batch_properties(shx_ctx* ctx, int shell, double* out)
+{
+  if (!ctx || !out) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  DShell& w = s[shell];
+  const int n = w.avg_n;
+  const double adt = moving_average(w.avg[0], n);
+  auto avg = [&](int q) { return moving_average(w.avg[q], n) / adt; };
+  /* print_properties (ShellSystem.C:2849-2898): dt, time, area, volume,
+   * density, cavity pressure, Nc_cav, Nc_self, E_kin, E_tens, E_bend,
+   * E_cav, E_self, E_fric, kGauss, kMean; the cavity columns are 0 on
+   * this path (their samples are 0, density is never set) */
+  const double zero_avg = moving_average(0.0, n) / adt;
+  const double vals[16] = {adt, w.t, avg(1), avg(2), 0.0, zero_avg, zero_avg, avg(3), avg(4), avg(5), avg(6),
+                           zero_avg, avg(7), avg(8), avg(9), avg(10)};
+  std::copy(vals, vals + 16, out);
+  for (double& a : w.avg) a = 0;
+  w.avg_n = 0;
+  return write_sh(ctx, s);
 }
 
 int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap)

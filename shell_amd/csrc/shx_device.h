@@ -76,7 +76,7 @@ constexpr int kNodeBlock = 256;     /* threads per node-kernel block, one node e
 constexpr int kNodePerBlock = kNodeBlock;
 constexpr int kBRec = 6;            /* per block-node record: sum of slot contributions q, sum of phi JxW a3 */
 constexpr int kNodeRec = 23;        /* node-kernel partial record length */
-constexpr int kElemRec = 6;         /* element-kernel partial record length */
+constexpr int kElemRec = 9;         /* element-kernel partial record length */
 constexpr double kTentative = -1.0;  /* d2 of a candidate pair whose exact distance is not yet formed */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
 constexpr int kBpLanes = 16;        /* pair-search lanes per element (cells, then bucket entries) */
@@ -140,7 +140,17 @@ struct DShell {
   int n_pairs, pair_buf, pair_src, bclear;
   int steps_target, done, attempts;
   int eprec;                       /* element-force precision of the shell (set by the host): 0 FP64, 1 FP32 split */
+  int measure_only;                /* set by the host for a measurement evaluation: the contact kernel
+                                      leaves the pairs' distance and friction state unchanged */
   int n_fp32;                      /* accepted steps taken with FP32 split forces */
+  double kgauss, kmean;            /* integrated Gaussian and mean curvature of the last accepted step's
+                                      force-evaluation configuration (measure_properties) */
+  double att_kgauss, att_kmean, att_narea;   /* att_narea: area of the evaluated configuration */
+  double avg[11];                  /* dt-weighted sums since the last properties read, in sample order
+                                      (MovingAverage, ShellSystem.C:2830-2843): dt, area, volume, Nc, E_kin,
+                                      E_tens, E_bend, E_self, E_fric, kGauss, kMean */
+  int avg_n;                       /* samples in avg */
+  int avg_pad;
   double load_p, load_r;           /* pressure and target volume ratio at the force time of the current
                                       attempt (the element epilogue evaluates them, the node kernel applies) */
   /* results of the element kernel of the current attempt */
@@ -197,6 +207,11 @@ struct DPtrs {
                                  order of their atomic adds */
   double* urb;                /* nV*3 displacement each node's contact list was built at (rebuild trigger 1) */
   double* fout;               /* nV*3 outputs of a plain force evaluation */
+  double* emeas;              /* 4*nE: per element E_bend, E_tens, kGauss, kMean over the element area
+                                 (the reference's elem_vectors), written by plain force evaluations */
+  unsigned long long* nmeasq; /* 4*nV: per node the self-contact and friction energy shares (128-bit
+                                 fixed point each, as fcq), plain force evaluations only */
+  double* nmeas;              /* 2*nV: the same in FP64 (E_self, E_fric per node) */
   double* fext_out;
   double* epart;              /* element kernel block partials (kElemRec per block) */
   double* npart;              /* node kernel block partials (kNodeRec per block) */

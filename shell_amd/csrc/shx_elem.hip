@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
     sn[3] = x.x; sn[4] = x.y; sn[5] = x.z;
   }
   __syncthreads();
-  double Et = 0, Eb = 0, det = 0, oarea = 0;
+  double Et = 0, Eb = 0, det = 0, oarea = 0, kgs = 0, kmn = 0, narea = 0;
   v3 du1 = mk(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
   double R00 = 0, R01 = 0, R10 = 0, R11 = 0;
   int N = 6;
@@ -206,6 +206,17 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
     const double new_area = 0.5 * jac;
     oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
     P.area[e] = new_area;
+    narea = new_area;
+    /* curvatures of the deformed surface, measure_properties (ShellSystem.C:
+     * 2746-2751): weight jac * qw (qw = 1/2), over det(ff1) */
+    const double wd = new_area / (ff1[0] * ff1[1] - ff1[2] * ff1[2]);
+    kgs = wd * (ff2[0] * ff2[1] - ff2[2] * ff2[2]);
+    kmn = wd * 0.5 * (ff2[0] * ff1[1] + ff2[1] * ff1[0] - 2 * ff2[2] * ff1[2]);
+    if (mode == M_FORCES) {
+      /* the per-element measurement vectors (ShellSystem.C:1299-1300, 2783-2784) */
+      double* em = P.emeas + 4 * (size_t)e;
+      em[0] = Eb / new_area; em[1] = Et / new_area; em[2] = kgs / new_area; em[3] = kmn / new_area;
+    }
   }
   /* block records in four passes of three patch slots: each element thread
    * writes q_i (f_i -= q_i) of its slots 3g .. 3g+2 to LDS, then each
@@ -272,7 +283,7 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
     }
   }
   /* block partial: wave reductions (DPP), then the block's waves in order */
-  double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
+  double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev, kgs, kmn, narea};
 #pragma unroll
   for (int c = 0; c < kElemRec; ++c) rr[c] = (c == 2 || c == 5) ? wave_reduce<true>(rr[c]) : wave_reduce<false>(rr[c]);
   if ((t & 63) == 0)
@@ -343,7 +354,7 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     sn[3] = x.x; sn[4] = x.y; sn[5] = x.z;
   }
   __syncthreads();
-  double Et = 0, Eb = 0, det = 0, oarea = 0;
+  double Et = 0, Eb = 0, det = 0, oarea = 0, kgs = 0, kmn = 0, narea = 0;
   v3f du1 = mkf(0, 0, 0), du2 = du1, du11 = du1, du22 = du1, du12 = du1;
   float R00 = 0, R01 = 0, R10 = 0, R11 = 0;
   int N = 6;
@@ -443,6 +454,17 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     const double new_area = 0.5 * (double)jac;
     oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
     P.area[e] = new_area;
+    narea = new_area;
+    /* curvatures (measure_properties, ShellSystem.C:2746-2751) from the
+     * deformed first fundamental form ff1 = A.A + the metric changes */
+    const float f0 = dotf(A1, A1) + 2 * alpha[0], f1 = dotf(A2, A2) + 2 * alpha[1], f2 = dotf(A1, A2) + alpha[2];
+    const double wd = new_area / ((double)f0 * f1 - (double)f2 * f2);
+    kgs = wd * ((double)ff2[0] * ff2[1] - (double)ff2[2] * ff2[2]);
+    kmn = wd * 0.5 * ((double)ff2[0] * f1 + (double)ff2[1] * f0 - 2.0 * ff2[2] * f2);
+    if (mode == M_FORCES) {
+      double* em = P.emeas + 4 * (size_t)e;
+      em[0] = Eb / new_area; em[1] = Et / new_area; em[2] = kgs / new_area; em[3] = kmn / new_area;
+    }
   }
   /* block records in four passes of three patch slots (as k_elem) */
   const float* rb = s_rb + 72 * (N - 3);
@@ -501,7 +523,7 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
       rec[0] = rq[m].x; rec[1] = rq[m].y; rec[2] = rq[m].z; rec[3] = rp[m].x; rec[4] = rp[m].y; rec[5] = rp[m].z;
     }
   }
-  double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev};
+  double rr[kElemRec] = {Et, Eb, dumax, det, oarea, dev, kgs, kmn, narea};
 #pragma unroll
   for (int c = 0; c < kElemRec; ++c) rr[c] = (c == 2 || c == 5) ? wave_reduce<true>(rr[c]) : wave_reduce<false>(rr[c]);
   if ((t & 63) == 0)

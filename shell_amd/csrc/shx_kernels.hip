@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   /* the corrector list of the previous attempt has been consumed */
   if (sid == 0 && threadIdx.x == 0) P.wl[1][0] = 0;
   if (w.done) return;
-  const int op[kElemRec] = {0, 0, 1, 0, 0, 1};
+  const int op[kElemRec] = {0, 0, 1, 0, 0, 1, 0, 0, 0};
   double r[kElemRec];
   wave_reduce_records<kElemRec>(P.epart + (size_t)C.blkE * kElemRec, C.nbE, kElemRec, op, r);
   if (threadIdx.x != 0) return;
@@ -330,6 +330,9 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   w.att_e_bend = r[1];
   w.att_volume = fabs(r[3] / 6) - r[4] * 0.5 * C.h;
   w.att_area = r[4];
+  w.att_kgauss = r[6];
+  w.att_kmean = r[7];
+  w.att_narea = r[8];
   w.bclear = 0;
   if (mode == M_STEP) {
     check_increment_dev(w, C, r[2], false);
@@ -483,7 +486,8 @@ __global__ void __launch_bounds__(256, 3) k_contact(DPtrs P, int mode)
       }
       tri_hit hit;
       tri_tri(X0, X1, hit);
-      P.pd2[buf][p] = hit.d2;
+      const bool keep_state = sh.measure_only;
+      if (!keep_state) P.pd2[buf][p] = hit.d2;
       if (hit.d2 <= 0) atomicMax(&P.sh[sid].problem, 6); /* CONTACT_PROBLEM */
       int stick = P.pstick[buf][p];
       v3 so = ld3(P.pso[buf], p);
@@ -528,12 +532,26 @@ __global__ void __launch_bounds__(256, 3) k_contact(DPtrs P, int mode)
           }
         }
         if (!fin) atomicMax(&P.sh[sid].problem, 7); /* NOT_FINITE */
+        if (mode == M_FORCES) {
+          /* the nodes' shares of the contact energies, prefactor u / (2 area_min)
+           * per barycentric weight u (ShellSystem.C:1862-1877) */
+          for (int s = 0; s < 2; ++s) {
+            const double vb = wv[s], wb = ww[s], ub = 1 - vb - wb;
+            const double pf[3] = {0.5 * ub / amin, 0.5 * vb / amin, 0.5 * wb / amin};
+            for (int j = 0; j < 3; ++j) {
+              fx_add(&P.nmeasq[4 * nid[s][j]], pf[j] * Es, C.fx_shift);
+              fx_add(&P.nmeasq[4 * nid[s][j] + 2], pf[j] * Ef, C.fx_shift);
+            }
+          }
+        }
       } else {
         so = mk(0, 0, 0);
         stick = 0;
       }
-      P.pstick[buf][p] = stick;
-      st3(P.pso[buf], p, so);
+      if (!keep_state) {
+        P.pstick[buf][p] = stick;
+        st3(P.pso[buf], p, so);
+      }
     }
   }
   double rr[kCRec] = {nc, es, ef, npair};
@@ -619,6 +637,12 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     w.volume = w.att_volume;
     w.volume0 = w.volume_ref > 0 ? w.volume_ref : w.att_volume;
     w.area = w.att_area;
+    w.kgauss = w.att_kgauss;
+    w.kmean = w.att_kmean;
+    w.e_kin = ek;
+    /* reset_properties (ShellSystem.C:876): no samples yet */
+    for (int q = 0; q < 11; ++q) w.avg[q] = 0;
+    w.avg_n = 0;
     /* residuals of the initial state (with its own V0 when volume_ref is 0) */
     w.residual = w.fext_norm > 0 ? w.fnorm / w.fext_norm : w.fnorm;
     w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
@@ -651,6 +675,18 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     w.residual = w.fext_norm > 0 ? w.fnorm / w.fext_norm : w.fnorm;
     w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
     w.e_kin = ek;
+    w.kgauss = w.att_kgauss;
+    w.kmean = w.att_kmean;
+    /* measure_properties (ShellSystem.C:1097, 2830-2843): the accepted
+     * step's sample weighted with its dt, added before dt changes; area and
+     * curvatures of the step's force-evaluation configuration (as the
+     * reference), the enclosed volume the step's force evaluation used */
+    {
+      const double smp[11] = {1.0, w.att_narea, w.att_volume, nc, ek, w.e_tens, w.e_bend, es, ef, w.kgauss,
+                              w.kmean};
+      for (int q = 0; q < 11; ++q) w.avg[q] += dt * smp[q];
+      w.avg_n++;
+    }
     w.rebuild = rebuild_due(w, C);
     if (w.rebuild) plan_rebuild(P, w, sid, 1);
     set_newmark_dev(w, C, new_dt);
@@ -759,8 +795,15 @@ __global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
     if (mode == M_FORCES) {
       st3(P.fout, n, f);
       st3(P.fext_out, n, fext);
+      if (C.contact) {
+        unsigned long long* q = P.nmeasq + 4 * (size_t)n;
+        P.nmeas[2 * (size_t)n] = fx_get(q[0], q[1], C.fx_shift);
+        P.nmeas[2 * (size_t)n + 1] = fx_get(q[2], q[3], C.fx_shift);
+        q[0] = q[1] = q[2] = q[3] = 0;
+      }
     } else if (mode == M_INIT) {
       st3(P.A[par], n, scl(f, mi));
+      rec[5] = 0.5 * (v.x * v.x / mi + v.y * v.y / mi + v.z * v.z / mi);
     } else {
       const double cd = P.cdamp[n] + sh.alpha_extra / mi;   /* + alpha_extra m */
       const double a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];

@@ -790,3 +790,147 @@ def test_volume_constraint_is_a_uniform_pressure_at_init(icosphere):
         o.close()
     assert np.linalg.norm(out[0]) > 0
     assert np.linalg.norm(out[0] - out[1]) <= 1e-13 * np.linalg.norm(out[1])
+
+
+def _ellipsoid_integrals(a, b, c, n=1600):
+    """integrated mean curvature, Gaussian curvature and area of the
+    ellipsoid with semi-axes a, b, c by dense midpoint quadrature of its
+    (theta, phi) parametrisation (outward normal: mean curvature < 0)"""
+    th = (np.arange(n) + 0.5) * np.pi / n
+    ph = (np.arange(2 * n) + 0.5) * np.pi / n
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    st, ct, sp, cp = np.sin(T), np.cos(T), np.sin(P), np.cos(P)
+    xt = np.stack([a * ct * cp, b * ct * sp, -c * st])
+    xp = np.stack([-a * st * sp, b * st * cp, 0 * T])
+    xtt = np.stack([-a * st * cp, -b * st * sp, -c * ct])
+    xpp = np.stack([-a * st * cp, -b * st * sp, 0 * T])
+    xtp = np.stack([-a * ct * sp, b * ct * cp, 0 * T])
+    E, F, G = (xt * xt).sum(0), (xt * xp).sum(0), (xp * xp).sum(0)
+    nv = np.cross(xt, xp, axis=0)
+    J = np.sqrt((nv ** 2).sum(0))
+    nv = nv / J
+    e, f, g = (xtt * nv).sum(0), (xtp * nv).sum(0), (xpp * nv).sum(0)
+    H = (e * G - 2 * f * F + g * E) / (2 * (E * G - F * F))
+    K = (e * g - f * f) / (E * G - F * F)
+    dA = J * (np.pi / n) ** 2
+    return (H * dA).sum(), (K * dA).sum(), dA.sum()
+
+
+def _measured(icosphere, level, axes, handle=None):
+    """area, integrated Gaussian and mean curvature measured by the oracle
+    (measure_properties) on the icosphere mapped by diag(axes)"""
+    X, T = icosphere(level)
+    o = O.Oracle(X, T, lib_handle=handle)
+    o.set_state((X * np.array(axes) - X).reshape(-1))
+    o.init_dynamics()
+    o.measure()
+    p = o.properties()
+    o.close()
+    return p[2], p[14], p[15]
+
+
+@pytest.mark.parametrize("axes", [(1.0, 1.0, 1.0), (1.3, 0.8, 1.1)])
+def test_curvature_measurement_closed_forms(icosphere, axes):
+    """measure_properties (ShellSystem.C:2728-2788) on the limit surface of
+    an icosphere mapped by diag(a, b, c) (Loop surfaces are affine
+    invariant: the limit of the mapped net is the mapped limit, which tends
+    to the ellipsoid): the integrated Gaussian curvature is 4 pi
+    (Gauss-Bonnet), the integrated mean curvature and the area are the
+    ellipsoid's (dense parametric quadrature; -4 pi and 4 pi on the unit
+    sphere).  The discrete values converge at second order; the Richardson
+    extrapolation of levels 3 and 4 meets the closed forms to 2e-4.  The
+    ellipsoid's off-diagonal metric (F != 0 in the element frames) brings in
+    every term of both formulas."""
+    H, K, A = _ellipsoid_integrals(*axes)
+    r3 = _measured(icosphere, 3, axes)
+    r4 = _measured(icosphere, 4, axes)
+    ext = [r4[i] + (r4[i] - r3[i]) / 3 for i in range(3)]
+    assert abs(K - 4 * np.pi) < 1e-5
+    assert abs(ext[0] - A) < 2e-4 * A
+    assert abs(ext[1] - K) < 2e-4 * K
+    assert abs(ext[2] - H) < 2e-4 * abs(H)
+
+
+def _shape_operator_curvatures(m):
+    """per element, K and H from the eigenvalues of the shape operator
+    I^-1 II (numpy), times the element weight jac / 2"""
+    out = []
+    for r in m:
+        I = np.array([[r[0], r[2]], [r[2], r[1]]])
+        II = np.array([[r[3], r[5]], [r[5], r[4]]])
+        lam = np.linalg.eigvals(np.linalg.solve(I, II)).real
+        w = r[6] * 0.5
+        out.append((w * lam[0] * lam[1], w * 0.5 * (lam[0] + lam[1])))
+    return np.array(out)
+
+
+def _sheared_elem_measure(handle=None):
+    X, T = O.icosphere(3)
+    M = np.array([[1.3, 0.5, 0.0], [-0.3, 0.8, 0.4], [0.2, -0.4, 1.1]])
+    o = O.Oracle(X, T, lib_handle=handle)
+    o.forces((X @ M.T - X).reshape(-1))
+    m = o.elem_measure()
+    o.close()
+    return m
+
+
+def test_curvature_per_element_is_the_shape_operator():
+    """each element's Gaussian and mean curvature (measure_properties,
+    ShellSystem.C:2746-2751) equal the determinant and half trace of the
+    shape operator I^-1 II formed from the same fundamental forms by an
+    eigen-decomposition, on a sheared shell whose element frames are far
+    from orthogonal (ff1_2 != 0)"""
+    m = _sheared_elem_measure()
+    assert np.max(np.abs(m[:, 2]) / np.sqrt(m[:, 0] * m[:, 1])) > 0.2
+    ref = _shape_operator_curvatures(m)
+    assert np.allclose(m[:, 7], ref[:, 0], rtol=1e-12, atol=0)
+    assert np.allclose(m[:, 8], ref[:, 1], rtol=1e-12, atol=0)
+    assert np.allclose(m[:, 9], 0.5 * m[:, 6], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("slip", ["mean without 1/2", "mean without cross term", "gauss without ff2_2^2"])
+def test_curvature_pins_catch_planted_slips(slip):
+    from oracle_mutants import build
+    kg = "kG = w * invdet * (ad.ff2[0] * ad.ff2[1] - ad.ff2[2] * ad.ff2[2]);"
+    km = "kM = w * invdet * 0.5 * (ad.ff2[0] * ad.ff1[1] + ad.ff2[1] * ad.ff1[0] - 2 * ad.ff2[2] * ad.ff1[2]);"
+    old, new = {
+        "mean without 1/2": (km, km.replace("0.5 * ", "")),
+        "mean without cross term": (km, km.replace(" - 2 * ad.ff2[2] * ad.ff1[2]", "")),
+        "gauss without ff2_2^2": (kg, kg.replace(" - ad.ff2[2] * ad.ff2[2]", "")),
+    }[slip]
+    L = build(old, new, "curv_%d" % ["mean without 1/2", "mean without cross term",
+                                     "gauss without ff2_2^2"].index(slip))
+    m = _sheared_elem_measure(L)
+    ref = _shape_operator_curvatures(m)
+    assert not (np.allclose(m[:, 7], ref[:, 0], rtol=1e-6) and np.allclose(m[:, 8], ref[:, 1], rtol=1e-6))
+
+
+@pytest.mark.parametrize("adapt", [0, 1])
+def test_moving_averages_are_dt_weighted_means(icosphere, adapt):
+    """print_properties (ShellSystem.C:2849-2898) with MovingAverage
+    (MovingAverage.h:25-35): each column is the dt-weighted mean over the
+    accepted steps since the last read (numpy's weighted average of the
+    per-step values here), the dt column the mean step, and a read resets"""
+    X, T = icosphere(3)
+    X = imperfect(X, 0.01 * H, seed=8)
+    o = O.Oracle(X, T, pressure_final=-2 * PCR, pressure_ramp_time=3.0, dt=0.1, damping_alpha=0.05,
+                 adapt_dt_every=adapt, newmark_beta=0.25 if adapt else 0.0)
+    o.set_state(np.zeros(X.size))
+    o.init_dynamics()
+    o.properties()                           # start clean
+    dts, et, eb, ek, t0 = [], [], [], [], 0.0
+    for _ in range(25):
+        r = o.step(1)
+        dts.append(r["t"] - t0)
+        t0 = r["t"]
+        et.append(r["e_tens"]); eb.append(r["e_bend"]); ek.append(r["e_kin"])
+    p = o.properties()
+    assert p[1] == t0
+    assert abs(p[0] - np.mean(dts)) <= 1e-14 * np.mean(dts)
+    for col, vals in ((9, et), (10, eb), (8, ek)):
+        assert abs(p[col] - np.average(vals, weights=dts)) <= 1e-12 * abs(np.average(vals, weights=dts))
+    if adapt:
+        assert np.std(dts) > 0                # the weights differ
+    again = o.properties()                    # reset: no samples left
+    assert again[0] == 0
+    o.close()
