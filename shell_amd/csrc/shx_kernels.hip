@@ -682,8 +682,10 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
      * curvatures of the step's force-evaluation configuration (as the
      * reference), the enclosed volume the step's force evaluation used */
     {
-      const double smp[11] = {1.0, w.att_narea, w.att_volume, nc, ek, w.e_tens, w.e_bend, es, ef, w.kgauss,
-                              w.kmean};
+      /* the volume the force evaluation used (the previous accepted step's
+       * after a rejection, when the volume is not taken at the prediction) */
+      const double vol = C.volume_at_prediction ? w.att_volume : w.volume;
+      const double smp[11] = {1.0, w.att_narea, vol, nc, ek, w.e_tens, w.e_bend, es, ef, w.kgauss, w.kmean};
       for (int q = 0; q < 11; ++q) w.avg[q] += dt * smp[q];
       w.avg_n++;
     }

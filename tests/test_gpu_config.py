@@ -38,7 +38,7 @@ def test_expression_equals_ramp(icosphere, kind):
     base = dict(dt=0.1, damping_alpha=0.05)
     if kind == "pressure":
         ramp = dict(base, pressure_final=-2 * PCR, pressure_ramp_time=5.0)
-        expr = dict(base, pressure_fn=f"{-2 * PCR!r}*min(t/5,1)", load_end_time=5.0)
+        expr = dict(base, pressure_fn=f"{float(-2 * PCR)!r}*min(t/5,1)", load_end_time=5.0)
     else:
         ramp = dict(base, bulk_modulus=0.05, volume_final_ratio=0.9, volume_ramp_time=8.0, contact=1,
                     cell_size_mult=1.0)
@@ -63,7 +63,7 @@ shell_poisson_ratio = 0.3
 shell_shear_modulus = 0.35
 thickness = {H}
 damping_alpha = 0.05
-pressure = {-1.5 * PCR!r}*(t<4 ? t/4 : 1)
+pressure = '{float(-1.5 * PCR)!r}*(t<4 ? t/4 : 1)'   # a quoted GetPot value may hold spaces
 shell_shell_contact = none
 shell_cavity_contact = false
 newmark_beta = 0

@@ -20,24 +20,40 @@ def say(*a):
     print(*a, flush=True)
 
 
-def test_properties_and_fields_against_oracle(icosphere):
+@pytest.mark.parametrize("case", ["deflation", "closing sheets"])
+def test_properties_and_fields_against_oracle(icosphere, case):
+    """deflation: the volume-controlled post-buckling run (constant dt);
+    closing sheets: a squashed shell whose sheets close into contact with
+    friction under the reference's adaptive dt (unequal sample weights)"""
     X, T = icosphere(3)
     X = imperfect(X, 0.01 * H, seed=41)
-    kw = dict(E2=0.6, bulk_modulus=0.05, volume_final_ratio=0.85, volume_ramp_time=30.0, dt=0.1, dt_max=0.1,
-              damping_alpha=0.05, volume_at_prediction=1, contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0,
-              cell_margin=0.1)
+    z = np.zeros(X.size)
+    if case == "deflation":
+        kw = dict(E2=0.6, bulk_modulus=0.05, volume_final_ratio=0.85, volume_ramp_time=30.0, dt=0.1, dt_max=0.1,
+                  damping_alpha=0.05, volume_at_prediction=1, contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0,
+                  cell_margin=0.1)
+        u0, v0, steps = z, z, 150
+    else:
+        kw = dict(dt=0.02, dt_max=0.02, damping_alpha=0.05, adapt_dt_every=1, contact=1, mu_s=0.3, mu_d=0.3,
+                  cell_size_mult=1.0, rebuild_trigger=1)
+        Y = X.copy()
+        Y[:, 2] *= 0.03
+        u0 = (Y - X).reshape(-1)
+        v0 = np.zeros_like(X)
+        v0[:, 2] = -0.02 * np.sign(X[:, 2])
+        v0 = v0.reshape(-1)
+        steps = 70
     g = S.ShellSystem(X, T, **kw)
     o = oracle.Oracle(X, T, **kw)
-    z = np.zeros(X.size)
-    g.set_state(z)
-    o.set_state(z)
+    g.set_state(u0, v0)
+    o.set_state(u0, v0)
     g.apply_initial_conditions()
     o.init_dynamics()
     g.measure()
     o.measure()
     for chunk in range(3):
-        g.advance(150)
-        o.step(150)
+        g.advance(steps)
+        o.step(steps)
         pg = g.properties()
         po = o.properties()
         say("case properties chunk", chunk, {k: f"{pg[k]:.6g}" for k in ("area", "volume", "Nc_self", "E_kin",
@@ -45,7 +61,8 @@ def test_properties_and_fields_against_oracle(icosphere):
         for k, name in enumerate(S.PROPERTY_COLUMNS):
             a, b = pg[name], po[k]
             assert abs(a - b) <= 1e-9 * max(abs(b), 1e-12), (name, a, b)
-    assert pg["Nc_self"] > 0                         # the contact columns are exercised
+    if case == "closing sheets":
+        assert pg["Nc_self"] > 0 and pg["E_self"] > 0    # the contact columns are exercised
     # fields at the current state, against the oracle evaluated at the same state
     nf, ef = g.fields()
     u, v, _, t = g.get_state()
@@ -72,21 +89,35 @@ def test_properties_and_fields_against_oracle(icosphere):
 
 
 def test_fields_leave_the_trajectory_unchanged(icosphere):
-    """a fields evaluation in the middle of a run changes nothing the run
-    uses: the run with it equals the run without it bit for bit"""
+    """a fields evaluation in the middle of a run with sticking and sliding
+    contacts changes nothing the run uses (pair distances, friction springs,
+    element areas): the run with it equals the run without it bit for bit"""
     X, T = icosphere(3)
     X = imperfect(X, 0.01 * H, seed=42)
-    kw = dict(bulk_modulus=0.05, volume_final_ratio=0.85, volume_ramp_time=20.0, dt=0.1, dt_max=0.1,
-              damping_alpha=0.05, contact=1, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0, cell_margin=0.1)
+    kw = dict(dt=0.02, dt_max=0.02, damping_alpha=0.05, adapt_dt_every=1, contact=1, mu_s=0.3, mu_d=0.3,
+              cell_size_mult=1.0, rebuild_trigger=1)
+    Y = X.copy()
+    Y[:, 2] *= 0.03
+    u0 = (Y - X).reshape(-1)
+    v0 = np.zeros_like(X)
+    v0[:, 2] = -0.02 * np.sign(X[:, 2])
     out = []
     for with_fields in (False, True):
         g = S.ShellSystem(X, T, **kw)
-        g.set_state(np.zeros(X.size))
+        g.set_state(u0, v0.reshape(-1))
         g.apply_initial_conditions()
-        g.advance(200)
+        steps = 0
+        while True:                                  # to a step with penetrating contacts
+            r = g.advance(10)
+            steps += 10
+            if r["n_contacts"] > 0 or steps >= 400:
+                break
+        assert r["n_contacts"] > 0
         if with_fields:
-            g.fields()
-        g.advance(200)
+            nf, _ = g.fields()
+            assert np.max(nf[:, 3]) > 0              # the contacts' energy shares
+        g.advance(80)
         out.append(g.get_state()[0])
         g.close()
+    say("case fields mid-run: contacts", r["n_contacts"])
     assert np.array_equal(out[0], out[1])
