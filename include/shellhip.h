@@ -284,12 +284,72 @@ int shx_batch_properties(shx_ctx* ctx, int shell, double* out);
  * 1862-1877), mass; elem_out[ne*4] = ebend, etens, kGauss, kMean, each over
  * the element's area.  The state is unchanged.  Either output may be NULL. */
 int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out);
+/* The contact candidates of one shell sorted by (e1, e2), element ids local
+ * to the shell, with each pair's last distance d2, stick flag and static
+ * friction spring origin (3 per pair); returns the count (may exceed cap;
+ * any output may be NULL).  The state of ShellShellContact (Contact.h:
+ * 19-20) that the reference saves (ShellSystem.C:3003-3025). */
+int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, double* d2, int32_t* stick,
+                         double* origin, int cap);
+/* Sticking contacts of a loaded state (ShellSystem.C:3093-3121): the n
+ * pairs become the shell's candidate list with their spring origins, and
+ * the shell's next init keeps them (they survive its contact search, as
+ * the reference keeps them) instead of starting without contacts. */
+int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, const uint32_t* e2,
+                         const double* origin);
+/* The raw moving-average accumulators of one shell (set == 0: read into
+ * sums[11] and *n; set != 0: write them): dt-weighted sums of dt, area,
+ * volume, Nc, E_kin, E_tens, E_bend, E_self, E_fric, kGauss, kMean and the
+ * sample count (state files carry them, ShellSystem.C:2984-3001). */
+int shx_batch_averages(shx_ctx* ctx, int shell, double* sums, int* n, int set);
+/* The parameters one shell was created with. */
+int shx_batch_params(shx_ctx* ctx, int shell, shx_params* out);
+/* The mesh of one shell: node positions as given at create, triangles with
+ * their vertices rotated so that the irregular vertex comes first (the
+ * element node order of the reference's subdivision mesh); any output may
+ * be NULL. */
+int shx_batch_mesh(shx_ctx* ctx, int shell, int* nv, int* ne, double* X, int32_t* tri);
+/* set_newmark_parameters(dt) of one shell (ShellSystem.C:744-759) */
+int shx_batch_set_dt(shx_ctx* ctx, int shell, double dt);
 /* sorted candidate list of one shell, element ids local to the shell */
 int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap);
 int shx_batch_cfl_omega_max(shx_ctx* ctx, int iters, double* omega /* nshell */);
 /* 1 (default): step attempts are replayed from a captured hipGraph of 64
  * attempts; 0: launched one by one.  Profiling launches one by one. */
 int shx_set_graphs(shx_ctx* ctx, int enable);
+
+/* On-disk formats.  shx_state_save writes one shell's state file
+ * (ShellSystem::save, ShellSystem.C:2956-3049, after the driver's leading
+ * time step, shell.C:380-387): dt, time, wall time, per node u v a per
+ * coordinate and the cavity spring origin (0), the measurement samples
+ * (written as their count of mean samples), the sticking contacts.
+ * shx_state_load reads such a file into one shell (ShellSystem::load,
+ * :3050-3153, then apply_initial_conditions keeping the loaded
+ * acceleration; the loaded dt is taken only when adapt_dt_every > 0).
+ * shx_write_vtu writes the shell's "Serial vtkUnstructuredGrid" file
+ * (ShellSystem::write_vtk, :3156-3320) with the fields of
+ * shx_batch_fields.  Errors: shx_io_last_error(). */
+int shx_state_save(shx_ctx* ctx, int shell, const char* path, uint32_t time_step, double wall_time);
+int shx_state_load(shx_ctx* ctx, int shell, const char* path, int precision, uint32_t* time_step, double* wall_time);
+int shx_write_vtu(shx_ctx* ctx, int shell, const char* path);
+const char* shx_io_last_error(void);
+
+/* Mesh preparation of the driver (shell.C:146-230), host only.  Counts are
+ * returned when the arrays are NULL.  shx_mesh_icosphere: the unit
+ * icosphere of a level (midpoint subdivision of the icosahedron projected
+ * to R = 1; round6: coordinates printed and read back with 6 significant
+ * digits, as the reference's icosasphere3/4.off hold them).
+ * shx_mesh_read_off / shx_mesh_write_off: OFF text of triangle meshes.
+ * shx_mesh_refine: uniform midpoint refinement (MeshRefinement::
+ * uniformly_refine).  shx_mesh_transform: the driver's project, scale,
+ * rotate (Euler z-x-z in turns) and translate, each NULL to skip. */
+int shx_mesh_icosphere(int level, int round6, int* nv, int* ne, double* X, int32_t* tri);
+int shx_mesh_read_off(const char* text, int* nv, int* ne, double* X, int32_t* tri);
+int shx_mesh_write_off(int nv, int ne, const double* X, const int32_t* tri, char* out, size_t cap, size_t* len);
+int shx_mesh_refine(int nv, int ne, const double* X, const int32_t* tri, int levels, int* nv_out, int* ne_out,
+                    double* X_out, int32_t* tri_out);
+int shx_mesh_transform(int nv, double* X, const double* project, const double* scale, const double* rotate,
+                       const double* translate);
 
 /* Learned warm start (no counterpart in the reference): a multilayer
  * perceptron width[0] -> width[1] -> ... -> width[n_layers] with GELU on the

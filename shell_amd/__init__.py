@@ -524,3 +524,90 @@ def expr_eval(text, t=0.0):
     if r != 0:
         raise ShellError(f"shx_expr_eval: cannot compile '{text}'", r)
     return v.value
+
+
+def _declare_mesh(L):
+    if getattr(L, "_mesh_declared", False):
+        return
+    IP = ctypes.POINTER(ctypes.c_int)
+    L.shx_mesh_icosphere.argtypes = [ctypes.c_int, ctypes.c_int, IP, IP, _D, _I]
+    L.shx_mesh_read_off.argtypes = [ctypes.c_char_p, IP, IP, _D, _I]
+    L.shx_mesh_write_off.argtypes = [ctypes.c_int, ctypes.c_int, _D, _I, ctypes.c_char_p, ctypes.c_size_t,
+                                     ctypes.POINTER(ctypes.c_size_t)]
+    L.shx_mesh_refine.argtypes = [ctypes.c_int, ctypes.c_int, _D, _I, ctypes.c_int, IP, IP, _D, _I]
+    L.shx_mesh_transform.argtypes = [ctypes.c_int, _D, _D, _D, _D, _D]
+    L.shx_state_save.argtypes = [_V, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_double]
+    L.shx_state_load.argtypes = [_V, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                 _D]
+    L.shx_write_vtu.argtypes = [_V, ctypes.c_int, ctypes.c_char_p]
+    L.shx_io_last_error.restype = ctypes.c_char_p
+    L._mesh_declared = True
+
+
+def mesh_icosphere(level, round6=True):
+    """the unit icosphere of a level from the library's C++ generator"""
+    L = lib()
+    _declare_mesh(L)
+    nv, ne = ctypes.c_int(), ctypes.c_int()
+    L.shx_mesh_icosphere(level, int(round6), ctypes.byref(nv), ctypes.byref(ne), None, None)
+    X = np.zeros((nv.value, 3))
+    T = np.zeros((ne.value, 3), dtype=np.int32)
+    if L.shx_mesh_icosphere(level, int(round6), ctypes.byref(nv), ctypes.byref(ne), _dp(X), T.ctypes.data_as(_I)):
+        raise ShellError("shx_mesh_icosphere failed")
+    return X, T
+
+
+def read_off(text):
+    L = lib()
+    _declare_mesh(L)
+    nv, ne = ctypes.c_int(), ctypes.c_int()
+    r = L.shx_mesh_read_off(text.encode(), ctypes.byref(nv), ctypes.byref(ne), None, None)
+    if r != 0:
+        raise ShellError(f"shx_mesh_read_off: {r}", r)
+    X = np.zeros((nv.value, 3))
+    T = np.zeros((ne.value, 3), dtype=np.int32)
+    r = L.shx_mesh_read_off(text.encode(), ctypes.byref(nv), ctypes.byref(ne), _dp(X), T.ctypes.data_as(_I))
+    if r != 0:
+        raise ShellError(f"shx_mesh_read_off: {r}", r)
+    return X, T
+
+
+def write_off(X, T):
+    L = lib()
+    _declare_mesh(L)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    n = ctypes.c_size_t()
+    L.shx_mesh_write_off(len(X), len(T), _dp(X), T.ctypes.data_as(_I), None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    L.shx_mesh_write_off(len(X), len(T), _dp(X), T.ctypes.data_as(_I), buf, n.value + 1, ctypes.byref(n))
+    return buf.value.decode()
+
+
+def refine(X, T, levels):
+    L = lib()
+    _declare_mesh(L)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    nv, ne = ctypes.c_int(), ctypes.c_int()
+    L.shx_mesh_refine(len(X), len(T), _dp(X), T.ctypes.data_as(_I), levels, ctypes.byref(nv), ctypes.byref(ne),
+                      None, None)
+    X2 = np.zeros((nv.value, 3))
+    T2 = np.zeros((ne.value, 3), dtype=np.int32)
+    L.shx_mesh_refine(len(X), len(T), _dp(X), T.ctypes.data_as(_I), levels, ctypes.byref(nv), ctypes.byref(ne),
+                      _dp(X2), T2.ctypes.data_as(_I))
+    return X2, T2
+
+
+def transform(X, project=None, scale=None, rotate=None, translate=None):
+    L = lib()
+    _declare_mesh(L)
+    Y = np.ascontiguousarray(X, dtype=np.float64).copy()
+
+    def vec(v):
+        return None if v is None else _dp(np.ascontiguousarray(v, dtype=np.float64))
+    L.shx_mesh_transform(len(Y), _dp(Y), vec(project), vec(scale), vec(rotate), vec(translate))
+    return Y
+
+
+DRIVER_PATH = os.path.join(HERE, "lib", "shellhip_run")

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libshellhip.so")
-SOURCES = ["shx_api.hip", "shx_kernels.hip", "shx_elem.hip", "shx_mlp.hip", "shx_setup.cpp", "shx_config.cpp"]
+SOURCES = ["shx_api.hip", "shx_kernels.hip", "shx_elem.hip", "shx_mlp.hip", "shx_setup.cpp", "shx_config.cpp", "shx_mesh.cpp", "shx_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -22,6 +22,9 @@ FLAGS = [
 ]
 
 
+DRIVER = os.path.join(HERE, "lib", "shellhip_run")
+
+
 def build(verbose=False):
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
@@ -31,6 +34,14 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the command-line driver: a plain C++ program over the C-ABI
+    dcmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+            os.path.join(CSRC, "shx_run.cpp"), "-L", os.path.dirname(LIB), "-lshellhip",
+            "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(dcmd), flush=True)
+    subprocess.run(dcmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

@@ -48,6 +48,10 @@ struct shx_ctx {
    * deformation after the load ramp (halvings only) and its step */
   std::vector<double> fp32_best;
   std::vector<int> fp32_best_step;
+  /* per shell: its contact rows were loaded (shx_batch_load_pairs) and the
+   * next init keeps them instead of starting without contacts */
+  std::vector<char> keep_rows;
+  std::vector<std::vector<double>> Xin;   /* per shell: the mesh positions given at create */
   std::string err;
   bool prof = false;
   std::vector<hipEvent_t> evpool;
@@ -505,6 +509,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   ctx->S = nshell;
   ctx->fp32_best.assign(nshell, HUGE_VAL);
   ctx->fp32_best_step.assign(nshell, 0);
+  ctx->keep_rows.assign(nshell, 0);
+  ctx->Xin.resize(nshell);
   ctx->prm.assign(prm, prm + nshell);
   auto fail = [&](int code) { g_create_err = ctx->err; shx_destroy(ctx); return code; };
   if (hipSetDevice(device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return fail(SHX_E_HIP); }
@@ -526,6 +532,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       ctx->err = "shell " + std::to_string(s) + ": " + terr;
       return fail(SHX_E_MESH);
     }
+    ctx->Xin[s].assign(X + xoff[s], X + xoff[s] + (size_t)3 * nv[s]);
     SConst C = shell_consts(&prm[s], nv[s], ne[s]);
     C.fx_shift = contact_fx_shift(&prm[s], C, ne[s], X + xoff[s], tri + toff[s]);
     C.pressure_prog = pprog[s];
@@ -881,10 +888,12 @@ int shx_batch_init_shells(shx_ctx* ctx, int precision, int n, const int* shells,
     ctx->fp32_best_step[i] = 0;
     d.volume_ref = volume0 ? volume0[k] : 0.0;
     d.rebuild = ctx->C[i].contact ? 1 : 0;
-    /* a fresh start carries no contact (stick) state over from an earlier run */
-    if (ctx->C[i].contact)
+    /* a fresh start carries no contact (stick) state over from an earlier
+     * run, unless the shell's sticking contacts were loaded from a state */
+    if (ctx->C[i].contact && !ctx->keep_rows[i])
       for (int b = 0; b < 2; ++b)
         HIPCHK(hipMemsetAsync(ctx->P.rcnt[b] + ctx->C[i].ebase, 0, sizeof(int32_t) * ctx->C[i].ne, ctx->st));
+    ctx->keep_rows[i] = 0;
   }
   /* the shells not listed keep their state and progress: they sit out the
    * init kernels as done shells; shx_batch_step recomputes done for all */
@@ -1068,6 +1077,142 @@ int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out
     }
   }
   return SHX_OK;
+}
+
+int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, double* d2, int32_t* stick,
+                         double* origin, int cap)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const SConst& C = ctx->C[shell];
+  if (!C.contact) return 0;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  const int b = s[shell].pair_buf;
+  const size_t slots = (size_t)C.ne * kRowCap, off = (size_t)C.ebase * kRowCap;
+  std::vector<int32_t> cnt(C.ne), pe(slots), ps(slots);
+  std::vector<double> pd(slots), po(3 * slots);
+  HIPCHK(hipMemcpy(cnt.data(), ctx->P.rcnt[b] + C.ebase, sizeof(int32_t) * C.ne, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pe.data(), ctx->P.pe2[b] + off, sizeof(int32_t) * slots, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ps.data(), ctx->P.pstick[b] + off, sizeof(int32_t) * slots, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pd.data(), ctx->P.pd2[b] + off, sizeof(double) * slots, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(po.data(), ctx->P.pso[b] + 3 * off, sizeof(double) * 3 * slots, hipMemcpyDeviceToHost));
+  int w = 0;
+  for (int e = 0; e < C.ne; ++e) {
+    const int n = std::min(cnt[e], kRowCap);
+    std::vector<int> ord(n);
+    for (int k = 0; k < n; ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int a, int c) { return pe[(size_t)e * kRowCap + a] < pe[(size_t)e * kRowCap + c]; });
+    for (int k : ord) {
+      const size_t q = (size_t)e * kRowCap + k;
+      if (w < cap) {
+        if (e1) e1[w] = (uint32_t)e;
+        if (e2) e2[w] = (uint32_t)(pe[q] - C.ebase);
+        if (d2) d2[w] = pd[q];
+        if (stick) stick[w] = ps[q];
+        if (origin) for (int c = 0; c < 3; ++c) origin[3 * w + c] = po[3 * q + c];
+      }
+      ++w;
+    }
+  }
+  return w;
+}
+
+int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, const uint32_t* e2,
+                         const double* origin)
+{
+  if (!ctx || n < 0 || (n > 0 && (!e1 || !e2 || !origin))) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const SConst& C = ctx->C[shell];
+  if (!C.contact) { ctx->err = "the shell has no self-contact"; return n ? SHX_E_ARG : SHX_OK; }
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  const int b = s[shell].pair_buf;
+  std::vector<int32_t> cnt(C.ne, 0);
+  const size_t slots = (size_t)C.ne * kRowCap, off = (size_t)C.ebase * kRowCap;
+  std::vector<int32_t> pe(slots, 0), ps(slots, 0);
+  std::vector<double> pd(slots, 0.0), po(3 * slots, 0.0);
+  for (int k = 0; k < n; ++k) {
+    const uint32_t a = std::min(e1[k], e2[k]), c = std::max(e1[k], e2[k]);
+    if (c >= (uint32_t)C.ne || a == c) { ctx->err = "loaded pair out of range"; return SHX_E_ARG; }
+    if (cnt[a] >= kRowCap) { ctx->err = "loaded pairs overflow a contact row"; return SHX_E_CAPACITY; }
+    const size_t q = (size_t)a * kRowCap + cnt[a]++;
+    pe[q] = C.ebase + (int32_t)c;
+    ps[q] = 1;                                 /* sticking: survives the next rebuild */
+    pd[q] = 1.7976931348623157e308;
+    for (int d = 0; d < 3; ++d) po[3 * q + d] = origin[3 * k + d];
+  }
+  /* rows sorted by e2, as the broad phase keeps them */
+  for (int e = 0; e < C.ne; ++e) {
+    std::vector<int> ord(cnt[e]);
+    for (int k = 0; k < cnt[e]; ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return pe[(size_t)e * kRowCap + x] < pe[(size_t)e * kRowCap + y]; });
+    std::vector<int32_t> te(cnt[e]);
+    std::vector<double> to(3 * cnt[e]);
+    for (int k = 0; k < cnt[e]; ++k) {
+      const size_t q = (size_t)e * kRowCap + ord[k];
+      te[k] = pe[q];
+      for (int d = 0; d < 3; ++d) to[3 * k + d] = po[3 * q + d];
+    }
+    for (int k = 0; k < cnt[e]; ++k) {
+      pe[(size_t)e * kRowCap + k] = te[k];
+      for (int d = 0; d < 3; ++d) po[3 * ((size_t)e * kRowCap + k) + d] = to[3 * k + d];
+    }
+  }
+  HIPCHK(hipMemcpy(ctx->P.rcnt[b] + C.ebase, cnt.data(), sizeof(int32_t) * C.ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->P.pe2[b] + off, pe.data(), sizeof(int32_t) * slots, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->P.pstick[b] + off, ps.data(), sizeof(int32_t) * slots, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->P.pd2[b] + off, pd.data(), sizeof(double) * slots, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->P.pso[b] + 3 * off, po.data(), sizeof(double) * 3 * slots, hipMemcpyHostToDevice));
+  ctx->keep_rows[shell] = 1;
+  return SHX_OK;
+}
+
+int shx_batch_averages(shx_ctx* ctx, int shell, double* sums, int* n, int set)
+{
+  if (!ctx || !sums || !n) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  if (!set) {
+    std::copy(s[shell].avg, s[shell].avg + 11, sums);
+    *n = s[shell].avg_n;
+    return SHX_OK;
+  }
+  if (*n < 0) return SHX_E_ARG;
+  std::copy(sums, sums + 11, s[shell].avg);
+  s[shell].avg_n = *n;
+  return write_sh(ctx, s);
+}
+
+int shx_batch_params(shx_ctx* ctx, int shell, shx_params* out)
+{
+  if (!ctx || !out) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  *out = ctx->prm[shell];
+  return SHX_OK;
+}
+
+int shx_batch_mesh(shx_ctx* ctx, int shell, int* nv, int* ne, double* X, int32_t* tri)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const Topology& T = ctx->T[shell];
+  if (nv) *nv = T.nv;
+  if (ne) *ne = T.ne;
+  if (X) std::copy(ctx->Xin[shell].begin(), ctx->Xin[shell].end(), X);
+  if (tri) std::copy(T.tri.begin(), T.tri.end(), tri);
+  return SHX_OK;
+}
+
+int shx_batch_set_dt(shx_ctx* ctx, int shell, double dt)
+{
+  if (!ctx || !(dt > 0)) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  set_newmark_host(s[shell], ctx->C[shell], dt);
+  return write_sh(ctx, s);
 }
 
 int shx_batch_measure(shx_ctx* ctx)

@@ -1,0 +1,143 @@
+"""The command-line driver shellhip_run (a C++ program over the C-ABI, the
+reference's shell.C) end to end on the GPU: config file and overrides, OFF
+mesh, initial distortion, output cadence (property lines every data_time,
+.vtu files every vtk_time, state files), a restart from a state file that
+continues the run exactly, and the same run driven through the Python
+mirror of the C-ABI (same config, same drand48 distortion) ending in the
+same state."""
+import ctypes
+import os
+import re
+import subprocess
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+import shell_amd as S
+
+pytestmark = pytest.mark.gpu
+
+H = 0.01
+PCR = 2 * H * H / np.sqrt(3 * (1 - 0.3 ** 2))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+CONFIG = f"""# explicit run of a level-3 sphere under a pressure ramp, with self-contact
+shell_mass_density   = 1
+shell_young_modulus  = 1
+shell_young_modulus2 = 0.5
+shell_poisson_ratio  = 0.3
+shell_shear_modulus  = 0.35
+thickness            = {H}
+damping_alpha        = 0.05
+pressure             = '{float(-2 * PCR)!r}*min(t/4,1)'
+shell_shell_contact  = linear
+shell_cavity_contact = false
+mu_s                 = 0.3
+mu_d                 = 0.3
+cell_size_mult       = 1
+newmark_beta         = 0
+newmark_gamma        = 0.5
+dt                   = 0.05
+adapt_dt_every       = 0
+time_steps           = 200
+data_time            = 1.0
+vtk_time             = 2.0
+state_time           = 0
+random_seed          = 3
+distortion           = 1e-2
+"""
+
+
+def _setup(tmp_path):
+    X, T = S.mesh_icosphere(3)
+    (tmp_path / "sphere.off").write_text(S.write_off(X, T))
+    (tmp_path / "config.cfg").write_text(CONFIG)
+    return X, T
+
+
+def _run(tmp_path, *args):
+    say("case driver:", " ".join(args))
+    r = subprocess.run([S.DRIVER_PATH, "-m", "sphere.off", *args], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    return r
+
+
+def _read_state(path):
+    with open(path) as f:
+        step = int(f.readline())
+        dt, t, _ = (float(f.readline()) for _ in range(3))
+        rows = []
+        for line in f:
+            vals = line.split()
+            if len(vals) == 12:
+                rows.append([float(x) for x in vals])
+            else:
+                break
+    a = np.array(rows)
+    u = a[:, [0, 3, 6]].reshape(-1)
+    v = a[:, [1, 4, 7]].reshape(-1)
+    return step, t, u, v
+
+
+def test_driver_run_outputs_and_restart(tmp_path):
+    _setup(tmp_path)
+    r = _run(tmp_path)
+    say(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("# ")]
+    assert len(lines) == 1 + 10                       # the initial line and one per data_time of 10 time units
+    cols = lines[-1][2:].split()
+    assert len(cols) == 19 and int(cols[1]) == 200 and float(cols[3]) == pytest.approx(10.0, rel=1e-12)
+    data = tmp_path / "data"
+    vtus = sorted(p.name for p in data.glob("shell*.vtu"))
+    assert vtus == [f"shell{s:010d}.vtu" for s in (0, 40, 80, 120, 160, 200)]
+    root = ET.parse(data / vtus[-1]).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    assert piece.get("NumberOfPoints") == "642" and piece.get("NumberOfCells") == "1280"
+    names = [a.get("Name") for a in piece.iter("DataArray")]
+    assert names == ["ux", "uy", "uz", "ecav", "efric", "ekin", "eself", "mass", "ebend", "etens", "kGauss", "kMean",
+                     "Points", "connectivity", "offsets", "types"]
+    kg = np.array(piece.find("CellData/DataArray[@Name='kGauss']").text.split(), dtype=float)
+    assert kg.size == 1280 and np.all(np.isfinite(kg))
+    assert (data / "shell_state0000000200.dat").exists() and os.path.islink(data / "shell_state_latest.dat")
+    step, t, u_full, _ = _read_state(data / "shell_state0000000200.dat")
+    assert step == 200
+    # the same run in two halves, the second one loading the first one's state file
+    r1 = _run(tmp_path, "-d", "half", "-time_steps", "100")
+    assert r1.returncode == 0, r1.stdout
+    r2 = _run(tmp_path, "-d", "half2", "-time_steps", "200", "-l", "half/shell_state0000000100.dat")
+    assert r2.returncode == 0, r2.stdout
+    step2, t2, u_half, _ = _read_state(tmp_path / "half2" / "shell_state0000000200.dat")
+    rel = np.linalg.norm(u_half - u_full) / np.linalg.norm(u_full)
+    say("  restart: step", step2, "t", t2, "rel diff", rel)
+    assert step2 == 200 and t2 == t
+    assert rel < 1e-12
+
+
+def test_driver_equals_the_python_mirror(tmp_path):
+    """the driver's run equals the same run through ShellSystem: config read
+    by the library, distortion u += distortion (2 drand48() - 1) h per DOF
+    after srand48(random_seed) (ShellSystem.C:342-343, 863-869)"""
+    X, T = _setup(tmp_path)
+    r = _run(tmp_path, "-time_steps", "120", "-vtk_time", "0", "-data_time", "0")
+    assert r.returncode == 0, r.stdout
+    _, t, u_drv, v_drv = _read_state(tmp_path / "data" / "shell_state0000000120.dat")
+    p, d, _ = S.read_config(CONFIG)
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48(d["random_seed"])
+    u0 = np.array([d["distortion"] * (2 * libc.drand48() - 1) * p["thickness"] for _ in range(3 * len(X))])
+    g = S.ShellSystem(X, T, **p)
+    g.set_state(u0)
+    g.apply_initial_conditions()
+    g.advance(120)
+    u, v, _, tg = g.get_state()
+    g.close()
+    say("  driver vs mirror: t", t, tg, "max |du|", np.max(np.abs(u - u_drv)))
+    assert tg == t
+    assert np.array_equal(u, u_drv) and np.array_equal(v, v_drv)
