@@ -388,10 +388,35 @@ int shx_mlp_train(shx_mlp* m, int n, const float* x, const float* t, int batch, 
  * or, transposed, K x M; B is K x N or N x K) */
 int shx_mlp_gemm(int M, int N, int K, const float* A, int ta, const float* B, int tb, float* C);
 
+/* Replaces SubdivEvaluation::evaluate and evaluate_deriv
+ * (SubdivEvaluation.h:105-106, SubdivEvaluation.C:14-167): the Loop limit
+ * surface of element eid of a shell at barycentric (xi, eta) (xi along
+ * node 0 -> 1, eta along node 0 -> 2 of the element's rotated triangle,
+ * node 0 its irregular vertex), with its tangents, at the current
+ * positions X + u.  Any output may be NULL. */
+int shx_subdiv_evaluate(shx_ctx* ctx, int shell, int eid, double xi, double eta, double* p /* 3 */,
+                        double* dpdxi /* 3 */, double* dpdeta /* 3 */);
+/* Replaces ShellShellContact(eid1, eid2) + compute() / compute_on_mesh()
+ * (ShellShellContact.h:24-28, ShellShellContact.C:35-145): the closest
+ * points of two elements of a shell at the current positions (on_mesh = 0)
+ * or on the mesh (on_mesh != 0).  out[11] = d2, dc[3] (element 1 to 2),
+ * v[2], w[2] (barycentric on each), point()[3] (ShellShellContact.C:81-84);
+ * *problem = SHX_CONTACT_PROBLEM when compute() finds d2 <= 0. */
+int shx_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, int on_mesh, double* out, int32_t* problem);
+/* Replaces Friction::apply (Friction.h:12-14, Friction.C:4-60), the same
+ * arguments as plain values and pointers (points as double[3], stick as
+ * int32); host only. */
+void shx_friction_apply(double* Efric, double* f_contact, double f_normal, const double* n, const double* v,
+                        double abs_v_normal, double v_stop, double spring_stiffness, double mu_s, double mu_d,
+                        double damping, int32_t* stick, double* spring_origin, const double* contact_point);
+
 /* Primitives exposed for parity tests (host-compiled copies of the device
  * routines; no GPU needed). */
 void shx_tri_distance(const double* p /* 18 */, double* out /* d2, dc[3], v[2], w[2] */);
 int  shx_raw_elem_basis(int valence, double* out /* 6*(valence+6) */);
+/* the limit-surface weights shx_subdiv_evaluate applies: value, d/dxi,
+ * d/deta over the valence+6 patch slots */
+int  shx_subdiv_weights(int valence, double xi, double eta, double* out /* 3*(valence+6) */);
 
 #ifdef __cplusplus
 }

@@ -611,3 +611,63 @@ def transform(X, project=None, scale=None, rotate=None, translate=None):
 
 
 DRIVER_PATH = os.path.join(HERE, "lib", "shellhip_run")
+
+
+def subdiv_weights(valence, xi, eta):
+    """Loop limit-surface weights of a patch at (xi, eta): value, d/dxi,
+    d/deta over the valence + 6 slots (host routine of the library)"""
+    L = lib()
+    L.shx_subdiv_weights.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
+    out = np.zeros(3 * (valence + 6))
+    if L.shx_subdiv_weights(valence, xi, eta, _dp(out)) != 0:
+        raise ValueError((valence, xi, eta))
+    return out.reshape(3, valence + 6)
+
+
+def friction_apply(Efric, f_contact, f_normal, n, v, abs_v_normal, v_stop, k_s, mu_s, mu_d, damping, stick,
+                   spring_origin, contact_point):
+    """Friction::apply through the library's host routine; returns
+    (Efric, f_contact, stick, spring_origin)"""
+    L = lib()
+    L.shx_friction_apply.argtypes = [_D, _D, ctypes.c_double, _D, _D, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _D, _D]
+    L.shx_friction_apply.restype = None
+    E = np.array([Efric], dtype=np.float64)
+    fc = np.array(f_contact, dtype=np.float64)
+    so = np.array(spring_origin, dtype=np.float64)
+    st = np.array([int(stick)], dtype=np.int32)
+    L.shx_friction_apply(_dp(E), _dp(fc), f_normal, _dp(np.asarray(n, dtype=np.float64)),
+                         _dp(np.asarray(v, dtype=np.float64)), abs_v_normal, v_stop, k_s, mu_s, mu_d, damping,
+                         st.ctypes.data_as(_I), _dp(so), _dp(np.asarray(contact_point, dtype=np.float64)))
+    return float(E[0]), fc, int(st[0]), so
+
+
+def _declare_interfaces(L):
+    if getattr(L, "_if_declared", False):
+        return
+    L.shx_subdiv_evaluate.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D]
+    L.shx_contact_compute.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I]
+    L._if_declared = True
+
+
+def subdiv_evaluate(shells, eid, xi, eta, shell=0):
+    """SubdivEvaluation::evaluate + evaluate_deriv of a ShellSystem or
+    ShellBatch shell at its current positions: (p, dp/dxi, dp/deta)"""
+    L = lib()
+    _declare_interfaces(L)
+    p, d1, d2 = np.zeros(3), np.zeros(3), np.zeros(3)
+    shells._check(L.shx_subdiv_evaluate(shells.h, shell, eid, xi, eta, _dp(p), _dp(d1), _dp(d2)),
+                  "shx_subdiv_evaluate")
+    return p, d1, d2
+
+
+def contact_compute(shells, eid1, eid2, on_mesh=False, shell=0):
+    """ShellShellContact(eid1, eid2).compute() (or compute_on_mesh()): dict
+    of d2, dc, v, w, point and the problem it registers"""
+    L = lib()
+    _declare_interfaces(L)
+    out = np.zeros(11)
+    prob = np.zeros(1, dtype=np.int32)
+    shells._check(L.shx_contact_compute(shells.h, shell, eid1, eid2, int(on_mesh), _dp(out), prob.ctypes.data_as(_I)),
+                  "shx_contact_compute")
+    return {"d2": out[0], "dc": out[1:4], "v": out[4:6], "w": out[6:8], "point": out[8:11], "problem": int(prob[0])}

@@ -1454,6 +1454,94 @@ int shx_kernel_times(shx_ctx* ctx, double* avg_ms, int64_t* counts)
   return SHX_OK;
 }
 
+namespace {
+/* positions of one shell: X + u of the current state, or the mesh (X) */
+int shell_positions(shx_ctx* ctx, int shell, bool on_mesh, std::vector<double>& x)
+{
+  const SConst& C = ctx->C[shell];
+  x = ctx->Xin[shell];
+  if (on_mesh) return SHX_OK;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  std::vector<double> u((size_t)3 * C.nv);
+  HIPCHK(hipMemcpy(u.data(), ctx->P.U[s[shell].par] + (size_t)3 * C.vbase, sizeof(double) * u.size(),
+                   hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < x.size(); ++i) x[i] += u[i];
+  return SHX_OK;
+}
+}
+
+int shx_subdiv_evaluate(shx_ctx* ctx, int shell, int eid, double xi, double eta, double* p, double* dpdxi,
+                        double* dpdeta)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const Topology& T = ctx->T[shell];
+  if (eid < 0 || eid >= T.ne || !(xi >= 0) || !(eta >= 0) || !(xi + eta <= 1)) {
+    ctx->err = "element or barycentric coordinates out of range";
+    return SHX_E_ARG;
+  }
+  std::vector<double> x;
+  if (int r = shell_positions(ctx, shell, false, x)) return r;
+  double w[3][kMaxPatch];
+  const int N = T.valence[eid];
+  if (!subdiv_weights(N, xi, eta, w)) { ctx->err = "unsupported valence"; return SHX_E_MESH; }
+  double out[3][3] = {};
+  for (int j = 0; j < N + 6; ++j) {
+    const int n = T.patch[(size_t)eid * kMaxPatch + j];
+    for (int q = 0; q < 3; ++q)
+      for (int c = 0; c < 3; ++c) out[q][c] += w[q][j] * x[3 * (size_t)n + c];
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (p) p[c] = out[0][c];
+    if (dpdxi) dpdxi[c] = out[1][c];
+    if (dpdeta) dpdeta[c] = out[2][c];
+  }
+  return SHX_OK;
+}
+
+int shx_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, int on_mesh, double* out, int32_t* problem)
+{
+  if (!ctx || !out) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const Topology& T = ctx->T[shell];
+  if (eid1 < 0 || eid1 >= T.ne || eid2 < 0 || eid2 >= T.ne) { ctx->err = "element out of range"; return SHX_E_ARG; }
+  std::vector<double> x;
+  if (int r = shell_positions(ctx, shell, on_mesh != 0, x)) return r;
+  v3 A[3], B[3];
+  for (int k = 0; k < 3; ++k) {
+    const int a = T.tri[3 * (size_t)eid1 + k], b = T.tri[3 * (size_t)eid2 + k];
+    A[k] = mk(x[3 * a], x[3 * a + 1], x[3 * a + 2]);
+    B[k] = mk(x[3 * b], x[3 * b + 1], x[3 * b + 2]);
+  }
+  tri_hit h;
+  tri_tri(A, B, h);
+  /* ShellShellContact::point (ShellShellContact.C:81-84) */
+  const double u0 = 1 - h.v0 - h.w0;
+  const v3 cp = add(add(add(scl(A[0], u0), scl(A[1], h.v0)), scl(A[2], h.w0)), scl(h.dc, 0.5));
+  const double o[11] = {h.d2, h.dc.x, h.dc.y, h.dc.z, h.v0, h.v1, h.w0, h.w1, cp.x, cp.y, cp.z};
+  std::copy(o, o + 11, out);
+  /* compute() registers CONTACT_PROBLEM at d2 <= 0; compute_on_mesh() does not (:45, :75-77) */
+  if (problem) *problem = (!on_mesh && h.d2 <= 0) ? SHX_CONTACT_PROBLEM : SHX_NO_PROBLEM;
+  return SHX_OK;
+}
+
+void shx_friction_apply(double* Efric, double* f_contact, double f_normal, const double* n, const double* v,
+                        double abs_v_normal, double v_stop, double spring_stiffness, double mu_s, double mu_d,
+                        double damping, int32_t* stick, double* spring_origin, const double* contact_point)
+{
+  v3 fc = mk(f_contact[0], f_contact[1], f_contact[2]);
+  v3 so = mk(spring_origin[0], spring_origin[1], spring_origin[2]);
+  int st = *stick ? 1 : 0;
+  double E = *Efric;
+  friction(E, fc, f_normal, mk(n[0], n[1], n[2]), mk(v[0], v[1], v[2]), abs_v_normal, v_stop, spring_stiffness, mu_s,
+           mu_d, damping, st, so, mk(contact_point[0], contact_point[1], contact_point[2]));
+  *Efric = E;
+  f_contact[0] = fc.x; f_contact[1] = fc.y; f_contact[2] = fc.z;
+  spring_origin[0] = so.x; spring_origin[1] = so.y; spring_origin[2] = so.z;
+  *stick = st;
+}
+
 void shx_tri_distance(const double* p, double* out)
 {
   v3 A[3], B[3];
@@ -1465,6 +1553,16 @@ void shx_tri_distance(const double* p, double* out)
   tri_tri(A, B, h);
   out[0] = h.d2; out[1] = h.dc.x; out[2] = h.dc.y; out[3] = h.dc.z;
   out[4] = h.v0; out[5] = h.v1; out[6] = h.w0; out[7] = h.w1;
+}
+
+int shx_subdiv_weights(int valence, double xi, double eta, double* out)
+{
+  double w[3][kMaxPatch];
+  if (!out || !(xi >= 0) || !(eta >= 0) || !(xi + eta <= 1) || !subdiv_weights(valence, xi, eta, w)) return SHX_E_ARG;
+  const int P = valence + 6;
+  for (int q = 0; q < 3; ++q)
+    for (int j = 0; j < P; ++j) out[q * P + j] = w[q][j];
+  return SHX_OK;
 }
 
 int shx_raw_elem_basis(int valence, double* out)

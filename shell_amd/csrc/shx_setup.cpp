@@ -6,6 +6,7 @@
 #include "shx_math.h"
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <unordered_map>
 
@@ -143,6 +144,117 @@ bool loop_basis_at_centroid(int valence, double out[6 * kMaxPatch])
       double s = 0;
       for (int i = 0; i < 12; ++i) s += S[i][j] * b[q][i];
       out[q * kMaxPatch + j] = f[q] * s;
+    }
+  return true;
+}
+
+namespace {
+
+/* One Loop refinement of an irregular patch: the N+12 fine points as
+ * combinations of the N+6 patch points (slots v0, ring 1..N with ring 1 =
+ * v1 and ring N = v2, then Q11 Q20 Q2m Q02 Qm2 at lattice (1,1) (2,0)
+ * (2,-1) (0,2) (-1,2)).  Fine points: 0 the vertex point of v0 (Loop's
+ * beta_N), 1..N the edge points of v0 and each ring vertex, then (v1,v2),
+ * v1, (v1,ring 2), v2, (v2,ring N-1), (v1,Q11), (v1,Q20), (v1,Q2m),
+ * (v2,Q11), (v2,Q02), (v2,Qm2).  The first N+6 fine points are the next
+ * level's irregular patch in the same slot order (Stam's subdivision matrix
+ * A; all N+12 are his extended matrix).  This is the numbering the
+ * selection matrices P_k of SubdivEvaluation.h:31-77 address. */
+void loop_extended(int N, double Ab[kMaxPatch + 6][kMaxPatch])
+{
+  for (int i = 0; i < kMaxPatch + 6; ++i)
+    for (int j = 0; j < kMaxPatch; ++j) Ab[i][j] = 0;
+  const int V0 = 0, V1 = 1, V2 = N, Q11 = N + 1, Q20 = N + 2, Q2m = N + 3, Q02 = N + 4, Qm2 = N + 5;
+  const int Ra = 2, Rz = N - 1;
+  auto E = [&](int row, int a, int b, int c, int d) {
+    Ab[row][a] += 0.375; Ab[row][b] += 0.375; Ab[row][c] += 0.125; Ab[row][d] += 0.125;
+  };
+  auto R6 = [&](int row, int ctr, std::initializer_list<int> nb) {
+    Ab[row][ctr] += 0.625;
+    for (int q : nb) Ab[row][q] += 0.0625;
+  };
+  const double cc = 0.375 + 0.25 * std::cos(2.0 * M_PI / N);
+  const double beta = (0.625 - cc * cc) / N; /* Loop (1987) */
+  Ab[0][V0] += 1.0 - N * beta;
+  for (int k = 1; k <= N; ++k) Ab[0][k] += beta;
+  for (int j = 1; j <= N; ++j) {
+    const int prev = j == 1 ? N : j - 1, next = j == N ? 1 : j + 1;
+    E(j, V0, j, prev, next);
+  }
+  E(N + 1, V1, V2, V0, Q11);
+  R6(N + 2, V1, {V0, Q20, Q11, V2, Q2m, Ra});
+  E(N + 3, V1, Ra, V0, Q2m);
+  R6(N + 4, V2, {V0, V1, Q11, Q02, Qm2, Rz});
+  E(N + 5, V2, Rz, V0, Qm2);
+  E(N + 6, V1, Q11, V2, Q20);
+  E(N + 7, V1, Q20, Q11, Q2m);
+  E(N + 8, V1, Q2m, Ra, Q20);
+  E(N + 9, V2, Q11, V1, Q02);
+  E(N + 10, V2, Q02, Q11, Qm2);
+  E(N + 11, V2, Qm2, Q02, Rz);
+}
+
+} // namespace
+
+bool subdiv_weights(int valence, double xi, double eta, double w[3][kMaxPatch])
+{
+  if (valence < 3 || valence > 6) return false;
+  const int N = valence, P = N + 6;
+  for (int q = 0; q < 3; ++q)
+    for (int j = 0; j < kMaxPatch; ++j) w[q][j] = 0;
+  double b[6][12];
+  if (N == 6) {
+    static const int cvi[12] = {3, 6, 2, 0, 1, 4, 7, 10, 9, 5, 11, 8};   /* SubdivEvaluation.C:10 */
+    box_eval(xi, eta, b);
+    for (int q = 0; q < 3; ++q)
+      for (int j = 0; j < 12; ++j) w[q][j] = b[q][cvi[j]];
+    return true;
+  }
+  /* SubdivEvaluation.C:37-80, 112-165, 170-187 */
+  const double tol = 1e-10;
+  if (xi + eta < tol) { xi = std::max(xi, tol); eta = std::max(eta, 0.0); }
+  const int n = (int)std::floor(1 - std::log(xi + eta) / std::log(2.0));
+  const double pow2 = std::pow(2.0, n - 1);
+  xi *= pow2;
+  eta *= pow2;
+  int k;
+  double jfac;
+  if (xi > 0.5) { k = 0; xi = 2 * xi - 1; eta = 2 * eta; jfac = std::pow(2.0, n); }
+  else if (eta > 0.5) { k = 2; xi = 2 * xi; eta = 2 * eta - 1; jfac = std::pow(2.0, n); }
+  else {
+    /* x' = 1 - 2^n xi: the chain rule gives -2^n.  The reference writes
+     * pow(-2, n) (SubdivEvaluation.C:136), whose sign is wrong for even n
+     * (xi + eta in [1/4, 1/2), [1/16, 1/8), ...); the tangents here are
+     * the derivatives of the evaluated surface */
+    k = 1; xi = 1 - 2 * xi; eta = 1 - 2 * eta; jfac = -std::pow(2.0, n);
+  }
+  /* P_k: box spline i takes fine point sel[k][i] (SubdivEvaluation.h:31-77) */
+  const int sel[3][12] = {{2, 0, N + 3, 1, N, N + 8, N + 2, N + 1, N + 4, N + 7, N + 6, N + 9},
+                          {N + 9, N + 6, N + 4, N + 1, N + 2, N + 5, N, 1, N + 3, N - 1, 0, 2},
+                          {0, N - 1, 1, N, N + 5, N + 2, N + 1, N + 4, N + 11, N + 6, N + 9, N + 10}};
+  double Ab[kMaxPatch + 6][kMaxPatch];
+  loop_extended(N, Ab);
+  /* M = P_k Ab A^(n-1): 12 x P; rows of A are the first P rows of Ab */
+  double M[12][kMaxPatch], T[12][kMaxPatch];
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j < P; ++j) M[i][j] = Ab[sel[k][i]][j];
+  for (int it = 1; it < n; ++it) {
+    for (int i = 0; i < 12; ++i)
+      for (int j = 0; j < P; ++j) {
+        double s = 0;
+        for (int m = 0; m < P; ++m) s += M[i][m] * Ab[m][j];
+        T[i][j] = s;
+      }
+    for (int i = 0; i < 12; ++i)
+      for (int j = 0; j < P; ++j) M[i][j] = T[i][j];
+  }
+  box_eval(xi, eta, b);
+  const double f[3] = {1.0, jfac, jfac};
+  for (int q = 0; q < 3; ++q)
+    for (int j = 0; j < P; ++j) {
+      double s = 0;
+      for (int i = 0; i < 12; ++i) s += M[i][j] * b[q][i];
+      w[q][j] = f[q] * s;
     }
   return true;
 }

@@ -39,6 +39,14 @@ bool build_topology(int nv, int ne, const int32_t* tri_in, Topology& T, std::str
  * d/deta, d2/dxi2, d2/deta2, d2/dxideta, j over the patch (zero padded). */
 bool loop_basis_at_centroid(int valence, double out[6 * kMaxPatch]);
 
+/* Loop limit-surface weights of a patch at barycentric (xi, eta) of its
+ * element (SubdivEvaluation::evaluate / evaluate_deriv, SubdivEvaluation.C:
+ * 14-167): w[0] position, w[1] d/dxi, w[2] d/deta over the patch slots.
+ * A regular patch uses the box splines directly; an irregular one
+ * subdivides n times toward its irregular vertex (Stam 1998) and evaluates
+ * the box splines of the sub-patch that holds the point. */
+bool subdiv_weights(int valence, double xi, double eta, double w[3][kMaxPatch]);
+
 /* elements whose distance on the undeformed mesh is below cutoff, closed
  * under neighbourhood (build_topological_neighborhood, ShellSystem.C:684-741);
  * CSR sorted by element id. */
