@@ -75,18 +75,24 @@ def sweep_counts(n_shells, mix=SWEEP_MIX):
     return [int(x) for x in c]
 
 
-def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX):
+def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX, loading="volume"):
     """Seeded batch of independent post-buckling problems.
 
     Per shell: icosphere level by the mix (2:2:1 of 3/4/5 by default),
     E2/E1 ~ U[0.2, 1], h/R ~ logU[0.005, 0.02], mu ~ U[0, 0.5], Gaussian
-    radial imperfection 0.01 h with its own seed, and deflation (bulk modulus
-    0.05, enclosed volume of the force configuration) to V/V0 ~ U[0.85, 0.92]
-    over SWEEP_RAMP_TIME, self-contact with friction (contact lists rebuilt on
-    the displacement trigger), mass damping 0.05.
+    radial imperfection 0.01 h with its own seed, self-contact with friction
+    (contact lists rebuilt on the displacement trigger), mass damping 0.05.
+    loading "volume": deflation (bulk modulus 0.05, enclosed volume of the
+    force configuration) to V/V0 ~ U[0.85, 0.92] over SWEEP_RAMP_TIME;
+    loading "pressure": external pressure ramped from 0 to p/p_cr ~
+    U[1.5, 2.5] over SWEEP_RAMP_TIME (the factors come from a second stream
+    seeded by seed + 1, so the other draws are those of the volume loading).
     Returns a list of dicts with keys level, X, T, params (dt left to the
     caller: it depends on the CFL bound of each shell)."""
+    if loading not in ("volume", "pressure"):
+        raise ValueError(f"unknown loading {loading!r}")
     rng = np.random.default_rng(seed)
+    prng = np.random.default_rng(seed + 1)
     counts = sweep_counts(n_shells, mix)
     lv = np.concatenate([[L] * c for L, c in zip(levels, counts)])
     meshes = {L: icosphere(L) for L in set(lv.tolist())}
@@ -102,6 +108,10 @@ def batch_sweep(n_shells=512, seed=7, levels=SWEEP_LEVELS, mix=SWEEP_MIX):
                       bulk_modulus=0.05, volume_final_ratio=ratio, volume_ramp_time=SWEEP_RAMP_TIME,
                       volume_at_prediction=1, contact=1, mu_s=mu, mu_d=mu, cell_size_mult=4.0,
                       cell_margin=0.1, rebuild_trigger=1)
+        if loading == "pressure":
+            f = prng.uniform(1.5, 2.5)
+            params.update(bulk_modulus=0.0, volume_final_ratio=1.0, volume_ramp_time=0.0, volume_at_prediction=0,
+                          pressure_final=-f * p_critical(1.0, h, 0.3), pressure_ramp_time=SWEEP_RAMP_TIME)
         out.append(dict(level=int(L), X=radial_imperfection(X, 0.01 * h, iseed), T=T, params=params,
                         imperfection_seed=iseed))
     return out
