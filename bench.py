@@ -313,20 +313,33 @@ def run_surrogate(args):
     if args.no_graph:
         b.set_graphs(False)
     vref = W.reference_volumes(b)
+    b.set_kinetic_damping(args.relax == "kinetic")
     log(f"setup {time.perf_counter() - t0:.1f} s: {b.S} level-4 shells")
     n_dof = 3 * int(b.nv[0])
-    results = {}
-    for _ in range(args.warmup):
-        W.warm_solve(b, sg, P, vref, args.tol, args.max_steps, args.chunk,
-                     precision=S.FP32_SPLIT if args.precision == "fp32" else S.FP64)
     prec = S.FP32_SPLIT if args.precision == "fp32" else S.FP64
+
+    def warm():
+        if args.stream:
+            return W.stream_solve(b, P, vref, args.stream, sg, args.tol, args.max_steps, args.chunk,
+                                  args.probe_steps, precision=prec)
+        return W.warm_solve(b, sg, P, vref, args.tol, args.max_steps, args.chunk, probe_steps=args.probe_steps,
+                            precision=prec, quench_steps=args.quench_steps)
+
+    def plain():
+        if args.stream:
+            reps, wall, _ = W.stream_solve(b, P, vref, args.stream, None, args.tol, args.max_steps, args.chunk)
+            return reps, wall
+        return W.plain_solve(b, args.tol, args.max_steps, args.chunk)
+
+    for _ in range(args.warmup):
+        warm()
     walls_w, conv_w = [], 0
     for _ in range(args.steps):
-        reps_w, wall_w, info = W.warm_solve(b, sg, P, vref, args.tol, args.max_steps, args.chunk, precision=prec)
+        reps_w, wall_w, info = warm()
         walls_w.append(wall_w)
         conv_w += sum(r["converged_step"] >= 0 for r in reps_w)
     Uw = b.get_state(S.ALL_SHELLS)[0].reshape(b.S, n_dof)
-    reps_p, wall_p = W.plain_solve(b, args.tol, args.max_steps, args.chunk)
+    reps_p, wall_p = plain()
     conv_p = sum(r["converged_step"] >= 0 for r in reps_p)
     Up = b.get_state(S.ALL_SHELLS)[0].reshape(b.S, n_dof)
     Xs = [X for X, _ in meshes]
@@ -364,12 +377,15 @@ def run_surrogate(args):
                    "surrogate_vs_plain": value / plain if plain > 0 else None,
                    "surrogate_wall_s": wall_w, "plain_wall_s": wall_p,
                    "converged_surrogate": int(conv_w / max(args.steps, 1)), "converged_plain": int(conv_p),
-                   "fallback_interpenetrating": info["fallback_interpenetrating"],
-                   "fallback_residual_growth": info["fallback_residual_growth"],
+                   "relaxation": args.relax, "quench_steps": 0 if args.stream else args.quench_steps,
+                   "probe_steps": args.probe_steps, "stream_slots": args.stream,
+                   "fallbacks": len(info["fallback"]),
+                   "kd_resets_median_surrogate_plain": [float(np.median([r["n_kd"] for r in reps_w])),
+                                                        float(np.median([r["n_kd"] for r in reps_p]))],
                    "solver_steps_median_surrogate_plain": [int(np.median(steps_w)), int(np.median(steps_p))],
                    "prediction_rel_err_median": float(np.median(pred_err)),
                    "equilibrium_rel_diff_surrogate_vs_plain_median": float(np.median(same)),
-                   "initial_residual_median": info["r0_median"],
+                   "initial_residual_median": info.get("r0_median"),
                    "training": {"equilibria": int(len(P_train)), "seconds": train_s, "epochs": args.epochs,
                                 "final_loss": losses[-1], "pca_variance_kept": float(d["explained"])},
                    "max_steps": args.max_steps},
@@ -405,11 +421,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch step attempts one by one (profilers)")
+    ap.add_argument("--relax", choices=["damped", "kinetic"], default="damped",
+                    help="relaxation of both legs: the configured mass damping, or with kinetic damping added "
+                         "once the load is complete")
+    ap.add_argument("--quench-steps", type=int, default=None, help="warm-start quench steps (default 2000, "
+                    "0 with --relax kinetic)")
+    ap.add_argument("--probe-steps", type=int, default=1000)
+    ap.add_argument("--stream", type=int, default=0, help="streaming batch: at most this many shells step at once, "
+                    "the next of the --shells queue starting as others finish (both legs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.quench_steps is None:
+        args.quench_steps = 0 if args.relax == "kinetic" else 2000
     if args.mode == "surrogate":
         args.shells = args.shells or 1024
         args.seed = 12 if args.seed is None else args.seed

@@ -151,7 +151,7 @@ typedef struct {
   int32_t capacity_err;            /* nonzero: a device list of this shell overflowed (bits: 1 bucket entries,
                                       2 pairs, 4 contact row, 8 cells); the shell stopped and the call returned
                                       SHX_E_CAPACITY; shx_batch_init_shells clears it */
-  int32_t reserved;
+  int32_t n_kd;                    /* kinetic-damping velocity resets (shx_batch_set_kinetic_damping) */
 } shx_report;
 
 /* Replaces: ShellSystem constructor + set_parameters + build_cache +
@@ -259,6 +259,23 @@ int shx_batch_init_shells(shx_ctx* ctx, int precision, int n, const int* shells,
  * CFL bound).  Equilibria do not depend on it: the residual is formed from
  * the undamped force. */
 int shx_batch_set_extra_damping(shx_ctx* ctx, const double* alpha);
+/* Kinetic damping on[s] (0 or 1) of every shell (nshell values; NULL turns
+ * it off): once t >= the load's end time (the time from which convergence is
+ * checked), after every accepted step whose kinetic energy is below the
+ * previous step's (a peak has passed), the shell's velocities are set to
+ * zero (Cundall's kinetic damping of dynamic relaxation).  No counterpart in
+ * the reference: a relaxation to equilibrium control for the learned warm
+ * start and the plain start it is compared with.  Equilibria do not depend
+ * on it (the residual is formed from forces only); the path between them
+ * does, so with friction the equilibrium reached can differ.  Resets are
+ * counted in shx_report::n_kd. */
+int shx_batch_set_kinetic_damping(shx_ctx* ctx, const int* on);
+/* Park the n listed shells: they sit out every following shx_batch_step
+ * (their state and reports unchanged, no kernel work) until
+ * shx_batch_init_shells initialises them again.  No counterpart in the
+ * reference: a streaming batch keeps a queue of shells in one context and
+ * starts the next ones as others converge. */
+int shx_batch_park(shx_ctx* ctx, int n, const int* shells);
 /* n_steps accepted steps of every shell (ShellSystem::advance, ShellSystem.C:
  * 952-1122), or until the shell converges if stop_on_converged; reps[nshell] */
 int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged,

@@ -37,7 +37,7 @@ EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_
            "shx_elem_tables", "shx_mass", "shx_patches", "shx_cfl_omega_max", "shx_set_profiling",
            "shx_kernel_times", "shx_tri_distance", "shx_raw_elem_basis", "shx_batch_create",
            "shx_batch_size", "shx_batch_set_state", "shx_batch_get_state", "shx_batch_init_dynamics",
-           "shx_batch_init_shells", "shx_batch_set_extra_damping",
+           "shx_batch_init_shells", "shx_batch_set_extra_damping", "shx_batch_set_kinetic_damping", "shx_batch_park",
            "shx_batch_step", "shx_batch_reports", "shx_batch_pairs", "shx_batch_cfl_omega_max",
            "shx_set_graphs", "shx_device_count"]
 
@@ -70,7 +70,7 @@ class Report(ctypes.Structure):
         "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
         "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_steps", "n_rejected", "converged_step",
         "n_steps_fp32")] + [("residual_deform", ctypes.c_double)] + [
-        (n, ctypes.c_int32) for n in ("capacity_err", "reserved")]
+        (n, ctypes.c_int32) for n in ("capacity_err", "n_kd")]
 
 
 DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_stiff_mult=1.0,
@@ -127,6 +127,8 @@ def lib():
         L.shx_batch_init_dynamics.argtypes = [_V, ctypes.c_int]
         L.shx_batch_init_shells.argtypes = [_V, ctypes.c_int, ctypes.c_int, _I, _D]
         L.shx_batch_set_extra_damping.argtypes = [_V, _D]
+        L.shx_batch_set_kinetic_damping.argtypes = [_V, _I]
+        L.shx_batch_park.argtypes = [_V, ctypes.c_int, _I]
         L.shx_batch_step.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(Report)]
         L.shx_batch_reports.argtypes = [_V, ctypes.POINTER(Report)]
@@ -428,6 +430,20 @@ class ShellBatch(_Measurements):
         """extra mass-proportional damping per shell (S values; None clears)"""
         a = None if alpha is None else _vec(alpha, self.S)
         self._check(lib().shx_batch_set_extra_damping(self.h, _dp(a)), "shx_batch_set_extra_damping")
+
+    def set_kinetic_damping(self, on=None):
+        """kinetic damping per shell (S flags 0/1, or one bool for all; None turns it off)"""
+        if on is None:
+            p = None
+        else:
+            flags = np.ascontiguousarray(np.broadcast_to(np.asarray(on, dtype=np.int32), (self.S,)))
+            p = flags.ctypes.data_as(_I)
+        self._check(lib().shx_batch_set_kinetic_damping(self.h, p), "shx_batch_set_kinetic_damping")
+
+    def park(self, shells):
+        """the listed shells sit out the steps until init_shells starts them again"""
+        idx = np.ascontiguousarray(shells, dtype=np.int32)
+        self._check(lib().shx_batch_park(self.h, len(idx), idx.ctypes.data_as(_I)), "shx_batch_park")
 
     def advance(self, n_steps=1, tol=0.0, stop_on_converged=False, precision=FP64):
         reps = (Report * self.S)()

@@ -42,6 +42,7 @@ struct shx_ctx {
   SConst* dC = nullptr;
   int nblkE = 0, nblkN = 0, ntile = 0;
   bool any_contact = false;
+  bool any_kd = false;             /* some shell has kinetic damping on: k_kd follows k_fin_node */
   std::vector<void*> allocs;
   std::vector<double> rawb;
   /* per shell, for the FP32 -> FP64 switch: the lowest residual of the
@@ -251,6 +252,7 @@ void launch_node(shx_ctx* ctx, int mode)
   prof_end(ctx);
   prof_begin(ctx, 3);
   k_fin_node<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->any_kd && mode == M_STEP && ctx->grid[3]) k_kd<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P);
   prof_end(ctx);
 }
 
@@ -335,7 +337,7 @@ void fill_report(const DShell& s, shx_report* rep)
   rep->converged_step = s.converged_step; rep->n_steps_fp32 = s.n_fp32;
   rep->residual_deform = s.residual_deform;
   rep->capacity_err = s.err;
-  rep->reserved = 0;
+  rep->n_kd = s.n_kd;
 }
 
 /* SConst of one shell from its parameters (update_derived,
@@ -884,6 +886,7 @@ int shx_batch_init_shells(shx_ctx* ctx, int precision, int n, const int* shells,
     d.done = 0; d.n_steps = 0; d.n_rejected = 0; d.converged_step = -1; d.fresh = 1; d.attempts = 0;
     d.problem = 0; d.err = 0; d.n_rebuilds = 0; d.du_sum = 0;
     d.eprec = precision; d.n_fp32 = 0;
+    d.n_kd = 0; d.kd_zero = 0; d.ek_prev = 0; d.parked = 0;
     ctx->fp32_best[i] = HUGE_VAL;
     ctx->fp32_best_step[i] = 0;
     d.volume_ref = volume0 ? volume0[k] : 0.0;
@@ -925,6 +928,43 @@ int shx_batch_set_extra_damping(shx_ctx* ctx, const double* alpha)
   return write_sh(ctx, s);
 }
 
+int shx_batch_set_kinetic_damping(shx_ctx* ctx, const int* on)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (on)
+    for (int i = 0; i < ctx->S; ++i)
+      if (on[i] != 0 && on[i] != 1) { ctx->err = "kinetic damping flags must be 0 or 1"; return SHX_E_ARG; }
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  bool any = false;
+  for (int i = 0; i < ctx->S; ++i) {
+    s[i].kd_on = on ? on[i] : 0;
+    s[i].kd_zero = 0;
+    s[i].ek_prev = 0;
+    any = any || s[i].kd_on;
+  }
+  if (any != ctx->any_kd && ctx->graph) {
+    /* the captured attempts change */
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  ctx->any_kd = any;
+  return write_sh(ctx, s);
+}
+
+int shx_batch_park(shx_ctx* ctx, int n, const int* shells)
+{
+  if (!ctx || n < 0 || n > ctx->S || (n > 0 && !shells)) return SHX_E_ARG;
+  std::vector<DShell> s;
+  if (int r = read_sh(ctx, s)) return r;
+  for (int k = 0; k < n; ++k) {
+    if (int r = check_shell(ctx, shells[k])) return r;
+    s[shells[k]].parked = 1;
+    s[shells[k]].done = 1;
+  }
+  return write_sh(ctx, s);
+}
+
 int shx_batch_init_dynamics(shx_ctx* ctx, int precision)
 {
   if (!ctx) return SHX_E_ARG;
@@ -961,7 +1001,7 @@ int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int sto
       DShell& d = s[i];
       d.steps_target = d.n_steps + n_steps;
       const bool stopped = d.problem >= SHX_STOP_SIM || (ctx->C[i].stop_on_problem && d.problem > SHX_NO_PROBLEM);
-      d.done = (stopped || d.err || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
+      d.done = (stopped || d.err || d.parked || (stop_on_converged && d.converged_step >= 0)) ? 1 : 0;
       d.eprec = (precision == SHX_FP32_SPLIT && d.eprec == SHX_FP32_SPLIT && !fp32_done(i, d, ctx->C[i]))
                     ? SHX_FP32_SPLIT : SHX_FP64;
     }

@@ -84,7 +84,7 @@ constexpr int kBpPerBlock = 256 / kBpLanes;
 constexpr int kCloseLds = 32;       /* close-set entries of an element staged in LDS by the pair search */
 constexpr int kRowCap = 64;         /* contact candidates per element row */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
-constexpr int kMaxShells = 1024;    /* shells per batch (work maps of the list-driven kernels are staged in LDS) */
+constexpr int kMaxShells = 2048;    /* shells per batch (work maps of the list-driven kernels are staged in LDS) */
 constexpr int kBpGrid = 1024;       /* blocks of the broad-phase kernels (they loop over the work) */
 
 /* per-shell constants (device array, read-only in the step kernels) */
@@ -156,6 +156,12 @@ struct DShell {
   /* results of the element kernel of the current attempt */
   double att_e_tens, att_e_bend, att_volume, att_area;
   double fdeform, residual_deform;
+  /* kinetic damping (set by the host per shell): when an accepted step's
+   * kinetic energy falls below the previous step's (a peak has passed), the
+   * velocities are set to zero before the next attempt */
+  int kd_on, kd_zero, n_kd;
+  int parked;                      /* set by the host: the shell sits out the steps until it is initialised */
+  double ek_prev;
 };
 
 /* error bits in DShell::err */

@@ -650,6 +650,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   }
   if (mode == M_FORCES) return;
   w.attempts++;
+  w.kd_zero = 0;
   if (w.fresh) { w.volume = w.att_volume; w.area = w.att_area; }
   const double dt = w.dt;
   const double target = sqrt(C.rel_loc_err_min * C.rel_loc_err_max);
@@ -688,6 +689,11 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
       const double smp[11] = {1.0, w.att_narea, vol, nc, ek, w.e_tens, w.e_bend, es, ef, w.kgauss, w.kmean};
       for (int q = 0; q < 11; ++q) w.avg[q] += dt * smp[q];
       w.avg_n++;
+    }
+    /* kinetic damping, once the load has reached its final value */
+    if (w.kd_on && w.t >= C.load_end_time) {
+      if (ek < w.ek_prev) { w.kd_zero = 1; w.n_kd++; w.ek_prev = 0; }
+      else w.ek_prev = ek;
     }
     w.rebuild = rebuild_due(w, C);
     if (w.rebuild) plan_rebuild(P, w, sid, 1);
@@ -846,6 +852,20 @@ __global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
     P.npart[(size_t)blk * kNodeRec + t] = acc;
   }
   }
+}
+
+/* kinetic damping: the velocities of the shells whose finalize found a
+ * kinetic-energy peak are set to zero in the state buffer of the next
+ * attempt (one thread per node, the node kernel's block map) */
+__global__ void __launch_bounds__(kNodeBlock) k_kd(DPtrs P)
+{
+  const int blk = uni(P.bmap[V_NODE][blockIdx.x]);
+  const int sid = uni(P.vshell[blk * kNodePerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
+  if (!sh.kd_zero) return;
+  const int n = blk * kNodePerBlock + threadIdx.x;
+  if (n - C.vbase < C.nv) st3(P.V[sh.par], n, mk(0, 0, 0));
 }
 
 /* ------------------------------------------------------------ broad phase */

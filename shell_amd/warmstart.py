@@ -237,6 +237,90 @@ def warm_solve(b, surrogate, P, vref, tol=1e-6, max_steps=60000, chunk=2000, pro
     return reps, wall, info
 
 
+def stream_solve(b, P, vref, slots, surrogate=None, tol=1e-6, max_steps=60000, chunk=500,
+                 probe_steps=PROBE_STEPS, precision=FP64):
+    """Streaming batch: the batch's shells form a queue of which at most
+    `slots` step at once; when a shell converges, stops or reaches max_steps,
+    the next shell of the queue starts in its place.  surrogate given: a new
+    shell starts from the prediction (t = RAMP_TIME, V0 = vref) and falls
+    back to a plain start when its warm state interpenetrates or overflows a
+    list, or when its residual after probe_steps steps is not below its
+    starting residual; otherwise shells start from rest.  No quench (the
+    relaxation is expected to carry the kinetic damping).
+    Returns (reports, wall seconds, info); reports count steps since each
+    shell's (last) start."""
+    t0 = time.perf_counter()
+    S = b.S
+    queue = list(range(S))
+    active, warm, r0 = set(), {}, {}
+    fell = []
+    b.park(np.arange(S))
+
+    def start(idx, use_warm):
+        if not idx:
+            return
+        if use_warm:
+            U0 = surrogate.predict(P[idx])
+            for k, i in enumerate(idx):
+                z = np.zeros(U0.shape[1])
+                b.set_state(i, U0[k], v=z, a=z, t=RAMP_TIME)
+        else:
+            for i in idx:
+                z = np.zeros(3 * int(b.nv[i]))
+                b.set_state(i, z, v=z, a=z, t=0.0)
+        try:
+            b.init_shells(idx, volume0=vref[idx] if use_warm else None, precision=precision)
+        except ShellError as e:
+            if e.code != E_CAPACITY:
+                raise
+        reps = b.reports()
+        for i in idx:
+            warm[i] = use_warm
+            r0[i] = reps[i]["residual_deform"]
+            active.add(i)
+        if use_warm:
+            bad = [i for i in idx if reps[i]["problem"] == CONTACT_PROBLEM or reps[i]["capacity_err"]
+                   or not np.isfinite(r0[i])]
+            fell.extend(bad)
+            start(bad, False)
+
+    def refill():
+        n = min(slots - len(active), len(queue))
+        idx = [queue.pop(0) for _ in range(n)]
+        start(idx, surrogate is not None)
+
+    refill()
+    final = [None] * S
+    while active:
+        reps = _advance(b, chunk, tol, precision, True)
+        finished, back = [], []
+        for i in sorted(active):
+            r = reps[i]
+            if r["converged_step"] >= 0 or stopped(r) or r["n_steps"] >= max_steps:
+                finished.append(i)
+            elif r["capacity_err"]:
+                if warm[i]:
+                    back.append(i)
+                else:
+                    finished.append(i)
+            elif warm[i] and r["n_steps"] >= probe_steps and r["n_steps"] - chunk < probe_steps \
+                    and not r["residual_deform"] < r0[i]:
+                back.append(i)
+        for i in finished:
+            active.discard(i)
+            final[i] = r_copy = dict(reps[i])
+            r_copy["warm"] = warm[i]
+        if finished:
+            b.park(finished)
+        for i in back:
+            active.discard(i)
+        fell.extend(back)
+        start(back, False)
+        refill()
+    wall = time.perf_counter() - t0
+    return final, wall, {"fallback": sorted(set(fell))}
+
+
 def align(Xs, U):
     """remove the rigid motion of each equilibrium: u' such that X + u' is
     X + u translated and rotated (Kabsch) onto X; Xs: one nv x 3 reference
