@@ -99,6 +99,17 @@ def single_workload(level=4):
     return X, T, params
 
 
+def apply_shapes(b, workload):
+    """the autotuned launch shapes of the workload (shell_amd/data/shapes.json,
+    written by python -m shell_amd.autotune), else the defaults"""
+    from shell_amd.autotune import load_shapes
+    shp = load_shapes(workload) if workload else None
+    if shp is None:
+        return "defaults"
+    b.set_shapes(shp)
+    return f"autotuned for {workload} (shell_amd/data/shapes.json)"
+
+
 def with_cfl(batch, params, iters):
     """dt = 0.5 * 2 / omega_max of M^-1 K per shell (power iteration on the GPU force kernel)"""
     omega = batch.cfl_omega_max(iters)
@@ -218,6 +229,7 @@ def run_batch(args):
     b = S.ShellBatch(meshes, params)
     if args.no_graph:
         b.set_graphs(False)
+    shapes_src = apply_shapes(b, "sweep512" if (args.shells == 512 and args.seed == 7) else None)
     log(f"setup {time.perf_counter() - t0:.1f} s: {len(shells)} shells, {sum(len(T) for _, T in meshes)} elements")
 
     def solve():
@@ -273,6 +285,7 @@ def run_batch(args):
                                f"residual {args.tol:g}; one step = the whole batch solved",
                    "shells": len(shells), "elements": int(sum(len(T) for _, T in meshes)),
                    "nodes": int(sum(len(X) for X, _ in meshes)), "converged": int(converged),
+                   "shapes": b.get_shapes(), "shapes_source": shapes_src,
                    "solver_steps_to_converge_min_median_max": {
                        f"L{L}": [int(np.min(v)), int(np.median(v)), int(np.max(v))] for L, v in sorted(by_level.items())},
                    "max_steps": args.max_steps},
@@ -427,6 +440,7 @@ def run_surrogate(args):
         b.set_graphs(False)
     vref = W.reference_volumes(b)
     b.set_kinetic_damping(args.relax == "kinetic")
+    shapes_src = apply_shapes(b, "warm1024" if (b.S == 1024 and args.seed == 12) else None)
     log(f"setup {time.perf_counter() - t0:.1f} s: {b.S} level-4 shells")
     n_dof = 3 * int(b.nv[0])
     prec = S.FP32_SPLIT if args.precision == "fp32" else S.FP64
@@ -490,6 +504,7 @@ def run_surrogate(args):
                    "surrogate_vs_plain": value / plain if plain > 0 else None,
                    "surrogate_wall_s": wall_w, "plain_wall_s": wall_p,
                    "converged_surrogate": int(conv_w / max(args.steps, 1)), "converged_plain": int(conv_p),
+                   "shapes": b.get_shapes(), "shapes_source": shapes_src,
                    "relaxation": args.relax, "quench_steps": 0 if args.stream else args.quench_steps,
                    "probe_steps": args.probe_steps, "stream_slots": args.stream,
                    "fallbacks": len(info["fallback"]),

@@ -214,10 +214,30 @@ int shx_cfl_omega_max(shx_ctx* ctx, int iters, double* omega);
  * stream bracket every launch, and shx_kernel_times gives the average
  * duration in ms and the launch count of each kernel class since profiling
  * was enabled.  Classes: 0 element force, 1 node gather + Newmark, 2 contact
- * narrow phase, 3 per-shell epilogues, 4..8 broad phase box, count, scan,
- * scatter, pairs. */
+ * narrow phase, 3 per-shell epilogues, 4..9 broad phase box, count, scan,
+ * scatter, pair search, exact distance. */
 int shx_set_profiling(shx_ctx* ctx, int enable);
-int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 9 */, int64_t* counts /* 9 */);
+int shx_kernel_times(shx_ctx* ctx, double* avg_ms /* 10 */, int64_t* counts /* 10 */);
+
+/* Launch shapes of the tunable kernels: waves per SIMD of the element,
+ * node, contact and exact-distance kernels (through their launch bounds,
+ * i.e. the registers each thread may use) and the pair search's lanes per
+ * element.  No counterpart in the reference (its loops are OpenMP).  The
+ * results do not depend on the shapes, only the speed; an autotuner
+ * (shell_amd.autotune) times the candidates on a workload and stores its
+ * choice.  Defaults: 4, 6, 3, 16, 2. */
+typedef struct shx_shapes {
+  int32_t elem_waves;              /* k_elem: 3 or 4 */
+  int32_t node_waves;              /* k_node: 4, 6 or 8 */
+  int32_t contact_waves;           /* k_contact: 2, 3 or 4 */
+  int32_t bp_lanes;                /* k_bp_pairs lanes per element: 8, 16 or 32 */
+  int32_t exact_waves;             /* k_bp_exact: 2, 3 or 4 */
+  int32_t reserved[3];
+} shx_shapes;
+/* NULL restores the defaults; other values than the listed candidates are
+ * refused with SHX_E_ARG */
+int shx_set_shapes(shx_ctx* ctx, const shx_shapes* shapes);
+int shx_get_shapes(shx_ctx* ctx, shx_shapes* shapes);
 
 /* Batches of independent shells (no counterpart in the reference, which runs
  * one ShellSystem per process; the batch replaces running many of them side

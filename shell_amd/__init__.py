@@ -38,6 +38,7 @@ EXPORTS = ["shx_create", "shx_destroy", "shx_last_error", "shx_set_state", "shx_
            "shx_kernel_times", "shx_tri_distance", "shx_raw_elem_basis", "shx_batch_create",
            "shx_batch_size", "shx_batch_set_state", "shx_batch_get_state", "shx_batch_init_dynamics",
            "shx_batch_init_shells", "shx_batch_set_extra_damping", "shx_batch_set_kinetic_damping", "shx_batch_park",
+           "shx_set_shapes", "shx_get_shapes",
            "shx_batch_step", "shx_batch_reports", "shx_batch_pairs", "shx_batch_cfl_omega_max",
            "shx_set_graphs", "shx_device_count"]
 
@@ -46,7 +47,7 @@ _lib = None
 ALL_SHELLS = -1
 
 KERNEL_CLASSES = ["elem_force", "node_newmark", "contact_narrow", "shell_epilogues", "bp_box", "bp_count",
-                  "bp_scan", "bp_scatter", "bp_pairs"]
+                  "bp_scan", "bp_scatter", "bp_pairs", "bp_exact"]
 
 
 class Params(ctypes.Structure):
@@ -64,6 +65,9 @@ class Params(ctypes.Structure):
         ("volume_growth_fn", ctypes.c_char * 160)]
 
 
+SHAPE_FIELDS = ("elem_waves", "node_waves", "contact_waves", "bp_lanes", "exact_waves")
+
+
 class Report(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "t", "dt", "residual", "fnorm", "fext_norm", "e_tens", "e_bend", "e_self", "e_fric", "e_kin",
@@ -71,6 +75,31 @@ class Report(ctypes.Structure):
         "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_steps", "n_rejected", "converged_step",
         "n_steps_fp32")] + [("residual_deform", ctypes.c_double)] + [
         (n, ctypes.c_int32) for n in ("capacity_err", "n_kd")]
+
+
+class Shapes(ctypes.Structure):
+    """launch shapes of the tunable kernels (shx_shapes)"""
+    _fields_ = [(n, ctypes.c_int32) for n in SHAPE_FIELDS] + [("reserved", ctypes.c_int32 * 3)]
+
+
+DEFAULT_SHAPES = dict(elem_waves=4, node_waves=6, contact_waves=3, bp_lanes=16, exact_waves=2)
+
+
+class _Shaped:
+    """shx_set_shapes / shx_get_shapes on a context"""
+
+    def set_shapes(self, shapes=None):
+        """launch shapes (dict of SHAPE_FIELDS; None restores the defaults)"""
+        if shapes is None:
+            self._check(lib().shx_set_shapes(self.h, None), "shx_set_shapes")
+            return
+        s = Shapes(**{k: int(shapes.get(k, DEFAULT_SHAPES[k])) for k in SHAPE_FIELDS})
+        self._check(lib().shx_set_shapes(self.h, ctypes.byref(s)), "shx_set_shapes")
+
+    def get_shapes(self):
+        s = Shapes()
+        self._check(lib().shx_get_shapes(self.h, ctypes.byref(s)), "shx_get_shapes")
+        return {k: int(getattr(s, k)) for k in SHAPE_FIELDS}
 
 
 DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_stiff_mult=1.0,
@@ -129,6 +158,8 @@ def lib():
         L.shx_batch_set_extra_damping.argtypes = [_V, _D]
         L.shx_batch_set_kinetic_damping.argtypes = [_V, _I]
         L.shx_batch_park.argtypes = [_V, ctypes.c_int, _I]
+        L.shx_set_shapes.argtypes = [_V, ctypes.POINTER(Shapes)]
+        L.shx_get_shapes.argtypes = [_V, ctypes.POINTER(Shapes)]
         L.shx_batch_step.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(Report)]
         L.shx_batch_reports.argtypes = [_V, ctypes.POINTER(Report)]
@@ -206,7 +237,7 @@ class _Measurements:
         return nf, ef
 
 
-class ShellSystem(_Measurements):
+class ShellSystem(_Measurements, _Shaped):
     """One shell on one GPU (mirrors class ShellSystem, ShellSystem.h:28)."""
 
     def __init__(self, X, T, device=0, **params):
@@ -343,7 +374,7 @@ def _report_dict(rep):
     return {k: getattr(rep, k) for k, _ in Report._fields_}
 
 
-class ShellBatch(_Measurements):
+class ShellBatch(_Measurements, _Shaped):
     """A batch of independent shells on one GPU: every phase of a step is one
     kernel launch over all shells (the per-shell step of
     ShellSystem::advance, ShellSystem.C:952-1122, run side by side; each

@@ -28,7 +28,7 @@ constexpr int kGraphAttempts = 64;   /* step attempts per replayed graph */
 /* profiled kernel classes: element force, node + Newmark, contact narrow
  * phase, per-shell epilogues, and the broad-phase box, count, scan,
  * scatter and pair kernels */
-constexpr int kProfClasses = 9;
+constexpr int kProfClasses = 10;
 }
 
 struct shx_ctx {
@@ -42,7 +42,8 @@ struct shx_ctx {
   SConst* dC = nullptr;
   int nblkE = 0, nblkN = 0, ntile = 0;
   bool any_contact = false;
-  bool any_kd = false;             /* some shell has kinetic damping on: k_kd follows k_fin_node */
+  bool any_kd = false;
+  shx_shapes shapes = {4, 6, 3, 16, 2, {0, 0, 0}};   /* launch shapes (shx_set_shapes) */             /* some shell has kinetic damping on: k_kd follows k_fin_node */
   std::vector<void*> allocs;
   std::vector<double> rawb;
   /* per shell, for the FP32 -> FP64 switch: the lowest residual of the
@@ -197,8 +198,18 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   k_bp_scatter<<<kBpGrid, 256, 0, ctx->st>>>(P, list);
   prof_end(ctx);
   prof_begin(ctx, 8);
-  k_bp_pairs<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
-  k_bp_exact<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  switch (ctx->shapes.bp_lanes) {
+    case 8: k_bp_pairs_l8<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
+    case 32: k_bp_pairs_l32<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
+    default: k_bp_pairs_l16<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  }
+  prof_end(ctx);
+  prof_begin(ctx, 9);
+  switch (ctx->shapes.exact_waves) {
+    case 3: k_bp_exact_w3<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
+    case 4: k_bp_exact_w4<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
+    default: k_bp_exact_w2<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  }
   prof_end(ctx);
 }
 
@@ -236,7 +247,10 @@ int set_active(shx_ctx* ctx, const std::vector<DShell>& s)
 void launch_elem(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 0);
-  if (ctx->grid[2]) k_elem<<<ctx->grid[2], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[2]) {
+    if (ctx->shapes.elem_waves == 3) k_elem_w3<<<ctx->grid[2], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
+    else k_elem_w4<<<ctx->grid[2], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
+  }
   if (ctx->grid[4]) k_elem32<<<ctx->grid[4], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
   prof_begin(ctx, 3);
@@ -248,7 +262,13 @@ void launch_elem(shx_ctx* ctx, int mode)
 void launch_node(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 1);
-  if (ctx->grid[3]) k_node<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[3]) {
+    switch (ctx->shapes.node_waves) {
+      case 4: k_node_w4<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode); break;
+      case 8: k_node_w8<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode); break;
+      default: k_node_w6<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
+    }
+  }
   prof_end(ctx);
   prof_begin(ctx, 3);
   k_fin_node<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
@@ -259,7 +279,13 @@ void launch_node(shx_ctx* ctx, int mode)
 void launch_contact(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 2);
-  if (ctx->grid[0]) k_contact<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode);
+  if (ctx->grid[0]) {
+    switch (ctx->shapes.contact_waves) {
+      case 2: k_contact_w2<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode); break;
+      case 4: k_contact_w4<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode); break;
+      default: k_contact_w3<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode);
+    }
+  }
   prof_end(ctx);
 }
 
@@ -544,7 +570,6 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     C.blkE = (int)(eb / kElemPerBlock); C.nbE = nep / kElemPerBlock;
     C.blkN = (int)(vb / kNodePerBlock); C.nbN = nvp / kNodePerBlock;
     C.tile0 = (int)(eb / 256); C.ntile = nep / 256;
-    C.nbL = nep / kBpPerBlock;
     if (C.contact) {
       /* hashed cell grid: a sixteenth of the bound (1/cell_size_mult + 2)^3
        * on the (cell, element) entries per element; a shell's surface
@@ -926,6 +951,38 @@ int shx_batch_set_extra_damping(shx_ctx* ctx, const double* alpha)
   if (int r = read_sh(ctx, s)) return r;
   for (int i = 0; i < ctx->S; ++i) s[i].alpha_extra = alpha ? alpha[i] : 0.0;
   return write_sh(ctx, s);
+}
+
+int shx_set_shapes(shx_ctx* ctx, const shx_shapes* shp)
+{
+  if (!ctx) return SHX_E_ARG;
+  const shx_shapes def = {4, 6, 3, 16, 2, {0, 0, 0}};
+  shx_shapes s = shp ? *shp : def;
+  auto in = [](int v, std::initializer_list<int> ok) {
+    for (int x : ok) if (v == x) return true;
+    return false;
+  };
+  if (!in(s.elem_waves, {3, 4}) || !in(s.node_waves, {4, 6, 8}) || !in(s.contact_waves, {2, 3, 4}) ||
+      !in(s.bp_lanes, {8, 16, 32}) || !in(s.exact_waves, {2, 3, 4})) {
+    ctx->err = "launch shapes outside the candidates (elem_waves 3|4, node_waves 4|6|8, contact_waves 2|3|4, "
+               "bp_lanes 8|16|32, exact_waves 2|3|4)";
+    return SHX_E_ARG;
+  }
+  s.reserved[0] = s.reserved[1] = s.reserved[2] = 0;
+  if (std::memcmp(&s, &ctx->shapes, sizeof(s)) != 0 && ctx->graph) {
+    /* the captured attempts launch other kernels */
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  ctx->shapes = s;
+  return SHX_OK;
+}
+
+int shx_get_shapes(shx_ctx* ctx, shx_shapes* out)
+{
+  if (!ctx || !out) return SHX_E_ARG;
+  *out = ctx->shapes;
+  return SHX_OK;
 }
 
 int shx_batch_set_kinetic_damping(shx_ctx* ctx, const int* on)

@@ -13,9 +13,9 @@
 namespace shx {
 
 /* blocks of one shell for a kernel family: V_TILE 256-element tiles,
- * V_LANE broad-phase lane blocks, V_ELEM element-kernel blocks, V_NODE
- * node-kernel blocks */
-enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3, V_ELEM32 = 4 };
+ * V_LANE / V_LANE8 / V_LANE32 pair-search blocks at 16 / 8 / 32 lanes per
+ * element, V_ELEM element-kernel blocks, V_NODE node-kernel blocks */
+enum { V_TILE = 0, V_LANE = 1, V_ELEM = 2, V_NODE = 3, V_ELEM32 = 4, V_LANE8 = 5, V_LANE32 = 6 };
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 

@@ -79,8 +79,8 @@ constexpr int kNodeRec = 23;        /* node-kernel partial record length */
 constexpr int kElemRec = 9;         /* element-kernel partial record length */
 constexpr double kTentative = -1.0;  /* d2 of a candidate pair whose exact distance is not yet formed */
 constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and rebuild (capacity) */
-constexpr int kBpLanes = 16;        /* pair-search lanes per element (cells, then bucket entries) */
-constexpr int kBpPerBlock = 256 / kBpLanes;
+/* pair-search lanes per element (cells, then bucket entries): 8, 16 or 32,
+ * a launch shape of the context (shx_shapes::bp_lanes) */
 constexpr int kCloseLds = 32;       /* close-set entries of an element staged in LDS by the pair search */
 constexpr int kRowCap = 64;         /* contact candidates per element row */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
@@ -95,7 +95,6 @@ struct SConst {
   int blkE, nbE;                   /* element-kernel blocks (kElemPerBlock elements each) */
   int blkN, nbN;                   /* node-kernel blocks (kNodePerBlock nodes each) */
   int tile0, ntile;                /* 256-element tiles (box and contact kernel blocks) */
-  int nbL;                         /* broad-phase lane blocks (kBpPerBlock elements each) */
   unsigned bbase, hb_bits;         /* hash-bucket region: 1 << hb_bits buckets from bbase */
   unsigned entbase;                /* (cell, element) entry region: ne * kEntPerElem entries */
   /* physics (update_derived, ShellSystem.C:421-487) */

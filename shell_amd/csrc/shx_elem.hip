@@ -61,7 +61,7 @@ __device__ __forceinline__ int rec_of(int t, int m) { return m == 0 ? t : 2 * kE
  * list was rebuilt at the end of the previous step clear its cell buckets.
  * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
  * active shells, set by the host). */
-__global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
+__device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
 {
   __shared__ double s_rb[4 * 72];
   __shared__ double s_w[kElemRec][kElemBlock / 64];
@@ -298,6 +298,10 @@ __global__ void __launch_bounds__(kElemBlock, 4) k_elem(DPtrs P, int mode)
   }
   }
 }
+
+/* launch-shape variants (waves per SIMD), picked by the context's shx_shapes */
+__global__ void __launch_bounds__(kElemBlock, 3) k_elem_w3(DPtrs P, int mode) { elem_body(P, mode); }
+__global__ void __launch_bounds__(kElemBlock, 4) k_elem_w4(DPtrs P, int mode) { elem_body(P, mode); }
 
 /* FP32 split element force: the same method as k_elem, with the metric and
  * curvature changes formed in FP32 from the displacement gradients relative

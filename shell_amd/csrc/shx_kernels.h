@@ -21,10 +21,15 @@ enum { SHX_PREC_FP64 = 0, SHX_PREC_FP32 = 1 };
 
 __global__ void k_precompute(DPtrs P);
 __global__ void k_mass(DPtrs P);
-__global__ void k_elem(DPtrs P, int mode);
+__global__ void k_elem_w3(DPtrs P, int mode);
+__global__ void k_elem_w4(DPtrs P, int mode);
 __global__ void k_fin_elem(DPtrs P, int mode);
-__global__ void k_contact(DPtrs P, int mode);
-__global__ void k_node(DPtrs P, int mode);
+__global__ void k_contact_w2(DPtrs P, int mode);
+__global__ void k_contact_w3(DPtrs P, int mode);
+__global__ void k_contact_w4(DPtrs P, int mode);
+__global__ void k_node_w4(DPtrs P, int mode);
+__global__ void k_node_w6(DPtrs P, int mode);
+__global__ void k_node_w8(DPtrs P, int mode);
 __global__ void k_precompute32(DPtrs P);
 __global__ void k_elem32(DPtrs P, int mode);
 __global__ void k_fin_node(DPtrs P, int mode);
@@ -34,8 +39,12 @@ __global__ void k_bp_grid(DPtrs P, int list);
 __global__ void k_bp_count(DPtrs P, int posmode, int list);
 __global__ void k_bp_scan(DPtrs P, int list);
 __global__ void k_bp_scatter(DPtrs P, int list);
-__global__ void k_bp_pairs(DPtrs P, int posmode, int list);
-__global__ void k_bp_exact(DPtrs P, int posmode, int list);
+__global__ void k_bp_pairs_l8(DPtrs P, int posmode, int list);
+__global__ void k_bp_pairs_l16(DPtrs P, int posmode, int list);
+__global__ void k_bp_pairs_l32(DPtrs P, int posmode, int list);
+__global__ void k_bp_exact_w2(DPtrs P, int posmode, int list);
+__global__ void k_bp_exact_w3(DPtrs P, int posmode, int list);
+__global__ void k_bp_exact_w4(DPtrs P, int posmode, int list);
 __global__ void k_bp_host_plan(DPtrs P);
 __global__ void k_bp_host_done(DPtrs P);
 

@@ -252,7 +252,9 @@ __device__ void plan_rebuild(const DPtrs& P, DShell& w, int sid, int list)
 
 __device__ __forceinline__ int vcount(const SConst& C, int kind)
 {
-  return kind == V_TILE ? C.ntile : kind == V_LANE ? C.nbL : kind == V_NODE ? C.nbN : C.nbE;
+  /* pair-search blocks: 256 / lanes elements each, nbE * lanes per shell */
+  return kind == V_TILE ? C.ntile : kind == V_LANE ? 16 * C.nbE : kind == V_LANE8 ? 8 * C.nbE
+       : kind == V_LANE32 ? 32 * C.nbE : kind == V_NODE ? C.nbN : C.nbE;
 }
 
 /* the work of the shells on a work list as virtual blocks of a kernel
@@ -399,7 +401,7 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
  * the order in which the pairs are processed.  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
  * a shell whose list was rebuilt in this step clear its cell buckets.
  * Launch block b takes 256-element tile bmap[V_TILE][b]. */
-__global__ void __launch_bounds__(256, 3) k_contact(DPtrs P, int mode)
+__device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
 {
   __shared__ double s_w[kCRec][4];
   __shared__ int s_pre[257];             /* exclusive prefix of the tile's row lengths */
@@ -725,7 +727,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
  * the rigid-body sums, for k_fin_node.  Launch block b computes node block
  * bmap[V_NODE][b].  Force time: t + dt in a step, t + dt in M_INIT
  * (apply_initial_conditions), DShell::t_eval in M_FORCES. */
-__global__ void __launch_bounds__(kNodeBlock, 6) k_node(DPtrs P, int mode)
+__device__ __forceinline__ void node_body(const DPtrs& P, int mode)
 {
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
   const int blk = uni(P.bmap[V_NODE][blockIdx.x]);
@@ -1276,8 +1278,10 @@ __global__ void __launch_bounds__(256) k_bp_scatter(DPtrs P, int list)
  * the element's old row.  Entries are appended to the element's row of the
  * new buffer with an atomic cursor, so the order within a row depends on
  * arrival; the pair set does not. */
-__global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list)
+template <int kBpLanes>
+__device__ __forceinline__ void pairs_body(const DPtrs& P, int posmode, int list)
 {
+  constexpr int kBpPerBlock = 256 / kBpLanes;
   if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
   /* per element of the block: its close set (when it has at most kCloseLds
@@ -1285,7 +1289,8 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
   __shared__ int s_close[kBpPerBlock][kCloseLds];
   __shared__ int s_old[kBpPerBlock][kRowCap];
   __shared__ int s_go;
-  const int nl = vmap_build(P, P.wl[list], V_LANE, s_sh, s_pref, s_tmp);
+  const int nl = vmap_build(P, P.wl[list], kBpLanes == 8 ? V_LANE8 : kBpLanes == 16 ? V_LANE : V_LANE32, s_sh,
+                            s_pref, s_tmp);
   for (int vb = blockIdx.x; vb < s_pref[nl]; vb += gridDim.x) {
     const int li = vmap_find(vb, nl, s_pref);
     const int sid = uni(s_sh[li]);
@@ -1397,7 +1402,7 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DPtrs P, int posmode, int list
  * CONTACT_PROBLEM): one thread per element row, the row compacted in place
  * and sorted by e2.  Kept apart from the pair search so that the search runs at
  * the occupancy of its own (small) register footprint. */
-__global__ void __launch_bounds__(256) k_bp_exact(DPtrs P, int posmode, int list)
+__device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list)
 {
   if (P.wl[list][0] == 0) return;
   __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
@@ -1509,5 +1514,21 @@ __global__ void __launch_bounds__(256) k_bp_host_done(DPtrs P)
   __syncthreads();
   if (threadIdx.x == 0) { w.bclear = 0; w.rebuild = 0; }
 }
+
+/* launch-shape variants of the tunable kernels (waves per SIMD through
+ * the launch bounds; pair-search lanes per element), picked by the
+ * context's shx_shapes */
+__global__ void __launch_bounds__(256, 2) k_contact_w2(DPtrs P, int mode) { contact_body(P, mode); }
+__global__ void __launch_bounds__(256, 3) k_contact_w3(DPtrs P, int mode) { contact_body(P, mode); }
+__global__ void __launch_bounds__(256, 4) k_contact_w4(DPtrs P, int mode) { contact_body(P, mode); }
+__global__ void __launch_bounds__(kNodeBlock, 4) k_node_w4(DPtrs P, int mode) { node_body(P, mode); }
+__global__ void __launch_bounds__(kNodeBlock, 6) k_node_w6(DPtrs P, int mode) { node_body(P, mode); }
+__global__ void __launch_bounds__(kNodeBlock, 8) k_node_w8(DPtrs P, int mode) { node_body(P, mode); }
+__global__ void __launch_bounds__(256) k_bp_pairs_l8(DPtrs P, int posmode, int list) { pairs_body<8>(P, posmode, list); }
+__global__ void __launch_bounds__(256) k_bp_pairs_l16(DPtrs P, int posmode, int list) { pairs_body<16>(P, posmode, list); }
+__global__ void __launch_bounds__(256) k_bp_pairs_l32(DPtrs P, int posmode, int list) { pairs_body<32>(P, posmode, list); }
+__global__ void __launch_bounds__(256, 2) k_bp_exact_w2(DPtrs P, int posmode, int list) { exact_body(P, posmode, list); }
+__global__ void __launch_bounds__(256, 3) k_bp_exact_w3(DPtrs P, int posmode, int list) { exact_body(P, posmode, list); }
+__global__ void __launch_bounds__(256, 4) k_bp_exact_w4(DPtrs P, int posmode, int list) { exact_body(P, posmode, list); }
 
 } // namespace shx
