@@ -54,8 +54,10 @@ enum { SHX_FP64 = 0, SHX_FP32_SPLIT = 1 };
 
 /* Parameters: the config.cfg keys of ShellSystem::set_parameters
  * (ShellSystem.C:170-418) that the path reads, plus build extensions marked
- * "ext".  Time functions of the reference (muparser, Function.h) are
- * replaced by the linear ramps below. */
+ * "ext".  The time functions of the reference (muparser, Function.h) that
+ * may vary in t are pressure, volume_growth and the cavity semi-axes (the
+ * *_fn expressions); a constant pressure or volume ratio may instead be
+ * ramped linearly by the ramp times below. */
 typedef struct {
   double E1, E2, nu, G12;          /* shell_young_modulus, shell_young_modulus2, shell_poisson_ratio, shell_shear_modulus */
   double thickness, rho;           /* thickness, shell_mass_density */
@@ -95,6 +97,18 @@ typedef struct {
   char volume_growth_fn[160];      /* volume_growth: target enclosed volume over V0 as an expression in t;
                                       non-empty replaces the linear volume ramp (the reference parses it but
                                       uses the fixed schedule of quirk_volume_schedule, ShellSystem.C:1474) */
+  /* the loads of compute_external_force beyond the pressure (ShellSystem.C:1587-1786) */
+  int32_t cavity_contact;          /* shell_cavity_contact: 1 keeps the shell inside the cavity (per-node penalty,
+                                      normal damping, Friction::apply with the node's own spring) */
+  int32_t cavity;                  /* cavity (enum Cavity, ShellSystem.h:43): 0 box, 1 cylinder (axis x),
+                                      2 ellipsoid */
+  double cavity_R[3];              /* Rx, Ry, Rz: the cavity's semi-axes (constant unless cavity_fn is set) */
+  double mu_s_cavity, mu_d_cavity, cavity_damping, cavity_friction_damping;
+  double gravity;                  /* gravity: f[gravity_dir] -= gravity m per node */
+  double damping_rotation;         /* damping_rotation: > 0 adds -damping_rotation (I^-1 L) x p per node */
+  int32_t gravity_dir;             /* gravity_dir: 0, 1 or 2 */
+  int32_t reserved1;
+  char cavity_fn[3][160];          /* Rx, Ry, Rz as expressions in t; non-empty replaces cavity_R[k] */
 } shx_params;
 
 /* The reference's defaults for every field (ShellSystem constructor and
@@ -108,11 +122,12 @@ void shx_params_default(shx_params* p);
  * `-key value` overrides, which win (argv[0] is not skipped: pass the
  * arguments only).  Keys get the reference's defaults when absent (a
  * warning, as macros.h:16-17), the reference's ASSERTs are checked, and
- * the [function] keys are muparser expressions: constant for the material
- * and contact keys, in t for pressure and volume_growth (compiled for the
- * GPU).  Keys of features outside this path (cavity contact, growth,
- * gravity, perturbation, smooth contact, non-local pressure, extra
- * quadrature points) are refused when set to values that turn them on.
+ * the [function] keys are muparser expressions: constant for the material,
+ * contact, gravity and damping keys, in t for pressure, volume_growth and
+ * the cavity semi-axes Rx, Ry, Rz (compiled for the GPU).  Keys of features
+ * outside this path (growth, perturbation, smooth contact, non-local
+ * pressure, extra quadrature points) are refused when set to values that
+ * turn them on.
  * Returns SHX_OK (message: warnings) or SHX_E_ARG (message: the error). */
 typedef struct {
   shx_params prm;
@@ -152,6 +167,9 @@ typedef struct {
                                       2 pairs, 4 contact row, 8 cells); the shell stopped and the call returned
                                       SHX_E_CAPACITY; shx_batch_init_shells clears it */
   int32_t n_kd;                    /* kinetic-damping velocity resets (shx_batch_set_kinetic_damping) */
+  double e_cav;                    /* cavity contact energy at the last accepted force evaluation */
+  int32_t n_cav_contacts;          /* nodes that penetrate the cavity (Nc_cav) */
+  int32_t reserved2;
 } shx_report;
 
 /* Replaces: ShellSystem constructor + set_parameters + build_cache +
@@ -301,6 +319,13 @@ int shx_batch_park(shx_ctx* ctx, int n, const int* shells);
 int shx_batch_step(shx_ctx* ctx, int precision, int n_steps, double tol, int stop_on_converged,
                    shx_report* reps);
 int shx_batch_reports(shx_ctx* ctx, shx_report* reps);
+/* per node of one shell (nv values or triples; any output may be NULL):
+ * the cavity friction spring origin (NodeCache::spring_origin, the state
+ * the reference saves per node, ShellSystem.C:2972-2982), set when io_set
+ * != 0 and read otherwise.  A zero origin means no spring.  Springs that
+ * were set survive the shell's next initialisation (shx_batch_init_shells
+ * otherwise releases them). */
+int shx_batch_cavity_springs(shx_ctx* ctx, int shell, double* origin, int io_set);
 /* measure_properties (ShellSystem.C:2714-2846) of every shell's current
  * configuration: one sample, weighted with the current dt, added to the
  * moving averages (each accepted step adds its own sample on the GPU; the
@@ -312,11 +337,12 @@ int shx_batch_measure(shx_ctx* ctx);
  * averages since the last call, as the reference forms them (MovingAverage:
  * mean of the samples over the mean dt), then reset.  out[16]: dt, time,
  * area, volume, density, pressure, Nc_cav, Nc_self, E_kin, E_tens, E_bend,
- * E_cav, E_self, E_fric, kGauss, kMean (the cavity columns are 0). */
+ * E_cav, E_self, E_fric, kGauss, kMean (density: area h / cavity volume of
+ * the last sample; the cavity columns are 0 without cavity contact). */
 int shx_batch_properties(shx_ctx* ctx, int shell, double* out);
 /* The measurement fields the reference writes to its .vtu files
  * (ShellSystem.C:3156-3320), from a force evaluation at the current state:
- * node_out[nv*5] = ecav (0 on this path), efric, ekin (over the node's mean
+ * node_out[nv*5] = ecav, efric, ekin (over the node's mean
  * element area), eself (the contacts' energy shares, ShellSystem.C:
  * 1862-1877), mass; elem_out[ne*4] = ebend, etens, kGauss, kMean, each over
  * the element's area.  The state is unchanged.  Either output may be NULL. */
@@ -335,9 +361,10 @@ int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, do
 int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, const uint32_t* e2,
                          const double* origin);
 /* The raw moving-average accumulators of one shell (set == 0: read into
- * sums[11] and *n; set != 0: write them): dt-weighted sums of dt, area,
- * volume, Nc, E_kin, E_tens, E_bend, E_self, E_fric, kGauss, kMean and the
- * sample count (state files carry them, ShellSystem.C:2984-3001). */
+ * sums[14] and *n; set != 0: write them): dt-weighted sums of dt, area,
+ * volume, Nc, E_kin, E_tens, E_bend, E_self, E_fric, kGauss, kMean, p_cav,
+ * Nc_cav, E_cav and the sample count (state files carry them,
+ * ShellSystem.C:2984-3001). */
 int shx_batch_averages(shx_ctx* ctx, int shell, double* sums, int* n, int set);
 /* The parameters one shell was created with. */
 int shx_batch_params(shx_ctx* ctx, int shell, shx_params* out);

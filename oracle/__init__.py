@@ -54,7 +54,10 @@ class Params(ctypes.Structure):
         "newmark_gamma", "dt", "dt_min", "dt_max", "rel_loc_err_min", "rel_loc_err_max")] + [
         (n, ctypes.c_int32) for n in ("adapt_dt_every",
         "contact", "quirk_fulltime", "quirk_volume_schedule", "omp_faithful", "volume_at_prediction",
-        "rebuild_trigger")]
+        "rebuild_trigger", "cavity_contact", "cavity", "gravity_dir", "pad0")] + [
+        ("cavity_R0", ctypes.c_double * 3)] + [(n, ctypes.c_double) for n in (
+        "cavity_rate", "mu_s_cavity", "mu_d_cavity", "cavity_damping", "cavity_friction_damping", "gravity",
+        "damping_rotation")]
 
 
 class Report(ctypes.Structure):
@@ -62,7 +65,10 @@ class Report(ctypes.Structure):
         "t", "residual", "fnorm", "fext_norm", "e_tens", "e_bend", "e_self", "e_fric", "e_kin",
         "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
         "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_rejected")] + [("dt", ctypes.c_double),
-                                                                          ("residual_deform", ctypes.c_double)]
+                                                                          ("residual_deform", ctypes.c_double),
+                                                                          ("e_cav", ctypes.c_double),
+                                                                          ("n_cav", ctypes.c_int32),
+                                                                          ("pad", ctypes.c_int32)]
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -89,6 +95,7 @@ def _declare(L):
     L.orc_step.argtypes = [_V, ctypes.c_int, ctypes.POINTER(Report)]
     L.orc_measure.argtypes = [_V]
     L.orc_properties.argtypes = [_V, _D]
+    L.orc_node_cavity.argtypes = [_V, _D, _D, _D]
     L.orc_elem_measure.argtypes = [_V, _D]
     L.orc_contact_rebuild.argtypes = [_V, ctypes.c_int]
     L.orc_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
@@ -139,6 +146,8 @@ def make_params(**kw):
     defaults = default_params()
     defaults.update(kw)
     for k, v in defaults.items():
+        if k == "cavity_R0":
+            v = (ctypes.c_double * 3)(*v)
         setattr(p, k, v)
     return p
 
@@ -153,7 +162,9 @@ def default_params():
                 newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0, rel_loc_err_min=5e-5,
                 rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0, quirk_fulltime=0,
                 quirk_volume_schedule=0, omp_faithful=0, volume_at_prediction=0,
-                rebuild_trigger=0)
+                rebuild_trigger=0, cavity_contact=0, cavity=2, gravity_dir=2, pad0=0,
+                cavity_R0=(1.0, 1.0, 1.0), cavity_rate=0.0, mu_s_cavity=0.0, mu_d_cavity=0.0,
+                cavity_damping=0.0, cavity_friction_damping=0.0, gravity=0.0, damping_rotation=0.0)
 
 
 class Oracle:
@@ -244,6 +255,15 @@ class Oracle:
         out = np.zeros(16)
         self.L.orc_properties(self.h, dp(out))
         return out
+
+    def node_cavity(self):
+        """per node: cavity spring origin (nv x 3), cavity and cavity-friction
+        energy over the node's mean element area, from the last evaluation"""
+        o = np.zeros((self.nv, 3))
+        ec = np.zeros(self.nv)
+        ef = np.zeros(self.nv)
+        self.L.orc_node_cavity(self.h, dp(o), dp(ec), dp(ef))
+        return o, ec, ef
 
     def elem_measure(self):
         out = np.zeros((self.ne, 10))

@@ -313,13 +313,57 @@ struct System {
   /* moving averages of measure_properties (MovingAverage samples summed in
    * order): dt, area, volume, Nc, E_kin, E_tens, E_bend, E_self, E_fric,
    * kGauss, kMean */
-  double avg[11] = {};
+  double avg[14] = {};
   int avg_n = 0;
+  /* cavity (update_derived, ShellSystem.C:421-439, at the force time of the
+   * last evaluation) and its measurements */
+  double R_eff[3] = {}, avg_R_eff = 0, inv_R2_eff[3] = {}, one_plus_delta_friction = 0;
+  double cavity_surface = 0, cavity_volume = 0, density = 0;
+  std::vector<Pt> cav_origin;        /* NodeCache::spring_origin */
+  std::vector<double> ecav_node, efric_node;
+  int Nc_cav = 0;
+  double Ecav = 0, cavity_force = 0;
 
   void reg(int pr) { if (pr > problem) problem = pr; }
 };
 
-/* update_derived, ShellSystem.C:421-487 (the cavity part is out of scope) */
+/* the cavity part of update_derived, ShellSystem.C:421-475, at time t:
+ * effective semi-axes, their mean and inverse squares, the friction scope,
+ * the cavity's surface (Knud Thomsen's approximation for the ellipsoid) and
+ * volume */
+void update_cavity(System& s, double t)
+{
+  const orc_params& P = s.prm;
+  const double f = P.cavity_rate != 0 ? std::exp(P.cavity_rate * t) : 1.0;
+  const double Rx = P.cavity_R0[0] * f, Ry = P.cavity_R0[1] * f, Rz = P.cavity_R0[2] * f;
+  const double h_half = 0.5 * P.thickness;
+  s.R_eff[0] = Rx - h_half;
+  s.R_eff[1] = Ry - h_half;
+  s.R_eff[2] = Rz - h_half;
+  s.avg_R_eff = (s.R_eff[0] + s.R_eff[1] + s.R_eff[2]) / 3;
+  for (int q = 0; q < 3; ++q) s.inv_R2_eff[q] = 1 / (s.R_eff[q] * s.R_eff[q]);
+  s.one_plus_delta_friction = std::pow(1 - P.static_friction_tol * P.thickness / s.avg_R_eff, 2);
+  const double pi = 3.14159265358979323846;
+  switch (P.cavity) {
+    case 0:
+      s.cavity_surface = 8 * (Rx * Ry + Rx * Rz + Ry * Rz);
+      s.cavity_volume = 8 * Rx * Ry * Rz;
+      break;
+    case 1: {
+      const double lambda = (Ry - Rz) / (Ry + Rz);
+      const double lambda32 = 3 * lambda * lambda;
+      s.cavity_surface = 2 * pi * ((Ry + Rz) * (1 + lambda32 / (10 + std::sqrt(4 - lambda32))) + Ry * Rz);
+      s.cavity_volume = 2 * pi * Rx * Ry * Rz;
+      break;
+    }
+    default:
+      s.cavity_surface = 4 * pi * std::pow((std::pow(Rx * Ry, 1.6) + std::pow(Rx * Rz, 1.6) + std::pow(Ry * Rz, 1.6)) / 3,
+                                           0.625);
+      s.cavity_volume = 4. / 3 * pi * Rx * Ry * Rz;
+  }
+}
+
+/* update_derived, ShellSystem.C:421-487 (the cavity part: update_cavity) */
 void update_derived(System& s)
 {
   const orc_params& P = s.prm;
@@ -705,6 +749,141 @@ void pressure_force(System& s, double t)
   }
 }
 
+/* avg_area_node, ShellSystem.C:2462-2481: mean deformed area of the
+ * elements that have the node as a vertex (element order) */
+double avg_area_node(const System& s, int n)
+{
+  double a = 0;
+  for (int e : s.node_elems[n]) a += s.ec[e].area;
+  return a / s.node_elems[n].size();
+}
+
+/* cavity_isoparameter 1 + delta, ShellSystem.h:336-365 */
+double cavity_isoparameter(const System& s, const Pt& p)
+{
+  double x = p(0) / s.R_eff[0], y = p(1) / s.R_eff[1], z = p(2) / s.R_eff[2];
+  x = x * x; y = y * y; z = z * z;
+  switch (s.prm.cavity) {
+    case 0: {
+      double m = x;
+      if (y > m) m = y;
+      if (z > m) m = z;
+      return m;
+    }
+    case 1: return std::max(x, y + z);
+    default: return x + y + z;
+  }
+}
+
+double sgn(double x) { return (double)((0 < x) - (x < 0)); }
+
+/* the loads of compute_external_force after the pressure, in its order
+ * (ShellSystem.C:1587-1786): gravity, cavity contact with friction, the
+ * rotation damping force; added to f and fext.  The node positions s.p are
+ * those of u_pred, velocities s.v, element areas those of the deformed
+ * metrics just formed. */
+void body_loads(System& s, double t)
+{
+  const orc_params& P = s.prm;
+  if (P.gravity != 0)
+    for (int n = 0; n < s.nv; ++n) {
+      const int dof = 3 * n + P.gravity_dir;
+      s.f[dof] -= P.gravity / s.Minv[dof];
+      s.fext[dof] -= P.gravity / s.Minv[dof];
+    }
+  if (P.cavity_contact) {
+    update_cavity(s, t);
+    const bool cavity_friction = P.mu_s_cavity != 0 || P.mu_d_cavity != 0;
+    for (int n = 0; n < s.nv; ++n) {
+      const Pt& p = s.p[n];
+      const double one_plus_delta = cavity_isoparameter(s, p);
+      if (one_plus_delta > s.one_plus_delta_friction) {
+        const double depth = s.avg_R_eff * (std::sqrt(one_plus_delta) - 1);
+        const double area_node = avg_area_node(s, n);
+        const double stiffness = s.stiff_density * area_node;
+        double f_normal = 0;
+        if (depth > 0) {
+          ++s.Nc_cav;
+          const double E = 0.5 * stiffness * depth * depth;   /* contact_energy, ShellSystem.h:411-417 */
+          s.Ecav += E;
+          s.ecav_node[n] += E / area_node;
+          f_normal = stiffness * depth;                       /* contact_force, ShellSystem.h:403-409 */
+          s.cavity_force += f_normal;
+        }
+        Pt nv;
+        switch (P.cavity) {
+          case 0: {
+            const double xr = p(0) * p(0) * s.inv_R2_eff[0], yr = p(1) * p(1) * s.inv_R2_eff[1],
+                         zr = p(2) * p(2) * s.inv_R2_eff[2];
+            if (xr > yr && xr > zr) nv(0) = sgn(p(0));
+            else if (yr > zr) nv(1) = sgn(p(1));
+            else nv(2) = sgn(p(2));
+            break;
+          }
+          case 1: {
+            const double xr = p(0) * p(0) * s.inv_R2_eff[0], yr = p(1) * p(1) * s.inv_R2_eff[1],
+                         zr = p(2) * p(2) * s.inv_R2_eff[2];
+            if (xr > yr + zr) nv(0) = sgn(p(0));
+            else { nv(1) = p(1) * s.inv_R2_eff[1]; nv(2) = p(2) * s.inv_R2_eff[2]; }
+            break;
+          }
+          default:
+            nv = Pt(p(0) * s.inv_R2_eff[0], p(1) * s.inv_R2_eff[1], p(2) * s.inv_R2_eff[2]);
+        }
+        nv *= 1 / nv.size();
+        const Pt vel(s.v[3 * n], s.v[3 * n + 1], s.v[3 * n + 2]);
+        const double abs_v_normal = vel * nv;
+        Pt f_contact;
+        if (cavity_friction) {
+          double Efric_node = 0;
+          bool stick = s.cav_origin[n].size_sq() != 0;
+          friction_apply(Efric_node, f_contact, f_normal, nv, vel, abs_v_normal, P.friction_stop_velocity,
+                         s.stiff_static_friction, P.mu_s_cavity, P.mu_d_cavity, P.cavity_friction_damping, stick,
+                         s.cav_origin[n], p);
+          s.Efric += Efric_node;
+          s.efric_node[n] += Efric_node / area_node;
+        }
+        f_normal += P.cavity_damping * stiffness * abs_v_normal;
+        f_contact -= f_normal * nv;
+        for (int q = 0; q < 3; ++q) { s.f[3 * n + q] += f_contact(q); s.fext[3 * n + q] += f_contact(q); }
+      } else {
+        s.cav_origin[n].zero();
+      }
+    }
+  }
+  if (P.damping_rotation > 0) {
+    Pt L;
+    double I[9] = {};
+    for (int n = 0; n < s.nv; ++n) {
+      const Pt& ri = s.p[n];
+      const Pt vi(s.v[3 * n], s.v[3 * n + 1], s.v[3 * n + 2]);
+      const double mi = 1 / s.Minv[3 * n];
+      L += mi * ri.cross(vi);
+      I[0] += mi * (ri(1) * ri(1) + ri(2) * ri(2));
+      I[4] += mi * (ri(0) * ri(0) + ri(2) * ri(2));
+      I[8] += mi * (ri(0) * ri(0) + ri(1) * ri(1));
+      I[1] -= mi * ri(0) * ri(1);
+      I[2] -= mi * ri(0) * ri(2);
+      I[5] -= mi * ri(1) * ri(2);
+    }
+    I[3] = I[1]; I[6] = I[2]; I[7] = I[5];
+    double Iinv[9];
+    inverse3(I, Iinv);
+    const Pt omega(Iinv[0] * L(0) + Iinv[1] * L(1) + Iinv[2] * L(2), Iinv[3] * L(0) + Iinv[4] * L(1) + Iinv[5] * L(2),
+                   Iinv[6] * L(0) + Iinv[7] * L(1) + Iinv[8] * L(2));
+    for (int n = 0; n < s.nv; ++n) {
+      const Pt& r = s.p[n];
+      /* W r with W the cross-product matrix of omega (ShellSystem.h:420-428) */
+      const Pt vi(-omega(2) * r(1) + omega(1) * r(2), omega(2) * r(0) - omega(0) * r(2),
+                  -omega(1) * r(0) + omega(0) * r(1));
+      for (int q = 0; q < 3; ++q) {
+        s.f[3 * n + q] -= P.damping_rotation * vi(q);
+        s.fext[3 * n + q] -= P.damping_rotation * vi(q);
+      }
+    }
+  }
+}
+
 void elem_points(const System& s, int e, Pt out[3], bool on_mesh)
 {
   for (int k = 0; k < 3; ++k) out[k] = on_mesh ? s.X[s.tri[3 * e + k]] : s.p[s.tri[3 * e + k]];
@@ -989,11 +1168,16 @@ void evaluate_forces(System& s, double t)
   s.fext.assign(3 * s.nv, 0.0);
   s.Etens = s.Ebend = s.Eself = s.Efric = 0;
   s.Nc = 0;
+  s.Nc_cav = 0;
+  s.Ecav = s.cavity_force = 0;
+  s.ecav_node.assign(s.nv, 0.0);
+  s.efric_node.assign(s.nv, 0.0);
   const bool omp = s.prm.omp_faithful != 0;
 #pragma omp parallel for if (omp)
   for (int e = 0; e < s.ne; ++e) deformed_metrics_elem(s, e, s.u_pred.data());
   if (s.prm.volume_at_prediction) compute_enclosed_volume(s); /* positions s.p are those of u_pred */
   pressure_force(s, t);
+  body_loads(s, t);
   if (s.prm.contact) contact_force(s);
   double Et_sum = 0, Eb_sum = 0;
   if (omp) {
@@ -1123,10 +1307,13 @@ void add_sample(System& s, double dt, double volume)
 {
   double kG, kM, area;
   measure_curvature(s, kG, kM, area);
-  const double smp[11] = {1.0, area, volume, (double)s.Nc, kinetic_energy(s), s.Etens, s.Ebend, s.Eself, s.Efric,
-                          kG, kM};
-  for (int q = 0; q < 11; ++q) s.avg[q] += dt * smp[q];
+  const double pcav = s.prm.cavity_contact ? s.cavity_force / s.cavity_surface : 0.0;
+  const double smp[14] = {1.0, area, volume, (double)s.Nc, kinetic_energy(s), s.Etens, s.Ebend, s.Eself, s.Efric,
+                          kG, kM, pcav, (double)s.Nc_cav, s.Ecav};
+  for (int q = 0; q < 14; ++q) s.avg[q] += dt * smp[q];
   s.avg_n++;
+  /* the current packing density (ShellSystem.C:2822-2824) */
+  if (s.prm.cavity_contact) s.density = area * s.prm.thickness / s.cavity_volume;
 }
 
 /* ShellSystem::advance, ShellSystem.C:952-1122: step attempts repeat until
@@ -1173,7 +1360,7 @@ void advance(System& s, orc_report* rep)
       else s.reg(NOT_FINITE);
       if (rel == 0) new_dt = 2 * s.dt;
       else if (rel > P.rel_loc_err_max || rel < P.rel_loc_err_min) new_dt = s.dt * std::pow(target / rel, 1. / 3);
-      if (P.contact) {
+      if (P.cavity_contact || P.contact) {
         const unsigned contact_resolution = 10;
         double vmax = 0;
         for (int i = 0; i < n3; ++i) vmax = std::max(vmax, std::fabs(s.v_pred[i]));
@@ -1221,6 +1408,8 @@ void advance(System& s, orc_report* rep)
     rep->problem = s.problem;
     rep->n_rejected = s.n_rejected;
     rep->dt = s.dt;
+    rep->e_cav = s.Ecav;
+    rep->n_cav = s.Nc_cav;
   }
 }
 
@@ -1264,6 +1453,10 @@ void* orc_create(int nv, int ne, const double* X, const int32_t* tri, const orc_
   s->u.assign(n3, 0); s->v.assign(n3, 0); s->a.assign(n3, 0); s->f.assign(n3, 0); s->fext.assign(n3, 0);
   s->u_pred.assign(n3, 0); s->v_pred.assign(n3, 0); s->a_pred.assign(n3, 0);
   compute_position(*s, s->u);
+  s->cav_origin.assign(nv, Pt());
+  s->ecav_node.assign(nv, 0.0);
+  s->efric_node.assign(nv, 0.0);
+  if (prm->cavity_contact) update_cavity(*s, 0.0);
   if (prm->contact) build_neighborhood(*s);
   return s;
 }
@@ -1425,11 +1618,22 @@ int orc_properties(void* h, double* out)
   auto mavg = [&](double sum) { return n == 0 ? 0.0 : sum / n; };
   const double adt = mavg(s.avg[0]);
   auto a = [&](int q) { return mavg(s.avg[q]) / adt; };
-  const double z = mavg(0.0) / adt;
-  const double vals[16] = {adt, s.t, a(1), a(2), 0.0, z, z, a(3), a(4), a(5), a(6), z, a(7), a(8), a(9), a(10)};
+  const double vals[16] = {adt, s.t, a(1), a(2), s.density, a(11), a(12), a(3), a(4), a(5), a(6), a(13), a(7), a(8),
+                           a(9), a(10)};
   for (int q = 0; q < 16; ++q) out[q] = vals[q];
   for (double& x : s.avg) x = 0;
   s.avg_n = 0;
+  return 0;
+}
+
+int orc_node_cavity(void* h, double* origin, double* ecav, double* efric)
+{
+  System& s = *static_cast<System*>(h);
+  for (int n = 0; n < s.nv; ++n) {
+    if (origin) for (int q = 0; q < 3; ++q) origin[3 * n + q] = s.cav_origin[n](q);
+    if (ecav) ecav[n] = s.ecav_node[n];
+    if (efric) efric[n] = s.efric_node[n];
+  }
   return 0;
 }
 

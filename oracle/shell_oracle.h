@@ -50,6 +50,16 @@ typedef struct {
                                       instead of the previous step's (ShellSystem.C:1097, 2505) */
   int32_t rebuild_trigger;         /* ext: 0: du_sum >= tolerance (ShellSystem.C:1917-1919); 1: 2 x the
                                       largest node displacement since the list's build >= tolerance */
+  /* the loads of compute_external_force beyond the pressure (ShellSystem.C:1587-1786) */
+  int32_t cavity_contact;          /* shell_cavity_contact */
+  int32_t cavity;                  /* enum Cavity (ShellSystem.h:43): 0 box, 1 cylinder, 2 ellipsoid */
+  int32_t gravity_dir;             /* coordinate of the gravity load */
+  int32_t pad0;
+  double cavity_R0[3];             /* Rx, Ry, Rz at t = 0 */
+  double cavity_rate;              /* R(t) = R(0) exp(rate t), the form of the reference's config.cfg
+                                      (Rx = 0.1*exp(-4*t)); 0: constant */
+  double mu_s_cavity, mu_d_cavity, cavity_damping, cavity_friction_damping;
+  double gravity, damping_rotation;
 } orc_params;
 
 typedef struct {
@@ -66,6 +76,9 @@ typedef struct {
   int32_t n_rejected;  /* rejected step attempts so far */
   double dt;           /* dt for the next step */
   double residual_deform; /* residual with the rigid-body inertial forces removed (least squares) */
+  double e_cav;        /* cavity contact energy */
+  int32_t n_cav;       /* nodes in cavity contact (depth > 0) */
+  int32_t pad;
 } orc_report;
 
 /* meshes */
@@ -102,6 +115,10 @@ int  orc_init_dynamics(void* h);       /* apply_initial_conditions without disto
 int  orc_step(void* h, int n_steps, orc_report* rep);
 int  orc_measure(void* h);
 int  orc_properties(void* h, double* out /* 16 */);
+/* per node, from the last force evaluation: the cavity spring origin (3),
+ * the cavity energy and friction energy over the node's mean element area
+ * (the .vtu fields ecav and efric, ShellSystem.C:1626-1706) */
+int  orc_node_cavity(void* h, double* origin /* nv*3 */, double* ecav /* nv */, double* efric /* nv */);
 int  orc_elem_measure(void* h, double* out /* ne*10: ff1, ff2, jac, kGauss, kMean, area */);
 int  orc_contact_rebuild(void* h, int force);
 int  orc_pairs(void* h, uint32_t* e1, uint32_t* e2, int cap);   /* returns count, sorted */

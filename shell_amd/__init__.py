@@ -62,7 +62,11 @@ class Params(ctypes.Structure):
                                       "quirk_volume_schedule", "volume_at_prediction", "rebuild_trigger",
                                       "stop_on_problem")] + [
         ("load_end_time", ctypes.c_double), ("pressure_fn", ctypes.c_char * 160),
-        ("volume_growth_fn", ctypes.c_char * 160)]
+        ("volume_growth_fn", ctypes.c_char * 160), ("cavity_contact", ctypes.c_int32), ("cavity", ctypes.c_int32),
+        ("cavity_R", ctypes.c_double * 3)] + [(n, ctypes.c_double) for n in (
+        "mu_s_cavity", "mu_d_cavity", "cavity_damping", "cavity_friction_damping", "gravity",
+        "damping_rotation")] + [("gravity_dir", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+                                ("cavity_fn", (ctypes.c_char * 160) * 3)]
 
 
 SHAPE_FIELDS = ("elem_waves", "node_waves", "contact_waves", "bp_lanes", "exact_waves")
@@ -74,7 +78,8 @@ class Report(ctypes.Structure):
         "volume", "area", "du_max")] + [(n, ctypes.c_int32) for n in (
         "n_contacts", "n_pairs", "n_rebuilds", "problem", "n_steps", "n_rejected", "converged_step",
         "n_steps_fp32")] + [("residual_deform", ctypes.c_double)] + [
-        (n, ctypes.c_int32) for n in ("capacity_err", "n_kd")]
+        (n, ctypes.c_int32) for n in ("capacity_err", "n_kd")] + [
+        ("e_cav", ctypes.c_double), ("n_cav_contacts", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
 
 
 class Shapes(ctypes.Structure):
@@ -111,7 +116,10 @@ DEFAULTS = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=0.01, rho=1.0, memb_
                 newmark_beta=0.0, newmark_gamma=0.5, dt=1e-3, dt_min=0.0, dt_max=0.0,
                 rel_loc_err_min=5e-5, rel_loc_err_max=5e-4, adapt_dt_every=0, contact=0,
                 quirk_fulltime=0, quirk_volume_schedule=0, volume_at_prediction=0,
-                rebuild_trigger=0, stop_on_problem=0, load_end_time=0.0, pressure_fn="", volume_growth_fn="")
+                rebuild_trigger=0, stop_on_problem=0, load_end_time=0.0, pressure_fn="", volume_growth_fn="",
+                cavity_contact=0, cavity=2, cavity_R=(1.0, 1.0, 1.0), mu_s_cavity=0.0, mu_d_cavity=0.0,
+                cavity_damping=0.0, cavity_friction_damping=0.0, gravity=0.0, damping_rotation=0.0,
+                gravity_dir=2, reserved1=0, cavity_fn=("", "", ""))
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -169,6 +177,7 @@ def lib():
         L.shx_batch_measure.argtypes = [_V]
         L.shx_batch_properties.argtypes = [_V, ctypes.c_int, _D]
         L.shx_batch_fields.argtypes = [_V, ctypes.c_int, _D, _D]
+        L.shx_batch_cavity_springs.argtypes = [_V, ctypes.c_int, _D, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -192,6 +201,16 @@ def make_params(**kw):
     d = dict(DEFAULTS)
     d.update(kw)
     for k, v in d.items():
+        if k == "cavity_R":
+            v = (ctypes.c_double * 3)(*v)
+        elif k == "cavity_fn":
+            arr = ((ctypes.c_char * 160) * 3)()
+            for q, e in enumerate(v):
+                e = e.encode() if isinstance(e, str) else e
+                if len(e) >= 160:
+                    raise ValueError("cavity_fn expression too long")
+                arr[q].value = e
+            v = arr
         setattr(p, k, v.encode() if isinstance(v, str) else v)
     return p
 
@@ -235,6 +254,37 @@ class _Measurements:
         ef = np.zeros((ne, 4))
         self._check(lib().shx_batch_fields(self.h, shell, _dp(nf), _dp(ef)), "shx_batch_fields")
         return nf, ef
+
+    def save_state(self, path, time_step, shell=0, wall_time=0.0):
+        """the shell's state file (ShellSystem::save, ShellSystem.C:2956-3049)"""
+        L = lib()
+        _declare_mesh(L)
+        r = L.shx_state_save(self.h, shell, str(path).encode(), time_step, wall_time)
+        if r != 0:
+            raise ShellError(f"shx_state_save: {L.shx_io_last_error().decode()}", r)
+
+    def load_state(self, path, shell=0, precision=FP64):
+        """load a state file into the shell and initialise it (ShellSystem::
+        load, then apply_initial_conditions); returns (time step, wall time)"""
+        L = lib()
+        _declare_mesh(L)
+        step = ctypes.c_uint32()
+        wall = ctypes.c_double()
+        r = L.shx_state_load(self.h, shell, str(path).encode(), precision, ctypes.byref(step), ctypes.byref(wall))
+        if r != 0:
+            raise ShellError(f"shx_state_load: {L.shx_io_last_error().decode()}", r)
+        return step.value, wall.value
+
+    def cavity_springs(self, shell=0, origin=None):
+        """the nodes' cavity friction spring origins (nv x 3); with origin
+        given, sets them instead (they survive the shell's next init)"""
+        nv, _ = self._sizes(shell)
+        if origin is None:
+            o = np.zeros((nv, 3))
+            self._check(lib().shx_batch_cavity_springs(self.h, shell, _dp(o), 0), "shx_batch_cavity_springs")
+            return o
+        o = np.ascontiguousarray(origin, dtype=np.float64).reshape(nv, 3)
+        self._check(lib().shx_batch_cavity_springs(self.h, shell, _dp(o), 1), "shx_batch_cavity_springs")
 
 
 class ShellSystem(_Measurements, _Shaped):
@@ -538,6 +588,10 @@ def params_dict(p):
     out = {}
     for name, _ in Params._fields_:
         v = getattr(p, name)
+        if name == "cavity_R":
+            v = tuple(v)
+        elif name == "cavity_fn":
+            v = tuple(x.value.decode() for x in v)
         out[name] = v.decode() if isinstance(v, bytes) else v
     return out
 

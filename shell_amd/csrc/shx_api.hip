@@ -43,6 +43,8 @@ struct shx_ctx {
   int nblkE = 0, nblkN = 0, ntile = 0;
   bool any_contact = false;
   bool any_kd = false;
+  bool any_rot = false;        /* some shell has rotation damping: k_rot precedes the element epilogue */
+  std::vector<char> keep_cso;  /* per shell: cavity springs were set by the caller (a loaded state) */
   shx_shapes shapes = {4, 6, 3, 16, 2, {0, 0, 0}};   /* launch shapes (shx_set_shapes) */             /* some shell has kinetic damping on: k_kd follows k_fin_node */
   std::vector<void*> allocs;
   std::vector<double> rawb;
@@ -254,6 +256,7 @@ void launch_elem(shx_ctx* ctx, int mode)
   if (ctx->grid[4]) k_elem32<<<ctx->grid[4], kElemBlock, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
   prof_begin(ctx, 3);
+  if (ctx->any_rot && ctx->grid[3]) k_rot<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
   k_fin_elem<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
   prof_end(ctx);
 }
@@ -364,6 +367,9 @@ void fill_report(const DShell& s, shx_report* rep)
   rep->residual_deform = s.residual_deform;
   rep->capacity_err = s.err;
   rep->n_kd = s.n_kd;
+  rep->e_cav = s.e_cav;
+  rep->n_cav_contacts = s.n_cav;
+  rep->reserved2 = 0;
 }
 
 /* SConst of one shell from its parameters (update_derived,
@@ -407,6 +413,15 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
   C.volume_at_prediction = prm->volume_at_prediction ? 1 : 0;
   C.rebuild_trigger = prm->rebuild_trigger ? 1 : 0;
   C.stop_on_problem = prm->stop_on_problem ? 1 : 0;
+  C.cavity_contact = prm->cavity_contact ? 1 : 0;
+  C.cavity = prm->cavity;
+  C.cavity_friction = (prm->cavity_contact && (prm->mu_s_cavity != 0 || prm->mu_d_cavity != 0)) ? 1 : 0;
+  for (int k = 0; k < 3; ++k) C.cav_R[k] = prm->cavity_R[k];
+  C.mu_s_cav = prm->mu_s_cavity; C.mu_d_cav = prm->mu_d_cavity;
+  C.cav_damping = prm->cavity_damping; C.cav_fric_damping = prm->cavity_friction_damping;
+  C.static_friction_tol = prm->static_friction_tol;
+  C.gravity = prm->gravity; C.gravity_dir = prm->gravity_dir;
+  C.damping_rotation = prm->damping_rotation;
   {
     unsigned m = (unsigned)std::ceil(std::sqrt((double)ne) / prm->cell_size_mult);
     C.n_cells_max = m * m * m;
@@ -438,7 +453,11 @@ bool params_ok(const shx_params* prm)
 {
   return !(prm->thickness <= 0 || prm->E1 <= 0 || prm->E2 <= 0 || prm->G12 <= 0 || prm->rho <= 0 ||
            prm->nu < 0 || prm->nu >= 1 || prm->dt <= 0 || prm->newmark_beta < 0 ||
-           (prm->contact && prm->cell_size_mult <= 0));
+           (prm->contact && prm->cell_size_mult <= 0) || prm->gravity_dir < 0 || prm->gravity_dir > 2 ||
+           !(prm->damping_rotation >= 0) ||
+           (prm->cavity_contact && (prm->cavity < 0 || prm->cavity > 2 || !(prm->mu_s_cavity >= 0) ||
+                                    !(prm->mu_d_cavity >= 0) || !(prm->cavity_damping >= 0) ||
+                                    !(prm->cavity_friction_damping >= 0) || !(prm->static_friction_tol >= 0))));
 }
 
 int check_shell(shx_ctx* ctx, int shell)
@@ -509,7 +528,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   if (!out || nshell <= 0 || !nv || !ne || !X || !tri || !prm) { g_create_err = "invalid argument"; return SHX_E_ARG; }
   if (nshell > kMaxShells) { g_create_err = "more than " + std::to_string(kMaxShells) + " shells"; return SHX_E_ARG; }
   *out = nullptr;
-  std::vector<ExprProg> pprog(nshell), vprog(nshell);
+  std::vector<ExprProg> pprog(nshell), vprog(nshell), cprog((size_t)3 * nshell);
   for (int s = 0; s < nshell; ++s) {
     if (nv[s] <= 0 || ne[s] <= 0) { g_create_err = "invalid mesh size of shell " + std::to_string(s); return SHX_E_ARG; }
     if (!params_ok(&prm[s])) {
@@ -531,6 +550,29 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
         return SHX_E_ARG;
       }
     }
+    /* the cavity's semi-axes Rx, Ry, Rz (expressions in t, or constants) */
+    for (int k = 0; k < 3; ++k) {
+      ExprProg& cp = cprog[(size_t)3 * s + k];
+      std::memset(&cp, 0, sizeof(ExprProg));
+      const char* fn = prm[s].cavity_fn[k];
+      if (strnlen(fn, sizeof(prm[s].cavity_fn[k])) == sizeof(prm[s].cavity_fn[k])) {
+        g_create_err = "shell " + std::to_string(s) + ": cavity function not NUL-terminated";
+        return SHX_E_ARG;
+      }
+      char e[256];
+      if (fn[0] && !expr_compile(fn, cp, e, sizeof(e))) {
+        g_create_err = "shell " + std::to_string(s) + ": R" + "xyz"[k] + ": " + e;
+        return SHX_E_ARG;
+      }
+      if (prm[s].cavity_contact) {
+        const double R = cp.n ? expr_eval(cp, 0.0) : prm[s].cavity_R[k];
+        if (!(R > 0.5 * prm[s].thickness)) {
+          g_create_err = "shell " + std::to_string(s) + ": cavity semi-axis R" + "xyz"[k] +
+                         " must exceed half the thickness (ShellSystem.C:424-435)";
+          return SHX_E_ARG;
+        }
+      }
+    }
   }
   shx_ctx* ctx = new shx_ctx();
   ctx->dev = device;
@@ -538,6 +580,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   ctx->fp32_best.assign(nshell, HUGE_VAL);
   ctx->fp32_best_step.assign(nshell, 0);
   ctx->keep_rows.assign(nshell, 0);
+  ctx->keep_cso.assign(nshell, 0);
   ctx->Xin.resize(nshell);
   ctx->prm.assign(prm, prm + nshell);
   auto fail = [&](int code) { g_create_err = ctx->err; shx_destroy(ctx); return code; };
@@ -565,6 +608,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     C.fx_shift = contact_fx_shift(&prm[s], C, ne[s], X + xoff[s], tri + toff[s]);
     C.pressure_prog = pprog[s];
     C.volume_prog = vprog[s];
+    for (int k = 0; k < 3; ++k) C.cav_prog[k] = cprog[(size_t)3 * s + k];
+    if (C.damping_rotation > 0) ctx->any_rot = true;
     const int nvp = round_up(nv[s], kAlign), nep = round_up(ne[s], kAlign);
     C.vbase = (int)vb; C.ebase = (int)eb;
     C.blkE = (int)(eb / kElemPerBlock); C.nbE = nep / kElemPerBlock;
@@ -644,6 +689,21 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     for (int q = nv[s]; q < C.nbN * kNodePerBlock; ++q) inc_start[C.vbase + q] = (int)inc.size();
   }
   inc_start[nV] = (int)inc.size();
+  /* vertex incidence (nodes_to_elem_map, ascending element ids): the mean
+   * element area of a node for the cavity penalty (avg_area_node) */
+  std::vector<int32_t> vinc_start((size_t)nV + 1, 0), vinc;
+  for (int s = 0; s < nshell; ++s) {
+    const SConst& C = ctx->C[s];
+    const Topology& T = ctx->T[s];
+    std::vector<std::vector<int32_t>> ve(nv[s]);
+    for (int e = 0; e < ne[s]; ++e)
+      for (int k = 0; k < 3; ++k) ve[T.tri[(size_t)3 * e + k]].push_back(C.ebase + e);
+    for (int q = 0; q < C.nbN * kNodePerBlock; ++q) {
+      vinc_start[C.vbase + q] = (int)vinc.size();
+      if (q < nv[s]) vinc.insert(vinc.end(), ve[q].begin(), ve[q].end());
+    }
+  }
+  vinc_start[nV] = (int)vinc.size();
   /* element blocks: their patch nodes (in descending entry count), each patch
    * slot's index into them, per block node its (element, slot) entries, and
    * per node the block records that hold its partial forces */
@@ -743,7 +803,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   int32_t *dbs, *dbi, *drs, *dns, *dnr;
   uint16_t *dli, *dre;
   uint32_t* drg;
-  int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci;
+  int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci, *dvis, *dvi;
   int8_t* dv;
   if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
       (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, inc_start)) ||
@@ -751,8 +811,9 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)) || (r = upload(ctx, dbs, bn_start)) ||
       (r = upload(ctx, dbi, bn_ids)) || (r = upload(ctx, dli, lidx)) || (r = upload(ctx, drs, rent_start)) ||
       (r = upload(ctx, dre, rent)) || (r = upload(ctx, dns, nrec_start)) || (r = upload(ctx, dnr, nrec)) ||
-      (r = upload(ctx, drg, rent_gcnt)))
+      (r = upload(ctx, drg, rent_gcnt)) || (r = upload(ctx, dvis, vinc_start)) || (r = upload(ctx, dvi, vinc)))
     return fail(r);
+  P.vinc_start = dvis; P.vinc = dvi;
   P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
   P.bn_start = dbs; P.bn_ids = dbi; P.lidx = dli; P.rent_start = drs; P.rent = dre; P.rent_gcnt = drg; P.nrec_start = dns; P.nrec = dnr;
   P.eshell = des; P.vshell = dvs; P.close_start = dcs; P.close_ids = dci;
@@ -767,6 +828,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)kElemRec * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
+      (r = dalloc(ctx, P.cvpart, (size_t)kCvRec * ctx->nblkN)) || (r = dalloc(ctx, P.rpart, (size_t)kRotRec * ctx->nblkN)) ||
+      (r = dalloc(ctx, P.cso, (size_t)3 * nV)) || (r = dalloc(ctx, P.ncav, (size_t)2 * nV)) ||
       (r = dalloc(ctx, P.wl[0], (size_t)nshell + 1)) || (r = dalloc(ctx, P.wl[1], (size_t)nshell + 1)) ||
       (r = dalloc(ctx, ctx->bmap_d[0], (size_t)ctx->ntile)) || (r = dalloc(ctx, ctx->bmap_d[2], (size_t)ctx->nblkE)) ||
       (r = dalloc(ctx, ctx->bmap_d[3], (size_t)ctx->nblkN)) || (r = dalloc(ctx, ctx->bmap_d[4], (size_t)ctx->nblkE)))
@@ -922,6 +985,10 @@ int shx_batch_init_shells(shx_ctx* ctx, int precision, int n, const int* shells,
       for (int b = 0; b < 2; ++b)
         HIPCHK(hipMemsetAsync(ctx->P.rcnt[b] + ctx->C[i].ebase, 0, sizeof(int32_t) * ctx->C[i].ne, ctx->st));
     ctx->keep_rows[i] = 0;
+    /* nor cavity springs, unless they were set (a loaded state) */
+    if (ctx->C[i].cavity_contact && !ctx->keep_cso[i])
+      HIPCHK(hipMemsetAsync(ctx->P.cso + (size_t)3 * ctx->C[i].vbase, 0, sizeof(double) * 3 * ctx->C[i].nv, ctx->st));
+    ctx->keep_cso[i] = 0;
   }
   /* the shells not listed keep their state and progress: they sit out the
    * init kernels as done shells; shx_batch_step recomputes done for all */
@@ -1143,11 +1210,14 @@ int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out
   HIPCHK(hipStreamSynchronize(ctx->st));
   const SConst& C = ctx->C[shell];
   const int nv = C.nv, ne = C.ne, par = s0[shell].par;
-  std::vector<double> area(ne), em((size_t)4 * ne), nm((size_t)2 * nv, 0.0), v((size_t)3 * nv), minv(nv);
+  std::vector<double> area(ne), em((size_t)4 * ne), nm((size_t)2 * nv, 0.0), nc((size_t)2 * nv, 0.0),
+      v((size_t)3 * nv), minv(nv);
   HIPCHK(hipMemcpy(area.data(), ctx->P.area + C.ebase, sizeof(double) * ne, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(em.data(), ctx->P.emeas + (size_t)4 * C.ebase, sizeof(double) * 4 * ne, hipMemcpyDeviceToHost));
   if (C.contact)
     HIPCHK(hipMemcpy(nm.data(), ctx->P.nmeas + (size_t)2 * C.vbase, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+  if (C.cavity_contact)
+    HIPCHK(hipMemcpy(nc.data(), ctx->P.ncav + (size_t)2 * C.vbase, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(v.data(), ctx->P.V[par] + (size_t)3 * C.vbase, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(minv.data(), ctx->P.minv + C.vbase, sizeof(double) * nv, hipMemcpyDeviceToHost));
   /* the evaluation leaves the shells' state as it was */
@@ -1166,8 +1236,8 @@ int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out
       for (int q = 0; q < 3; ++q) ek += v[3 * n + q] * v[3 * n + q] / minv[n];
       ek *= 0.5;
       double* o = node_out + 5 * (size_t)n;
-      o[0] = 0.0;                                      /* ecav */
-      o[1] = nm[2 * n + 1];                            /* efric */
+      o[0] = nc[2 * n];                                /* ecav */
+      o[1] = nm[2 * n + 1] + nc[2 * n + 1];            /* efric: self-contact and cavity friction */
       o[2] = ek / (asum[n] / acnt[n]);                 /* ekin */
       o[3] = nm[2 * n];                                /* eself */
       o[4] = 1 / minv[n];                              /* mass */
@@ -1213,6 +1283,21 @@ int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, do
     }
   }
   return w;
+}
+
+int shx_batch_cavity_springs(shx_ctx* ctx, int shell, double* origin, int io_set)
+{
+  if (!ctx || !origin) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const SConst& C = ctx->C[shell];
+  const size_t bytes = sizeof(double) * 3 * C.nv;
+  if (io_set) {
+    HIPCHK(hipMemcpy(ctx->P.cso + (size_t)3 * C.vbase, origin, bytes, hipMemcpyHostToDevice));
+    ctx->keep_cso[shell] = 1;
+  } else {
+    HIPCHK(hipMemcpy(origin, ctx->P.cso + (size_t)3 * C.vbase, bytes, hipMemcpyDeviceToHost));
+  }
+  return SHX_OK;
 }
 
 int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, const uint32_t* e2,
@@ -1272,12 +1357,12 @@ int shx_batch_averages(shx_ctx* ctx, int shell, double* sums, int* n, int set)
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
   if (!set) {
-    std::copy(s[shell].avg, s[shell].avg + 11, sums);
+    std::copy(s[shell].avg, s[shell].avg + 14, sums);
     *n = s[shell].avg_n;
     return SHX_OK;
   }
   if (*n < 0) return SHX_E_ARG;
-  std::copy(sums, sums + 11, s[shell].avg);
+  std::copy(sums, sums + 14, s[shell].avg);
   s[shell].avg_n = *n;
   return write_sh(ctx, s);
 }
@@ -1317,13 +1402,16 @@ int shx_batch_measure(shx_ctx* ctx)
   if (!ctx) return SHX_E_ARG;
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
-  for (auto& w : s) {
+  for (int i = 0; i < ctx->S; ++i) {
+    DShell& w = s[i];
     /* measure_properties (ShellSystem.C:2714-2846) of the current
      * configuration, weighted with the current dt */
-    const double smp[11] = {1.0, w.area, w.volume, (double)w.n_contacts, w.e_kin, w.e_tens, w.e_bend,
-                            w.e_self, w.e_fric, w.kgauss, w.kmean};
-    for (int q = 0; q < 11; ++q) w.avg[q] += w.dt * smp[q];
+    const double pcav = w.cav_surface > 0 ? w.fcav / w.cav_surface : 0.0;
+    const double smp[14] = {1.0, w.area, w.volume, (double)w.n_contacts, w.e_kin, w.e_tens, w.e_bend,
+                            w.e_self, w.e_fric, w.kgauss, w.kmean, pcav, (double)w.n_cav, w.e_cav};
+    for (int q = 0; q < 14; ++q) w.avg[q] += w.dt * smp[q];
     w.avg_n++;
+    if (ctx->C[i].cavity_contact) w.density = w.area * ctx->C[i].h / w.cav_volume;
   }
   return write_sh(ctx, s);
 }
@@ -1340,11 +1428,10 @@ int shx_batch_properties(shx_ctx* ctx, int shell, double* out)
   auto avg = [&](int q) { return moving_average(w.avg[q], n) / adt; };
   /* print_properties (ShellSystem.C:2849-2898): dt, time, area, volume,
    * density, cavity pressure, Nc_cav, Nc_self, E_kin, E_tens, E_bend,
-   * E_cav, E_self, E_fric, kGauss, kMean; the cavity columns are 0 on
-   * this path (their samples are 0, density is never set) */
-  const double zero_avg = moving_average(0.0, n) / adt;
-  const double vals[16] = {adt, w.t, avg(1), avg(2), 0.0, zero_avg, zero_avg, avg(3), avg(4), avg(5), avg(6),
-                           zero_avg, avg(7), avg(8), avg(9), avg(10)};
+   * E_cav, E_self, E_fric, kGauss, kMean; density is the last sample's
+   * (it is not averaged), 0 without cavity contact */
+  const double vals[16] = {adt, w.t, avg(1), avg(2), w.density, avg(11), avg(12), avg(3), avg(4), avg(5), avg(6),
+                           avg(13), avg(7), avg(8), avg(9), avg(10)};
   std::copy(vals, vals + 16, out);
   for (double& a : w.avg) a = 0;
   w.avg_n = 0;

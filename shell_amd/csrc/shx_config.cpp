@@ -8,8 +8,8 @@
  * expressions, Function.h) compiled by expr_compile below.
  *
  * Features the path does not contain are refused when a config turns them
- * on (cavity contact, growth, gravity, perturbation, rotation damping,
- * smooth contact, a non-local pressure base, more than one quadrature
+ * on (growth, perturbation, smooth contact, a non-local pressure base, more
+ * than one quadrature
  * point, stand_still); keys that only matter to them are read and ignored.
  */
 #include <algorithm>
@@ -359,8 +359,8 @@ struct Reader {
     std::string expr;
     bool c;
     if (!function(name, def, expr, value, c)) return false;
-    if (!c) return fail(name + " = '" + expr + "' depends on t: only pressure and volume_growth may vary in time "
-                        "on this path");
+    if (!c) return fail(name + " = '" + expr + "' depends on t: only pressure, volume_growth and the cavity's "
+                        "Rx, Ry, Rz may vary in time on this path");
     return true;
   }
   /* a [function] key of a feature the path does not contain: it must keep
@@ -471,9 +471,23 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
   bool cavity_contact = true;
   if (ok) {
     const std::string cav = R.upper("cavity", "ELLIPSOID");
-    if (cav != "BOX" && cav != "CYLINDER" && cav != "ELLIPSOID") ok = R.fail("cavity = " + cav + " (ASSERT(cavity != INVALID_CAV))");
+    if (cav == "BOX") p.cavity = 0;
+    else if (cav == "CYLINDER") p.cavity = 1;
+    else if (cav == "ELLIPSOID") p.cavity = 2;
+    else ok = R.fail("cavity = " + cav + " (ASSERT(cavity != INVALID_CAV))");
   }
-  for (const char* k : {"Rx", "Ry", "Rz"}) ok = ok && R.function(k, "0.1005", expr, v, c);
+  {
+    const char* axes[3] = {"Rx", "Ry", "Rz"};
+    for (int k = 0; k < 3 && ok; ++k) {
+      ok = R.function(axes[k], "0.1005", expr, v, c);
+      if (!ok) break;
+      p.cavity_R[k] = v;
+      if (!c) {
+        if (expr.size() >= sizeof(p.cavity_fn[k])) ok = R.fail(std::string(axes[k]) + " expression too long");
+        else put(p.cavity_fn[k], sizeof(p.cavity_fn[k]), expr);
+      }
+    }
+  }
   if (ok) {
     const std::string bc = R.upper("shell_boundary", "FREE");
     static const std::set<std::string> bcs = {"CLAMPED", "CLAMPED_WEAK", "PINNED", "PINNED_WEAK", "FREE", "FREE_WEAK"};
@@ -488,11 +502,13 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
   ok = ok && R.constant_function("damping_alpha", "0", p.damping_alpha);
   ok = ok && R.constant_function("damping_beta", "0", p.damping_beta);
   ok = ok && R.constant_function("damping_gamma", "0", p.damping_gamma);
-  ok = ok && R.off_function("damping_rotation", "0", 0, "rotation damping");
+  ok = ok && R.constant_function("damping_rotation", "0", p.damping_rotation);
   ok = ok && R.off_function("perturbation_rate", "0", 0, "thermal perturbation");
   ok = ok && R.off_function("perturbation_energy", "0", 0, "thermal perturbation");
-  ok = ok && R.off_function("gravity", "0", 0, "gravity");
+  ok = ok && R.constant_function("gravity", "0", p.gravity);
   ok = ok && R.integer("gravity_dir", 2, iv);
+  if (ok && (iv < 0 || iv > 2)) ok = R.fail("ASSERT(gravity_dir < 3)");
+  if (ok) p.gravity_dir = (int32_t)iv;
   if (ok) {
     ok = R.function("pressure", "0", expr, v, c);
     if (ok) {
@@ -539,12 +555,13 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
     else ok = R.fail("shell_shell_contact = " + ct + " (ASSERT(shell_shell_contact != INVALID_CT))");
   }
   ok = ok && R.boolean("shell_cavity_contact", true, cavity_contact);
-  if (ok && cavity_contact)
-    ok = R.fail("shell_cavity_contact = true: cavity contact is not part of this path (set it false)");
+  if (ok) p.cavity_contact = cavity_contact ? 1 : 0;
   ok = ok && R.constant_function("mu_s", "0", p.mu_s);
   ok = ok && R.constant_function("mu_d", "0", p.mu_d);
-  for (const char* k : {"mu_s_cavity", "mu_d_cavity", "cavity_damping", "cavity_friction_damping"})
-    ok = ok && R.function(k, "0", expr, v, c);
+  ok = ok && R.constant_function("mu_s_cavity", "0", p.mu_s_cavity);
+  ok = ok && R.constant_function("mu_d_cavity", "0", p.mu_d_cavity);
+  ok = ok && R.constant_function("cavity_damping", "0", p.cavity_damping);
+  ok = ok && R.constant_function("cavity_friction_damping", "0", p.cavity_friction_damping);
   ok = ok && R.constant_function("self_damping", "0", p.self_damping);
   ok = ok && R.constant_function("self_friction_damping", "0", p.self_friction_damping);
   ok = ok && R.constant_function("fric_stiff_mult", "1", p.fric_stiff_mult);
@@ -599,6 +616,10 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
     else if (p.contact && !(p.mu_s >= 0 && p.mu_d >= 0 && p.self_damping >= 0 && p.self_friction_damping >= 0 &&
                             p.cell_size_mult > 0 && p.cell_margin >= 0 && p.neighbor_margin > 0))
       ok = R.fail("a self-contact ASSERT of ShellSystem.C:381-390 failed");
+    else if (!(p.damping_rotation >= 0)) ok = R.fail("ASSERT(damping_rotation >= 0)");
+    else if (p.cavity_contact && !(p.mu_s_cavity >= 0 && p.mu_d_cavity >= 0 && p.cavity_damping >= 0 &&
+                                   p.cavity_friction_damping >= 0 && p.static_friction_tol >= 0))
+      ok = R.fail("a cavity-contact ASSERT of ShellSystem.C:373-380 failed");
   }
   /* the driver's keys (shell.C:114-124, 147-230) */
   ok = ok && R.integer("time_steps", 0, iv);

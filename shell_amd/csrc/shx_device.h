@@ -84,6 +84,9 @@ constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and r
 constexpr int kCloseLds = 32;       /* close-set entries of an element staged in LDS by the pair search */
 constexpr int kRowCap = 64;         /* contact candidates per element row */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
+constexpr int kCvRec = 4;           /* cavity partial record: nodes in contact, E_cav, sum of normal forces,
+                                       cavity friction energy */
+constexpr int kRotRec = 9;          /* rotation-damping partial record: L, then I xx yy zz xy xz yz */
 constexpr int kMaxShells = 2048;    /* shells per batch (work maps of the list-driven kernels are staged in LDS) */
 constexpr int kBpGrid = 1024;       /* blocks of the broad-phase kernels (they loop over the work) */
 
@@ -117,6 +120,13 @@ struct SConst {
   unsigned n_cells_max;
   ExprProg pressure_prog;          /* pressure(t) of the config (n == 0: the linear ramp) */
   ExprProg volume_prog;            /* volume_growth(t), the target ratio (n == 0: the linear ramp) */
+  /* the loads of compute_external_force after the pressure (ShellSystem.C:1587-1786) */
+  int cavity_contact, cavity, cavity_friction;   /* cavity: 0 box, 1 cylinder (axis x), 2 ellipsoid */
+  int gravity_dir;
+  double cav_R[3];                 /* constant semi-axes (when the program of that axis is empty) */
+  double mu_s_cav, mu_d_cav, cav_damping, cav_fric_damping, static_friction_tol;
+  double gravity, damping_rotation;
+  ExprProg cav_prog[3];            /* Rx(t), Ry(t), Rz(t) of the config (n == 0: the constant) */
 };
 
 /* per-shell mutable scalars (device array, written by the finalize and
@@ -145,9 +155,9 @@ struct DShell {
   double kgauss, kmean;            /* integrated Gaussian and mean curvature of the last accepted step's
                                       force-evaluation configuration (measure_properties) */
   double att_kgauss, att_kmean, att_narea;   /* att_narea: area of the evaluated configuration */
-  double avg[11];                  /* dt-weighted sums since the last properties read, in sample order
+  double avg[14];                  /* dt-weighted sums since the last properties read, in sample order
                                       (MovingAverage, ShellSystem.C:2830-2843): dt, area, volume, Nc, E_kin,
-                                      E_tens, E_bend, E_self, E_fric, kGauss, kMean */
+                                      E_tens, E_bend, E_self, E_fric, kGauss, kMean, p_cav, Nc_cav, E_cav */
   int avg_n;                       /* samples in avg */
   int avg_pad;
   double load_p, load_r;           /* pressure and target volume ratio at the force time of the current
@@ -161,6 +171,15 @@ struct DShell {
   int kd_on, kd_zero, n_kd;
   int parked;                      /* set by the host: the shell sits out the steps until it is initialised */
   double ek_prev;
+  /* cavity at the force time of the current attempt (update_derived, ShellSystem.C:421-470, evaluated
+   * by the element epilogue): effective semi-axes, their mean and inverse squares, the friction scope
+   * (1 - static_friction_tol h / mean)^2, the cavity's surface and volume */
+  double cav_Reff[3], cav_avgR, cav_inv2[3], cav_opdf, cav_surface, cav_volume;
+  double omega[3];                 /* angular velocity I^-1 L of the attempt (rotation damping) */
+  double e_cav, fcav, density;     /* cavity energy and sum of the nodes' normal forces of the last accepted
+                                      step (or init); packing density of the
+                                      last sample (area h / cavity volume, ShellSystem.C:2822-2824) */
+  int n_cav, cav_pad;
 };
 
 /* error bits in DShell::err */
@@ -221,6 +240,13 @@ struct DPtrs {
   double* epart;              /* element kernel block partials (kElemRec per block) */
   double* npart;              /* node kernel block partials (kNodeRec per block) */
   double* cpart;              /* contact kernel block partials (kCRec per 256-element tile) */
+  double* cvpart;             /* node kernel block partials of the cavity contact (kCvRec per block) */
+  double* rpart;              /* rotation-damping block partials: L (3), I (6 unique) per node block */
+  double* cso;                /* nV*3: cavity friction spring origin of each node (NodeCache::spring_origin) */
+  double* ncav;               /* 2*nV: per node E_cav and cavity E_fric over the node's mean element area,
+                                 plain force evaluations only */
+  const int32_t* vinc_start;  /* nV+1: elements that have the node as a vertex, ascending */
+  const int32_t* vinc;
   /* contact */
   const int32_t* close_start; /* nE+1 */
   const int32_t* close_ids;

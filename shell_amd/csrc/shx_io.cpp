@@ -13,8 +13,7 @@
  *  - the reference's load reads the Nc column twice (the Nc_cav column into
  *    avg_Nc, ShellSystem.C:3089-3090); this reader takes the columns as
  *    save writes them;
- *  - the per-node cavity spring origin is written as 0 (no cavity contact
- *    on this path) and growth data are absent (no growth).
+ *  - growth data are absent (no growth on this path).
  */
 #include <cmath>
 #include <cstdio>
@@ -51,9 +50,11 @@ int shx_state_save(shx_ctx* ctx, int shell, const char* path, uint32_t time_step
   if (int r = shx_batch_get_state(ctx, shell, u.data(), v.data(), a.data(), &t)) return io_fail(shx_last_error(ctx), r);
   std::vector<shx_report> reps(shx_batch_size(ctx));
   if (int r = shx_batch_reports(ctx, reps.data())) return io_fail(shx_last_error(ctx), r);
-  double sums[11];
+  double sums[14];
   int n = 0;
   if (int r = shx_batch_averages(ctx, shell, sums, &n, 0)) return io_fail(shx_last_error(ctx), r);
+  std::vector<double> cso(3 * (size_t)nv);
+  if (int r = shx_batch_cavity_springs(ctx, shell, cso.data(), 0)) return io_fail(shx_last_error(ctx), r);
   const int np = shx_batch_pair_state(ctx, shell, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
   if (np < 0) return io_fail(shx_last_error(ctx), np);
   std::vector<uint32_t> e1(np > 0 ? np : 1), e2(e1.size());
@@ -67,16 +68,17 @@ int shx_state_save(shx_ctx* ctx, int shell, const char* path, uint32_t time_step
   f << reps[shell].dt << "\n" << t << "\n" << wall_time << "\n";
   for (int i = 0; i < nv; ++i) {
     for (int c = 0; c < 3; ++c) f << u[3 * i + c] << "\t" << v[3 * i + c] << "\t" << a[3 * i + c] << "\t";
-    f << 0.0 << "\t" << 0.0 << "\t" << 0.0 << "\n";
+    f << cso[3 * i] << "\t" << cso[3 * i + 1] << "\t" << cso[3 * i + 2] << "\n";
   }
   /* columns dt, area, vol, pcav, Nc_cav, Nc, Ekin, Etens, Ebend, Ecav, Eself,
    * Efric, kGauss, kMean; each line the mean sample */
   f << n << "\n";
   for (int k = 0; k < n; ++k) {
-    const double m[11] = {sums[0] / n, sums[1] / n, sums[2] / n, sums[3] / n, sums[4] / n, sums[5] / n,
-                          sums[6] / n, sums[7] / n, sums[8] / n, sums[9] / n, sums[10] / n};
-    f << m[0] << "\t" << m[1] << "\t" << m[2] << "\t" << 0.0 << "\t" << 0.0 << "\t" << m[3] << "\t" << m[4] << "\t"
-      << m[5] << "\t" << m[6] << "\t" << 0.0 << "\t" << m[7] << "\t" << m[8] << "\t" << m[9] << "\t" << m[10] << "\n";
+    double m[14];
+    for (int q = 0; q < 14; ++q) m[q] = sums[q] / n;
+    f << m[0] << "\t" << m[1] << "\t" << m[2] << "\t" << m[11] << "\t" << m[12] << "\t" << m[3] << "\t" << m[4]
+      << "\t" << m[5] << "\t" << m[6] << "\t" << m[13] << "\t" << m[7] << "\t" << m[8] << "\t" << m[9] << "\t"
+      << m[10] << "\n";
   }
   int nstick = 0;
   for (int k = 0; k < np; ++k) nstick += st[k] != 0;
@@ -100,21 +102,20 @@ int shx_state_load(shx_ctx* ctx, int shell, const char* path, int precision, uin
   uint32_t step = 0;
   double dt = 0, t = 0, wall = 0;
   f >> step >> dt >> t >> wall;
-  std::vector<double> u(3 * (size_t)nv), v(u.size()), a(u.size());
+  std::vector<double> u(3 * (size_t)nv), v(u.size()), a(u.size()), cso(u.size());
   for (int i = 0; i < nv; ++i) {
     for (int c = 0; c < 3; ++c) f >> u[3 * i + c] >> v[3 * i + c] >> a[3 * i + c];
-    double so[3];
-    f >> so[0] >> so[1] >> so[2];
+    f >> cso[3 * i] >> cso[3 * i + 1] >> cso[3 * i + 2];
   }
   int n = 0;
   f >> n;
   if (!f || n < 0) return io_fail(std::string("malformed state file ") + path, SHX_E_ARG);
-  double sums[11] = {};
+  double sums[14] = {};
   for (int k = 0; k < n; ++k) {
     double c[14];
     for (double& x : c) f >> x;
-    const double smp[11] = {c[0], c[1], c[2], c[5], c[6], c[7], c[8], c[10], c[11], c[12], c[13]};
-    for (int q = 0; q < 11; ++q) sums[q] += smp[q];
+    const double smp[14] = {c[0], c[1], c[2], c[5], c[6], c[7], c[8], c[10], c[11], c[12], c[13], c[3], c[4], c[9]};
+    for (int q = 0; q < 14; ++q) sums[q] += smp[q];
   }
   int nc = 0;
   f >> nc;
@@ -134,6 +135,10 @@ int shx_state_load(shx_ctx* ctx, int shell, const char* path, int precision, uin
       return io_fail(shx_last_error(ctx), r);
   if (int r = shx_batch_init_shells(ctx, precision, 1, &shell, nullptr)) return io_fail(shx_last_error(ctx), r);
   if (int r = shx_batch_set_state(ctx, shell, u.data(), v.data(), a.data(), t)) return io_fail(shx_last_error(ctx), r);
+  /* the loaded cavity springs, as load leaves them (the initialisation's
+   * force evaluation does not run on a loaded state, ShellSystem.C:887-896) */
+  if (prm.cavity_contact)
+    if (int r = shx_batch_cavity_springs(ctx, shell, cso.data(), 1)) return io_fail(shx_last_error(ctx), r);
   if (int r = shx_batch_averages(ctx, shell, sums, &n, 1)) return io_fail(shx_last_error(ctx), r);
   if (time_step) *time_step = step;
   if (wall_time) *wall_time = wall;

@@ -34,6 +34,7 @@ __global__ void k_precompute32(DPtrs P);
 __global__ void k_elem32(DPtrs P, int mode);
 __global__ void k_fin_node(DPtrs P, int mode);
 __global__ void k_kd(DPtrs P);
+__global__ void k_rot(DPtrs P, int mode);
 __global__ void k_bp_box(DPtrs P, int posmode, int list);
 __global__ void k_bp_grid(DPtrs P, int list);
 __global__ void k_bp_count(DPtrs P, int posmode, int list);

@@ -52,6 +52,60 @@ __device__ double volume_ratio_at(const SConst& C, double t)
   return 1.0 - (1.0 - C.volume_final_ratio) * fmin(t / C.volume_ramp_time, 1.0);
 }
 
+/* the cavity part of update_derived (ShellSystem.C:421-470) at time t:
+ * effective semi-axes R - h/2, their mean and inverse squares, the friction
+ * scope (1 - static_friction_tol h / mean)^2, and the cavity's surface
+ * (Knud Thomsen's approximation for the ellipsoid; the cylinder's formula
+ * as the reference writes it, without the length factor of its side) and
+ * volume.  The semi-axes are the config's expressions in t or constants. */
+__device__ void cavity_at(const SConst& C, double t, DShell& w)
+{
+  double R[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) R[k] = C.cav_prog[k].n ? expr_eval(C.cav_prog[k], t) : C.cav_R[k];
+  const double hh = 0.5 * C.h;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w.cav_Reff[k] = R[k] - hh;
+  w.cav_avgR = (w.cav_Reff[0] + w.cav_Reff[1] + w.cav_Reff[2]) / 3;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w.cav_inv2[k] = 1 / (w.cav_Reff[k] * w.cav_Reff[k]);
+  const double q = 1 - C.static_friction_tol * C.h / w.cav_avgR;
+  w.cav_opdf = q * q;
+  const double pi = 3.14159265358979323846;
+  const double Rx = R[0], Ry = R[1], Rz = R[2];
+  if (C.cavity == 0) {
+    w.cav_surface = 8 * (Rx * Ry + Rx * Rz + Ry * Rz);
+    w.cav_volume = 8 * Rx * Ry * Rz;
+  } else if (C.cavity == 1) {
+    const double lambda = (Ry - Rz) / (Ry + Rz);
+    const double lambda32 = 3 * lambda * lambda;
+    w.cav_surface = 2 * pi * ((Ry + Rz) * (1 + lambda32 / (10 + sqrt(4 - lambda32))) + Ry * Rz);
+    w.cav_volume = 2 * pi * Rx * Ry * Rz;
+  } else {
+    w.cav_surface = 4 * pi * pow((pow(Rx * Ry, 1.6) + pow(Rx * Rz, 1.6) + pow(Ry * Rz, 1.6)) / 3, 0.625);
+    w.cav_volume = 4. / 3 * pi * Rx * Ry * Rz;
+  }
+}
+
+/* ShellSystem::inverse (ShellSystem.h:431-451) of a row-major 3x3 matrix */
+__device__ void inverse3_dev(const double* A, double* O)
+{
+  auto a = [&](int r, int c) { return A[r * 3 + c]; };
+  const double det00 = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
+  const double det01 = a(1, 2) * a(2, 0) - a(1, 0) * a(2, 2);
+  const double det02 = a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0);
+  const double det10 = a(0, 2) * a(2, 1) - a(0, 1) * a(2, 2);
+  const double det11 = a(0, 0) * a(2, 2) - a(0, 2) * a(2, 0);
+  const double det12 = a(0, 1) * a(2, 0) - a(0, 0) * a(2, 1);
+  const double det20 = a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1);
+  const double det21 = a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2);
+  const double det22 = a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0);
+  const double inv_det = 1 / (a(0, 0) * det00 + a(0, 1) * det01 + a(0, 2) * det02);
+  O[0] = inv_det * det00; O[1] = inv_det * det10; O[2] = inv_det * det20;
+  O[3] = inv_det * det01; O[4] = inv_det * det11; O[5] = inv_det * det21;
+  O[6] = inv_det * det02; O[7] = inv_det * det12; O[8] = inv_det * det22;
+}
+
 /* ------------------------------------------------------------ precompute */
 
 /* build_cache + reinit_mod + compute_reference_metrics for one element
@@ -321,12 +375,27 @@ __global__ void __launch_bounds__(64) k_fin_elem(DPtrs P, int mode)
   const int op[kElemRec] = {0, 0, 1, 0, 0, 1, 0, 0, 0};
   double r[kElemRec];
   wave_reduce_records<kElemRec>(P.epart + (size_t)C.blkE * kElemRec, C.nbE, kElemRec, op, r);
+  double rot[kRotRec];
+  if (C.damping_rotation > 0) {
+    const int oz[kRotRec] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    wave_reduce_records<kRotRec>(P.rpart + (size_t)C.blkN * kRotRec, C.nbN, kRotRec, oz, rot);
+  }
   if (threadIdx.x != 0) return;
   {
     /* the loads at the force time of this attempt (the node kernel's t_eval) */
     const double t_eval = (mode == M_FORCES) ? w.t_eval : w.t + w.dt;
     w.load_p = pressure_at(C, t_eval);
     w.load_r = volume_ratio_at(C, t_eval);
+    if (C.cavity_contact) cavity_at(C, t_eval, w);
+  }
+  if (C.damping_rotation > 0) {
+    /* the angular velocity I^-1 L about the origin (ShellSystem.C:1727-1770) */
+    const double I[9] = {rot[3], rot[6], rot[7], rot[6], rot[4], rot[8], rot[7], rot[8], rot[5]};
+    double Iinv[9];
+    inverse3_dev(I, Iinv);
+    w.omega[0] = Iinv[0] * rot[0] + Iinv[1] * rot[1] + Iinv[2] * rot[2];
+    w.omega[1] = Iinv[3] * rot[0] + Iinv[4] * rot[1] + Iinv[5] * rot[2];
+    w.omega[2] = Iinv[6] * rot[0] + Iinv[7] * rot[1] + Iinv[8] * rot[2];
   }
   w.att_e_tens = r[0];
   w.att_e_bend = r[1];
@@ -569,6 +638,118 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
 
 /* ---------------------------------------------------- node gather + step */
 
+/* avg_area_node (ShellSystem.C:2462-2481): mean deformed area of the
+ * elements that have node n as a vertex, in element order */
+__device__ __forceinline__ double mean_vertex_area(const DPtrs& P, int n)
+{
+  const int k0 = P.vinc_start[n], k1 = P.vinc_start[n + 1];
+  double a = 0;
+  for (int k = k0; k < k1; ++k) a += P.area[P.vinc[k]];
+  return a / (k1 - k0);
+}
+
+/* cavity contact of one node (ShellSystem.C:1601-1723): the isoparameter
+ * 1 + delta of the node's position x (ShellSystem.h:336-365); within the
+ * friction scope the penetration depth, penalty k depth with k =
+ * stiff_density times the node's mean element area, the outward normal of
+ * the box face, cylinder wall or cap, or ellipsoid, Friction::apply with the
+ * node's own spring (origin in P.cso, updated in place as the reference
+ * does, except in measurement evaluations), and the normal damping; outside
+ * the scope the spring is released.  cv accumulates the node's contact
+ * count, energy, normal force and friction energy. */
+__device__ v3 cavity_force(const DPtrs& P, const SConst& C, const DShell& sh, int n, v3 x, v3 v, int mode,
+                           double* cv)
+{
+  double xr = x.x / sh.cav_Reff[0], yr = x.y / sh.cav_Reff[1], zr = x.z / sh.cav_Reff[2];
+  xr = xr * xr; yr = yr * yr; zr = zr * zr;
+  const double opd = C.cavity == 0 ? fmax(xr, fmax(yr, zr)) : C.cavity == 1 ? fmax(xr, yr + zr) : xr + yr + zr;
+  const bool keep = sh.measure_only;
+  if (!(opd > sh.cav_opdf)) {
+    if (!keep) st3(P.cso, n, mk(0, 0, 0));
+    return mk(0, 0, 0);
+  }
+  const double depth = sh.cav_avgR * (sqrt(opd) - 1);
+  const double area_node = mean_vertex_area(P, n);
+  const double k = C.stiff_density * area_node;
+  double fn = 0;
+  if (depth > 0) {
+    const double E = 0.5 * k * depth * depth;
+    fn = k * depth;
+    cv[0] += 1;
+    cv[1] += E;
+    cv[2] += fn;
+    if (mode == M_FORCES) P.ncav[2 * (size_t)n] = E / area_node;
+  }
+  v3 nv = mk(0, 0, 0);
+  const double xx = x.x * x.x * sh.cav_inv2[0], yy = x.y * x.y * sh.cav_inv2[1], zz = x.z * x.z * sh.cav_inv2[2];
+  auto sgn = [](double q) { return (double)((0 < q) - (q < 0)); };
+  if (C.cavity == 0) {
+    if (xx > yy && xx > zz) nv.x = sgn(x.x);
+    else if (yy > zz) nv.y = sgn(x.y);
+    else nv.z = sgn(x.z);
+  } else if (C.cavity == 1) {
+    if (xx > yy + zz) nv.x = sgn(x.x);
+    else { nv.y = x.y * sh.cav_inv2[1]; nv.z = x.z * sh.cav_inv2[2]; }
+  } else {
+    nv = mk(x.x * sh.cav_inv2[0], x.y * sh.cav_inv2[1], x.z * sh.cav_inv2[2]);
+  }
+  nv = scl(nv, 1 / nrm(nv));
+  const double vn = dot(v, nv);
+  v3 fc = mk(0, 0, 0);
+  if (C.cavity_friction) {
+    double Ef = 0;
+    v3 so = ld3(P.cso, n);
+    int stick = nrm2(so) != 0;
+    friction(Ef, fc, fn, nv, v, vn, C.friction_stop_velocity, C.stiff_static_friction, C.mu_s_cav, C.mu_d_cav,
+             C.cav_fric_damping, stick, so, x);
+    if (!keep) st3(P.cso, n, so);
+    cv[3] += Ef;
+    if (mode == M_FORCES) P.ncav[2 * (size_t)n + 1] = Ef / area_node;
+  }
+  fn += C.cav_damping * k * vn;
+  return sub(fc, scl(nv, fn));
+}
+
+/* rotation-damping partials of a node block (ShellSystem.C:1727-1760):
+ * angular momentum m x v and inertia tensor about the origin of the nodes
+ * at the force-evaluation positions, with the current velocities; the
+ * element epilogue reduces them per shell into I^-1 L.  Launched only when
+ * a shell has damping_rotation > 0. */
+__global__ void __launch_bounds__(kNodeBlock) k_rot(DPtrs P, int mode)
+{
+  __shared__ double s_w[kRotRec][kNodeBlock / 64];
+  const int blk = uni(P.bmap[V_NODE][blockIdx.x]);
+  const int sid = uni(P.vshell[blk * kNodePerBlock]);
+  const SConst& C = P.cs[sid];
+  const DShell& sh = P.sh[sid];
+  if (sh.done || !(C.damping_rotation > 0)) return;
+  const int t = threadIdx.x, n = blk * kNodePerBlock + t;
+  double r[kRotRec];
+#pragma unroll
+  for (int q = 0; q < kRotRec; ++q) r[q] = 0;
+  if (n - C.vbase < C.nv) {
+    const int par = sh.par;
+    const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
+    const v3 x = add(ld3(P.X, n), mode == M_STEP ? pred3(u, v, a, sh.dt, sh.nm[8]) : u);
+    const double m = 1 / P.minv[n];
+    const v3 L = scl(crs(x, v), m);
+    r[0] = L.x; r[1] = L.y; r[2] = L.z;
+    r[3] = m * (x.y * x.y + x.z * x.z);
+    r[4] = m * (x.x * x.x + x.z * x.z);
+    r[5] = m * (x.x * x.x + x.y * x.y);
+    r[6] = -(m * x.x * x.y);
+    r[7] = -(m * x.x * x.z);
+    r[8] = -(m * x.y * x.z);
+  }
+#pragma unroll
+  for (int q = 0; q < kRotRec; ++q) r[q] = wave_reduce<false>(r[q]);
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < kRotRec; ++q) s_w[q][t / 64] = r[q];
+  __syncthreads();
+  if (t < kRotRec) P.rpart[(size_t)blk * kRotRec + t] = ((s_w[t][0] + s_w[t][1]) + s_w[t][2]) + s_w[t][3];
+}
+
 /* ||f - B q|| for the least-squares rigid-body inertial force B q (node i:
  * m_i [I | -[x_i]x], about the origin at the force-evaluation positions)
  * from the sums b = B^T f (6) and B^T B (10 unique: m^2, m^2 x, m^2 (|x|^2 I
@@ -624,10 +805,16 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   double rc[kCRec];
   wave_reduce_records<kCRec>(P.cpart + (size_t)C.tile0 * kCRec, C.contact ? C.ntile : 0, kCRec, oc, rc);
   const double f2 = rn[0], fe2 = rn[1], err = rn[2], vmax = rn[3], du = rn[4], ek = rn[5];
-  const double nc = rc[0], es = rc[1], ef = rc[2];
+  const int ov[kCvRec] = {0, 0, 0, 0};
+  double rv[kCvRec];
+  wave_reduce_records<kCvRec>(P.cvpart + (size_t)C.blkN * kCvRec, C.cavity_contact ? C.nbN : 0, kCvRec, ov, rv);
+  const double nc = rc[0], es = rc[1], ef = rc[2] + rv[3];
   if (threadIdx.x != 0) return;
   DShell& w = P.sh[sid];
   w.e_tens = w.att_e_tens; w.e_bend = w.att_e_bend; w.e_self = es; w.e_fric = ef;
+  w.e_cav = rv[1];
+  w.n_cav = (int)rv[0];
+  const double pcav = C.cavity_contact ? rv[2] / w.cav_surface : 0.0;
   w.n_contacts = (int)nc;
   w.n_pairs = (int)rc[3];
   w.bclear = 0;
@@ -643,8 +830,9 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     w.kmean = w.att_kmean;
     w.e_kin = ek;
     /* reset_properties (ShellSystem.C:876): no samples yet */
-    for (int q = 0; q < 11; ++q) w.avg[q] = 0;
+    for (int q = 0; q < 14; ++q) w.avg[q] = 0;
     w.avg_n = 0;
+    w.fcav = rv[2];
     /* residuals of the initial state (with its own V0 when volume_ref is 0) */
     w.residual = w.fext_norm > 0 ? w.fnorm / w.fext_norm : w.fnorm;
     w.residual_deform = w.fext_norm > 0 ? w.fdeform / w.fext_norm : w.fdeform;
@@ -663,7 +851,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
     else w.problem = max(w.problem, 7);
     if (rel == 0) new_dt = 2 * dt;
     else if (rel > C.rel_loc_err_max || rel < C.rel_loc_err_min) new_dt = dt * pow(target / rel, 1. / 3);
-    if (C.contact) {
+    if (C.contact || C.cavity_contact) {
       if (new_dt * vmax * 10 * 2 > C.h) new_dt = C.h / (2 * 10 * vmax);
     }
   }
@@ -688,9 +876,13 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
       /* the volume the force evaluation used (the previous accepted step's
        * after a rejection, when the volume is not taken at the prediction) */
       const double vol = C.volume_at_prediction ? w.att_volume : w.volume;
-      const double smp[11] = {1.0, w.att_narea, vol, nc, ek, w.e_tens, w.e_bend, es, ef, w.kgauss, w.kmean};
-      for (int q = 0; q < 11; ++q) w.avg[q] += dt * smp[q];
+      const double smp[14] = {1.0, w.att_narea, vol, nc, ek, w.e_tens, w.e_bend, es, ef, w.kgauss, w.kmean,
+                              pcav, rv[0], rv[1]};
+      for (int q = 0; q < 14; ++q) w.avg[q] += dt * smp[q];
       w.avg_n++;
+      w.fcav = rv[2];
+      /* the current packing density (ShellSystem.C:2822-2824) */
+      if (C.cavity_contact) w.density = w.att_narea * C.h / w.cav_volume;
     }
     /* kinetic damping, once the load has reached its final value */
     if (w.kd_on && w.t >= C.load_end_time) {
@@ -768,6 +960,7 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
   double rec[kNodeRec];
 #pragma unroll
   for (int q = 0; q < kNodeRec; ++q) rec[q] = 0;
+  double cv[kCvRec] = {0, 0, 0, 0};
   if (valid) {
     double ptot = sh.load_p;
     if (C.bulk_modulus != 0) {
@@ -785,14 +978,29 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
       const ulonglong2 z = make_ulonglong2(0ull, 0ull);
       qw[0] = z; qw[1] = z; qw[2] = z;
     }
-    const v3 fext = add(scl(fp, ptot), fcn);
+    const double mi = P.minv[n];
+    const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
+    const v3 x = add(ld3(P.X, n), mode == M_STEP ? pred3(u, v, a, dt, sh.nm[8]) : u);
+    v3 fext = add(scl(fp, ptot), fcn);
+    if (C.gravity != 0) {
+      /* ShellSystem.C:1587-1597 */
+      const double g = C.gravity / mi;
+      if (C.gravity_dir == 0) fext.x -= g; else if (C.gravity_dir == 1) fext.y -= g; else fext.z -= g;
+    }
+    if (C.cavity_contact) {
+      if (mode == M_FORCES) { P.ncav[2 * (size_t)n] = 0; P.ncav[2 * (size_t)n + 1] = 0; }
+      fext = add(fext, cavity_force(P, C, sh, n, x, v, mode, cv));
+    }
+    if (C.damping_rotation > 0) {
+      /* -c W p, W the cross-product matrix of the shell's angular velocity (ShellSystem.C:1772-1782) */
+      const v3 om = mk(sh.omega[0], sh.omega[1], sh.omega[2]);
+      const v3 wv = mk(-om.z * x.y + om.y * x.z, om.z * x.x - om.x * x.z, -om.y * x.x + om.x * x.y);
+      fext = sub(fext, scl(wv, C.damping_rotation));
+    }
     const v3 f = add(fint, fext);
     rec[0] = nrm2(f);
     rec[1] = nrm2(fext);
-    const double mi = P.minv[n];
-    const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
     {
-      const v3 x = add(ld3(P.X, n), mode == M_STEP ? pred3(u, v, a, dt, sh.nm[8]) : u);
       const double m = 1 / mi, m2 = m * m;
       const v3 xf = crs(x, f);
       rec[6] = m * f.x; rec[7] = m * f.y; rec[8] = m * f.z;
@@ -852,6 +1060,17 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
     double acc = s_wave[t][0];
     for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
     P.npart[(size_t)blk * kNodeRec + t] = acc;
+  }
+  if (C.cavity_contact) {
+    /* the block's cavity partials (the branch is uniform: a block is one shell) */
+#pragma unroll
+    for (int q = 0; q < kCvRec; ++q) cv[q] = wave_reduce<false>(cv[q]);
+    __syncthreads();
+    if ((t & 63) == 0)
+#pragma unroll
+      for (int q = 0; q < kCvRec; ++q) s_wave[q][wave] = cv[q];
+    __syncthreads();
+    if (t < kCvRec) P.cvpart[(size_t)blk * kCvRec + t] = ((s_wave[t][0] + s_wave[t][1]) + s_wave[t][2]) + s_wave[t][3];
   }
   }
 }

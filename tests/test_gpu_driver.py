@@ -152,3 +152,74 @@ def test_driver_equals_the_python_mirror(tmp_path):
     say("  driver vs mirror: t", t, tg, "max |du|", np.max(np.abs(u - u_drv)))
     assert tg == t
     assert np.array_equal(u, u_drv) and np.array_equal(v, v_drv)
+
+
+CAVITY_CONFIG = """# a shell of radius 0.099 crumpled by a shrinking ellipsoidal cavity, in the
+# reference's units and with its config.cfg keys (SI: Pa, kg/m^3, m, s)
+shell_mass_density      = 1e3
+shell_young_modulus     = 1e9
+shell_young_modulus2    = 1e9
+shell_poisson_ratio     = 1/3
+shell_shear_modulus     = 3.75e8
+thickness               = 1e-3
+scale                   = 0.099,0.099,0.099
+cavity                  = ellipsoid
+Rx                      = 0.1*exp(-4*t)
+Ry                      = 0.1*exp(-4*t)
+Rz                      = 0.1*exp(-4*t)
+shell_cavity_contact    = true
+mu_s_cavity             = 0.1
+mu_d_cavity             = 0.1
+shell_shell_contact     = linear
+damping_alpha           = 160
+newmark_beta            = 0.25
+dt                      = 1e-6
+adapt_dt_every          = 1
+rel_loc_err_max         = 5e-4
+rel_loc_err_min         = 5e-5
+friction_stop_velocity  = 1e-3
+cell_margin             = 0.02
+time_steps              = 600
+data_time               = 0.0005
+vtk_time                = 0
+state_time              = 0
+random_seed             = 0
+distortion              = 1e-4
+"""
+
+
+def test_driver_cavity_run_equals_the_python_mirror(tmp_path):
+    """the driver on a cavity configuration in the reference's units (the
+    cavity closes on the shell after about 200 steps): property lines with
+    cavity columns, the same state as the Python mirror's run, and cavity
+    springs in the state file"""
+    X, T = S.mesh_icosphere(3)
+    (tmp_path / "sphere.off").write_text(S.write_off(X, T))
+    (tmp_path / "config.cfg").write_text(CAVITY_CONFIG)
+    say("case driver: cavity config, 600 steps")
+    r = subprocess.run([S.DRIVER_PATH, "-m", "sphere.off"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("# ")]
+    cols = lines[-1][2:].split()
+    say("  last property line:", lines[-1])
+    # columns: date, step, dt, time, secs, area, volume, density, p_cav, Nc_cav, Nc, ...
+    assert float(cols[7]) > 0 and float(cols[8]) > 0 and float(cols[9]) > 0
+    _, t, u_drv, v_drv = _read_state(tmp_path / "data" / "shell_state0000000600.dat")
+    p, d, _ = S.read_config(CAVITY_CONFIG)
+    Xs = S.transform(X, scale=d["scale"])
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48(d["random_seed"])
+    u0 = np.array([d["distortion"] * (2 * libc.drand48() - 1) * p["thickness"] for _ in range(3 * len(X))])
+    g = S.ShellSystem(Xs, T, **p)
+    g.set_state(u0)
+    g.apply_initial_conditions()
+    g.advance(600)
+    u, v, _, tg = g.get_state()
+    springs = g.cavity_springs()
+    g.close()
+    say("  driver vs mirror: t", t, tg, "max |du|", np.max(np.abs(u - u_drv)), "springs", int(np.any(springs, 1).sum()))
+    assert tg == t and np.array_equal(u, u_drv) and np.array_equal(v, v_drv)
+    rows = np.array([[float(x) for x in l.split()] for l in
+                     open(tmp_path / "data" / "shell_state0000000600.dat").read().splitlines()[4:4 + len(X)]])
+    assert np.array_equal(rows[:, 9:12], springs)

@@ -934,3 +934,220 @@ def test_moving_averages_are_dt_weighted_means(icosphere, adapt):
     again = o.properties()                    # reset: no samples left
     assert again[0] == 0
     o.close()
+
+
+# ---------------------------------------------------------------- cavity, gravity, rotation damping
+
+def _stiff_density(E1, E2, nu, h):
+    """the Hertz force density of update_derived: half the mean Young's
+    modulus over (1 - nu nu21) h (ShellSystem.C:477-483)"""
+    return 0.5 * 0.5 * (E1 + E2) / ((1 - nu * (E2 * nu / E1)) * h)
+
+
+def _mean_vertex_area(T, area, nv):
+    """per node, the mean area of the elements that have it as a vertex"""
+    s = np.zeros(nv)
+    c = np.zeros(nv)
+    for k in range(3):
+        np.add.at(s, T[:, k], area)
+        np.add.at(c, T[:, k], 1)
+    return s / c
+
+
+def _cavity_sphere(handle=None, **kw):
+    X, T = O.icosphere(3, round_g=False)
+    base = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=H, cavity_contact=1, cavity=2,
+                cavity_R0=(0.99, 0.99, 0.99))
+    base.update(kw)
+    return X, T, O.Oracle(X, T, lib_handle=handle, **base)
+
+
+def test_cavity_sphere_in_sphere_closed_form():
+    """a unit sphere dilated by (1 + eps) inside a spherical cavity of radius
+    R: every node penetrates by depth = 1 + eps - (R - h/2) (the effective
+    semi-axis of update_derived, ShellSystem.C:424-435, and the isoparameter
+    of ShellSystem.h:336-365 reduce to this for a sphere), the element areas
+    scale by (1 + eps)^2, and the node is pushed back along its radius by
+    stiff_density * (mean area of its elements) * depth; its energy over
+    that area is stiff_density depth^2 / 2 (ShellSystem.C:1615-1717)"""
+    X, T, o = _cavity_sphere()
+    sd = _stiff_density(1.0, 0.5, 0.3, H)
+    R_eff = 0.99 - H / 2
+    area0 = o.forces(np.zeros(X.size))[3]
+    abar = _mean_vertex_area(T, area0, len(X))
+    for eps in (0.0, -0.004, 0.003):
+        fext = o.forces((eps * X).reshape(-1))[1].reshape(-1, 3)
+        depth = 1 + eps - R_eff
+        expect = -(sd * (1 + eps) ** 2 * abar * depth)[:, None] * X
+        assert np.abs(fext - expect).max() <= 1e-12 * np.abs(expect).max()
+        _, ecav, _ = o.node_cavity()
+        np.testing.assert_allclose(ecav, 0.5 * sd * depth ** 2, rtol=1e-12)
+    o.close()
+
+
+def test_cavity_box_and_cylinder_normals():
+    """special cases of the box and cylinder normals (ShellSystem.C:1635-1666):
+    a shell pushed against a face of a box is pushed back along the face
+    normal only, by stiff_density * mean area * depth, the depth being the
+    overshoot x - R_eff scaled by mean(R_eff) / R_eff (the isoparameter's
+    depth, ShellSystem.C:1621); pushed against the wall of a cylinder (axis
+    x) with equal radial semi-axes, along the radius in the (y, z) plane;
+    pushed against the cylinder's end cap, along x"""
+    sd = _stiff_density(1.0, 0.5, 0.3, H)
+    for cav, R0, shift, axis in ((0, (1.0, 1.2, 1.2), 0.012, 0), (1, (1.2, 1.006, 1.006), 0.012, 1),
+                                 (1, (1.0, 1.2, 1.2), 0.012, 0)):
+        X, T, o = _cavity_sphere(cavity=cav, cavity_R0=R0)
+        u = np.zeros_like(X)
+        u[:, axis] = shift
+        fext, area = (lambda r: (r[1].reshape(-1, 3), r[3]))(o.forces(u.reshape(-1)))
+        abar = _mean_vertex_area(T, area, len(X))
+        p = X + u
+        R_eff = np.array(R0) - H / 2
+        if cav == 1 and axis == 1:
+            rho = np.hypot(p[:, 1], p[:, 2])
+            hit = rho > R_eff[1]
+            n = np.zeros_like(p)
+            n[:, 1:] = p[:, 1:] / rho[:, None]
+            depth = (rho - R_eff[1]) * (R_eff.mean() / R_eff[1])
+        else:
+            hit = p[:, axis] > R_eff[axis]
+            n = np.zeros_like(p)
+            n[:, axis] = 1
+            depth = (p[:, axis] - R_eff[axis]) * (R_eff.mean() / R_eff[axis])
+        assert 5 < hit.sum() < len(X) // 4
+        expect = np.where(hit[:, None], -(sd * abar * depth)[:, None] * n, 0.0)
+        assert np.abs(fext - expect).max() <= 1e-12 * np.abs(expect).max()
+        o.close()
+
+
+def test_cavity_damping_and_friction_special_cases():
+    """with a uniform radial velocity c the normal force grows by
+    cavity_damping * k * c (ShellSystem.C:1709-1712); a tangential velocity
+    above friction_stop_velocity adds the dynamic friction -mu_d f_n v_t /
+    (|v_t| + 1e-8 v_stop) formed from the undamped normal force f_n = k depth
+    (Friction.C:4-60 called before the damping term, ShellSystem.C:1688-1712)"""
+    X, T, o = _cavity_sphere(cavity_damping=0.7, mu_d_cavity=0.4, friction_stop_velocity=1e-3)
+    sd = _stiff_density(1.0, 0.5, 0.3, H)
+    R_eff = 0.99 - H / 2
+    z = np.zeros(X.size)
+    area = o.forces(z)[3]
+    k = sd * _mean_vertex_area(T, area, len(X))
+    depth = 1 - R_eff
+    c, w = 0.02, 0.05
+    vt = w * np.cross([0.0, 0.0, 1.0], X)              # rotation about z
+    v = c * X + vt
+    fext = o.forces(z, v.reshape(-1))[1].reshape(-1, 3)
+    avt = np.linalg.norm(vt, axis=1)
+    fric = -(0.4 * k * depth / (avt + 1e-8 * 1e-3))[:, None] * vt
+    expect = -((k * depth + 0.7 * k * c))[:, None] * X + fric
+    fast = avt > 1e-3
+    assert fast.sum() > len(X) // 2
+    assert np.abs(fext[fast] - expect[fast]).max() <= 1e-12 * np.abs(expect).max()
+    o.close()
+
+
+def test_gravity_and_rotation_damping_closed_forms(icosphere):
+    """gravity loads each node with -g m_n along its direction
+    (ShellSystem.C:1587-1597); for a rigid rotation v = w0 x p the angular
+    velocity I^-1 L of ShellSystem.C:1727-1770 is w0 itself, so the rotation
+    damping force is -c (w0 x p) at every node, also on an imperfect shell
+    whose inertia tensor is full"""
+    X, T = icosphere(3)
+    X = imperfect(X, 0.05, seed=11) * np.array([1.0, 1.3, 0.8])
+    u = np.random.default_rng(4).normal(size=X.size) * 1e-3
+    p = X + u.reshape(-1, 3)
+    g, cr = 0.37, 0.8
+    w0 = np.array([0.3, -0.7, 0.45])
+    o = O.Oracle(X, T, gravity=g, gravity_dir=1, damping_rotation=cr)
+    Minv, _ = o.mass()
+    m = 1 / Minv[0::3]
+    fext = o.forces(u, np.cross(w0, p).reshape(-1))[1].reshape(-1, 3)
+    expect = -cr * np.cross(w0, p)
+    expect[:, 1] -= g * m
+    assert np.abs(fext - expect).max() <= 1e-12 * np.abs(expect).max()
+    o.close()
+
+
+def test_cavity_time_functions_and_measures():
+    """R(t) = R(0) exp(rate t) at the force time; the properties of one
+    sample: packing density area h / (4/3 pi R^3), cavity pressure
+    (sum of the normal forces) / (4 pi R^2) (Knud Thomsen's surface is exact
+    for a sphere), the count of nodes in contact and the cavity energy
+    (ShellSystem.C:440-470, 2818-2843)"""
+    X, T, o = _cavity_sphere(cavity_rate=-0.5)
+    X2, T2, o2 = _cavity_sphere(cavity_R0=tuple([0.99 * np.exp(-0.5 * 0.03)] * 3))
+    z = np.zeros(X.size)
+    f1 = o.forces(z, None, 0.03)[1]
+    f2 = o2.forces(z, None, 0.0)[1]
+    assert np.abs(f1 - f2).max() <= 1e-15 * np.abs(f2).max()
+    o2.close()
+    o.close()
+    X, T, o = _cavity_sphere()
+    o.set_state(z, z, z, 0.0)
+    o.init_dynamics()
+    o.measure()
+    _, fext, _, area = o.forces(z, None, 0.0)
+    fn = np.linalg.norm(fext.reshape(-1, 3), axis=1)
+    _, ecav, _ = o.node_cavity()
+    abar = _mean_vertex_area(T, area, len(X))
+    R = 0.99
+    pr = o.properties()
+    assert abs(pr[4] - area.sum() * H / (4 / 3 * np.pi * R ** 3)) <= 1e-13 * pr[4]
+    assert abs(pr[5] - fn.sum() / (4 * np.pi * R ** 2)) <= 1e-12 * pr[5]
+    assert pr[6] == len(X)
+    assert abs(pr[11] - (ecav * abar).sum()) <= 1e-12 * pr[11]
+    o.close()
+
+
+@pytest.mark.parametrize("slip", ["R instead of R_eff", "area sum", "damping sign"])
+def test_cavity_pins_catch_planted_slips(slip):
+    from oracle_mutants import build
+    old, new = {
+        "R instead of R_eff": ("const double h_half = 0.5 * P.thickness;", "const double h_half = 0.0;"),
+        "area sum": ("  return a / s.node_elems[n].size();", "  return a;"),
+        "damping sign": ("f_normal += P.cavity_damping * stiffness * abs_v_normal;",
+                         "f_normal -= P.cavity_damping * stiffness * abs_v_normal;"),
+    }[slip]
+    L = build(old, new, "cav" + str(abs(hash(slip)) % 1000))
+    if slip == "damping sign":
+        X, T, o = _cavity_sphere(handle=L, cavity_damping=0.7)
+        sd = _stiff_density(1.0, 0.5, 0.3, H)
+        z = np.zeros(X.size)
+        area = o.forces(z)[3]
+        k = sd * _mean_vertex_area(T, area, len(X))
+        fext = o.forces(z, (0.02 * X).reshape(-1))[1].reshape(-1, 3)
+        expect = -(k * (1 - 0.99 + H / 2) + 0.7 * k * 0.02)[:, None] * X
+        assert np.abs(fext - expect).max() > 1e-6 * np.abs(expect).max()
+        o.close()
+        return
+    X, T, o = _cavity_sphere(handle=L)
+    sd = _stiff_density(1.0, 0.5, 0.3, H)
+    area0 = o.forces(np.zeros(X.size))[3]
+    abar = _mean_vertex_area(T, area0, len(X))
+    fext = o.forces(np.zeros(X.size))[1].reshape(-1, 3)
+    expect = -(sd * abar * (1 - 0.99 + H / 2))[:, None] * X
+    assert np.abs(fext - expect).max() > 1e-6 * np.abs(expect).max()
+    o.close()
+
+
+@pytest.mark.parametrize("slip", ["inertia off-diagonal sign", "gravity on the mass inverse"])
+def test_gravity_rotation_pins_catch_planted_slips(icosphere, slip):
+    from oracle_mutants import build
+    old, new = {
+        "inertia off-diagonal sign": ("      I[1] -= mi * ri(0) * ri(1);", "      I[1] += mi * ri(0) * ri(1);"),
+        "gravity on the mass inverse": ("      s.fext[dof] -= P.gravity / s.Minv[dof];",
+                                        "      s.fext[dof] -= P.gravity * s.Minv[dof];"),
+    }[slip]
+    L = build(old, new, "rot" + str(abs(hash(slip)) % 1000))
+    X, T = icosphere(3)
+    X = imperfect(X, 0.05, seed=11) * np.array([1.0, 1.3, 0.8])
+    u = np.random.default_rng(4).normal(size=X.size) * 1e-3
+    p = X + u.reshape(-1, 3)
+    w0 = np.array([0.3, -0.7, 0.45])
+    o = O.Oracle(X, T, lib_handle=L, gravity=0.37, gravity_dir=1, damping_rotation=0.8)
+    Minv, _ = o.mass()
+    fext = o.forces(u, np.cross(w0, p).reshape(-1))[1].reshape(-1, 3)
+    expect = -0.8 * np.cross(w0, p)
+    expect[:, 1] -= 0.37 / Minv[0::3]
+    assert np.abs(fext - expect).max() > 1e-6 * np.abs(expect).max()
+    o.close()
