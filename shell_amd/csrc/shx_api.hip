@@ -44,6 +44,7 @@ struct shx_ctx {
   bool any_contact = false;
   bool any_kd = false;
   bool any_rot = false;        /* some shell has rotation damping: k_rot precedes the element epilogue */
+  bool any_loads = false;      /* some shell has gravity, cavity contact or rotation damping: k_node_loads */
   std::vector<char> keep_cso;  /* per shell: cavity springs were set by the caller (a loaded state) */
   shx_shapes shapes = {4, 6, 3, 16, 2, {0, 0, 0}};   /* launch shapes (shx_set_shapes) */             /* some shell has kinetic damping on: k_kd follows k_fin_node */
   std::vector<void*> allocs;
@@ -265,7 +266,9 @@ void launch_elem(shx_ctx* ctx, int mode)
 void launch_node(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 1);
-  if (ctx->grid[3]) {
+  if (ctx->grid[3] && ctx->any_loads) {
+    k_node_loads<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
+  } else if (ctx->grid[3]) {
     switch (ctx->shapes.node_waves) {
       case 4: k_node_w4<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode); break;
       case 8: k_node_w8<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode); break;
@@ -610,6 +613,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     C.volume_prog = vprog[s];
     for (int k = 0; k < 3; ++k) C.cav_prog[k] = cprog[(size_t)3 * s + k];
     if (C.damping_rotation > 0) ctx->any_rot = true;
+    if (C.damping_rotation > 0 || C.gravity != 0 || C.cavity_contact) ctx->any_loads = true;
     const int nvp = round_up(nv[s], kAlign), nep = round_up(ne[s], kAlign);
     C.vbase = (int)vb; C.ebase = (int)eb;
     C.blkE = (int)(eb / kElemPerBlock); C.nbE = nep / kElemPerBlock;

@@ -919,6 +919,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
  * the rigid-body sums, for k_fin_node.  Launch block b computes node block
  * bmap[V_NODE][b].  Force time: t + dt in a step, t + dt in M_INIT
  * (apply_initial_conditions), DShell::t_eval in M_FORCES. */
+template <bool LOADS>
 __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
 {
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
@@ -982,20 +983,22 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
     const v3 u = ld3(P.U[par], n), v = ld3(P.V[par], n), a = ld3(P.A[par], n);
     const v3 x = add(ld3(P.X, n), mode == M_STEP ? pred3(u, v, a, dt, sh.nm[8]) : u);
     v3 fext = add(scl(fp, ptot), fcn);
-    if (C.gravity != 0) {
-      /* ShellSystem.C:1587-1597 */
-      const double g = C.gravity / mi;
-      if (C.gravity_dir == 0) fext.x -= g; else if (C.gravity_dir == 1) fext.y -= g; else fext.z -= g;
-    }
-    if (C.cavity_contact) {
-      if (mode == M_FORCES) { P.ncav[2 * (size_t)n] = 0; P.ncav[2 * (size_t)n + 1] = 0; }
-      fext = add(fext, cavity_force(P, C, sh, n, x, v, mode, cv));
-    }
-    if (C.damping_rotation > 0) {
-      /* -c W p, W the cross-product matrix of the shell's angular velocity (ShellSystem.C:1772-1782) */
-      const v3 om = mk(sh.omega[0], sh.omega[1], sh.omega[2]);
-      const v3 wv = mk(-om.z * x.y + om.y * x.z, om.z * x.x - om.x * x.z, -om.y * x.x + om.x * x.y);
-      fext = sub(fext, scl(wv, C.damping_rotation));
+    if constexpr (LOADS) {
+      if (C.gravity != 0) {
+        /* ShellSystem.C:1587-1597 */
+        const double g = C.gravity / mi;
+        if (C.gravity_dir == 0) fext.x -= g; else if (C.gravity_dir == 1) fext.y -= g; else fext.z -= g;
+      }
+      if (C.cavity_contact) {
+        if (mode == M_FORCES) { P.ncav[2 * (size_t)n] = 0; P.ncav[2 * (size_t)n + 1] = 0; }
+        fext = add(fext, cavity_force(P, C, sh, n, x, v, mode, cv));
+      }
+      if (C.damping_rotation > 0) {
+        /* -c W p, W the cross-product matrix of the shell's angular velocity (ShellSystem.C:1772-1782) */
+        const v3 om = mk(sh.omega[0], sh.omega[1], sh.omega[2]);
+        const v3 wv = mk(-om.z * x.y + om.y * x.z, om.z * x.x - om.x * x.z, -om.y * x.x + om.x * x.y);
+        fext = sub(fext, scl(wv, C.damping_rotation));
+      }
     }
     const v3 f = add(fint, fext);
     rec[0] = nrm2(f);
@@ -1061,7 +1064,7 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
     for (int w = 1; w < kNodeBlock / 64; ++w) acc = mx ? fmax(acc, s_wave[t][w]) : acc + s_wave[t][w];
     P.npart[(size_t)blk * kNodeRec + t] = acc;
   }
-  if (C.cavity_contact) {
+  if (LOADS && C.cavity_contact) {
     /* the block's cavity partials (the branch is uniform: a block is one shell) */
 #pragma unroll
     for (int q = 0; q < kCvRec; ++q) cv[q] = wave_reduce<false>(cv[q]);
@@ -1740,9 +1743,12 @@ __global__ void __launch_bounds__(256) k_bp_host_done(DPtrs P)
 __global__ void __launch_bounds__(256, 2) k_contact_w2(DPtrs P, int mode) { contact_body(P, mode); }
 __global__ void __launch_bounds__(256, 3) k_contact_w3(DPtrs P, int mode) { contact_body(P, mode); }
 __global__ void __launch_bounds__(256, 4) k_contact_w4(DPtrs P, int mode) { contact_body(P, mode); }
-__global__ void __launch_bounds__(kNodeBlock, 4) k_node_w4(DPtrs P, int mode) { node_body(P, mode); }
-__global__ void __launch_bounds__(kNodeBlock, 6) k_node_w6(DPtrs P, int mode) { node_body(P, mode); }
-__global__ void __launch_bounds__(kNodeBlock, 8) k_node_w8(DPtrs P, int mode) { node_body(P, mode); }
+__global__ void __launch_bounds__(kNodeBlock, 4) k_node_w4(DPtrs P, int mode) { node_body<false>(P, mode); }
+__global__ void __launch_bounds__(kNodeBlock, 6) k_node_w6(DPtrs P, int mode) { node_body<false>(P, mode); }
+__global__ void __launch_bounds__(kNodeBlock, 8) k_node_w8(DPtrs P, int mode) { node_body<false>(P, mode); }
+/* the node kernel with the loads after the pressure (gravity, cavity contact,
+ * rotation damping), for batches where a shell has one of them */
+__global__ void __launch_bounds__(kNodeBlock, 4) k_node_loads(DPtrs P, int mode) { node_body<true>(P, mode); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l8(DPtrs P, int posmode, int list) { pairs_body<8>(P, posmode, list); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l16(DPtrs P, int posmode, int list) { pairs_body<16>(P, posmode, list); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l32(DPtrs P, int posmode, int list) { pairs_body<32>(P, posmode, list); }
