@@ -222,6 +222,10 @@ int shx_pairs(shx_ctx* ctx, uint32_t* e1, uint32_t* e2, int cap);
 int shx_elem_tables(shx_ctx* ctx, double* out);
 int shx_mass(shx_ctx* ctx, double* Minv, double* C);
 int shx_patches(shx_ctx* ctx, int32_t* idx /* ne*12, -1 padded */, int32_t* tri /* ne*3 */);
+/* the same for one shell of a batch (node ids local to the shell): the
+ * one-ring patches and rotated triangles the GPU preprocessor formed
+ * (build_cache's find_one_ring, ShellSystem.C:490-571) */
+int shx_batch_patches(shx_ctx* ctx, int shell, int32_t* idx, int32_t* tri);
 
 /* Largest angular frequency of M^-1 K at u = 0 (power iteration on the GPU
  * force kernel); the stable central-difference step is 2/omega. */
@@ -473,6 +477,12 @@ int shx_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, int on_mesh
 void shx_friction_apply(double* Efric, double* f_contact, double f_normal, const double* n, const double* v,
                         double abs_v_normal, double v_stop, double spring_stiffness, double mu_s, double mu_d,
                         double damping, int32_t* stick, double* spring_origin, const double* contact_point);
+
+/* The contact exclusion sets of one shell as the GPU preprocessor formed
+ * them (build_topological_neighborhood, ShellSystem.C:684-741): CSR start
+ * (ne+1) and sorted element ids local to the shell; returns the number of
+ * ids (ids may be NULL or cap too small to take them all). */
+int shx_batch_close_sets(shx_ctx* ctx, int shell, int32_t* start, int32_t* ids, int cap);
 
 /* Primitives exposed for parity tests (host-compiled copies of the device
  * routines; no GPU needed). */

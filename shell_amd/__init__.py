@@ -178,6 +178,8 @@ def lib():
         L.shx_batch_properties.argtypes = [_V, ctypes.c_int, _D]
         L.shx_batch_fields.argtypes = [_V, ctypes.c_int, _D, _D]
         L.shx_batch_cavity_springs.argtypes = [_V, ctypes.c_int, _D, ctypes.c_int]
+        L.shx_batch_close_sets.argtypes = [_V, ctypes.c_int, _I, _I, ctypes.c_int]
+        L.shx_batch_patches.argtypes = [_V, ctypes.c_int, _I, _I]
         _lib = L
     return _lib
 
@@ -274,6 +276,25 @@ class _Measurements:
         if r != 0:
             raise ShellError(f"shx_state_load: {L.shx_io_last_error().decode()}", r)
         return step.value, wall.value
+
+    def shell_patches(self, shell=0):
+        """one shell's one-ring patches (ne x 12, -1 padded) and rotated triangles"""
+        nv, ne = self._sizes(shell)
+        idx = np.zeros((ne, 12), dtype=np.int32)
+        tri = np.zeros((ne, 3), dtype=np.int32)
+        self._check(lib().shx_batch_patches(self.h, shell, idx.ctypes.data_as(_I), tri.ctypes.data_as(_I)),
+                    "shx_batch_patches")
+        return idx, tri
+
+    def close_sets(self, shell=0):
+        """the contact exclusion sets (CSR start, sorted ids) of one shell"""
+        nv, ne = self._sizes(shell)
+        start = np.zeros(ne + 1, dtype=np.int32)
+        n = lib().shx_batch_close_sets(self.h, shell, start.ctypes.data_as(_I), None, 0)
+        self._check(n, "shx_batch_close_sets")
+        ids = np.zeros(max(n, 1), dtype=np.int32)
+        lib().shx_batch_close_sets(self.h, shell, None, ids.ctypes.data_as(_I), n)
+        return start, ids[:n]
 
     def cavity_springs(self, shell=0, origin=None):
         """the nodes' cavity friction spring origins (nv x 3); with origin

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libshellhip.so")
-SOURCES = ["shx_api.hip", "shx_kernels.hip", "shx_elem.hip", "shx_mlp.hip", "shx_setup.cpp", "shx_config.cpp", "shx_mesh.cpp", "shx_io.cpp"]
+SOURCES = ["shx_api.hip", "shx_kernels.hip", "shx_elem.hip", "shx_topo.hip", "shx_mlp.hip", "shx_setup.cpp", "shx_config.cpp", "shx_mesh.cpp", "shx_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

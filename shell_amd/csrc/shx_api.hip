@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -20,6 +21,7 @@
 #include "shx_kernels.h"
 #include "shx_math.h"
 #include "shx_setup.h"
+#include "shx_topo.h"
 
 using namespace shx;
 
@@ -106,6 +108,119 @@ int upload(shx_ctx* ctx, T*& p, const std::vector<T>& h)
 }
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+/* build_cache's topology half on the GPU (shx_topo.hip) for every shell at
+ * once: vertex incidence, validation, rotated triangles, patches and
+ * valences (downloaded into ctx->T for the host's launch planning), patch
+ * incidence, and the contact exclusion sets of the shells with
+ * self-contact.  Needs the placement (ctx->C), P.X and P.eshell. */
+int build_topology_dev(shx_ctx* ctx, const int* nv, const int* ne, const int32_t* tri, const std::vector<size_t>& toff,
+                       const shx_params* prm)
+{
+  DPtrs& P = ctx->P;
+  const int S = ctx->S, nE = ctx->nblkE * kElemPerBlock, nV = ctx->nblkN * kNodePerBlock;
+  std::vector<int32_t> tin((size_t)3 * nE, -1), contact(S, 0);
+  std::vector<double> cutoff(S, 0.0);
+  for (int s = 0; s < S; ++s) {
+    const SConst& C = ctx->C[s];
+    for (int e = 0; e < ne[s]; ++e)
+      for (int k = 0; k < 3; ++k) tin[(size_t)3 * (C.ebase + e) + k] = C.vbase + tri[toff[s] + (size_t)3 * e + k];
+    contact[s] = C.contact;
+    cutoff[s] = (1 + prm[s].neighbor_margin) * prm[s].thickness;
+  }
+  TopoPtrs T{};
+  T.nE = nE; T.nV = nV;
+  T.eshell = P.eshell; T.X = P.X;
+  int32_t *dtin, *dcon, *derr;
+  double* dcut;
+  std::vector<int32_t> err0((size_t)2 * S);
+  for (int s = 0; s < S; ++s) { err0[2 * s] = 0; err0[2 * s + 1] = INT32_MAX; }
+  int r = 0;
+  if ((r = upload(ctx, dtin, tin)) || (r = upload(ctx, dcon, contact)) || (r = upload(ctx, dcut, cutoff)) ||
+      (r = upload(ctx, derr, err0)) || (r = dalloc(ctx, T.vcnt, nV)) || (r = dalloc(ctx, T.vcur, nV)) ||
+      (r = dalloc(ctx, T.vstart, (size_t)nV + 1)) || (r = dalloc(ctx, T.tri, (size_t)3 * nE)) ||
+      (r = dalloc(ctx, T.patch, (size_t)12 * nE)) || (r = dalloc(ctx, T.valence, nE)) ||
+      (r = dalloc(ctx, T.icnt, nV)) || (r = dalloc(ctx, T.icur, nV)) || (r = dalloc(ctx, T.istart, (size_t)nV + 1)) ||
+      (r = dalloc(ctx, T.ccnt, nE)) || (r = dalloc(ctx, T.cstart, (size_t)nE + 1)))
+    return r;
+  T.tin = dtin; T.contact = dcon; T.cutoff = dcut; T.err = derr;
+  const int gE = nE / 256, gV = nV / 256;
+  k_topo_valence<<<gE, 256, 0, ctx->st>>>(T);
+  k_topo_scan<<<1, 1024, 0, ctx->st>>>(T.vcnt, T.vstart, nV);
+  int32_t nvinc = 0;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&nvinc, T.vstart + nV, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if ((r = dalloc(ctx, T.vinc, nvinc))) return r;
+  k_topo_vfill<<<gE, 256, 0, ctx->st>>>(T);
+  k_topo_vsort<<<gV, 256, 0, ctx->st>>>(T);
+  k_topo_patch<<<gE, 256, 0, ctx->st>>>(T);
+  HIPCHK(hipGetLastError());
+  std::vector<int32_t> err((size_t)2 * S);
+  HIPCHK(hipMemcpyAsync(err.data(), derr, sizeof(int32_t) * err.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  static const char* what[] = {"non-manifold or inconsistently oriented edge", "mesh is not closed",
+                               "triangle with more than one irregular vertex", "unsupported valence (3..6)",
+                               "one-ring walk did not close",
+                               "contact exclusion set larger than its capacity (neighbor_margin too large)"};
+  auto check = [&]() -> int {
+    for (int s = 0; s < S; ++s)
+      for (int b = 0; b < 6; ++b)
+        if (err[2 * s] & (1 << b)) {
+          ctx->err = "shell " + std::to_string(s) + ": " + what[b] + " (element " +
+                     std::to_string(err[2 * s + 1] - ctx->C[s].ebase) + ")";
+          return SHX_E_MESH;
+        }
+    return 0;
+  };
+  if ((r = check())) return r;
+  k_topo_scan<<<1, 1024, 0, ctx->st>>>(T.icnt, T.istart, nV);
+  int32_t ninc = 0;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&ninc, T.istart + nV, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if ((r = dalloc(ctx, T.inc, ninc))) return r;
+  k_topo_ifill<<<gE, 256, 0, ctx->st>>>(T);
+  k_topo_isort<<<gV, 256, 0, ctx->st>>>(T);
+  k_topo_close<<<gE, 256, 0, ctx->st>>>(T, 0);
+  k_topo_scan<<<1, 1024, 0, ctx->st>>>(T.ccnt, T.cstart, nE);
+  int32_t nclose = 0;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&nclose, T.cstart + nE, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(err.data(), derr, sizeof(int32_t) * err.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if ((r = check())) return r;
+  if ((r = dalloc(ctx, T.cids, nclose))) return r;
+  k_topo_close<<<gE, 256, 0, ctx->st>>>(T, 1);
+  HIPCHK(hipGetLastError());
+  /* the patches for the host's launch planning and the mesh queries */
+  std::vector<int32_t> htri((size_t)3 * nE), hpatch((size_t)12 * nE);
+  std::vector<int8_t> hval(nE);
+  HIPCHK(hipMemcpyAsync(htri.data(), T.tri, sizeof(int32_t) * htri.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(hpatch.data(), T.patch, sizeof(int32_t) * hpatch.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(hval.data(), T.valence, nE, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  for (int s = 0; s < S; ++s) {
+    Topology& Ts = ctx->T[s];
+    const SConst& C = ctx->C[s];
+    Ts.tri.assign((size_t)3 * ne[s], 0);
+    Ts.patch.assign((size_t)12 * ne[s], -1);
+    Ts.valence.assign(ne[s], 6);
+    for (int e = 0; e < ne[s]; ++e) {
+      const size_t g = (size_t)C.ebase + e;
+      for (int k = 0; k < 3; ++k) Ts.tri[(size_t)3 * e + k] = htri[3 * g + k] - C.vbase;
+      for (int i = 0; i < 12; ++i) {
+        const int n = hpatch[12 * g + i];
+        Ts.patch[(size_t)12 * e + i] = n >= 0 ? n - C.vbase : -1;
+      }
+      Ts.valence[e] = (uint8_t)hval[g];
+    }
+  }
+  P.inc_start = T.istart; P.inc = T.inc;
+  P.vinc_start = T.vstart; P.vinc = T.vinc;
+  P.close_start = T.cstart; P.close_ids = T.cids;
+  return 0;
+}
 
 void set_newmark_host(DShell& s, const SConst& C, double dt)
 {
@@ -601,11 +716,13 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   long long vb = 0, eb = 0;
   unsigned long long bb = 0, entb = 0;
   for (int s = 0; s < nshell; ++s) {
-    std::string terr;
-    if (!build_topology(nv[s], ne[s], tri + toff[s], ctx->T[s], terr)) {
-      ctx->err = "shell " + std::to_string(s) + ": " + terr;
-      return fail(SHX_E_MESH);
-    }
+    for (size_t q = toff[s]; q < toff[s + 1]; ++q)
+      if (tri[q] < 0 || tri[q] >= nv[s]) {
+        ctx->err = "shell " + std::to_string(s) + ": vertex index out of range";
+        return fail(SHX_E_MESH);
+      }
+    ctx->T[s].nv = nv[s];
+    ctx->T[s].ne = ne[s];
     ctx->Xin[s].assign(X + xoff[s], X + xoff[s] + (size_t)3 * nv[s]);
     SConst C = shell_consts(&prm[s], nv[s], ne[s]);
     C.fx_shift = contact_fx_shift(&prm[s], C, ne[s], X + xoff[s], tri + toff[s]);
@@ -647,23 +764,36 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   ctx->nblkN = nV / kNodePerBlock;
   ctx->ntile = nE / 256;
 
+  /* the topology on the GPU: patches, incidences, exclusion sets */
+  std::vector<double> Xh((size_t)3 * nV, 0.0);
+  std::vector<int32_t> eshell(nE), vshell(nV);
+  for (int s = 0; s < nshell; ++s) {
+    const SConst& C = ctx->C[s];
+    for (int q = 0; q < C.ntile * 256; ++q) eshell[C.ebase + q] = s;
+    for (int q = 0; q < C.nbN * kNodePerBlock; ++q) vshell[C.vbase + q] = s;
+    for (int q = 0; q < 3 * nv[s]; ++q) Xh[(size_t)3 * C.vbase + q] = X[xoff[s] + q];
+  }
+  DPtrs& P = ctx->P;
+  int r = 0;
+  double* dX;
+  int32_t *des, *dvs;
+  if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell))) return fail(r);
+  P.X = dX; P.eshell = des; P.vshell = dvs;
+  if ((r = build_topology_dev(ctx, nv, ne, tri, toff, prm))) return fail(r);
+
   /* Loop basis at the quadrature point, valences 3..6 */
   ctx->rawb.assign(4 * 72, 0.0);
   for (int val = 3; val <= 6; ++val) loop_basis_at_centroid(val, &ctx->rawb[(val - 3) * 72]);
 
-  /* global structure-of-arrays copies of the topology */
-  std::vector<int32_t> pidx((size_t)12 * nE), triS((size_t)3 * nE), eshell(nE), vshell(nV);
+  /* global structure-of-arrays copies of the patches */
+  std::vector<int32_t> pidx((size_t)12 * nE), triS((size_t)3 * nE);
   std::vector<int8_t> vcls(nE, 6);
-  std::vector<double> Xh((size_t)3 * nV, 0.0);
-  std::vector<int32_t> inc_start((size_t)nV + 1, 0), inc;
-  std::vector<int32_t> cstart((size_t)nE + 1, 0), cids;
   for (int s = 0; s < nshell; ++s) {
     const SConst& C = ctx->C[s];
     const Topology& T = ctx->T[s];
-    const int nep = C.ntile * 256, nvp = C.nbN * kNodePerBlock;
+    const int nep = C.ntile * 256;
     for (int q = 0; q < nep; ++q) {
       const int e = C.ebase + q;
-      eshell[e] = s;
       for (int i = 0; i < 12; ++i) {
         int n = C.vbase;
         if (q < ne[s]) {
@@ -675,39 +805,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       for (int k = 0; k < 3; ++k) triS[(size_t)k * nE + e] = C.vbase + (q < ne[s] ? T.tri[(size_t)3 * q + k] : 0);
       if (q < ne[s]) vcls[e] = (int8_t)T.valence[q];
     }
-    for (int q = 0; q < nvp; ++q) vshell[C.vbase + q] = s;
-    for (int q = 0; q < 3 * nv[s]; ++q) Xh[(size_t)3 * C.vbase + q] = X[xoff[s] + q];
   }
-  /* node incidence and contact exclusion as global CSR lists */
-  for (int s = 0; s < nshell; ++s) {
-    const SConst& C = ctx->C[s];
-    const Topology& T = ctx->T[s];
-    for (int q = 0; q < nv[s]; ++q) {
-      const int n = C.vbase + q;
-      inc_start[n] = (int)inc.size();
-      for (int k = T.inc_start[q]; k < T.inc_start[q + 1]; ++k) {
-        const int ent = T.inc[k];
-        inc.push_back(((C.ebase + (ent >> 4)) << 4) | (ent & 15));
-      }
-    }
-    for (int q = nv[s]; q < C.nbN * kNodePerBlock; ++q) inc_start[C.vbase + q] = (int)inc.size();
-  }
-  inc_start[nV] = (int)inc.size();
-  /* vertex incidence (nodes_to_elem_map, ascending element ids): the mean
-   * element area of a node for the cavity penalty (avg_area_node) */
-  std::vector<int32_t> vinc_start((size_t)nV + 1, 0), vinc;
-  for (int s = 0; s < nshell; ++s) {
-    const SConst& C = ctx->C[s];
-    const Topology& T = ctx->T[s];
-    std::vector<std::vector<int32_t>> ve(nv[s]);
-    for (int e = 0; e < ne[s]; ++e)
-      for (int k = 0; k < 3; ++k) ve[T.tri[(size_t)3 * e + k]].push_back(C.ebase + e);
-    for (int q = 0; q < C.nbN * kNodePerBlock; ++q) {
-      vinc_start[C.vbase + q] = (int)vinc.size();
-      if (q < nv[s]) vinc.insert(vinc.end(), ve[q].begin(), ve[q].end());
-    }
-  }
-  vinc_start[nV] = (int)vinc.size();
   /* element blocks: their patch nodes (in descending entry count), each patch
    * slot's index into them, per block node its (element, slot) entries, and
    * per node the block records that hold its partial forces */
@@ -786,41 +884,22 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     std::vector<int32_t> fill(nrec_start.begin(), nrec_start.end() - 1);
     for (int r = 0; r < nrecs; ++r) nrec[fill[bn_ids[r]]++] = r;   /* ascending block order per node */
   }
-  for (int s = 0; s < nshell; ++s) {
-    const SConst& C = ctx->C[s];
-    std::vector<int32_t> cs, ci;
-    if (C.contact)
-      build_close_sets(ctx->T[s], X + xoff[s], (1 + prm[s].neighbor_margin) * prm[s].thickness, cs, ci);
-    for (int q = 0; q < C.ntile * 256; ++q) {
-      const int e = C.ebase + q;
-      cstart[e] = (int)cids.size();
-      if (C.contact && q < ne[s])
-        for (int k = cs[q]; k < cs[q + 1]; ++k) cids.push_back(C.ebase + ci[k]);
-    }
-  }
-  cstart[nE] = (int)cids.size();
 
-  DPtrs& P = ctx->P;
   P.nE = nE; P.nV = nV; P.S = nshell;
-  int r = 0;
-  double *dX, *drb;
+  double* drb;
   int32_t *dbs, *dbi, *drs, *dns, *dnr;
   uint16_t *dli, *dre;
   uint32_t* drg;
-  int32_t *dp, *dt3, *dis, *di, *des, *dvs, *dcs, *dci, *dvis, *dvi;
+  int32_t *dp, *dt3;
   int8_t* dv;
-  if ((r = upload(ctx, dX, Xh)) || (r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
-      (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dis, inc_start)) ||
-      (r = upload(ctx, di, inc)) || (r = upload(ctx, des, eshell)) || (r = upload(ctx, dvs, vshell)) ||
-      (r = upload(ctx, dcs, cstart)) || (r = upload(ctx, dci, cids)) || (r = upload(ctx, dbs, bn_start)) ||
+  if ((r = upload(ctx, dp, pidx)) || (r = upload(ctx, dv, vcls)) ||
+      (r = upload(ctx, dt3, triS)) || (r = upload(ctx, drb, ctx->rawb)) || (r = upload(ctx, dbs, bn_start)) ||
       (r = upload(ctx, dbi, bn_ids)) || (r = upload(ctx, dli, lidx)) || (r = upload(ctx, drs, rent_start)) ||
       (r = upload(ctx, dre, rent)) || (r = upload(ctx, dns, nrec_start)) || (r = upload(ctx, dnr, nrec)) ||
-      (r = upload(ctx, drg, rent_gcnt)) || (r = upload(ctx, dvis, vinc_start)) || (r = upload(ctx, dvi, vinc)))
+      (r = upload(ctx, drg, rent_gcnt)))
     return fail(r);
-  P.vinc_start = dvis; P.vinc = dvi;
-  P.X = dX; P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb; P.inc_start = dis; P.inc = di;
+  P.pidx = dp; P.vcls = dv; P.tri = dt3; P.rawb = drb;
   P.bn_start = dbs; P.bn_ids = dbi; P.lidx = dli; P.rent_start = drs; P.rent = dre; P.rent_gcnt = drg; P.nrec_start = dns; P.nrec = dnr;
-  P.eshell = des; P.vshell = dvs; P.close_start = dcs; P.close_ids = dci;
   if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
   P.cs = ctx->dC;
   if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * nE)) || (r = dalloc(ctx, P.minv, nV)) ||
@@ -1289,6 +1368,24 @@ int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, do
   return w;
 }
 
+int shx_batch_close_sets(shx_ctx* ctx, int shell, int32_t* start, int32_t* ids, int cap)
+{
+  if (!ctx) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const SConst& C = ctx->C[shell];
+  std::vector<int32_t> cs((size_t)C.ne + 1);
+  HIPCHK(hipMemcpy(cs.data(), ctx->P.close_start + C.ebase, sizeof(int32_t) * cs.size(), hipMemcpyDeviceToHost));
+  const int n = cs[C.ne] - cs[0];
+  if (start)
+    for (int e = 0; e <= C.ne; ++e) start[e] = cs[e] - cs[0];
+  if (ids && cap > 0 && n > 0) {
+    std::vector<int32_t> g(n);
+    HIPCHK(hipMemcpy(g.data(), ctx->P.close_ids + cs[0], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < std::min(n, cap); ++k) ids[k] = g[k] - C.ebase;
+  }
+  return n;
+}
+
 int shx_batch_cavity_springs(shx_ctx* ctx, int shell, double* origin, int io_set)
 {
   if (!ctx || !origin) return SHX_E_ARG;
@@ -1520,13 +1617,16 @@ int shx_mass(shx_ctx* ctx, double* Minv, double* Cd)
   return SHX_OK;
 }
 
-int shx_patches(shx_ctx* ctx, int32_t* idx, int32_t* tri)
+int shx_batch_patches(shx_ctx* ctx, int shell, int32_t* idx, int32_t* tri)
 {
   if (!ctx) return SHX_E_ARG;
-  if (idx) std::copy(ctx->T[0].patch.begin(), ctx->T[0].patch.end(), idx);
-  if (tri) std::copy(ctx->T[0].tri.begin(), ctx->T[0].tri.end(), tri);
+  if (int r = check_shell(ctx, shell)) return r;
+  if (idx) std::copy(ctx->T[shell].patch.begin(), ctx->T[shell].patch.end(), idx);
+  if (tri) std::copy(ctx->T[shell].tri.begin(), ctx->T[shell].tri.end(), tri);
   return SHX_OK;
 }
+
+int shx_patches(shx_ctx* ctx, int32_t* idx, int32_t* tri) { return shx_batch_patches(ctx, 0, idx, tri); }
 
 /* power iteration on M^-1 K for every shell at once: K x by central
  * differences of the internal force (pressure, volume and contact off) at

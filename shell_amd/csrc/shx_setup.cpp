@@ -1,5 +1,5 @@
 /*
- * shx_setup.cpp -- host-side mesh preparation of libshellhip (see
+ * shx_setup.cpp -- host-side Loop basis tables of libshellhip (see
  * shx_setup.h).
  */
 #include "shx_setup.h"
@@ -8,13 +8,10 @@
 #include <algorithm>
 #include <initializer_list>
 #include <cmath>
-#include <unordered_map>
 
 namespace shx {
 
 namespace {
-
-inline uint64_t dkey(int a, int b) { return (uint64_t(uint32_t(a)) << 32) | uint32_t(b); }
 
 /* Twelve quartic box splines of the regular Loop patch (J. Stam, SIGGRAPH
  * 1998 course notes; libMesh FESubdiv::regular_shape) as 12*B(u,v,w) with
@@ -257,132 +254,6 @@ bool subdiv_weights(int valence, double xi, double eta, double w[3][kMaxPatch])
       w[q][j] = f[q] * s;
     }
   return true;
-}
-
-bool build_topology(int nv, int ne, const int32_t* tri_in, Topology& T, std::string& err)
-{
-  T.nv = nv; T.ne = ne;
-  std::vector<int> val(nv, 0);
-  std::unordered_map<uint64_t, int> opp; /* directed edge a->b -> third vertex */
-  opp.reserve(size_t(ne) * 3 * 2);
-  for (int e = 0; e < ne; ++e) {
-    const int* t = tri_in + 3 * e;
-    for (int k = 0; k < 3; ++k) {
-      if (t[k] < 0 || t[k] >= nv) { err = "vertex index out of range"; return false; }
-      val[t[k]]++;
-      const int a = t[k], b = t[(k + 1) % 3], c = t[(k + 2) % 3];
-      if (!opp.emplace(dkey(a, b), c).second) { err = "non-manifold or inconsistently oriented edge"; return false; }
-    }
-  }
-  for (auto& kv : opp)
-    if (!opp.count(dkey(int(kv.first & 0xffffffffu), int(kv.first >> 32)))) { err = "mesh is not closed"; return false; }
-  auto O = [&](int a, int b) { return opp.at(dkey(a, b)); };
-  T.tri.assign(size_t(ne) * 3, 0);
-  T.patch.assign(size_t(ne) * kMaxPatch, -1);
-  T.valence.assign(ne, 6);
-  for (int e = 0; e < ne; ++e) {
-    int t[3] = {tri_in[3 * e], tri_in[3 * e + 1], tri_in[3 * e + 2]};
-    int nirr = 0, at = 0;
-    for (int k = 0; k < 3; ++k) if (val[t[k]] != 6) { ++nirr; at = k; }
-    if (nirr > 1) { err = "triangle with more than one irregular vertex"; return false; }
-    const int v0 = t[at], v1 = t[(at + 1) % 3], v2 = t[(at + 2) % 3];
-    T.tri[3 * e] = v0; T.tri[3 * e + 1] = v1; T.tri[3 * e + 2] = v2;
-    const int N = val[v0];
-    if (N < 3 || N > 6) { err = "unsupported valence"; return false; }
-    T.valence[e] = uint8_t(N);
-    int32_t* p = &T.patch[size_t(e) * kMaxPatch];
-    p[0] = v0;
-    p[1] = v1;
-    for (int k = 2; k <= N; ++k) p[k] = O(p[k - 1], v0);
-    if (p[N] != v2) { err = "one-ring walk did not close"; return false; }
-    const int q11 = O(v2, v1);
-    const int q20 = O(q11, v1);
-    p[N + 1] = q11;
-    p[N + 2] = q20;
-    p[N + 3] = O(q20, v1);
-    p[N + 4] = O(v2, q11);
-    p[N + 5] = O(v2, p[N + 4]);
-  }
-  /* node -> (element, slot) incidence, element order */
-  T.inc_start.assign(nv + 1, 0);
-  for (int e = 0; e < ne; ++e)
-    for (int i = 0; i < kMaxPatch; ++i) {
-      const int n = T.patch[size_t(e) * kMaxPatch + i];
-      if (n >= 0) T.inc_start[n + 1]++;
-    }
-  for (int n = 0; n < nv; ++n) T.inc_start[n + 1] += T.inc_start[n];
-  T.inc.assign(T.inc_start[nv], 0);
-  {
-    std::vector<int32_t> fill(T.inc_start.begin(), T.inc_start.end() - 1);
-    for (int e = 0; e < ne; ++e)
-      for (int i = 0; i < kMaxPatch; ++i) {
-        const int n = T.patch[size_t(e) * kMaxPatch + i];
-        if (n >= 0) T.inc[fill[n]++] = e * 16 + i;
-      }
-  }
-  T.vtx_start.assign(nv + 1, 0);
-  for (int e = 0; e < ne; ++e) for (int k = 0; k < 3; ++k) T.vtx_start[T.tri[3 * e + k] + 1]++;
-  for (int n = 0; n < nv; ++n) T.vtx_start[n + 1] += T.vtx_start[n];
-  T.vtx.assign(T.vtx_start[nv], 0);
-  {
-    std::vector<int32_t> fill(T.vtx_start.begin(), T.vtx_start.end() - 1);
-    for (int e = 0; e < ne; ++e) for (int k = 0; k < 3; ++k) T.vtx[fill[T.tri[3 * e + k]]++] = e;
-  }
-  return true;
-}
-
-void build_close_sets(const Topology& T, const double* X, double cutoff, std::vector<int32_t>& start,
-                      std::vector<int32_t>& ids)
-{
-  const int ne = T.ne;
-  const double cutoff2 = cutoff * cutoff;
-  /* direct neighbours: triangles sharing a vertex */
-  std::vector<std::vector<int>> nb(ne);
-  for (int e = 0; e < ne; ++e) {
-    std::vector<int>& L = nb[e];
-    for (int k = 0; k < 3; ++k) {
-      const int n = T.tri[3 * e + k];
-      for (int q = T.vtx_start[n]; q < T.vtx_start[n + 1]; ++q) if (T.vtx[q] != e) L.push_back(T.vtx[q]);
-    }
-    std::sort(L.begin(), L.end());
-    L.erase(std::unique(L.begin(), L.end()), L.end());
-  }
-  auto corners = [&](int e, v3 P[3]) {
-    for (int k = 0; k < 3; ++k) {
-      const int n = T.tri[3 * e + k];
-      P[k] = mk(X[3 * n], X[3 * n + 1], X[3 * n + 2]);
-    }
-  };
-  start.assign(ne + 1, 0);
-  ids.clear();
-  std::vector<int> close, frontier, next;
-  std::vector<char> mark(ne, 0);
-  for (int e = 0; e < ne; ++e) {
-    close = nb[e];
-    for (int c : close) mark[c] = 1;
-    frontier = close;
-    v3 Pe[3];
-    corners(e, Pe);
-    while (!frontier.empty()) {
-      next.clear();
-      for (int a : frontier)
-        for (int c : nb[a]) if (c != e && !mark[c]) next.push_back(c);
-      std::sort(next.begin(), next.end());
-      next.erase(std::unique(next.begin(), next.end()), next.end());
-      frontier.clear();
-      for (int c : next) {
-        v3 Pc[3];
-        corners(c, Pc);
-        tri_hit hit;
-        tri_tri(Pe, Pc, hit);
-        if (hit.d2 < cutoff2) { mark[c] = 1; close.push_back(c); frontier.push_back(c); }
-      }
-    }
-    std::sort(close.begin(), close.end());
-    for (int c : close) mark[c] = 0;
-    ids.insert(ids.end(), close.begin(), close.end());
-    start[e + 1] = (int32_t)ids.size();
-  }
 }
 
 } // namespace shx
