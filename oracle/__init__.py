@@ -57,7 +57,9 @@ class Params(ctypes.Structure):
         "rebuild_trigger", "cavity_contact", "cavity", "gravity_dir", "pad0")] + [
         ("cavity_R0", ctypes.c_double * 3)] + [(n, ctypes.c_double) for n in (
         "cavity_rate", "mu_s_cavity", "mu_d_cavity", "cavity_damping", "cavity_friction_damping", "gravity",
-        "damping_rotation")]
+        "damping_rotation")] + [(n, ctypes.c_int32) for n in ("growth_base", "thickness_dir", "grow_mass", "pad1")] + [
+        (n, ctypes.c_double * 2) for n in ("growth_g0", "growth_gt", "growth_gx", "growth_gy", "growth_gz")] + [
+        ("metric_growth_rate", ctypes.c_double), ("curvature_growth_rate", ctypes.c_double)]
 
 
 class Report(ctypes.Structure):
@@ -96,6 +98,7 @@ def _declare(L):
     L.orc_measure.argtypes = [_V]
     L.orc_properties.argtypes = [_V, _D]
     L.orc_node_cavity.argtypes = [_V, _D, _D, _D]
+    L.orc_elem_growth.argtypes = [_V, _D]
     L.orc_elem_measure.argtypes = [_V, _D]
     L.orc_contact_rebuild.argtypes = [_V, ctypes.c_int]
     L.orc_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
@@ -148,6 +151,8 @@ def make_params(**kw):
     for k, v in defaults.items():
         if k == "cavity_R0":
             v = (ctypes.c_double * 3)(*v)
+        elif k.startswith("growth_g"):
+            v = (ctypes.c_double * 2)(*v)
         setattr(p, k, v)
     return p
 
@@ -164,7 +169,10 @@ def default_params():
                 quirk_volume_schedule=0, omp_faithful=0, volume_at_prediction=0,
                 rebuild_trigger=0, cavity_contact=0, cavity=2, gravity_dir=2, pad0=0,
                 cavity_R0=(1.0, 1.0, 1.0), cavity_rate=0.0, mu_s_cavity=0.0, mu_d_cavity=0.0,
-                cavity_damping=0.0, cavity_friction_damping=0.0, gravity=0.0, damping_rotation=0.0)
+                cavity_damping=0.0, cavity_friction_damping=0.0, gravity=0.0, damping_rotation=0.0,
+                growth_base=0, thickness_dir=2, grow_mass=1, pad1=0, growth_g0=(1.0, 1.0), growth_gt=(0.0, 0.0),
+                growth_gx=(0.0, 0.0), growth_gy=(0.0, 0.0), growth_gz=(0.0, 0.0), metric_growth_rate=0.0,
+                curvature_growth_rate=0.0)
 
 
 class Oracle:
@@ -255,6 +263,12 @@ class Oracle:
         out = np.zeros(16)
         self.L.orc_properties(self.h, dp(out))
         return out
+
+    def elem_growth(self):
+        """per element G (3), GV (3 x 3 row-major), alpha (3), beta (3)"""
+        out = np.zeros((self.ne, 18))
+        self.L.orc_elem_growth(self.h, dp(out))
+        return out[:, :3], out[:, 3:12].reshape(-1, 3, 3), out[:, 12:15], out[:, 15:18]
 
     def node_cavity(self):
         """per node: cavity spring origin (nv x 3), cavity and cavity-friction

@@ -279,6 +279,9 @@ struct ElemCache {
   SurfaceMetric a_ref, a_def;
   double bmin[3], bmax[3];   /* BoundingBox<Real> */
   std::vector<int> neighbor_elems, close_elems; /* sorted */
+  /* growth: G (Voigt), its value before the attempt, GV (row-major), the last strains */
+  double G[3] = {1, 1, 0}, G_old[3] = {1, 1, 0}, GV[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double alpha[3] = {0, 0, 0}, beta[3] = {0, 0, 0};
 };
 
 struct Contact {
@@ -323,6 +326,9 @@ struct System {
   std::vector<double> ecav_node, efric_node;
   int Nc_cav = 0;
   double Ecav = 0, cavity_force = 0;
+
+  int grow = 0;                    /* 0 none, 1 static, 2 dynamic */
+  int growth_dir[3] = {0, 1, 2};
 
   void reg(int pr) { if (pr > problem) problem = pr; }
 };
@@ -577,13 +583,21 @@ void compute_mass_damping(System& s)
   const orc_params& P = s.prm;
   s.Minv.assign(3 * s.nv, 0.0);
   s.C.assign(3 * s.nv, 0.0);
+  const bool gm = s.grow && P.grow_mass;
   for (int e = 0; e < s.ne; ++e) {
     const ElemCache& c = s.ec[e];
     double Me[12], Ce[12];
+    /* with mass growth the weight is JxW det G (ShellSystem.C:795-811) */
+    double JxW = c.JxW, avg_growth = 1;
+    if (gm) {
+      const double det = c.G[0] * c.G[1] - 0.25 * c.G[2] * c.G[2];
+      JxW = c.JxW * det;
+      avg_growth = det;
+    }
     for (int i = 0; i < c.N; ++i) {
       double lumped = 0;
       for (int j = 0; j < c.N; ++j) lumped += c.phi[j];
-      lumped *= c.JxW * c.phi[i];
+      lumped *= JxW * c.phi[i];
       Me[i] = lumped;
     }
     for (int i = 0; i < c.N; ++i) Me[i] *= P.rho * P.thickness;
@@ -594,7 +608,7 @@ void compute_mass_damping(System& s)
         const double d = (s.X[s.tri[3 * e + q]] - s.X[s.tri[3 * e + r]]).size();
         hmin = std::min(hmin, d); hmax = std::max(hmax, d);
       }
-    const double avg_stiff = 2 * (s.K + s.D / (hmin * hmax)) / 3;
+    const double avg_stiff = 2 * (s.K + s.D / (avg_growth * hmin * hmax)) / 3;
     for (int i = 0; i < c.N; ++i) Ce[i] = P.damping_alpha * Me[i] + P.damping_beta * avg_stiff;
     for (int i = 0; i < c.N; ++i)
       for (int q = 0; q < 3; ++q) {
@@ -650,15 +664,35 @@ void deformed_metrics_elem(System& s, int e, const double* up)
 }
 
 /* element internal force qe (3 x N) and energies, ShellSystem.C:1164-1300 */
-void internal_force_elem(const System& s, int e, double qe[3][12], double& Et, double& Eb)
+void internal_force_elem(System& s, int e, double qe[3][12], double& Et, double& Eb)
 {
   const orc_params& P = s.prm;
-  const ElemCache& c = s.ec[e];
+  ElemCache& c = s.ec[e];
   const SurfaceMetric& ar = c.a_ref;
   const SurfaceMetric& ad = c.a_def;
   const double JxW = c.JxW;
-  double alpha[3], beta[3];
-  for (int q = 0; q < 3; ++q) { alpha[q] = 0.5 * (ad.ff1[q] - ar.ff1[q]); beta[q] = ar.ff2[q] - ad.ff2[q]; }
+  double* alpha = c.alpha;
+  double* beta = c.beta;
+  double GVT[9];
+  if (s.grow) {
+    /* alpha = (GV ff1 - ff1_ref) / 2, beta = ff2_ref - GV ff2, and the
+     * Voigt-corrected transpose GVT (ShellSystem.C:1176-1186) */
+    const double* G = c.GV;
+    for (int q = 0; q < 3; ++q) {
+      const double f1 = G[3 * q] * ad.ff1[0] + G[3 * q + 1] * ad.ff1[1] + G[3 * q + 2] * ad.ff1[2];
+      const double f2 = G[3 * q] * ad.ff2[0] + G[3 * q + 1] * ad.ff2[1] + G[3 * q + 2] * ad.ff2[2];
+      alpha[q] = 0.5 * (f1 - ar.ff1[q]);
+      beta[q] = ar.ff2[q] - f2;
+    }
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) GVT[3 * r + q] = G[3 * q + r];
+    GVT[2] *= 2;
+    GVT[5] *= 2;
+    GVT[6] *= 0.5;
+    GVT[7] *= 0.5;
+  } else {
+    for (int q = 0; q < 3; ++q) { alpha[q] = 0.5 * (ad.ff1[q] - ar.ff1[q]); beta[q] = ar.ff2[q] - ad.ff2[q]; }
+  }
   alpha[2] *= 2;
   beta[2] *= 2;
   const double* g = ar.ff1;
@@ -697,6 +731,21 @@ void internal_force_elem(const System& s, int e, double qe[3][12], double& Et, d
     const Pt bb0 = ad.vec[2] * -c.d2phi[i][0] + (d0 * p11 + d1 * p21);
     const Pt bb1 = ad.vec[2] * -c.d2phi[i][1] + (d0 * p12 + d1 * p22);
     const Pt bb2 = 2 * (ad.vec[2] * -c.d2phi[i][2] + (d0 * p13 + d1 * p23));
+    if (s.grow) {
+      /* BmT = BmT GVT, BbT = BbT GVT (ShellSystem.C:1270-1275) */
+      const Pt bm[3] = {bm0, bm1, bm2}, bb[3] = {bb0, bb1, bb2};
+      Pt bmg[3], bbg[3];
+      for (int k = 0; k < 3; ++k) {
+        bmg[k] = bm[0] * GVT[k] + bm[1] * GVT[3 + k] + bm[2] * GVT[6 + k];
+        bbg[k] = bb[0] * GVT[k] + bb[1] * GVT[3 + k] + bb[2] * GVT[6 + k];
+      }
+      for (int q = 0; q < 3; ++q) {
+        const double fm = bmg[0](q) * nn[0] + bmg[1](q) * nn[1] + bmg[2](q) * nn[2];
+        const double fb = bbg[0](q) * mm[0] + bbg[1](q) * mm[1] + bbg[2](q) * mm[2];
+        qe[q][i] = JxW * (fm + fb);
+      }
+      continue;
+    }
     for (int q = 0; q < 3; ++q) {
       const double fm = bm0(q) * nn[0] + bm1(q) * nn[1] + bm2(q) * nn[2];
       const double fb = bb0(q) * mm[0] + bb1(q) * mm[1] + bb2(q) * mm[2];
@@ -705,6 +754,101 @@ void internal_force_elem(const System& s, int e, double qe[3][12], double& Et, d
   }
   Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
   Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
+}
+
+/* apply_growth (ShellSystem.C:2295-2440) at time t.  increment: dynamic
+ * growth steps G (G_old kept for reject_growth); without it (a plain force
+ * evaluation) GV is formed from the current G */
+void apply_growth(System& s, double t, bool increment)
+{
+  const orc_params& P = s.prm;
+  for (int e = 0; e < s.ne; ++e) {
+    ElemCache& c = s.ec[e];
+    const SurfaceMetric& ar = c.a_ref;
+    double Ginv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (s.grow == 2) {
+      if (increment) {
+        for (int q = 0; q < 3; ++q) c.G_old[q] = c.G[q];
+        const double fm = s.dt * P.metric_growth_rate, fc = s.dt * P.curvature_growth_rate;
+        for (int q = 0; q < 3; ++q) c.G[q] += c.alpha[q] * fm;
+        for (int q = 0; q < 3; ++q) c.G[q] += c.beta[q] * fc;
+      }
+      const double det = c.G[0] * c.G[1] - 0.25 * c.G[2] * c.G[2];
+      Ginv[0] = c.G[1] / det;
+      Ginv[4] = c.G[0] / det;
+      Ginv[1] = Ginv[3] = -0.5 * c.G[2] / det;
+      Ginv[8] = 1;
+    } else {
+      double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      if (P.growth_base == 0) {
+        for (int i = 0; i < 3; ++i) {
+          const double inv_length = 1 / ar.vec[i].size();
+          for (int r = 0; r < 3; ++r) B[3 * r + i] = inv_length * ar.vec[i](r);
+        }
+      } else if (P.growth_base == 1) {
+        B[0] = B[4] = B[8] = 1;
+      } else {
+        const double inv_r = 1 / std::sqrt(ar.xyz(1) * ar.xyz(1) + ar.xyz(2) * ar.xyz(2));
+        if (!std::isfinite(inv_r)) {
+          for (int q = 0; q < 9; ++q) c.GV[q] = (q % 4 == 0) ? 1 : 0;
+          continue;
+        }
+        B[0] = 1;
+        B[4] = inv_r * ar.xyz(1);
+        B[7] = inv_r * ar.xyz(2);
+        B[5] = -inv_r * ar.xyz(2);
+        B[8] = inv_r * ar.xyz(1);
+      }
+      double A[9];
+      for (int i = 0; i < 3; ++i)
+        for (int r = 0; r < 3; ++r) A[3 * r + i] = ar.vec[i](r);
+      for (int r = 0; r < 3; ++r) A[3 * r + 2] *= ar.jac;
+      const Pt& x = ar.xyz;
+      double g[2];
+      for (int k = 0; k < 2; ++k)
+        g[k] = P.growth_g0[k] + P.growth_gt[k] * t + P.growth_gx[k] * x(0) + P.growth_gy[k] * x(1) +
+               P.growth_gz[k] * x(2);
+      c.G[0] = g[0];
+      c.G[1] = g[1];
+      Ginv[s.growth_dir[0] * 4] = 1 / g[0];
+      Ginv[s.growth_dir[1] * 4] = 1 / g[1];
+      c.G[2] = 0;
+      Ginv[s.growth_dir[2] * 4] = 1;
+      double Bi[9], BA[9], Ai[9], T1[9];
+      inverse3(B, Bi);
+      matmul3(Bi, A, BA);
+      inverse3(BA, Ai);
+      matmul3(Ai, Ginv, T1);
+      matmul3(T1, BA, Ginv);
+    }
+    auto gi = [&](int r, int q) { return Ginv[3 * r + q]; };
+    double* GV = c.GV;
+    GV[0] = gi(0, 0) * gi(0, 0);
+    GV[1] = gi(1, 0) * gi(1, 0);
+    GV[2] = 2 * gi(0, 0) * gi(1, 0);
+    GV[3] = gi(0, 1) * gi(0, 1);
+    GV[4] = gi(1, 1) * gi(1, 1);
+    GV[5] = 2 * gi(0, 1) * gi(1, 1);
+    GV[6] = gi(0, 0) * gi(0, 1);
+    GV[7] = gi(1, 0) * gi(1, 1);
+    GV[8] = gi(0, 1) * gi(1, 0) + gi(0, 0) * gi(1, 1);
+  }
+}
+
+/* reject_growth (ShellSystem.C:2443-2459) */
+void reject_growth(System& s)
+{
+  for (auto& c : s.ec)
+    for (int q = 0; q < 3; ++q) c.G[q] = c.G_old[q];
+}
+
+/* the part of prepare_next_step (ShellSystem.C:945-948) that growth adds:
+ * apply_growth, and the masses when they follow the growth */
+void prepare_growth(System& s, double t, bool increment)
+{
+  if (!s.grow) return;
+  apply_growth(s, t, increment);
+  if (s.prm.grow_mass) compute_mass_damping(s);
 }
 
 double pressure_at(const System& s, double t)
@@ -1268,12 +1412,18 @@ double rigid_free_norm(const System& s, const std::vector<double>& f)
  * evaluation, w = jac * qw (qw = 1/2), kGauss += w det(ff2) / det(ff1),
  * kMean += w (ff2_0 ff1_1 + ff2_1 ff1_0 - 2 ff2_2 ff1_2) / (2 det(ff1)),
  * area = sum of the element areas; summed in element order */
-void elem_curvature(const SurfaceMetric& ad, double& kG, double& kM)
+void elem_curvature(const SurfaceMetric& ad, double& kG, double& kM, const double* GV = nullptr)
 {
+  double ff1[3], ff2[3];
+  for (int q = 0; q < 3; ++q) {
+    /* growth-enhanced fundamental forms (ShellSystem.C:2745-2756) */
+    ff1[q] = GV ? GV[3 * q] * ad.ff1[0] + GV[3 * q + 1] * ad.ff1[1] + GV[3 * q + 2] * ad.ff1[2] : ad.ff1[q];
+    ff2[q] = GV ? GV[3 * q] * ad.ff2[0] + GV[3 * q + 1] * ad.ff2[1] + GV[3 * q + 2] * ad.ff2[2] : ad.ff2[q];
+  }
   const double w = ad.jac * 0.5;
-  const double invdet = 1 / (ad.ff1[0] * ad.ff1[1] - ad.ff1[2] * ad.ff1[2]);
-  kG = w * invdet * (ad.ff2[0] * ad.ff2[1] - ad.ff2[2] * ad.ff2[2]);
-  kM = w * invdet * 0.5 * (ad.ff2[0] * ad.ff1[1] + ad.ff2[1] * ad.ff1[0] - 2 * ad.ff2[2] * ad.ff1[2]);
+  const double invdet = 1 / (ff1[0] * ff1[1] - ff1[2] * ff1[2]);
+  kG = w * invdet * (ff2[0] * ff2[1] - ff2[2] * ff2[2]);
+  kM = w * invdet * 0.5 * (ff2[0] * ff1[1] + ff2[1] * ff1[0] - 2 * ff2[2] * ff1[2]);
 }
 
 void measure_curvature(const System& s, double& kG, double& kM, double& area)
@@ -1281,7 +1431,7 @@ void measure_curvature(const System& s, double& kG, double& kM, double& area)
   kG = 0; kM = 0; area = 0;
   for (int e = 0; e < s.ne; ++e) {
     double g, m;
-    elem_curvature(s.ec[e].a_def, g, m);
+    elem_curvature(s.ec[e].a_def, g, m, s.grow ? s.ec[e].GV : nullptr);
     kG += g;
     kM += m;
     area += s.ec[e].area;
@@ -1329,6 +1479,7 @@ void advance(System& s, orc_report* rep)
   double fnorm = 0, fext_norm = 0, frigid = 0;
   while (true) {
     s.t += s.dt;
+    prepare_growth(s, s.t, true);
     const double* A = s.newmark_a;
     for (int i = 0; i < n3; ++i) {
       double x = s.u[i];
@@ -1382,6 +1533,7 @@ void advance(System& s, orc_report* rep)
       break;
     }
     ++s.n_rejected;
+    if (s.grow == 2) reject_growth(s);
     s.t -= s.dt;
     set_newmark(s, new_dt);
     if (s.dt <= std::numeric_limits<double>::epsilon()) { s.reg(SMALL_TIMESTEP); break; }
@@ -1447,6 +1599,18 @@ void* orc_create(int nv, int ne, const double* X, const int32_t* tri, const orc_
   if (!build_topology(*s, tri)) { delete s; return nullptr; }
   update_derived(*s);
   set_newmark(*s, prm->dt);
+  {
+    /* set_parameters: growth kind and growth_dir (ShellSystem.C:246-250, 346-353) */
+    const bool stat = prm->growth_g0[0] != 1 || prm->growth_g0[1] != 1 || prm->growth_gt[0] != 0 ||
+                      prm->growth_gt[1] != 0 || prm->growth_gx[0] != 0 || prm->growth_gx[1] != 0 ||
+                      prm->growth_gy[0] != 0 || prm->growth_gy[1] != 0 || prm->growth_gz[0] != 0 ||
+                      prm->growth_gz[1] != 0;
+    const bool dyn = prm->metric_growth_rate != 0 || prm->curvature_growth_rate != 0;
+    s->grow = dyn ? 2 : stat ? 1 : 0;
+    s->growth_dir[0] = prm->thickness_dir < 1 ? 1 : 0;
+    s->growth_dir[1] = prm->thickness_dir < 2 ? 2 : 1;
+    s->growth_dir[2] = prm->thickness_dir;
+  }
   build_cache(*s);
   compute_mass_damping(*s);
   const int n3 = 3 * nv;
@@ -1538,6 +1702,7 @@ int orc_forces(void* h, const double* up, const double* v, double t, double* f, 
   std::copy(up, up + n3, s.u_pred.begin());
   if (v) std::copy(v, v + n3, s.v.begin());
   compute_position(s, s.u_pred);
+  prepare_growth(s, t, false);
   evaluate_forces(s, t);
   if (f) std::copy(s.f.begin(), s.f.end(), f);
   if (fext) std::copy(s.fext.begin(), s.fext.end(), fext);
@@ -1574,6 +1739,7 @@ int orc_init_dynamics(void* h)
   compute_enclosed_volume(s);
   s.volume0 = s.volume;
   contact_candidates(s, true, s.u_pred);
+  prepare_growth(s, s.t + s.dt, true);
   evaluate_forces(s, s.t + s.dt);
   for (int i = 0; i < 3 * s.nv; ++i) s.a[i] = s.Minv[i] * s.f[i];
   /* reset_properties (ShellSystem.C:876) */
@@ -1592,7 +1758,7 @@ int orc_elem_measure(void* h, double* out /* ne*10 */)
     double* o = out + 10 * (size_t)e;
     for (int q = 0; q < 3; ++q) { o[q] = ad.ff1[q]; o[3 + q] = ad.ff2[q]; }
     o[6] = ad.jac;
-    elem_curvature(ad, o[7], o[8]);
+    elem_curvature(ad, o[7], o[8], s.grow ? s.ec[e].GV : nullptr);
     o[9] = s.ec[e].area;
   }
   return 0;
@@ -1623,6 +1789,18 @@ int orc_properties(void* h, double* out)
   for (int q = 0; q < 16; ++q) out[q] = vals[q];
   for (double& x : s.avg) x = 0;
   s.avg_n = 0;
+  return 0;
+}
+
+int orc_elem_growth(void* h, double* out)
+{
+  System& s = *static_cast<System*>(h);
+  for (int e = 0; e < s.ne; ++e) {
+    const ElemCache& c = s.ec[e];
+    double* o = out + 18 * (size_t)e;
+    for (int q = 0; q < 3; ++q) { o[q] = c.G[q]; o[12 + q] = c.alpha[q]; o[15 + q] = c.beta[q]; }
+    for (int q = 0; q < 9; ++q) o[3 + q] = c.GV[q];
+  }
   return 0;
 }
 

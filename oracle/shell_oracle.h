@@ -60,6 +60,12 @@ typedef struct {
                                       (Rx = 0.1*exp(-4*t)); 0: constant */
   double mu_s_cavity, mu_d_cavity, cavity_damping, cavity_friction_damping;
   double gravity, damping_rotation;
+  /* in-plane growth (apply_growth, ShellSystem.C:2295-2459): static growth factors of the form
+     g_k(x, y, z, t) = g0 + gt t + gx x + gy y + gz z at the element's reference point, or dynamic growth
+     by the rates; growth_base 0 local, 1 cartesian, 2 cylindrical */
+  int32_t growth_base, thickness_dir, grow_mass, pad1;
+  double growth_g0[2], growth_gt[2], growth_gx[2], growth_gy[2], growth_gz[2];
+  double metric_growth_rate, curvature_growth_rate;
 } orc_params;
 
 typedef struct {
@@ -119,6 +125,9 @@ int  orc_properties(void* h, double* out /* 16 */);
  * the cavity energy and friction energy over the node's mean element area
  * (the .vtu fields ecav and efric, ShellSystem.C:1626-1706) */
 int  orc_node_cavity(void* h, double* origin /* nv*3 */, double* ecav /* nv */, double* efric /* nv */);
+/* per element: the growth vector G (3), the Voigt inverse growth tensor GV (9, row-major) and the last
+ * strains alpha, beta (6): 18 values */
+int  orc_elem_growth(void* h, double* out /* ne*18 */);
 int  orc_elem_measure(void* h, double* out /* ne*10: ff1, ff2, jac, kGauss, kMean, area */);
 int  orc_contact_rebuild(void* h, int force);
 int  orc_pairs(void* h, uint32_t* e1, uint32_t* e2, int cap);   /* returns count, sorted */

@@ -61,6 +61,7 @@ __device__ __forceinline__ int rec_of(int t, int m) { return m == 0 ? t : 2 * kE
  * list was rebuilt at the end of the previous step clear its cell buckets.
  * Launch block b computes element block bmap[V_ELEM][b] (the blocks of the
  * active shells, set by the host). */
+template <bool GROW>
 __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
 {
   __shared__ double s_rb[4 * 72];
@@ -68,7 +69,7 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
   __shared__ double s_buf[kElemLds];     /* staged nodes, then per element three q_i and JxW a3 */
   __shared__ uint16_t s_ent[kMaxBlockEnt], s_nst[kMaxBlockNodes];
   __shared__ uint32_t s_ngc[kMaxBlockNodes];
-  const int blk = uni(P.bmap[V_ELEM][blockIdx.x]);
+  const int blk = uni(P.bmap[GROW ? kMapElemGrow : V_ELEM][blockIdx.x]);
   const int sid = uni(P.eshell[blk * kElemPerBlock]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
@@ -165,10 +166,28 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
     const v3 a12 = add(A12, du12);
     const double ff1[3] = {dot(a1, a1), dot(a2, a2), dot(a1, a2)};
     const double ff2[3] = {dot(a11, a3), dot(a22, a3), dot(a12, a3)};
-    double alpha[3] = {0.5 * (ff1[0] - g.x), 0.5 * (ff1[1] - g.y), 0.5 * (ff1[2] - g.z)};
-    double beta[3] = {fr2.x - ff2[0], fr2.y - ff2[1], fr2.z - ff2[2]};
+    /* gf1, gf2: the forms the strains and the curvature measurement see; with
+     * in-plane growth the deformed forms through the inverse growth tensor,
+     * GV ff (ShellSystem.C:1176-1186, 2747-2756); the p-vectors keep ff2 */
+    double gf1[3] = {ff1[0], ff1[1], ff1[2]}, gf2[3] = {ff2[0], ff2[1], ff2[2]};
+    double GV[9];
+    const bool grows = GROW && C.grow;
+    if (grows) {
+      for (int q = 0; q < 9; ++q) GV[q] = P.gGV[(size_t)q * nE + e];
+      for (int r = 0; r < 3; ++r) {
+        gf1[r] = GV[3 * r] * ff1[0] + GV[3 * r + 1] * ff1[1] + GV[3 * r + 2] * ff1[2];
+        gf2[r] = GV[3 * r] * ff2[0] + GV[3 * r + 1] * ff2[1] + GV[3 * r + 2] * ff2[2];
+      }
+    }
+    double alpha[3] = {0.5 * (gf1[0] - g.x), 0.5 * (gf1[1] - g.y), 0.5 * (gf1[2] - g.z)};
+    double beta[3] = {fr2.x - gf2[0], fr2.y - gf2[1], fr2.z - gf2[2]};
     alpha[2] *= 2;
     beta[2] *= 2;
+    if (grows && C.grow == 2 && !sh.measure_only)
+      for (int k = 0; k < 3; ++k) {
+        P.gstr[(size_t)k * nE + e] = alpha[k];
+        P.gstr[(size_t)(3 + k) * nE + e] = beta[k];
+      }
     /* orthotropic H, ShellSystem.C:1199-1213 */
     double H[9];
     H[0] = C.E1 * g.y * g.y;
@@ -184,6 +203,20 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
       nn[r] = C.K * (H[r * 3] * alpha[0] + H[r * 3 + 1] * alpha[1] + H[r * 3 + 2] * alpha[2]);
       mm[r] = C.D * (H[r * 3] * beta[0] + H[r * 3 + 1] * beta[1] + H[r * 3 + 2] * beta[2]);
     }
+    const double JxW = T[(size_t)T_JXW * nE + e];
+    Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
+    Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
+    if (grows) {
+      /* Bm^T GVT n + Bb^T GVT m with GVT the Voigt-corrected transpose of GV
+       * (ShellSystem.C:1182-1186, 1270-1275): the resultants carried by GVT */
+      const double GT[9] = {GV[0], GV[3], 2 * GV[6], GV[1], GV[4], 2 * GV[7], 0.5 * GV[2], 0.5 * GV[5], GV[8]};
+      double n2[3], m2[3];
+      for (int r = 0; r < 3; ++r) {
+        n2[r] = GT[3 * r] * nn[0] + GT[3 * r + 1] * nn[1] + GT[3 * r + 2] * nn[2];
+        m2[r] = GT[3 * r] * mm[0] + GT[3 * r + 1] * mm[1] + GT[3 * r + 2] * mm[2];
+      }
+      for (int r = 0; r < 3; ++r) { nn[r] = n2[r]; mm[r] = m2[r]; }
+    }
     const v3 a2xa3 = crs(a2, a3), a3xa1 = crs(a3, a1);
     const double invjac = 1 / jac;
     const v3 p11 = scl(add(crs(a11, a2), scl(a2xa3, ff2[0])), invjac);
@@ -192,7 +225,6 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
     const v3 p21 = scl(add(crs(a1, a11), scl(a3xa1, ff2[0])), invjac);
     const v3 p22 = scl(add(crs(a1, a22), scl(a3xa1, ff2[1])), invjac);
     const v3 p23 = scl(add(crs(a1, a12), scl(a3xa1, ff2[2])), invjac);
-    const double JxW = T[(size_t)T_JXW * nE + e];
     const double m2x2 = 2 * mm[2];
     v3 g1 = add(add(add(add(scl(a1, nn[0]), scl(a2, nn[2])), scl(p11, mm[0])), scl(p12, mm[1])), scl(p13, m2x2));
     v3 g2 = add(add(add(add(scl(a2, nn[1]), scl(a1, nn[2])), scl(p21, mm[0])), scl(p22, mm[1])), scl(p23, m2x2));
@@ -201,17 +233,15 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
     /* G0 and G3..G5 are a3 scaled: kept as a3 and the four scales */
     ka3 = a3; kG1 = G[1]; kG2 = G[2];
     kJ = JxW; ks3 = JxW * mm[0]; ks4 = JxW * mm[1]; ks5 = JxW * m2x2;
-    Et = 0.5 * JxW * (nn[0] * alpha[0] + nn[1] * alpha[1] + nn[2] * alpha[2]);
-    Eb = 0.5 * JxW * (mm[0] * beta[0] + mm[1] * beta[1] + mm[2] * beta[2]);
     const double new_area = 0.5 * jac;
     oarea = (mode == M_INIT || C.volume_at_prediction) ? new_area : P.area[e];
     P.area[e] = new_area;
     narea = new_area;
     /* curvatures of the deformed surface, measure_properties (ShellSystem.C:
      * 2746-2751): weight jac * qw (qw = 1/2), over det(ff1) */
-    const double wd = new_area / (ff1[0] * ff1[1] - ff1[2] * ff1[2]);
-    kgs = wd * (ff2[0] * ff2[1] - ff2[2] * ff2[2]);
-    kmn = wd * 0.5 * (ff2[0] * ff1[1] + ff2[1] * ff1[0] - 2 * ff2[2] * ff1[2]);
+    const double wd = new_area / (gf1[0] * gf1[1] - gf1[2] * gf1[2]);
+    kgs = wd * (gf2[0] * gf2[1] - gf2[2] * gf2[2]);
+    kmn = wd * 0.5 * (gf2[0] * gf1[1] + gf2[1] * gf1[0] - 2 * gf2[2] * gf1[2]);
     if (mode == M_FORCES) {
       /* the per-element measurement vectors (ShellSystem.C:1299-1300, 2783-2784) */
       double* em = P.emeas + 4 * (size_t)e;
@@ -300,8 +330,10 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
 }
 
 /* launch-shape variants (waves per SIMD), picked by the context's shx_shapes */
-__global__ void __launch_bounds__(kElemBlock, 3) k_elem_w3(DPtrs P, int mode) { elem_body(P, mode); }
-__global__ void __launch_bounds__(kElemBlock, 4) k_elem_w4(DPtrs P, int mode) { elem_body(P, mode); }
+__global__ void __launch_bounds__(kElemBlock, 3) k_elem_w3(DPtrs P, int mode) { elem_body<false>(P, mode); }
+__global__ void __launch_bounds__(kElemBlock, 4) k_elem_w4(DPtrs P, int mode) { elem_body<false>(P, mode); }
+/* with in-plane growth (batches where a shell grows) */
+__global__ void __launch_bounds__(kElemBlock, 3) k_elem_grow(DPtrs P, int mode) { elem_body<true>(P, mode); }
 
 /* FP32 split element force: the same method as k_elem, with the metric and
  * curvature changes formed in FP32 from the displacement gradients relative

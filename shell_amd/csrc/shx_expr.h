@@ -10,9 +10,10 @@
  * with numbers, the variable t, the constants pi, _pi, _e, and the
  * functions sin cos tan asin acos atan sinh cosh tanh asinh acosh atanh
  * exp ln log (base 10) log10 log2 sqrt abs sign rint, min(a,b), max(a,b).
- * An expression that uses d, x, y or z (density and position, which the
- * reference also binds) is refused: on this path every time function is
- * uniform over the shell.
+ * The position variables x, y, z (MAKE_FUNCTION, macros.h:56-58) are
+ * accepted where the caller allows them: the growth factors, which the
+ * reference evaluates at each element's reference point (ShellSystem.C:
+ * 2401-2404).  The density d is refused.
  */
 #ifndef SHX_EXPR_H
 #define SHX_EXPR_H
@@ -32,7 +33,7 @@ constexpr int kExprConsts = 24;   /* numeric literals of one program */
 constexpr int kExprStack = 16;    /* evaluation stack depth */
 
 enum ExprOp : unsigned char {
-  EX_END = 0, EX_CONST, EX_T,
+  EX_END = 0, EX_CONST, EX_T, EX_X, EX_Y, EX_Z,
   EX_ADD, EX_SUB, EX_MUL, EX_DIV, EX_POW, EX_NEG,
   EX_LT, EX_GT, EX_LE, EX_GE, EX_EQ, EX_NE, EX_AND, EX_OR, EX_SEL,
   EX_SIN, EX_COS, EX_TAN, EX_ASIN, EX_ACOS, EX_ATAN, EX_SINH, EX_COSH, EX_TANH,
@@ -43,13 +44,15 @@ enum ExprOp : unsigned char {
 /* a compiled expression; n == 0: empty (the caller's default applies) */
 struct ExprProg {
   int n;                          /* operations */
-  int uses_t;                     /* 0: constant */
+  int uses_t;                     /* 0: constant in time */
+  int uses_xyz;                   /* 1: depends on the position (x, y, z) */
   unsigned char op[kExprOps];
   unsigned char arg[kExprOps];    /* EX_CONST: index into k */
   double k[kExprConsts];
 };
 
-SHX_EXPR_HD double expr_eval(const ExprProg& p, double t)
+/* value at time t and position (x, y, z) */
+SHX_EXPR_HD double expr_eval_at(const ExprProg& p, double t, double x, double y, double z)
 {
   double s[kExprStack];
   int sp = 0;
@@ -57,6 +60,9 @@ SHX_EXPR_HD double expr_eval(const ExprProg& p, double t)
     const unsigned char o = p.op[i];
     if (o == EX_CONST) { s[sp++] = p.k[p.arg[i]]; continue; }
     if (o == EX_T) { s[sp++] = t; continue; }
+    if (o == EX_X) { s[sp++] = x; continue; }
+    if (o == EX_Y) { s[sp++] = y; continue; }
+    if (o == EX_Z) { s[sp++] = z; continue; }
     if (o == EX_SEL) {            /* c ? a : b, both branches evaluated (pure functions) */
       const double b = s[--sp], a = s[--sp], c = s[--sp];
       s[sp++] = c != 0 ? a : b;
@@ -118,10 +124,12 @@ SHX_EXPR_HD double expr_eval(const ExprProg& p, double t)
   return sp > 0 ? s[sp - 1] : 0.0;
 }
 
+SHX_EXPR_HD double expr_eval(const ExprProg& p, double t) { return expr_eval_at(p, t, 0.0, 0.0, 0.0); }
+
 /* host: compile `text`; false with a message in err (errlen bytes) on a
- * syntax error, an unknown name, a variable other than t, or a program
- * beyond the fixed capacity */
-bool expr_compile(const char* text, ExprProg& out, char* err, int errlen);
+ * syntax error, an unknown name, a variable other than t (and x, y, z when
+ * positions is true), or a program beyond the fixed capacity */
+bool expr_compile(const char* text, ExprProg& out, char* err, int errlen, bool positions = false);
 
 } // namespace shx
 #endif

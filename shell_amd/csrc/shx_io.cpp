@@ -13,7 +13,11 @@
  *  - the reference's load reads the Nc column twice (the Nc_cav column into
  *    avg_Nc, ShellSystem.C:3089-3090); this reader takes the columns as
  *    save writes them;
- *  - growth data are absent (no growth on this path).
+ *  - dynamic growth data (per element alpha, beta, G) follow the sticking
+ *    contacts as in the reference; after a load, the growth state and the
+ *    cavity springs are set after the initialisation, as load leaves them
+ *    (the reference's initialisation does not evaluate forces on a loaded
+ *    state).
  */
 #include <cmath>
 #include <cstdio>
@@ -86,6 +90,17 @@ int shx_state_save(shx_ctx* ctx, int shell, const char* path, uint32_t time_step
   for (int k = 0; k < np; ++k)
     if (st[k])
       f << e1[k] << "\t" << e2[k] << "\t" << so[3 * k] << "\t" << so[3 * k + 1] << "\t" << so[3 * k + 2] << "\n";
+  shx_params prm;
+  if (int r = shx_batch_params(ctx, shell, &prm)) return io_fail(shx_last_error(ctx), r);
+  if (prm.metric_growth_rate != 0 || prm.curvature_growth_rate != 0) {
+    std::vector<double> g(9 * (size_t)ne);
+    if (int r = shx_batch_growth_state(ctx, shell, g.data(), 0)) return io_fail(shx_last_error(ctx), r);
+    for (int e = 0; e < ne; ++e) {
+      const double* q = &g[9 * (size_t)e];
+      f << q[0] << "\t" << q[1] << "\t" << q[2] << "\t" << q[3] << "\t" << q[4] << "\t" << q[5] << "\t" << q[6]
+        << "\t" << q[7] << "\t" << q[8] << "\n";
+    }
+  }
   if (!f) return io_fail(std::string("write failed: ") + path);
   return SHX_OK;
 }
@@ -124,6 +139,10 @@ int shx_state_load(shx_ctx* ctx, int shell, const char* path, int precision, uin
   std::vector<double> so(3 * e1.size());
   for (int k = 0; k < nc; ++k) f >> e1[k] >> e2[k] >> so[3 * k] >> so[3 * k + 1] >> so[3 * k + 2];
   if (!f) return io_fail(std::string("malformed state file ") + path, SHX_E_ARG);
+  const bool dyn = prm.metric_growth_rate != 0 || prm.curvature_growth_rate != 0;
+  std::vector<double> grow(dyn ? 9 * (size_t)ne : 0);
+  for (double& x : grow) f >> x;
+  if (!f) return io_fail(std::string("malformed state file ") + path + " (growth data)", SHX_E_ARG);
   /* load (ShellSystem.C:3050-3153), then apply_initial_conditions without
    * the initial acceleration (:855-905): the loaded a is kept; V0 becomes
    * the loaded state's enclosed volume, as in the reference */
@@ -139,6 +158,8 @@ int shx_state_load(shx_ctx* ctx, int shell, const char* path, int precision, uin
    * force evaluation does not run on a loaded state, ShellSystem.C:887-896) */
   if (prm.cavity_contact)
     if (int r = shx_batch_cavity_springs(ctx, shell, cso.data(), 1)) return io_fail(shx_last_error(ctx), r);
+  if (dyn)
+    if (int r = shx_batch_growth_state(ctx, shell, grow.data(), 1)) return io_fail(shx_last_error(ctx), r);
   if (int r = shx_batch_averages(ctx, shell, sums, &n, 1)) return io_fail(shx_last_error(ctx), r);
   if (time_step) *time_step = step;
   if (wall_time) *wall_time = wall;

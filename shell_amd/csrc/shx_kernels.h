@@ -21,6 +21,9 @@ enum { SHX_PREC_FP64 = 0, SHX_PREC_FP32 = 1 };
 
 __global__ void k_precompute(DPtrs P);
 __global__ void k_mass(DPtrs P);
+__global__ void k_mass_grow(DPtrs P);
+__global__ void k_grow(DPtrs P, int mode);
+__global__ void k_elem_grow(DPtrs P, int mode);
 __global__ void k_elem_w3(DPtrs P, int mode);
 __global__ void k_elem_w4(DPtrs P, int mode);
 __global__ void k_fin_elem(DPtrs P, int mode);

@@ -678,10 +678,10 @@ def test_damping_beta_pin_catches_planted_slips(icosphere, slip):
     """the brute-force beta pin fails for an oracle copy with a plausible
     slip in the beta term (planted in a copy, on the CPU)"""
     from oracle_mutants import build
-    old = "const double avg_stiff = 2 * (s.K + s.D / (hmin * hmax)) / 3;"
-    new = {"hmin*hmin": "const double avg_stiff = 2 * (s.K + s.D / (hmin * hmin)) / 3;",
-           "hmax only": "const double avg_stiff = 2 * (s.K + s.D / (hmax * hmax)) / 3;",
-           "K without 2/3": "const double avg_stiff = (s.K + s.D / (hmin * hmax));"}[slip]
+    old = "const double avg_stiff = 2 * (s.K + s.D / (avg_growth * hmin * hmax)) / 3;"
+    new = {"hmin*hmin": "const double avg_stiff = 2 * (s.K + s.D / (avg_growth * hmin * hmin)) / 3;",
+           "hmax only": "const double avg_stiff = 2 * (s.K + s.D / (avg_growth * hmax * hmax)) / 3;",
+           "K without 2/3": "const double avg_stiff = (s.K + s.D / (avg_growth * hmin * hmax));"}[slip]
     L = build(old, new, slip.replace(" ", "_").replace("*", "x").replace("/", "_"))
     X, T = icosphere(2)
     X = imperfect(X, 0.05, seed=3)
@@ -891,12 +891,12 @@ def test_curvature_per_element_is_the_shape_operator():
 @pytest.mark.parametrize("slip", ["mean without 1/2", "mean without cross term", "gauss without ff2_2^2"])
 def test_curvature_pins_catch_planted_slips(slip):
     from oracle_mutants import build
-    kg = "kG = w * invdet * (ad.ff2[0] * ad.ff2[1] - ad.ff2[2] * ad.ff2[2]);"
-    km = "kM = w * invdet * 0.5 * (ad.ff2[0] * ad.ff1[1] + ad.ff2[1] * ad.ff1[0] - 2 * ad.ff2[2] * ad.ff1[2]);"
+    kg = "kG = w * invdet * (ff2[0] * ff2[1] - ff2[2] * ff2[2]);"
+    km = "kM = w * invdet * 0.5 * (ff2[0] * ff1[1] + ff2[1] * ff1[0] - 2 * ff2[2] * ff1[2]);"
     old, new = {
         "mean without 1/2": (km, km.replace("0.5 * ", "")),
-        "mean without cross term": (km, km.replace(" - 2 * ad.ff2[2] * ad.ff1[2]", "")),
-        "gauss without ff2_2^2": (kg, kg.replace(" - ad.ff2[2] * ad.ff2[2]", "")),
+        "mean without cross term": (km, km.replace(" - 2 * ff2[2] * ff1[2]", "")),
+        "gauss without ff2_2^2": (kg, kg.replace(" - ff2[2] * ff2[2]", "")),
     }[slip]
     L = build(old, new, "curv_%d" % ["mean without 1/2", "mean without cross term",
                                      "gauss without ff2_2^2"].index(slip))
@@ -1151,3 +1151,185 @@ def test_gravity_rotation_pins_catch_planted_slips(icosphere, slip):
     expect[:, 1] -= 0.37 / Minv[0::3]
     assert np.abs(fext - expect).max() > 1e-6 * np.abs(expect).max()
     o.close()
+
+
+# ---------------------------------------------------------------- growth
+
+def _grown(X, T, handle=None, **kw):
+    base = dict(E1=1.0, E2=0.5, nu=0.3, G12=0.35, thickness=H)
+    base.update(kw)
+    return O.Oracle(X, T, lib_handle=handle, **base)
+
+
+@pytest.mark.parametrize("g1,g2", [(1.1, 1.1), (1.08, 0.95), (0.9, 1.2)])
+def test_local_growth_energy_closed_form(g1, g2):
+    """growth factors g1 along the east tangent a1 and g2 along the north
+    tangent a2 (growth_base local, thickness_dir 2, ShellSystem.C:2339-2415)
+    make the unloaded sphere's membrane strain the Green strain of the
+    inverse growth, eps_k = (1/g_k^2 - 1)/2, in the element's orthogonal
+    frame, so that E_tens = 1/2 K (E1 eps1^2 + 2 nu E1 eps1 eps2 + E2 eps2^2)
+    * area exactly, K = h/(1 - nu nu21); and the mass follows det G = g1 g2
+    (grow_mass, ShellSystem.C:795-811).  For g1 = g2 = g the dilated sphere
+    u = (g - 1) X is free of membrane strain, and its bending energy is that
+    of the ungrown sphere dilated by 1/g - 1."""
+    X, T = O.icosphere(3, round_g=False)
+    o = _grown(X, T, growth_g0=(g1, g2))
+    z = np.zeros(X.size)
+    _, _, en, area = o.forces(z)
+    e1, e2 = 0.5 * (1 / g1 ** 2 - 1), 0.5 * (1 / g2 ** 2 - 1)
+    K = H / (1 - 0.3 * (0.5 * 0.3 / 1.0))
+    closed = 0.5 * K * (1.0 * e1 ** 2 + 2 * 0.3 * 1.0 * e1 * e2 + 0.5 * e2 ** 2) * area.sum()
+    assert abs(en[0] - closed) <= 1e-11 * closed
+    Minv, _ = o.mass()
+    assert abs((1 / Minv[0::3]).sum() - H * g1 * g2 * area.sum()) <= 1e-12 * H * area.sum()
+    if g1 == g2:
+        en_g = o.forces(((g1 - 1) * X).reshape(-1))[2]
+        p = _grown(X, T)
+        en_p = p.forces(((1 / g1 - 1) * X).reshape(-1))[2]
+        assert abs(en_g[0]) <= 1e-12 * en_p[0]
+        assert abs(en_g[1] - en_p[1]) <= 1e-10 * en_p[1]
+        p.close()
+    o.close()
+
+
+def _rot(axis, ang):
+    c, s = np.cos(ang), np.sin(ang)
+    i, j = [(1, 2), (2, 0), (0, 1)][axis]
+    R = np.eye(3)
+    R[i, i] = R[j, j] = c
+    R[i, j], R[j, i] = -s, s
+    return R
+
+
+def test_cartesian_and_cylindrical_growth_symmetries():
+    """cartesian growth diag(g1, g2, 1) (thickness_dir 2) on a mesh turned a
+    quarter about z equals diag(g2, g1, 1) on the original; cylindrical
+    growth about x (axial g1, hoop g2; thickness_dir 1) is unchanged by a
+    turn of the mesh about x, for an isotropic material (whose energy does
+    not see the lat/long frame).  Energies at the same strained state (the
+    displacement turned with the mesh) agree to rounding."""
+    X, T = O.icosphere(3)
+    X = imperfect(X, 0.02, seed=4)
+    u = np.random.default_rng(5).normal(size=X.shape) * 1e-3
+    iso = dict(E1=1.0, E2=1.0, nu=0.3, G12=1.0 / 2.6)
+    R = _rot(2, np.pi / 2)
+    a = _grown(X, T, growth_base=1, growth_g0=(1.1, 0.93), **iso)
+    b = _grown(X @ R.T, T, growth_base=1, growth_g0=(0.93, 1.1), **iso)
+    ea, eb = a.forces(u.reshape(-1))[2], b.forces((u @ R.T).reshape(-1))[2]
+    assert np.allclose(ea[:2], eb[:2], rtol=1e-10, atol=0)
+    R = _rot(0, 0.61)
+    a = _grown(X, T, growth_base=2, thickness_dir=1, growth_g0=(1.07, 0.9), **iso)
+    b = _grown(X @ R.T, T, growth_base=2, thickness_dir=1, growth_g0=(1.07, 0.9), **iso)
+    ea, eb = a.forces(u.reshape(-1))[2], b.forces((u @ R.T).reshape(-1))[2]
+    assert np.allclose(ea[:2], eb[:2], rtol=1e-10, atol=0)
+    # and the anisotropic growth is seen: swapping the factors changes the energy
+    c = _grown(X, T, growth_base=2, thickness_dir=1, growth_g0=(0.9, 1.07), **iso)
+    assert abs(c.forces(u.reshape(-1))[2][0] - ea[0]) > 1e-3 * ea[0]
+
+
+def test_dynamic_growth_steps_with_the_previous_strains(icosphere):
+    """dynamic growth (apply_growth, ShellSystem.C:2317-2336, in
+    prepare_next_step before the step's force evaluation): each step adds
+    dt (metric_growth_rate alpha + curvature_growth_rate beta) to G, with the
+    strains of the previous force evaluation"""
+    X, T = icosphere(3)
+    X = imperfect(X, 1e-3, seed=6)
+    o = _grown(X, T, metric_growth_rate=0.8, curvature_growth_rate=30.0, pressure_final=-2e-4, dt=0.05)
+    o.set_state(np.zeros(X.size))
+    o.init_dynamics()
+    G0, _, al, be = o.elem_growth()
+    assert np.array_equal(G0, np.tile([1.0, 1.0, 0.0], (len(T), 1)))
+    for _ in range(3):
+        o.step(1)
+        G1, GV, al1, be1 = o.elem_growth()
+        expect = (G0 + al * (0.05 * 0.8)) + be * (0.05 * 30.0)
+        assert np.array_equal(G1, expect)
+        # GV is the Voigt form of the inverse of [[G0, G2/2], [G2/2, G1]]
+        det = G1[:, 0] * G1[:, 1] - 0.25 * G1[:, 2] ** 2
+        gi00, gi11, gi01 = G1[:, 1] / det, G1[:, 0] / det, -0.5 * G1[:, 2] / det
+        assert np.allclose(GV[:, 0, 0], gi00 ** 2, rtol=1e-14) and np.allclose(GV[:, 2, 2], gi01 ** 2 + gi00 * gi11,
+                                                                               rtol=1e-13)
+        G0, al, be = G1, al1, be1
+    assert np.abs(G0 - np.array([1.0, 1.0, 0.0])).max() > 1e-6
+    o.close()
+
+
+def _growth_energy_geometric(o, X, T, base, td, g1, g2, E1=1.0, E2=0.5, nu=0.3, G12=0.35):
+    """membrane energy of the unloaded shell under static growth, formed in
+    each element's orthonormal (east, north) frame without the reference's
+    basis transforms: the inverse growth F^-1 = B diag(1/g) B^T of
+    growth_base B (local: the element frame; cartesian; cylindrical about
+    x), projected on the tangent plane (the in-plane block G of the inverse
+    growth tensor), the Green strain eps = (G^T G - I)/2, and the orthotropic
+    energy density 1/2 K (E1 e11^2 + 2 nu E1 e11 e22 + E2 e22^2 +
+    G12 (1 - nu nu') gamma^2), gamma = 2 e12, nu' = E1 nu / E2 (the shear
+    term as the torsion pin fixes it)"""
+    tab = o.elem_tables()
+    A1, A2, xyz, JxW = tab[:, 72:75], tab[:, 75:78], tab[:, 98:101], tab[:, 93]
+    e = A1 / np.linalg.norm(A1, axis=1)[:, None]
+    n = A2 / np.linalg.norm(A2, axis=1)[:, None]
+    N = np.cross(e, n)
+    gd = [1 if td < 1 else 0, 2 if td < 2 else 1, td]
+    K = H / (1 - nu * (E2 * nu / E1))
+    total = 0.0
+    for k in range(len(T)):
+        if base == 0:
+            B = np.column_stack([e[k], n[k], N[k]])
+        elif base == 1:
+            B = np.eye(3)
+        else:
+            r = np.hypot(xyz[k, 1], xyz[k, 2])
+            c, s_ = xyz[k, 1] / r, xyz[k, 2] / r
+            B = np.array([[1, 0, 0], [0, c, -s_], [0, s_, c]])
+        d = np.ones(3)
+        d[gd[0]], d[gd[1]] = 1 / g1, 1 / g2
+        Fi = B @ np.diag(d) @ B.T
+        G = np.array([[e[k] @ Fi @ e[k], e[k] @ Fi @ n[k]], [n[k] @ Fi @ e[k], n[k] @ Fi @ n[k]]])
+        eps = 0.5 * (G.T @ G - np.eye(2))
+        total += JxW[k] * 0.5 * K * (E1 * eps[0, 0] ** 2 + 2 * nu * E1 * eps[0, 0] * eps[1, 1] + E2 * eps[1, 1] ** 2 +
+                                     G12 * (1 - nu * E1 * nu / E2) * (2 * eps[0, 1]) ** 2)
+    return total
+
+
+GROWTH_CASES = [(0, 2, 1.08, 0.95), (1, 2, 1.1, 0.93), (1, 0, 1.05, 1.0), (2, 1, 1.07, 0.9), (2, 0, 0.92, 1.06)]
+
+
+@pytest.mark.parametrize("base,td,g1,g2", GROWTH_CASES)
+def test_growth_energy_geometric_closed_form(base, td, g1, g2):
+    """static growth in every growth_base and thickness_dir
+    (ShellSystem.C:2339-2440, with growth_dir from thickness_dir, :246-250):
+    the unloaded shell's membrane energy equals the energy of the projected
+    inverse growth formed geometrically in each element's orthonormal frame
+    (_growth_energy_geometric), to rounding"""
+    X, T = O.icosphere(3)
+    X = imperfect(X, 0.02, seed=4)
+    o = _grown(X, T, growth_base=base, thickness_dir=td, growth_g0=(g1, g2))
+    en = o.forces(np.zeros(X.size))[2]
+    closed = _growth_energy_geometric(o, X, T, base, td, g1, g2)
+    assert abs(en[0] - closed) <= 1e-12 * closed
+    o.close()
+
+
+@pytest.mark.parametrize("slip", ["growth_dir swapped", "GV transposed", "mass without det", "B^-1 left out"])
+def test_growth_pins_catch_planted_slips(slip):
+    from oracle_mutants import build
+    old, new = {
+        "growth_dir swapped": ("      Ginv[s.growth_dir[0] * 4] = 1 / g[0];\n      Ginv[s.growth_dir[1] * 4] = 1 / g[1];",
+                               "      Ginv[s.growth_dir[1] * 4] = 1 / g[0];\n      Ginv[s.growth_dir[0] * 4] = 1 / g[1];"),
+        "GV transposed": ("    GV[1] = gi(1, 0) * gi(1, 0);", "    GV[1] = gi(0, 1) * gi(0, 1);"),
+        "mass without det": ("      JxW = c.JxW * det;", "      JxW = c.JxW;"),
+        "B^-1 left out": ("      matmul3(Bi, A, BA);", "      matmul3(B, A, BA);"),
+    }[slip]
+    L = build(old, new, "grow" + str(abs(hash(slip)) % 1000))
+    X, T = O.icosphere(3)
+    X = imperfect(X, 0.02, seed=4)
+    caught = False
+    for base, td, g1, g2 in GROWTH_CASES:
+        o = _grown(X, T, handle=L, growth_base=base, thickness_dir=td, growth_g0=(g1, g2))
+        _, _, en, area = o.forces(np.zeros(X.size))
+        closed = _growth_energy_geometric(o, X, T, base, td, g1, g2)
+        Minv, _ = o.mass()
+        caught |= abs(en[0] - closed) > 1e-9 * closed
+        caught |= abs((1 / Minv[0::3]).sum() - H * g1 * g2 * area.sum()) > 1e-9 * H * area.sum()
+        o.close()
+    assert caught
