@@ -77,7 +77,7 @@ typedef struct {
   double dt_min, dt_max;           /* 0: machine epsilon / unbounded */
   double rel_loc_err_min, rel_loc_err_max;
   int32_t adapt_dt_every;          /* 0: constant dt; else the reference's error control every step */
-  int32_t contact;                 /* shell_shell_contact: 0 none, 1 linear */
+  int32_t contact;                 /* shell_shell_contact: 0 none, 1 linear, 2 smooth (limit surfaces) */
   int32_t quirk_fulltime;          /* 1: reproduce `pressure *= fulltime` (ShellSystem.C:1480) */
   int32_t quirk_volume_schedule;   /* 1: reproduce r(t) = max(1-floor(5t)*0.01, 0.5) (ShellSystem.C:1474) */
   int32_t volume_at_prediction;    /* ext: 1: volume pressure from the enclosed volume of the configuration
@@ -377,6 +377,12 @@ int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out
  * 19-20) that the reference saves (ShellSystem.C:3003-3025). */
 int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, double* d2, int32_t* stick,
                          double* origin, int cap);
+/* The closest-point parameters (v1, w1, v2, w2) of each candidate of a
+ * smooth-contact shell, in the order of shx_batch_pair_state (the members
+ * v[2], w[2] of SmoothShellShellContact, ShellShellContact.h:19, that the
+ * next compute() starts from); zeros for a linear-contact shell; returns
+ * the count. */
+int shx_batch_pair_coords(shx_ctx* ctx, int shell, double* vw /* 4 per pair */, int cap);
 /* Sticking contacts of a loaded state (ShellSystem.C:3093-3121): the n
  * pairs become the shell's candidate list with their spring origins, and
  * the shell's next init keeps them (they survive its contact search, as
@@ -490,6 +496,15 @@ int shx_subdiv_evaluate(shx_ctx* ctx, int shell, int eid, double xi, double eta,
  * v[2], w[2] (barycentric on each), point()[3] (ShellShellContact.C:81-84);
  * *problem = SHX_CONTACT_PROBLEM when compute() finds d2 <= 0. */
 int shx_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, int on_mesh, double* out, int32_t* problem);
+/* Replaces SmoothShellShellContact(eid1, eid2) + compute()
+ * (SmoothShellShellContact.h:19-21, SmoothShellShellContact.C:12-64): the
+ * closest points of the two elements' Loop limit surfaces at the current
+ * positions, searched from x = (v1, w1, v2, w2) (the constructor's start is
+ * 1/3 each), which takes the minimiser.  out[8] = d2, dc[3] (p2 - p1),
+ * point()[3] (ShellShellContact.C:81-84 at the minimiser), the Newton
+ * iterations used; *problem = SHX_CONTACT_PROBLEM when d2 <= 0. */
+int shx_smooth_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, double* x /* 4 */, double* out,
+                               int32_t* problem);
 /* Replaces Friction::apply (Friction.h:12-14, Friction.C:4-60), the same
  * arguments as plain values and pointers (points as double[3], stick as
  * int32); host only. */
@@ -507,6 +522,10 @@ int shx_batch_close_sets(shx_ctx* ctx, int shell, int32_t* start, int32_t* ids, 
  * routines; no GPU needed). */
 void shx_tri_distance(const double* p /* 18 */, double* out /* d2, dc[3], v[2], w[2] */);
 int  shx_raw_elem_basis(int valence, double* out /* 6*(valence+6) */);
+/* the closest points of two limit-surface patches (valence + 6 points each,
+ * xyz) as shx_smooth_contact_compute searches them, from x (4, in/out);
+ * out[5] = d2, dc[3], Newton iterations */
+int  shx_smooth_closest(int valence1, const double* X1, int valence2, const double* X2, double* x, double* out);
 /* the limit-surface weights shx_subdiv_evaluate applies: value, d/dxi,
  * d/deta over the valence+6 patch slots */
 int  shx_subdiv_weights(int valence, double xi, double eta, double* out /* 3*(valence+6) */);

@@ -103,6 +103,12 @@ def _declare(L):
     L.orc_contact_rebuild.argtypes = [_V, ctypes.c_int]
     L.orc_pairs.argtypes = [_V, _U, _U, ctypes.c_int]
     L.orc_pair_state.argtypes = [_V, _D, _I, _D, ctypes.c_int]
+    L.orc_pair_coords.argtypes = [_V, _D, ctypes.c_int]
+    L.orc_smooth_contact.argtypes = [_V, ctypes.c_int, ctypes.c_int, _D, _D]
+    L.orc_limit_point.argtypes = [ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, _D]
+    L.orc_limit_point.restype = None
+    L.orc_smooth_closest.argtypes = [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D]
+    L.orc_smooth_closest.restype = None
     L.orc_close_counts.argtypes = [_V, _I]
     L.orc_close_sets.argtypes = [_V, _I, _I, ctypes.c_int]
     L.orc_cfl_omega_max.restype = ctypes.c_double
@@ -302,6 +308,22 @@ class Oracle:
         self.L.orc_pair_state(self.h, dp(d2), ip(st), dp(so), n)
         return d2[:n], st[:n], so[:n]
 
+    def pair_coords(self):
+        """smooth contact: v1 w1 v2 w2 of each pair's last closest points (n x 4)"""
+        n = self.L.orc_pairs(self.h, None, None, 0)
+        vw = np.zeros((max(n, 1), 4))
+        self.L.orc_pair_coords(self.h, dp(vw), n)
+        return vw[:n]
+
+    def smooth_contact(self, e1, e2, x=(1 / 3, 1 / 3, 1 / 3, 1 / 3)):
+        """SmoothShellShellContact(e1, e2) + compute() from x at the state's
+        positions: dict of d2, dc, point, x (the minimiser), iterations"""
+        xs = np.array(x, dtype=np.float64)
+        out = np.zeros(8)
+        if self.L.orc_smooth_contact(self.h, e1, e2, dp(xs), dp(out)) != 0:
+            raise ValueError((e1, e2))
+        return {"d2": out[0], "dc": out[1:4], "point": out[4:7], "x": xs, "iterations": int(out[7])}
+
     def close_sets(self):
         start = np.zeros(self.ne + 1, dtype=np.int32)
         n = self.L.orc_close_sets(self.h, ip(start), None, 0)
@@ -330,6 +352,26 @@ def subdiv_rows(valence):
     S = np.zeros(12 * (valence + 6))
     lib().orc_loop_subdiv_rows(valence, dp(S))
     return S.reshape(12, valence + 6)
+
+
+def limit_point(valence, X, xi, eta, handle=None):
+    """the oracle's limit surface of a patch (points X, (valence+6) x 3) at
+    (xi, eta): 6 x 3 = point, d/dxi, d/deta, xixi, etaeta, xieta"""
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+    out = np.zeros(18)
+    (handle or lib()).orc_limit_point(valence, dp(X), xi, eta, dp(out))
+    return out.reshape(6, 3)
+
+
+def smooth_closest(valence1, X1, valence2, X2, x=(1 / 3, 1 / 3, 1 / 3, 1 / 3), handle=None):
+    """the oracle's closest points of two limit-surface patches: dict of d2,
+    dc, x (the minimiser), iterations"""
+    X1 = np.ascontiguousarray(X1, dtype=np.float64).reshape(-1)
+    X2 = np.ascontiguousarray(X2, dtype=np.float64).reshape(-1)
+    xs = np.array(x, dtype=np.float64)
+    out = np.zeros(5)
+    (handle or lib()).orc_smooth_closest(valence1, dp(X1), valence2, dp(X2), dp(xs), dp(out))
+    return {"d2": out[0], "dc": out[1:4], "x": xs, "iterations": int(out[4])}
 
 
 def tri_distance(P):

@@ -32,6 +32,7 @@
 #include "oracle_geom.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -263,6 +264,394 @@ void raw_elem_basis(int N, double* out /* 6 x (N+6): phi, xi, eta, xixi, etaeta,
     }
 }
 
+/* ------------------------------------------- limit surface at any point */
+
+/* One Loop step of an irregular patch's points c[0 .. N+5] (numbering as
+ * above) into the N+12 fine points Stam's extended subdivision matrix
+ * produces: 0 the vertex point of v0, k = 1..N the edge point of (v0, ring
+ * k), then N+1 edge (v1, v2), N+2 vertex point of v1, N+3 edge (v1, ring 2),
+ * N+4 vertex point of v2, N+5 edge (v2, ring N-1), N+6 .. N+11 the edges
+ * (v1, P11), (v1, P20), (v1, P2m), (v2, P11), (v2, P02), (v2, Pm2).  The first
+ * N+6 are the next level's patch around v0 in the same numbering; the
+ * selection matrices of SubdivEvaluation.h:31-77 address all N+12.  Edge
+ * points weigh the edge's ends 3/8 and the two vertices opposite it 1/8
+ * (lattice neighbours in the patch); vertex points of the regular v1, v2
+ * 5/8 and 1/16 per neighbour, of v0 Loop's beta_N. */
+void loop_step(int N, const Pt* c, Pt* f)
+{
+  const int V1 = 1, V2 = N, P11 = N + 1, P20 = N + 2, P2m = N + 3, P02 = N + 4, Pm2 = N + 5;
+  auto edge = [&](int a, int b, int o1, int o2) { return (c[a] + c[b]) * (3.0 / 8.0) + (c[o1] + c[o2]) * (1.0 / 8.0); };
+  auto vtx6 = [&](int v, const int nb[6]) {
+    Pt sum;
+    for (int q = 0; q < 6; ++q) sum = sum + c[nb[q]];
+    return c[v] * (5.0 / 8.0) + sum * (1.0 / 16.0);
+  };
+  const double b = loop_beta(N);
+  Pt ring;
+  for (int k = 1; k <= N; ++k) ring = ring + c[k];
+  f[0] = c[0] * (1.0 - N * b) + ring * b;
+  for (int k = 1; k <= N; ++k) f[k] = edge(0, k, k == 1 ? N : k - 1, k == N ? 1 : k + 1);
+  f[N + 1] = edge(V1, V2, 0, P11);
+  { const int nb[6] = {0, V2, P11, P20, P2m, 2}; f[N + 2] = vtx6(V1, nb); }
+  f[N + 3] = edge(V1, 2, 0, P2m);
+  { const int nb[6] = {0, V1, P11, P02, Pm2, N - 1}; f[N + 4] = vtx6(V2, nb); }
+  f[N + 5] = edge(V2, N - 1, 0, Pm2);
+  f[N + 6] = edge(V1, P11, V2, P20);
+  f[N + 7] = edge(V1, P20, P11, P2m);
+  f[N + 8] = edge(V1, P2m, 2, P20);
+  f[N + 9] = edge(V2, P11, V1, P02);
+  f[N + 10] = edge(V2, P02, P11, Pm2);
+  f[N + 11] = edge(V2, Pm2, P02, N - 1);
+}
+
+/* SubdivEvaluation::evaluate / evaluate_deriv (SubdivEvaluation.C:14-187)
+ * extended to second derivatives, for a patch of valence N (6: regular)
+ * with points X: d[0] the point, d[1], d[2] d/dxi, d/deta, d[3..5] xixi,
+ * etaeta, xieta.  The irregular case subdivides the points n times
+ * (rescale_irregular_coords, :170-187) and evaluates the box splines of the
+ * sub-patch P_k picks (:39-77).  Departure: the derivative factor of the
+ * central sub-patch (k = 1) is -2^n; the reference's pow(-2, n) (:134) has
+ * the wrong sign for even n. */
+void limit_point(int N, const Pt* X, double xi, double eta, Pt d[6])
+{
+  double b[72];
+  for (int q = 0; q < 6; ++q) d[q] = Pt();
+  if (N == 6) {
+    stam_eval(xi, eta, b);
+    for (int j = 0; j < 12; ++j)
+      for (int q = 0; q < 6; ++q) d[q] = d[q] + X[j] * b[q * 12 + CVI[j]];
+    return;
+  }
+  const double irregular_tol = 1e-10;
+  if (xi + eta < irregular_tol) { xi = std::max(xi, irregular_tol); eta = std::max(eta, 0.0); }
+  const int n = (int)std::floor(1 - std::log(xi + eta) / std::log(2.0));
+  const double pow2 = std::pow(2.0, n - 1);
+  xi *= pow2;
+  eta *= pow2;
+  int k;
+  double jfac;
+  if (xi > 0.5) { k = 0; xi = 2 * xi - 1; eta = 2 * eta; jfac = std::pow(2.0, n); }
+  else if (eta > 0.5) { k = 2; xi = 2 * xi; eta = 2 * eta - 1; jfac = std::pow(2.0, n); }
+  else { k = 1; xi = 1 - 2 * xi; eta = 1 - 2 * eta; jfac = -std::pow(2.0, n); }
+  /* P_k of SubdivEvaluation.h:33-76: box spline i takes fine point sel[k][i] */
+  const int sel[3][12] = {{2, 0, N + 3, 1, N, N + 8, N + 2, N + 1, N + 4, N + 7, N + 6, N + 9},
+                          {N + 9, N + 6, N + 4, N + 1, N + 2, N + 5, N, 1, N + 3, N - 1, 0, 2},
+                          {0, N - 1, 1, N, N + 5, N + 2, N + 1, N + 4, N + 11, N + 6, N + 9, N + 10}};
+  std::vector<Pt> c(X, X + N + 6), f(N + 12);
+  for (int it = 0; it < n; ++it) {
+    loop_step(N, c.data(), f.data());
+    std::copy(f.begin(), f.begin() + N + 6, c.begin());
+  }
+  stam_eval(xi, eta, b);
+  for (int i = 0; i < 12; ++i)
+    for (int q = 0; q < 6; ++q) d[q] = d[q] + f[sel[k][i]] * b[q * 12 + i];
+  for (int q = 1; q < 6; ++q) d[q] = d[q] * (q < 3 ? jfac : jfac * jfac);
+}
+
+/* ------------------------------------------- smooth contact closest points */
+
+/* The problem of SmoothShellShellContact::compute (SmoothShellShellContact.C:
+ * 21-41): min f(x) = |p2(v2,w2) - p1(v1,w1)|^2 over x = (v1, w1, v2, w2)
+ * with A x >= b, each (v, w) in its triangle.  The reference's minimiser
+ * (ConstrainedMinimizer.h) is absent from its sources, so parity with the
+ * reference's iterates is unpinned; this restates the library's active-set
+ * Newton search (Newton steps on the face the active constraints leave
+ * free, eigenvalue-modified where the Hessian is indefinite, cut at the
+ * first constraint met, Armijo backtracking outside the quadratic regime,
+ * a direct test of an irregular vertex the iterate approaches, release of
+ * the most negative multiplier once stationary), whose result
+ * -- the constrained minimum -- is
+ * pinned in tests/test_oracle_pins.py by brute-force sampling, by the KKT
+ * conditions and by planar patches (whose limit surface is the triangle
+ * itself: the triangle-triangle distance). */
+struct SmoothProblem {
+  int N[2];
+  Pt X[2][12];
+
+  double eval(const double* x, int order, double* g, double* H, Pt& p1, Pt& p2) const
+  {
+    Pt d1[6], d2[6];
+    limit_point(N[0], X[0], x[0], x[1], d1);
+    limit_point(N[1], X[1], x[2], x[3], d2);
+    p1 = d1[0];
+    p2 = d2[0];
+    const Pt r = p2 - p1;
+    if (order >= 1) {
+      const Pt J[4] = {-d1[1], -d1[2], d2[1], d2[2]};
+      for (int a = 0; a < 4; ++a) g[a] = 2 * (J[a] * r);
+      if (order >= 2) {
+        for (int a = 0; a < 4; ++a)
+          for (int c = 0; c < 4; ++c) H[4 * a + c] = 2 * (J[a] * J[c]);
+        const double s1[3] = {-(r * d1[3]), -(r * d1[4]), -(r * d1[5])};
+        const double s2[3] = {r * d2[3], r * d2[4], r * d2[5]};
+        H[0] += 2 * s1[0]; H[5] += 2 * s1[1]; H[1] += 2 * s1[2]; H[4] += 2 * s1[2];
+        H[10] += 2 * s2[0]; H[15] += 2 * s2[1]; H[11] += 2 * s2[2]; H[14] += 2 * s2[2];
+      }
+    }
+    return r * r;
+  }
+};
+
+/* (v, w) into its triangle, then onto the active constraints (bits 1 v = 0,
+ * 2 w = 0, 4 v + w = 1) */
+void tri_clamp(double& v, double& w, int act)
+{
+  v = std::max(v, 0.0);
+  w = std::max(w, 0.0);
+  if (v + w > 1) { if (v >= w) v = 1 - w; else w = 1 - v; }
+  if (act & 1) v = 0;
+  if (act & 2) w = 0;
+  if (act & 4) {
+    if (act & 1) w = 1;
+    else if (act & 2) v = 1;
+    else v = 1 - w;
+  }
+}
+
+bool chol_solve(const double* A, int m, double mu, const double* b, double* y)
+{
+  double L[4][4];
+  double dmax = 0;
+  for (int i = 0; i < m; ++i) dmax = std::max(dmax, std::fabs(A[5 * i]));
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double sum = A[4 * i + j] + (i == j ? mu : 0.0);
+      for (int k = 0; k < j; ++k) sum -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (!(sum > 1e-13 * dmax)) return false;
+        L[i][i] = std::sqrt(sum);
+      } else {
+        L[i][j] = sum / L[j][j];
+      }
+    }
+  double z[4];
+  for (int i = 0; i < m; ++i) {
+    double sum = b[i];
+    for (int k = 0; k < i; ++k) sum -= L[i][k] * z[k];
+    z[i] = sum / L[i][i];
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double sum = z[i];
+    for (int k = i + 1; k < m; ++k) sum -= L[k][i] * y[k];
+    y[i] = sum / L[i][i];
+  }
+  return true;
+}
+
+/* symmetric eigenproblem of an m x m (m <= 4, stride 4) matrix by cyclic
+ * Jacobi rotations (Golub & Van Loan sec. 8.5); A is overwritten, the
+ * eigenvectors are the columns of V */
+void jacobi_eig(double* A, int m, double* lam, double* V)
+{
+  for (int i = 0; i < 16; ++i) V[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    double off = 0, dia = 0;
+    for (int i = 0; i < m; ++i) {
+      dia += A[5 * i] * A[5 * i];
+      for (int j = i + 1; j < m; ++j) off += A[4 * i + j] * A[4 * i + j];
+    }
+    if (!(off > 1e-32 * dia)) break;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = A[4 * p + q];
+        if (apq == 0) continue;
+        const double theta = (A[5 * q] - A[5 * p]) / (2 * apq);
+        double t = std::fabs(theta) > 1e150 ? 0.5 / std::fabs(theta)
+                                           : 1.0 / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        if (theta < 0) t = -t;
+        const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+        for (int k = 0; k < m; ++k) {
+          const double a = A[4 * k + p], b = A[4 * k + q];
+          A[4 * k + p] = c * a - sn * b;
+          A[4 * k + q] = sn * a + c * b;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double a = A[4 * p + k], b = A[4 * q + k];
+          A[4 * p + k] = c * a - sn * b;
+          A[4 * q + k] = sn * a + c * b;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double a = V[4 * k + p], b = V[4 * k + q];
+          V[4 * k + p] = c * a - sn * b;
+          V[4 * k + q] = sn * a + c * b;
+        }
+      }
+  }
+  for (int i = 0; i < m; ++i) lam[i] = A[5 * i];
+}
+
+double smooth_closest(const SmoothProblem& S, double x[4], Pt& dc, int& iters)
+{
+  int act[2];
+  for (int e = 0; e < 2; ++e) {
+    double& v = x[2 * e];
+    double& w = x[2 * e + 1];
+    tri_clamp(v, w, 0);
+    int a = 0;
+    if (v <= 1e-14) a |= 1;
+    if (w <= 1e-14) a |= 2;
+    if (v + w >= 1 - 1e-14) a |= 4;
+    if (a == 7) a = v <= w ? 5 : 6;
+    act[e] = a;
+    tri_clamp(v, w, a);
+  }
+  Pt p1, p2;
+  int it = 0;
+  bool tried[2] = {false, false};
+  for (; it < 40; ++it) {
+    /* the degenerate parametrisation at an irregular vertex: within xi + eta
+     * < 1e-3 of it, the vertex itself is tried (not again once a multiplier
+     * released it) */
+    for (int e = 0; e < 2; ++e) {
+      if (S.N[e] == 6 || tried[e] || act[e] == 3 || !(x[2 * e] + x[2 * e + 1] < 1e-3)) continue;
+      double xv[4] = {x[0], x[1], x[2], x[3]};
+      xv[2 * e] = xv[2 * e + 1] = 0;
+      Pt q1, q2;
+      if (S.eval(xv, 0, nullptr, nullptr, q1, q2) <= S.eval(x, 0, nullptr, nullptr, q1, q2)) {
+        x[2 * e] = x[2 * e + 1] = 0;
+        act[e] = 3;
+      }
+    }
+    double g[4], H[16];
+    const double f = S.eval(x, 2, g, H, p1, p2);
+    /* free directions of the face */
+    std::vector<std::array<double, 4>> Z;
+    for (int e = 0; e < 2; ++e) {
+      std::array<double, 4> z{0, 0, 0, 0};
+      switch (act[e]) {
+        case 0:
+          z[2 * e] = 1; Z.push_back(z);
+          z[2 * e] = 0; z[2 * e + 1] = 1; Z.push_back(z);
+          break;
+        case 1: z[2 * e + 1] = 1; Z.push_back(z); break;
+        case 2: z[2 * e] = 1; Z.push_back(z); break;
+        case 4: z[2 * e] = 1; z[2 * e + 1] = -1; Z.push_back(z); break;
+        default: break;
+      }
+    }
+    const int m = (int)Z.size();
+    double d[4] = {0, 0, 0, 0}, slope = 0;
+    bool pure = true;
+    if (m > 0) {
+      double rhs[4], Hz[16], dz[4];
+      for (int i = 0; i < m; ++i) {
+        double sum = 0;
+        for (int c = 0; c < 4; ++c) sum += Z[i][c] * g[c];
+        rhs[i] = -sum;
+        for (int j = 0; j < m; ++j) {
+          double t = 0;
+          for (int a = 0; a < 4; ++a) {
+            if (Z[i][a] == 0) continue;
+            for (int c = 0; c < 4; ++c) t += Z[i][a] * H[4 * a + c] * Z[j][c];
+          }
+          Hz[4 * i + j] = t;
+        }
+      }
+      if (!chol_solve(Hz, m, 0.0, rhs, dz)) {
+        /* indefinite on the face: Newton on the eigenvalues' magnitudes
+         * (floored at 1e-3 of the largest) */
+        pure = false;
+        double lam[4], Vz[16];
+        jacobi_eig(Hz, m, lam, Vz);
+        double lmax = 0;
+        for (int i = 0; i < m; ++i) lmax = std::max(lmax, std::fabs(lam[i]));
+        const double lfloor = lmax > 0 ? 1e-3 * lmax : 1.0;
+        std::fill(dz, dz + 4, 0.0);
+        for (int i = 0; i < m; ++i) {
+          double c = 0;
+          for (int j = 0; j < m; ++j) c += Vz[4 * j + i] * rhs[j];
+          c /= std::max(std::fabs(lam[i]), lfloor);
+          for (int j = 0; j < m; ++j) dz[j] += c * Vz[4 * j + i];
+        }
+      }
+      for (int i = 0; i < m; ++i)
+        for (int c = 0; c < 4; ++c) d[c] += Z[i][c] * dz[i];
+      for (int c = 0; c < 4; ++c) slope += g[c] * d[c];
+    }
+    double dinf = 0;
+    for (int c = 0; c < 4; ++c) dinf = std::max(dinf, std::fabs(d[c]));
+    /* Newton decrement below f's rounding: the face search is done */
+    const bool resolved = !(pure && -slope <= 1e-13 * f);
+    if (m > 0 && dinf >= 1e-13 && slope < 0 && resolved) {
+      /* find_max_alpha (SmoothShellShellContact.C:93-133) over the inactive constraints */
+      double amax = std::numeric_limits<double>::max();
+      int be = -1, bbit = 0;
+      for (int e = 0; e < 2; ++e) {
+        const double v = x[2 * e], w = x[2 * e + 1], dv = d[2 * e], dw = d[2 * e + 1];
+        if (!(act[e] & 1) && dv < 0 && -v / dv < amax) { amax = -v / dv; be = e; bbit = 1; }
+        if (!(act[e] & 2) && dw < 0 && -w / dw < amax) { amax = -w / dw; be = e; bbit = 2; }
+        if (!(act[e] & 4) && dv + dw > 0 && (1 - v - w) / (dv + dw) < amax) {
+          amax = (1 - v - w) / (dv + dw);
+          be = e;
+          bbit = 4;
+        }
+      }
+      amax = std::max(amax, 0.0);
+      double alpha = std::min(amax, 1.0);
+      if (!(pure && alpha * dinf <= 1e-3)) {
+        /* Armijo backtracking, then (modified Newton, full step taken) doubling
+         * while f falls, up to the first constraint */
+        auto f_at = [&](double a) {
+          double xt[4];
+          for (int c = 0; c < 4; ++c) xt[c] = x[c] + a * d[c];
+          Pt q1, q2;
+          return S.eval(xt, 0, nullptr, nullptr, q1, q2);
+        };
+        int bt = 0;
+        double fa = f;
+        for (; bt < 60; ++bt) {
+          fa = f_at(alpha);
+          if (fa <= f + 1e-4 * alpha * slope) break;
+          alpha *= 0.5;
+        }
+        if (bt == 60) break;
+        if (!pure && bt == 0)
+          while (alpha < amax) {
+            const double a2 = std::min(2 * alpha, amax);
+            const double f2 = f_at(a2);
+            if (!(f2 < fa)) break;
+            alpha = a2;
+            fa = f2;
+          }
+      }
+      for (int c = 0; c < 4; ++c) x[c] += alpha * d[c];
+      if (alpha == amax && be >= 0) {
+        act[be] |= bbit;
+        if (act[be] == 7) act[be] = bbit == 1 ? 5 : 6;
+      }
+      for (int e = 0; e < 2; ++e) tri_clamp(x[2 * e], x[2 * e + 1], act[e]);
+      continue;
+    }
+    if (m > 0 && slope < 0)
+      for (int c = 0; c < 4; ++c) x[c] += d[c];
+    for (int e = 0; e < 2; ++e) tri_clamp(x[2 * e], x[2 * e + 1], act[e]);
+    /* KKT multipliers of the active constraints: g = sum lambda_i a_i */
+    double gs = 0;
+    for (int c = 0; c < 4; ++c) gs = std::max(gs, std::fabs(g[c]));
+    double lmin = -1e-10 * gs;
+    int re = -1, rbit = 0;
+    for (int e = 0; e < 2; ++e) {
+      const double gv = g[2 * e], gw = g[2 * e + 1];
+      double lam[3] = {0, 0, 0};
+      if (act[e] == 1) lam[0] = gv;
+      else if (act[e] == 2) lam[1] = gw;
+      else if (act[e] == 4) lam[2] = -0.5 * (gv + gw);
+      else if (act[e] == 3) { lam[0] = gv; lam[1] = gw; }
+      else if (act[e] == 5) { lam[2] = -gw; lam[0] = gv - gw; }
+      else if (act[e] == 6) { lam[2] = -gv; lam[1] = gw - gv; }
+      for (int q = 0; q < 3; ++q)
+        if ((act[e] & (1 << q)) && lam[q] < lmin) { lmin = lam[q]; re = e; rbit = 1 << q; }
+    }
+    if (re < 0) { ++it; break; }
+    if (act[re] == 3) tried[re] = true;
+    act[re] &= ~rbit;
+  }
+  iters = it;
+  const double d2 = S.eval(x, 0, nullptr, nullptr, p1, p2);
+  dc = p2 - p1;
+  return d2;
+}
+
 /* --------------------------------------------------------------- system */
 
 struct SurfaceMetric {
@@ -288,6 +677,9 @@ struct Contact {
   double d2 = 0;
   bool stick = false;
   Pt spring_origin;
+  /* smooth contact: v1 w1 v2 w2 of the last closest points, the next
+   * search's start (SmoothShellShellContact.C:16, 33-41) */
+  double x[4] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};
 };
 
 struct System {
@@ -1041,16 +1433,48 @@ void pair_contact(const System& s, int e1, int e2, TriContact& tc, bool on_mesh)
   tri_tri_contact(P, tc);
 }
 
+/* SmoothShellShellContact::compute (SmoothShellShellContact.C:21-64) at the
+ * positions s.p, from the start x (which takes the minimiser); tc gets d2,
+ * dc = p2 - p1 and the parameters */
+void smooth_pair(const System& s, int e1, int e2, double x[4], TriContact& tc)
+{
+  SmoothProblem S;
+  const int es[2] = {e1, e2};
+  for (int e = 0; e < 2; ++e) {
+    const ElemCache& c = s.ec[es[e]];
+    S.N[e] = c.N - 6;
+    for (int j = 0; j < c.N; ++j) S.X[e][j] = s.p[c.patch[j]];
+  }
+  int iters = 0;
+  tc.d2 = smooth_closest(S, x, tc.dc, iters);
+  tc.v[0] = x[0];
+  tc.w[0] = x[1];
+  tc.v[1] = x[2];
+  tc.w[1] = x[3];
+}
+
 /* add_shell_shell_contact_force, ShellSystem.C:1792-1907 */
 void contact_force(System& s)
 {
   const orc_params& P = s.prm;
   const double h = P.thickness;
-  for (auto& kv : s.pairs) {
+  /* the pairs' distances first (independent: in parallel for smooth
+   * contact, whose searches dominate), then the forces in pair order */
+  std::vector<std::pair<const std::pair<int, int>, Contact>*> list;
+  list.reserve(s.pairs.size());
+  for (auto& kv : s.pairs) list.push_back(&kv);
+  std::vector<TriContact> tcs(list.size());
+#pragma omp parallel for schedule(dynamic, 16) if (P.contact == 2)
+  for (long i = 0; i < (long)list.size(); ++i) {
+    auto& kv = *list[i];
+    if (P.contact == 2) smooth_pair(s, kv.first.first, kv.first.second, kv.second.x, tcs[i]);
+    else pair_contact(s, kv.first.first, kv.first.second, tcs[i], false);
+  }
+  for (size_t i = 0; i < list.size(); ++i) {
+    auto& kv = *list[i];
     const int e1 = kv.first.first, e2 = kv.first.second;
     Contact& ct = kv.second;
-    TriContact tc;
-    pair_contact(s, e1, e2, tc, false);
+    const TriContact& tc = tcs[i];
     ct.d2 = tc.d2;
     if (tc.d2 <= 0) s.reg(CONTACT_PROBLEM);
     if (tc.d2 < s.d2_friction) {
@@ -1219,7 +1643,9 @@ void contact_candidates(System& s, bool force, const std::vector<double>& ucur)
   cell_size = P.cell_size_mult * s.avg_elem_size;
   tolerance = P.cell_margin * cell_size;
   cell_size += tolerance;
-  const double d_max = P.thickness + tolerance;
+  double d_max = P.thickness + tolerance;
+  /* smooth contact: one more averaged element size (ShellSystem.C:1940-1944) */
+  if (P.contact == 2) d_max += s.avg_elem_size;
   const double d2_max = d_max * d_max;
   unsigned nd[3];
   for (int q = 0; q < 3; ++q) nd[q] = static_cast<int>(((s.bmax[q] - s.bmin[q]) + d_max) / cell_size + 1);
@@ -1248,6 +1674,10 @@ void contact_candidates(System& s, bool force, const std::vector<double>& ucur)
     if (!it->second.stick && it->second.d2 > d2_max) it = s.pairs.erase(it);
     else ++it;
   }
+  /* the new candidates once each, then their distances (in parallel for
+   * smooth contact), then the ones within d_max join the list */
+  std::vector<std::pair<int, int>> cand;
+  std::set<std::pair<int, int>> seen;
   for (int e1 = 0; e1 < s.ne; ++e1)
     for (unsigned cell : e2c[e1])
       for (unsigned e2u : c2e[cell]) {
@@ -1256,14 +1686,21 @@ void contact_candidates(System& s, bool force, const std::vector<double>& ucur)
         if (elems_are_close(s, e1, e2)) continue;
         if (s.pairs.count({e1, e2})) continue;
         if (boxes_apart(s.ec[e1], s.ec[e2], d_max)) continue;
-        TriContact tc;
-        pair_contact(s, e1, e2, tc, false);
-        if (tc.d2 <= 0) s.reg(CONTACT_PROBLEM);
-        if (tc.d2 > d2_max) continue;
-        Contact ct;
-        ct.d2 = tc.d2;
-        s.pairs[{e1, e2}] = ct;
+        if (seen.insert({e1, e2}).second) cand.push_back({e1, e2});
       }
+  std::vector<TriContact> tcs(cand.size());
+  std::vector<Contact> cts(cand.size());
+#pragma omp parallel for schedule(dynamic, 16) if (P.contact == 2)
+  for (long i = 0; i < (long)cand.size(); ++i) {
+    if (P.contact == 2) smooth_pair(s, cand[i].first, cand[i].second, cts[i].x, tcs[i]);
+    else pair_contact(s, cand[i].first, cand[i].second, tcs[i], false);
+  }
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (tcs[i].d2 <= 0) s.reg(CONTACT_PROBLEM);
+    if (tcs[i].d2 > d2_max) continue;
+    cts[i].d2 = tcs[i].d2;
+    s.pairs[cand[i]] = cts[i];
+  }
 }
 
 /* check_increment, ShellSystem.C:2591-2652 (no ghosts) */
@@ -1854,6 +2291,67 @@ int orc_pair_state(void* h, double* d2, int32_t* stick, double* so, int cap)
     ++k;
   }
   return k;
+}
+
+int orc_pair_coords(void* h, double* vw, int cap)
+{
+  System& s = *static_cast<System*>(h);
+  int k = 0;
+  for (auto& kv : s.pairs) {
+    if (k < cap)
+      for (int q = 0; q < 4; ++q) vw[4 * k + q] = s.prm.contact == 2 ? kv.second.x[q] : 0.0;
+    ++k;
+  }
+  return k;
+}
+
+int orc_smooth_contact(void* h, int e1, int e2, double* x, double* out)
+{
+  System& s = *static_cast<System*>(h);
+  if (e1 < 0 || e1 >= s.ne || e2 < 0 || e2 >= s.ne) return -1;
+  /* the current positions X + u (the state's node_cache p) */
+  auto pos = [&](int n) { return Pt(s.X[n](0) + s.u[3 * n], s.X[n](1) + s.u[3 * n + 1], s.X[n](2) + s.u[3 * n + 2]); };
+  TriContact tc;
+  SmoothProblem S;
+  const int es[2] = {e1, e2};
+  for (int e = 0; e < 2; ++e) {
+    const ElemCache& c = s.ec[es[e]];
+    S.N[e] = c.N - 6;
+    for (int j = 0; j < c.N; ++j) S.X[e][j] = pos(c.patch[j]);
+  }
+  int iters = 0;
+  tc.d2 = smooth_closest(S, x, tc.dc, iters);
+  /* ShellShellContact::point, ShellShellContact.C:81-84 */
+  const Pt cp = pos(s.tri[3 * e1]) * (1 - x[0] - x[1]) + pos(s.tri[3 * e1 + 1]) * x[0] +
+                pos(s.tri[3 * e1 + 2]) * x[1] + tc.dc * 0.5;
+  const double o[8] = {tc.d2, tc.dc(0), tc.dc(1), tc.dc(2), cp(0), cp(1), cp(2), (double)iters};
+  std::copy(o, o + 8, out);
+  return 0;
+}
+
+void orc_smooth_closest(int valence1, const double* X1, int valence2, const double* X2, double* x, double* out)
+{
+  SmoothProblem S;
+  S.N[0] = valence1;
+  S.N[1] = valence2;
+  for (int e = 0; e < 2; ++e) {
+    const double* X = e ? X2 : X1;
+    for (int j = 0; j < S.N[e] + 6; ++j) S.X[e][j] = Pt(X[3 * j], X[3 * j + 1], X[3 * j + 2]);
+  }
+  Pt dc;
+  int iters = 0;
+  out[0] = smooth_closest(S, x, dc, iters);
+  for (int c = 0; c < 3; ++c) out[1 + c] = dc(c);
+  out[4] = iters;
+}
+
+void orc_limit_point(int valence, const double* X, double xi, double eta, double* out)
+{
+  Pt P[12], d[6];
+  for (int j = 0; j < valence + 6; ++j) P[j] = Pt(X[3 * j], X[3 * j + 1], X[3 * j + 2]);
+  limit_point(valence, P, xi, eta, d);
+  for (int q = 0; q < 6; ++q)
+    for (int c = 0; c < 3; ++c) out[3 * q + c] = d[q](c);
 }
 
 int orc_close_sets(void* h, int32_t* start, int32_t* ids, int cap)

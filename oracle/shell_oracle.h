@@ -42,7 +42,7 @@ typedef struct {
   double dt_min, dt_max;           /* set_newmark_parameters clamp (ShellSystem.C:746-747); 0: none */
   double rel_loc_err_min, rel_loc_err_max;
   int32_t adapt_dt_every;          /* 0: constant dt (ShellSystem.C:957) */
-  int32_t contact;                 /* 0: none, 1: linear (ShellSystem.h:45) */
+  int32_t contact;                 /* 0: none, 1: linear, 2: smooth (ShellSystem.h:45) */
   int32_t quirk_fulltime;          /* 1: pressure *= fulltime (== 0), ShellSystem.C:1480 */
   int32_t quirk_volume_schedule;   /* 1: r(t) = max(1-floor(5t)*0.01, 0.5), ShellSystem.C:1474 */
   int32_t omp_faithful;            /* 1: OpenMP loops with critical assembly (CPU baseline timing) */
@@ -132,6 +132,17 @@ int  orc_elem_measure(void* h, double* out /* ne*10: ff1, ff2, jac, kGauss, kMea
 int  orc_contact_rebuild(void* h, int force);
 int  orc_pairs(void* h, uint32_t* e1, uint32_t* e2, int cap);   /* returns count, sorted */
 int  orc_pair_state(void* h, double* d2, int32_t* stick, double* spring_origin, int cap);
+/* smooth contact: per pair (same order) v1 w1 v2 w2 of the last closest points */
+int  orc_pair_coords(void* h, double* vw, int cap);
+/* SmoothShellShellContact(e1, e2) + compute() at the state's positions from
+ * x (4, in/out): out[8] = d2, dc[3], point()[3], Newton iterations */
+int  orc_smooth_contact(void* h, int e1, int e2, double* x, double* out);
+/* the limit surface of a patch of the given valence (points X, 3 per patch
+ * slot) at (xi, eta): out[18] = point, d/dxi, d/deta, xixi, etaeta, xieta */
+void orc_limit_point(int valence, const double* X, double xi, double eta, double* out);
+/* closest points of two limit-surface patches (points X1, X2) from x (4,
+ * in/out): out[5] = d2, dc[3], Newton iterations */
+void orc_smooth_closest(int valence1, const double* X1, int valence2, const double* X2, double* x, double* out);
 int  orc_close_counts(void* h, int32_t* counts);
 int  orc_close_sets(void* h, int32_t* start /* ne+1 */, int32_t* ids, int cap);
 double orc_cfl_omega_max(void* h, int iters, uint64_t seed);

@@ -313,6 +313,20 @@ class _Measurements:
         lib().shx_batch_close_sets(self.h, shell, None, ids.ctypes.data_as(_I), n)
         return start, ids[:n]
 
+    def pair_state(self, shell=0):
+        """the contact candidates sorted by (e1, e2): dict of e1, e2, d2,
+        stick, spring origin (n x 3) and the closest-point parameters (n x 4:
+        v1 w1 v2 w2; zeros for linear contact)"""
+        L = lib()
+        n = self._check(L.shx_batch_pair_state(self.h, shell, None, None, None, None, None, 0), "shx_batch_pair_state")
+        m = max(n, 1)
+        e1, e2 = np.zeros(m, dtype=np.uint32), np.zeros(m, dtype=np.uint32)
+        d2, st, so, vw = np.zeros(m), np.zeros(m, dtype=np.int32), np.zeros((m, 3)), np.zeros((m, 4))
+        self._check(L.shx_batch_pair_state(self.h, shell, e1.ctypes.data_as(_U), e2.ctypes.data_as(_U), _dp(d2),
+                                           st.ctypes.data_as(_I), _dp(so), n), "shx_batch_pair_state")
+        self._check(L.shx_batch_pair_coords(self.h, shell, _dp(vw), n), "shx_batch_pair_coords")
+        return {"e1": e1[:n], "e2": e2[:n], "d2": d2[:n], "stick": st[:n], "origin": so[:n], "vw": vw[:n]}
+
     def cavity_springs(self, shell=0, origin=None):
         """the nodes' cavity friction spring origins (nv x 3); with origin
         given, sets them instead (they survive the shell's next init)"""
@@ -798,6 +812,35 @@ def subdiv_evaluate(shells, eid, xi, eta, shell=0):
     shells._check(L.shx_subdiv_evaluate(shells.h, shell, eid, xi, eta, _dp(p), _dp(d1), _dp(d2)),
                   "shx_subdiv_evaluate")
     return p, d1, d2
+
+
+def smooth_closest(valence1, X1, valence2, X2, x=(1 / 3, 1 / 3, 1 / 3, 1 / 3)):
+    """the library's closest-point search between two limit-surface patches
+    (host copy, no GPU): dict of d2, dc, x (the minimiser), iterations"""
+    L = lib()
+    L.shx_smooth_closest.argtypes = [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D]
+    X1 = np.ascontiguousarray(X1, dtype=np.float64).reshape(-1)
+    X2 = np.ascontiguousarray(X2, dtype=np.float64).reshape(-1)
+    xs = np.array(x, dtype=np.float64)
+    out = np.zeros(5)
+    if L.shx_smooth_closest(valence1, _dp(X1), valence2, _dp(X2), _dp(xs), _dp(out)) != 0:
+        raise ShellError("shx_smooth_closest: bad arguments", -1)
+    return {"d2": out[0], "dc": out[1:4], "x": xs, "iterations": int(out[4])}
+
+
+def smooth_contact_compute(shells, eid1, eid2, x=(1 / 3, 1 / 3, 1 / 3, 1 / 3), shell=0):
+    """SmoothShellShellContact(eid1, eid2) + compute() from the start x =
+    (v1, w1, v2, w2) at the shell's current positions: dict of d2, dc,
+    point, x (the minimiser), iterations and the problem it registers"""
+    L = lib()
+    L.shx_smooth_contact_compute.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _I]
+    xs = np.array(x, dtype=np.float64)
+    out = np.zeros(8)
+    prob = np.zeros(1, dtype=np.int32)
+    shells._check(L.shx_smooth_contact_compute(shells.h, shell, eid1, eid2, _dp(xs), _dp(out),
+                                               prob.ctypes.data_as(_I)), "shx_smooth_contact_compute")
+    return {"d2": out[0], "dc": out[1:4], "point": out[4:7], "x": xs, "iterations": int(out[7]),
+            "problem": int(prob[0])}
 
 
 def contact_compute(shells, eid1, eid2, on_mesh=False, shell=0):

@@ -21,6 +21,7 @@
 #include "shx_kernels.h"
 #include "shx_math.h"
 #include "shx_setup.h"
+#include "shx_smooth.h"
 #include "shx_topo.h"
 
 using namespace shx;
@@ -44,6 +45,7 @@ struct shx_ctx {
   SConst* dC = nullptr;
   int nblkE = 0, nblkN = 0, ntile = 0;
   bool any_contact = false;
+  bool any_smooth = false;     /* some shell has smooth contact: k_contact_smooth, k_bp_exact_smooth */
   bool any_kd = false;
   bool any_rot = false;        /* some shell has rotation damping: k_rot precedes the element epilogue */
   bool any_loads = false;      /* some shell has gravity, cavity contact or rotation damping: k_node_loads */
@@ -341,7 +343,9 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   }
   prof_end(ctx);
   prof_begin(ctx, 9);
-  switch (ctx->shapes.exact_waves) {
+  if (ctx->any_smooth)
+    k_bp_exact_smooth<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  else switch (ctx->shapes.exact_waves) {
     case 3: k_bp_exact_w3<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
     case 4: k_bp_exact_w4<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
     default: k_bp_exact_w2<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
@@ -426,7 +430,9 @@ void launch_contact(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 2);
   if (ctx->grid[0]) {
-    switch (ctx->shapes.contact_waves) {
+    if (ctx->any_smooth)
+      k_contact_smooth<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode);
+    else switch (ctx->shapes.contact_waves) {
       case 2: k_contact_w2<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode); break;
       case 4: k_contact_w4<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode); break;
       default: k_contact_w3<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode);
@@ -546,7 +552,7 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
   C.damping_alpha = prm->damping_alpha; C.damping_beta = prm->damping_beta;
   C.damping_gamma = prm->damping_gamma;
   C.adapt_dt_every = prm->adapt_dt_every;
-  C.contact = prm->contact ? 1 : 0;
+  C.contact = prm->contact;                      /* 0 none, 1 linear, 2 smooth */
   C.self_friction = (prm->mu_s != 0 || prm->mu_d != 0) ? 1 : 0;
   C.quirk_fulltime = prm->quirk_fulltime; C.quirk_volume_schedule = prm->quirk_volume_schedule;
   C.conv_tol = 0;
@@ -609,7 +615,7 @@ bool params_ok(const shx_params* prm)
 {
   return !(prm->thickness <= 0 || prm->E1 <= 0 || prm->E2 <= 0 || prm->G12 <= 0 || prm->rho <= 0 ||
            prm->nu < 0 || prm->nu >= 1 || prm->dt <= 0 || prm->newmark_beta < 0 ||
-           (prm->contact && prm->cell_size_mult <= 0) || prm->gravity_dir < 0 || prm->gravity_dir > 2 ||
+           prm->contact < 0 || prm->contact > 2 || (prm->contact && prm->cell_size_mult <= 0) || prm->gravity_dir < 0 || prm->gravity_dir > 2 ||
            !(prm->damping_rotation >= 0) ||
            prm->growth_base < 0 || prm->growth_base > 2 || prm->thickness_dir < 0 || prm->thickness_dir > 2 ||
            !(prm->metric_growth_rate >= 0) || !(prm->curvature_growth_rate >= 0) ||
@@ -817,6 +823,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       C.entbase = (unsigned)entb;
       entb += (unsigned long long)ne[s] * kEntPerElem;
       ctx->any_contact = true;
+      if (C.contact == 2) ctx->any_smooth = true;
     }
     ctx->C[s] = C;
     vb += nvp;
@@ -994,18 +1001,25 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
     if ((r = dalloc(ctx, P.U[b], (size_t)3 * nV)) || (r = dalloc(ctx, P.V[b], (size_t)3 * nV)) ||
         (r = dalloc(ctx, P.A[b], (size_t)3 * nV)))
       return fail(r);
+  P.rowcap = ctx->any_smooth ? kRowCapSmooth : kRowCap;
   if (ctx->any_contact) {
     if ((r = dalloc(ctx, P.ebox, (size_t)6 * nE)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ctx->ntile)) ||
         (r = dalloc(ctx, P.ecell, (size_t)8 * nE)) || (r = dalloc(ctx, P.bcnt, (size_t)bb)) ||
         (r = dalloc(ctx, P.boff, (size_t)bb)) || (r = dalloc(ctx, P.bcur, (size_t)bb)) ||
         (r = dalloc(ctx, P.bent, (size_t)entb)))
       return fail(r);
-    const size_t slots = (size_t)nE * kRowCap;
+    const size_t slots = (size_t)nE * P.rowcap;
     for (int b = 0; b < 2; ++b)
       if ((r = dalloc(ctx, P.rcnt[b], (size_t)nE)) || (r = dalloc(ctx, P.pe2[b], slots)) ||
           (r = dalloc(ctx, P.pd2[b], slots)) || (r = dalloc(ctx, P.pstick[b], slots)) ||
-          (r = dalloc(ctx, P.pso[b], 3 * slots)))
+          (r = dalloc(ctx, P.pso[b], 3 * slots)) || (ctx->any_smooth && (r = dalloc(ctx, P.pvw[b], 4 * slots))))
         return fail(r);
+    if (ctx->any_smooth) {
+      const double* lh = limit_table();
+      double* lt = nullptr;
+      if ((r = upload(ctx, lt, std::vector<double>(lh, lh + kLimitTabSize)))) return fail(r);
+      P.ltab = lt;
+    }
   }
 
   for (int s = 0; s < nshell; ++s)
@@ -1412,8 +1426,9 @@ int shx_batch_fields(shx_ctx* ctx, int shell, double* node_out, double* elem_out
   return SHX_OK;
 }
 
-int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, double* d2, int32_t* stick,
-                         double* origin, int cap)
+namespace {
+int pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, double* d2, int32_t* stick, double* origin,
+               double* vw, int cap)
 {
   if (!ctx) return SHX_E_ARG;
   if (int r = check_shell(ctx, shell)) return r;
@@ -1422,7 +1437,7 @@ int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, do
   std::vector<DShell> s;
   if (int r = read_sh(ctx, s)) return r;
   const int b = s[shell].pair_buf;
-  const size_t slots = (size_t)C.ne * kRowCap, off = (size_t)C.ebase * kRowCap;
+  const size_t slots = (size_t)C.ne * ctx->P.rowcap, off = (size_t)C.ebase * ctx->P.rowcap;
   std::vector<int32_t> cnt(C.ne), pe(slots), ps(slots);
   std::vector<double> pd(slots), po(3 * slots);
   HIPCHK(hipMemcpy(cnt.data(), ctx->P.rcnt[b] + C.ebase, sizeof(int32_t) * C.ne, hipMemcpyDeviceToHost));
@@ -1430,25 +1445,43 @@ int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, do
   HIPCHK(hipMemcpy(ps.data(), ctx->P.pstick[b] + off, sizeof(int32_t) * slots, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(pd.data(), ctx->P.pd2[b] + off, sizeof(double) * slots, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(po.data(), ctx->P.pso[b] + 3 * off, sizeof(double) * 3 * slots, hipMemcpyDeviceToHost));
+  std::vector<double> pv;
+  if (vw && C.contact == 2) {
+    pv.resize(4 * slots);
+    HIPCHK(hipMemcpy(pv.data(), ctx->P.pvw[b] + 4 * off, sizeof(double) * 4 * slots, hipMemcpyDeviceToHost));
+  }
   int w = 0;
   for (int e = 0; e < C.ne; ++e) {
-    const int n = std::min(cnt[e], kRowCap);
+    const int n = std::min(cnt[e], ctx->P.rowcap);
     std::vector<int> ord(n);
     for (int k = 0; k < n; ++k) ord[k] = k;
-    std::sort(ord.begin(), ord.end(), [&](int a, int c) { return pe[(size_t)e * kRowCap + a] < pe[(size_t)e * kRowCap + c]; });
+    std::sort(ord.begin(), ord.end(), [&](int a, int c) { return pe[(size_t)e * ctx->P.rowcap + a] < pe[(size_t)e * ctx->P.rowcap + c]; });
     for (int k : ord) {
-      const size_t q = (size_t)e * kRowCap + k;
+      const size_t q = (size_t)e * ctx->P.rowcap + k;
       if (w < cap) {
         if (e1) e1[w] = (uint32_t)e;
         if (e2) e2[w] = (uint32_t)(pe[q] - C.ebase);
         if (d2) d2[w] = pd[q];
         if (stick) stick[w] = ps[q];
         if (origin) for (int c = 0; c < 3; ++c) origin[3 * w + c] = po[3 * q + c];
+        if (vw) for (int c = 0; c < 4; ++c) vw[4 * w + c] = pv.empty() ? 0.0 : pv[4 * q + c];
       }
       ++w;
     }
   }
   return w;
+}
+} // namespace
+
+int shx_batch_pair_state(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, double* d2, int32_t* stick,
+                         double* origin, int cap)
+{
+  return pair_state(ctx, shell, e1, e2, d2, stick, origin, nullptr, cap);
+}
+
+int shx_batch_pair_coords(shx_ctx* ctx, int shell, double* vw, int cap)
+{
+  return pair_state(ctx, shell, nullptr, nullptr, nullptr, nullptr, nullptr, vw, cap);
 }
 
 int shx_batch_close_sets(shx_ctx* ctx, int shell, int32_t* start, int32_t* ids, int cap)
@@ -1519,14 +1552,14 @@ int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, con
   if (int r = read_sh(ctx, s)) return r;
   const int b = s[shell].pair_buf;
   std::vector<int32_t> cnt(C.ne, 0);
-  const size_t slots = (size_t)C.ne * kRowCap, off = (size_t)C.ebase * kRowCap;
+  const size_t slots = (size_t)C.ne * ctx->P.rowcap, off = (size_t)C.ebase * ctx->P.rowcap;
   std::vector<int32_t> pe(slots, 0), ps(slots, 0);
   std::vector<double> pd(slots, 0.0), po(3 * slots, 0.0);
   for (int k = 0; k < n; ++k) {
     const uint32_t a = std::min(e1[k], e2[k]), c = std::max(e1[k], e2[k]);
     if (c >= (uint32_t)C.ne || a == c) { ctx->err = "loaded pair out of range"; return SHX_E_ARG; }
-    if (cnt[a] >= kRowCap) { ctx->err = "loaded pairs overflow a contact row"; return SHX_E_CAPACITY; }
-    const size_t q = (size_t)a * kRowCap + cnt[a]++;
+    if (cnt[a] >= ctx->P.rowcap) { ctx->err = "loaded pairs overflow a contact row"; return SHX_E_CAPACITY; }
+    const size_t q = (size_t)a * ctx->P.rowcap + cnt[a]++;
     pe[q] = C.ebase + (int32_t)c;
     ps[q] = 1;                                 /* sticking: survives the next rebuild */
     pd[q] = 1.7976931348623157e308;
@@ -1536,17 +1569,17 @@ int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, con
   for (int e = 0; e < C.ne; ++e) {
     std::vector<int> ord(cnt[e]);
     for (int k = 0; k < cnt[e]; ++k) ord[k] = k;
-    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return pe[(size_t)e * kRowCap + x] < pe[(size_t)e * kRowCap + y]; });
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return pe[(size_t)e * ctx->P.rowcap + x] < pe[(size_t)e * ctx->P.rowcap + y]; });
     std::vector<int32_t> te(cnt[e]);
     std::vector<double> to(3 * cnt[e]);
     for (int k = 0; k < cnt[e]; ++k) {
-      const size_t q = (size_t)e * kRowCap + ord[k];
+      const size_t q = (size_t)e * ctx->P.rowcap + ord[k];
       te[k] = pe[q];
       for (int d = 0; d < 3; ++d) to[3 * k + d] = po[3 * q + d];
     }
     for (int k = 0; k < cnt[e]; ++k) {
-      pe[(size_t)e * kRowCap + k] = te[k];
-      for (int d = 0; d < 3; ++d) po[3 * ((size_t)e * kRowCap + k) + d] = to[3 * k + d];
+      pe[(size_t)e * ctx->P.rowcap + k] = te[k];
+      for (int d = 0; d < 3; ++d) po[3 * ((size_t)e * ctx->P.rowcap + k) + d] = to[3 * k + d];
     }
   }
   HIPCHK(hipMemcpy(ctx->P.rcnt[b] + C.ebase, cnt.data(), sizeof(int32_t) * C.ne, hipMemcpyHostToDevice));
@@ -1554,6 +1587,12 @@ int shx_batch_load_pairs(shx_ctx* ctx, int shell, int n, const uint32_t* e1, con
   HIPCHK(hipMemcpy(ctx->P.pstick[b] + off, ps.data(), sizeof(int32_t) * slots, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx->P.pd2[b] + off, pd.data(), sizeof(double) * slots, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx->P.pso[b] + 3 * off, po.data(), sizeof(double) * 3 * slots, hipMemcpyHostToDevice));
+  if (C.contact == 2) {
+    /* a loaded smooth contact starts its search at the element centroids
+     * (SmoothShellShellContact.C:12-18; ShellSystem.C:3116-3119) */
+    const std::vector<double> pv(4 * slots, 1.0 / 3.0);
+    HIPCHK(hipMemcpy(ctx->P.pvw[b] + 4 * off, pv.data(), sizeof(double) * 4 * slots, hipMemcpyHostToDevice));
+  }
   ctx->keep_rows[shell] = 1;
   return SHX_OK;
 }
@@ -1658,15 +1697,15 @@ int shx_batch_pairs(shx_ctx* ctx, int shell, uint32_t* e1, uint32_t* e2, int cap
   std::vector<int32_t> cnt(C.ne);
   HIPCHK(hipMemcpy(cnt.data(), ctx->P.rcnt[b] + C.ebase, sizeof(int32_t) * C.ne, hipMemcpyDeviceToHost));
   int n = 0;
-  for (int& c : cnt) { c = std::min(c, kRowCap); n += c; }
+  for (int& c : cnt) { c = std::min(c, ctx->P.rowcap); n += c; }
   if ((e1 || e2) && cap > 0 && n > 0) {
     /* rows in element order, each sorted by e2: the list sorted by (eid1, eid2) */
-    std::vector<int32_t> rows((size_t)C.ne * kRowCap);
-    HIPCHK(hipMemcpy(rows.data(), ctx->P.pe2[b] + (size_t)C.ebase * kRowCap, sizeof(int32_t) * rows.size(),
+    std::vector<int32_t> rows((size_t)C.ne * ctx->P.rowcap);
+    HIPCHK(hipMemcpy(rows.data(), ctx->P.pe2[b] + (size_t)C.ebase * ctx->P.rowcap, sizeof(int32_t) * rows.size(),
                      hipMemcpyDeviceToHost));
     int w = 0;
     for (int e = 0; e < C.ne && w < cap; ++e) {
-      int32_t* r = &rows[(size_t)e * kRowCap];
+      int32_t* r = &rows[(size_t)e * ctx->P.rowcap];
       std::sort(r, r + cnt[e]);
       for (int k = 0; k < cnt[e] && w < cap; ++k, ++w) {
         if (e1) e1[w] = (uint32_t)e;
@@ -1921,6 +1960,48 @@ int shx_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, int on_mesh
   return SHX_OK;
 }
 
+int shx_smooth_contact_compute(shx_ctx* ctx, int shell, int eid1, int eid2, double* x, double* out,
+                               int32_t* problem)
+{
+  if (!ctx || !x || !out) return SHX_E_ARG;
+  if (int r = check_shell(ctx, shell)) return r;
+  const Topology& T = ctx->T[shell];
+  if (eid1 < 0 || eid1 >= T.ne || eid2 < 0 || eid2 >= T.ne) { ctx->err = "element out of range"; return SHX_E_ARG; }
+  for (int e = 0; e < 2; ++e)
+    if (!(x[2 * e] >= 0) || !(x[2 * e + 1] >= 0) || !(x[2 * e] + x[2 * e + 1] <= 1)) {
+      ctx->err = "start point outside the elements";
+      return SHX_E_ARG;
+    }
+  std::vector<double> pos;
+  if (int r = shell_positions(ctx, shell, false, pos)) return r;
+  SmoothPair S;
+  S.tab = limit_table();
+  const int es[2] = {eid1, eid2};
+  for (int e = 0; e < 2; ++e) {
+    S.N[e] = T.valence[es[e]];
+    for (int j = 0; j < 12; ++j) {
+      int n = T.patch[(size_t)es[e] * kMaxPatch + j];
+      if (n < 0) n = T.patch[(size_t)es[e] * kMaxPatch];
+      S.X[e][j] = mk(pos[3 * (size_t)n], pos[3 * (size_t)n + 1], pos[3 * (size_t)n + 2]);
+    }
+  }
+  v3 dc;
+  int iters = 0;
+  const double d2 = smooth_closest(S, x, dc, &iters);
+  /* point() is ShellShellContact's (ShellShellContact.C:81-84): element 1's
+   * triangle at (v1, w1) plus half of dc */
+  v3 A[3];
+  for (int k = 0; k < 3; ++k) {
+    const int a = T.tri[3 * (size_t)eid1 + k];
+    A[k] = mk(pos[3 * (size_t)a], pos[3 * (size_t)a + 1], pos[3 * (size_t)a + 2]);
+  }
+  const v3 cp = add(add(add(scl(A[0], 1 - x[0] - x[1]), scl(A[1], x[0])), scl(A[2], x[1])), scl(dc, 0.5));
+  const double o[8] = {d2, dc.x, dc.y, dc.z, cp.x, cp.y, cp.z, (double)iters};
+  std::copy(o, o + 8, out);
+  if (problem) *problem = d2 <= 0 ? SHX_CONTACT_PROBLEM : SHX_NO_PROBLEM;
+  return SHX_OK;
+}
+
 void shx_friction_apply(double* Efric, double* f_contact, double f_normal, const double* n, const double* v,
                         double abs_v_normal, double v_stop, double spring_stiffness, double mu_s, double mu_d,
                         double damping, int32_t* stick, double* spring_origin, const double* contact_point)
@@ -1948,6 +2029,31 @@ void shx_tri_distance(const double* p, double* out)
   tri_tri(A, B, h);
   out[0] = h.d2; out[1] = h.dc.x; out[2] = h.dc.y; out[3] = h.dc.z;
   out[4] = h.v0; out[5] = h.v1; out[6] = h.w0; out[7] = h.w1;
+}
+
+int shx_smooth_closest(int valence1, const double* X1, int valence2, const double* X2, double* x, double* out)
+{
+  if (!X1 || !X2 || !x || !out || valence1 < 3 || valence1 > 6 || valence2 < 3 || valence2 > 6) return SHX_E_ARG;
+  for (int e = 0; e < 2; ++e)
+    if (!(x[2 * e] >= 0) || !(x[2 * e + 1] >= 0) || !(x[2 * e] + x[2 * e + 1] <= 1)) return SHX_E_ARG;
+  SmoothPair S;
+  S.tab = limit_table();
+  S.N[0] = valence1;
+  S.N[1] = valence2;
+  for (int e = 0; e < 2; ++e) {
+    const double* X = e ? X2 : X1;
+    const int P = S.N[e] + 6;
+    for (int j = 0; j < 12; ++j) {
+      const int q = j < P ? j : 0;
+      S.X[e][j] = mk(X[3 * q], X[3 * q + 1], X[3 * q + 2]);
+    }
+  }
+  v3 dc;
+  int iters = 0;
+  out[0] = smooth_closest(S, x, dc, &iters);
+  out[1] = dc.x; out[2] = dc.y; out[3] = dc.z;
+  out[4] = iters;
+  return SHX_OK;
 }
 
 int shx_subdiv_weights(int valence, double xi, double eta, double* out)

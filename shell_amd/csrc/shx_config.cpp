@@ -8,8 +8,7 @@
  * expressions, Function.h) compiled by expr_compile below.
  *
  * Features the path does not contain are refused when a config turns them
- * on (perturbation, smooth contact, a non-local pressure base, more than one
- * quadrature
+ * on (perturbation, a non-local pressure base, more than one quadrature
  * point, stand_still); keys that only matter to them are read and ignored.
  */
 #include <algorithm>
@@ -597,7 +596,7 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
     const std::string ct = R.upper("shell_shell_contact", "NONE");
     if (ct == "LINEAR") p.contact = 1;
     else if (ct == "NONE") p.contact = 0;
-    else if (ct == "SMOOTH") ok = R.fail("shell_shell_contact = smooth (limit-surface contact) is not part of this path");
+    else if (ct == "SMOOTH") p.contact = 2;
     else ok = R.fail("shell_shell_contact = " + ct + " (ASSERT(shell_shell_contact != INVALID_CT))");
   }
   ok = ok && R.boolean("shell_cavity_contact", true, cavity_contact);

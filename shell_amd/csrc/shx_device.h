@@ -82,7 +82,9 @@ constexpr int kEntPerElem = 64;     /* (cell, element) entries per element and r
 /* pair-search lanes per element (cells, then bucket entries): 8, 16 or 32,
  * a launch shape of the context (shx_shapes::bp_lanes) */
 constexpr int kCloseLds = 32;       /* close-set entries of an element staged in LDS by the pair search */
-constexpr int kRowCap = 64;         /* contact candidates per element row */
+constexpr int kRowCap = 64;         /* contact candidates per element row (DPtrs::rowcap) */
+constexpr int kRowCapSmooth = 256;  /* the same when a shell of the batch has smooth contact, whose
+                                       candidate distance is an element size longer */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
 constexpr int kCvRec = 4;           /* cavity partial record: nodes in contact, E_cav, sum of normal forces,
                                        cavity friction energy */
@@ -278,11 +280,17 @@ struct DPtrs {
   const int32_t* bmap[5];     /* per kernel family (tiles, FP64 element blocks of growing shells, FP64
                                  element blocks of the others, node blocks, FP32 element blocks): launch
                                  block -> global block of the active shells, set by the host */
+  int rowcap;                 /* slots per element row: kRowCap, or kRowCapSmooth with smooth contact */
   int32_t* rcnt[2];           /* nE: candidates in each element row */
-  int32_t* pe2[2];            /* nE*kRowCap: slot e*kRowCap + k */
+  int32_t* pe2[2];            /* nE*rowcap: slot e*rowcap + k */
   double* pd2[2];
   int32_t* pstick[2];
   double* pso[2];             /* 3 per slot */
+  /* smooth contact (allocated when a shell uses it): per slot the pair's last
+   * closest-point parameters v1 w1 v2 w2 (the next search starts there), and
+   * the limit-surface tables (kLimitTabSize doubles, shx_smooth.h) */
+  double* pvw[2];             /* 4 per slot */
+  const double* ltab;
 };
 
 } // namespace shx

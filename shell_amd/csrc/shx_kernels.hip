@@ -21,6 +21,7 @@
 #include "shx_device.h"
 #include "shx_kernels.h"
 #include "shx_math.h"
+#include "shx_smooth.h"
 #include "shx_dev.h"
 
 namespace shx {
@@ -599,6 +600,33 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
   return add(x, u);
 }
 
+/* SmoothShellShellContact::compute (SmoothShellShellContact.C:21-64) of
+ * the pair (e0, e1) at the current (or predicted) positions: the closest
+ * points of the two elements' limit surfaces, searched from xs (the pair's
+ * last minimiser; v1 w1 v2 w2), which takes the new one when store is set */
+__device__ void smooth_pair_hit(const DPtrs& P, int e0, int e1, const double* U, const double* V, const double* A,
+                                bool predict, double dt, double a8, double* xs, bool store, tri_hit& hit)
+{
+  SmoothPair S;
+  S.tab = P.ltab;
+  const int nE = P.nE;
+  const int es[2] = {e0, e1};
+  for (int s = 0; s < 2; ++s) {
+    S.N[s] = P.vcls[es[s]];
+    for (int j = 0; j < 12; ++j) S.X[s][j] = node_pos(P, P.pidx[(size_t)j * nE + es[s]], U, V, A, predict, dt, a8);
+  }
+  double x[4] = {xs[0], xs[1], xs[2], xs[3]};
+  v3 dc;
+  hit.d2 = smooth_closest(S, x, dc, nullptr);
+  hit.dc = dc;
+  hit.v0 = x[0];
+  hit.w0 = x[1];
+  hit.v1 = x[2];
+  hit.w1 = x[3];
+  if (store)
+    for (int q = 0; q < 4; ++q) xs[q] = x[q];
+}
+
 /* add_shell_shell_contact_force, ShellSystem.C:1792-1907: the candidate
  * pairs (e1, e2 > e1) of a 256-element tile, its rows laid end to end, are
  * split into 256 equal runs, one per thread (so a long row does not hold up
@@ -609,7 +637,11 @@ __device__ __forceinline__ v3 node_pos(const DPtrs& P, int n, const double* U, c
  * node kernel reads and clears them), so the node forces do not depend on
  * the order in which the pairs are processed.  Block partials: contacts, E_self, E_fric, candidates.  The blocks of
  * a shell whose list was rebuilt in this step clear its cell buckets.
- * Launch block b takes 256-element tile bmap[V_TILE][b]. */
+ * Launch block b takes 256-element tile bmap[V_TILE][b].  SMOOTH: shells
+ * with smooth contact (contact == 2) take the pair's limit-surface closest
+ * points instead of the triangles' (the force is distributed the same way,
+ * ShellSystem.C:1833-1882). */
+template <bool SMOOTH>
 __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
 {
   __shared__ double s_w[kCRec][4];
@@ -637,7 +669,7 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
   double nc = 0, es = 0, ef = 0, npair = 0;
   {
     const int e = tile * 256 + t;
-    const int np = e - C.ebase < C.ne ? min(P.rcnt[buf][e], kRowCap) : 0;
+    const int np = e - C.ebase < C.ne ? min(P.rcnt[buf][e], SMOOTH ? P.rowcap : kRowCap) : 0;
     npair = np;
     if (np > 0)
       for (int k = 0; k < 3; ++k) {
@@ -686,7 +718,7 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
       X0[k] = mk(s_x0[3 * k][lo], s_x0[3 * k + 1][lo], s_x0[3 * k + 2][lo]);
     }
     {
-      const size_t p = (size_t)e0 * kRowCap + q;
+      const size_t p = (size_t)e0 * (SMOOTH ? P.rowcap : kRowCap) + q;
       const int e1 = P.pe2[buf][p];
       v3 X1[3];
       int nid[2][3];
@@ -696,8 +728,11 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
         X1[k] = node_pos(P, nid[1][k], U, V, A, predict, dt, a8);
       }
       tri_hit hit;
-      tri_tri(X0, X1, hit);
       const bool keep_state = sh.measure_only;
+      if (SMOOTH && C.contact == 2)
+        smooth_pair_hit(P, e0, e1, U, V, A, predict, dt, a8, P.pvw[buf] + 4 * p, !keep_state, hit);
+      else
+        tri_tri(X0, X1, hit);
       if (!keep_state) P.pd2[buf][p] = hit.d2;
       if (hit.d2 <= 0) atomicMax(&P.sh[sid].problem, 6); /* CONTACT_PROBLEM */
       int stick = P.pstick[buf][p];
@@ -1459,6 +1494,9 @@ __device__ bool shell_grid(const DPtrs& P, const SConst& C, Grid& g)
   cell += tol;
   g.cell = cell;
   g.dmx = C.h + tol;
+  /* smooth contact: the limit surfaces can be up to an element closer than
+   * the triangles (ShellSystem.C:1940-1944) */
+  if (C.contact == 2) g.dmx += g.avg;
   for (int c = 0; c < 3; ++c) g.nd[c] = (unsigned)(int)(((g.bmax[c] - g.bmin[c]) + g.dmx) / cell + 1);
   g.ncell = g.nd[0] * g.nd[1] * g.nd[2];
   if (g.ncell > C.n_cells_max) { g.problem = 4; return false; }
@@ -1669,13 +1707,13 @@ __device__ __forceinline__ void pairs_body(const DPtrs& P, int posmode, int list
     const int e1 = C.ebase + (vb - s_pref[li]) * kBpPerBlock + el;
     const bool valid = e1 - C.ebase < C.ne;
     const int ob = sh->pair_src, nb = sh->pair_buf;
-    const size_t row = (size_t)e1 * kRowCap;
+    const size_t row = (size_t)e1 * P.rowcap;
     int nold = 0, cl0 = 0, ncl = 0;
     if (valid) {
-      nold = min(P.rcnt[ob][e1], kRowCap);
+      nold = min(P.rcnt[ob][e1], P.rowcap);
       cl0 = P.close_start[e1];
       ncl = P.close_start[e1 + 1] - cl0;
-      for (int r = k0; r < nold; r += kBpLanes) s_old[el][r] = P.pe2[ob][row + r];
+      for (int r = k0; r < min(nold, kRowCap); r += kBpLanes) s_old[el][r] = P.pe2[ob][row + r];
       if (ncl <= kCloseLds)
         for (int r = k0; r < ncl; r += kBpLanes) s_close[el][r] = P.close_ids[cl0 + r];
     }
@@ -1692,12 +1730,14 @@ __device__ __forceinline__ void pairs_body(const DPtrs& P, int posmode, int list
         const size_t po = row + r;
         if (!(P.pstick[ob][po] || !(P.pd2[ob][po] > d2mx))) continue;
         const int slot = atomicAdd(&P.rcnt[nb][e1], 1);
-        if (slot >= kRowCap) { atomicOr(&sh->err, ERR_ROW); continue; }
+        if (slot >= P.rowcap) { atomicOr(&sh->err, ERR_ROW); continue; }
         const size_t pn = row + slot;
         P.pe2[nb][pn] = P.pe2[ob][po];
         P.pd2[nb][pn] = P.pd2[ob][po];
         P.pstick[nb][pn] = P.pstick[ob][po];
         st3(P.pso[nb], pn, ld3(P.pso[ob], po));
+        if (P.pvw[0])
+          for (int q = 0; q < 4; ++q) P.pvw[nb][4 * pn + q] = P.pvw[ob][4 * po + q];
       }
     const int ny = hi1[1] - lo1[1] + 1, nz = hi1[2] - lo1[2] + 1;
     const int ncells = (hi1[0] - lo1[0] + 1) * ny * nz;
@@ -1740,15 +1780,17 @@ __device__ __forceinline__ void pairs_body(const DPtrs& P, int posmode, int list
         if (boxes_apart(b1, load_box(P, e2), dmx)) continue;
         if (ncl <= kCloseLds ? sorted_has(s_close[el], ncl, e2) : close_pair(P, e1, e2)) continue;
         bool listed = false;
+        /* the old row's first kRowCap ids are staged in LDS, the rest (rows
+         * of the larger smooth-contact capacity) read from memory */
         for (int r = 0; r < nold; ++r)
-          if (s_old[el][r] == e2) {
+          if ((r < kRowCap ? s_old[el][r] : P.pe2[ob][row + r]) == e2) {
             listed = P.pstick[ob][row + r] || !(P.pd2[ob][row + r] > d2mx);
             break;
           }
         if (listed) continue;
         /* a candidate: k_bp_exact forms its distance and keeps it if d2 <= d_max^2 */
         const int slot = atomicAdd(&P.rcnt[nb][e1], 1);
-        if (slot >= kRowCap) { atomicOr(&sh->err, ERR_ROW); continue; }
+        if (slot >= P.rowcap) { atomicOr(&sh->err, ERR_ROW); continue; }
         const size_t pn = row + slot;
         P.pe2[nb][pn] = e2;
         P.pd2[nb][pn] = kTentative;
@@ -1766,6 +1808,7 @@ __device__ __forceinline__ void pairs_body(const DPtrs& P, int posmode, int list
  * CONTACT_PROBLEM): one thread per element row, the row compacted in place
  * and sorted by e2.  Kept apart from the pair search so that the search runs at
  * the occupancy of its own (small) register footprint. */
+template <bool SMOOTH>
 __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list)
 {
   if (P.wl[list][0] == 0) return;
@@ -1788,9 +1831,12 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
     const double* A = P.A[par];
     const double d2mx = sh->d2_max;
     const int nb = sh->pair_buf;
-    const size_t row = (size_t)e1 * kRowCap;
+    /* the plain variants run only in batches without smooth contact, whose
+     * rows have kRowCap slots */
+    const int rcap = SMOOTH ? P.rowcap : kRowCap;
+    const size_t row = (size_t)e1 * rcap;
     const int n = P.rcnt[nb][e1];
-    if (n > kRowCap) continue;             /* the row overflowed: the shell reports ERR_ROW */
+    if (n > rcap) continue;             /* the row overflowed: the shell reports ERR_ROW */
     bool have_x0 = false;
     v3 X0[3];
     int w = 0;
@@ -1803,10 +1849,18 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
           for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * nE + e1], U, V, A, predict, dt, a8);
           have_x0 = true;
         }
-        v3 X1[3];
-        for (int j = 0; j < 3; ++j) X1[j] = node_pos(P, P.tri[(size_t)j * nE + e2], U, V, A, predict, dt, a8);
         tri_hit hit;
-        tri_tri(X0, X1, hit);
+        if (SMOOTH && C.contact == 2) {
+          /* a new smooth contact starts at the element centroids
+           * (SmoothShellShellContact.C:12-18) */
+          double* xs = P.pvw[nb] + 4 * p;
+          for (int q = 0; q < 4; ++q) xs[q] = 1.0 / 3.0;
+          smooth_pair_hit(P, e1, e2, U, V, A, predict, dt, a8, xs, true, hit);
+        } else {
+          v3 X1[3];
+          for (int j = 0; j < 3; ++j) X1[j] = node_pos(P, P.tri[(size_t)j * nE + e2], U, V, A, predict, dt, a8);
+          tri_tri(X0, X1, hit);
+        }
         if (hit.d2 <= 0) {
           atomicMax(&sh->problem, 6);
           if (list == 1) stop_after_corrector(sh);
@@ -1819,6 +1873,8 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
         P.pe2[nb][q] = P.pe2[nb][p];
         P.pstick[nb][q] = P.pstick[nb][p];
         st3(P.pso[nb], q, ld3(P.pso[nb], p));
+        if (SMOOTH)
+          for (int c = 0; c < 4; ++c) P.pvw[nb][4 * q + c] = P.pvw[nb][4 * p + c];
       }
       P.pd2[nb][q] = d2;
       ++w;
@@ -1833,6 +1889,9 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
       const double d2 = P.pd2[nb][pi];
       const int stk = P.pstick[nb][pi];
       const v3 so = ld3(P.pso[nb], pi);
+      double xs[4] = {0, 0, 0, 0};
+      if (SMOOTH)
+        for (int c = 0; c < 4; ++c) xs[c] = P.pvw[nb][4 * pi + c];
       int j = i - 1;
       for (; j >= 0 && P.pe2[nb][row + j] > e2; --j) {
         const size_t a = row + j, b = row + j + 1;
@@ -1840,6 +1899,8 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
         P.pd2[nb][b] = P.pd2[nb][a];
         P.pstick[nb][b] = P.pstick[nb][a];
         st3(P.pso[nb], b, ld3(P.pso[nb], a));
+        if (SMOOTH)
+          for (int c = 0; c < 4; ++c) P.pvw[nb][4 * b + c] = P.pvw[nb][4 * a + c];
       }
       if (j + 1 != i) {
         const size_t b = row + j + 1;
@@ -1847,6 +1908,8 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
         P.pd2[nb][b] = d2;
         P.pstick[nb][b] = stk;
         st3(P.pso[nb], b, so);
+        if (SMOOTH)
+          for (int c = 0; c < 4; ++c) P.pvw[nb][4 * b + c] = xs[c];
       }
     }
   }
@@ -1882,9 +1945,17 @@ __global__ void __launch_bounds__(256) k_bp_host_done(DPtrs P)
 /* launch-shape variants of the tunable kernels (waves per SIMD through
  * the launch bounds; pair-search lanes per element), picked by the
  * context's shx_shapes */
-__global__ void __launch_bounds__(256, 2) k_contact_w2(DPtrs P, int mode) { contact_body(P, mode); }
-__global__ void __launch_bounds__(256, 3) k_contact_w3(DPtrs P, int mode) { contact_body(P, mode); }
-__global__ void __launch_bounds__(256, 4) k_contact_w4(DPtrs P, int mode) { contact_body(P, mode); }
+__global__ void __launch_bounds__(256, 2) k_contact_w2(DPtrs P, int mode) { contact_body<false>(P, mode); }
+__global__ void __launch_bounds__(256, 3) k_contact_w3(DPtrs P, int mode) { contact_body<false>(P, mode); }
+__global__ void __launch_bounds__(256, 4) k_contact_w4(DPtrs P, int mode) { contact_body<false>(P, mode); }
+/* batches with smooth contact: the narrow phase and the candidate distances
+ * with the limit-surface search (linear shells of the batch take the
+ * triangle path inside them) */
+__global__ void __launch_bounds__(256) k_contact_smooth(DPtrs P, int mode) { contact_body<true>(P, mode); }
+__global__ void __launch_bounds__(256) k_bp_exact_smooth(DPtrs P, int posmode, int list)
+{
+  exact_body<true>(P, posmode, list);
+}
 __global__ void __launch_bounds__(kNodeBlock, 4) k_node_w4(DPtrs P, int mode) { node_body<false>(P, mode); }
 __global__ void __launch_bounds__(kNodeBlock, 6) k_node_w6(DPtrs P, int mode) { node_body<false>(P, mode); }
 __global__ void __launch_bounds__(kNodeBlock, 8) k_node_w8(DPtrs P, int mode) { node_body<false>(P, mode); }
@@ -1894,8 +1965,8 @@ __global__ void __launch_bounds__(kNodeBlock, 4) k_node_loads(DPtrs P, int mode)
 __global__ void __launch_bounds__(256) k_bp_pairs_l8(DPtrs P, int posmode, int list) { pairs_body<8>(P, posmode, list); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l16(DPtrs P, int posmode, int list) { pairs_body<16>(P, posmode, list); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l32(DPtrs P, int posmode, int list) { pairs_body<32>(P, posmode, list); }
-__global__ void __launch_bounds__(256, 2) k_bp_exact_w2(DPtrs P, int posmode, int list) { exact_body(P, posmode, list); }
-__global__ void __launch_bounds__(256, 3) k_bp_exact_w3(DPtrs P, int posmode, int list) { exact_body(P, posmode, list); }
-__global__ void __launch_bounds__(256, 4) k_bp_exact_w4(DPtrs P, int posmode, int list) { exact_body(P, posmode, list); }
+__global__ void __launch_bounds__(256, 2) k_bp_exact_w2(DPtrs P, int posmode, int list) { exact_body<false>(P, posmode, list); }
+__global__ void __launch_bounds__(256, 3) k_bp_exact_w3(DPtrs P, int posmode, int list) { exact_body<false>(P, posmode, list); }
+__global__ void __launch_bounds__(256, 4) k_bp_exact_w4(DPtrs P, int posmode, int list) { exact_body<false>(P, posmode, list); }
 
 } // namespace shx

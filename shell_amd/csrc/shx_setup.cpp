@@ -4,10 +4,12 @@
  */
 #include "shx_setup.h"
 #include "shx_math.h"
+#include "shx_smooth.h"
 
 #include <algorithm>
 #include <initializer_list>
 #include <cmath>
+#include <vector>
 
 namespace shx {
 
@@ -193,66 +195,59 @@ void loop_extended(int N, double Ab[kMaxPatch + 6][kMaxPatch])
 
 } // namespace
 
+const double* limit_table()
+{
+  /* built once: the box splines' monomial coefficients and, per valence
+   * 3..5, sub-patch k and level n, M = P_k Ab A^(n-1) with rows of A the
+   * first N+6 rows of Ab, multiplied out left to right */
+  static const std::vector<double> tab = [] {
+    std::vector<double> t(kLimitTabSize, 0.0);
+    for (int k = 0; k < 12; ++k) {
+      long long M[5][5];
+      box_monomials(k, M);
+      int m = 0;
+      for (int a = 0; a < 5; ++a)
+        for (int b = 0; a + b < 5; ++b, ++m) t[k * kBoxMono + m] = (double)M[a][b];
+    }
+    for (int N = 3; N <= 5; ++N) {
+      const int P = N + 6;
+      /* P_k: box spline i takes fine point sel[k][i] (SubdivEvaluation.h:31-77) */
+      const int sel[3][12] = {{2, 0, N + 3, 1, N, N + 8, N + 2, N + 1, N + 4, N + 7, N + 6, N + 9},
+                              {N + 9, N + 6, N + 4, N + 1, N + 2, N + 5, N, 1, N + 3, N - 1, 0, 2},
+                              {0, N - 1, 1, N, N + 5, N + 2, N + 1, N + 4, N + 11, N + 6, N + 9, N + 10}};
+      double Ab[kMaxPatch + 6][kMaxPatch];
+      loop_extended(N, Ab);
+      for (int k = 0; k < 3; ++k) {
+        double M[12][kMaxPatch], T[12][kMaxPatch];
+        for (int i = 0; i < 12; ++i)
+          for (int j = 0; j < P; ++j) M[i][j] = Ab[sel[k][i]][j];
+        for (int n = 1; n <= kLimitLevels; ++n) {
+          if (n > 1) {
+            for (int i = 0; i < 12; ++i)
+              for (int j = 0; j < P; ++j) {
+                double s = 0;
+                for (int q = 0; q < P; ++q) s += M[i][q] * Ab[q][j];
+                T[i][j] = s;
+              }
+            for (int i = 0; i < 12; ++i)
+              for (int j = 0; j < P; ++j) M[i][j] = T[i][j];
+          }
+          double* o = t.data() + limit_tab_off(N, k, n);
+          for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < P; ++j) o[i * P + j] = M[i][j];
+        }
+      }
+    }
+    return t;
+  }();
+  return tab.data();
+}
+
+/* SubdivEvaluation.C:37-80, 112-165, 170-187 through the tables */
 bool subdiv_weights(int valence, double xi, double eta, double w[3][kMaxPatch])
 {
   if (valence < 3 || valence > 6) return false;
-  const int N = valence, P = N + 6;
-  for (int q = 0; q < 3; ++q)
-    for (int j = 0; j < kMaxPatch; ++j) w[q][j] = 0;
-  double b[6][12];
-  if (N == 6) {
-    static const int cvi[12] = {3, 6, 2, 0, 1, 4, 7, 10, 9, 5, 11, 8};   /* SubdivEvaluation.C:10 */
-    box_eval(xi, eta, b);
-    for (int q = 0; q < 3; ++q)
-      for (int j = 0; j < 12; ++j) w[q][j] = b[q][cvi[j]];
-    return true;
-  }
-  /* SubdivEvaluation.C:37-80, 112-165, 170-187 */
-  const double tol = 1e-10;
-  if (xi + eta < tol) { xi = std::max(xi, tol); eta = std::max(eta, 0.0); }
-  const int n = (int)std::floor(1 - std::log(xi + eta) / std::log(2.0));
-  const double pow2 = std::pow(2.0, n - 1);
-  xi *= pow2;
-  eta *= pow2;
-  int k;
-  double jfac;
-  if (xi > 0.5) { k = 0; xi = 2 * xi - 1; eta = 2 * eta; jfac = std::pow(2.0, n); }
-  else if (eta > 0.5) { k = 2; xi = 2 * xi; eta = 2 * eta - 1; jfac = std::pow(2.0, n); }
-  else {
-    /* x' = 1 - 2^n xi: the chain rule gives -2^n.  The reference writes
-     * pow(-2, n) (SubdivEvaluation.C:136), whose sign is wrong for even n
-     * (xi + eta in [1/4, 1/2), [1/16, 1/8), ...); the tangents here are
-     * the derivatives of the evaluated surface */
-    k = 1; xi = 1 - 2 * xi; eta = 1 - 2 * eta; jfac = -std::pow(2.0, n);
-  }
-  /* P_k: box spline i takes fine point sel[k][i] (SubdivEvaluation.h:31-77) */
-  const int sel[3][12] = {{2, 0, N + 3, 1, N, N + 8, N + 2, N + 1, N + 4, N + 7, N + 6, N + 9},
-                          {N + 9, N + 6, N + 4, N + 1, N + 2, N + 5, N, 1, N + 3, N - 1, 0, 2},
-                          {0, N - 1, 1, N, N + 5, N + 2, N + 1, N + 4, N + 11, N + 6, N + 9, N + 10}};
-  double Ab[kMaxPatch + 6][kMaxPatch];
-  loop_extended(N, Ab);
-  /* M = P_k Ab A^(n-1): 12 x P; rows of A are the first P rows of Ab */
-  double M[12][kMaxPatch], T[12][kMaxPatch];
-  for (int i = 0; i < 12; ++i)
-    for (int j = 0; j < P; ++j) M[i][j] = Ab[sel[k][i]][j];
-  for (int it = 1; it < n; ++it) {
-    for (int i = 0; i < 12; ++i)
-      for (int j = 0; j < P; ++j) {
-        double s = 0;
-        for (int m = 0; m < P; ++m) s += M[i][m] * Ab[m][j];
-        T[i][j] = s;
-      }
-    for (int i = 0; i < 12; ++i)
-      for (int j = 0; j < P; ++j) M[i][j] = T[i][j];
-  }
-  box_eval(xi, eta, b);
-  const double f[3] = {1.0, jfac, jfac};
-  for (int q = 0; q < 3; ++q)
-    for (int j = 0; j < P; ++j) {
-      double s = 0;
-      for (int i = 0; i < 12; ++i) s += M[i][j] * b[q][i];
-      w[q][j] = f[q] * s;
-    }
+  limit_weights(limit_table(), valence, xi, eta, w);
   return true;
 }
 

@@ -36,6 +36,11 @@ bool loop_basis_at_centroid(int valence, double out[6 * kMaxPatch]);
  * the box splines of the sub-patch that holds the point. */
 bool subdiv_weights(int valence, double xi, double eta, double w[3][kMaxPatch]);
 
+/* the limit-surface tables of shx_smooth.h (kLimitTabSize doubles: box
+ * spline monomials, then P_k Ab A^(n-1) per valence 3..5, k, n), built on
+ * first use */
+const double* limit_table();
+
 
 } // namespace shx
 #endif

@@ -91,6 +91,14 @@ def test_config_maps_keys_onto_params():
     assert "damping_beta=0" in warn and "pressure_ramp_time" not in warn
 
 
+def test_contact_types():
+    """shell_shell_contact (STR_TO_ENUM, ShellSystem.C:257-260): none,
+    linear and smooth (the limit-surface contact)"""
+    for word, code in (("none", 0), ("linear", 1), ("smooth", 2), ("SMOOTH", 2)):
+        p, _, _ = S.read_config(BASE.replace("shell_shell_contact     = linear", "shell_shell_contact = " + word))
+        assert p["contact"] == code
+
+
 def test_command_line_overrides_and_constant_functions():
     p, d, warn = S.read_config(BASE, ["-dt", "0.02", "-pressure", "-3e-4", "-time_steps", "7", "-bogus", "1"])
     assert p["dt"] == 0.02 and d["time_steps"] == 7
@@ -111,7 +119,7 @@ def test_command_line_overrides_and_constant_functions():
     ("shell_cavity_contact = true\ncavity = sphere", "INVALID_CAV"),
     ("shell_cavity_contact = true\nmu_s_cavity = -0.1", "cavity-contact ASSERT"),
     ("shell_cavity_contact = true\nmu_d_cavity = 0.1*t", "depends on t"),
-    ("shell_shell_contact = smooth", "smooth"),
+    ("shell_shell_contact = curved", "INVALID_CT"),
     ("pressure_base = cartesian", "local"),
     ("extra_order = 1", "quadrature"),
     ("shell_shear_modulus = 0", "ASSERT(shell_shear_modulus > 0)"),

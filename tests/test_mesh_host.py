@@ -80,12 +80,12 @@ def test_driver_argument_and_config_errors(tmp_path):
     r = _run(["-c", "missing.cfg"], tmp_path)
     assert r.returncode == 3 and "Can't open config file \"missing.cfg\"" in r.stdout
     cfg = tmp_path / "config.cfg"
-    cfg.write_text("shell_young_modulus2 = 1e9\nshell_shear_modulus = 3.5e8\nshell_shell_contact = smooth\n")
+    cfg.write_text("shell_young_modulus2 = 1e9\nshell_shear_modulus = 3.5e8\nextra_order = 1\n")
     r = _run([], tmp_path)
-    assert r.returncode == 3 and "smooth" in r.stdout
-    r = _run(["-shell_shell_contact", "linear"], tmp_path)            # no mesh given
+    assert r.returncode == 3 and "quadrature" in r.stdout
+    r = _run(["-extra_order", "0"], tmp_path)            # no mesh given
     assert r.returncode == 3 and "-m meshfile.off" in r.stdout
-    r = _run(["-shell_shell_contact", "linear", "-m", "nope.off"], tmp_path)
+    r = _run(["-extra_order", "0", "-m", "nope.off"], tmp_path)
     assert r.returncode == 3 and "Can't open mesh file" in r.stdout
     # the config is copied into the data directory, as shell.C:105 does
     assert (tmp_path / "data" / "config.cfg").exists()
