@@ -343,8 +343,10 @@ void launch_broad(shx_ctx* ctx, int posmode, int list)
   }
   prof_end(ctx);
   prof_begin(ctx, 9);
-  if (ctx->any_smooth)
+  if (ctx->any_smooth) {
+    k_bp_smooth_new<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
     k_bp_exact_smooth<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list);
+  }
   else switch (ctx->shapes.exact_waves) {
     case 3: k_bp_exact_w3<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
     case 4: k_bp_exact_w4<<<kBpGrid, 256, 0, ctx->st>>>(P, posmode, list); break;
@@ -431,7 +433,7 @@ void launch_contact(shx_ctx* ctx, int mode)
   prof_begin(ctx, 2);
   if (ctx->grid[0]) {
     if (ctx->any_smooth)
-      k_contact_smooth<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode);
+      k_contact_smooth<<<ctx->grid[0] * kSmoothSplit, 256, 0, ctx->st>>>(ctx->P, mode);
     else switch (ctx->shapes.contact_waves) {
       case 2: k_contact_w2<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode); break;
       case 4: k_contact_w4<<<ctx->grid[0], 256, 0, ctx->st>>>(ctx->P, mode); break;
@@ -984,7 +986,8 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       (r = dalloc(ctx, P.nmeasq, (size_t)4 * nV)) || (r = dalloc(ctx, P.nmeas, (size_t)2 * nV)) ||
       (r = dalloc(ctx, P.fext_out, (size_t)3 * nV)) || (r = dalloc(ctx, P.epart, (size_t)kElemRec * ctx->nblkE)) ||
       (r = dalloc(ctx, P.npart, (size_t)kNodeRec * ctx->nblkN)) ||
-      (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile)) || (r = dalloc(ctx, P.sh, nshell)) ||
+      (r = dalloc(ctx, P.cpart, (size_t)kCRec * ctx->ntile * (ctx->any_smooth ? kSmoothSplit : 1))) ||
+      (r = dalloc(ctx, P.sh, nshell)) ||
       (r = dalloc(ctx, P.cvpart, (size_t)kCvRec * ctx->nblkN)) || (r = dalloc(ctx, P.rpart, (size_t)kRotRec * ctx->nblkN)) ||
       (r = dalloc(ctx, P.cso, (size_t)3 * nV)) || (r = dalloc(ctx, P.ncav, (size_t)2 * nV)) ||
       (r = dalloc(ctx, P.gG, ctx->any_grow ? (size_t)3 * nE : 1)) ||
@@ -1002,6 +1005,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
         (r = dalloc(ctx, P.A[b], (size_t)3 * nV)))
       return fail(r);
   P.rowcap = ctx->any_smooth ? kRowCapSmooth : kRowCap;
+  P.csplit = ctx->any_smooth ? kSmoothSplit : 1;
   if (ctx->any_contact) {
     if ((r = dalloc(ctx, P.ebox, (size_t)6 * nE)) || (r = dalloc(ctx, P.bpart, (size_t)7 * ctx->ntile)) ||
         (r = dalloc(ctx, P.ecell, (size_t)8 * nE)) || (r = dalloc(ctx, P.bcnt, (size_t)bb)) ||

@@ -85,6 +85,7 @@ constexpr int kCloseLds = 32;       /* close-set entries of an element staged in
 constexpr int kRowCap = 64;         /* contact candidates per element row (DPtrs::rowcap) */
 constexpr int kRowCapSmooth = 256;  /* the same when a shell of the batch has smooth contact, whose
                                        candidate distance is an element size longer */
+constexpr int kSmoothSplit = 16;    /* blocks per 256-element tile in the smooth-contact kernels */
 constexpr int kCRec = 4;            /* contact-kernel partial record length */
 constexpr int kCvRec = 4;           /* cavity partial record: nodes in contact, E_cav, sum of normal forces,
                                        cavity friction energy */
@@ -250,7 +251,7 @@ struct DPtrs {
   double* fext_out;
   double* epart;              /* element kernel block partials (kElemRec per block) */
   double* npart;              /* node kernel block partials (kNodeRec per block) */
-  double* cpart;              /* contact kernel block partials (kCRec per 256-element tile) */
+  double* cpart;              /* contact kernel block partials (kCRec per tile and share, csplit shares) */
   double* cvpart;             /* node kernel block partials of the cavity contact (kCvRec per block) */
   double* rpart;              /* rotation-damping block partials: L (3), I (6 unique) per node block */
   double* cso;                /* nV*3: cavity friction spring origin of each node (NodeCache::spring_origin) */
@@ -281,6 +282,8 @@ struct DPtrs {
                                  element blocks of the others, node blocks, FP32 element blocks): launch
                                  block -> global block of the active shells, set by the host */
   int rowcap;                 /* slots per element row: kRowCap, or kRowCapSmooth with smooth contact */
+  int csplit;                 /* contact-kernel blocks (and partial records) per tile: 1, or
+                                 kSmoothSplit with smooth contact */
   int32_t* rcnt[2];           /* nE: candidates in each element row */
   int32_t* pe2[2];            /* nE*rowcap: slot e*rowcap + k */
   double* pd2[2];

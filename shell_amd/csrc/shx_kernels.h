@@ -52,6 +52,7 @@ __global__ void k_bp_exact_w3(DPtrs P, int posmode, int list);
 __global__ void k_bp_exact_w4(DPtrs P, int posmode, int list);
 __global__ void k_contact_smooth(DPtrs P, int mode);
 __global__ void k_bp_exact_smooth(DPtrs P, int posmode, int list);
+__global__ void k_bp_smooth_new(DPtrs P, int posmode, int list);
 __global__ void k_bp_host_plan(DPtrs P);
 __global__ void k_bp_host_done(DPtrs P);
 

@@ -640,7 +640,10 @@ __device__ void smooth_pair_hit(const DPtrs& P, int e0, int e1, const double* U,
  * Launch block b takes 256-element tile bmap[V_TILE][b].  SMOOTH: shells
  * with smooth contact (contact == 2) take the pair's limit-surface closest
  * points instead of the triangles' (the force is distributed the same way,
- * ShellSystem.C:1833-1882). */
+ * ShellSystem.C:1833-1882), and a tile's pairs are shared by P.csplit
+ * blocks (block b: tile bmap[V_TILE][b / csplit], share b % csplit), each
+ * with its own partial record, since one search costs hundreds of the
+ * triangle test's operations. */
 template <bool SMOOTH>
 __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
 {
@@ -648,12 +651,14 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
   __shared__ int s_pre[257];             /* exclusive prefix of the tile's row lengths */
   __shared__ double s_x0[9][256];        /* per row with pairs: its element's vertex positions */
   __shared__ int s_n0[3][256];           /* and vertex ids */
-  const int tile = uni(P.bmap[V_TILE][blockIdx.x]);
+  const int split = SMOOTH ? P.csplit : 1;
+  const int part = SMOOTH ? (int)blockIdx.x % split : 0;
+  const int tile = uni(P.bmap[V_TILE][SMOOTH ? (int)blockIdx.x / split : (int)blockIdx.x]);
   const int sid = uni(P.eshell[tile * 256]);
   const SConst& C = P.cs[sid];
   const DShell& sh = P.sh[sid];
   if (!C.contact) return;
-  if (sh.bclear) clear_buckets(P, C, tile - C.tile0, C.ntile);
+  if (sh.bclear) clear_buckets(P, C, (tile - C.tile0) * split + part, C.ntile * split);
   if (sh.done) return;
   {
   const bool predict = (mode == M_STEP);
@@ -670,7 +675,7 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
   {
     const int e = tile * 256 + t;
     const int np = e - C.ebase < C.ne ? min(P.rcnt[buf][e], SMOOTH ? P.rowcap : kRowCap) : 0;
-    npair = np;
+    npair = part == 0 ? np : 0;          /* the tile's candidates, counted by its first share */
     if (np > 0)
       for (int k = 0; k < 3; ++k) {
         const int n = P.tri[(size_t)k * nE + e];
@@ -695,9 +700,12 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
     if (t >= 64) s_pre[t + 1] += (t & 63) == 63 ? 0 : s_pre[64 * (t >> 6)];
     __syncthreads();
   }
-  const int total = s_pre[256];
-  const int run = (total + 255) / 256;
-  const int k0 = t * run, k1 = min(total, k0 + run);
+  /* this block's share of the tile's pairs, in runs of equal length per thread */
+  const int all = s_pre[256];
+  const int share = (all + split - 1) / split;
+  const int first = min(all, part * share), total = min(all, first + share);
+  const int run = (total - first + 255) / 256;
+  const int k0 = first + t * run, k1 = min(total, k0 + run);
   int lo = 0;
   if (k0 < k1) {
     /* row of pair k0: the last r with s_pre[r] <= k0 */
@@ -708,7 +716,7 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
     }
   }
   for (int kp = k0; kp < k1; ++kp) {
-    while (kp >= s_pre[lo + 1]) ++lo;    /* kp < total = s_pre[256]: lo stays below 256 */
+    while (kp >= s_pre[lo + 1]) ++lo;    /* kp < total <= s_pre[256]: lo stays below 256 */
     const int q = kp - s_pre[lo];
     const int e0 = tile * 256 + lo;
     int nid0[3];
@@ -807,7 +815,8 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
 #pragma unroll
     for (int c = 0; c < kCRec; ++c) s_w[c][t / 64] = rr[c];
   __syncthreads();
-  if (t < kCRec) P.cpart[(size_t)tile * kCRec + t] = ((s_w[t][0] + s_w[t][1]) + s_w[t][2]) + s_w[t][3];
+  if (t < kCRec)
+    P.cpart[((size_t)tile * split + part) * kCRec + t] = ((s_w[t][0] + s_w[t][1]) + s_w[t][2]) + s_w[t][3];
   }
 }
 
@@ -978,7 +987,8 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   wave_reduce_records<kNodeRec>(P.npart + (size_t)C.blkN * kNodeRec, C.nbN, kNodeRec, on, rn);
   const int oc[kCRec] = {0, 0, 0, 0};
   double rc[kCRec];
-  wave_reduce_records<kCRec>(P.cpart + (size_t)C.tile0 * kCRec, C.contact ? C.ntile : 0, kCRec, oc, rc);
+  wave_reduce_records<kCRec>(P.cpart + (size_t)C.tile0 * P.csplit * kCRec, C.contact ? C.ntile * P.csplit : 0, kCRec,
+                             oc, rc);
   const double f2 = rn[0], fe2 = rn[1], err = rn[2], vmax = rn[3], du = rn[4], ek = rn[5];
   const int ov[kCvRec] = {0, 0, 0, 0};
   double rv[kCvRec];
@@ -1802,6 +1812,84 @@ __device__ __forceinline__ void pairs_body(const DPtrs& P, int posmode, int list
   }
 }
 
+/* exclusive prefix of the 256 counts np (one per thread) into s_pre[0..256] */
+__device__ void tile_scan(int np, int* s_pre)
+{
+  const int t = threadIdx.x;
+  int x = np;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if ((t & 63) >= off) x += y;
+  }
+  s_pre[t + 1] = x;
+  __syncthreads();
+  if (t == 0) {
+    s_pre[0] = 0;
+    for (int w = 1; w < 4; ++w) s_pre[64 * w + 64] += s_pre[64 * w];
+  }
+  __syncthreads();
+  if (t >= 64) s_pre[t + 1] += (t & 63) == 63 ? 0 : s_pre[64 * (t >> 6)];
+  __syncthreads();
+}
+
+/* the limit-surface searches of the candidates k_bp_pairs appended to the
+ * rows of the listed shells with smooth contact, from the element
+ * centroids (SmoothShellShellContact.C:12-18, then compute(), ShellSystem.C:
+ * 2094-2101): the pairs of a tile are shared by csplit virtual blocks in
+ * runs of equal length per thread; each result goes to its slot (d2, the
+ * parameters) marked pstick = 2 for k_bp_exact_smooth, which keeps it if
+ * d2 <= d_max^2 and forms the rows */
+__global__ void __launch_bounds__(256) k_bp_smooth_new(DPtrs P, int posmode, int list)
+{
+  if (P.wl[list][0] == 0) return;
+  __shared__ int s_sh[kMaxShells], s_pref[kMaxShells + 1], s_tmp[256];
+  __shared__ int s_pre[257];
+  const int nl = vmap_build(P, P.wl[list], V_TILE, s_sh, s_pref, s_tmp);
+  const int split = P.csplit;
+  for (int vb2 = blockIdx.x; vb2 < s_pref[nl] * split; vb2 += gridDim.x) {
+    const int vb = vb2 / split, part = vb2 % split;
+    const int li = vmap_find(vb, nl, s_pref);
+    const int sid = uni(s_sh[li]);
+    const SConst& C = P.cs[sid];
+    DShell* sh = P.sh + sid;
+    if (!bp_active(sh) || C.contact != 2) continue;     /* block-uniform */
+    const int tile = C.tile0 + (vb - s_pref[li]);
+    const int nb = sh->pair_buf;
+    const int e = tile * 256 + (int)threadIdx.x;
+    __syncthreads();                                     /* the previous tile is done with s_pre */
+    tile_scan(e - C.ebase < C.ne ? min(P.rcnt[nb][e], P.rowcap) : 0, s_pre);
+    const bool predict = (posmode == M_STEP);
+    const double dt = sh->dt, a8 = sh->nm[8];
+    const int par = sh->par;
+    const int all = s_pre[256];
+    const int share = (all + split - 1) / split;
+    const int first = min(all, part * share), total = min(all, first + share);
+    const int run = (total - first + 255) / 256;
+    const int k0 = first + (int)threadIdx.x * run, k1 = min(total, k0 + run);
+    int lo = 0;
+    if (k0 < k1) {
+      int hi = 255;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= k0) lo = mid; else hi = mid - 1;
+      }
+    }
+    for (int kp = k0; kp < k1; ++kp) {
+      while (kp >= s_pre[lo + 1]) ++lo;
+      const int e1 = tile * 256 + lo;
+      const size_t p = (size_t)e1 * P.rowcap + (kp - s_pre[lo]);
+      if (P.pd2[nb][p] != kTentative) continue;          /* a survivor of the old list */
+      double* xs = P.pvw[nb] + 4 * p;
+      for (int q = 0; q < 4; ++q) xs[q] = 1.0 / 3.0;
+      tri_hit hit;
+      smooth_pair_hit(P, e1, P.pe2[nb][p], P.U[par], P.V[par], P.A[par], predict, dt, a8, xs, true, hit);
+      P.pd2[nb][p] = hit.d2;
+      P.pstick[nb][p] = 2;
+    }
+  }
+}
+
 /* exact distance of the candidates k_bp_pairs appended to the listed
  * shells' new rows (ShellShellContact::compute, ShellShellContact.C:35-145,
  * kept if d2 <= d_max^2, ShellSystem.C:2096-2101; d2 <= 0 registers
@@ -1843,7 +1931,15 @@ __device__ __forceinline__ void exact_body(const DPtrs& P, int posmode, int list
     for (int r = 0; r < n; ++r) {
       const size_t p = row + r;
       double d2 = P.pd2[nb][p];
-      if (d2 == kTentative) {
+      if (SMOOTH && P.pstick[nb][p] == 2) {
+        /* a new smooth candidate, searched by k_bp_smooth_new */
+        P.pstick[nb][p] = 0;
+        if (d2 <= 0) {
+          atomicMax(&sh->problem, 6);
+          if (list == 1) stop_after_corrector(sh);
+        }
+        if (d2 > d2mx) continue;
+      } else if (d2 == kTentative) {
         const int e2 = P.pe2[nb][p];
         if (!have_x0) {
           for (int j = 0; j < 3; ++j) X0[j] = node_pos(P, P.tri[(size_t)j * nE + e1], U, V, A, predict, dt, a8);

@@ -79,6 +79,29 @@ SHX_HD void box_splines(const double* tab, double v, double w, int order, double
   }
 }
 
+/* the same for one box spline k, from the powers pv, pw of v and w: r[q] */
+SHX_HD void box_spline_one(const double* tab, int k, const double* pv, const double* pw, int order, double r[6])
+{
+  for (int q = 0; q < 6; ++q) r[q] = 0;
+  int m = 0;
+  for (int a = 0; a < 5; ++a)
+    for (int c = 0; a + c < 5; ++c, ++m) {
+      const double co = tab[k * kBoxMono + m];
+      if (co == 0) continue;
+      r[0] += co * pv[a] * pw[c];
+      if (order >= 1) {
+        if (a > 0) r[1] += co * a * pv[a - 1] * pw[c];
+        if (c > 0) r[2] += co * c * pv[a] * pw[c - 1];
+      }
+      if (order >= 2) {
+        if (a > 1) r[3] += co * (a * (a - 1)) * pv[a - 2] * pw[c];
+        if (c > 1) r[4] += co * (c * (c - 1)) * pv[a] * pw[c - 2];
+        if (a > 0 && c > 0) r[5] += co * (a * c) * pv[a - 1] * pw[c - 1];
+      }
+    }
+  for (int q = 0; q < 6; ++q) r[q] = r[q] / 12.0;
+}
+
 /* rescale_irregular_coords and the sub-patch choice of SubdivEvaluation.C:
  * 39-60, 110-135, 170-187: n is floor(1 - log2(xi + eta)), the count of
  * exact doublings that bring xi + eta into (1/2, 1]; (xi, eta) become the
@@ -116,27 +139,33 @@ struct Limit {
 
 SHX_HD void limit_eval(const double* tab, int N, const v3* X, double xi, double eta, int order, Limit& L)
 {
+  /* one box spline at a time (no 6 x 12 table in registers) */
   const int nq = order == 0 ? 1 : order == 1 ? 3 : 6;
-  double b[6][12];
   for (int q = 0; q < 6; ++q) L.d[q] = mk(0, 0, 0);
+  double pv[5], pw[5], r[6];
   if (N == 6) {
-    box_splines(tab, xi, eta, order, b);
+    pv[0] = 1;
+    pw[0] = 1;
+    for (int q = 1; q < 5; ++q) { pv[q] = pv[q - 1] * xi; pw[q] = pw[q - 1] * eta; }
     for (int j = 0; j < 12; ++j) {
-      const int c = limit_cvi(j);
-      for (int q = 0; q < nq; ++q) axpy(L.d[q], X[j], b[q][c]);
+      box_spline_one(tab, limit_cvi(j), pv, pw, order, r);
+      for (int q = 0; q < nq; ++q) axpy(L.d[q], X[j], r[q]);
     }
     return;
   }
   int k, n;
   double jfac;
   limit_locate(xi, eta, k, n, jfac);
-  box_splines(tab, xi, eta, order, b);
+  pv[0] = 1;
+  pw[0] = 1;
+  for (int q = 1; q < 5; ++q) { pv[q] = pv[q - 1] * xi; pw[q] = pw[q - 1] * eta; }
   const double* M = tab + limit_tab_off(N, k, n);
   const int P = N + 6;
   for (int i = 0; i < 12; ++i) {
     v3 Y = mk(0, 0, 0);
     for (int j = 0; j < P; ++j) axpy(Y, X[j], M[i * P + j]);
-    for (int q = 0; q < nq; ++q) axpy(L.d[q], Y, b[q][i]);
+    box_spline_one(tab, i, pv, pw, order, r);
+    for (int q = 0; q < nq; ++q) axpy(L.d[q], Y, r[q]);
   }
   const double j2 = jfac * jfac;
   for (int q = 1; q < nq; ++q) L.d[q] = scl(L.d[q], q < 3 ? jfac : j2);
