@@ -530,7 +530,7 @@ double smooth_closest(const SmoothProblem& S, double x[4], Pt& dc, int& iters)
       }
     }
     const int m = (int)Z.size();
-    double d[4] = {0, 0, 0, 0}, slope = 0;
+    double d[4] = {0, 0, 0, 0}, dn[4] = {0, 0, 0, 0}, slope = 0;   /* dn: d's part of small or negative curvature */
     bool pure = true;
     if (m > 0) {
       double rhs[4], Hz[16], dz[4];
@@ -556,13 +556,18 @@ double smooth_closest(const SmoothProblem& S, double x[4], Pt& dc, int& iters)
         double lmax = 0;
         for (int i = 0; i < m; ++i) lmax = std::max(lmax, std::fabs(lam[i]));
         const double lfloor = lmax > 0 ? 1e-3 * lmax : 1.0;
+        double dzn[4] = {0, 0, 0, 0};
         std::fill(dz, dz + 4, 0.0);
         for (int i = 0; i < m; ++i) {
           double c = 0;
           for (int j = 0; j < m; ++j) c += Vz[4 * j + i] * rhs[j];
           c /= std::max(std::fabs(lam[i]), lfloor);
           for (int j = 0; j < m; ++j) dz[j] += c * Vz[4 * j + i];
+          if (!(lam[i] > lfloor))
+            for (int j = 0; j < m; ++j) dzn[j] += c * Vz[4 * j + i];
         }
+        for (int i = 0; i < m; ++i)
+          for (int c = 0; c < 4; ++c) dn[c] += Z[i][c] * dzn[i];
       }
       for (int i = 0; i < m; ++i)
         for (int c = 0; c < 4; ++c) d[c] += Z[i][c] * dz[i];
@@ -571,53 +576,68 @@ double smooth_closest(const SmoothProblem& S, double x[4], Pt& dc, int& iters)
     double dinf = 0;
     for (int c = 0; c < 4; ++c) dinf = std::max(dinf, std::fabs(d[c]));
     /* Newton decrement below f's rounding: the face search is done */
-    const bool resolved = !(pure && -slope <= 1e-13 * f);
+    const bool resolved = !(-slope <= 1e-13 * f);
     if (m > 0 && dinf >= 1e-13 && slope < 0 && resolved) {
-      /* find_max_alpha (SmoothShellShellContact.C:93-133) over the inactive constraints */
-      double amax = std::numeric_limits<double>::max();
-      int be = -1, bbit = 0;
-      for (int e = 0; e < 2; ++e) {
-        const double v = x[2 * e], w = x[2 * e + 1], dv = d[2 * e], dw = d[2 * e + 1];
-        if (!(act[e] & 1) && dv < 0 && -v / dv < amax) { amax = -v / dv; be = e; bbit = 1; }
-        if (!(act[e] & 2) && dw < 0 && -w / dw < amax) { amax = -w / dw; be = e; bbit = 2; }
-        if (!(act[e] & 4) && dv + dw > 0 && (1 - v - w) / (dv + dw) < amax) {
-          amax = (1 - v - w) / (dv + dw);
-          be = e;
-          bbit = 4;
+      /* find_max_alpha (SmoothShellShellContact.C:93-133) over the inactive
+       * constraints, from a point along a direction */
+      auto max_step = [&](const double* x0, const double* dd, int& be, int& bbit) {
+        double amax = std::numeric_limits<double>::max();
+        be = -1;
+        bbit = 0;
+        for (int e = 0; e < 2; ++e) {
+          const double v = x0[2 * e], w = x0[2 * e + 1], dv = dd[2 * e], dw = dd[2 * e + 1];
+          if (!(act[e] & 1) && dv < 0 && -v / dv < amax) { amax = -v / dv; be = e; bbit = 1; }
+          if (!(act[e] & 2) && dw < 0 && -w / dw < amax) { amax = -w / dw; be = e; bbit = 2; }
+          if (!(act[e] & 4) && dv + dw > 0 && (1 - v - w) / (dv + dw) < amax) {
+            amax = (1 - v - w) / (dv + dw);
+            be = e;
+            bbit = 4;
+          }
         }
-      }
-      amax = std::max(amax, 0.0);
-      double alpha = std::min(amax, 1.0);
+        return std::max(amax, 0.0);
+      };
+      auto f_at = [&](const double* x0, const double* dd, double a) {
+        double xt[4];
+        for (int c = 0; c < 4; ++c) xt[c] = x0[c] + a * dd[c];
+        Pt q1, q2;
+        return S.eval(xt, 0, nullptr, nullptr, q1, q2);
+      };
+      int be, bbit, ge = -1, gbit = 0;
+      const double amax = max_step(x, d, be, bbit);
+      double alpha = std::min(amax, 1.0), gam = 0;
       if (!(pure && alpha * dinf <= 1e-3)) {
-        /* Armijo backtracking, then (modified Newton, full step taken) doubling
+        /* Armijo backtracking; then, the full modified step taken, its part
+         * of small or negative curvature continued from x + d, doubling
          * while f falls, up to the first constraint */
-        auto f_at = [&](double a) {
-          double xt[4];
-          for (int c = 0; c < 4; ++c) xt[c] = x[c] + a * d[c];
-          Pt q1, q2;
-          return S.eval(xt, 0, nullptr, nullptr, q1, q2);
-        };
         int bt = 0;
         double fa = f;
         for (; bt < 60; ++bt) {
-          fa = f_at(alpha);
+          fa = f_at(x, d, alpha);
           if (fa <= f + 1e-4 * alpha * slope) break;
           alpha *= 0.5;
         }
         if (bt == 60) break;
-        if (!pure && bt == 0)
-          while (alpha < amax) {
-            const double a2 = std::min(2 * alpha, amax);
-            const double f2 = f_at(a2);
+        if (!pure && bt == 0 && alpha == 1.0) {
+          double x1[4];
+          for (int c = 0; c < 4; ++c) x1[c] = x[c] + d[c];
+          const double gmax = max_step(x1, dn, ge, gbit);
+          double gt = 1.0;
+          while (gam < gmax) {
+            const double g2 = std::min(gt, gmax);
+            const double f2 = f_at(x1, dn, g2);
             if (!(f2 < fa)) break;
-            alpha = a2;
+            gam = g2;
             fa = f2;
+            gt = 2 * g2;
           }
+          if (!(gam == gmax)) ge = -1;
+        }
       }
-      for (int c = 0; c < 4; ++c) x[c] += alpha * d[c];
-      if (alpha == amax && be >= 0) {
-        act[be] |= bbit;
-        if (act[be] == 7) act[be] = bbit == 1 ? 5 : 6;
+      for (int c = 0; c < 4; ++c) x[c] += alpha * d[c] + gam * dn[c];
+      const int ae = gam > 0 ? ge : (alpha == amax ? be : -1), abit = gam > 0 ? gbit : bbit;
+      if (ae >= 0) {
+        act[ae] |= abit;
+        if (act[ae] == 7) act[ae] = abit == 1 ? 5 : 6;
       }
       for (int e = 0; e < 2; ++e) tri_clamp(x[2 * e], x[2 * e + 1], act[e]);
       continue;

@@ -19,7 +19,8 @@
  * this is an active-set Newton method of its own: Newton steps on the face
  * of the feasible set the active constraints leave free (exact Hessian from
  * the surfaces' second derivatives; where it is not positive definite, its
- * eigenvalues taken by magnitude and the step extended while f falls), the
+ * eigenvalues taken by magnitude and the step's part of small or negative
+ * curvature extended while f falls), the
  * step cut where it meets a constraint (which becomes active), an Armijo
  * backtracking line search outside the quadratic regime, a direct test of
  * an irregular vertex the iterate approaches, and, once
@@ -335,6 +336,26 @@ SHX_HD void smooth_eig(double A[16], int m, double lam[4], double V[16])
   for (int i = 0; i < m; ++i) lam[i] = A[5 * i];
 }
 
+/* the longest feasible step along d from x over the inactive constraints,
+ * and the constraint (element be, bit bbit) that stops it (be = -1: none) */
+SHX_HD double smooth_maxstep(const double x[4], const double d[4], const int act[2], int& be, int& bbit)
+{
+  double amax = 1.7976931348623157e308;
+  be = -1;
+  bbit = 0;
+  for (int e = 0; e < 2; ++e) {
+    const double v = x[2 * e], w = x[2 * e + 1], dv = d[2 * e], dw = d[2 * e + 1];
+    if (!(act[e] & 1) && dv < 0 && -v / dv < amax) { amax = -v / dv; be = e; bbit = 1; }
+    if (!(act[e] & 2) && dw < 0 && -w / dw < amax) { amax = -w / dw; be = e; bbit = 2; }
+    if (!(act[e] & 4) && dv + dw > 0 && (1 - v - w) / (dv + dw) < amax) {
+      amax = (1 - v - w) / (dv + dw);
+      be = e;
+      bbit = 4;
+    }
+  }
+  return amax > 0 ? amax : 0;
+}
+
 /* the closest points: x (feasible start) becomes the minimiser; returns
  * |p2 - p1|^2 with dc = p2 - p1 at it; *iters the Newton iterations used */
 SHX_HD double smooth_closest(const SmoothPair& S, double x[4], v3& dc, int* iters)
@@ -393,7 +414,7 @@ SHX_HD double smooth_closest(const SmoothPair& S, double x[4], v3& dc, int* iter
         ++m;
       }
     }
-    double d[4] = {0, 0, 0, 0};
+    double d[4] = {0, 0, 0, 0}, dn[4] = {0, 0, 0, 0};   /* step; its part of small or negative curvature */
     bool pure = true;
     double slope = 0;
     if (m > 0) {
@@ -422,13 +443,18 @@ SHX_HD double smooth_closest(const SmoothPair& S, double x[4], v3& dc, int* iter
         double lmax = 0;
         for (int i = 0; i < m; ++i) lmax = fmax(lmax, fabs(lam[i]));
         const double floor_l = lmax > 0 ? 1e-3 * lmax : 1.0;
+        double dzn[4] = {0, 0, 0, 0};
         for (int i = 0; i < m; ++i) dz[i] = 0;
         for (int i = 0; i < m; ++i) {
           double c = 0;
           for (int j = 0; j < m; ++j) c += Vz[4 * j + i] * gz[j];
           c /= fmax(fabs(lam[i]), floor_l);
           for (int j = 0; j < m; ++j) dz[j] += c * Vz[4 * j + i];
+          if (!(lam[i] > floor_l))
+            for (int j = 0; j < m; ++j) dzn[j] += c * Vz[4 * j + i];
         }
+        for (int i = 0; i < m; ++i)
+          for (int c = 0; c < 4; ++c) dn[c] += Z[i][c] * dzn[i];
       }
       for (int i = 0; i < m; ++i)
         for (int c = 0; c < 4; ++c) d[c] += Z[i][c] * dz[i];
@@ -438,23 +464,13 @@ SHX_HD double smooth_closest(const SmoothPair& S, double x[4], v3& dc, int* iter
     for (int c = 0; c < 4; ++c) dinf = fmax(dinf, fabs(d[c]));
     /* a step whose predicted decrease of f (the Newton decrement) is below
      * f's rounding cannot be resolved: the face search is done */
-    const bool resolved = !(pure && -slope <= 1e-13 * f);
+    const bool resolved = !(-slope <= 1e-13 * f);
     if (m > 0 && dinf >= kStep && slope < 0 && resolved) {
-      /* the longest feasible step along d and the constraint that stops it */
-      double amax = 1.7976931348623157e308;
-      int be = -1, bbit = 0;
-      for (int e = 0; e < 2; ++e) {
-        const double v = x[2 * e], w = x[2 * e + 1], dv = d[2 * e], dw = d[2 * e + 1];
-        if (!(act[e] & 1) && dv < 0 && -v / dv < amax) { amax = -v / dv; be = e; bbit = 1; }
-        if (!(act[e] & 2) && dw < 0 && -w / dw < amax) { amax = -w / dw; be = e; bbit = 2; }
-        if (!(act[e] & 4) && dv + dw > 0 && (1 - v - w) / (dv + dw) < amax) {
-          amax = (1 - v - w) / (dv + dw);
-          be = e;
-          bbit = 4;
-        }
-      }
-      amax = amax > 0 ? amax : 0;
+      int be, bbit;
+      const double amax = smooth_maxstep(x, d, act, be, bbit);
       double alpha = amax < 1 ? amax : 1.0;
+      double gam = 0;            /* extension along dn beyond the full step */
+      int ge = -1, gbit = 0;
       if (!(pure && alpha * dinf <= 1e-3)) {
         /* Armijo backtracking on f (outside the quadratic regime) */
         int bt = 0;
@@ -468,22 +484,36 @@ SHX_HD double smooth_closest(const SmoothPair& S, double x[4], v3& dc, int* iter
           alpha *= 0.5;
         }
         if (bt == 60) break;     /* no decrease left to find: the current point stands */
-        /* along negative curvature the full step was short: extend it,
-         * doubling while f keeps falling, up to the first constraint */
-        if (!pure && bt == 0)
-          while (alpha < amax) {
-            const double a2 = 2 * alpha < amax ? 2 * alpha : amax;
+        /* the full step taken, the part of small or negative curvature may
+         * have been short: continue along it from x + d, doubling while f
+         * keeps falling, up to the first constraint (the rest of the step
+         * is Newton's and stays as it is) */
+        if (!pure && bt == 0 && alpha == 1.0) {
+          double x1[4];
+          for (int c = 0; c < 4; ++c) x1[c] = x[c] + d[c];
+          const double gmax = smooth_maxstep(x1, dn, act, ge, gbit);
+          double gt = 1.0;
+          while (gam < gmax) {
+            const double g2 = gt < gmax ? gt : gmax;
             double xt[4];
-            for (int c = 0; c < 4; ++c) xt[c] = x[c] + a2 * d[c];
+            for (int c = 0; c < 4; ++c) xt[c] = x1[c] + g2 * dn[c];
             v3 q1, q2;
             const double f2 = smooth_eval(S, xt, 0, nullptr, nullptr, q1, q2);
             if (!(f2 < fa)) break;
-            alpha = a2;
+            gam = g2;
             fa = f2;
+            gt = 2 * g2;
           }
+          if (!(gam == gmax)) ge = -1;
+        }
       }
-      for (int c = 0; c < 4; ++c) x[c] += alpha * d[c];
-      if (alpha == amax && be >= 0) {
+      for (int c = 0; c < 4; ++c) x[c] += alpha * d[c] + gam * dn[c];
+      if (gam > 0) {
+        if (ge >= 0) {
+          act[ge] |= gbit;
+          if (act[ge] == 7) act[ge] = gbit == 1 ? 5 : 6;
+        }
+      } else if (alpha == amax && be >= 0) {
         act[be] |= bbit;
         if (act[be] == 7) act[be] = bbit == 1 ? 5 : 6;
       }

@@ -2,9 +2,11 @@
 for profilers (rocprofv3 --kernel-trace / --pmc): the level-4 single shell,
 the seeded batched sweep, or the 1024 level-4 shells of the warm-start test
 set (plain start, dt = 0.5 x their CFL bound), FP64 or FP32 split, attempts
-launched one by one (no graph).
+launched one by one (no graph); and `smooth`: a level-4 shell pancaked to
+3 % of its height whose sheets close at 0.04 per unit time, with smooth
+(limit-surface) contact and friction, adaptive dt.
 
-usage: python -m shell_amd.profile_steps single|batch|surrogate STEPS [fp32]
+usage: python -m shell_amd.profile_steps single|batch|surrogate|smooth STEPS [fp32]
 """
 import sys
 
@@ -33,6 +35,18 @@ def main():
         from shell_amd import warmstart as W
         b, meshes = W.build(W.draw_params(1024, 12))
         params = None
+    elif what == "smooth":
+        X, T = icosphere(4)
+        X = radial_imperfection(X, 1e-4, seed=7)
+        Y = X.copy()
+        Y[:, 2] *= 0.03
+        u0 = (Y - X).reshape(-1)
+        v0 = np.zeros_like(X)
+        v0[:, 2] = -0.02 * np.sign(X[:, 2])
+        v0 = v0.reshape(-1)
+        meshes = [(X, T)]
+        params = [dict(contact=2, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0, dt=0.01, dt_max=0.01, damping_alpha=0.05,
+                       adapt_dt_every=1)]
     else:
         shells = batch_sweep(512, seed=7)
         meshes = [(s["X"], s["T"]) for s in shells]
@@ -41,7 +55,10 @@ def main():
         b = S.ShellBatch(meshes, params)
     b.set_graphs(False)
     z = np.zeros(3 * int(b.nv.sum()))
-    b.set_state(S.ALL_SHELLS, z, v=z, a=z, t=0.0)
+    if what == "smooth":
+        b.set_state(0, u0, v=v0, a=z, t=0.0)
+    else:
+        b.set_state(S.ALL_SHELLS, z, v=z, a=z, t=0.0)
     b.apply_initial_conditions(precision=prec)
     reps = b.advance(steps, precision=prec)
     print(f"{what}: {steps} steps, {sum(r['n_rebuilds'] for r in reps)} rebuilds, "
