@@ -59,7 +59,9 @@ class Params(ctypes.Structure):
         "cavity_rate", "mu_s_cavity", "mu_d_cavity", "cavity_damping", "cavity_friction_damping", "gravity",
         "damping_rotation")] + [(n, ctypes.c_int32) for n in ("growth_base", "thickness_dir", "grow_mass", "pad1")] + [
         (n, ctypes.c_double * 2) for n in ("growth_g0", "growth_gt", "growth_gx", "growth_gy", "growth_gz")] + [
-        ("metric_growth_rate", ctypes.c_double), ("curvature_growth_rate", ctypes.c_double)]
+        ("metric_growth_rate", ctypes.c_double), ("curvature_growth_rate", ctypes.c_double)] + [
+        (n, ctypes.c_int32) for n in ("ghost_elems", "shell_boundary", "bc_coords", "bc_every")] + [
+        ("bc_stiff_mult", ctypes.c_double)]
 
 
 class Report(ctypes.Structure):
@@ -83,6 +85,8 @@ def _declare(L):
     L.orc_create.restype = _V
     L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _D, _I, ctypes.POINTER(Params)]
     L.orc_destroy.argtypes = [_V]
+    L.orc_boundary_edges.argtypes = [_V, _I]
+    L.orc_bc_keep.argtypes = [_V, ctypes.POINTER(ctypes.c_int8)]
     L.orc_last_error.restype = ctypes.c_char_p
     for name in ("orc_patch_size",):
         getattr(L, name).argtypes = [_V, _I]
@@ -178,7 +182,8 @@ def default_params():
                 cavity_damping=0.0, cavity_friction_damping=0.0, gravity=0.0, damping_rotation=0.0,
                 growth_base=0, thickness_dir=2, grow_mass=1, pad1=0, growth_g0=(1.0, 1.0), growth_gt=(0.0, 0.0),
                 growth_gx=(0.0, 0.0), growth_gy=(0.0, 0.0), growth_gz=(0.0, 0.0), metric_growth_rate=0.0,
-                curvature_growth_rate=0.0)
+                curvature_growth_rate=0.0, ghost_elems=0, shell_boundary=4, bc_coords=7, bc_every=0,
+                bc_stiff_mult=1.0)
 
 
 class Oracle:
@@ -188,9 +193,9 @@ class Oracle:
         self.L = lib_handle if lib_handle is not None else lib()
         self.X = np.ascontiguousarray(X, dtype=np.float64)
         self.T = np.ascontiguousarray(T, dtype=np.int32)
-        self.nv, self.ne = len(self.X), len(self.T)
         self.params = make_params(**params)
-        self.h = self.L.orc_create(self.nv, self.ne, dp(self.X), ip(self.T), ctypes.byref(self.params))
+        self.nv, self.ne = len(self.X), len(self.T) - self.params.ghost_elems    # ne: the real triangles
+        self.h = self.L.orc_create(self.nv, len(self.T), dp(self.X), ip(self.T), ctypes.byref(self.params))
         if not self.h:
             raise RuntimeError("oracle: " + self.L.orc_last_error().decode())
 
@@ -204,6 +209,18 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def boundary_edges(self):
+        """(m, 4) BoundaryCache nodes n1 n2 n3 n4 of the boundary edges, in order"""
+        m = self.L.orc_boundary_edges(self.h, None)
+        out = np.zeros((m, 4), dtype=np.int32)
+        self.L.orc_boundary_edges(self.h, ip(out))
+        return out
+
+    def bc_keep(self, keep):
+        """keep the boundary edges whose flag is set (the bc_criterion filter)"""
+        k = np.ascontiguousarray(keep, dtype=np.int8)
+        self.L.orc_bc_keep(self.h, k.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)))
 
     def patches(self):
         idx = np.zeros((self.ne, 12), dtype=np.int32)

@@ -704,7 +704,11 @@ struct Contact {
 
 struct System {
   orc_params prm;
-  int nv, ne;
+  int nv, ne;                 /* ne: the real triangles (the ghost layer only completes their patches) */
+  int nvr = 0;                /* real nodes: the ghost layer's nodes follow them */
+  /* BoundaryCache n1 n2 n3 n4 per boundary edge (ShellSystem.h:103-106), bc_stiff (ShellSystem.C:480) */
+  std::vector<std::array<int, 4>> bcache;
+  double bc_stiff = 0;
   std::vector<Pt> X;          /* mesh (control point) positions */
   std::vector<int> tri;       /* node-0-irregular ordering */
   std::vector<ElemCache> ec;
@@ -793,6 +797,7 @@ void update_derived(System& s)
   s.D = P.bend_stiff_mult * std::pow(h, 3) / (12 * one_nu2);
   const double Eavg = (P.E1 + P.E2) * 0.5;
   s.stiff_density = 0.5 * Eavg / (one_nu2 * h);
+  s.bc_stiff = P.bc_stiff_mult * Eavg * h / one_nu2;
   s.stiff_static_friction = P.fric_stiff_mult * Eavg * h / one_nu2;
 }
 
@@ -852,19 +857,51 @@ void matmul3(const double* A, const double* B, double* O)
  * one-ring patch (find_one_ring order, see subdiv_rows) */
 bool build_topology(System& s, const int32_t* tri_in)
 {
-  const int nv = s.nv, ne = s.ne;
+  /* the triangles after the first ne - ghost_elems are the ghost layer:
+   * they count for the valences and rings, and get no patch */
+  const int nv = s.nv, nall = s.ne, ne = s.ne - s.prm.ghost_elems;
+  if (s.prm.ghost_elems < 0 || ne <= 0) { g_err = "ghost_elems out of range"; return false; }
+  s.ne = ne;
   std::vector<int> val(nv, 0);
-  for (int e = 0; e < ne; ++e)
+  for (int e = 0; e < nall; ++e)
     for (int k = 0; k < 3; ++k) val[tri_in[3 * e + k]]++;
   /* next_ccw[v][x] = y for each triangle (v, x, y) in cyclic order */
   std::vector<std::map<int, int>> nxt(nv), prv(nv);
-  for (int e = 0; e < ne; ++e) {
+  for (int e = 0; e < nall; ++e) {
     const int a = tri_in[3 * e], b = tri_in[3 * e + 1], c = tri_in[3 * e + 2];
     nxt[a][b] = c; nxt[b][c] = a; nxt[c][a] = b;
     prv[a][c] = b; prv[b][a] = c; prv[c][b] = a;
   }
-  for (int n = 0; n < nv; ++n)
+  /* the real nodes (node_cache[].is_ghost, ShellSystem.C:576-600) have
+   * closed rings */
+  std::vector<char> real(nv, 0);
+  for (int q = 0; q < 3 * ne; ++q) real[tri_in[q]] = 1;
+  s.nvr = 0;
+  while (s.nvr < nv && real[s.nvr]) ++s.nvr;
+  for (int n = s.nvr; n < nv; ++n)
+    if (real[n]) { g_err = "the ghost layer's nodes must follow the real nodes"; return false; }
+  for (int n = 0; n < s.nvr; ++n)
     if ((int)nxt[n].size() != val[n]) { g_err = "mesh is not a closed 2-manifold"; return false; }
+  /* the boundary edges (ShellSystem.C:603-651): ghost triangles in order,
+   * their sides s whose neighbour is real: n2 = t[s], n3 = t[s+1], n4 =
+   * t[s+2], n1 the neighbour's node off the edge */
+  s.bcache.clear();
+  if (s.nvr < nv && (s.prm.bc_coords & 7) && s.prm.bc_every > 0) {
+    std::map<std::pair<int, int>, int> owner;     /* directed edge -> triangle */
+    for (int e = 0; e < nall; ++e)
+      for (int k = 0; k < 3; ++k) owner[{tri_in[3 * e + k], tri_in[3 * e + (k + 1) % 3]}] = e;
+    for (int g = ne; g < nall; ++g)
+      for (int k = 0; k < 3; ++k) {
+        const int n2 = tri_in[3 * g + k], n3 = tri_in[3 * g + (k + 1) % 3], n4 = tri_in[3 * g + (k + 2) % 3];
+        const auto it = owner.find({n3, n2});
+        if (it == owner.end() || it->second >= ne) continue;
+        const int nb = it->second;
+        int n1 = -1;
+        for (int q = 0; q < 3; ++q)
+          if (tri_in[3 * nb + q] != n2 && tri_in[3 * nb + q] != n3) n1 = tri_in[3 * nb + q];
+        s.bcache.push_back({n1, n2, n3, n4});
+      }
+  }
   s.tri.resize(3 * ne);
   s.ec.resize(ne);
   s.node_elems.assign(nv, {});
@@ -1350,7 +1387,7 @@ void body_loads(System& s, double t)
   if (P.cavity_contact) {
     update_cavity(s, t);
     const bool cavity_friction = P.mu_s_cavity != 0 || P.mu_d_cavity != 0;
-    for (int n = 0; n < s.nv; ++n) {
+    for (int n = 0; n < s.nvr; ++n) {            /* not the ghost nodes (ShellSystem.C:1613) */
       const Pt& p = s.p[n];
       const double one_plus_delta = cavity_isoparameter(s, p);
       if (one_plus_delta > s.one_plus_delta_friction) {
@@ -1727,7 +1764,7 @@ void contact_candidates(System& s, bool force, const std::vector<double>& ucur)
 void check_increment(System& s, bool require_change)
 {
   double dmax = 0;
-  for (int n = 0; n < s.nv; ++n) {
+  for (int n = 0; n < s.nvr; ++n) {              /* not the ghost nodes (ShellSystem.C:2609) */
     const double dx = s.u_pred[3 * n] - s.u[3 * n];
     const double dy = s.u_pred[3 * n + 1] - s.u[3 * n + 1];
     const double dz = s.u_pred[3 * n + 2] - s.u[3 * n + 2];
@@ -1898,7 +1935,7 @@ void measure_curvature(const System& s, double& kG, double& kM, double& area)
 double kinetic_energy(const System& s)
 {
   double ekin = 0;
-  for (int n = 0; n < s.nv; ++n) {
+  for (int n = 0; n < s.nvr; ++n) {              /* ShellSystem.C:2798 */
     double en = 0;
     for (int q = 0; q < 3; ++q) en += s.v[3 * n + q] * s.v[3 * n + q] / s.Minv[3 * n + q];
     ekin += 0.5 * en;
@@ -1926,6 +1963,50 @@ void add_sample(System& s, double dt, double volume)
 /* ShellSystem::advance, ShellSystem.C:952-1122: step attempts repeat until
  * the local-error estimate accepts dt (adapt_dt_every > 0), as in the
  * reference; with adapt_dt_every = 0 every attempt is accepted. */
+/* apply_boundary_conditions, ShellSystem.C:2138-2292: over the boundary
+ * edges in order and the coordinates of bc_coords, the strong conditions
+ * set u_pred, v_pred and a_pred of the edge's nodes, the weak ones subtract
+ * spring forces from f */
+void apply_boundary_conditions(System& s)
+{
+  const orc_params& P = s.prm;
+  if (P.bc_every <= 0 || (s.time_step - 1) % P.bc_every > 0) return;
+  std::vector<double>&up = s.u_pred, &vp = s.v_pred, &ap = s.a_pred, &f = s.f;
+  const double k = s.bc_stiff;
+  for (const auto& c : s.bcache)
+    for (int q = 0; q < 3; ++q) {
+      if (!(P.bc_coords & (1 << q))) continue;
+      const int d0 = 3 * c[0] + q, d1 = 3 * c[1] + q, d2 = 3 * c[2] + q, d3 = 3 * c[3] + q;
+      const int d[4] = {d0, d1, d2, d3};
+      switch (P.shell_boundary) {
+        case 0:
+          for (int r = 0; r < 4; ++r) up[d[r]] = vp[d[r]] = ap[d[r]] = 0;
+          break;
+        case 1:
+          for (int r = 0; r < 4; ++r) f[d[r]] -= k * up[d[r]];
+          break;
+        case 2:
+          up[d1] = vp[d1] = ap[d1] = 0;
+          up[d2] = vp[d2] = ap[d2] = 0;
+          up[d3] = -up[d0]; vp[d3] = -vp[d0]; ap[d3] = -ap[d0];
+          break;
+        case 3:
+          f[d1] -= k * up[d1];
+          f[d2] -= k * up[d2];
+          f[d3] -= k * (up[d3] + up[d0]);
+          break;
+        case 4:
+          up[d3] = up[d1] + up[d2] - up[d0];
+          vp[d3] = vp[d1] + vp[d2] - vp[d0];
+          ap[d3] = ap[d1] + ap[d2] - ap[d0];
+          break;
+        default:
+          f[d3] -= k * (up[d3] + up[d0] - up[d1] - up[d2]);
+          break;
+      }
+    }
+}
+
 void advance(System& s, orc_report* rep)
 {
   const orc_params& P = s.prm;
@@ -1949,6 +2030,7 @@ void advance(System& s, orc_report* rep)
     for (int i = 0; i < n3; ++i) s.v_pred[i] = A[6] * s.a[i] + s.v[i];
     contact_candidates(s, false, s.u_pred);
     evaluate_forces(s, s.t);
+    apply_boundary_conditions(s);
     const double volume_used = s.volume;
     fnorm = l2(s.f);
     fext_norm = l2(s.fext);
@@ -1976,6 +2058,7 @@ void advance(System& s, orc_report* rep)
       }
     }
     if (new_dt >= s.dt) {
+      apply_boundary_conditions(s);
       for (int i = 0; i < n3; ++i) s.u_pred[i] = A[0] * s.a_pred[i] + s.u_pred[i];
       check_increment(s, true);
       compute_position(s, s.u_pred);
@@ -1997,7 +2080,7 @@ void advance(System& s, orc_report* rep)
   }
   if (rep) {
     double ekin = 0;
-    for (int n = 0; n < s.nv; ++n) {
+    for (int n = 0; n < s.nvr; ++n) {
       double en = 0;
       for (int q = 0; q < 3; ++q) en += s.v[3 * n + q] * s.v[3 * n + q] / s.Minv[3 * n + q];
       ekin += 0.5 * en;
@@ -2083,6 +2166,25 @@ void* orc_create(int nv, int ne, const double* X, const int32_t* tri, const orc_
 }
 
 void orc_destroy(void* h) { delete static_cast<System*>(h); }
+
+int orc_boundary_edges(void* h, int32_t* nodes)
+{
+  System& s = *static_cast<System*>(h);
+  if (nodes)
+    for (size_t i = 0; i < s.bcache.size(); ++i)
+      for (int r = 0; r < 4; ++r) nodes[4 * i + r] = s.bcache[i][r];
+  return (int)s.bcache.size();
+}
+
+int orc_bc_keep(void* h, const int8_t* keep)
+{
+  System& s = *static_cast<System*>(h);
+  std::vector<std::array<int, 4>> kept;
+  for (size_t i = 0; i < s.bcache.size(); ++i)
+    if (keep[i]) kept.push_back(s.bcache[i]);
+  s.bcache.swap(kept);
+  return 0;
+}
 
 int orc_patch_size(void* h, int32_t* N)
 {

@@ -66,6 +66,10 @@ typedef struct {
   int32_t growth_base, thickness_dir, grow_mass, pad1;
   double growth_g0[2], growth_gt[2], growth_gx[2], growth_gy[2], growth_gz[2];
   double metric_growth_rate, curvature_growth_rate;
+  /* open meshes (ShellSystem.C:576-651, 2138-2292): the last ghost_elems triangles are the ghost layer;
+     shell_boundary 0 clamped, 1 clamped weak, 2 pinned, 3 pinned weak, 4 free, 5 free weak */
+  int32_t ghost_elems, shell_boundary, bc_coords, bc_every;
+  double bc_stiff_mult;
 } orc_params;
 
 typedef struct {
@@ -95,6 +99,10 @@ int  orc_icosphere(int level, int round_g, double* X, int32_t* tri);
 void* orc_create(int nv, int ne, const double* X, const int32_t* tri, const orc_params* prm);
 void  orc_destroy(void* h);
 const char* orc_last_error(void);
+/* the BoundaryCache list (n1 n2 n3 n4 per edge, ShellSystem.C:603-651) before the bc_criterion filter
+   (nodes may be null: the count); orc_bc_keep keeps the edges whose flag is set, in order */
+int orc_boundary_edges(void* h, int32_t* nodes);
+int orc_bc_keep(void* h, const int8_t* keep);
 
 int  orc_patch_size(void* h, int32_t* N /* ne */);
 int  orc_patches(void* h, int32_t* idx /* ne*12, -1 padded */, int32_t* tri_reordered /* ne*3 */);

@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -48,7 +49,10 @@ struct shx_ctx {
   bool any_smooth = false;     /* some shell has smooth contact: k_contact_smooth, k_bp_exact_smooth */
   bool any_kd = false;
   bool any_rot = false;        /* some shell has rotation damping: k_rot precedes the element epilogue */
-  bool any_loads = false;      /* some shell has gravity, cavity contact or rotation damping: k_node_loads */
+  std::vector<int32_t> bc_nodes, bc_z, bc_shell;   /* boundary edges of the batch (global nodes n1..n4), pinned
+                                                     z codes, shells */
+  bool any_loads = false;      /* some shell has gravity, cavity contact, rotation damping or boundary
+                                 conditions: k_node_loads */
   bool any_grow = false;       /* some shell grows: k_grow, then k_elem_grow */
   bool any_grow_mass = false;  /* some growing shell recomputes its masses every attempt: k_mass_grow */
   std::vector<char> keep_cso;  /* per shell: cavity springs were set by the caller (a loaded state) */
@@ -140,11 +144,14 @@ int build_topology_dev(shx_ctx* ctx, const int* nv, const int* ne, const int32_t
   DPtrs& P = ctx->P;
   const int S = ctx->S, nE = ctx->nblkE * kElemPerBlock, nV = ctx->nblkN * kNodePerBlock;
   std::vector<int32_t> tin((size_t)3 * nE, -1), contact(S, 0);
+  std::vector<int8_t> ghost(nE, 0);
   std::vector<double> cutoff(S, 0.0);
   for (int s = 0; s < S; ++s) {
     const SConst& C = ctx->C[s];
-    for (int e = 0; e < ne[s]; ++e)
+    for (int e = 0; e < ne[s]; ++e) {
       for (int k = 0; k < 3; ++k) tin[(size_t)3 * (C.ebase + e) + k] = C.vbase + tri[toff[s] + (size_t)3 * e + k];
+      ghost[(size_t)C.ebase + e] = e >= C.ne ? 1 : 0;      /* the triangles after the real ones: ghost layer */
+    }
     contact[s] = C.contact;
     cutoff[s] = (1 + prm[s].neighbor_margin) * prm[s].thickness;
   }
@@ -152,18 +159,20 @@ int build_topology_dev(shx_ctx* ctx, const int* nv, const int* ne, const int32_t
   T.nE = nE; T.nV = nV;
   T.eshell = P.eshell; T.X = P.X;
   int32_t *dtin, *dcon, *derr;
+  int8_t* dghost;
   double* dcut;
   std::vector<int32_t> err0((size_t)2 * S);
   for (int s = 0; s < S; ++s) { err0[2 * s] = 0; err0[2 * s + 1] = INT32_MAX; }
   int r = 0;
-  if ((r = upload(ctx, dtin, tin)) || (r = upload(ctx, dcon, contact)) || (r = upload(ctx, dcut, cutoff)) ||
+  if ((r = upload(ctx, dtin, tin)) || (r = upload(ctx, dghost, ghost)) || (r = upload(ctx, dcon, contact)) ||
+      (r = upload(ctx, dcut, cutoff)) ||
       (r = upload(ctx, derr, err0)) || (r = dalloc(ctx, T.vcnt, nV)) || (r = dalloc(ctx, T.vcur, nV)) ||
       (r = dalloc(ctx, T.vstart, (size_t)nV + 1)) || (r = dalloc(ctx, T.tri, (size_t)3 * nE)) ||
       (r = dalloc(ctx, T.patch, (size_t)12 * nE)) || (r = dalloc(ctx, T.valence, nE)) ||
       (r = dalloc(ctx, T.icnt, nV)) || (r = dalloc(ctx, T.icur, nV)) || (r = dalloc(ctx, T.istart, (size_t)nV + 1)) ||
       (r = dalloc(ctx, T.ccnt, nE)) || (r = dalloc(ctx, T.cstart, (size_t)nE + 1)))
     return r;
-  T.tin = dtin; T.contact = dcon; T.cutoff = dcut; T.err = derr;
+  T.tin = dtin; T.ghost = dghost; T.contact = dcon; T.cutoff = dcut; T.err = derr;
   const int gE = nE / 256, gV = nV / 256;
   k_topo_valence<<<gE, 256, 0, ctx->st>>>(T);
   k_topo_scan<<<1, 1024, 0, ctx->st>>>(T.vcnt, T.vstart, nV);
@@ -223,10 +232,10 @@ int build_topology_dev(shx_ctx* ctx, const int* nv, const int* ne, const int32_t
   for (int s = 0; s < S; ++s) {
     Topology& Ts = ctx->T[s];
     const SConst& C = ctx->C[s];
-    Ts.tri.assign((size_t)3 * ne[s], 0);
-    Ts.patch.assign((size_t)12 * ne[s], -1);
-    Ts.valence.assign(ne[s], 6);
-    for (int e = 0; e < ne[s]; ++e) {
+    Ts.tri.assign((size_t)3 * C.ne, 0);
+    Ts.patch.assign((size_t)12 * C.ne, -1);
+    Ts.valence.assign(C.ne, 6);
+    for (int e = 0; e < C.ne; ++e) {
       const size_t g = (size_t)C.ebase + e;
       for (int k = 0; k < 3; ++k) Ts.tri[(size_t)3 * e + k] = htri[3 * g + k] - C.vbase;
       for (int i = 0; i < 12; ++i) {
@@ -423,6 +432,8 @@ void launch_node(shx_ctx* ctx, int mode)
   }
   prof_end(ctx);
   prof_begin(ctx, 3);
+  if (mode == M_STEP && ctx->P.n_bc_rel)
+    k_bc_strong<<<(ctx->P.n_bc_rel + 255) / 256, 256, 0, ctx->st>>>(ctx->P);
   k_fin_node<<<ctx->S, 64, 0, ctx->st>>>(ctx->P, mode);
   if (ctx->any_kd && mode == M_STEP && ctx->grid[3]) k_kd<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P);
   prof_end(ctx);
@@ -529,6 +540,8 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
 {
   SConst C{};
   C.nv = nv; C.ne = ne;
+  C.nvr = nv;
+  C.bc_type = prm->shell_boundary; C.bc_coords = prm->bc_coords;
   C.h = prm->thickness; C.rho = prm->rho;
   C.E1 = prm->E1; C.E2 = prm->E2; C.nu = prm->nu; C.G12 = prm->G12;
   C.nu2 = (prm->E1 * prm->nu) / prm->E2;
@@ -537,6 +550,7 @@ SConst shell_consts(const shx_params* prm, int nv, int ne)
   C.K = prm->memb_stiff_mult * prm->thickness / one_nu2;
   C.D = prm->bend_stiff_mult * std::pow(prm->thickness, 3) / (12 * one_nu2);
   const double Eavg = (prm->E1 + prm->E2) * 0.5;
+  C.bc_stiff = prm->bc_stiff_mult * Eavg * prm->thickness / one_nu2;   /* ShellSystem.C:480 */
   C.stiff_density = 0.5 * Eavg / (one_nu2 * prm->thickness);
   C.stiff_static_friction = prm->fric_stiff_mult * Eavg * prm->thickness / one_nu2;
   C.d2_friction = std::pow((1 + prm->static_friction_tol) * prm->thickness, 2);
@@ -613,6 +627,74 @@ int contact_fx_shift(const shx_params* prm, const SConst& C, int ne, const doubl
   return std::max(-900, std::min(900, 60 - std::ilogb(f)));
 }
 
+/* The real nodes of an open mesh and its BoundaryCache list
+ * (ShellSystem.C:576-651): a node is a ghost when only ghost triangles use
+ * it, and the ghosts must follow the real nodes (*nvr: the real ones).  For
+ * every ghost triangle (in order) and side s (0, 1, 2) whose neighbour is a
+ * real triangle, the edge n2 = t[s], n3 = t[s+1], the ghost node n4 =
+ * t[s+2] and n1 the real triangle's third node, kept where bc_criterion at
+ * the edge's midpoint (reference positions, t = 0) equals 1.  nodes gets 4
+ * local ids per edge, z one code per edge for pinned conditions (1: n1 is
+ * an edge node of an earlier edge, 2: of a later one only, 0: of none).
+ * Empty string, or why the mesh is refused. */
+std::string boundary_edges(const shx_params* prm, int nv, int ne, int ner, const double* X, const int32_t* tri,
+                           int* nvr, std::vector<int32_t>& nodes, std::vector<int32_t>& z)
+{
+  nodes.clear();
+  z.clear();
+  *nvr = nv;
+  if (ner == ne) return "";
+  std::vector<char> real(nv, 0);
+  for (int q = 0; q < 3 * ner; ++q) real[tri[q]] = 1;
+  int nr = 0;
+  while (nr < nv && real[nr]) ++nr;
+  for (int n = nr; n < nv; ++n)
+    if (real[n]) return "the ghost layer's nodes must follow the real nodes";
+  *nvr = nr;
+  if (nr == nv || !(prm->bc_coords & 7) || prm->bc_every <= 0) return "";
+  ExprProg crit{};
+  char err[128] = {0};
+  const bool use_crit = prm->bc_criterion_fn[0] != 0;
+  if (use_crit && !expr_compile(prm->bc_criterion_fn, crit, err, sizeof(err), true))
+    return std::string("bc_criterion: ") + err;
+  std::map<std::pair<int, int>, std::vector<int>> by_edge;      /* undirected edge -> triangles */
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) {
+      const int a = tri[3 * e + k], b = tri[3 * e + (k + 1) % 3];
+      by_edge[std::make_pair(std::min(a, b), std::max(a, b))].push_back(e);
+    }
+  for (int g = ner; g < ne; ++g)
+    for (int k = 0; k < 3; ++k) {
+      const int n2 = tri[3 * g + k], n3 = tri[3 * g + (k + 1) % 3], n4 = tri[3 * g + (k + 2) % 3];
+      int nb = -1;
+      for (int e : by_edge[std::make_pair(std::min(n2, n3), std::max(n2, n3))])
+        if (e != g) nb = e;
+      if (nb < 0 || nb >= ner) continue;
+      int n1 = tri[3 * nb];
+      for (int q = 1; n1 == n2 || n1 == n3; ++q) n1 = tri[3 * nb + q];
+      if (use_crit) {
+        const double* a = X + 3 * n2;
+        const double* b = X + 3 * n3;
+        if (expr_eval_at(crit, 0.0, 0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1]), 0.5 * (a[2] + b[2])) != 1) continue;
+      }
+      nodes.insert(nodes.end(), {n1, n2, n3, n4});
+    }
+  const int m = (int)nodes.size() / 4;
+  std::vector<int> first(nv, INT32_MAX), last(nv, -1);
+  for (int i = 0; i < m; ++i)
+    for (int r = 1; r <= 2; ++r) {
+      const int n = nodes[4 * i + r];
+      first[n] = std::min(first[n], i);
+      last[n] = std::max(last[n], i);
+    }
+  z.resize(m);
+  for (int i = 0; i < m; ++i) {
+    const int n1 = nodes[4 * i];
+    z[i] = first[n1] < i ? 1 : last[n1] > i ? 2 : 0;
+  }
+  return "";
+}
+
 bool params_ok(const shx_params* prm)
 {
   return !(prm->thickness <= 0 || prm->E1 <= 0 || prm->E2 <= 0 || prm->G12 <= 0 || prm->rho <= 0 ||
@@ -620,6 +702,8 @@ bool params_ok(const shx_params* prm)
            prm->contact < 0 || prm->contact > 2 || (prm->contact && prm->cell_size_mult <= 0) || prm->gravity_dir < 0 || prm->gravity_dir > 2 ||
            !(prm->damping_rotation >= 0) ||
            prm->growth_base < 0 || prm->growth_base > 2 || prm->thickness_dir < 0 || prm->thickness_dir > 2 ||
+           prm->shell_boundary < 0 || prm->shell_boundary > 5 || prm->bc_coords < 0 || prm->bc_coords > 7 ||
+           prm->bc_every < 0 || !(prm->bc_stiff_mult > 0) ||
            !(prm->metric_growth_rate >= 0) || !(prm->curvature_growth_rate >= 0) ||
            (prm->cavity_contact && (prm->cavity < 0 || prm->cavity > 2 || !(prm->mu_s_cavity >= 0) ||
                                     !(prm->mu_d_cavity >= 0) || !(prm->cavity_damping >= 0) ||
@@ -788,11 +872,18 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
         ctx->err = "shell " + std::to_string(s) + ": vertex index out of range";
         return fail(SHX_E_MESH);
       }
+    /* an open mesh's ghost triangles come after its real ones: they take part
+     * in the topology only (their range is inside the shell's padding) */
+    if (prm[s].ghost_elems < 0 || prm[s].ghost_elems >= ne[s]) {
+      ctx->err = "shell " + std::to_string(s) + ": ghost_elems out of range";
+      return fail(SHX_E_ARG);
+    }
+    const int ner = ne[s] - prm[s].ghost_elems;
     ctx->T[s].nv = nv[s];
-    ctx->T[s].ne = ne[s];
+    ctx->T[s].ne = ner;
     ctx->Xin[s].assign(X + xoff[s], X + xoff[s] + (size_t)3 * nv[s]);
-    SConst C = shell_consts(&prm[s], nv[s], ne[s]);
-    C.fx_shift = contact_fx_shift(&prm[s], C, ne[s], X + xoff[s], tri + toff[s]);
+    SConst C = shell_consts(&prm[s], nv[s], ner);
+    C.fx_shift = contact_fx_shift(&prm[s], C, ner, X + xoff[s], tri + toff[s]);
     C.pressure_prog = pprog[s];
     C.volume_prog = vprog[s];
     for (int k = 0; k < 3; ++k) C.cav_prog[k] = cprog[(size_t)3 * s + k];
@@ -816,16 +907,33 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
        * on the (cell, element) entries per element; a shell's surface
        * occupies far fewer cells than its grid box holds */
       const double kc = std::pow(1.0 / prm[s].cell_size_mult + 2.0, 3.0);
-      const double want = std::max(1024.0, (double)ne[s] * kc / 16.0);
+      const double want = std::max(1024.0, (double)ner * kc / 16.0);
       unsigned bits = 10;
       while ((double)(1u << bits) < want && bits < 24) ++bits;
       C.hb_bits = bits;
       C.bbase = (unsigned)bb;
       bb += 1ull << bits;
       C.entbase = (unsigned)entb;
-      entb += (unsigned long long)ne[s] * kEntPerElem;
+      entb += (unsigned long long)ner * kEntPerElem;
       ctx->any_contact = true;
       if (C.contact == 2) ctx->any_smooth = true;
+    }
+    {
+      std::vector<int32_t> bn, bz;
+      const std::string why = boundary_edges(&prm[s], nv[s], ne[s], ner, X + xoff[s], tri + toff[s], &C.nvr, bn, bz);
+      if (!why.empty()) {
+        ctx->err = "shell " + std::to_string(s) + ": " + why;
+        return fail(SHX_E_MESH);
+      }
+      C.bc = bn.empty() ? 0 : prm[s].bc_every;
+      if (C.bc) {
+        ctx->any_loads = true;
+        for (size_t q = 0; q < bn.size(); ++q) ctx->bc_nodes.push_back(C.vbase + bn[q]);
+        for (size_t i = 0; i < bz.size(); ++i) {
+          ctx->bc_z.push_back(C.bc_type == 2 ? bz[i] : 0);
+          ctx->bc_shell.push_back(s);
+        }
+      }
     }
     ctx->C[s] = C;
     vb += nvp;
@@ -872,14 +980,14 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       const int e = C.ebase + q;
       for (int i = 0; i < 12; ++i) {
         int n = C.vbase;
-        if (q < ne[s]) {
+        if (q < C.ne) {
           const int ln = T.patch[(size_t)q * 12 + i];
           n = C.vbase + (ln >= 0 ? ln : T.patch[(size_t)q * 12]); /* padded slot: basis is zero */
         }
         pidx[(size_t)i * nE + e] = n;
       }
-      for (int k = 0; k < 3; ++k) triS[(size_t)k * nE + e] = C.vbase + (q < ne[s] ? T.tri[(size_t)3 * q + k] : 0);
-      if (q < ne[s]) vcls[e] = (int8_t)T.valence[q];
+      for (int k = 0; k < 3; ++k) triS[(size_t)k * nE + e] = C.vbase + (q < C.ne ? T.tri[(size_t)3 * q + k] : 0);
+      if (q < C.ne) vcls[e] = (int8_t)T.valence[q];
     }
   }
   /* element blocks: their patch nodes (in descending entry count), each patch
@@ -896,7 +1004,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       std::vector<int32_t> ids;
       for (int le = 0; le < kElemPerBlock; ++le) {
         const int q = first + le - C.ebase;
-        if (q < ne[s])
+        if (q < C.ne)
           for (int i = 0; i < 12; ++i) ids.push_back(pidx[(size_t)i * nE + first + le]);
       }
       std::sort(ids.begin(), ids.end());
@@ -912,7 +1020,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       std::vector<std::vector<uint16_t>> ent(ids.size());
       for (int le = 0; le < kElemPerBlock; ++le) {
         const int e = first + le, q = e - C.ebase;
-        if (q >= ne[s]) continue;
+        if (q >= C.ne) continue;
         const int vc = vcls[(size_t)e] - 3;       /* 0 .. 3 */
         for (int i = 0; i < 12; ++i) {
           const int j = (int)(std::lower_bound(ids.begin(), ids.end(), pidx[(size_t)i * nE + e]) - ids.begin());
@@ -929,7 +1037,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       for (size_t j = 0; j < ids.size(); ++j) pos[ord[j]] = (int)j;
       for (int le = 0; le < kElemPerBlock; ++le) {
         const int e = first + le;
-        if (e - C.ebase >= ne[s]) continue;
+        if (e - C.ebase >= C.ne) continue;
         for (int i = 0; i < 12; ++i) lidx[(size_t)i * nE + e] = (uint16_t)pos[lidx[(size_t)i * nE + e]];
       }
       for (size_t jj = 0; jj < ids.size(); ++jj) {
@@ -978,6 +1086,35 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
   P.bn_start = dbs; P.bn_ids = dbi; P.lidx = dli; P.rent_start = drs; P.rent = dre; P.rent_gcnt = drg; P.nrec_start = dns; P.nrec = dnr;
   if ((r = upload(ctx, ctx->dC, ctx->C))) return fail(r);
   P.cs = ctx->dC;
+  if (!ctx->bc_nodes.empty()) {
+    /* boundary conditions: each node's (edge, role) entries in edge order,
+     * and per ghost node of a pinned or free condition its last edge */
+    const int m = (int)ctx->bc_nodes.size() / 4;
+    std::vector<int32_t> start((size_t)nV + 1, 0), ent(4 * (size_t)m), rel;
+    for (int q = 0; q < 4 * m; ++q) ++start[ctx->bc_nodes[q] + 1];
+    for (int n = 0; n < nV; ++n) start[n + 1] += start[n];
+    std::vector<int32_t> cur(start.begin(), start.end() - 1);
+    std::map<int, int> last;                       /* ghost node -> its last edge */
+    bool need_apre = false;
+    for (int i = 0; i < m; ++i) {
+      for (int role = 0; role < 4; ++role)
+        ent[cur[ctx->bc_nodes[4 * i + role]]++] = i << 4 | ctx->bc_z[i] << 2 | role;
+      const int t = ctx->C[ctx->bc_shell[i]].bc_type;
+      if (t == 2 || t == 4) last[ctx->bc_nodes[4 * i + 3]] = i;
+      if (ctx->bc_z[i] == 2) need_apre = true;
+    }
+    for (const auto& kv : last) {
+      const int i = kv.second;
+      rel.insert(rel.end(), {ctx->bc_shell[i], kv.first, ctx->bc_nodes[4 * i], ctx->bc_nodes[4 * i + 1],
+                             ctx->bc_nodes[4 * i + 2], ctx->bc_z[i]});
+    }
+    int32_t *dn, *ds, *de, *dr = nullptr;
+    if ((r = upload(ctx, dn, ctx->bc_nodes)) || (r = upload(ctx, ds, start)) || (r = upload(ctx, de, ent)) ||
+        (!rel.empty() && (r = upload(ctx, dr, rel))) || (need_apre && (r = dalloc(ctx, P.bc_apre, (size_t)3 * nV))))
+      return fail(r);
+    P.bc_nodes = dn; P.bc_start = ds; P.bc_ent = de; P.bc_rel = dr;
+    P.n_bc_rel = (int)rel.size() / 6;
+  }
   if ((r = dalloc(ctx, P.tab, (size_t)T_ROWS * nE)) || (r = dalloc(ctx, P.minv, nV)) ||
       (r = dalloc(ctx, P.cdamp, nV)) || (r = dalloc(ctx, P.brec, (size_t)kBRec * std::max(nrecs, 1))) ||
       (r = dalloc(ctx, P.area, nE)) || (r = dalloc(ctx, P.tab32, (size_t)F_ROWS * nE)) ||

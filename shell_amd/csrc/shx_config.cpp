@@ -443,6 +443,7 @@ void shx_params_default(shx_params* p)
   p->cell_size_mult = 4; p->cell_margin = 0.02; p->neighbor_margin = 0.4;
   p->dt = 1e-6; p->newmark_beta = 0.25; p->newmark_gamma = 0.5;
   p->adapt_dt_every = 1; p->rel_loc_err_max = 3e-4; p->rel_loc_err_min = 3e-5;
+  p->shell_boundary = 4; p->bc_coords = 7; p->bc_stiff_mult = 1;   /* FREE, x y z (ShellSystem.C:49-51, 199-212) */
 }
 
 int shx_config_parse(const char* text, int argc, const char* const* argv, shx_config* out)
@@ -508,15 +509,27 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
   }
   if (ok) {
     const std::string bc = R.upper("shell_boundary", "FREE");
-    static const std::set<std::string> bcs = {"CLAMPED", "CLAMPED_WEAK", "PINNED", "PINNED_WEAK", "FREE", "FREE_WEAK"};
-    if (!bcs.count(bc)) ok = R.fail("shell_boundary = " + bc + " (ASSERT(shell_boundary != INVALID_BC))");
+    static const char* bcs[6] = {"CLAMPED", "CLAMPED_WEAK", "PINNED", "PINNED_WEAK", "FREE", "FREE_WEAK"};
+    int which = -1;
+    for (int k = 0; k < 6; ++k)
+      if (bc == bcs[k]) which = k;
+    if (which < 0) ok = R.fail("shell_boundary = " + bc + " (ASSERT(shell_boundary != INVALID_BC))");
+    else p.shell_boundary = which;
   }
   ok = ok && R.integer("bc_coords", 7, iv);
   if (ok && (iv < 0 || iv > 7)) ok = R.fail("ASSERT(bc_coords <= 7)");
+  if (ok) p.bc_coords = (int32_t)iv;
   ok = ok && R.integer("bc_every", 0, iv);      /* a closed mesh has no boundary nodes: no effect */
-  ok = ok && R.function("bc_stiff_mult", "1", expr, v, c);
-  if (ok && !(v > 0)) ok = R.fail("ASSERT(bc_stiff_mult > 0)");
-  ok = ok && R.function("bc_criterion", "1", expr, v, c);
+  if (ok) p.bc_every = (int32_t)iv;
+  ok = ok && R.constant_function("bc_stiff_mult", "1", p.bc_stiff_mult);
+  if (ok && !(p.bc_stiff_mult > 0)) ok = R.fail("ASSERT(bc_stiff_mult > 0)");
+  /* bc_criterion: evaluated at each boundary edge's midpoint when the
+   * boundary conditions are set up (ShellSystem.C:645-648) */
+  ok = ok && R.position_function("bc_criterion", "1", expr, v, c);
+  if (ok && expr != "1") {
+    if (expr.size() >= sizeof(p.bc_criterion_fn)) ok = R.fail("bc_criterion expression too long");
+    else put(p.bc_criterion_fn, sizeof(p.bc_criterion_fn), expr);
+  }
   ok = ok && R.constant_function("damping_alpha", "0", p.damping_alpha);
   ok = ok && R.constant_function("damping_beta", "0", p.damping_beta);
   ok = ok && R.constant_function("damping_gamma", "0", p.damping_gamma);
@@ -684,6 +697,7 @@ int shx_config_parse(const char* text, int argc, const char* const* argv, shx_co
   ok = ok && R.vector3("rotate", "0", out->rotate);
   ok = ok && R.vector3("translate", "0", out->translate);
   ok = ok && R.boolean("add_ghosts", true, bv);
+  if (ok) out->add_ghosts = bv ? 1 : 0;
   /* build extensions of shx_params, accepted as keys of the same file */
   R.quiet = true;
   ok = ok && R.number("pressure_ramp_time", "0", v);

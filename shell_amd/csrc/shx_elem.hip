@@ -98,8 +98,10 @@ __device__ __forceinline__ void elem_body(const DPtrs& P, int mode)
     const v3 u = ld3(U, n);
     const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
     const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
-    if (predict) dumax = fmax(dumax, nrm2_rn(sub(up, u)));
-    if (track) dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
+    if (n - C.vbase < C.nvr) {           /* a ghost node's increment is not checked (ShellSystem.C:2609) */
+      if (predict) dumax = fmax(dumax, nrm2_rn(sub(up, u)));
+      if (track) dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
+    }
     double* sn = s_buf + kNodeStage * j;
     sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
     sn[3] = x.x; sn[4] = x.y; sn[5] = x.z;
@@ -383,8 +385,10 @@ __global__ void __launch_bounds__(kElemBlock) k_elem32(DPtrs P, int mode)
     const v3 u = ld3(U, n);
     const v3 up = predict ? pred3(u, ld3(V, n), ld3(Aa, n), dt, a8) : u;
     const v3 x = add(ld3(P.X, n), C.volume_at_prediction ? up : u);
-    if (predict) dumax = fmax(dumax, nrm2_rn(sub(up, u)));
-    if (track) dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
+    if (n - C.vbase < C.nvr) {           /* a ghost node's increment is not checked (ShellSystem.C:2609) */
+      if (predict) dumax = fmax(dumax, nrm2_rn(sub(up, u)));
+      if (track) dev = fmax(dev, nrm2_rn(sub(up, ld3(P.urb, n))));
+    }
     double* sn = s_buf + kNodeStage * j;
     sn[0] = up.x; sn[1] = up.y; sn[2] = up.z;
     sn[3] = x.x; sn[4] = x.y; sn[5] = x.z;

@@ -824,12 +824,18 @@ __device__ __forceinline__ void contact_body(const DPtrs& P, int mode)
 
 /* avg_area_node (ShellSystem.C:2462-2481): mean deformed area of the
  * elements that have node n as a vertex, in element order */
-__device__ __forceinline__ double mean_vertex_area(const DPtrs& P, int n)
+__device__ __forceinline__ double mean_vertex_area(const DPtrs& P, const SConst& C, int n)
 {
   const int k0 = P.vinc_start[n], k1 = P.vinc_start[n + 1];
   double a = 0;
-  for (int k = k0; k < k1; ++k) a += P.area[P.vinc[k]];
-  return a / (k1 - k0);
+  int nr = 0;
+  for (int k = k0; k < k1; ++k) {
+    const int e = P.vinc[k];
+    if (e - C.ebase >= C.ne) continue;     /* ghost elements (avg_area_node, ShellSystem.C:2471) */
+    a += P.area[e];
+    ++nr;
+  }
+  return a / nr;
 }
 
 /* cavity contact of one node (ShellSystem.C:1601-1723): the isoparameter
@@ -853,7 +859,7 @@ __device__ v3 cavity_force(const DPtrs& P, const SConst& C, const DShell& sh, in
     return mk(0, 0, 0);
   }
   const double depth = sh.cav_avgR * (sqrt(opd) - 1);
-  const double area_node = mean_vertex_area(P, n);
+  const double area_node = mean_vertex_area(P, C, n);
   const double k = C.stiff_density * area_node;
   double fn = 0;
   if (depth > 0) {
@@ -1106,6 +1112,81 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
  * the rigid-body sums, for k_fin_node.  Launch block b computes node block
  * bmap[V_NODE][b].  Force time: t + dt in a step, t + dt in M_INIT
  * (apply_initial_conditions), DShell::t_eval in M_FORCES. */
+/* boundary conditions at node n (apply_boundary_conditions, ShellSystem.C:
+ * 2138-2292), over its entries (edge << 4 | z << 2 | role) in edge order,
+ * z of a pinned edge: 1 when its n1 is an edge node (held at zero) of an
+ * earlier edge, 2 of a later one, 0 of none; roles 0..3 are
+ * the BoundaryCache nodes n1 (interior node of the real element), n2, n3
+ * (the boundary edge), n4 (the ghost node).  up(m), vp(m) are node m's
+ * predicted displacement and velocity.  Weak conditions: the spring force
+ * sum fb (f -= fb).  Strong ones: zmask, the coordinates the condition holds
+ * at zero (clamped: every role; pinned: n2, n3), and for the ghost node's
+ * relations (pinned n4 = -n1, free n4 = n2 + n3 - n1) the related predicted
+ * values (rel set, coordinates in rmask). */
+struct BcAt {
+  v3 fb, up, vp;
+  int zmask, rmask;
+};
+
+__device__ void bc_at(const DPtrs& P, const SConst& C, int n, const double* U, const double* V, const double* A,
+                      bool predict, double dt, double a8, double a6, BcAt& o)
+{
+  o.fb = mk(0, 0, 0);
+  o.zmask = 0;
+  o.rmask = 0;
+  auto upred = [&](int m) {
+    const v3 u = ld3(U, m);
+    return predict ? pred3(u, ld3(V, m), ld3(A, m), dt, a8) : u;
+  };
+  auto vpred = [&](int m) {
+    const v3 v = ld3(V, m), a = ld3(A, m);
+    return mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
+  };
+  const v3 un = upred(n);
+  for (int k = P.bc_start[n]; k < P.bc_start[n + 1]; ++k) {
+    const int ent = P.bc_ent[k], role = ent & 3;
+    const int* nd = P.bc_nodes + 4 * (ent >> 4);
+    switch (C.bc_type) {
+      case 0: o.zmask = C.bc_coords; break;
+      case 1: o.fb = add(o.fb, scl(un, C.bc_stiff)); break;
+      case 2:
+        if (role == 1 || role == 2) o.zmask = C.bc_coords;
+        else if (role == 3) {
+          /* n1 held at zero by an earlier edge (bits 2-3 of the entry: 1) */
+          const bool z = ((ent >> 2) & 3) == 1;
+          o.rmask = C.bc_coords;
+          o.up = neg(z ? mk(0, 0, 0) : upred(nd[0]));
+          o.vp = neg(z ? mk(0, 0, 0) : vpred(nd[0]));
+        }
+        break;
+      case 3:
+        if (role == 1 || role == 2) o.fb = add(o.fb, scl(un, C.bc_stiff));
+        else if (role == 3) o.fb = add(o.fb, scl(add(un, upred(nd[0])), C.bc_stiff));
+        break;
+      case 4:
+        if (role == 3) {
+          o.rmask = C.bc_coords;
+          o.up = sub(add(upred(nd[1]), upred(nd[2])), upred(nd[0]));
+          o.vp = sub(add(vpred(nd[1]), vpred(nd[2])), vpred(nd[0]));
+        }
+        break;
+      default:
+        if (role == 3) o.fb = add(o.fb, scl(sub(sub(add(un, upred(nd[0])), upred(nd[1])), upred(nd[2])), C.bc_stiff));
+        break;
+    }
+  }
+  /* weak conditions act on the coordinates of bc_coords only */
+  if (!(C.bc_coords & 1)) o.fb.x = 0;
+  if (!(C.bc_coords & 2)) o.fb.y = 0;
+  if (!(C.bc_coords & 4)) o.fb.z = 0;
+}
+
+/* coordinates of mask set to those of b */
+__device__ __forceinline__ v3 masked(v3 a, v3 b, int mask)
+{
+  return mk(mask & 1 ? b.x : a.x, mask & 2 ? b.y : a.y, mask & 4 ? b.z : a.z);
+}
+
 template <bool LOADS>
 __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
 {
@@ -1119,6 +1200,7 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
   const int t = threadIdx.x;
   const int n = blk * kNodePerBlock + t;
   const bool valid = n - C.vbase < C.nv;
+  const bool real = n - C.vbase < C.nvr;         /* not a node of the ghost layer */
   const int par = sh.par;
   const double dt = sh.dt;
   v3 fint = mk(0, 0, 0), fp = fint;
@@ -1178,7 +1260,7 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
       }
       if (C.cavity_contact) {
         if (mode == M_FORCES) { P.ncav[2 * (size_t)n] = 0; P.ncav[2 * (size_t)n + 1] = 0; }
-        fext = add(fext, cavity_force(P, C, sh, n, x, v, mode, cv));
+        if (real) fext = add(fext, cavity_force(P, C, sh, n, x, v, mode, cv));   /* ghosts: ShellSystem.C:1613 */
       }
       if (C.damping_rotation > 0) {
         /* -c W p, W the cross-product matrix of the shell's angular velocity (ShellSystem.C:1772-1782) */
@@ -1187,7 +1269,15 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
         fext = sub(fext, scl(wv, C.damping_rotation));
       }
     }
-    const v3 f = add(fint, fext);
+    v3 f = add(fint, fext);
+    BcAt bca;
+    bca.zmask = 0;
+    bca.rmask = 0;
+    if (LOADS && C.bc && mode == M_STEP && sh.n_steps % C.bc == 0) {
+      /* the weak springs after the other forces (ShellSystem.C:1000-1003) */
+      bc_at(P, C, n, P.U[par], P.V[par], P.A[par], mode == M_STEP, dt, sh.nm[8], sh.nm[6], bca);
+      f = sub(f, bca.fb);
+    }
     rec[0] = nrm2(f);
     rec[1] = nrm2(fext);
     {
@@ -1211,24 +1301,40 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
       }
     } else if (mode == M_INIT) {
       st3(P.A[par], n, scl(f, mi));
-      rec[5] = 0.5 * (v.x * v.x / mi + v.y * v.y / mi + v.z * v.z / mi);
+      if (real) rec[5] = 0.5 * (v.x * v.x / mi + v.y * v.y / mi + v.z * v.z / mi);
     } else {
       const double cd = P.cdamp[n] + sh.alpha_extra / mi;   /* + alpha_extra m */
       const double a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];
-      const v3 up = pred3(u, v, a, dt, a8);
-      const v3 vp = mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
-      const v3 an = mk(mi * (f.x - cd * vp.x), mi * (f.y - cd * vp.y), mi * (f.z - cd * vp.z));
+      v3 up = pred3(u, v, a, dt, a8);
+      v3 vp = mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
+      if (LOADS) {
+        /* strong conditions on the predicted values before the damping (the
+         * first apply_boundary_conditions of advance, ShellSystem.C:1003) */
+        const v3 zero = mk(0, 0, 0);
+        up = masked(masked(up, bca.up, bca.rmask), zero, bca.zmask);
+        vp = masked(masked(vp, bca.vp, bca.rmask), zero, bca.zmask);
+      }
+      v3 an = mk(mi * (f.x - cd * vp.x), mi * (f.y - cd * vp.y), mi * (f.z - cd * vp.z));
       rec[2] = fmax(fabs(an.x - a.x), fmax(fabs(an.y - a.y), fabs(an.z - a.z)));
       rec[3] = fmax(fabs(vp.x), fmax(fabs(vp.y), fabs(vp.z)));
+      if (LOADS && bca.zmask) {
+        /* the second one (:1051): a held at zero too; the acceleration before
+         * it is what a later pinned edge's ghost relation reads */
+        if (P.bc_apre) st3(P.bc_apre, n, an);
+        an = masked(an, mk(0, 0, 0), bca.zmask);
+      }
       const v3 un = mk(a0 * an.x + up.x, a0 * an.y + up.y, a0 * an.z + up.z);
       const v3 vn = mk(a7 * an.x + vp.x, a7 * an.y + vp.y, a7 * an.z + vp.z);
-      const v3 d = sub(un, u);
-      rec[4] = d.x * d.x + d.y * d.y + d.z * d.z;
-      rec[5] = 0.5 * (vn.x * vn.x / mi + vn.y * vn.y / mi + vn.z * vn.z / mi);
+      if (real) {
+        /* check_increment and the kinetic energy over the real nodes (ShellSystem.C:2609, 2798) */
+        const v3 d = sub(un, u);
+        rec[4] = d.x * d.x + d.y * d.y + d.z * d.z;
+        rec[5] = 0.5 * (vn.x * vn.x / mi + vn.y * vn.y / mi + vn.z * vn.z / mi);
+      }
       st3(P.U[1 - par], n, un);
       st3(P.V[1 - par], n, vn);
       st3(P.A[1 - par], n, an);
-      if (C.contact && C.rebuild_trigger) {
+      if (C.contact && C.rebuild_trigger && real) {
         const v3 g = sub(un, ld3(P.urb, n));
         rec[22] = g.x * g.x + g.y * g.y + g.z * g.z;
       }
@@ -2052,6 +2158,56 @@ __global__ void __launch_bounds__(256) k_bp_exact_smooth(DPtrs P, int posmode, i
 {
   exact_body<true>(P, posmode, list);
 }
+/* the ghost nodes' relations of the second apply_boundary_conditions
+ * (ShellSystem.C:1051, 2252-2277) after the corrector, one thread per
+ * ghost node: the reference sets the ghost's predicted u, v and its new a
+ * from n1, n2, n3 before u += a0 a, v += a7 a; the predictions of the
+ * relation are recomputed from the step's start, the accelerations read from
+ * the node kernel's output.  Pinned: u4 = -u1, or zero where an earlier
+ * edge holds n1 (z 1); where a later one does (z 2) the reference's u1, v1
+ * are already zero and a1 is n1's acceleration before it was held.  Free:
+ * u4 = u2 + u3 - u1.  A rejected attempt's output is discarded as a whole. */
+__global__ void __launch_bounds__(256) k_bc_strong(DPtrs P)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P.n_bc_rel) return;
+  const int32_t* q = P.bc_rel + 6 * (size_t)r;
+  const SConst& C = P.cs[q[0]];
+  const DShell& sh = P.sh[q[0]];
+  if (sh.done || sh.n_steps % C.bc != 0) return;
+  const int par = sh.par, g = q[1];
+  const double dt = sh.dt, a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];
+  const double *U = P.U[par], *V = P.V[par], *A = P.A[par], *An = P.A[1 - par];
+  auto up = [&](int m) { return pred3(ld3(U, m), ld3(V, m), ld3(A, m), dt, a8); };
+  auto vp = [&](int m) {
+    const v3 v = ld3(V, m), a = ld3(A, m);
+    return mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
+  };
+  v3 u4, v4, a4;
+  if (C.bc_type == 2) {
+    const int z = q[5];
+    if (z == 1) {
+      u4 = v4 = a4 = neg(mk(0, 0, 0));
+    } else {
+      a4 = neg(z == 2 ? ld3(P.bc_apre, q[2]) : ld3(An, q[2]));
+      const v3 pu = z == 2 ? neg(mk(0, 0, 0)) : neg(up(q[2]));
+      const v3 pv = z == 2 ? neg(mk(0, 0, 0)) : neg(vp(q[2]));
+      u4 = mk(a0 * a4.x + pu.x, a0 * a4.y + pu.y, a0 * a4.z + pu.z);
+      v4 = mk(a7 * a4.x + pv.x, a7 * a4.y + pv.y, a7 * a4.z + pv.z);
+    }
+  } else {
+    a4 = sub(add(ld3(An, q[3]), ld3(An, q[4])), ld3(An, q[2]));
+    const v3 pu = sub(add(up(q[3]), up(q[4])), up(q[2]));
+    const v3 pv = sub(add(vp(q[3]), vp(q[4])), vp(q[2]));
+    u4 = mk(a0 * a4.x + pu.x, a0 * a4.y + pu.y, a0 * a4.z + pu.z);
+    v4 = mk(a7 * a4.x + pv.x, a7 * a4.y + pv.y, a7 * a4.z + pv.z);
+  }
+  const int m = C.bc_coords;
+  st3(P.U[1 - par], g, masked(ld3(P.U[1 - par], g), u4, m));
+  st3(P.V[1 - par], g, masked(ld3(P.V[1 - par], g), v4, m));
+  st3(P.A[1 - par], g, masked(ld3(P.A[1 - par], g), a4, m));
+}
+
 __global__ void __launch_bounds__(kNodeBlock, 4) k_node_w4(DPtrs P, int mode) { node_body<false>(P, mode); }
 __global__ void __launch_bounds__(kNodeBlock, 6) k_node_w6(DPtrs P, int mode) { node_body<false>(P, mode); }
 __global__ void __launch_bounds__(kNodeBlock, 8) k_node_w8(DPtrs P, int mode) { node_body<false>(P, mode); }

@@ -63,6 +63,79 @@ double round_g(double x)
   return std::strtod(b, nullptr);
 }
 
+/* The ghost layer of an open mesh (the reference's prepare_subdiv_mesh with
+ * ghosts, shell.C:228-230: libMesh's MeshTools::Subdivision
+ * add_boundary_ghosts, not part of the reference's sources), so that every
+ * real element has a complete one-ring patch:
+ *  - every boundary edge a -> b of a real triangle (a, b, c) gets the ghost
+ *    triangle (b, a, m) with m = a + b - c, c mirrored through the edge's
+ *    midpoint;
+ *  - every boundary vertex V, with r real triangles, outgoing boundary edge
+ *    V -> A (ghost node mA) and incoming B -> V (ghost node mB), gets
+ *    k = 4 - r more ghost triangles closing its ring at valence 6: k - 1 new
+ *    nodes made by mirroring the last triangle around V (h = V + prev -
+ *    prevprev, starting from (A, V, mA)), then (V, h_j, h_j-1) and finally
+ *    (V, mB, h_k-1), or (V, mB, mA) when k = 1.
+ * Real nodes and triangles keep their numbers; ghost nodes and triangles
+ * follow, in the order made (boundary edges in element and side order, then
+ * boundary vertices in ascending order).  Returns an empty string or the
+ * reason the mesh cannot take a ghost layer. */
+std::string add_ghosts(int nv, int ne, const double* X, const int32_t* tri, std::vector<double>& Xo,
+                       std::vector<int32_t>& To)
+{
+  Xo.assign(X, X + 3 * (size_t)nv);
+  To.assign(tri, tri + 3 * (size_t)ne);
+  std::map<std::pair<int, int>, int> edge;       /* directed edge -> triangle */
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) {
+      const int a = tri[3 * e + k], b = tri[3 * e + (k + 1) % 3];
+      if (a < 0 || a >= nv || b < 0 || b >= nv) return "vertex index out of range";
+      if (!edge.emplace(std::make_pair(a, b), e).second) return "non-manifold or inconsistently oriented mesh";
+    }
+  auto add_node = [&](const double* p) {
+    for (int q = 0; q < 3; ++q) Xo.push_back(p[q]);
+    return (int)(Xo.size() / 3) - 1;
+  };
+  std::map<std::pair<int, int>, int> mirror;     /* boundary edge a -> b: its ghost node */
+  std::map<int, int> out_to, in_from;            /* boundary vertex: its boundary edges' other ends */
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) {
+      const int a = tri[3 * e + k], b = tri[3 * e + (k + 1) % 3], c = tri[3 * e + (k + 2) % 3];
+      if (edge.count(std::make_pair(b, a))) continue;
+      double m[3];
+      for (int q = 0; q < 3; ++q) m[q] = Xo[3 * a + q] + Xo[3 * b + q] - Xo[3 * c + q];
+      const int g = add_node(m);
+      To.push_back(b); To.push_back(a); To.push_back(g);
+      mirror[std::make_pair(a, b)] = g;
+      if (!out_to.emplace(a, b).second || !in_from.emplace(b, a).second)
+        return "the boundary touches itself at a vertex";
+    }
+  std::vector<int> nreal(nv, 0);
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) ++nreal[tri[3 * e + k]];
+  for (const auto& kv : out_to) {
+    const int V = kv.first, A = kv.second;
+    const auto ib = in_from.find(V);
+    if (ib == in_from.end()) return "the boundary touches itself at a vertex";
+    const int B = ib->second, mA = mirror[std::make_pair(V, A)], mB = mirror[std::make_pair(B, V)];
+    const int k = 4 - nreal[V];
+    if (k < 1)
+      return "a boundary vertex with " + std::to_string(nreal[V]) +
+             " elements (more than 3: a reflex corner) takes no regular ghost ring";
+    int prev = mA, prevprev = A;
+    for (int j = 1; j < k; ++j) {
+      double h[3];
+      for (int q = 0; q < 3; ++q) h[q] = Xo[3 * V + q] + Xo[3 * prev + q] - Xo[3 * prevprev + q];
+      const int hn = add_node(h);
+      To.push_back(V); To.push_back(hn); To.push_back(prev);
+      prevprev = prev;
+      prev = hn;
+    }
+    To.push_back(V); To.push_back(mB); To.push_back(prev);
+  }
+  return "";
+}
+
 } // namespace
 
 extern "C" {
@@ -203,6 +276,21 @@ int shx_mesh_transform(int nv, double* X, const double* project, const double* s
   if (translate && (translate[0] != 0 || translate[1] != 0 || translate[2] != 0))
     for (int i = 0; i < nv; ++i)
       for (int q = 0; q < 3; ++q) X[3 * i + q] += translate[q];
+  return SHX_OK;
+}
+
+int shx_mesh_add_ghosts(int nv, int ne, const double* X, const int32_t* tri, int* nv_out, int* ne_out,
+                        double* X_out, int32_t* tri_out)
+{
+  if (nv <= 0 || ne <= 0 || !X || !tri || !nv_out || !ne_out) return SHX_E_ARG;
+  std::vector<double> P;
+  std::vector<int32_t> T;
+  const std::string err = add_ghosts(nv, ne, X, tri, P, T);
+  if (!err.empty()) return SHX_E_MESH;
+  *nv_out = (int)(P.size() / 3);
+  *ne_out = (int)(T.size() / 3);
+  if (X_out) std::memcpy(X_out, P.data(), sizeof(double) * P.size());
+  if (tri_out) std::memcpy(tri_out, T.data(), sizeof(int32_t) * T.size());
   return SHX_OK;
 }
 

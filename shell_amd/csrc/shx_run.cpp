@@ -233,6 +233,26 @@ int main(int argc, char** argv)
     return ERROR;
   }
   shx_mesh_transform(nv, X.data(), cfg.project, cfg.scale, cfg.rotate, cfg.translate);
+  if (cfg.add_ghosts) {
+    /* prepare_subdiv_mesh with ghosts (shell.C:228-230): an open mesh gets
+     * its ghost layer, a closed one is unchanged */
+    int nv2 = 0, ne2 = 0;
+    if (shx_mesh_add_ghosts(nv, ne, X.data(), tri.data(), &nv2, &ne2, nullptr, nullptr) != SHX_OK) {
+      std::cout << "ERROR: the mesh cannot take a ghost layer (a boundary vertex with more than 3 elements, or a "
+                   "boundary touching itself)\n";
+      return ERROR;
+    }
+    if (ne2 > ne) {
+      std::vector<double> X2(3 * (size_t)nv2);
+      std::vector<int32_t> T2(3 * (size_t)ne2);
+      shx_mesh_add_ghosts(nv, ne, X.data(), tri.data(), &nv2, &ne2, X2.data(), T2.data());
+      cfg.prm.ghost_elems = ne2 - ne;
+      X.swap(X2);
+      tri.swap(T2);
+      nv = nv2;
+      ne = ne2;
+    }
+  }
 
   shx_ctx* ctx = nullptr;
   if (shx_create(&ctx, nv, ne, X.data(), tri.data(), &cfg.prm, 0) != SHX_OK) {

@@ -27,6 +27,9 @@ struct TopoPtrs {
   int nE, nV;                 /* padded global element / node counts */
   const int32_t* eshell;      /* nE */
   const int32_t* tin;         /* 3*nE: input triangles, global node ids; -1 for padding */
+  const int8_t* ghost;        /* nE: 1 for a ghost triangle of an open mesh's ghost layer: part of the
+                                 adjacency (valences, rings), without a patch or an exclusion set, and
+                                 never in another element's exclusion set */
   const double* X;            /* 3*nV: undeformed node positions */
   const int32_t* contact;     /* per shell: self-contact on (exclusion sets needed) */
   const double* cutoff;       /* per shell: (1 + neighbor_margin) h */

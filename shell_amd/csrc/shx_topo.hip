@@ -11,7 +11,9 @@
  * ShellSystem.C:684-741) on the undeformed mesh.
  *
  * Element and node ids are global over the batch (shell ranges padded to
- * kAlign); padded elements hold tin = -1 and padded nodes no incidence.
+ * kAlign); padded elements hold tin = -1 and padded nodes no incidence.  The
+ * ghost triangles of an open mesh (after its real ones) are adjacency only:
+ * they close the rings of the boundary vertices and carry no patch.
  * CSR lists are filled through atomic cursors and then sorted per row, so
  * their contents do not depend on the order of the atomics.  Compiled with
  * -ffp-contract=off: the close-set distances round like the host's and the
@@ -128,7 +130,7 @@ __global__ void __launch_bounds__(256) k_topo_vsort(TopoPtrs T)
 __global__ void __launch_bounds__(256) k_topo_patch(TopoPtrs T)
 {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= T.nE || T.tin[3 * (size_t)e] < 0) return;
+  if (e >= T.nE || T.tin[3 * (size_t)e] < 0 || T.ghost[e]) return;
   const int* t = T.tin + 3 * (size_t)e;
   int nirr = 0, at = 0;
   for (int k = 0; k < 3; ++k) {
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(256) k_topo_patch(TopoPtrs T)
 __global__ void __launch_bounds__(256) k_topo_ifill(TopoPtrs T)
 {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= T.nE || T.tin[3 * (size_t)e] < 0) return;
+  if (e >= T.nE || T.tin[3 * (size_t)e] < 0 || T.ghost[e]) return;
   for (int i = 0; i < 12; ++i) {
     const int n = T.patch[12 * (size_t)e + i];
     if (n >= 0) T.inc[T.istart[n] + atomicAdd(&T.icur[n], 1)] = (e << 4) | i;
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(256) k_topo_close(TopoPtrs T, int pass)
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= T.nE || T.tin[3 * (size_t)e] < 0) return;
   const int s = T.eshell[e];
-  if (!T.contact[s]) { if (pass == 0) T.ccnt[e] = 0; return; }
+  if (!T.contact[s] || T.ghost[e]) { if (pass == 0) T.ccnt[e] = 0; return; }
   const double cut2 = T.cutoff[s] * T.cutoff[s];
   int32_t set[kTopoCloseMax];
   int n = 0;
@@ -216,7 +218,7 @@ __global__ void __launch_bounds__(256) k_topo_close(TopoPtrs T, int pass)
     const int v = T.tri[3 * (size_t)e + k];
     for (int q = T.vstart[v]; q < T.vstart[v + 1]; ++q) {
       const int c = T.vinc[q];
-      if (c != e && !has(c)) {
+      if (c != e && !T.ghost[c] && !has(c)) {
         if (n == kTopoCloseMax) { fail(T, e, TOPO_CLOSE_CAP); return; }
         set[n++] = c;
       }
@@ -231,7 +233,7 @@ __global__ void __launch_bounds__(256) k_topo_close(TopoPtrs T, int pass)
       const int v = T.tri[3 * (size_t)a + k];
       for (int q = T.vstart[v]; q < T.vstart[v + 1]; ++q) {
         const int c = T.vinc[q];
-        if (c == e || has(c)) continue;
+        if (c == e || T.ghost[c] || has(c)) continue;
         v3 Pc[3];
         corners(T, c, Pc);
         tri_hit hit;

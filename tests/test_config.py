@@ -99,6 +99,24 @@ def test_contact_types():
         assert p["contact"] == code
 
 
+def test_boundary_condition_keys():
+    """shell_boundary (STR_TO_ENUM, ShellSystem.C:199-206), bc_coords,
+    bc_every, bc_stiff_mult and bc_criterion (:208-212), with the reference's
+    defaults FREE, 7, 0, 1 and 1"""
+    p, _, _ = S.read_config(BASE)
+    assert (p["shell_boundary"], p["bc_coords"], p["bc_every"], p["bc_stiff_mult"]) == (4, 7, 0, 1.0)
+    assert p["bc_criterion_fn"] in ("", "1")
+    for word, code in (("clamped", 0), ("CLAMPED_WEAK", 1), ("pinned", 2), ("pinned_weak", 3), ("free", 4),
+                       ("free_weak", 5)):
+        p, _, _ = S.read_config(BASE + "shell_boundary = " + word + "\n")
+        assert p["shell_boundary"] == code
+    p, _, _ = S.read_config(BASE + "bc_coords = 3\nbc_every = 2\nbc_stiff_mult = 10\nbc_criterion = x<0.5\n")
+    assert (p["bc_coords"], p["bc_every"], p["bc_stiff_mult"], p["bc_criterion_fn"]) == (3, 2, 10.0, "x<0.5")
+    for line in ("shell_boundary = hinged", "bc_coords = 8", "bc_stiff_mult = 0"):
+        with pytest.raises(S.ShellError):
+            S.read_config(BASE + line + "\n")
+
+
 def test_command_line_overrides_and_constant_functions():
     p, d, warn = S.read_config(BASE, ["-dt", "0.02", "-pressure", "-3e-4", "-time_steps", "7", "-bogus", "1"])
     assert p["dt"] == 0.02 and d["time_steps"] == 7
