@@ -19,6 +19,7 @@
  *    (the reference's initialisation does not evaluate forces on a loaded
  *    state).
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -176,6 +177,10 @@ int shx_write_vtu(shx_ctx* ctx, int shell, const char* path)
   shx_batch_mesh(ctx, shell, nullptr, nullptr, X.data(), tri.data());
   if (int r = shx_batch_get_state(ctx, shell, u.data(), nullptr, nullptr, nullptr)) return io_fail(shx_last_error(ctx), r);
   if (int r = shx_batch_fields(ctx, shell, nf.data(), ef.data())) return io_fail(shx_last_error(ctx), r);
+  /* the real nodes: an open mesh's ghost layer follows them and is not
+   * written (write_vtk, ShellSystem.C:3189) */
+  int nw = 0;
+  for (const int32_t q : tri) nw = std::max(nw, q + 1);
   std::ofstream file(path);
   if (!file) return io_fail(std::string("cannot write ") + path);
   unsigned char count;                         /* a line break after every 256 values */
@@ -191,18 +196,18 @@ int shx_write_vtu(shx_ctx* ctx, int shell, const char* path)
   file << "<?xml version=\"1.0\"?>\n";
   file << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
   file << "  <UnstructuredGrid>\n";
-  file << "    <Piece NumberOfPoints=\"" << nv << "\" NumberOfCells=\"" << ne << "\">\n";
+  file << "    <Piece NumberOfPoints=\"" << nw << "\" NumberOfCells=\"" << ne << "\">\n";
   file << "      <PointData>\n";
   const char* var[3] = {"ux", "uy", "uz"};
   for (int c = 0; c < 3; ++c) {
     array("Float64", var[c], "");
-    for (int i = 0; i < nv; ++i) put(u[3 * i + c]);
+    for (int i = 0; i < nw; ++i) put(u[3 * i + c]);
     end();
   }
   const char* nname[5] = {"ecav", "efric", "ekin", "eself", "mass"};
   for (int k = 0; k < 5; ++k) {
     array("Float64", nname[k], "");
-    for (int i = 0; i < nv; ++i) put(nf[5 * i + k]);
+    for (int i = 0; i < nw; ++i) put(nf[5 * i + k]);
     end();
   }
   file << "      </PointData>\n";
@@ -216,7 +221,7 @@ int shx_write_vtu(shx_ctx* ctx, int shell, const char* path)
   file << "      </CellData>\n";
   file << "      <Points>\n";
   array("Float64", "Points", " NumberOfComponents=\"3\"");
-  for (size_t i = 0; i < X.size(); ++i) put(X[i]);
+  for (size_t i = 0; i < 3 * (size_t)nw; ++i) put(X[i]);
   end();
   file << "      </Points>\n";
   file << "      <Cells>\n";

@@ -18,8 +18,8 @@
  * 0 success, 1 wall-time limit, 2 signal, 3 error.
  *
  * Differences from shell.C: a mesh file is required (the reference's
- * fallback, a Delaunay square with boundary ghosts, is an open mesh that
- * this path does not treat); randomness (libMesh's random node
+ * fallback, a Delaunay square with boundary ghosts, comes from libMesh's
+ * mesh generators); randomness (libMesh's random node
  * distortion) is refused; the hard-coded change of vtk_time after t > 2
  * (shell.C:333) and the "Volume growth red" debug line of
  * print_properties are not reproduced.  Steps between outputs run on the
@@ -200,7 +200,7 @@ int main(int argc, char** argv)
 
   /* the mesh (shell.C:146-230) */
   if (mesh_filename.empty()) {
-    std::cout << "ERROR: a closed triangle mesh is required (-m meshfile.off)\n";
+    std::cout << "ERROR: a triangle mesh is required (-m meshfile.off)\n";
     return ERROR;
   }
   std::string mesh_text;

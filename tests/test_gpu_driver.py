@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import shell_amd as S
+from conftest import plate
 
 pytestmark = pytest.mark.gpu
 
@@ -223,3 +224,65 @@ def test_driver_cavity_run_equals_the_python_mirror(tmp_path):
     rows = np.array([[float(x) for x in l.split()] for l in
                      open(tmp_path / "data" / "shell_state0000000600.dat").read().splitlines()[4:4 + len(X)]])
     assert np.array_equal(rows[:, 9:12], springs)
+
+
+PLATE_CONFIG = """# a square plate pinned on its edges, sagging under its own weight
+shell_mass_density   = 1
+shell_young_modulus  = 1
+shell_young_modulus2 = 1
+shell_poisson_ratio  = 0.3
+shell_shear_modulus  = 1/2.6
+thickness            = 0.05
+gravity              = 3e-3
+gravity_dir          = 1
+damping_alpha        = 0.1
+shell_boundary       = pinned
+bc_every             = 1
+shell_shell_contact  = none
+shell_cavity_contact = false
+newmark_beta         = 0
+newmark_gamma        = 0.5
+dt                   = 0.02
+adapt_dt_every       = 0
+time_steps           = 80
+data_time            = 0
+vtk_time             = 1.6
+state_time           = 0
+random_seed          = 5
+distortion           = 1e-2
+"""
+
+
+def test_driver_open_mesh_equals_the_python_mirror(tmp_path):
+    """an open mesh through the driver: its ghost layer added (shell.C:
+    228-230), the pinned edges' conditions, the distortion over every node
+    (ghosts included, ShellSystem.C:863-869), the .vtu files holding the
+    real nodes only (ShellSystem.C:3189); equal to the same run through
+    ShellSystem on the mesh with its ghost layer"""
+    X, T = plate(8)
+    (tmp_path / "sphere.off").write_text(S.write_off(X, T))
+    (tmp_path / "config.cfg").write_text(PLATE_CONFIG)
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stdout
+    _, t, u_drv, v_drv = _read_state(tmp_path / "data" / "shell_state0000000080.dat")
+    Xo, To = S.read_off((tmp_path / "sphere.off").read_text())
+    Xg, Tg, ng = S.mesh_add_ghosts(Xo, To)
+    p, d, _ = S.read_config(PLATE_CONFIG)
+    p["ghost_elems"] = ng
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48(d["random_seed"])
+    u0 = np.array([d["distortion"] * (2 * libc.drand48() - 1) * p["thickness"] for _ in range(3 * len(Xg))])
+    g = S.ShellSystem(Xg, Tg, **p)
+    g.set_state(u0)
+    g.apply_initial_conditions()
+    g.advance(80)
+    u, v, _, tg = g.get_state()
+    g.close()
+    say("  open mesh driver vs mirror: t", t, tg, "max |du|", np.max(np.abs(u - u_drv)), "ghost triangles", ng)
+    assert len(u_drv) == 3 * len(Xg) and tg == t
+    assert np.array_equal(u, u_drv) and np.array_equal(v, v_drv)
+    vtu = sorted((tmp_path / "data").glob("shell*.vtu"))
+    assert vtu
+    piece = ET.parse(vtu[-1]).getroot().find("UnstructuredGrid/Piece")
+    assert int(piece.get("NumberOfPoints")) == len(X) and int(piece.get("NumberOfCells")) == len(T)
