@@ -51,8 +51,8 @@ struct shx_ctx {
   bool any_rot = false;        /* some shell has rotation damping: k_rot precedes the element epilogue */
   std::vector<int32_t> bc_nodes, bc_z, bc_shell;   /* boundary edges of the batch (global nodes n1..n4), pinned
                                                      z codes, shells */
-  bool any_loads = false;      /* some shell has gravity, cavity contact, rotation damping or boundary
-                                 conditions: k_node_loads */
+  bool any_bc = false;         /* some shell has boundary conditions: k_node_bc */
+  bool any_loads = false;      /* some shell has gravity, cavity contact or rotation damping: k_node_loads */
   bool any_grow = false;       /* some shell grows: k_grow, then k_elem_grow */
   bool any_grow_mass = false;  /* some growing shell recomputes its masses every attempt: k_mass_grow */
   std::vector<char> keep_cso;  /* per shell: cavity springs were set by the caller (a loaded state) */
@@ -421,7 +421,9 @@ void launch_elem(shx_ctx* ctx, int mode)
 void launch_node(shx_ctx* ctx, int mode)
 {
   prof_begin(ctx, 1);
-  if (ctx->grid[3] && ctx->any_loads) {
+  if (ctx->grid[3] && ctx->any_bc) {
+    k_node_bc<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
+  } else if (ctx->grid[3] && ctx->any_loads) {
     k_node_loads<<<ctx->grid[3], kNodeBlock, 0, ctx->st>>>(ctx->P, mode);
   } else if (ctx->grid[3]) {
     switch (ctx->shapes.node_waves) {
@@ -927,7 +929,7 @@ int shx_batch_create(shx_ctx** out, int nshell, const int* nv, const int* ne, co
       }
       C.bc = bn.empty() ? 0 : prm[s].bc_every;
       if (C.bc) {
-        ctx->any_loads = true;
+        ctx->any_bc = true;
         for (size_t q = 0; q < bn.size(); ++q) ctx->bc_nodes.push_back(C.vbase + bn[q]);
         for (size_t i = 0; i < bz.size(); ++i) {
           ctx->bc_z.push_back(C.bc_type == 2 ? bz[i] : 0);

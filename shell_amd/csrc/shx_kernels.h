@@ -34,6 +34,7 @@ __global__ void k_node_w4(DPtrs P, int mode);
 __global__ void k_node_w6(DPtrs P, int mode);
 __global__ void k_node_w8(DPtrs P, int mode);
 __global__ void k_node_loads(DPtrs P, int mode);
+__global__ void k_node_bc(DPtrs P, int mode);
 __global__ void k_bc_strong(DPtrs P);
 __global__ void k_precompute32(DPtrs P);
 __global__ void k_elem32(DPtrs P, int mode);

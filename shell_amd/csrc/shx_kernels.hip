@@ -1102,6 +1102,76 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
   if (stops(w, C) || w.err) w.done = 1;
 }
 
+/* boundary conditions at node n (apply_boundary_conditions, ShellSystem.C:
+ * 2138-2292), over its entries (edge << 4 | z << 2 | role) in edge order,
+ * z of a pinned edge: 1 when its n1 is an edge node (held at zero) of an
+ * earlier edge, 2 of a later one, 0 of none; roles 0..3 are the
+ * BoundaryCache nodes n1 (interior node of the real element), n2, n3 (the
+ * boundary edge), n4 (the ghost node).  The predicted displacement and
+ * velocity of a node m are formed from the step's start state (U, V, A). */
+struct BcState {
+  const double *U, *V, *A;
+  double dt, a8, a6;
+  __device__ __forceinline__ v3 up(int m) const { return pred3(ld3(U, m), ld3(V, m), ld3(A, m), dt, a8); }
+  __device__ __forceinline__ v3 vp(int m) const
+  {
+    const v3 v = ld3(V, m), a = ld3(A, m);
+    return mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
+  }
+};
+
+/* weak conditions (clamped, pinned and free weak): the springs' force sum
+ * at node n, on the coordinates of bc_coords (f -= it) */
+__device__ __forceinline__ v3 bc_weak_force(const DPtrs& P, const SConst& C, const BcState& B, int n, v3 un)
+{
+  v3 fb = mk(0, 0, 0);
+  for (int k = P.bc_start[n]; k < P.bc_start[n + 1]; ++k) {
+    const int ent = P.bc_ent[k], role = ent & 3;
+    const int* nd = P.bc_nodes + 4 * (ent >> 4);
+    if (C.bc_type == 1) fb = add(fb, scl(un, C.bc_stiff));
+    else if (C.bc_type == 3) {
+      if (role == 1 || role == 2) fb = add(fb, scl(un, C.bc_stiff));
+      else if (role == 3) fb = add(fb, scl(add(un, B.up(nd[0])), C.bc_stiff));
+    } else if (role == 3) {
+      fb = add(fb, scl(sub(sub(add(un, B.up(nd[0])), B.up(nd[1])), B.up(nd[2])), C.bc_stiff));
+    }
+  }
+  return mk(C.bc_coords & 1 ? fb.x : 0, C.bc_coords & 2 ? fb.y : 0, C.bc_coords & 4 ? fb.z : 0);
+}
+
+/* coordinates of mask set to those of b */
+__device__ __forceinline__ v3 masked(v3 a, v3 b, int mask)
+{
+  return mk(mask & 1 ? b.x : a.x, mask & 2 ? b.y : a.y, mask & 4 ? b.z : a.z);
+}
+
+/* strong conditions (clamped, pinned, free) on node n's predicted u and v:
+ * held at zero (clamped: every role; pinned: n2, n3) or, for the ghost
+ * node, set by its relation (pinned n4 = -n1, or zero where an earlier edge
+ * holds n1; free n4 = n2 + n3 - n1); returns the coordinates held at zero */
+__device__ __forceinline__ int bc_strong(const DPtrs& P, const SConst& C, const BcState& B, int n, v3& up, v3& vp)
+{
+  int zmask = 0;
+  const v3 zero = mk(0, 0, 0);
+  for (int k = P.bc_start[n]; k < P.bc_start[n + 1]; ++k) {
+    const int ent = P.bc_ent[k], role = ent & 3;
+    const int* nd = P.bc_nodes + 4 * (ent >> 4);
+    if (C.bc_type == 0 || (C.bc_type == 2 && (role == 1 || role == 2))) {
+      zmask = C.bc_coords;
+    } else if (role == 3 && C.bc_type == 2) {
+      const bool z = ((ent >> 2) & 3) == 1;
+      up = masked(up, neg(z ? zero : B.up(nd[0])), C.bc_coords);
+      vp = masked(vp, neg(z ? zero : B.vp(nd[0])), C.bc_coords);
+    } else if (role == 3) {
+      up = masked(up, sub(add(B.up(nd[1]), B.up(nd[2])), B.up(nd[0])), C.bc_coords);
+      vp = masked(vp, sub(add(B.vp(nd[1]), B.vp(nd[2])), B.vp(nd[0])), C.bc_coords);
+    }
+  }
+  up = masked(up, zero, zmask);
+  vp = masked(vp, zero, zmask);
+  return zmask;
+}
+
 /* One thread per node: the node's internal force and pressure direction are
  * the sums of its element-block records (written by the element kernel, in
  * ascending block order), then the contact force is added and the damped
@@ -1112,82 +1182,7 @@ __global__ void __launch_bounds__(64) k_fin_node(DPtrs P, int mode)
  * the rigid-body sums, for k_fin_node.  Launch block b computes node block
  * bmap[V_NODE][b].  Force time: t + dt in a step, t + dt in M_INIT
  * (apply_initial_conditions), DShell::t_eval in M_FORCES. */
-/* boundary conditions at node n (apply_boundary_conditions, ShellSystem.C:
- * 2138-2292), over its entries (edge << 4 | z << 2 | role) in edge order,
- * z of a pinned edge: 1 when its n1 is an edge node (held at zero) of an
- * earlier edge, 2 of a later one, 0 of none; roles 0..3 are
- * the BoundaryCache nodes n1 (interior node of the real element), n2, n3
- * (the boundary edge), n4 (the ghost node).  up(m), vp(m) are node m's
- * predicted displacement and velocity.  Weak conditions: the spring force
- * sum fb (f -= fb).  Strong ones: zmask, the coordinates the condition holds
- * at zero (clamped: every role; pinned: n2, n3), and for the ghost node's
- * relations (pinned n4 = -n1, free n4 = n2 + n3 - n1) the related predicted
- * values (rel set, coordinates in rmask). */
-struct BcAt {
-  v3 fb, up, vp;
-  int zmask, rmask;
-};
-
-__device__ void bc_at(const DPtrs& P, const SConst& C, int n, const double* U, const double* V, const double* A,
-                      bool predict, double dt, double a8, double a6, BcAt& o)
-{
-  o.fb = mk(0, 0, 0);
-  o.zmask = 0;
-  o.rmask = 0;
-  auto upred = [&](int m) {
-    const v3 u = ld3(U, m);
-    return predict ? pred3(u, ld3(V, m), ld3(A, m), dt, a8) : u;
-  };
-  auto vpred = [&](int m) {
-    const v3 v = ld3(V, m), a = ld3(A, m);
-    return mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
-  };
-  const v3 un = upred(n);
-  for (int k = P.bc_start[n]; k < P.bc_start[n + 1]; ++k) {
-    const int ent = P.bc_ent[k], role = ent & 3;
-    const int* nd = P.bc_nodes + 4 * (ent >> 4);
-    switch (C.bc_type) {
-      case 0: o.zmask = C.bc_coords; break;
-      case 1: o.fb = add(o.fb, scl(un, C.bc_stiff)); break;
-      case 2:
-        if (role == 1 || role == 2) o.zmask = C.bc_coords;
-        else if (role == 3) {
-          /* n1 held at zero by an earlier edge (bits 2-3 of the entry: 1) */
-          const bool z = ((ent >> 2) & 3) == 1;
-          o.rmask = C.bc_coords;
-          o.up = neg(z ? mk(0, 0, 0) : upred(nd[0]));
-          o.vp = neg(z ? mk(0, 0, 0) : vpred(nd[0]));
-        }
-        break;
-      case 3:
-        if (role == 1 || role == 2) o.fb = add(o.fb, scl(un, C.bc_stiff));
-        else if (role == 3) o.fb = add(o.fb, scl(add(un, upred(nd[0])), C.bc_stiff));
-        break;
-      case 4:
-        if (role == 3) {
-          o.rmask = C.bc_coords;
-          o.up = sub(add(upred(nd[1]), upred(nd[2])), upred(nd[0]));
-          o.vp = sub(add(vpred(nd[1]), vpred(nd[2])), vpred(nd[0]));
-        }
-        break;
-      default:
-        if (role == 3) o.fb = add(o.fb, scl(sub(sub(add(un, upred(nd[0])), upred(nd[1])), upred(nd[2])), C.bc_stiff));
-        break;
-    }
-  }
-  /* weak conditions act on the coordinates of bc_coords only */
-  if (!(C.bc_coords & 1)) o.fb.x = 0;
-  if (!(C.bc_coords & 2)) o.fb.y = 0;
-  if (!(C.bc_coords & 4)) o.fb.z = 0;
-}
-
-/* coordinates of mask set to those of b */
-__device__ __forceinline__ v3 masked(v3 a, v3 b, int mask)
-{
-  return mk(mask & 1 ? b.x : a.x, mask & 2 ? b.y : a.y, mask & 4 ? b.z : a.z);
-}
-
-template <bool LOADS>
+template <bool LOADS, bool BC = false>
 __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
 {
   __shared__ double s_wave[kNodeRec][kNodeBlock / 64];
@@ -1270,13 +1265,11 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
       }
     }
     v3 f = add(fint, fext);
-    BcAt bca;
-    bca.zmask = 0;
-    bca.rmask = 0;
-    if (LOADS && C.bc && mode == M_STEP && sh.n_steps % C.bc == 0) {
+    const bool bc_now = BC && C.bc && mode == M_STEP && sh.n_steps % C.bc == 0;
+    const BcState bcs = {P.U[par], P.V[par], P.A[par], dt, sh.nm[8], sh.nm[6]};
+    if (bc_now && (C.bc_type & 1)) {
       /* the weak springs after the other forces (ShellSystem.C:1000-1003) */
-      bc_at(P, C, n, P.U[par], P.V[par], P.A[par], mode == M_STEP, dt, sh.nm[8], sh.nm[6], bca);
-      f = sub(f, bca.fb);
+      f = sub(f, bc_weak_force(P, C, bcs, n, bcs.up(n)));
     }
     rec[0] = nrm2(f);
     rec[1] = nrm2(fext);
@@ -1307,21 +1300,18 @@ __device__ __forceinline__ void node_body(const DPtrs& P, int mode)
       const double a8 = sh.nm[8], a6 = sh.nm[6], a0 = sh.nm[0], a7 = sh.nm[7];
       v3 up = pred3(u, v, a, dt, a8);
       v3 vp = mk(a6 * a.x + v.x, a6 * a.y + v.y, a6 * a.z + v.z);
-      if (LOADS) {
-        /* strong conditions on the predicted values before the damping (the
-         * first apply_boundary_conditions of advance, ShellSystem.C:1003) */
-        const v3 zero = mk(0, 0, 0);
-        up = masked(masked(up, bca.up, bca.rmask), zero, bca.zmask);
-        vp = masked(masked(vp, bca.vp, bca.rmask), zero, bca.zmask);
-      }
+      int zmask = 0;
+      /* strong conditions on the predicted values before the damping (the
+       * first apply_boundary_conditions of advance, ShellSystem.C:1003) */
+      if (bc_now && !(C.bc_type & 1)) zmask = bc_strong(P, C, bcs, n, up, vp);
       v3 an = mk(mi * (f.x - cd * vp.x), mi * (f.y - cd * vp.y), mi * (f.z - cd * vp.z));
       rec[2] = fmax(fabs(an.x - a.x), fmax(fabs(an.y - a.y), fabs(an.z - a.z)));
       rec[3] = fmax(fabs(vp.x), fmax(fabs(vp.y), fabs(vp.z)));
-      if (LOADS && bca.zmask) {
+      if (zmask) {
         /* the second one (:1051): a held at zero too; the acceleration before
          * it is what a later pinned edge's ghost relation reads */
         if (P.bc_apre) st3(P.bc_apre, n, an);
-        an = masked(an, mk(0, 0, 0), bca.zmask);
+        an = masked(an, mk(0, 0, 0), zmask);
       }
       const v3 un = mk(a0 * an.x + up.x, a0 * an.y + up.y, a0 * an.z + up.z);
       const v3 vn = mk(a7 * an.x + vp.x, a7 * an.y + vp.y, a7 * an.z + vp.z);
@@ -2214,6 +2204,10 @@ __global__ void __launch_bounds__(kNodeBlock, 8) k_node_w8(DPtrs P, int mode) { 
 /* the node kernel with the loads after the pressure (gravity, cavity contact,
  * rotation damping), for batches where a shell has one of them */
 __global__ void __launch_bounds__(kNodeBlock, 4) k_node_loads(DPtrs P, int mode) { node_body<true>(P, mode); }
+/* ... and with the boundary conditions of open meshes, for batches where a
+ * shell has them (three waves per SIMD: the conditions' state fits without
+ * spilling) */
+__global__ void __launch_bounds__(kNodeBlock, 3) k_node_bc(DPtrs P, int mode) { node_body<true, true>(P, mode); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l8(DPtrs P, int posmode, int list) { pairs_body<8>(P, posmode, list); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l16(DPtrs P, int posmode, int list) { pairs_body<16>(P, posmode, list); }
 __global__ void __launch_bounds__(256) k_bp_pairs_l32(DPtrs P, int posmode, int list) { pairs_body<32>(P, posmode, list); }
