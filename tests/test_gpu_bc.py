@@ -155,3 +155,44 @@ def test_batch_of_open_and_closed_shells_equals_standalone():
         assert tb == tg and np.array_equal(ub, ug) and np.array_equal(vb, vg) and np.array_equal(ab, ag)
         g.close()
     b.close()
+
+
+def test_contact_on_an_open_tube():
+    """an open tube with pinned ends squashed flat until its walls touch:
+    self-contact with friction on an open mesh (the ghost elements take no
+    part in the cell lists or close sets, ShellSystem.C:664-674, 700,
+    1974, 2035); pair lists bit for bit, states within 1e-10"""
+    say("case contact on an open tube")
+    X, T = tube(24, 6, r=0.5, length=1.0)
+    Xg, Tg, ng = S.mesh_add_ghosts(X, T)
+    kw = dict(ISO, thickness=0.02, damping_alpha=0.1, ghost_elems=ng, shell_boundary=2, bc_every=1, contact=1,
+              mu_s=0.3, mu_d=0.3, cell_size_mult=1.0)
+    kw["dt"] = stable_dt(Xg, Tg, kw, cfl=0.4)
+    # adaptive steps: the flattened folds start far from equilibrium
+    kw.update(adapt_dt_every=1, dt_max=kw["dt"], rel_loc_err_min=5e-5, rel_loc_err_max=5e-4)
+    u = np.zeros_like(Xg)
+    u[:, 0] = -Xg[:, 0] * 0.97                   # the walls 0.03 apart
+    v = np.zeros_like(Xg)
+    v[:, 0] = -0.02 * np.sign(Xg[:, 0])
+    g = S.ShellSystem(Xg, Tg, **kw)
+    o = oracle.Oracle(Xg, Tg, **kw)
+    g.set_state(u.reshape(-1), v.reshape(-1))
+    o.set_state(u.reshape(-1), v.reshape(-1))
+    g.apply_initial_conditions()
+    o.init_dynamics()
+    for k in range(2):
+        rg = g.advance(20)
+        ro = o.step(20)
+        ug, vg, _, tg = g.get_state()
+        uo, vo, _, to = o.get_state()
+        e1g, e2g = g.pairs()
+        e1o, e2o = o.pairs()
+        say("  t", tg, to, "pairs", len(e1g), len(e1o), "contacts", rg["n_contacts"], ro["n_contacts"], "rel u",
+            relerr(ug, uo), "v", relerr(vg, vo))
+        assert np.array_equal(e1g, e1o) and np.array_equal(e2g, e2o) and len(e1g) > 0
+        assert rg["n_contacts"] == ro["n_contacts"] and rg["n_rebuilds"] == ro["n_rebuilds"]
+        assert rg["n_rejected"] == ro["n_rejected"] and abs(tg - to) <= 1e-12 * max(1.0, abs(to))
+        assert relerr(ug, uo) < 1e-10 and relerr(vg, vo) < 1e-10
+    assert ro["n_contacts"] > 0
+    g.close()
+    o.close()
