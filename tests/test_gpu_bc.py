@@ -157,15 +157,25 @@ def test_batch_of_open_and_closed_shells_equals_standalone():
     b.close()
 
 
-def test_contact_on_an_open_tube():
+@pytest.mark.parametrize("contact", [1, 2])
+def test_contact_on_an_open_tube(contact):
     """an open tube with pinned ends squashed flat until its walls touch:
-    self-contact with friction on an open mesh (the ghost elements take no
-    part in the cell lists or close sets, ShellSystem.C:664-674, 700,
-    1974, 2035); pair lists bit for bit, states within 1e-10"""
-    say("case contact on an open tube")
+    self-contact with friction on an open mesh, linear and smooth (the ghost
+    elements take no part in the cell lists or close sets, ShellSystem.C:
+    664-674, 700, 1974, 2035; the smooth contact's limit surfaces of the
+    boundary elements run over ghost nodes).  Linear: pair lists bit for
+    bit and states within 1e-10 over two chunks of steps.  Smooth: the
+    flattened walls are straight along the tube's axis, so two facing
+    elements' closest points are not unique (only their distance is) and
+    each side's search stops at another point of the minimising set; the
+    contact forces then differ at 1e-6 and the adaptive step carries that
+    into dt.  So the pair lists and distances at the start are compared (1e-8),
+    and after one chunk the pair lists and contact counts exactly, u
+    within 1e-5 and t within 1e-4 (measured 1e-6 and 5e-5)."""
+    say("case contact on an open tube", contact)
     X, T = tube(24, 6, r=0.5, length=1.0)
     Xg, Tg, ng = S.mesh_add_ghosts(X, T)
-    kw = dict(ISO, thickness=0.02, damping_alpha=0.1, ghost_elems=ng, shell_boundary=2, bc_every=1, contact=1,
+    kw = dict(ISO, thickness=0.02, damping_alpha=0.1, ghost_elems=ng, shell_boundary=2, bc_every=1, contact=contact,
               mu_s=0.3, mu_d=0.3, cell_size_mult=1.0)
     kw["dt"] = stable_dt(Xg, Tg, kw, cfl=0.4)
     # adaptive steps: the flattened folds start far from equilibrium
@@ -180,7 +190,12 @@ def test_contact_on_an_open_tube():
     o.set_state(u.reshape(-1), v.reshape(-1))
     g.apply_initial_conditions()
     o.init_dynamics()
-    for k in range(2):
+    if contact == 2:
+        assert np.array_equal(np.concatenate(g.pairs()), np.concatenate(o.pairs()))
+        d2g, d2o = g.pair_state()["d2"], o.pair_state()[0]
+        say("  start: pairs", len(d2o), "max rel d2 diff", np.max(np.abs(d2g - d2o) / np.maximum(d2o, 1e-300)))
+        assert np.allclose(d2g, d2o, rtol=1e-8, atol=0)
+    for k in range(2 if contact == 1 else 1):
         rg = g.advance(20)
         ro = o.step(20)
         ug, vg, _, tg = g.get_state()
@@ -191,8 +206,12 @@ def test_contact_on_an_open_tube():
             relerr(ug, uo), "v", relerr(vg, vo))
         assert np.array_equal(e1g, e1o) and np.array_equal(e2g, e2o) and len(e1g) > 0
         assert rg["n_contacts"] == ro["n_contacts"] and rg["n_rebuilds"] == ro["n_rebuilds"]
-        assert rg["n_rejected"] == ro["n_rejected"] and abs(tg - to) <= 1e-12 * max(1.0, abs(to))
-        assert relerr(ug, uo) < 1e-10 and relerr(vg, vo) < 1e-10
+        assert rg["n_rejected"] == ro["n_rejected"]
+        if contact == 1:
+            assert abs(tg - to) <= 1e-12 * max(1.0, abs(to))
+            assert relerr(ug, uo) < 1e-10 and relerr(vg, vo) < 1e-10
+        else:
+            assert abs(tg - to) <= 1e-4 * abs(to) and relerr(ug, uo) < 1e-5
     assert ro["n_contacts"] > 0
     g.close()
     o.close()
