@@ -2,11 +2,13 @@
 for profilers (rocprofv3 --kernel-trace / --pmc): the level-4 single shell,
 the seeded batched sweep, or the 1024 level-4 shells of the warm-start test
 set (plain start, dt = 0.5 x their CFL bound), FP64 or FP32 split, attempts
-launched one by one (no graph); and `smooth`: a level-4 shell pancaked to
+launched one by one (no graph); `smooth`: a level-4 shell pancaked to
 3 % of its height whose sheets close at 0.04 per unit time, with smooth
-(limit-surface) contact and friction, adaptive dt.
+(limit-surface) contact and friction, adaptive dt; and `open`: 64 open
+square plates of 64 x 64 squares (8192 elements each, with their ghost
+layers) pinned on their edges and sagging under their own weight.
 
-usage: python -m shell_amd.profile_steps single|batch|surrogate|smooth STEPS [fp32]
+usage: python -m shell_amd.profile_steps single|batch|surrogate|smooth|open STEPS [fp32]
 """
 import sys
 
@@ -47,6 +49,16 @@ def main():
         meshes = [(X, T)]
         params = [dict(contact=2, mu_s=0.3, mu_d=0.3, cell_size_mult=1.0, dt=0.01, dt_max=0.01, damping_alpha=0.05,
                        adapt_dt_every=1)]
+    elif what == "open":
+        n = 64
+        i, j = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+        X = np.stack([i.ravel() / n, np.zeros(i.size), j.ravel() / n], axis=1)     # the x-z plane
+        q = (i[:-1, :-1] * (n + 1) + j[:-1, :-1]).ravel()
+        T = np.concatenate([np.stack([q, q + n + 1, q + n + 2], 1), np.stack([q, q + n + 2, q + 1], 1)])
+        Xg, Tg, ng = S.mesh_add_ghosts(X, T.astype(np.int32))
+        meshes = [(Xg, Tg)] * 64
+        params = [dict(E2=1.0, G12=1.0 / 2.6, thickness=0.05, gravity=3e-3, gravity_dir=1, damping_alpha=0.1,
+                       ghost_elems=ng, shell_boundary=2, bc_every=1, dt=0.0018)] * 64
     else:
         shells = batch_sweep(512, seed=7)
         meshes = [(s["X"], s["T"]) for s in shells]
