@@ -27,6 +27,11 @@
  * patch-incidence sum, and the orthotropic shear stiffness (H22) by the
  * closed-form membrane energy of a torsion about the polar axis; copies of
  * this file with a planted slip in either fail those pins.
+ * Open meshes: the ghost test, the BoundaryCache list and
+ * apply_boundary_conditions are pinned in tests/test_bc_host.py by the
+ * list's defining properties and by closed forms (Navier's pinned plate and
+ * the clamped plate approached as the mesh is refined, the weak conditions
+ * tending to the strong ones, the free tube's membrane solution).
  */
 #include "shell_oracle.h"
 #include "oracle_geom.h"
